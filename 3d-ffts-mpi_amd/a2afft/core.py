@@ -1,0 +1,245 @@
+"""Python mirror of the reference's host interface over the C ABI.
+
+``Distribution`` mirrors ``a2aDistribution`` (src/a2afft/distribution/
+a2adistribution.h:5-42) and ``Dfft`` mirrors ``a2adfft`` (src/a2afft/dfft/
+a2adfft.h:9-30): same method names, same buffer roles, same stage order.
+The reference's P MPI ranks are P virtual ranks on one GPU, so every buffer is
+a device tensor of shape (P, nlocal) complex; torch supplies the device memory
+and the stream, the arithmetic runs in liba2afft.so.
+"""
+import ctypes
+
+import torch
+
+from . import _lib as L
+
+DTYPES = {"c128": L.C128, "c64": L.C64}
+TORCH_DTYPES = {L.C128: torch.complex128, L.C64: torch.complex64}
+PRECISIONS = {"fp64": L.FP64, "fp32": L.FP32, "split_bf16": L.SPLIT_BF16, "split_fp16": L.SPLIT_FP16}
+EXECS = {"fused": L.EXEC_FUSED, "staged": L.EXEC_STAGED}
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def dims_create(nnodes):
+    """MPI_Dims_create(nnodes, 3) (a2adistribution.cpp:100)."""
+    d = (ctypes.c_int * 3)()
+    L.check(L.lib().a2afft_dims_create(nnodes, d), "dims_create")
+    return tuple(d)
+
+
+def set_sync(on):
+    """Block after every call, as the reference's launchers do."""
+    L.lib().a2afft_set_sync(1 if on else 0)
+
+
+class Distribution:
+    """a2aDistribution over ``world_size`` virtual ranks (a2adistribution.cpp:85-154)."""
+
+    def __init__(self, Ng, world_size, dims=None, blockSize=64, dtype="c128"):
+        self._h = ctypes.c_void_p()
+        self.dtype_code = DTYPES[dtype] if isinstance(dtype, str) else int(dtype)
+        dims_arg = L.ints(dims) if dims is not None else None
+        L.check(L.lib().a2afft_dist_create(ctypes.byref(self._h), Ng, world_size, dims_arg, blockSize,
+                                           self.dtype_code), "a2aDistribution")
+        info = L.DistInfo()
+        L.check(L.lib().a2afft_dist_get_info(self._h, ctypes.byref(info)), "dist_get_info")
+        self.Ng = info.Ng
+        self.nlocal = info.nlocal
+        self.world_size = info.world_size
+        self.local_grid_size = tuple(info.local_grid_size)
+        self.dims = tuple(info.dims)
+        self.blockSize = info.blockSize
+        self.reference_domain = bool(info.reference_domain)
+        self.torch_dtype = TORCH_DTYPES[self.dtype_code]
+
+    @property
+    def handle(self):
+        return self._h
+
+    def coords(self, rank):
+        """rank2coords + get_local_grid_start (a2adistribution.cpp:55-83)."""
+        c = (ctypes.c_int * 3)()
+        s = (ctypes.c_int * 3)()
+        L.check(L.lib().a2afft_dist_rank_coords(self._h, rank, c, s), "rank_coords")
+        return tuple(c), tuple(s)
+
+    def write_dims(self, path):
+        """The saveDims header (a2adistribution.cpp:142-152)."""
+        L.check(L.lib().a2afft_dist_write_dims(self._h, str(path).encode()), "write_dims")
+
+    def empty(self, device="cuda"):
+        return torch.empty((self.world_size, self.nlocal), dtype=self.torch_dtype, device=device)
+
+    def _stage(self, name, b1, b2):
+        self._check_buf(b1)
+        self._check_buf(b2)
+        return L.check(getattr(L.lib(), name)(self._h, _ptr(b1), _ptr(b2), _stream()), name)
+
+    def _check_buf(self, b):
+        if b.dtype != self.torch_dtype or b.numel() != self.world_size * self.nlocal or not b.is_contiguous():
+            raise ValueError(f"buffer must be a contiguous {self.torch_dtype} tensor of "
+                             f"{self.world_size}x{self.nlocal} elements")
+        if not b.is_cuda:
+            raise ValueError("buffer must be a device tensor")
+
+    def getZPencils(self, d_Buff1, d_Buff2):
+        return self._stage("a2afft_get_z_pencils", d_Buff1, d_Buff2)
+
+    def returnZPencils(self, d_Buff1, d_Buff2):
+        return self._stage("a2afft_return_z_pencils", d_Buff1, d_Buff2)
+
+    def getXPencils(self, d_Buff1, d_Buff2):
+        return self._stage("a2afft_get_x_pencils", d_Buff1, d_Buff2)
+
+    def returnXPencils(self, d_Buff1, d_Buff2):
+        return self._stage("a2afft_return_x_pencils", d_Buff1, d_Buff2)
+
+    def getYPencils(self, d_Buff1, d_Buff2):
+        return self._stage("a2afft_get_y_pencils", d_Buff1, d_Buff2)
+
+    def returnYPencils(self, d_Buff1, d_Buff2):
+        return self._stage("a2afft_return_y_pencils", d_Buff1, d_Buff2)
+
+    def finalize(self):
+        """a2aDistribution::finalize (a2adistribution.cpp:258-265)."""
+        if self._h:
+            L.lib().a2afft_dist_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.finalize()
+        except Exception:
+            pass
+
+
+class Dfft:
+    """a2adfft (a2adfft.cpp:36-188) with a precision and an execution strategy."""
+
+    def __init__(self, dist, precision=None, exec="fused"):
+        if precision is None:
+            precision = "fp64" if dist.dtype_code == L.C128 else "fp32"
+        self.dist = dist
+        self.precision = precision
+        self.exec = exec
+        self._h = ctypes.c_void_p()
+        L.check(L.lib().a2afft_plan_create(ctypes.byref(self._h), dist.handle, PRECISIONS[precision],
+                                           EXECS[exec]), "a2adfft")
+        self._scratch = None
+
+    def make_plans(self, d_scratch=None):
+        """a2adfft::make_plans (a2adfft.cpp:47-53)."""
+        if d_scratch is not None:
+            self.dist._check_buf(d_scratch)
+        self._scratch = d_scratch
+        L.check(L.lib().a2afft_make_plans(self._h, _ptr(d_scratch) if d_scratch is not None else None),
+                "make_plans")
+
+    def forward(self, d_data):
+        self.dist._check_buf(d_data)
+        return L.check(L.lib().a2afft_forward(self._h, _ptr(d_data), _stream()), "forward")
+
+    def inverse(self, d_data):
+        self.dist._check_buf(d_data)
+        return L.check(L.lib().a2afft_inverse(self._h, _ptr(d_data), _stream()), "inverse")
+
+    def destroy(self):
+        if self._h:
+            L.lib().a2afft_plan_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+# ---- launcher level --------------------------------------------------------
+
+_REMAP_FNS = [
+    "a2afft_launch_d_z_a2a_to_z_pencils", "a2afft_launch_d_z_pencils_to_z_a2a",
+    "a2afft_launch_d_x_a2a_to_x_pencils", "a2afft_launch_d_x_pencils_to_x_a2a",
+    "a2afft_launch_d_y_a2a_to_y_pencils", "a2afft_launch_d_y_pencils_to_y_a2a",
+]
+_TRANSPOSE_FNS = ["a2afft_launch_d_fast_z_to_x", "a2afft_launch_d_fast_x_to_y", "a2afft_launch_d_fast_y_to_z"]
+
+
+def _dtype_code(t):
+    if t.dtype == torch.complex128:
+        return L.C128
+    if t.dtype == torch.complex64:
+        return L.C64
+    raise ValueError("complex64 or complex128 tensor expected")
+
+
+def launch_remap(kind, source, dest, world_size, nlocal, local_grid_size, dims, nbuffers, blockSize=64):
+    """One of the six pack/unpack launchers (kind 0..5) or a transpose (6..8)."""
+    lib = L.lib()
+    if kind < 6:
+        fn = getattr(lib, _REMAP_FNS[kind])
+        st = fn(_ptr(source), _ptr(dest), blockSize, world_size, nlocal, L.ints(local_grid_size), L.ints(dims),
+                _dtype_code(source), nbuffers, _stream())
+        return L.check(st, _REMAP_FNS[kind])
+    fn = getattr(lib, _TRANSPOSE_FNS[kind - 6])
+    st = fn(_ptr(source), _ptr(dest), L.ints(local_grid_size), blockSize, nlocal, _dtype_code(source), nbuffers,
+            _stream())
+    return L.check(st, _TRANSPOSE_FNS[kind - 6])
+
+
+def alltoall(send, recv, world_size, nlocal):
+    return L.check(L.lib().a2afft_alltoall(_ptr(send), _ptr(recv), world_size, nlocal, _dtype_code(send),
+                                           _stream()), "alltoall")
+
+
+def forward_1d_fft(data, Ng):
+    return L.check(L.lib().a2afft_forward_1d_fft(_ptr(data), Ng, data.numel(), _dtype_code(data), _stream()),
+                   "forward_1d_fft")
+
+
+def inverse_1d_fft(data, Ng):
+    return L.check(L.lib().a2afft_inverse_1d_fft(_ptr(data), Ng, data.numel(), _dtype_code(data), _stream()),
+                   "inverse_1d_fft")
+
+
+def scale_fft(out, data, Ng, blockSize=64):
+    return L.check(L.lib().a2afft_launch_scale_fft(_ptr(out), _ptr(data), Ng, data.numel(), blockSize,
+                                                   _dtype_code(data), _stream()), "scale_fft")
+
+
+def fast_copy_fft(out, data, blockSize=64):
+    return L.check(L.lib().a2afft_launch_fast_copy_fft(_ptr(out), _ptr(data), data.numel(), blockSize,
+                                                       _dtype_code(data), _stream()), "fast_copy_fft")
+
+
+def error_metrics(y, ref128):
+    """(rel_l2, max_abs) of y against a complex128 device reference, reduced in FP64 on the GPU."""
+    if ref128.dtype != torch.complex128:
+        raise ValueError("reference must be complex128")
+    if y.numel() != ref128.numel():
+        raise ValueError("size mismatch")
+    rel = ctypes.c_double()
+    mx = ctypes.c_double()
+    L.check(L.lib().a2afft_error_metrics(_ptr(y), _dtype_code(y), _ptr(ref128), y.numel(), ctypes.byref(rel),
+                                         ctypes.byref(mx), _stream()), "error_metrics")
+    return rel.value, mx.value
+
+
+def fill_random(out, seed, kind="uniform", g0=0):
+    """Philox4x32-10 synthetic input on the device (kind 'uniform' or 'normal')."""
+    k = {"uniform": 0, "normal": 1}[kind]
+    return L.check(L.lib().a2afft_fill_random(_ptr(out), _dtype_code(out), out.numel(), seed, k, g0, _stream()),
+                   "fill_random")
+
+
+def execute_pass(plan, data, pass_index, direction=-1):
+    """One axis pass of a fused plan (0 z, 1 x, 2 y), as forward/inverse run it."""
+    return L.check(L.lib().a2afft_execute_pass(plan._h, _ptr(data), pass_index, direction, _stream()),
+                   "execute_pass")

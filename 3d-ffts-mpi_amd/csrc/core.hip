@@ -1,0 +1,471 @@
+// Library core: status handling, the virtual-rank distribution that replaces
+// a2aDistribution (src/a2afft/distribution/a2adistribution.{h,cpp}), and the
+// plan / execute layer that replaces a2adfft (src/a2afft/dfft/a2adfft.{h,cpp}).
+#include "common.h"
+#include "fft.h"
+#include "remap.h"
+
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <new>
+
+namespace a2a {
+
+static thread_local char g_err[512] = "";
+static int g_sync = 0;
+
+int set_error(int status, const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return status;
+}
+
+int hip_check(int hip_status, const char* what) {
+    if (hip_status == hipSuccess) return A2AFFT_OK;
+    return set_error(A2AFFT_E_HIP, "%s: %s", what, hipGetErrorString(hipError_t(hip_status)));
+}
+
+bool sync_mode() { return g_sync != 0; }
+
+int finish_call(void* stream, int status) {
+    if (status < 0 || !g_sync) return status;
+    (void)stream;
+    int st = hip_check(int(hipDeviceSynchronize()), "device synchronize");
+    return st < 0 ? st : status;
+}
+
+}  // namespace a2a
+
+using namespace a2a;
+
+// ===========================================================================
+// Library
+// ===========================================================================
+
+extern "C" const char* a2afft_status_string(int s) {
+    switch (s) {
+    case A2AFFT_OK: return "ok";
+    case A2AFFT_W_DIVERGES_FROM_REFERENCE: return "warning: dims outside the reference y-return domain";
+    case A2AFFT_E_INVALID_ARG: return "invalid argument";
+    case A2AFFT_E_DIVISIBILITY: return "Ng not divisible by dims";
+    case A2AFFT_E_RANKS: return "world_size not compatible with dims";
+    case A2AFFT_E_CHUNK: return "all-to-all chunk does not divide";
+    case A2AFFT_E_HIP: return "HIP runtime error";
+    case A2AFFT_E_ALLOC: return "allocation failed";
+    case A2AFFT_E_UNSUPPORTED: return "unsupported length or precision";
+    case A2AFFT_E_NO_SCRATCH: return "make_plans not called (no scratch)";
+    case A2AFFT_E_SIZE: return "size out of range";
+    default: return "unknown status";
+    }
+}
+
+extern "C" const char* a2afft_last_error(void) { return g_err; }
+extern "C" void a2afft_set_sync(int on) { g_sync = on ? 1 : 0; }
+extern "C" int a2afft_version(void) { return 100; }
+
+// ===========================================================================
+// Distribution (a2aDistribution)
+// ===========================================================================
+
+struct a2afft_dist_s {
+    int Ng;
+    int P;
+    int dims[3];
+    int lgs[3];
+    int64_t nlocal;
+    int blockSize;
+    a2afft_dtype dtype;
+};
+
+// MPI_Dims_create(nnodes, 3) with all dims free: as balanced as possible,
+// non-increasing (MPI standard).  Prime factors, largest first, each go to
+// the currently smallest dimension.  Gives 4->[2,2,1], 6->[3,2,1],
+// 8->[2,2,2], 12->[3,2,2], 16->[4,2,2], 64->[4,4,4].
+extern "C" int a2afft_dims_create(int nnodes, int dims[3]) {
+    if (nnodes <= 0 || !dims) return set_error(A2AFFT_E_INVALID_ARG, "dims_create: bad arguments");
+    int f[64], nf = 0, p = nnodes;
+    for (int q = 2; (long)q * q <= p; ++q)
+        while (p % q == 0) { f[nf++] = q; p /= q; }
+    if (p > 1) f[nf++] = p;
+    int d[3] = {1, 1, 1};
+    for (int i = nf - 1; i >= 0; --i) {
+        int m = 0;
+        for (int a = 1; a < 3; ++a)
+            if (d[a] < d[m]) m = a;
+        d[m] *= f[i];
+    }
+    for (int a = 0; a < 3; ++a)
+        for (int b = a + 1; b < 3; ++b)
+            if (d[b] > d[a]) { int t = d[a]; d[a] = d[b]; d[b] = t; }
+    dims[0] = d[0]; dims[1] = d[1]; dims[2] = d[2];
+    return A2AFFT_OK;
+}
+
+extern "C" int a2afft_dist_create(a2afft_dist* out, int Ng, int world_size, const int* dims, int blockSize,
+                                  a2afft_dtype dtype) {
+    if (!out) return set_error(A2AFFT_E_INVALID_ARG, "dist_create: null out");
+    *out = nullptr;
+    if (Ng <= 0 || world_size <= 0 || blockSize <= 0 || blockSize > 1024 ||
+        (dtype != A2AFFT_C128 && dtype != A2AFFT_C64))
+        return set_error(A2AFFT_E_INVALID_ARG, "dist_create: bad arguments");
+    int d[3];
+    if (dims) {
+        for (int a = 0; a < 3; ++a) d[a] = dims[a];
+        if (d[0] <= 0 || d[1] <= 0 || d[2] <= 0) return set_error(A2AFFT_E_INVALID_ARG, "dist_create: dims <= 0");
+        if (int64_t(d[0]) * d[1] * d[2] != world_size)
+            return set_error(A2AFFT_E_RANKS, "dist_create: dims product != world_size");
+    } else {
+        a2afft_dims_create(world_size, d);
+    }
+    // a2adistribution.cpp:102-104
+    for (int a = 0; a < 3; ++a)
+        if (Ng % d[a]) return set_error(A2AFFT_E_DIVISIBILITY, "Ng %% dims[%d] != 0", a);
+    int l[3] = {Ng / d[0], Ng / d[1], Ng / d[2]};
+    const int64_t nlocal = int64_t(l[0]) * l[1] * l[2];
+    // a2adistribution.cpp:115-121
+    for (int a = 0; a < 3; ++a)
+        if (world_size % d[a]) return set_error(A2AFFT_E_RANKS, "world_size %% dims[%d] != 0", a);
+    if ((int64_t(l[0]) * l[1]) % world_size) return set_error(A2AFFT_E_CHUNK, "(lx*ly) %% world_size != 0");
+    if ((int64_t(l[0]) * l[2]) % world_size) return set_error(A2AFFT_E_CHUNK, "(lx*lz) %% world_size != 0");
+    if ((int64_t(l[1]) * l[2]) % world_size) return set_error(A2AFFT_E_CHUNK, "(ly*lz) %% world_size != 0");
+    if (nlocal >= (int64_t(1) << 32)) return set_error(A2AFFT_E_SIZE, "nlocal >= 2^32");
+    a2afft_dist_s* s = new (std::nothrow) a2afft_dist_s;
+    if (!s) return set_error(A2AFFT_E_ALLOC, "dist_create");
+    s->Ng = Ng;
+    s->P = world_size;
+    for (int a = 0; a < 3; ++a) { s->dims[a] = d[a]; s->lgs[a] = l[a]; }
+    s->nlocal = nlocal;
+    s->blockSize = blockSize;
+    s->dtype = dtype;
+    *out = s;
+    return A2AFFT_OK;
+}
+
+static bool in_reference_domain(const a2afft_dist_s* d) { return d->dims[1] == 2 || d->dims[0] == 1; }
+
+extern "C" int a2afft_dist_get_info(a2afft_dist d, a2afft_dist_info* info) {
+    if (!d || !info) return set_error(A2AFFT_E_INVALID_ARG, "dist_get_info: null");
+    info->Ng = d->Ng;
+    info->nlocal = d->nlocal;
+    info->world_size = d->P;
+    for (int a = 0; a < 3; ++a) {
+        info->local_grid_size[a] = d->lgs[a];
+        info->dims[a] = d->dims[a];
+    }
+    info->blockSize = d->blockSize;
+    info->dtype = d->dtype;
+    info->reference_domain = in_reference_domain(d) ? 1 : 0;
+    return A2AFFT_OK;
+}
+
+// rank2coords (a2adistribution.cpp:55-61), get_local_grid_start (:77-83)
+extern "C" int a2afft_dist_rank_coords(a2afft_dist d, int rank, int coords[3], int start[3]) {
+    if (!d || rank < 0 || rank >= d->P) return set_error(A2AFFT_E_INVALID_ARG, "rank_coords: bad rank");
+    int c[3];
+    c[0] = rank / (d->dims[1] * d->dims[2]);
+    c[1] = (rank - c[0] * (d->dims[1] * d->dims[2])) / d->dims[2];
+    c[2] = rank - c[0] * (d->dims[1] * d->dims[2]) - c[1] * d->dims[2];
+    for (int a = 0; a < 3; ++a) {
+        if (coords) coords[a] = c[a];
+        if (start) start[a] = d->lgs[a] * c[a];
+    }
+    return A2AFFT_OK;
+}
+
+// a2adistribution.cpp:142-152
+extern "C" int a2afft_dist_write_dims(a2afft_dist d, const char* path) {
+    if (!d || !path) return set_error(A2AFFT_E_INVALID_ARG, "write_dims: null");
+    if (d->nlocal > 0x7fffffff) return set_error(A2AFFT_E_SIZE, "write_dims: nlocal does not fit int32");
+    FILE* f = fopen(path, "wb");
+    if (!f) return set_error(A2AFFT_E_INVALID_ARG, "write_dims: cannot open %s", path);
+    int32_t hdr[9] = {d->P, d->Ng, int32_t(d->nlocal), d->dims[0], d->dims[1], d->dims[2],
+                      d->lgs[0], d->lgs[1], d->lgs[2]};
+    size_t w = fwrite(hdr, sizeof(int32_t), 9, f);
+    fclose(f);
+    return w == 9 ? A2AFFT_OK : set_error(A2AFFT_E_INVALID_ARG, "write_dims: short write");
+}
+
+extern "C" int a2afft_dist_destroy(a2afft_dist d) {
+    delete d;
+    return A2AFFT_OK;
+}
+
+// ---- stages (a2adistribution.cpp:156-256) ---------------------------------
+
+enum { Z_A2P = 0, Z_P2A = 1, X_A2P = 2, X_P2A = 3, Y_A2P = 4, Y_P2A = 5 };
+enum { T_ZX = 0, T_XY = 1, T_YZ = 2 };
+
+static int remap(const a2afft_dist_s* d, int kind, const void* s, void* t, hipStream_t st) {
+    return launch_remap(kind, s, t, d->P, d->nlocal, d->lgs, d->dims, d->dtype, d->P, st);
+}
+static int transpose(const a2afft_dist_s* d, int which, const void* s, void* t, hipStream_t st) {
+    return launch_transpose(which, s, t, d->lgs, d->nlocal, d->dtype, d->P, st);
+}
+static int all2all(const a2afft_dist_s* d, const void* s, void* t, hipStream_t st) {
+    return launch_alltoall(s, t, d->P, d->nlocal, d->dtype, st);
+}
+
+static int stage(const a2afft_dist_s* d, int which, void* b1, void* b2, hipStream_t s) {
+    if (!d || !b1 || !b2 || b1 == b2) return set_error(A2AFFT_E_INVALID_ARG, "stage: bad buffers");
+    switch (which) {
+    case 0:  // getZPencils :156-170
+        A2A_TRY(all2all(d, b1, b2, s));
+        return remap(d, Z_A2P, b2, b1, s);
+    case 1:  // returnZPencils :172-186
+        A2A_TRY(remap(d, Z_P2A, b1, b2, s));
+        return all2all(d, b2, b1, s);
+    case 2:  // getXPencils :188-204
+        A2A_TRY(transpose(d, T_ZX, b1, b2, s));
+        A2A_TRY(all2all(d, b2, b1, s));
+        return remap(d, X_A2P, b1, b2, s);
+    case 3:  // returnXPencils :206-220
+        A2A_TRY(remap(d, X_P2A, b2, b1, s));
+        return all2all(d, b1, b2, s);
+    case 4:  // getYPencils :222-238
+        A2A_TRY(transpose(d, T_XY, b2, b1, s));
+        A2A_TRY(all2all(d, b1, b2, s));
+        return remap(d, Y_A2P, b2, b1, s);
+    default: {  // returnYPencils :240-256
+        int w = remap(d, Y_P2A, b2, b1, s);
+        if (w < 0) return w;
+        A2A_TRY(all2all(d, b1, b2, s));
+        A2A_TRY(transpose(d, T_YZ, b2, b1, s));
+        return w;
+    }
+    }
+}
+
+#define A2A_STAGE_ENTRY(NAME, WHICH)                                                       \
+    extern "C" int NAME(a2afft_dist d, void* b1, void* b2, void* stream) {                 \
+        return finish_call(stream, stage(d, WHICH, b1, b2, (hipStream_t)stream));          \
+    }
+A2A_STAGE_ENTRY(a2afft_get_z_pencils, 0)
+A2A_STAGE_ENTRY(a2afft_return_z_pencils, 1)
+A2A_STAGE_ENTRY(a2afft_get_x_pencils, 2)
+A2A_STAGE_ENTRY(a2afft_return_x_pencils, 3)
+A2A_STAGE_ENTRY(a2afft_get_y_pencils, 4)
+A2A_STAGE_ENTRY(a2afft_return_y_pencils, 5)
+
+// ===========================================================================
+// Plan / execute (a2adfft)
+// ===========================================================================
+
+struct a2afft_plan_s {
+    a2afft_dist_s d;
+    a2afft_precision prec;
+    a2afft_exec exec;
+    void* scratch;
+    AxisPass pass[3];   // fused: z, x, y over the block layout
+};
+
+// Block-layout axis maps (a2adistribution.cpp:25-42 restated as offsets).
+static void block_axes(const a2afft_dist_s* d, AxisMap ax[3]) {
+    const int64_t nl = d->nlocal;
+    const int64_t lx = d->lgs[0], ly = d->lgs[1], lz = d->lgs[2];
+    ax[0] = make_axis(uint32_t(lx), uint32_t(d->Ng), int64_t(d->dims[1]) * d->dims[2] * nl, ly * lz);
+    ax[1] = make_axis(uint32_t(ly), uint32_t(d->Ng), int64_t(d->dims[2]) * nl, lz);
+    ax[2] = make_axis(uint32_t(lz), uint32_t(d->Ng), nl, 1);
+}
+
+extern "C" int a2afft_plan_create(a2afft_plan* out, a2afft_dist dist, a2afft_precision prec, a2afft_exec exec) {
+    if (!out || !dist) return set_error(A2AFFT_E_INVALID_ARG, "plan_create: null");
+    *out = nullptr;
+    if (exec != A2AFFT_EXEC_FUSED && exec != A2AFFT_EXEC_STAGED)
+        return set_error(A2AFFT_E_INVALID_ARG, "plan_create: bad exec");
+    if (prec == A2AFFT_FP64 && dist->dtype != A2AFFT_C128)
+        return set_error(A2AFFT_E_INVALID_ARG, "plan_create: FP64 needs complex128 data");
+    if (prec != A2AFFT_FP64 && dist->dtype != A2AFFT_C64)
+        return set_error(A2AFFT_E_INVALID_ARG, "plan_create: FP32 / split precisions need complex64 data");
+    if (prec == A2AFFT_SPLIT_BF16 || prec == A2AFFT_SPLIT_FP16)
+        return set_error(A2AFFT_E_UNSUPPORTED, "plan_create: split precisions not built");
+    if (prec != A2AFFT_FP64 && prec != A2AFFT_FP32) return set_error(A2AFFT_E_INVALID_ARG, "plan_create: precision");
+    if (!fft_length_supported(dist->Ng)) return set_error(A2AFFT_E_UNSUPPORTED, "plan: Ng=%d", dist->Ng);
+    if (int64_t(dist->P) * dist->nlocal >= (int64_t(1) << 40)) return set_error(A2AFFT_E_SIZE, "plan: too large");
+
+    a2afft_plan_s* p = new (std::nothrow) a2afft_plan_s;
+    if (!p) return set_error(A2AFFT_E_ALLOC, "plan_create");
+    p->d = *dist;
+    p->prec = prec;
+    p->exec = exec;
+    p->scratch = nullptr;
+    AxisMap ax[3];
+    block_axes(dist, ax);
+    // z pass: lines (x, y); x pass: tiles (y, z); y pass: tiles (x, z)
+    p->pass[0] = AxisPass{ax[2], ax[0], ax[1], 0, 0, 1.0};
+    p->pass[1] = AxisPass{ax[0], ax[1], ax[2], 1, 0, 1.0};
+    p->pass[2] = AxisPass{ax[1], ax[0], ax[2], 1, 0, 1.0};
+    FftShape sh;
+    for (int i = 0; i < 3; ++i) {
+        int st = fft_pass_shape(p->pass[i], dist->dtype, dist->Ng, &sh);
+        if (st < 0) { delete p; return st; }
+    }
+    // Build the twiddle tables now so execute allocates nothing.
+    if (!twiddle_table(dist->Ng, -1, dist->dtype) || !twiddle_table(dist->Ng, 1, dist->dtype)) {
+        delete p;
+        return set_error(A2AFFT_E_ALLOC, "plan: twiddle tables");
+    }
+    *out = p;
+    return A2AFFT_OK;
+}
+
+extern "C" int a2afft_make_plans(a2afft_plan p, void* scratch) {
+    if (!p) return set_error(A2AFFT_E_INVALID_ARG, "make_plans: null plan");
+    p->scratch = scratch;
+    return A2AFFT_OK;
+}
+
+// a2adfft.cpp:55-188 in the reference stage order.
+static int exec_staged(a2afft_plan_s* p, void* D, int dir, hipStream_t s) {
+    if (!p->scratch) return set_error(A2AFFT_E_NO_SCRATCH, "staged execute needs scratch (make_plans)");
+    const a2afft_dist_s* d = &p->d;
+    void* S = p->scratch;
+    const int64_t total = int64_t(d->P) * d->nlocal;
+    auto fft = [&](void* buf) -> int {
+        AxisPass ps;
+        const int64_t lines = total / d->Ng;
+        ps.fft = make_axis(uint32_t(d->Ng), uint32_t(d->Ng), 0, 1);
+        ps.a1 = make_axis(uint32_t(lines), uint32_t(lines), 0, d->Ng);
+        ps.a2 = make_axis(1, 1, 0, 0);
+        ps.strided = 0;
+        ps.scale_mode = 0;
+        ps.scale = 1.0;
+        return launch_fft_pass(ps, buf, d->dtype, d->Ng, dir, s);
+    };
+    int warn = 0;
+    A2A_TRY(stage(d, 0, D, S, s));
+    A2A_TRY(fft(D));
+    A2A_TRY(stage(d, 1, D, S, s));
+    A2A_TRY(stage(d, 2, D, S, s));
+    A2A_TRY(fft(S));
+    A2A_TRY(stage(d, 3, D, S, s));
+    A2A_TRY(stage(d, 4, D, S, s));
+    A2A_TRY(fft(D));
+    if (dir < 0) {
+        A2A_TRY(launch_copy(S, D, total, d->dtype, s));   // a2adfft.cpp:108
+    } else {
+        const double scale = double(int64_t(d->Ng) * d->Ng * d->Ng);
+        A2A_TRY(launch_scale(S, D, total, scale, d->dtype, s));   // a2adfft.cpp:176
+    }
+    int w = stage(d, 5, D, S, s);
+    if (w < 0) return w;
+    warn = w;
+    return warn;
+}
+
+// Three in-place passes over the block layout: z, x, y (the reference's axis
+// order), with 1/Ng^3 folded into the last inverse pass.
+static int exec_fused(a2afft_plan_s* p, void* D, int dir, hipStream_t s) {
+    const a2afft_dist_s* d = &p->d;
+    for (int i = 0; i < 3; ++i) {
+        AxisPass ps = p->pass[i];
+        if (i == 2 && dir > 0) {
+            ps.scale = double(int64_t(d->Ng) * d->Ng * d->Ng);
+            ps.scale_mode = 2;
+        }
+        A2A_TRY(launch_fft_pass(ps, D, d->dtype, d->Ng, dir, s));
+    }
+    return A2AFFT_OK;
+}
+
+static int execute(a2afft_plan p, void* data, int dir, void* stream) {
+    if (!p || !data) return set_error(A2AFFT_E_INVALID_ARG, "execute: null");
+    if (p->exec == A2AFFT_EXEC_STAGED) return exec_staged(p, data, dir, (hipStream_t)stream);
+    return exec_fused(p, data, dir, (hipStream_t)stream);
+}
+
+extern "C" int a2afft_forward(a2afft_plan p, void* data, void* stream) {
+    return finish_call(stream, execute(p, data, -1, stream));
+}
+
+extern "C" int a2afft_inverse(a2afft_plan p, void* data, void* stream) {
+    return finish_call(stream, execute(p, data, +1, stream));
+}
+
+extern "C" int a2afft_execute_pass(a2afft_plan p, void* data, int pass, int direction, void* stream) {
+    if (!p || !data || pass < 0 || pass > 2 || (direction != -1 && direction != 1))
+        return set_error(A2AFFT_E_INVALID_ARG, "execute_pass: bad arguments");
+    if (p->exec != A2AFFT_EXEC_FUSED) return set_error(A2AFFT_E_INVALID_ARG, "execute_pass: fused plans only");
+    AxisPass ps = p->pass[pass];
+    if (pass == 2 && direction > 0) {
+        ps.scale = double(int64_t(p->d.Ng) * p->d.Ng * p->d.Ng);
+        ps.scale_mode = 2;
+    }
+    return finish_call(stream, launch_fft_pass(ps, data, p->d.dtype, p->d.Ng, direction, (hipStream_t)stream));
+}
+
+extern "C" int a2afft_plan_destroy(a2afft_plan p) {
+    delete p;
+    return A2AFFT_OK;
+}
+
+// ===========================================================================
+// FP64 error reduction
+// ===========================================================================
+
+template <typename V>
+__global__ __launch_bounds__(256) void err_partials(const V* __restrict__ y, const double2* __restrict__ ref,
+                                                    uint64_t n, double* __restrict__ part) {
+    __shared__ double s_d[256], s_r[256], s_m[256];
+    double sd = 0.0, sr = 0.0, mx = 0.0;
+    for (uint64_t i = uint64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += uint64_t(gridDim.x) * 256) {
+        const double2 r = ref[i];
+        const double dx = double(y[i].x) - r.x, dy = double(y[i].y) - r.y;
+        const double d2 = dx * dx + dy * dy;
+        sd += d2;
+        sr += r.x * r.x + r.y * r.y;
+        mx = fmax(mx, sqrt(d2));
+    }
+    s_d[threadIdx.x] = sd;
+    s_r[threadIdx.x] = sr;
+    s_m[threadIdx.x] = mx;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (int(threadIdx.x) < w) {
+            s_d[threadIdx.x] += s_d[threadIdx.x + w];
+            s_r[threadIdx.x] += s_r[threadIdx.x + w];
+            s_m[threadIdx.x] = fmax(s_m[threadIdx.x], s_m[threadIdx.x + w]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[3 * blockIdx.x + 0] = s_d[0];
+        part[3 * blockIdx.x + 1] = s_r[0];
+        part[3 * blockIdx.x + 2] = s_m[0];
+    }
+}
+
+extern "C" int a2afft_error_metrics(const void* y, a2afft_dtype dtype, const void* ref, int64_t n, double* rel_l2,
+                                    double* max_abs, void* stream) {
+    if (!y || !ref || n <= 0 || !rel_l2 || !max_abs) return set_error(A2AFFT_E_INVALID_ARG, "error_metrics: bad args");
+    const int blocks = 1024;
+    double* part = nullptr;
+    A2A_TRY(hip_check(int(hipMalloc(&part, sizeof(double) * 3 * blocks)), "error_metrics alloc"));
+    hipStream_t s = (hipStream_t)stream;
+    if (dtype == A2AFFT_C128)
+        hipLaunchKernelGGL(err_partials<double2>, dim3(blocks), dim3(256), 0, s, (const double2*)y,
+                           (const double2*)ref, uint64_t(n), part);
+    else
+        hipLaunchKernelGGL(err_partials<float2>, dim3(blocks), dim3(256), 0, s, (const float2*)y,
+                           (const double2*)ref, uint64_t(n), part);
+    int st = hip_check(int(hipGetLastError()), "error_metrics launch");
+    double h[3 * 1024];
+    if (st == A2AFFT_OK) st = hip_check(int(hipStreamSynchronize(s)), "error_metrics sync");
+    if (st == A2AFFT_OK)
+        st = hip_check(int(hipMemcpy(h, part, sizeof(h), hipMemcpyDeviceToHost)), "error_metrics copy");
+    (void)hipFree(part);
+    if (st < 0) return st;
+    long double sd = 0.0L, sr = 0.0L;
+    double mx = 0.0;
+    for (int b = 0; b < blocks; ++b) {
+        sd += h[3 * b];
+        sr += h[3 * b + 1];
+        mx = h[3 * b + 2] > mx ? h[3 * b + 2] : mx;
+    }
+    *rel_l2 = sr > 0 ? double(sqrtl(sd / sr)) : double(sqrtl(sd));
+    *max_abs = mx;
+    return A2AFFT_OK;
+}

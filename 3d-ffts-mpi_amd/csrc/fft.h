@@ -1,0 +1,44 @@
+// Host-side interface of the Stockham FFT kernels.
+#pragma once
+
+#include "common.h"
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#endif
+
+namespace a2a {
+
+// One batched 1D FFT pass over an arbitrary strided layout.  Lines are
+// enumerated by (u1 < a1.n, u2 < a2.n); the element n of line (u1, u2) sits at
+// a1.off(u1) + a2.off(u2) + fft.off(n).  strided = 1 means the lines along
+// a2 are adjacent in memory (a2.s_elem == 1 inside a block) and a workgroup
+// takes T of them side by side; strided = 0 means each line's own elements
+// are the contiguous ones.
+struct AxisPass {
+    AxisMap fft, a1, a2;
+    int strided;
+    int scale_mode;   // 0 none, 1 multiply, 2 divide
+    double scale;
+};
+
+// Supported transform lengths: powers of two 2..4096.
+bool fft_length_supported(int N);
+
+// Device twiddle table for (N, dir, dtype); created on first use, cached for
+// the life of the process (per device).  Returns null on failure.
+const void* twiddle_table(int N, int dir, a2afft_dtype dtype);
+
+// Launch one pass.  data: complex of dtype.  dir: -1 forward, +1 inverse.
+int launch_fft_pass(const AxisPass& pass, void* data, a2afft_dtype dtype, int N, int dir, hipStream_t s);
+
+// Launch shape the dispatcher picks for a pass (for tests and tuning reports).
+struct FftShape {
+    int T, E, NT, grid, lds_bytes, fallback;
+};
+int fft_pass_shape(const AxisPass& pass, a2afft_dtype dtype, int N, FftShape* out);
+
+// exp(dir*2*pi*i*m/M) in long double (quadrant-reduced), for table building.
+void twiddle_ld(int64_t m, int64_t M, int dir, long double* re, long double* im);
+
+}  // namespace a2a

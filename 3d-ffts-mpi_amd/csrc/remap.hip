@@ -1,0 +1,376 @@
+// Remap kernels of the a2afft hot path on gfx950: the six pack/unpack index
+// maps (src/a2afft/distribution/kernels/a2apencils.cu), the three fast-axis
+// transposes (a2achangefastaxis.cu), the device-local all-to-all that
+// replaces MPI_Alltoall (a2adistribution.cpp:158-166), and the copy / scale
+// kernels of dfft/kernels/a2affts.cu:78-126.
+//
+// All remaps are pure permutations of complex values: the index arithmetic is
+// the reference's, so results are bit-identical by construction.  Pack/unpack
+// maps move runs of l_axis contiguous elements (idx % l_axis is preserved), so
+// one thread moves a 16-byte unit and both the load and the store coalesce.
+// The transposes stage 32x32 tiles through LDS.
+#include "common.h"
+#include "remap.h"
+
+#include <stdio.h>
+
+namespace a2a {
+
+// ---------------------------------------------------------------------------
+// Index maps (one rank, scatter form: returns where source element idx goes).
+// ---------------------------------------------------------------------------
+
+enum RemapKind {
+    Z_A2A_TO_PENCILS = 0,
+    Z_PENCILS_TO_A2A = 1,
+    X_A2A_TO_PENCILS = 2,
+    X_PENCILS_TO_A2A = 3,
+    Y_A2A_TO_PENCILS = 4,
+    Y_PENCILS_TO_A2A = 5,
+};
+
+struct RemapParams {
+    uint32_t L;      // n_cells_mini_pencils (l_axis)
+    uint32_t cpr;    // n_cells_per_rank = nlocal / world_size
+    uint32_t mpr;    // n_mini_pencils_per_rank = cpr / L
+    uint32_t st;     // n_mini_pencils_stacked (dims[axis])
+    uint32_t cps;    // n_cells_per_stack (x maps) = nlocal / dims[0]
+    uint32_t rpt;    // n_ranks_per_tower (y maps) = (P / dims[1]) / dims[0]
+    FastDiv dL, dcpr, dmpr, dst, dst_cpr, dL_st, dcps, dcps_L, dcpr_st, drpt_mpr,
+        drpt_st_cpr, drpt_cpr, drpt;
+};
+
+template <int KIND>
+A2A_HD uint32_t remap_dest(uint32_t idx, const RemapParams& p) {
+    const uint32_t run = p.dL.div(idx);
+    const uint32_t i = idx - run * p.L;
+    if (KIND == Z_A2A_TO_PENCILS) {            // a2apencils.cu:9-15
+        uint32_t rank = p.dcpr.div(idx);
+        uint32_t mp = p.dmpr.mod(run);
+        uint32_t mpo = p.dst.mod(rank) * p.L;
+        uint32_t stack = p.dst_cpr.div(idx);
+        return stack * p.st * p.cpr + mpo + mp * p.st * p.L + i;
+    } else if (KIND == Z_PENCILS_TO_A2A) {     // a2apencils.cu:42-50
+        uint32_t stack = p.dst_cpr.div(idx);
+        uint32_t mp = p.dst.mod(run);
+        uint32_t rank = stack * p.st + mp;
+        uint32_t pid = p.dmpr.mod(p.dL_st.div(idx));
+        return rank * p.cpr + pid * p.L + i;
+    } else if (KIND == X_A2A_TO_PENCILS) {     // a2apencils.cu:78-85
+        uint32_t mp = p.dcps_L.mod(run);
+        uint32_t stack = p.dcps.div(idx);
+        return stack * p.L + mp * p.st * p.L + i;
+    } else if (KIND == X_PENCILS_TO_A2A) {     // a2apencils.cu:115-123
+        uint32_t stack = p.dst.mod(run);
+        uint32_t rank = p.dcpr_st.div(idx) + stack * (p.cps / p.cpr);
+        uint32_t pid = p.dmpr.mod(p.dL_st.div(idx));
+        return rank * p.cpr + pid * p.L + i;
+    } else if (KIND == Y_A2A_TO_PENCILS) {     // a2apencils.cu:153-161
+        uint32_t mp = p.drpt_mpr.mod(run);
+        uint32_t stack = p.drpt_st_cpr.div(idx);
+        uint32_t lst = p.dst.mod(p.drpt_cpr.div(idx));
+        return mp * p.st * p.L + lst * p.L + stack * p.rpt * p.st * p.cpr + i;
+    } else {                                   // Y_PENCILS_TO_A2A, a2apencils.cu:193-201
+        // The reference's rank expression (:197) is
+        //   mp + stack + (stack / rpt) * rpt,
+        // which inverts y_a2a_to_y_pencils only when dims[1] == 2 or
+        // dims[0] == 1.  This is the exact inverse for every dims, and equals
+        // the reference's expression inside that domain.
+        uint32_t mp = p.dst.mod(run) * p.rpt;
+        uint32_t stack = p.dst.div(p.dcpr.div(idx));
+        uint32_t rank = mp + p.drpt.mod(stack) + p.drpt.div(stack) * p.rpt * p.st;
+        uint32_t lst = p.dmpr.mod(p.dL_st.div(idx));
+        return rank * p.cpr + lst * p.L + i;
+    }
+}
+
+static RemapParams make_params(int kind, int P, int64_t nlocal, const int lgs[3], const int dims[3]) {
+    RemapParams p{};
+    int axis = (kind <= 1) ? 2 : (kind <= 3 ? 0 : 1);
+    p.L = uint32_t(lgs[axis]);
+    p.cpr = uint32_t(nlocal / P);
+    p.mpr = p.cpr / p.L;
+    p.st = uint32_t(dims[axis]);
+    p.cps = uint32_t(nlocal / dims[0]);
+    p.rpt = uint32_t((P / dims[1]) / dims[0]);
+    if (p.rpt == 0) p.rpt = 1;
+    p.dL.set(p.L);
+    p.dcpr.set(p.cpr);
+    p.dmpr.set(p.mpr ? p.mpr : 1);
+    p.dst.set(p.st);
+    p.dst_cpr.set(p.st * p.cpr);
+    p.dL_st.set(p.L * p.st);
+    p.dcps.set(p.cps);
+    p.dcps_L.set(p.cps / p.L ? p.cps / p.L : 1);
+    p.dcpr_st.set(p.cpr * p.st);
+    p.drpt_mpr.set(p.rpt * p.mpr ? p.rpt * p.mpr : 1);
+    p.drpt_st_cpr.set(p.rpt * p.st * p.cpr);
+    p.drpt_cpr.set(p.rpt * p.cpr);
+    p.drpt.set(p.rpt);
+    return p;
+}
+
+// One thread moves one 16-byte unit (VEC complex values of one run).
+template <int KIND, typename U, int VEC>
+__global__ __launch_bounds__(256) void remap_runs_kernel(const U* __restrict__ src, U* __restrict__ dst,
+                                                         RemapParams p, uint32_t units_per_rank) {
+    const uint32_t u = blockIdx.x * 256u + threadIdx.x;
+    if (u >= units_per_rank) return;
+    const size_t rbase = size_t(blockIdx.y) * units_per_rank;
+    const uint32_t nd = remap_dest<KIND>(u * VEC, p) / VEC;
+    dst[rbase + nd] = src[rbase + u];
+}
+
+// recv[s][q*ns + i] = send[q][s*ns + i]; grid.y = source rank q.
+template <typename U>
+__global__ __launch_bounds__(256) void alltoall_kernel(const U* __restrict__ send, U* __restrict__ recv,
+                                                       uint32_t units_per_rank, uint32_t ns_units,
+                                                       FastDiv dns) {
+    const uint32_t e = blockIdx.x * 256u + threadIdx.x;
+    if (e >= units_per_rank) return;
+    const uint32_t q = blockIdx.y;
+    const uint32_t s = dns.div(e);
+    const uint32_t i = e - s * ns_units;
+    recv[size_t(s) * units_per_rank + size_t(q) * ns_units + i] = send[size_t(q) * units_per_rank + e];
+}
+
+// Batched 3D transpose with a fixed middle axis, per rank buffer:
+//   dst[((b*C + c)*M + m)*A + a] = src[((b*A + a)*M + m)*C + c]
+// fast_z_to_x: (B,A,M,C) = (1, lx, ly, lz)          a2achangefastaxis.cu:14-15
+// fast_x_to_y: (B,A,M,C) = (1, lz*ly, 1, lx)        a2achangefastaxis.cu:45-46
+// fast_y_to_z: (B,A,M,C) = (lx, lz, 1, ly)          a2achangefastaxis.cu:76-77
+// grid.x: tiles of 32x32 over (a, c); grid.y: b*M + m; grid.z: rank.
+template <typename V>
+__global__ __launch_bounds__(256) void transpose_kernel(const V* __restrict__ src, V* __restrict__ dst,
+                                                        uint32_t A, uint32_t M, uint32_t C,
+                                                        uint32_t tiles_c, uint64_t rank_elems) {
+    __shared__ V tile[32][33];
+    const uint32_t ta = blockIdx.x / tiles_c, tc = blockIdx.x - ta * tiles_c;
+    const uint32_t bm = blockIdx.y;
+    const uint32_t b = bm / M, m = bm - b * M;
+    const size_t rbase = size_t(blockIdx.z) * rank_elems;
+    const size_t bbase = size_t(b) * A * M * C;
+    const uint32_t tx = threadIdx.x & 31u, ty = threadIdx.x >> 5;   // 32 x 8
+    const uint32_t a0 = ta * 32u, c0 = tc * 32u;
+#pragma unroll
+    for (uint32_t r = 0; r < 32; r += 8) {
+        uint32_t a = a0 + ty + r, c = c0 + tx;
+        if (a < A && c < C) tile[ty + r][tx] = src[rbase + bbase + (size_t(a) * M + m) * C + c];
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t r = 0; r < 32; r += 8) {
+        uint32_t c = c0 + ty + r, a = a0 + tx;
+        if (a < A && c < C) dst[rbase + bbase + (size_t(c) * M + m) * A + a] = tile[tx][ty + r];
+    }
+}
+
+template <typename V>
+__global__ __launch_bounds__(256) void copy_kernel(V* __restrict__ out, const V* __restrict__ in, uint64_t n) {
+    const uint64_t i = uint64_t(blockIdx.x) * 256u + threadIdx.x;
+    if (i < n) out[i] = in[i];
+}
+
+// out = data / scale, the division of scale_fft (a2affts.cu:83-88) per component.
+template <typename R>
+__global__ __launch_bounds__(256) void scale_kernel(R* __restrict__ out, const R* __restrict__ in, uint64_t n,
+                                                    R scale) {
+    const uint64_t i = uint64_t(blockIdx.x) * 256u + threadIdx.x;
+    if (i < n) out[i] = in[i] / scale;
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers
+// ---------------------------------------------------------------------------
+
+static int check_geometry(int world_size, int64_t nlocal, const int lgs[3], const int dims[3], int nbuffers) {
+    if (world_size <= 0 || nlocal <= 0 || !lgs || !dims || nbuffers <= 0)
+        return set_error(A2AFFT_E_INVALID_ARG, "remap: bad arguments");
+    if (nbuffers > 65535) return set_error(A2AFFT_E_SIZE, "remap: more than 65535 rank buffers");
+    if (int64_t(lgs[0]) * lgs[1] * lgs[2] != nlocal)
+        return set_error(A2AFFT_E_INVALID_ARG, "remap: nlocal != lx*ly*lz");
+    if (nlocal >= (int64_t(1) << 32)) return set_error(A2AFFT_E_SIZE, "remap: nlocal >= 2^32");
+    if (int64_t(dims[0]) * dims[1] * dims[2] != world_size)
+        return set_error(A2AFFT_E_RANKS, "remap: dims product != world_size");
+    for (int a = 0; a < 3; ++a)
+        if (lgs[a] <= 0 || dims[a] <= 0) return set_error(A2AFFT_E_INVALID_ARG, "remap: non-positive extent");
+    if (nlocal % world_size) return set_error(A2AFFT_E_CHUNK, "remap: nlocal %% world_size != 0");
+    if ((int64_t(lgs[0]) * lgs[1]) % world_size || (int64_t(lgs[0]) * lgs[2]) % world_size ||
+        (int64_t(lgs[1]) * lgs[2]) % world_size)
+        return set_error(A2AFFT_E_CHUNK, "remap: (l_a*l_b) %% world_size != 0 (a2adistribution.cpp:119-121)");
+    for (int a = 0; a < 3; ++a)
+        if (world_size % dims[a]) return set_error(A2AFFT_E_RANKS, "remap: world_size %% dims != 0");
+    return A2AFFT_OK;
+}
+
+template <int KIND>
+static int launch_kind(const void* src, void* dst, const RemapParams& p, int64_t nlocal, int esize, int nbuffers,
+                       hipStream_t s) {
+    const bool vec2 = (esize == 8) && (p.L % 2 == 0);
+    const uint32_t units = uint32_t(vec2 ? nlocal / 2 : nlocal);
+    dim3 grid((units + 255) / 256, nbuffers);
+    if (esize == 16)
+        hipLaunchKernelGGL((remap_runs_kernel<KIND, uint4, 1>), grid, dim3(256), 0, s, (const uint4*)src,
+                           (uint4*)dst, p, units);
+    else if (vec2)
+        hipLaunchKernelGGL((remap_runs_kernel<KIND, uint4, 2>), grid, dim3(256), 0, s, (const uint4*)src,
+                           (uint4*)dst, p, units);
+    else
+        hipLaunchKernelGGL((remap_runs_kernel<KIND, uint2, 1>), grid, dim3(256), 0, s, (const uint2*)src,
+                           (uint2*)dst, p, units);
+    return hip_check(int(hipGetLastError()), "remap kernel launch");
+}
+
+int launch_remap(int kind, const void* src, void* dst, int world_size, int64_t nlocal, const int lgs[3],
+                 const int dims[3], a2afft_dtype dtype, int nbuffers, hipStream_t s) {
+    A2A_TRY(check_geometry(world_size, nlocal, lgs, dims, nbuffers));
+    if (!src || !dst) return set_error(A2AFFT_E_INVALID_ARG, "remap: null buffer");
+    if (src == dst) return set_error(A2AFFT_E_INVALID_ARG, "remap: source and dest alias");
+    const int es = esize_of(dtype);
+    RemapParams p = make_params(kind, world_size, nlocal, lgs, dims);
+    switch (kind) {
+    case Z_A2A_TO_PENCILS: return launch_kind<Z_A2A_TO_PENCILS>(src, dst, p, nlocal, es, nbuffers, s);
+    case Z_PENCILS_TO_A2A: return launch_kind<Z_PENCILS_TO_A2A>(src, dst, p, nlocal, es, nbuffers, s);
+    case X_A2A_TO_PENCILS: return launch_kind<X_A2A_TO_PENCILS>(src, dst, p, nlocal, es, nbuffers, s);
+    case X_PENCILS_TO_A2A: return launch_kind<X_PENCILS_TO_A2A>(src, dst, p, nlocal, es, nbuffers, s);
+    case Y_A2A_TO_PENCILS: return launch_kind<Y_A2A_TO_PENCILS>(src, dst, p, nlocal, es, nbuffers, s);
+    case Y_PENCILS_TO_A2A: {
+        int st = launch_kind<Y_PENCILS_TO_A2A>(src, dst, p, nlocal, es, nbuffers, s);
+        if (st < 0) return st;
+        return (dims[1] == 2 || dims[0] == 1) ? A2AFFT_OK : A2AFFT_W_DIVERGES_FROM_REFERENCE;
+    }
+    default: return set_error(A2AFFT_E_INVALID_ARG, "remap: unknown kind %d", kind);
+    }
+}
+
+int launch_transpose(int which, const void* src, void* dst, const int lgs[3], int64_t nlocal, a2afft_dtype dtype,
+                     int nbuffers, hipStream_t s) {
+    if (!src || !dst || !lgs || nbuffers <= 0 || nlocal <= 0)
+        return set_error(A2AFFT_E_INVALID_ARG, "transpose: bad arguments");
+    if (src == dst) return set_error(A2AFFT_E_INVALID_ARG, "transpose: source and dest alias");
+    if (int64_t(lgs[0]) * lgs[1] * lgs[2] != nlocal)
+        return set_error(A2AFFT_E_INVALID_ARG, "transpose: nlocal != lx*ly*lz");
+    if (nbuffers > 65535) return set_error(A2AFFT_E_SIZE, "transpose: more than 65535 rank buffers");
+    const uint32_t lx = lgs[0], ly = lgs[1], lz = lgs[2];
+    uint32_t B, A, M, C;
+    if (which == 0) { B = 1; A = lx; M = ly; C = lz; }            // fast_z_to_x
+    else if (which == 1) { B = 1; A = lz * ly; M = 1; C = lx; }   // fast_x_to_y
+    else { B = lx; A = lz; M = 1; C = ly; }                       // fast_y_to_z
+    const uint32_t ta = (A + 31) / 32, tc = (C + 31) / 32;
+    if (uint64_t(ta) * tc >= (uint64_t(1) << 31) || uint64_t(B) * M > 65535)
+        return set_error(A2AFFT_E_SIZE, "transpose: grid too large");
+    dim3 grid(ta * tc, B * M, nbuffers);
+    if (dtype == A2AFFT_C128)
+        hipLaunchKernelGGL(transpose_kernel<double2>, grid, dim3(256), 0, s, (const double2*)src, (double2*)dst, A,
+                           M, C, tc, uint64_t(nlocal));
+    else
+        hipLaunchKernelGGL(transpose_kernel<float2>, grid, dim3(256), 0, s, (const float2*)src, (float2*)dst, A, M,
+                           C, tc, uint64_t(nlocal));
+    return hip_check(int(hipGetLastError()), "transpose kernel launch");
+}
+
+int launch_alltoall(const void* send, void* recv, int world_size, int64_t nlocal, a2afft_dtype dtype,
+                    hipStream_t s) {
+    if (!send || !recv || world_size <= 0 || nlocal <= 0 || nlocal % world_size)
+        return set_error(A2AFFT_E_INVALID_ARG, "alltoall: bad arguments");
+    if (send == recv) return set_error(A2AFFT_E_INVALID_ARG, "alltoall: send and recv alias");
+    if (world_size > 65535 || nlocal >= (int64_t(1) << 32)) return set_error(A2AFFT_E_SIZE, "alltoall: too large");
+    const int es = esize_of(dtype);
+    const int64_t ns = nlocal / world_size;
+    const bool vec2 = (es == 8) && (ns % 2 == 0);
+    const uint32_t upr = uint32_t(vec2 ? nlocal / 2 : nlocal);
+    const uint32_t nsu = uint32_t(vec2 ? ns / 2 : ns);
+    FastDiv dns(nsu);
+    dim3 grid((upr + 255) / 256, world_size);
+    if (es == 16 || vec2)
+        hipLaunchKernelGGL(alltoall_kernel<uint4>, grid, dim3(256), 0, s, (const uint4*)send, (uint4*)recv, upr,
+                           nsu, dns);
+    else
+        hipLaunchKernelGGL(alltoall_kernel<uint2>, grid, dim3(256), 0, s, (const uint2*)send, (uint2*)recv, upr,
+                           nsu, dns);
+    return hip_check(int(hipGetLastError()), "alltoall kernel launch");
+}
+
+int launch_copy(void* out, const void* in, int64_t n, a2afft_dtype dtype, hipStream_t s) {
+    if (!out || !in || n < 0) return set_error(A2AFFT_E_INVALID_ARG, "copy: bad arguments");
+    if (n == 0) return A2AFFT_OK;
+    const uint64_t units = dtype == A2AFFT_C128 ? uint64_t(n) : uint64_t(n);
+    const uint64_t blocks = (units + 255) / 256;
+    if (blocks >= (uint64_t(1) << 31)) return set_error(A2AFFT_E_SIZE, "copy: too large");
+    if (dtype == A2AFFT_C128)
+        hipLaunchKernelGGL(copy_kernel<uint4>, dim3(blocks), dim3(256), 0, s, (uint4*)out, (const uint4*)in, units);
+    else
+        hipLaunchKernelGGL(copy_kernel<uint2>, dim3(blocks), dim3(256), 0, s, (uint2*)out, (const uint2*)in, units);
+    return hip_check(int(hipGetLastError()), "copy kernel launch");
+}
+
+int launch_scale(void* out, const void* in, int64_t n, double scale, a2afft_dtype dtype, hipStream_t s) {
+    if (!out || !in || n < 0) return set_error(A2AFFT_E_INVALID_ARG, "scale: bad arguments");
+    if (n == 0) return A2AFFT_OK;
+    const uint64_t reals = uint64_t(n) * 2;
+    const uint64_t blocks = (reals + 255) / 256;
+    if (blocks >= (uint64_t(1) << 31)) return set_error(A2AFFT_E_SIZE, "scale: too large");
+    if (dtype == A2AFFT_C128)
+        hipLaunchKernelGGL(scale_kernel<double>, dim3(blocks), dim3(256), 0, s, (double*)out, (const double*)in, reals,
+                           scale);
+    else
+        hipLaunchKernelGGL(scale_kernel<float>, dim3(blocks), dim3(256), 0, s, (float*)out, (const float*)in, reals,
+                           float(scale));
+    return hip_check(int(hipGetLastError()), "scale kernel launch");
+}
+
+}  // namespace a2a
+
+// ---------------------------------------------------------------------------
+// C ABI: launcher level (distribution/kernels/kernels.h:3-17,
+// dfft/kernels/kernels.h:9-11).
+// ---------------------------------------------------------------------------
+using namespace a2a;
+
+#define A2A_REMAP_ENTRY(NAME, KIND)                                                                         \
+    extern "C" int NAME(const void* source, void* dest, int blockSize, int world_size, int64_t nlocal,      \
+                        const int local_grid_size[3], const int dims[3], a2afft_dtype dtype, int nbuffers,  \
+                        void* stream) {                                                                     \
+        if (blockSize <= 0 || blockSize > 1024) return set_error(A2AFFT_E_INVALID_ARG, "bad blockSize");   \
+        int st = launch_remap(KIND, source, dest, world_size, nlocal, local_grid_size, dims, dtype, nbuffers, \
+                              (hipStream_t)stream);                                                         \
+        return finish_call(stream, st);                                                                     \
+    }
+
+A2A_REMAP_ENTRY(a2afft_launch_d_z_a2a_to_z_pencils, Z_A2A_TO_PENCILS)
+A2A_REMAP_ENTRY(a2afft_launch_d_z_pencils_to_z_a2a, Z_PENCILS_TO_A2A)
+A2A_REMAP_ENTRY(a2afft_launch_d_x_a2a_to_x_pencils, X_A2A_TO_PENCILS)
+A2A_REMAP_ENTRY(a2afft_launch_d_x_pencils_to_x_a2a, X_PENCILS_TO_A2A)
+A2A_REMAP_ENTRY(a2afft_launch_d_y_a2a_to_y_pencils, Y_A2A_TO_PENCILS)
+A2A_REMAP_ENTRY(a2afft_launch_d_y_pencils_to_y_a2a, Y_PENCILS_TO_A2A)
+
+#define A2A_TRANSPOSE_ENTRY(NAME, WHICH)                                                                    \
+    extern "C" int NAME(const void* source, void* dest, const int local_grid_size[3], int blockSize,        \
+                        int64_t nlocal, a2afft_dtype dtype, int nbuffers, void* stream) {                   \
+        if (blockSize <= 0 || blockSize > 1024) return set_error(A2AFFT_E_INVALID_ARG, "bad blockSize");   \
+        int st = launch_transpose(WHICH, source, dest, local_grid_size, nlocal, dtype, nbuffers,            \
+                                  (hipStream_t)stream);                                                     \
+        return finish_call(stream, st);                                                                     \
+    }
+
+A2A_TRANSPOSE_ENTRY(a2afft_launch_d_fast_z_to_x, 0)
+A2A_TRANSPOSE_ENTRY(a2afft_launch_d_fast_x_to_y, 1)
+A2A_TRANSPOSE_ENTRY(a2afft_launch_d_fast_y_to_z, 2)
+
+extern "C" int a2afft_alltoall(const void* send, void* recv, int world_size, int64_t nlocal, a2afft_dtype dtype,
+                               void* stream) {
+    return finish_call(stream, launch_alltoall(send, recv, world_size, nlocal, dtype, (hipStream_t)stream));
+}
+
+extern "C" int a2afft_launch_scale_fft(void* out, const void* data, int Ng, int64_t n_elems, int blockSize,
+                                       a2afft_dtype dtype, void* stream) {
+    if (Ng <= 0 || blockSize <= 0 || blockSize > 1024) return set_error(A2AFFT_E_INVALID_ARG, "scale: bad args");
+    const double scale = double(int64_t(Ng) * Ng * Ng);
+    return finish_call(stream, launch_scale(out, data, n_elems, scale, dtype, (hipStream_t)stream));
+}
+
+extern "C" int a2afft_launch_fast_copy_fft(void* out, const void* data, int64_t n_elems, int blockSize,
+                                           a2afft_dtype dtype, void* stream) {
+    if (blockSize <= 0 || blockSize > 1024) return set_error(A2AFFT_E_INVALID_ARG, "copy: bad blockSize");
+    return finish_call(stream, launch_copy(out, data, n_elems, dtype, (hipStream_t)stream));
+}

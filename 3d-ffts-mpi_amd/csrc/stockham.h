@@ -1,0 +1,307 @@
+// Stockham radix-4/8/16 FFT kernels for gfx950.
+//
+// One workgroup owns T complete lines of length N.  Each thread holds E
+// elements of one line in registers: element n = t + e*TPL (TPL = N/E
+// threads per line).  A radix-R pass (decimation in time, autosort) reads
+// x[j + r*N/R] for butterfly j, which for j = t + b*TPL is exactly the
+// thread's own register e = b + r*(E/R); so the first pass runs straight
+// from the global load, the last pass leaves its outputs in the registers
+// the global store needs (out[j + q*N/R]), and only the passes in between
+// exchange through LDS.  Global loads and stores therefore touch the same
+// addresses, which makes the kernel in-place safe: a workgroup reads all of
+// its lines before it writes any of them.
+//
+// Two line shapes:
+//   contiguous (STRIDED=false): lane order runs along the line, so each
+//     wavefront reads whole 256-512 B runs; used for the z axis of the block
+//     layout (segments of lz elements, one per rank along z) and for batched
+//     contiguous pencils.
+//   strided (STRIDED=true): the T lines of a workgroup are T consecutive z
+//     columns, lane order runs across the lines, so a wavefront reads rows of
+//     T*sizeof(complex) = 128 B; used for the x and y axes.
+//
+// Arithmetic is written with explicit fma and built with -ffp-contract=off,
+// so a line is computed bit-identically whichever shape or launch computes it.
+#pragma once
+
+#include "common.h"
+
+namespace a2a {
+
+template <typename real> struct CT;
+template <> struct CT<float> { using V = float2; };
+template <> struct CT<double> { using V = double2; };
+
+template <typename V> __device__ __forceinline__ V c_add(V a, V b) {
+    V r; r.x = a.x + b.x; r.y = a.y + b.y; return r;
+}
+template <typename V> __device__ __forceinline__ V c_sub(V a, V b) {
+    V r; r.x = a.x - b.x; r.y = a.y - b.y; return r;
+}
+template <typename V> __device__ __forceinline__ V c_mul(V a, V w) {
+    V r;
+    r.x = fma(a.x, w.x, -(a.y * w.y));
+    r.y = fma(a.x, w.y, a.y * w.x);
+    return r;
+}
+// a * exp(DIR * i*pi/2)
+template <int DIR, typename V> __device__ __forceinline__ V c_w4(V a) {
+    V r;
+    if (DIR < 0) { r.x = a.y; r.y = -a.x; } else { r.x = -a.y; r.y = a.x; }
+    return r;
+}
+
+template <typename real> struct K;
+template <> struct K<float> {
+    static constexpr float r2 = 0.70710678118654752440f;   // sqrt(2)/2
+    static constexpr float c8 = 0.92387953251128675613f;   // cos(pi/8)
+    static constexpr float s8 = 0.38268343236508977173f;   // sin(pi/8)
+};
+template <> struct K<double> {
+    static constexpr double r2 = 0.70710678118654752440;
+    static constexpr double c8 = 0.92387953251128675613;
+    static constexpr double s8 = 0.38268343236508977173;
+};
+
+// a * exp(DIR * i*pi/4)
+template <int DIR, typename V> __device__ __forceinline__ V c_w8(V a) {
+    using real = decltype(a.x);
+    const real c = K<real>::r2;
+    V r;
+    if (DIR < 0) { r.x = (a.x + a.y) * c; r.y = (a.y - a.x) * c; }
+    else { r.x = (a.x - a.y) * c; r.y = (a.x + a.y) * c; }
+    return r;
+}
+// a * exp(DIR * 3i*pi/4)
+template <int DIR, typename V> __device__ __forceinline__ V c_w8_3(V a) {
+    using real = decltype(a.x);
+    const real c = K<real>::r2;
+    V r;
+    if (DIR < 0) { r.x = (a.y - a.x) * c; r.y = -((a.x + a.y) * c); }
+    else { r.x = -((a.x + a.y) * c); r.y = (a.x - a.y) * c; }
+    return r;
+}
+// a * exp(DIR * i*pi/8) and a * exp(DIR * 3i*pi/8)
+template <int DIR, typename V> __device__ __forceinline__ V c_w16(V a) {
+    using real = decltype(a.x);
+    V w; w.x = K<real>::c8; w.y = DIR < 0 ? -K<real>::s8 : K<real>::s8;
+    return c_mul(a, w);
+}
+template <int DIR, typename V> __device__ __forceinline__ V c_w16_3(V a) {
+    using real = decltype(a.x);
+    V w; w.x = K<real>::s8; w.y = DIR < 0 ? -K<real>::c8 : K<real>::c8;
+    return c_mul(a, w);
+}
+
+// In-place R-point DFT, natural order in and out: y[q] = sum_r x[r] W_R^{DIR*r*q}.
+template <int DIR, typename V> __device__ __forceinline__ void dft2(V& a, V& b) {
+    V t = a; a = c_add(t, b); b = c_sub(t, b);
+}
+template <int DIR, typename V> __device__ __forceinline__ void dft4(V& x0, V& x1, V& x2, V& x3) {
+    V t0 = c_add(x0, x2), t1 = c_sub(x0, x2);
+    V t2 = c_add(x1, x3), t3 = c_w4<DIR>(c_sub(x1, x3));
+    x0 = c_add(t0, t2); x2 = c_sub(t0, t2);
+    x1 = c_add(t1, t3); x3 = c_sub(t1, t3);
+}
+template <int DIR, typename V> __device__ __forceinline__ void dft8(V* x) {
+    V e0 = x[0], e1 = x[2], e2 = x[4], e3 = x[6];
+    V o0 = x[1], o1 = x[3], o2 = x[5], o3 = x[7];
+    dft4<DIR>(e0, e1, e2, e3);
+    dft4<DIR>(o0, o1, o2, o3);
+    o1 = c_w8<DIR>(o1);
+    o2 = c_w4<DIR>(o2);
+    o3 = c_w8_3<DIR>(o3);
+    x[0] = c_add(e0, o0); x[4] = c_sub(e0, o0);
+    x[1] = c_add(e1, o1); x[5] = c_sub(e1, o1);
+    x[2] = c_add(e2, o2); x[6] = c_sub(e2, o2);
+    x[3] = c_add(e3, o3); x[7] = c_sub(e3, o3);
+}
+template <int DIR, typename V> __device__ __forceinline__ void dft16(V* x) {
+    // n = n1 + 4*n2, k = k1 + 4*k2: inner DFT4 over n2, twiddle W16^{n1*k1},
+    // outer DFT4 over n1.
+    V a[4][4];
+#pragma unroll
+    for (int n1 = 0; n1 < 4; ++n1) {
+        a[n1][0] = x[n1]; a[n1][1] = x[n1 + 4]; a[n1][2] = x[n1 + 8]; a[n1][3] = x[n1 + 12];
+        dft4<DIR>(a[n1][0], a[n1][1], a[n1][2], a[n1][3]);
+    }
+    a[1][1] = c_w16<DIR>(a[1][1]);
+    a[1][2] = c_w8<DIR>(a[1][2]);
+    a[1][3] = c_w16_3<DIR>(a[1][3]);
+    a[2][1] = c_w8<DIR>(a[2][1]);
+    a[2][2] = c_w4<DIR>(a[2][2]);
+    a[2][3] = c_w8_3<DIR>(a[2][3]);
+    a[3][1] = c_w16_3<DIR>(a[3][1]);
+    a[3][2] = c_w8_3<DIR>(a[3][2]);
+    {   // W16^9 = -W16^1
+        V t = c_w16<DIR>(a[3][3]);
+        a[3][3].x = -t.x; a[3][3].y = -t.y;
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+        V y0 = a[0][k1], y1 = a[1][k1], y2 = a[2][k1], y3 = a[3][k1];
+        dft4<DIR>(y0, y1, y2, y3);
+        x[k1] = y0; x[k1 + 4] = y1; x[k1 + 8] = y2; x[k1 + 12] = y3;
+    }
+}
+template <int R, int DIR, typename V> __device__ __forceinline__ void dft(V* x) {
+    if constexpr (R == 2) dft2<DIR>(x[0], x[1]);
+    else if constexpr (R == 4) dft4<DIR>(x[0], x[1], x[2], x[3]);
+    else if constexpr (R == 8) dft8<DIR>(x);
+    else dft16<DIR>(x);
+}
+
+// Radix schedules.  Pass p has radix radix_of(N,p); Ns = product of earlier radices.
+__host__ __device__ constexpr int npass_of(int N) {
+    return N <= 16 ? 1 : (N <= 256 ? 2 : 3);
+}
+__host__ __device__ constexpr int radix_of(int N, int p) {
+    // 2:[2] 4:[4] 8:[8] 16:[16] 32:[8,4] 64:[8,8] 128:[16,8] 256:[16,16]
+    // 512:[8,8,8] 1024:[16,8,8] 2048:[16,16,8] 4096:[16,16,16]
+    return N <= 16 ? N
+         : N == 32 ? (p == 0 ? 8 : 4)
+         : N == 64 ? 8
+         : N == 128 ? (p == 0 ? 16 : 8)
+         : N == 256 ? 16
+         : N == 512 ? 8
+         : N == 1024 ? (p == 0 ? 16 : 8)
+         : N == 2048 ? (p == 2 ? 8 : 16)
+         : 16;
+}
+__host__ __device__ constexpr int rmax_of(int N) {
+    return N <= 16 ? N : (N == 32 || N == 64 || N == 512) ? 8 : 16;
+}
+__host__ __device__ constexpr int ns_of(int N, int p) {
+    int ns = 1;
+    for (int i = 0; i < p; ++i) ns *= radix_of(N, i);
+    return ns;
+}
+// Offset of pass p's twiddles in the per-length table: entries [k*R + r] for
+// k < Ns, r < R hold exp(DIR*2*pi*i*r*k/(Ns*R)); pass 0 has none.
+__host__ __device__ constexpr int twoff_of(int N, int p) {
+    int off = 0;
+    for (int i = 1; i < p; ++i) off += ns_of(N, i) * radix_of(N, i);
+    return off;
+}
+__host__ __device__ constexpr int twsize_of(int N) { return twoff_of(N, npass_of(N)); }
+__host__ __device__ constexpr int ilog2c(int v) { return v <= 1 ? 0 : 1 + ilog2c(v / 2); }
+
+// Launch geometry of one axis pass (all offsets in elements).
+struct FftGeom {
+    AxisMap fft;        // the transform axis
+    AxisMap a1;         // outer enumeration axis of lines / tiles
+    AxisMap a2;         // inner enumeration axis (z columns for strided tiles)
+    uint32_t n2;        // units along a2 (lines, or tiles of T columns)
+    uint32_t nunits;    // total lines (contiguous) or tiles (strided)
+    FastDiv dn2;
+    int64_t tstride;    // strided tiles: element step between the T columns
+    int scale_mode;     // 0: none, 1: multiply by scale, 2: divide by scale
+    double scale;
+};
+
+// Passes p..NP-1 of the Stockham schedule on the thread's registers.
+template <typename real, int N, int E, int T, bool STRIDED, int DIR, int p>
+__device__ __forceinline__ void stockham_passes(typename CT<real>::V (&v)[E], typename CT<real>::V* lds,
+                                                const typename CT<real>::V* __restrict__ tw, int t, int line) {
+    using V = typename CT<real>::V;
+    constexpr int NP = npass_of(N);
+    if constexpr (p < NP) {
+        constexpr int TPL = N / E;
+        constexpr int R = radix_of(N, p);
+        constexpr int Ns = ns_of(N, p);
+        constexpr int NB = E / R;
+        constexpr int toff = twoff_of(N, p);
+        static_assert(E % R == 0, "E must be a multiple of every radix");
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            V x[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) x[r] = v[b + r * NB];
+            if constexpr (p > 0) {
+                const int j = t + b * TPL;
+                const int k = j & (Ns - 1);
+#pragma unroll
+                for (int r = 1; r < R; ++r) x[r] = c_mul(x[r], tw[toff + k * R + r]);
+            }
+            dft<R, DIR>(x);
+#pragma unroll
+            for (int q = 0; q < R; ++q) v[b + q * NB] = x[q];
+        }
+        if constexpr (p + 1 < NP) {
+            if constexpr (p > 0) __syncthreads();
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                const int j = t + b * TPL;
+                const int d0 = (j / Ns) * (Ns * R) + (j & (Ns - 1));
+#pragma unroll
+                for (int q = 0; q < R; ++q) {
+                    const int d = d0 + q * Ns;
+                    lds[STRIDED ? d * T + line : line * N + d] = v[b + q * NB];
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const int n = t + e * TPL;
+                v[e] = lds[STRIDED ? n * T + line : line * N + n];
+            }
+        }
+        stockham_passes<real, N, E, T, STRIDED, DIR, p + 1>(v, lds, tw, t, line);
+    }
+}
+
+template <typename real, int N, int E, int T, bool STRIDED, int DIR>
+__global__ __launch_bounds__(T * (N / E)) void stockham_kernel(typename CT<real>::V* __restrict__ data, FftGeom g,
+                                                               const typename CT<real>::V* __restrict__ tw) {
+    using V = typename CT<real>::V;
+    constexpr int TPL = N / E;
+    constexpr int NP = npass_of(N);
+    constexpr int LDS = NP > 1 ? N * T : 1;
+    __shared__ V lds[LDS];
+
+    const int tid = threadIdx.x;
+    const int t = STRIDED ? tid / T : tid % TPL;
+    const int line = STRIDED ? tid % T : tid / TPL;
+
+    int64_t base;
+    bool valid = true;
+    if constexpr (STRIDED) {
+        const uint32_t tile = blockIdx.x;
+        const uint32_t u1 = g.dn2.div(tile);
+        const uint32_t zt = tile - u1 * g.n2;
+        base = g.a1.off(u1) + g.a2.off(zt * T) + int64_t(line) * g.tstride;
+    } else {
+        const uint32_t ln = blockIdx.x * T + line;
+        valid = ln < g.nunits;
+        const uint32_t u1 = g.dn2.div(ln);
+        const uint32_t u2 = ln - u1 * g.n2;
+        base = valid ? g.a1.off(u1) + g.a2.off(u2) : 0;
+    }
+    // off(n) = off(e*TPL) + off(t): the host guarantees that the axis block
+    // extent and TPL divide one another, which makes the offset additive.
+    base += g.fft.off(uint32_t(t));
+
+    V v[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        if (valid) v[e] = data[base + g.fft.off(uint32_t(e * TPL))];
+        else { v[e].x = 0; v[e].y = 0; }
+    }
+
+    stockham_passes<real, N, E, T, STRIDED, DIR, 0>(v, lds, tw, t, line);
+
+    if (!valid) return;
+    if (g.scale_mode == 1) {
+        const real s = real(g.scale);
+#pragma unroll
+        for (int e = 0; e < E; ++e) { v[e].x *= s; v[e].y *= s; }
+    } else if (g.scale_mode == 2) {
+        const real s = real(g.scale);
+#pragma unroll
+        for (int e = 0; e < E; ++e) { v[e].x = v[e].x / s; v[e].y = v[e].y / s; }
+    }
+#pragma unroll
+    for (int e = 0; e < E; ++e) data[base + g.fft.off(uint32_t(e * TPL))] = v[e];
+}
+
+}  // namespace a2a

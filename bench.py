@@ -1,0 +1,199 @@
+"""Benchmark: one 512^3 complex-to-complex forward 3D FFT per step on one MI355X.
+
+Workload (the headline configuration): a 512^3 complex64 Gaussian random
+field with P(k) ~ k^-3 (seed 2) plus 1e-3 uniform noise, block-decomposed over
+16 virtual ranks with dims [1,4,4] (local blocks 512x128x128), transformed in
+place by the fused FP32 plan: three Stockham passes (z, x, y) over the block
+layout.  Inputs are resident in HBM before the timed region starts.
+
+Prints one JSON line with ms per transform, GFLOP/s (5 N log2 N), the HBM
+roofline of the dominant pass, the rel-L2 / max-abs error against the FP64
+path, and a CPU baseline (the C restatement of the reference dataflow).
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "3d-ffts-mpi_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_grf(a2afft, torch, Ng, seed, slope=-3.0, noise=1e-3):
+    """The GRF input on the device (complex128 global C-order grid)."""
+    dev = "cuda"
+    n = Ng ** 3
+    w = torch.empty(n, dtype=torch.complex128, device=dev)
+    a2afft.fill_random(w, seed, "normal")
+    k = torch.fft.fftfreq(Ng, 1.0 / Ng, dtype=torch.float64, device=dev)
+    k2 = (k[:, None, None] ** 2 + k[None, :, None] ** 2 + k[None, None, :] ** 2).reshape(-1)
+    amp = torch.where(k2 >= 1.0, k2.clamp(min=1.0) ** (slope / 4.0), torch.zeros_like(k2))
+    w *= amp
+    del amp, k2
+    # inverse FFT in FP64 with the library (one rank, dims [1,1,1]); includes 1/Ng^3
+    dist = a2afft.Distribution(Ng, 1, dims=(1, 1, 1), dtype="c128")
+    plan = a2afft.Dfft(dist, precision="fp64", exec="fused")
+    plan.make_plans(None)
+    w = w.reshape(1, n)
+    plan.inverse(w)
+    rms = torch.sqrt(torch.mean(w.real ** 2 + w.imag ** 2))
+    w /= rms
+    u = torch.empty_like(w)
+    a2afft.fill_random(u, seed, "uniform")
+    w += noise * u
+    del u
+    return w.reshape(Ng, Ng, Ng)
+
+
+def cpu_baseline(Ng, P, dims, threads):
+    """The oracle (C restatement of the reference stage order, FP64) on one
+    full transform of the same grid shape; OpenMP over `threads` cores."""
+    import numpy as np
+    sys.path.insert(0, ROOT)
+    from oracle import oracle as O
+    O.set_threads(threads)
+    rc, d = O.decompose(Ng, P, dims)
+    assert rc == 0
+    x = O.fill_uniform(2, Ng ** 3).reshape(P, -1)
+    S = np.zeros_like(x)
+    import ctypes
+    t0 = time.perf_counter()
+    st = O.lib().orc_forward(ctypes.byref(d), x.ctypes.data_as(ctypes.c_void_p), S.ctypes.data_as(ctypes.c_void_p))
+    t = time.perf_counter() - t0
+    assert st == 0
+    return t * 1e3, O.get_threads()
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
+    ap.add_argument("--ng", type=int, default=512)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: every box here has one GPU and the multi-GPU exchange is "
+              "design-only; run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import a2afft
+
+    Ng, P, dims = args.ng, 16, (1, 4, 4)
+    seed = 2
+    dtype = "c64" if args.precision == "fp32" else "c128"
+    esize = 8 if dtype == "c64" else 16
+    N = Ng ** 3
+
+    log(f"[bench] generating {Ng}^3 GRF input on the device")
+    g = make_grf(a2afft, torch, Ng, seed)
+    x = a2afft.inputs.global_to_blocks(g.to(torch.complex64 if dtype == "c64" else torch.complex128), dims)
+    del g
+    x = x.contiguous()
+    torch.cuda.synchronize()
+
+    dist = a2afft.Distribution(Ng, P, dims=dims, dtype=dtype)
+    plan = a2afft.Dfft(dist, precision=args.precision, exec="fused")
+    plan.make_plans(None)
+
+    # ---- accuracy vs the FP64 path on the same (already rounded) input -----
+    log("[bench] accuracy check vs FP64")
+    data = x.clone()
+    plan.forward(data)
+    dist64 = a2afft.Distribution(Ng, P, dims=dims, dtype="c128")
+    plan64 = a2afft.Dfft(dist64, precision="fp64", exec="fused")
+    plan64.make_plans(None)
+    ref = x.to(torch.complex128)
+    plan64.forward(ref)
+    rel_l2, max_abs = a2afft.error_metrics(data, ref)
+    del ref
+    torch.cuda.synchronize()
+
+    # ---- timing ----------------------------------------------------------
+    log(f"[bench] warmup {args.warmup}, timed {args.steps}")
+    data.copy_(x)
+    for _ in range(args.warmup):
+        plan.forward(data)
+    torch.cuda.synchronize()
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        ev[s][0].record()
+        for p in range(3):
+            a2afft.execute_pass(plan, data, p, -1)
+            ev[s][p + 1].record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = wall * 1e3 / args.steps
+    pass_ms = [sum(ev[s][p].elapsed_time(ev[s][p + 1]) for s in range(args.steps)) / args.steps for p in range(3)]
+    ev_ms = sum(ev[s][0].elapsed_time(ev[s][3]) for s in range(args.steps)) / args.steps
+
+    bytes_pass = 2 * N * esize
+    names = ["z", "x", "y"]
+    dom = max(range(3), key=lambda p: pass_ms[p])
+    achieved = bytes_pass / (pass_ms[dom] * 1e-3) / 1e9
+    gflops = 5.0 * N * math.log2(N) / (ms_per_step * 1e-3) / 1e9
+    whole_gbs = 3 * bytes_pass / (ms_per_step * 1e-3) / 1e9
+
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tpath):
+        try:
+            with open(tpath) as f:
+                traffic = json.load(f).get(f"{Ng}_{args.precision}_{names[dom]}")
+        except Exception:
+            traffic = None
+
+    cpu = None
+    if not args.no_cpu:
+        threads = min(16, len(os.sched_getaffinity(0)))
+        log(f"[bench] CPU baseline on {threads} threads")
+        cpu_ms, used = cpu_baseline(Ng, P, dims, threads)
+        cpu = {"value": round(cpu_ms, 1), "unit": "ms", "cores": used, "kind": "port",
+               "sample": f"one full {Ng}^3 forward, complex128, 16 virtual ranks dims [1,4,4], reference stage "
+                         f"order (oracle C restatement, OpenMP)"}
+
+    out = {
+        "metric": "ms/512^3 c2c 3D FFT + % HBM roofline",
+        "value": round(ms_per_step, 4),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": False,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "f32" if args.precision == "fp32" else "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{Ng}^3 c2c forward 3D FFT, {'complex64' if dtype == 'c64' else 'complex128'}, "
+                               f"16 virtual ranks dims [1,4,4], GRF P(k)~k^-3 seed {seed} + 1e-3 noise",
+                   "precision": args.precision, "exec": "fused", "Ng": Ng, "ranks": P, "dims": list(dims)},
+        "gflops": round(gflops, 1),
+        "hbm_gbs_whole": round(whole_gbs, 1),
+        "roofline_frac_whole": round(whole_gbs / HBM_PEAK_GBS, 4),
+        "pass_ms": {names[p]: round(pass_ms[p], 4) for p in range(3)},
+        "event_ms_per_step": round(ev_ms, 4),
+        "rel_l2_vs_fp64": rel_l2,
+        "max_abs_vs_fp64": max_abs,
+        "roofline": {"bound": "hbm", "kernel": f"stockham {names[dom]} pass", "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

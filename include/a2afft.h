@@ -1,0 +1,205 @@
+/*
+ * a2afft -- MI355X-native pencil 3D FFT (C ABI).
+ *
+ * Drop-in for the a2afft hot path of humz2k/3D-FFTs-MPI (src/a2afft/).  The
+ * reference's P MPI ranks become P *virtual ranks* whose blocks sit side by
+ * side in one HBM allocation, so every buffer handed to this library is
+ * [P][nlocal] interleaved complex (re, im) of the distribution's dtype, and
+ * each MPI_Alltoall becomes a device-local permutation.
+ *
+ * Conventions
+ *   - Device pointers are plain `void*`; the element type is the dtype enum.
+ *   - `stream` is a hipStream_t passed as `void*` (NULL = the null stream).
+ *   - Every entry point returns an int status (A2AFFT_OK == 0).  Negative
+ *     values are errors; positive values are warnings and the call has run.
+ *   - Calls are stream-ordered and return without synchronising, unless
+ *     a2afft_set_sync(1) was called, in which case every call ends with a
+ *     device synchronise as the reference's launchers do
+ *     (e.g. src/a2afft/distribution/kernels/a2apencils.cu:34).
+ *   - Handles are not thread-safe; distinct plans may run on distinct streams.
+ */
+#ifndef A2AFFT_H
+#define A2AFFT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------- */
+#define A2AFFT_OK                          0
+/* y-return ran with dims outside the domain where the reference's formula
+ * (a2apencils.cu:197) is a bijection; the true inverse was applied instead. */
+#define A2AFFT_W_DIVERGES_FROM_REFERENCE   1
+#define A2AFFT_E_INVALID_ARG              -1
+#define A2AFFT_E_DIVISIBILITY             -2  /* Ng % dims[i] != 0           (a2adistribution.cpp:102-104) */
+#define A2AFFT_E_RANKS                    -3  /* P % dims[i] != 0 / dims != P (a2adistribution.cpp:115-117) */
+#define A2AFFT_E_CHUNK                    -4  /* (l_a*l_b) % P != 0           (a2adistribution.cpp:119-121) */
+#define A2AFFT_E_HIP                      -5
+#define A2AFFT_E_ALLOC                    -6
+#define A2AFFT_E_UNSUPPORTED              -7  /* length / precision not built */
+#define A2AFFT_E_NO_SCRATCH               -8  /* execute before make_plans    (a2adfft.cpp:47-53) */
+#define A2AFFT_E_SIZE                     -9  /* index range exceeded */
+
+typedef enum { A2AFFT_C128 = 0, A2AFFT_C64 = 1 } a2afft_dtype;
+
+typedef enum {
+    A2AFFT_FP64 = 0,        /* c128 data, FP64 Stockham (the reference's precision) */
+    A2AFFT_FP32 = 1,        /* c64 data, FP32 Stockham */
+    A2AFFT_SPLIT_BF16 = 2,  /* c64 data, radix-16 stages on MFMA, split bf16 hi+lo */
+    A2AFFT_SPLIT_FP16 = 3   /* c64 data, radix-16 stages on MFMA, split fp16 hi+lo */
+} a2afft_precision;
+
+typedef enum {
+    A2AFFT_EXEC_FUSED = 0,  /* 3 in-place axis passes over the block layout */
+    A2AFFT_EXEC_STAGED = 1  /* the reference stage order: 18 launches (a2adfft.cpp:55-188) */
+} a2afft_exec;
+
+typedef struct a2afft_dist_s* a2afft_dist;
+typedef struct a2afft_plan_s* a2afft_plan;
+
+/* Public fields of a2aDistribution (a2adistribution.h:8-17). */
+typedef struct {
+    int Ng;
+    int64_t nlocal;
+    int world_size;
+    int local_grid_size[3];
+    int dims[3];
+    int blockSize;
+    int dtype;               /* a2afft_dtype */
+    int reference_domain;    /* 1 when every remap equals the reference bit for bit */
+} a2afft_dist_info;
+
+/* ---- library ------------------------------------------------------------ */
+const char* a2afft_status_string(int status);
+/* Detail of the last error raised on the calling thread ("" if none). */
+const char* a2afft_last_error(void);
+/* 1: synchronise the device after every call (the reference's behaviour). */
+void a2afft_set_sync(int on);
+int a2afft_version(void);
+
+/* ---- decomposition: a2aDistribution ------------------------------------- */
+/* MPI_Dims_create(nnodes, 3, dims) with all dims free
+ * (replaces the call at a2adistribution.cpp:100). */
+int a2afft_dims_create(int nnodes, int dims[3]);
+
+/* a2aDistribution::a2aDistribution(MPI_Comm, Ng, blockSize)
+ * (a2adistribution.cpp:85-154).  The communicator becomes `world_size`
+ * virtual ranks; dims == NULL applies the MPI_Dims_create rule, otherwise
+ * dims[0]*dims[1]*dims[2] must equal world_size.  Checks the reference's
+ * asserted preconditions and returns their codes instead of aborting. */
+int a2afft_dist_create(a2afft_dist* out, int Ng, int world_size, const int* dims,
+                       int blockSize, a2afft_dtype dtype);
+int a2afft_dist_get_info(a2afft_dist dist, a2afft_dist_info* info);
+/* rank2coords + get_local_grid_start (a2adistribution.cpp:55-83). */
+int a2afft_dist_rank_coords(a2afft_dist dist, int rank, int coords[3],
+                            int local_coordinates_start[3]);
+/* The `dims` file written under saveDims (a2adistribution.cpp:142-152):
+ * int32 world_size, Ng, nlocal, dims[3], local_grid_size[3]. */
+int a2afft_dist_write_dims(a2afft_dist dist, const char* path);
+/* a2aDistribution::finalize (a2adistribution.cpp:258-265). */
+int a2afft_dist_destroy(a2afft_dist dist);
+
+/* Stage methods (a2adistribution.cpp:156-256) over all P virtual ranks.
+ * buff1/buff2 play d_Buff1/d_Buff2; each is [P][nlocal] complex. */
+int a2afft_get_z_pencils(a2afft_dist dist, void* buff1, void* buff2, void* stream);
+int a2afft_return_z_pencils(a2afft_dist dist, void* buff1, void* buff2, void* stream);
+int a2afft_get_x_pencils(a2afft_dist dist, void* buff1, void* buff2, void* stream);
+int a2afft_return_x_pencils(a2afft_dist dist, void* buff1, void* buff2, void* stream);
+int a2afft_get_y_pencils(a2afft_dist dist, void* buff1, void* buff2, void* stream);
+int a2afft_return_y_pencils(a2afft_dist dist, void* buff1, void* buff2, void* stream);
+
+/* ---- launcher level (distribution/kernels/kernels.h:3-17) --------------- */
+/* Each applies the reference kernel's index map to `nbuffers` consecutive
+ * rank buffers of `nlocal` elements (nbuffers = world_size for all virtual
+ * ranks, 1 for one rank's share).  blockSize is accepted for signature
+ * parity and only bounds-checked: the launch shape is the library's. */
+int a2afft_launch_d_z_a2a_to_z_pencils(const void* source, void* dest, int blockSize,
+        int world_size, int64_t nlocal, const int local_grid_size[3], const int dims[3],
+        a2afft_dtype dtype, int nbuffers, void* stream);   /* a2apencils.cu:23-35 */
+int a2afft_launch_d_z_pencils_to_z_a2a(const void* source, void* dest, int blockSize,
+        int world_size, int64_t nlocal, const int local_grid_size[3], const int dims[3],
+        a2afft_dtype dtype, int nbuffers, void* stream);   /* a2apencils.cu:58-71 */
+int a2afft_launch_d_x_a2a_to_x_pencils(const void* source, void* dest, int blockSize,
+        int world_size, int64_t nlocal, const int local_grid_size[3], const int dims[3],
+        a2afft_dtype dtype, int nbuffers, void* stream);   /* a2apencils.cu:94-108 */
+int a2afft_launch_d_x_pencils_to_x_a2a(const void* source, void* dest, int blockSize,
+        int world_size, int64_t nlocal, const int local_grid_size[3], const int dims[3],
+        a2afft_dtype dtype, int nbuffers, void* stream);   /* a2apencils.cu:132-146 */
+int a2afft_launch_d_y_a2a_to_y_pencils(const void* source, void* dest, int blockSize,
+        int world_size, int64_t nlocal, const int local_grid_size[3], const int dims[3],
+        a2afft_dtype dtype, int nbuffers, void* stream);   /* a2apencils.cu:170-186 */
+/* Returns A2AFFT_W_DIVERGES_FROM_REFERENCE when dims are outside the
+ * reference formula's valid domain (a2apencils.cu:197); the true inverse runs. */
+int a2afft_launch_d_y_pencils_to_y_a2a(const void* source, void* dest, int blockSize,
+        int world_size, int64_t nlocal, const int local_grid_size[3], const int dims[3],
+        a2afft_dtype dtype, int nbuffers, void* stream);   /* a2apencils.cu:210-226 */
+int a2afft_launch_d_fast_z_to_x(const void* source, void* dest, const int local_grid_size[3],
+        int blockSize, int64_t nlocal, a2afft_dtype dtype, int nbuffers,
+        void* stream);                                     /* a2achangefastaxis.cu:24-32 */
+int a2afft_launch_d_fast_x_to_y(const void* source, void* dest, const int local_grid_size[3],
+        int blockSize, int64_t nlocal, a2afft_dtype dtype, int nbuffers,
+        void* stream);                                     /* a2achangefastaxis.cu:55-63 */
+int a2afft_launch_d_fast_y_to_z(const void* source, void* dest, const int local_grid_size[3],
+        int blockSize, int64_t nlocal, a2afft_dtype dtype, int nbuffers,
+        void* stream);                                     /* a2achangefastaxis.cu:86-94 */
+
+/* MPI_Alltoall(send, nlocal/P, TYPE_COMPLEX, recv, ...) over the P virtual
+ * ranks of [P][nlocal] buffers: recv[s][q*ns+i] = send[q][s*ns+i]
+ * (a2adistribution.cpp:158-166 and the five other call sites). */
+int a2afft_alltoall(const void* send, void* recv, int world_size, int64_t nlocal,
+                    a2afft_dtype dtype, void* stream);
+
+/* ---- FFT launchers (dfft/kernels/kernels.h:5-11) ------------------------ */
+/* forward_1d_fft / inverse_1d_fft (a2affts.cu:7-76): n_elems/Ng in-place
+ * unnormalised DFTs of contiguous length-Ng pencils (replaces cuFFT Z2Z/C2C). */
+int a2afft_forward_1d_fft(void* data, int Ng, int64_t n_elems, a2afft_dtype dtype, void* stream);
+int a2afft_inverse_1d_fft(void* data, int Ng, int64_t n_elems, a2afft_dtype dtype, void* stream);
+/* launch_scale_fft (a2affts.cu:78-92,108-116): out = data / Ng^3 (64-bit Ng^3). */
+int a2afft_launch_scale_fft(void* out, const void* data, int Ng, int64_t n_elems,
+                            int blockSize, a2afft_dtype dtype, void* stream);
+/* launch_fast_copy_fft (a2affts.cu:94-106,118-126): out = data. */
+int a2afft_launch_fast_copy_fft(void* out, const void* data, int64_t n_elems,
+                                int blockSize, a2afft_dtype dtype, void* stream);
+
+/* ---- plan / execute: a2adfft -------------------------------------------- */
+/* a2adfft::a2adfft(dist) (a2adfft.cpp:36-45) plus the precision and
+ * execution strategy.  Builds twiddle tables and kernel choices. */
+int a2afft_plan_create(a2afft_plan* out, a2afft_dist dist, a2afft_precision precision,
+                       a2afft_exec exec);
+/* a2adfft::make_plans(d_scratch) (a2adfft.cpp:47-53).  scratch is a
+ * [P][nlocal] device buffer; the staged strategy needs it, the fused one
+ * accepts NULL. */
+int a2afft_make_plans(a2afft_plan plan, void* d_scratch);
+/* a2adfft::forward / inverse (a2adfft.cpp:55-188): in place on d_data in the
+ * block layout; inverse applies the 1/Ng^3 of scale_fft. */
+int a2afft_forward(a2afft_plan plan, void* d_data, void* stream);
+int a2afft_inverse(a2afft_plan plan, void* d_data, void* stream);
+int a2afft_plan_destroy(a2afft_plan plan);
+
+/* ---- measurement helpers (no reference counterpart) --------------------- */
+/* rel_l2 = ||y - ref||_2 / ||ref||_2 and max_abs = max |y - ref| over n
+ * complex values, reduced on the device in FP64.  y has dtype; ref is
+ * complex128 on the device.  Blocks until the result is on the host. */
+int a2afft_error_metrics(const void* y, a2afft_dtype dtype, const void* ref, int64_t n,
+                         double* rel_l2, double* max_abs, void* stream);
+
+/* One axis pass of a fused plan, exactly as a2afft_forward (direction -1) or
+ * a2afft_inverse (+1) runs it: pass 0 = z, 1 = x, 2 = y (the reference's axis
+ * order, a2adfft.cpp:60-114); the inverse's 1/Ng^3 rides on pass 2.  For
+ * per-kernel timing; the three passes in order equal one transform. */
+int a2afft_execute_pass(a2afft_plan plan, void* d_data, int pass, int direction, void* stream);
+
+/* Seeded synthetic input (no reference counterpart: the reference's test
+ * fills with an unseeded rand(), tests/test.cpp:12-17).  Philox4x32-10 with
+ * key (seed lo, seed hi) and counter (g lo, g hi, kind, 0) for global index
+ * g = g0 + i; kind 0: re, im uniform in [-1, 1); kind 1: standard normal by
+ * Box-Muller.  Writes n complex values of dtype. */
+int a2afft_fill_random(void* out, a2afft_dtype dtype, int64_t n, uint64_t seed, int kind, int64_t g0,
+                       void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* A2AFFT_H */

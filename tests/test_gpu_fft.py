@@ -1,0 +1,137 @@
+"""GPU parity of the Stockham kernels and of the 3D transform.
+
+1D: forward_1d_fft / inverse_1d_fft against the FP64 oracle FFT and against
+known answers.  3D: a2adfft forward / inverse, both execution strategies,
+against the oracle's reference-stage-order transform; the fused strategy must
+equal the staged one bit for bit (the remaps are permutations and both use the
+same per-line arithmetic)."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+LENGTHS = [2, 4, 8, 16, 32, 64, 128, 256, 512, 1024, 2048, 4096]
+# rel-L2 bounds against the FP64 oracle
+TOL_1D = {"c128": 1e-15, "c64": 1.5e-7}
+
+
+def _relerr(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+
+
+def _bits(a):
+    a = np.ascontiguousarray(a)
+    return a.view(np.uint64 if a.dtype == np.complex128 else np.uint32)
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+@pytest.mark.parametrize("N", LENGTHS)
+def test_batched_1d_fft_vs_oracle(a2afft_mod, cuda, N, dtype):
+    import torch
+    rng = np.random.default_rng(N)
+    lines = max(3, 4096 // N) + 1          # ragged batch: not a multiple of the lines per block
+    x = rng.standard_normal((lines, N)) + 1j * rng.standard_normal((lines, N))
+    x = x.astype(np.complex128 if dtype == "c128" else np.complex64)
+    ref_f = O.fft_batched(x.astype(np.complex128), -1)
+    ref_i = O.fft_batched(x.astype(np.complex128), +1)
+    d = torch.from_numpy(x).cuda()
+    a2afft_mod.forward_1d_fft(d, N)
+    torch.cuda.synchronize()
+    assert _relerr(d.cpu().numpy(), ref_f) < TOL_1D[dtype] * max(1, np.log2(N))
+    d = torch.from_numpy(x).cuda()
+    a2afft_mod.inverse_1d_fft(d, N)
+    torch.cuda.synchronize()
+    assert _relerr(d.cpu().numpy(), ref_i) < TOL_1D[dtype] * max(1, np.log2(N))
+
+
+@pytest.mark.parametrize("N", [8, 64, 512, 2048])
+def test_1d_known_answers(a2afft_mod, cuda, N):
+    """delta -> ones, ones -> N*delta, single tone -> N*delta_k0 (FP64)."""
+    import torch
+    k0 = 3
+    x = np.zeros((3, N), dtype=np.complex128)
+    x[0, 0] = 1.0
+    x[1, :] = 1.0
+    x[2, :] = np.exp(2j * np.pi * k0 * np.arange(N) / N)
+    d = torch.from_numpy(x).cuda()
+    a2afft_mod.forward_1d_fft(d, N)
+    torch.cuda.synchronize()
+    y = d.cpu().numpy()
+    assert np.allclose(y[0], 1.0, atol=1e-15)
+    e = np.zeros(N, dtype=np.complex128)
+    e[0] = N
+    assert np.allclose(y[1], e, atol=1e-12 * N)
+    e = np.zeros(N, dtype=np.complex128)
+    e[k0] = N
+    assert np.allclose(y[2], e, atol=1e-12 * N)
+
+
+# (Ng, P, dims, dtype) -- C1 shape, the reference's own small configs and the
+# decompositions of the benchmark configs at reduced Ng.
+CONFIGS_3D = [
+    (64, 4, (2, 2, 1), "c128"),
+    (8, 8, None, "c128"),
+    (32, 8, None, "c128"),
+    (4, 4, None, "c128"),
+    (32, 8, (1, 2, 4), "c128"),
+    (64, 16, (1, 4, 4), "c64"),
+    (64, 16, (4, 2, 2), "c64"),
+    (128, 64, (1, 8, 8), "c64"),
+    (32, 64, (4, 4, 4), "c128"),
+]
+TOL_3D = {"c128": 2e-15, "c64": 6e-7}
+
+
+def _blocks(d, dtype, seed):
+    g = O.fill_uniform(seed, d.Ng ** 3).reshape((d.Ng,) * 3)
+    b = O.global_to_blocks(d, g)
+    return b.astype(np.complex128 if dtype == "c128" else np.complex64)
+
+
+@pytest.mark.parametrize("cfg", CONFIGS_3D, ids=lambda c: f"Ng{c[0]}_P{c[1]}_{c[2]}_{c[3]}")
+def test_3d_forward_inverse_vs_oracle(a2afft_mod, cuda, cfg):
+    import torch
+    Ng, P, dims, dtype = cfg
+    rc, d = O.decompose(Ng, P, dims)
+    assert rc == 0
+    x = _blocks(d, dtype, seed=Ng + P)
+    ref_f = O.forward(d, x.astype(np.complex128))
+    dist = a2afft_mod.Distribution(Ng, P, dims=d.dims_t, dtype=dtype)
+    outs = {}
+    for ex in ("staged", "fused"):
+        plan = a2afft_mod.Dfft(dist, exec=ex)
+        scratch = dist.empty()
+        plan.make_plans(scratch)
+        data = torch.from_numpy(x).cuda()
+        plan.forward(data)
+        torch.cuda.synchronize()
+        y = data.cpu().numpy()
+        assert _relerr(y, ref_f) < TOL_3D[dtype], ex
+        outs[ex] = y
+        plan.inverse(data)
+        torch.cuda.synchronize()
+        z = data.cpu().numpy()
+        assert _relerr(z, x.astype(np.complex128)) < TOL_3D[dtype] * 2, ex
+        outs[ex + "_inv"] = z
+    assert np.array_equal(_bits(outs["staged"]), _bits(outs["fused"])), "fused != staged (forward)"
+    assert np.array_equal(_bits(outs["staged_inv"]), _bits(outs["fused_inv"])), "fused != staged (inverse)"
+
+
+def test_3d_matches_numpy_fftn(a2afft_mod, cuda):
+    """The reference's own pin: np.fft.fftn of the stitched global grid (tests/stitch.py:89)."""
+    import torch
+    from a2afft import inputs
+    Ng, dims = 32, (2, 2, 2)
+    g = inputs.uniform(5, Ng ** 3).reshape((Ng,) * 3)
+    dist = a2afft_mod.Distribution(Ng, 8, dims=dims, dtype="c128")
+    plan = a2afft_mod.Dfft(dist)
+    plan.make_plans(dist.empty())
+    data = torch.from_numpy(inputs.global_to_blocks(g, dims)).cuda()
+    plan.forward(data)
+    out = inputs.blocks_to_global(data.cpu().numpy(), dims, g.shape)
+    assert _relerr(out, np.fft.fftn(g)) < 1e-15
+    plan.inverse(data)
+    inv = inputs.blocks_to_global(data.cpu().numpy(), dims, g.shape)
+    assert _relerr(inv, g) < 1e-15
