@@ -3,6 +3,7 @@
 // plan / execute layer that replaces a2adfft (src/a2afft/dfft/a2adfft.{h,cpp}).
 #include "common.h"
 #include "fft.h"
+#include "fft_shape.h"
 #include "remap.h"
 
 #include <stdarg.h>
@@ -262,15 +263,6 @@ struct a2afft_plan_s {
     AxisPass pass[3];   // fused: z, x, y over the block layout
 };
 
-// Block-layout axis maps (a2adistribution.cpp:25-42 restated as offsets).
-static void block_axes(const a2afft_dist_s* d, AxisMap ax[3]) {
-    const int64_t nl = d->nlocal;
-    const int64_t lx = d->lgs[0], ly = d->lgs[1], lz = d->lgs[2];
-    ax[0] = make_axis(uint32_t(lx), uint32_t(d->Ng), int64_t(d->dims[1]) * d->dims[2] * nl, ly * lz);
-    ax[1] = make_axis(uint32_t(ly), uint32_t(d->Ng), int64_t(d->dims[2]) * nl, lz);
-    ax[2] = make_axis(uint32_t(lz), uint32_t(d->Ng), nl, 1);
-}
-
 extern "C" int a2afft_plan_create(a2afft_plan* out, a2afft_dist dist, a2afft_precision prec, a2afft_exec exec) {
     if (!out || !dist) return set_error(A2AFFT_E_INVALID_ARG, "plan_create: null");
     *out = nullptr;
@@ -292,12 +284,9 @@ extern "C" int a2afft_plan_create(a2afft_plan* out, a2afft_dist dist, a2afft_pre
     p->prec = prec;
     p->exec = exec;
     p->scratch = nullptr;
-    AxisMap ax[3];
-    block_axes(dist, ax);
-    // z pass: lines (x, y); x pass: tiles (y, z); y pass: tiles (x, z)
-    p->pass[0] = AxisPass{ax[2], ax[0], ax[1], 0, 0, 1.0};
-    p->pass[1] = AxisPass{ax[0], ax[1], ax[2], 1, 0, 1.0};
-    p->pass[2] = AxisPass{ax[1], ax[0], ax[2], 1, 0, 1.0};
+    PassMaps pm[3];
+    fused_pass_maps(dist->Ng, dist->dims, dist->lgs, dist->nlocal, pm);
+    for (int i = 0; i < 3; ++i) p->pass[i] = AxisPass{pm[i].fft, pm[i].a1, pm[i].a2, pm[i].strided, 0, 1.0};
     FftShape sh;
     for (int i = 0; i < 3; ++i) {
         int st = fft_pass_shape(p->pass[i], dist->dtype, dist->Ng, &sh);
@@ -325,14 +314,8 @@ static int exec_staged(a2afft_plan_s* p, void* D, int dir, hipStream_t s) {
     void* S = p->scratch;
     const int64_t total = int64_t(d->P) * d->nlocal;
     auto fft = [&](void* buf) -> int {
-        AxisPass ps;
-        const int64_t lines = total / d->Ng;
-        ps.fft = make_axis(uint32_t(d->Ng), uint32_t(d->Ng), 0, 1);
-        ps.a1 = make_axis(uint32_t(lines), uint32_t(lines), 0, d->Ng);
-        ps.a2 = make_axis(1, 1, 0, 0);
-        ps.strided = 0;
-        ps.scale_mode = 0;
-        ps.scale = 1.0;
+        const PassMaps pm = batched_pass_maps(d->Ng, uint32_t(total / d->Ng));
+        const AxisPass ps{pm.fft, pm.a1, pm.a2, 0, 0, 1.0};
         return launch_fft_pass(ps, buf, d->dtype, d->Ng, dir, s);
     };
     int warn = 0;

@@ -11,143 +11,78 @@
 
 namespace a2a {
 
-bool fft_length_supported(int N) { return N >= 2 && N <= 4096 && (N & (N - 1)) == 0; }
+bool fft_length_supported(int N) { return fft_len_ok(N); }
 
-// ---------------------------------------------------------------------------
-// Launch shapes.  Contiguous lines: E elements per thread (at least the
-// largest radix), 256 threads per workgroup.  Strided tiles: T adjacent
-// columns (128 B of complex64 / complex128 per row) and E chosen so a
-// workgroup has at most 256 threads; LDS stays at or below 64 KiB.
-// ---------------------------------------------------------------------------
-template <typename real> constexpr int cont_E(int N) { return N <= 16 ? N : (N <= 64 ? 8 : 16); }
-template <typename real> constexpr int cont_T(int N) {
-    return (256 * cont_E<real>(N)) / N > 0 ? (256 * cont_E<real>(N)) / N : 1;
-}
-template <typename real> constexpr int str_T(int N) {
-    return sizeof(real) == 4 ? (N <= 512 ? 16 : 8192 / N) : (N <= 512 ? 8 : 4096 / N);
-}
-template <typename real> constexpr int str_E(int N) {
-    return (N * str_T<real>(N)) / 256 > rmax_of(N) ? (N * str_T<real>(N)) / 256 : rmax_of(N);
-}
-
-struct KInfo {
-    const void* fn;
-    int T, E, NT, lds;
-};
-
-template <typename real, int N, int E, int T, bool STRIDED, int DIR>
-static KInfo kinfo() {
-    KInfo k;
-    k.fn = (const void*)&stockham_kernel<real, N, E, T, STRIDED, DIR>;
-    k.T = T;
-    k.E = E;
-    k.NT = T * (N / E);
-    k.lds = npass_of(N) > 1 ? N * T * int(2 * sizeof(real)) : 0;
-    return k;
-}
-
-// variant: 0 contiguous, 1 strided (wide tile), 2 strided single column
 template <typename real, int N, int DIR>
-static KInfo kinfo_variant(int variant) {
-    if (variant == 0) return kinfo<real, N, cont_E<real>(N), cont_T<real>(N), false, DIR>();
-    if (variant == 1) return kinfo<real, N, str_E<real>(N), str_T<real>(N), true, DIR>();
-    return kinfo<real, N, rmax_of(N), 1, true, DIR>();
+static const void* kernel_variant(int variant) {
+    constexpr bool dbl = sizeof(real) == 8;
+    if (variant == 0)
+        return (const void*)&stockham_kernel<real, N, cont_E(N), cont_T(N), false, DIR>;
+    if (variant == 1)
+        return (const void*)&stockham_kernel<real, N, str_E(N, dbl), str_T(N, dbl), true, DIR>;
+    return (const void*)&stockham_kernel<real, N, rmax_of(N), 1, true, DIR>;
 }
 
 template <typename real, int DIR>
-static KInfo kinfo_n(int N, int variant) {
+static const void* kernel_n(int N, int variant) {
     switch (N) {
-    case 2: return kinfo_variant<real, 2, DIR>(variant);
-    case 4: return kinfo_variant<real, 4, DIR>(variant);
-    case 8: return kinfo_variant<real, 8, DIR>(variant);
-    case 16: return kinfo_variant<real, 16, DIR>(variant);
-    case 32: return kinfo_variant<real, 32, DIR>(variant);
-    case 64: return kinfo_variant<real, 64, DIR>(variant);
-    case 128: return kinfo_variant<real, 128, DIR>(variant);
-    case 256: return kinfo_variant<real, 256, DIR>(variant);
-    case 512: return kinfo_variant<real, 512, DIR>(variant);
-    case 1024: return kinfo_variant<real, 1024, DIR>(variant);
-    case 2048: return kinfo_variant<real, 2048, DIR>(variant);
-    case 4096: return kinfo_variant<real, 4096, DIR>(variant);
-    default: return KInfo{nullptr, 0, 0, 0, 0};
+    case 2: return kernel_variant<real, 2, DIR>(variant);
+    case 4: return kernel_variant<real, 4, DIR>(variant);
+    case 8: return kernel_variant<real, 8, DIR>(variant);
+    case 16: return kernel_variant<real, 16, DIR>(variant);
+    case 32: return kernel_variant<real, 32, DIR>(variant);
+    case 64: return kernel_variant<real, 64, DIR>(variant);
+    case 128: return kernel_variant<real, 128, DIR>(variant);
+    case 256: return kernel_variant<real, 256, DIR>(variant);
+    case 512: return kernel_variant<real, 512, DIR>(variant);
+    case 1024: return kernel_variant<real, 1024, DIR>(variant);
+    case 2048: return kernel_variant<real, 2048, DIR>(variant);
+    case 4096: return kernel_variant<real, 4096, DIR>(variant);
+    default: return nullptr;
     }
 }
 
-static KInfo kinfo_for(a2afft_dtype dt, int N, int dir, int variant) {
-    if (dt == A2AFFT_C128) return dir < 0 ? kinfo_n<double, -1>(N, variant) : kinfo_n<double, 1>(N, variant);
-    return dir < 0 ? kinfo_n<float, -1>(N, variant) : kinfo_n<float, 1>(N, variant);
+static const void* kernel_for(a2afft_dtype dt, int N, int dir, int variant) {
+    if (dt == A2AFFT_C128) return dir < 0 ? kernel_n<double, -1>(N, variant) : kernel_n<double, 1>(N, variant);
+    return dir < 0 ? kernel_n<float, -1>(N, variant) : kernel_n<float, 1>(N, variant);
 }
 
-static bool divides_either(uint32_t a, uint32_t b) { return a && b && (a % b == 0 || b % a == 0); }
-
-// Resolve the variant and final geometry of a pass.
-static int resolve(const AxisPass& p, a2afft_dtype dt, int N, KInfo* ki, FftGeom* g, int* grid, int* fallback) {
-    if (!fft_length_supported(N)) return set_error(A2AFFT_E_UNSUPPORTED, "FFT length %d not supported", N);
-    if (p.fft.n != uint32_t(N)) return set_error(A2AFFT_E_INVALID_ARG, "pass: axis extent %u != N %d", p.fft.n, N);
-    int variant = p.strided ? 1 : 0;
-    *fallback = 0;
-    if (p.strided) {
-        KInfo wide = kinfo_for(dt, N, -1, 1);
-        if (p.a2.s_elem != 1 || p.a2.l % wide.T || p.a2.n % wide.T) {
-            variant = 2;
-            *fallback = 1;
-        }
-    }
-    *ki = kinfo_for(dt, N, -1, variant);
-    const int TPL = N / ki->E;
-    if (!divides_either(p.fft.l, uint32_t(TPL)))
-        return set_error(A2AFFT_E_UNSUPPORTED, "pass: block extent %u and %d threads per line do not divide", p.fft.l,
-                         TPL);
-    g->fft = p.fft;
-    g->a1 = p.a1;
-    g->a2 = p.a2;
+static int resolve(const AxisPass& p, a2afft_dtype dt, int N, Shape* sh, FftGeom* g, uint32_t* grid) {
+    int st = resolve_pass(p.fft, p.a1, p.a2, p.strided, N, dt == A2AFFT_C128, sh, g, grid);
+    if (st == A2AFFT_E_UNSUPPORTED)
+        return set_error(st, "fft pass: length %d / block extent %u not supported", N, p.fft.l);
+    if (st < 0) return set_error(st, "fft pass: bad geometry (N=%d)", N);
     g->scale_mode = p.scale_mode;
     g->scale = p.scale;
-    g->tstride = p.a2.s_elem;
-    uint64_t units;
-    if (variant == 0) {
-        g->n2 = p.a2.n;
-        units = uint64_t(p.a1.n) * p.a2.n;
-        *grid = int((units + ki->T - 1) / ki->T);
-        if ((units + ki->T - 1) / ki->T >= (uint64_t(1) << 31)) return set_error(A2AFFT_E_SIZE, "pass: grid too large");
-    } else {
-        g->n2 = p.a2.n / ki->T;
-        units = uint64_t(p.a1.n) * g->n2;
-        if (units >= (uint64_t(1) << 31)) return set_error(A2AFFT_E_SIZE, "pass: grid too large");
-        *grid = int(units);
-    }
-    if (units >= (uint64_t(1) << 32)) return set_error(A2AFFT_E_SIZE, "pass: too many lines");
-    g->nunits = uint32_t(units);
-    g->dn2.set(g->n2 ? g->n2 : 1);
     return A2AFFT_OK;
 }
 
 int fft_pass_shape(const AxisPass& p, a2afft_dtype dt, int N, FftShape* out) {
-    KInfo ki;
+    Shape sh;
     FftGeom g;
-    int grid, fb;
-    A2A_TRY(resolve(p, dt, N, &ki, &g, &grid, &fb));
-    out->T = ki.T;
-    out->E = ki.E;
-    out->NT = ki.NT;
-    out->grid = grid;
-    out->lds_bytes = ki.lds;
-    out->fallback = fb;
+    uint32_t grid;
+    A2A_TRY(resolve(p, dt, N, &sh, &g, &grid));
+    out->T = sh.T;
+    out->E = sh.E;
+    out->NT = sh.NT;
+    out->grid = int(grid);
+    out->lds_bytes = sh.lds;
+    out->fallback = sh.variant == 2;
     return A2AFFT_OK;
 }
 
 int launch_fft_pass(const AxisPass& p, void* data, a2afft_dtype dt, int N, int dir, hipStream_t s) {
     if (!data) return set_error(A2AFFT_E_INVALID_ARG, "fft: null data");
-    KInfo ki;
+    Shape sh;
     FftGeom g;
-    int grid, fb;
-    A2A_TRY(resolve(p, dt, N, &ki, &g, &grid, &fb));
+    uint32_t grid;
+    A2A_TRY(resolve(p, dt, N, &sh, &g, &grid));
     if (grid == 0) return A2AFFT_OK;
     const void* tw = twiddle_table(N, dir, dt);
     if (!tw) return set_error(A2AFFT_E_ALLOC, "fft: twiddle table for N=%d", N);
-    KInfo k = kinfo_for(dt, N, dir, p.strided ? (fb ? 2 : 1) : 0);
+    const void* fn = kernel_for(dt, N, dir, sh.variant);
     void* args[] = {&data, &g, (void*)&tw};
-    int st = hip_check(int(hipLaunchKernel(k.fn, dim3(grid), dim3(k.NT), args, 0, s)), "stockham launch");
+    int st = hip_check(int(hipLaunchKernel(fn, dim3(grid), dim3(sh.NT), args, 0, s)), "stockham launch");
     if (st < 0) return st;
     return hip_check(int(hipGetLastError()), "stockham launch");
 }
@@ -242,13 +177,8 @@ static int fft_1d(void* data, int Ng, int64_t n_elems, a2afft_dtype dtype, int d
         return set_error(A2AFFT_E_INVALID_ARG, "1d fft: bad arguments");
     const int64_t lines = n_elems / Ng;
     if (lines >= (int64_t(1) << 32)) return set_error(A2AFFT_E_SIZE, "1d fft: too many lines");
-    AxisPass p;
-    p.fft = make_axis(uint32_t(Ng), uint32_t(Ng), 0, 1);
-    p.a1 = make_axis(uint32_t(lines), uint32_t(lines), 0, Ng);
-    p.a2 = make_axis(1, 1, 0, 0);
-    p.strided = 0;
-    p.scale_mode = 0;
-    p.scale = 1.0;
+    const PassMaps pm = batched_pass_maps(Ng, uint32_t(lines));
+    const AxisPass p{pm.fft, pm.a1, pm.a2, 0, 0, 1.0};
     return launch_fft_pass(p, data, dtype, Ng, dir, (hipStream_t)stream);
 }
 

@@ -1,0 +1,181 @@
+// Launch shapes and address arithmetic of the Stockham kernels, shared by
+// the device code and by host code (dispatch, and the CPU test that checks
+// every address a launch touches).
+#pragma once
+
+#include "common.h"
+
+namespace a2a {
+
+// Radix schedules.  Pass p has radix radix_of(N,p); Ns = product of earlier radices.
+A2A_HD constexpr int npass_of(int N) {
+    return N <= 16 ? 1 : (N <= 256 ? 2 : 3);
+}
+A2A_HD constexpr int radix_of(int N, int p) {
+    // 2:[2] 4:[4] 8:[8] 16:[16] 32:[8,4] 64:[8,8] 128:[16,8] 256:[16,16]
+    // 512:[8,8,8] 1024:[16,8,8] 2048:[16,16,8] 4096:[16,16,16]
+    return N <= 16 ? N
+         : N == 32 ? (p == 0 ? 8 : 4)
+         : N == 64 ? 8
+         : N == 128 ? (p == 0 ? 16 : 8)
+         : N == 256 ? 16
+         : N == 512 ? 8
+         : N == 1024 ? (p == 0 ? 16 : 8)
+         : N == 2048 ? (p == 2 ? 8 : 16)
+         : 16;
+}
+A2A_HD constexpr int rmax_of(int N) {
+    return N <= 16 ? N : (N == 32 || N == 64 || N == 512) ? 8 : 16;
+}
+A2A_HD constexpr int ns_of(int N, int p) {
+    int ns = 1;
+    for (int i = 0; i < p; ++i) ns *= radix_of(N, i);
+    return ns;
+}
+// Offset of pass p's twiddles in the per-length table: entries [k*R + r] for
+// k < Ns, r < R hold exp(DIR*2*pi*i*r*k/(Ns*R)); pass 0 has none.
+A2A_HD constexpr int twoff_of(int N, int p) {
+    int off = 0;
+    for (int i = 1; i < p; ++i) off += ns_of(N, i) * radix_of(N, i);
+    return off;
+}
+A2A_HD constexpr int twsize_of(int N) { return twoff_of(N, npass_of(N)); }
+A2A_HD constexpr int ilog2c(int v) { return v <= 1 ? 0 : 1 + ilog2c(v / 2); }
+
+// Launch geometry of one axis pass (all offsets in elements).
+struct FftGeom {
+    AxisMap fft;        // the transform axis
+    AxisMap a1;         // outer enumeration axis of lines / tiles
+    AxisMap a2;         // inner enumeration axis (z columns for strided tiles)
+    uint32_t n2;        // units along a2 (lines, or tiles of T columns)
+    uint32_t nunits;    // total lines (contiguous) or tiles (strided)
+    FastDiv dn2;
+    int64_t tstride;    // strided tiles: element step between the T columns
+    int scale_mode;     // 0: none, 1: multiply by scale, 2: divide by scale
+    double scale;
+};
+
+
+// Start address of a thread's line, including off(t) of its first element.
+// Contiguous: workgroup `block` owns lines block*T .. block*T+T-1.  Strided:
+// workgroup `block` owns tile (u1, zt) = T adjacent columns.  The kernels
+// call this with compile-time T and TPL.
+A2A_HD int64_t line_base(const FftGeom& g, bool strided, int T, int TPL, uint32_t block, int tid, bool* valid) {
+    const int t = strided ? tid / T : tid % TPL;
+    const int line = strided ? tid % T : tid / TPL;
+    int64_t base;
+    *valid = true;
+    if (strided) {
+        const uint32_t u1 = g.dn2.div(block);
+        const uint32_t zt = block - u1 * g.n2;
+        base = g.a1.off(u1) + g.a2.off(zt * uint32_t(T)) + int64_t(line) * g.tstride;
+    } else {
+        const uint32_t ln = block * uint32_t(T) + uint32_t(line);
+        *valid = ln < g.nunits;
+        const uint32_t lv = *valid ? ln : 0u;
+        const uint32_t u1 = g.dn2.div(lv);
+        const uint32_t u2 = lv - u1 * g.n2;
+        base = g.a1.off(u1) + g.a2.off(u2);
+    }
+    // off(n) = off(e*TPL) + off(t) for n = t + e*TPL: the dispatcher only
+    // launches when the axis block extent and TPL divide one another.
+    return base + g.fft.off(uint32_t(t));
+}
+
+A2A_HD int64_t elem_off(const FftGeom& g, int n0) { return g.fft.off(uint32_t(n0)); }
+
+// ---------------------------------------------------------------------------
+// Launch shapes.  Contiguous lines: E elements per thread (at least the
+// largest radix), 256 threads per workgroup.  Strided tiles: T adjacent
+// columns (128 B rows of complex64 / complex128) and E chosen so a workgroup
+// has at most 256 threads; LDS stays at or below 64 KiB.
+// ---------------------------------------------------------------------------
+A2A_HD constexpr int cont_E(int N) { return N <= 16 ? N : (N <= 64 ? 8 : 16); }
+A2A_HD constexpr int cont_T(int N) { return (256 * cont_E(N)) / N > 0 ? (256 * cont_E(N)) / N : 1; }
+A2A_HD constexpr int str_T(int N, bool dbl) {
+    return !dbl ? (N <= 512 ? 16 : 8192 / N) : (N <= 512 ? 8 : 4096 / N);
+}
+A2A_HD constexpr int str_E(int N, bool dbl) {
+    return (N * str_T(N, dbl)) / 256 > rmax_of(N) ? (N * str_T(N, dbl)) / 256 : rmax_of(N);
+}
+
+// variant 0: contiguous; 1: strided wide tile; 2: strided single column.
+struct Shape {
+    int variant, T, E, NT, lds;
+};
+
+inline Shape shape_of(int N, bool dbl, int variant) {
+    Shape s;
+    s.variant = variant;
+    if (variant == 0) { s.T = cont_T(N); s.E = cont_E(N); }
+    else if (variant == 1) { s.T = str_T(N, dbl); s.E = str_E(N, dbl); }
+    else { s.T = 1; s.E = rmax_of(N); }
+    s.NT = s.T * (N / s.E);
+    s.lds = npass_of(N) > 1 ? N * s.T * (dbl ? 16 : 8) : 0;
+    return s;
+}
+
+inline bool fft_len_ok(int N) { return N >= 2 && N <= 4096 && (N & (N - 1)) == 0; }
+
+// Resolve the shape and the final geometry of one pass; returns 0 or a
+// negative status (the caller attaches the message).
+inline int resolve_pass(const AxisMap& fft, const AxisMap& a1, const AxisMap& a2, int strided, int N, bool dbl,
+                        Shape* sh, FftGeom* g, uint32_t* grid) {
+    if (!fft_len_ok(N)) return A2AFFT_E_UNSUPPORTED;
+    if (fft.n != uint32_t(N)) return A2AFFT_E_INVALID_ARG;
+    int variant = strided ? 1 : 0;
+    if (strided) {
+        Shape wide = shape_of(N, dbl, 1);
+        if (a2.s_elem != 1 || a2.l % wide.T || a2.n % wide.T) variant = 2;
+    }
+    *sh = shape_of(N, dbl, variant);
+    const uint32_t TPL = uint32_t(N / sh->E);
+    if (!(fft.l % TPL == 0 || TPL % fft.l == 0)) return A2AFFT_E_UNSUPPORTED;
+    g->fft = fft;
+    g->a1 = a1;
+    g->a2 = a2;
+    g->scale_mode = 0;
+    g->scale = 1.0;
+    g->tstride = a2.s_elem;
+    uint64_t units, blocks;
+    if (variant == 0) {
+        g->n2 = a2.n;
+        units = uint64_t(a1.n) * a2.n;
+        blocks = (units + sh->T - 1) / sh->T;
+    } else {
+        g->n2 = a2.n / uint32_t(sh->T);
+        units = uint64_t(a1.n) * g->n2;
+        blocks = units;
+    }
+    if (units >= (uint64_t(1) << 32) || blocks >= (uint64_t(1) << 31)) return A2AFFT_E_SIZE;
+    g->nunits = uint32_t(units);
+    g->dn2.set(g->n2 ? g->n2 : 1);
+    *grid = uint32_t(blocks);
+    return A2AFFT_OK;
+}
+
+// Axis maps of the three fused passes over the virtual-rank block layout:
+// z pass = lines (x, y), contiguous; x pass = tiles (y, z-columns); y pass =
+// tiles (x, z-columns).  The reference's axis order z, x, y (a2adfft.cpp:60-114).
+struct PassMaps {
+    AxisMap fft, a1, a2;
+    int strided;
+};
+
+inline void fused_pass_maps(int Ng, const int dims[3], const int lgs[3], int64_t nlocal, PassMaps out[3]) {
+    const int64_t lx = lgs[0], ly = lgs[1], lz = lgs[2];
+    const AxisMap ax = make_axis(uint32_t(lx), uint32_t(Ng), int64_t(dims[1]) * dims[2] * nlocal, ly * lz);
+    const AxisMap ay = make_axis(uint32_t(ly), uint32_t(Ng), int64_t(dims[2]) * nlocal, lz);
+    const AxisMap az = make_axis(uint32_t(lz), uint32_t(Ng), nlocal, 1);
+    out[0] = PassMaps{az, ax, ay, 0};
+    out[1] = PassMaps{ax, ay, az, 1};
+    out[2] = PassMaps{ay, ax, az, 1};
+}
+
+// `lines` contiguous pencils of length N (forward_1d_fft, a2affts.cu:7-40).
+inline PassMaps batched_pass_maps(int N, uint32_t lines) {
+    return PassMaps{make_axis(uint32_t(N), uint32_t(N), 0, 1), make_axis(lines, lines, 0, N), make_axis(1, 1, 0, 0),
+                    0};
+}
+
+}  // namespace a2a

@@ -25,6 +25,7 @@
 #pragma once
 
 #include "common.h"
+#include "fft_shape.h"
 
 namespace a2a {
 
@@ -151,54 +152,6 @@ template <int R, int DIR, typename V> __device__ __forceinline__ void dft(V* x) 
     else dft16<DIR>(x);
 }
 
-// Radix schedules.  Pass p has radix radix_of(N,p); Ns = product of earlier radices.
-__host__ __device__ constexpr int npass_of(int N) {
-    return N <= 16 ? 1 : (N <= 256 ? 2 : 3);
-}
-__host__ __device__ constexpr int radix_of(int N, int p) {
-    // 2:[2] 4:[4] 8:[8] 16:[16] 32:[8,4] 64:[8,8] 128:[16,8] 256:[16,16]
-    // 512:[8,8,8] 1024:[16,8,8] 2048:[16,16,8] 4096:[16,16,16]
-    return N <= 16 ? N
-         : N == 32 ? (p == 0 ? 8 : 4)
-         : N == 64 ? 8
-         : N == 128 ? (p == 0 ? 16 : 8)
-         : N == 256 ? 16
-         : N == 512 ? 8
-         : N == 1024 ? (p == 0 ? 16 : 8)
-         : N == 2048 ? (p == 2 ? 8 : 16)
-         : 16;
-}
-__host__ __device__ constexpr int rmax_of(int N) {
-    return N <= 16 ? N : (N == 32 || N == 64 || N == 512) ? 8 : 16;
-}
-__host__ __device__ constexpr int ns_of(int N, int p) {
-    int ns = 1;
-    for (int i = 0; i < p; ++i) ns *= radix_of(N, i);
-    return ns;
-}
-// Offset of pass p's twiddles in the per-length table: entries [k*R + r] for
-// k < Ns, r < R hold exp(DIR*2*pi*i*r*k/(Ns*R)); pass 0 has none.
-__host__ __device__ constexpr int twoff_of(int N, int p) {
-    int off = 0;
-    for (int i = 1; i < p; ++i) off += ns_of(N, i) * radix_of(N, i);
-    return off;
-}
-__host__ __device__ constexpr int twsize_of(int N) { return twoff_of(N, npass_of(N)); }
-__host__ __device__ constexpr int ilog2c(int v) { return v <= 1 ? 0 : 1 + ilog2c(v / 2); }
-
-// Launch geometry of one axis pass (all offsets in elements).
-struct FftGeom {
-    AxisMap fft;        // the transform axis
-    AxisMap a1;         // outer enumeration axis of lines / tiles
-    AxisMap a2;         // inner enumeration axis (z columns for strided tiles)
-    uint32_t n2;        // units along a2 (lines, or tiles of T columns)
-    uint32_t nunits;    // total lines (contiguous) or tiles (strided)
-    FastDiv dn2;
-    int64_t tstride;    // strided tiles: element step between the T columns
-    int scale_mode;     // 0: none, 1: multiply by scale, 2: divide by scale
-    double scale;
-};
-
 // Passes p..NP-1 of the Stockham schedule on the thread's registers.
 template <typename real, int N, int E, int T, bool STRIDED, int DIR, int p>
 __device__ __forceinline__ void stockham_passes(typename CT<real>::V (&v)[E], typename CT<real>::V* lds,
@@ -262,29 +215,13 @@ __global__ __launch_bounds__(T * (N / E)) void stockham_kernel(typename CT<real>
     const int tid = threadIdx.x;
     const int t = STRIDED ? tid / T : tid % TPL;
     const int line = STRIDED ? tid % T : tid / TPL;
-
-    int64_t base;
-    bool valid = true;
-    if constexpr (STRIDED) {
-        const uint32_t tile = blockIdx.x;
-        const uint32_t u1 = g.dn2.div(tile);
-        const uint32_t zt = tile - u1 * g.n2;
-        base = g.a1.off(u1) + g.a2.off(zt * T) + int64_t(line) * g.tstride;
-    } else {
-        const uint32_t ln = blockIdx.x * T + line;
-        valid = ln < g.nunits;
-        const uint32_t u1 = g.dn2.div(ln);
-        const uint32_t u2 = ln - u1 * g.n2;
-        base = valid ? g.a1.off(u1) + g.a2.off(u2) : 0;
-    }
-    // off(n) = off(e*TPL) + off(t): the host guarantees that the axis block
-    // extent and TPL divide one another, which makes the offset additive.
-    base += g.fft.off(uint32_t(t));
+    bool valid;
+    const int64_t base = line_base(g, STRIDED, T, TPL, blockIdx.x, tid, &valid);
 
     V v[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-        if (valid) v[e] = data[base + g.fft.off(uint32_t(e * TPL))];
+        if (valid) v[e] = data[base + elem_off(g, e * TPL)];
         else { v[e].x = 0; v[e].y = 0; }
     }
 
@@ -301,7 +238,7 @@ __global__ __launch_bounds__(T * (N / E)) void stockham_kernel(typename CT<real>
         for (int e = 0; e < E; ++e) { v[e].x = v[e].x / s; v[e].y = v[e].y / s; }
     }
 #pragma unroll
-    for (int e = 0; e < E; ++e) data[base + g.fft.off(uint32_t(e * TPL))] = v[e];
+    for (int e = 0; e < E; ++e) data[base + elem_off(g, e * TPL)] = v[e];
 }
 
 }  // namespace a2a

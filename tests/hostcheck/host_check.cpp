@@ -1,0 +1,83 @@
+// CPU-side checker of the HIP library's index arithmetic.  Compiled with g++
+// against the library's own host+device headers, so it evaluates exactly the
+// expressions the kernels evaluate: FastDiv, the six pack/unpack index maps,
+// the launch shapes the dispatcher picks and every address an FFT launch
+// touches.
+#include <stdint.h>
+
+#include "fft_shape.h"
+#include "remap_index.h"
+
+using namespace a2a;
+
+extern "C" {
+
+void hc_fastdiv(uint32_t d, const uint32_t* n, uint32_t* q, int64_t count) {
+    FastDiv f(d);
+    for (int64_t i = 0; i < count; ++i) q[i] = f.div(n[i]);
+}
+
+int hc_remap_dest(int kind, int P, int64_t nlocal, const int* lgs, const int* dims, uint32_t* out) {
+    RemapParams p = make_params(kind, P, nlocal, lgs, dims);
+    for (int64_t i = 0; i < nlocal; ++i) {
+        uint32_t idx = uint32_t(i);
+        switch (kind) {
+        case 0: out[i] = remap_dest<0>(idx, p); break;
+        case 1: out[i] = remap_dest<1>(idx, p); break;
+        case 2: out[i] = remap_dest<2>(idx, p); break;
+        case 3: out[i] = remap_dest<3>(idx, p); break;
+        case 4: out[i] = remap_dest<4>(idx, p); break;
+        default: out[i] = remap_dest<5>(idx, p); break;
+        }
+    }
+    return 0;
+}
+
+// Enumerates (address, n, line) for every element an FFT launch loads and
+// stores.  pass 0..2: the fused passes of a (Ng, P, dims) decomposition;
+// pass -1: forward_1d_fft over P contiguous pencils of length Ng.
+// shape_out: variant, T, E, NT, grid.  Returns the count, or a negative status.
+int64_t hc_fft_enumerate(int Ng, int P, const int* dims, int dbl, int pass, int64_t* addr, int32_t* n_out,
+                         int64_t* line_out, int64_t cap, int64_t* shape_out) {
+    PassMaps pm;
+    if (pass < 0) {
+        pm = batched_pass_maps(Ng, uint32_t(P));
+    } else {
+        int lgs[3] = {Ng / dims[0], Ng / dims[1], Ng / dims[2]};
+        int64_t nlocal = int64_t(lgs[0]) * lgs[1] * lgs[2];
+        PassMaps all[3];
+        fused_pass_maps(Ng, dims, lgs, nlocal, all);
+        pm = all[pass];
+    }
+    Shape sh;
+    FftGeom g;
+    uint32_t grid;
+    int st = resolve_pass(pm.fft, pm.a1, pm.a2, pm.strided, Ng, dbl != 0, &sh, &g, &grid);
+    if (st < 0) return st;
+    shape_out[0] = sh.variant;
+    shape_out[1] = sh.T;
+    shape_out[2] = sh.E;
+    shape_out[3] = sh.NT;
+    shape_out[4] = grid;
+    const bool strided = sh.variant != 0;
+    const int TPL = Ng / sh.E;
+    int64_t c = 0;
+    for (uint32_t b = 0; b < grid; ++b)
+        for (int tid = 0; tid < sh.NT; ++tid) {
+            bool valid;
+            const int64_t base = line_base(g, strided, sh.T, TPL, b, tid, &valid);
+            if (!valid) continue;
+            const int t = strided ? tid / sh.T : tid % TPL;
+            const int line = strided ? tid % sh.T : tid / TPL;
+            for (int e = 0; e < sh.E; ++e) {
+                if (c >= cap) return -100;
+                addr[c] = base + elem_off(g, e * TPL);
+                n_out[c] = t + e * TPL;
+                line_out[c] = int64_t(b) * sh.T + line;
+                ++c;
+            }
+        }
+    return c;
+}
+
+}  // extern "C"

@@ -1,0 +1,129 @@
+"""CPU checks of the HIP library's index arithmetic (no GPU needed).
+
+tests/hostcheck/host_check.cpp is compiled with g++ against the library's own
+host+device headers (common.h, remap_index.h, fft_shape.h), so these tests
+evaluate the exact expressions the kernels run:
+  - FastDiv against integer division;
+  - the six pack/unpack index maps against the oracle (itself pinned to the
+    reference's compiled C twins), every element;
+  - every address each FFT launch loads / stores: in bounds, each element of
+    the [P][nlocal] buffer exactly once, and each line holding one global line
+    in natural order.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+@pytest.fixture(scope="module")
+def hc(tmp_path_factory):
+    out = str(tmp_path_factory.mktemp("hc") / "libhostcheck.so")
+    subprocess.run([os.path.join(HERE, "hostcheck", "build.sh"), out], check=True)
+    L = ctypes.CDLL(out)
+    P = ctypes.c_void_p
+    L.hc_fastdiv.argtypes = [ctypes.c_uint32, P, P, ctypes.c_int64]
+    L.hc_remap_dest.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int64, P, P, P]
+    L.hc_fft_enumerate.argtypes = [ctypes.c_int, ctypes.c_int, P, ctypes.c_int, ctypes.c_int, P, P, P,
+                                   ctypes.c_int64, P]
+    L.hc_fft_enumerate.restype = ctypes.c_int64
+    return L
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def test_fastdiv_exact(hc):
+    rng = np.random.default_rng(0)
+    n = np.concatenate([np.arange(0, 70000, dtype=np.uint64),
+                        rng.integers(0, 2 ** 32, 200000, dtype=np.uint64),
+                        np.array([2 ** 32 - 1, 2 ** 32 - 2, 2 ** 31, 2 ** 31 - 1], dtype=np.uint64)]).astype(np.uint32)
+    divisors = list(range(1, 300)) + [2 ** k for k in range(32)] + [2 ** k - 1 for k in range(2, 32)] + \
+        [2 ** k + 1 for k in range(2, 31)] + list(rng.integers(1, 2 ** 32, 60))
+    q = np.empty_like(n)
+    for d in divisors:
+        hc.hc_fastdiv(int(d), _p(n), _p(q), n.size)
+        assert np.array_equal(q, n // np.uint32(d)), d
+
+
+REMAP_CONFIGS = [
+    (8, 8, None), (32, 8, None), (12, 6, None), (4, 4, None), (64, 4, (2, 2, 1)), (32, 8, (1, 2, 4)),
+    (32, 16, (1, 4, 4)), (64, 64, (1, 8, 8)), (16, 16, (4, 2, 2)), (32, 64, (4, 4, 4)), (16, 8, (2, 4, 1)),
+    (24, 12, None), (48, 24, None), (64, 32, None), (128, 16, (1, 4, 4)),
+]
+
+
+@pytest.mark.parametrize("cfg", REMAP_CONFIGS, ids=lambda c: f"Ng{c[0]}_P{c[1]}_{c[2]}")
+def test_remap_index_maps_match_oracle(hc, cfg):
+    Ng, P, dims = cfg
+    rc, d = O.decompose(Ng, P, dims)
+    assert rc == 0
+    lgs = np.array(d.lgs_t, dtype=np.int32)
+    dm = np.array(d.dims_t, dtype=np.int32)
+    out = np.empty(d.nlocal, dtype=np.uint32)
+    for kind in range(6):
+        hc.hc_remap_dest(kind, d.P, d.nlocal, _p(lgs), _p(dm), _p(out))
+        ref = O.remap_index(kind, d)
+        assert np.array_equal(out.astype(np.int64), ref), O.REMAP_NAMES[kind]
+        assert np.array_equal(np.sort(out), np.arange(d.nlocal)), "not a permutation"
+
+
+FFT_CONFIGS = [
+    (64, 4, (2, 2, 1)), (8, 8, (2, 2, 2)), (32, 8, (2, 2, 2)), (4, 4, (2, 2, 1)), (32, 8, (1, 2, 4)),
+    (64, 16, (1, 4, 4)), (64, 16, (4, 2, 2)), (128, 64, (1, 8, 8)), (32, 64, (4, 4, 4)), (256, 8, (1, 2, 4)),
+    (16, 1, (1, 1, 1)), (128, 16, (1, 4, 4)), (2, 1, (1, 1, 1)), (512, 16, (1, 4, 4)),
+]
+
+
+def _enumerate(hc, Ng, P, dims, dbl, pass_):
+    cap = (P * (Ng // dims[0]) * (Ng // dims[1]) * (Ng // dims[2])) if pass_ >= 0 else P * Ng
+    addr = np.empty(cap, dtype=np.int64)
+    n = np.empty(cap, dtype=np.int32)
+    line = np.empty(cap, dtype=np.int64)
+    shape = np.zeros(5, dtype=np.int64)
+    dm = np.array(dims, dtype=np.int32)
+    c = hc.hc_fft_enumerate(Ng, P, _p(dm), int(dbl), pass_, _p(addr), _p(n), _p(line), cap, _p(shape))
+    assert c >= 0, c
+    return addr[:c], n[:c], line[:c], shape
+
+
+@pytest.mark.parametrize("dbl", [0, 1])
+@pytest.mark.parametrize("cfg", FFT_CONFIGS, ids=lambda c: f"Ng{c[0]}_P{c[1]}_{c[2]}")
+def test_fused_pass_addresses(hc, cfg, dbl):
+    Ng, P, dims = cfg
+    if Ng ** 3 > 2 ** 24 and dbl:
+        pytest.skip("covered by the complex64 case")
+    rc, d = O.decompose(Ng, P, dims)
+    assert rc == 0
+    total = P * d.nlocal
+    # global index of every block-layout element
+    gid = O.global_to_blocks(d, np.arange(Ng ** 3, dtype=np.int64).reshape((Ng,) * 3)).reshape(-1)
+    for pass_, axis in ((0, 2), (1, 0), (2, 1)):
+        addr, n, line, shape = _enumerate(hc, Ng, P, dims, dbl, pass_)
+        assert addr.size == total
+        assert addr.min() >= 0 and addr.max() < total
+        assert np.array_equal(np.sort(addr), np.arange(total)), f"pass {pass_}: not every element exactly once"
+        coords = np.stack(np.unravel_index(gid[addr], (Ng,) * 3))
+        assert np.array_equal(coords[axis], n), f"pass {pass_}: element n is not coordinate n along the axis"
+        other = [a for a in range(3) if a != axis]
+        key = coords[other[0]] * Ng + coords[other[1]]
+        order = np.argsort(line, kind="stable")
+        k_sorted = key[order].reshape(-1, Ng)
+        assert (k_sorted == k_sorted[:, :1]).all(), f"pass {pass_}: a line mixes global lines"
+
+
+@pytest.mark.parametrize("N", [2, 4, 8, 16, 32, 64, 128, 256, 512, 1024, 2048, 4096])
+@pytest.mark.parametrize("lines", [1, 7, 33])
+def test_batched_pencil_addresses(hc, N, lines):
+    for dbl in (0, 1):
+        addr, n, line, shape = _enumerate(hc, N, lines, (1, 1, 1), dbl, -1)
+        assert np.array_equal(np.sort(addr), np.arange(N * lines))
+        assert np.array_equal(addr % N, n)
+        assert np.array_equal(addr // N, line)
