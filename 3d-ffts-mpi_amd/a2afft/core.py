@@ -150,6 +150,19 @@ class Dfft:
         self.dist._check_buf(d_data)
         return L.check(L.lib().a2afft_inverse(self._h, _ptr(d_data), _stream()), "inverse")
 
+    def autotune(self, work, iters=5):
+        """Pick the fastest kernel variant per pass; `work` contents are destroyed."""
+        self.dist._check_buf(work)
+        return L.check(L.lib().a2afft_plan_autotune(self._h, _ptr(work), iters, _stream()), "autotune")
+
+    def set_kernel(self, pass_index, T, E, xcd=0, pad=0):
+        return L.check(L.lib().a2afft_plan_set_kernel(self._h, pass_index, T, E, xcd, pad), "set_kernel")
+
+    def describe(self):
+        buf = ctypes.create_string_buffer(8192)
+        L.check(L.lib().a2afft_plan_describe(self._h, buf, len(buf)), "describe")
+        return buf.value.decode()
+
     def destroy(self):
         if self._h:
             L.lib().a2afft_plan_destroy(self._h)

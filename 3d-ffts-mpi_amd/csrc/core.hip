@@ -261,6 +261,8 @@ struct a2afft_plan_s {
     a2afft_exec exec;
     void* scratch;
     AxisPass pass[3];   // fused: z, x, y over the block layout
+    KSel sel[3];        // kernel variant per fused pass
+    char report[4096];  // last autotune table
 };
 
 extern "C" int a2afft_plan_create(a2afft_plan* out, a2afft_dist dist, a2afft_precision prec, a2afft_exec exec) {
@@ -287,9 +289,11 @@ extern "C" int a2afft_plan_create(a2afft_plan* out, a2afft_dist dist, a2afft_pre
     PassMaps pm[3];
     fused_pass_maps(dist->Ng, dist->dims, dist->lgs, dist->nlocal, pm);
     for (int i = 0; i < 3; ++i) p->pass[i] = AxisPass{pm[i].fft, pm[i].a1, pm[i].a2, pm[i].strided, 0, 1.0};
+    p->report[0] = 0;
     FftShape sh;
     for (int i = 0; i < 3; ++i) {
-        int st = fft_pass_shape(p->pass[i], dist->dtype, dist->Ng, &sh);
+        p->sel[i] = fft_default_sel(p->pass[i], dist->dtype, dist->Ng);
+        int st = fft_pass_shape(p->pass[i], dist->dtype, dist->Ng, &p->sel[i], &sh);
         if (st < 0) { delete p; return st; }
     }
     // Build the twiddle tables now so execute allocates nothing.
@@ -316,7 +320,7 @@ static int exec_staged(a2afft_plan_s* p, void* D, int dir, hipStream_t s) {
     auto fft = [&](void* buf) -> int {
         const PassMaps pm = batched_pass_maps(d->Ng, uint32_t(total / d->Ng));
         const AxisPass ps{pm.fft, pm.a1, pm.a2, 0, 0, 1.0};
-        return launch_fft_pass(ps, buf, d->dtype, d->Ng, dir, s);
+        return launch_fft_pass(ps, buf, d->dtype, d->Ng, dir, s, nullptr);
     };
     int warn = 0;
     A2A_TRY(stage(d, 0, D, S, s));
@@ -349,7 +353,7 @@ static int exec_fused(a2afft_plan_s* p, void* D, int dir, hipStream_t s) {
             ps.scale = double(int64_t(d->Ng) * d->Ng * d->Ng);
             ps.scale_mode = 2;
         }
-        A2A_TRY(launch_fft_pass(ps, D, d->dtype, d->Ng, dir, s));
+        A2A_TRY(launch_fft_pass(ps, D, d->dtype, d->Ng, dir, s, &p->sel[i]));
     }
     return A2AFFT_OK;
 }
@@ -377,7 +381,85 @@ extern "C" int a2afft_execute_pass(a2afft_plan p, void* data, int pass, int dire
         ps.scale = double(int64_t(p->d.Ng) * p->d.Ng * p->d.Ng);
         ps.scale_mode = 2;
     }
-    return finish_call(stream, launch_fft_pass(ps, data, p->d.dtype, p->d.Ng, direction, (hipStream_t)stream));
+    return finish_call(stream,
+                       launch_fft_pass(ps, data, p->d.dtype, p->d.Ng, direction, (hipStream_t)stream, &p->sel[pass]));
+}
+
+extern "C" int a2afft_plan_set_kernel(a2afft_plan p, int pass, int T, int E, int xcd, int pad) {
+    if (!p || pass < 0 || pass > 2) return set_error(A2AFFT_E_INVALID_ARG, "set_kernel: bad arguments");
+    KSel s{T, E, xcd ? 1 : 0, pad ? 1 : 0};
+    KSel cand[256];
+    const int n = fft_candidates(p->pass[pass], p->d.dtype, p->d.Ng, cand, 256);
+    for (int i = 0; i < n; ++i)
+        if (cand[i].T == s.T && cand[i].E == s.E && cand[i].xcd == s.xcd && cand[i].pad == s.pad) {
+            p->sel[pass] = s;
+            return A2AFFT_OK;
+        }
+    return set_error(A2AFFT_E_UNSUPPORTED, "set_kernel: no built variant T=%d E=%d xcd=%d pad=%d fits pass %d", T, E,
+                     xcd, pad, pass);
+}
+
+// Times every built variant that fits each fused pass (forward direction,
+// `iters` launches each, in place on `work`, whose contents are destroyed)
+// and keeps the fastest.  Every variant computes identical bits.
+extern "C" int a2afft_plan_autotune(a2afft_plan p, void* work, int iters, void* stream) {
+    if (!p || !work || iters <= 0) return set_error(A2AFFT_E_INVALID_ARG, "autotune: bad arguments");
+    if (p->exec != A2AFFT_EXEC_FUSED) return set_error(A2AFFT_E_INVALID_ARG, "autotune: fused plans only");
+    hipStream_t s = (hipStream_t)stream;
+    hipEvent_t e0, e1;
+    A2A_TRY(hip_check(int(hipEventCreate(&e0)), "autotune event"));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return set_error(A2AFFT_E_HIP, "autotune event");
+    }
+    int len = 0;
+    const char* names[3] = {"z", "x", "y"};
+    int st = A2AFFT_OK;
+    for (int i = 0; i < 3 && st >= 0; ++i) {
+        KSel cand[256];
+        const int n = fft_candidates(p->pass[i], p->d.dtype, p->d.Ng, cand, 256);
+        float best = 1e30f;
+        for (int c = 0; c < n && st >= 0; ++c) {
+            st = launch_fft_pass(p->pass[i], work, p->d.dtype, p->d.Ng, -1, s, &cand[c]);   // warm
+            if (st < 0) break;
+            st = hip_check(int(hipEventRecord(e0, s)), "autotune record");
+            for (int it = 0; it < iters && st >= 0; ++it)
+                st = launch_fft_pass(p->pass[i], work, p->d.dtype, p->d.Ng, -1, s, &cand[c]);
+            if (st < 0) break;
+            st = hip_check(int(hipEventRecord(e1, s)), "autotune record");
+            if (st >= 0) st = hip_check(int(hipEventSynchronize(e1)), "autotune sync");
+            float ms = 0.f;
+            if (st >= 0) st = hip_check(int(hipEventElapsedTime(&ms, e0, e1)), "autotune elapsed");
+            if (st < 0) break;
+            ms /= float(iters);
+            if (ms < best) {
+                best = ms;
+                p->sel[i] = cand[c];
+            }
+            if (len < int(sizeof(p->report)) - 96)
+                len += snprintf(p->report + len, sizeof(p->report) - len,
+                                "pass %s T=%d E=%d NT=%d xcd=%d pad=%d %.4f ms\n", names[i], cand[c].T, cand[c].E,
+                                cand[c].T * (p->d.Ng / cand[c].E), cand[c].xcd, cand[c].pad, ms);
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return st < 0 ? st : A2AFFT_OK;
+}
+
+// Text description of the chosen kernel per pass and the last autotune table.
+extern "C" int a2afft_plan_describe(a2afft_plan p, char* buf, int len) {
+    if (!p || !buf || len <= 0) return set_error(A2AFFT_E_INVALID_ARG, "describe: bad arguments");
+    const char* names[3] = {"z", "x", "y"};
+    int n = 0;
+    for (int i = 0; i < 3 && n < len; ++i) {
+        FftShape sh;
+        if (fft_pass_shape(p->pass[i], p->d.dtype, p->d.Ng, &p->sel[i], &sh) < 0) continue;
+        n += snprintf(buf + n, len - n, "chosen %s T=%d E=%d NT=%d xcd=%d pad=%d grid=%d\n", names[i], sh.T, sh.E,
+                      sh.NT, sh.xcd, sh.pad, sh.grid);
+    }
+    if (n < len) snprintf(buf + n, len - n, "%s", p->report);
+    return A2AFFT_OK;
 }
 
 extern "C" int a2afft_plan_destroy(a2afft_plan p) {

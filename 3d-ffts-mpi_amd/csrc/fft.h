@@ -2,6 +2,7 @@
 #pragma once
 
 #include "common.h"
+#include "fft_shape.h"
 
 #if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
@@ -30,13 +31,19 @@ bool fft_length_supported(int N);
 const void* twiddle_table(int N, int dir, a2afft_dtype dtype);
 
 // Launch one pass.  data: complex of dtype.  dir: -1 forward, +1 inverse.
-int launch_fft_pass(const AxisPass& pass, void* data, a2afft_dtype dtype, int N, int dir, hipStream_t s);
+// sel: the kernel variant (null = the default for this geometry).
+int launch_fft_pass(const AxisPass& pass, void* data, a2afft_dtype dtype, int N, int dir, hipStream_t s,
+                    const KSel* sel);
 
-// Launch shape the dispatcher picks for a pass (for tests and tuning reports).
+// Launch shape of a pass (for tests and tuning reports).
 struct FftShape {
-    int T, E, NT, grid, lds_bytes, fallback;
+    int T, E, NT, grid, xcd, pad;
 };
-int fft_pass_shape(const AxisPass& pass, a2afft_dtype dtype, int N, FftShape* out);
+int fft_pass_shape(const AxisPass& pass, a2afft_dtype dtype, int N, const KSel* sel, FftShape* out);
+
+// The built variants that can run this pass; returns the count.
+int fft_candidates(const AxisPass& pass, a2afft_dtype dtype, int N, KSel* out, int max);
+KSel fft_default_sel(const AxisPass& pass, a2afft_dtype dtype, int N);
 
 // exp(dir*2*pi*i*m/M) in long double (quadrant-reduced), for table building.
 void twiddle_ld(int64_t m, int64_t M, int dir, long double* re, long double* im);

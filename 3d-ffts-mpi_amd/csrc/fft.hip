@@ -2,6 +2,7 @@
 // and the forward_1d_fft / inverse_1d_fft entry points that replace the cuFFT
 // calls of src/a2afft/dfft/kernels/a2affts.cu:7-76.
 #include "fft.h"
+#include "fft_shape.h"
 #include "stockham.h"
 
 #include <math.h>
@@ -11,78 +12,93 @@
 
 namespace a2a {
 
+int register_kernels_f32(KEntry* out);
+int register_kernels_f64(KEntry* out);
+
 bool fft_length_supported(int N) { return fft_len_ok(N); }
 
-template <typename real, int N, int DIR>
-static const void* kernel_variant(int variant) {
-    constexpr bool dbl = sizeof(real) == 8;
-    if (variant == 0)
-        return (const void*)&stockham_kernel<real, N, cont_E(N), cont_T(N), false, DIR>;
-    if (variant == 1)
-        return (const void*)&stockham_kernel<real, N, str_E(N, dbl), str_T(N, dbl), true, DIR>;
-    return (const void*)&stockham_kernel<real, N, rmax_of(N), 1, true, DIR>;
-}
+namespace {
+constexpr int kMaxKernels = 4096;
+KEntry g_kernels[kMaxKernels];
+int g_nkernels = 0;
+std::once_flag g_reg_once;
 
-template <typename real, int DIR>
-static const void* kernel_n(int N, int variant) {
-    switch (N) {
-    case 2: return kernel_variant<real, 2, DIR>(variant);
-    case 4: return kernel_variant<real, 4, DIR>(variant);
-    case 8: return kernel_variant<real, 8, DIR>(variant);
-    case 16: return kernel_variant<real, 16, DIR>(variant);
-    case 32: return kernel_variant<real, 32, DIR>(variant);
-    case 64: return kernel_variant<real, 64, DIR>(variant);
-    case 128: return kernel_variant<real, 128, DIR>(variant);
-    case 256: return kernel_variant<real, 256, DIR>(variant);
-    case 512: return kernel_variant<real, 512, DIR>(variant);
-    case 1024: return kernel_variant<real, 1024, DIR>(variant);
-    case 2048: return kernel_variant<real, 2048, DIR>(variant);
-    case 4096: return kernel_variant<real, 4096, DIR>(variant);
-    default: return nullptr;
+void init_registry() {
+    std::call_once(g_reg_once, [] {
+        int n = register_kernels_f32(g_kernels);
+        n += register_kernels_f64(g_kernels + n);
+        g_nkernels = n;
+    });
+}
+}  // namespace
+
+const void* find_kernel(int N, bool dbl, int strided, const KSel& s, int dir) {
+    init_registry();
+    for (int i = 0; i < g_nkernels; ++i) {
+        const KEntry& k = g_kernels[i];
+        if (k.N == N && k.dbl == int(dbl) && k.strided == strided && k.T == s.T && k.E == s.E && k.xcd == s.xcd &&
+            k.pad == s.pad && k.dir == (dir < 0 ? -1 : 1))
+            return k.fn;
     }
+    return nullptr;
 }
 
-static const void* kernel_for(a2afft_dtype dt, int N, int dir, int variant) {
-    if (dt == A2AFFT_C128) return dir < 0 ? kernel_n<double, -1>(N, variant) : kernel_n<double, 1>(N, variant);
-    return dir < 0 ? kernel_n<float, -1>(N, variant) : kernel_n<float, 1>(N, variant);
+int fft_candidates(const AxisPass& p, a2afft_dtype dt, int N, KSel* out, int max) {
+    init_registry();
+    const bool dbl = dt == A2AFFT_C128;
+    int n = 0;
+    for (int i = 0; i < g_nkernels && n < max; ++i) {
+        const KEntry& k = g_kernels[i];
+        if (k.N != N || k.dbl != int(dbl) || k.strided != p.strided || k.dir != -1) continue;
+        if (!shape_fits(p.fft, p.a2, p.strided, N, k.T, k.E)) continue;
+        out[n++] = KSel{k.T, k.E, k.xcd, k.pad};
+    }
+    return n;
 }
 
-static int resolve(const AxisPass& p, a2afft_dtype dt, int N, Shape* sh, FftGeom* g, uint32_t* grid) {
-    int st = resolve_pass(p.fft, p.a1, p.a2, p.strided, N, dt == A2AFFT_C128, sh, g, grid);
+KSel fft_default_sel(const AxisPass& p, a2afft_dtype dt, int N) {
+    return default_sel(N, dt == A2AFFT_C128, p.strided, p.fft, p.a2);
+}
+
+static int resolve(const AxisPass& p, a2afft_dtype dt, int N, const KSel& s, FftGeom* g, uint32_t* grid) {
+    int st = resolve_with(p.fft, p.a1, p.a2, p.strided, N, s.T, s.E, g, grid);
     if (st == A2AFFT_E_UNSUPPORTED)
-        return set_error(st, "fft pass: length %d / block extent %u not supported", N, p.fft.l);
+        return set_error(st, "fft pass: length %d (block extent %u, T=%d, E=%d) not supported", N, p.fft.l, s.T, s.E);
     if (st < 0) return set_error(st, "fft pass: bad geometry (N=%d)", N);
     g->scale_mode = p.scale_mode;
     g->scale = p.scale;
     return A2AFFT_OK;
 }
 
-int fft_pass_shape(const AxisPass& p, a2afft_dtype dt, int N, FftShape* out) {
-    Shape sh;
+int fft_pass_shape(const AxisPass& p, a2afft_dtype dt, int N, const KSel* sel, FftShape* out) {
+    const KSel s = sel ? *sel : fft_default_sel(p, dt, N);
     FftGeom g;
     uint32_t grid;
-    A2A_TRY(resolve(p, dt, N, &sh, &g, &grid));
-    out->T = sh.T;
-    out->E = sh.E;
-    out->NT = sh.NT;
+    A2A_TRY(resolve(p, dt, N, s, &g, &grid));
+    out->T = s.T;
+    out->E = s.E;
+    out->NT = s.T * (N / s.E);
     out->grid = int(grid);
-    out->lds_bytes = sh.lds;
-    out->fallback = sh.variant == 2;
+    out->xcd = s.xcd;
+    out->pad = s.pad;
     return A2AFFT_OK;
 }
 
-int launch_fft_pass(const AxisPass& p, void* data, a2afft_dtype dt, int N, int dir, hipStream_t s) {
+int launch_fft_pass(const AxisPass& p, void* data, a2afft_dtype dt, int N, int dir, hipStream_t s, const KSel* sel) {
     if (!data) return set_error(A2AFFT_E_INVALID_ARG, "fft: null data");
-    Shape sh;
+    const KSel ks = sel ? *sel : fft_default_sel(p, dt, N);
     FftGeom g;
     uint32_t grid;
-    A2A_TRY(resolve(p, dt, N, &sh, &g, &grid));
+    A2A_TRY(resolve(p, dt, N, ks, &g, &grid));
     if (grid == 0) return A2AFFT_OK;
     const void* tw = twiddle_table(N, dir, dt);
     if (!tw) return set_error(A2AFFT_E_ALLOC, "fft: twiddle table for N=%d", N);
-    const void* fn = kernel_for(dt, N, dir, sh.variant);
+    const void* fn = find_kernel(N, dt == A2AFFT_C128, p.strided, ks, dir);
+    if (!fn)
+        return set_error(A2AFFT_E_UNSUPPORTED, "fft: no kernel built for N=%d T=%d E=%d xcd=%d pad=%d", N, ks.T, ks.E,
+                         ks.xcd, ks.pad);
     void* args[] = {&data, &g, (void*)&tw};
-    int st = hip_check(int(hipLaunchKernel(fn, dim3(grid), dim3(sh.NT), args, 0, s)), "stockham launch");
+    int st = hip_check(int(hipLaunchKernel(fn, dim3(grid), dim3(ks.T * (N / ks.E)), args, 0, s)), "stockham launch");
     if (st < 0) return st;
     return hip_check(int(hipGetLastError()), "stockham launch");
 }
@@ -179,7 +195,7 @@ static int fft_1d(void* data, int Ng, int64_t n_elems, a2afft_dtype dtype, int d
     if (lines >= (int64_t(1) << 32)) return set_error(A2AFFT_E_SIZE, "1d fft: too many lines");
     const PassMaps pm = batched_pass_maps(Ng, uint32_t(lines));
     const AxisPass p{pm.fft, pm.a1, pm.a2, 0, 0, 1.0};
-    return launch_fft_pass(p, data, dtype, Ng, dir, (hipStream_t)stream);
+    return launch_fft_pass(p, data, dtype, Ng, dir, (hipStream_t)stream, nullptr);
 }
 
 extern "C" int a2afft_forward_1d_fft(void* data, int Ng, int64_t n_elems, a2afft_dtype dtype, void* stream) {

@@ -1,0 +1,38 @@
+// complex64 Stockham kernel variants.
+#include "stockham.h"
+
+namespace a2a {
+
+int register_kernels_f32(KEntry* out) {
+    int n = 0;
+    add_defaults<float, 2>(out, &n);
+    add_defaults<float, 4>(out, &n);
+    add_defaults<float, 8>(out, &n);
+    add_defaults<float, 16>(out, &n);
+    add_defaults<float, 32>(out, &n);
+    add_defaults<float, 64>(out, &n);
+    add_defaults<float, 128>(out, &n);
+    add_defaults<float, 256>(out, &n);
+    add_defaults<float, 512>(out, &n);
+    add_defaults<float, 1024>(out, &n);
+    add_defaults<float, 2048>(out, &n);
+    add_defaults<float, 4096>(out, &n);
+    // tuning candidates for the benchmark lengths
+    add_shape<float, 512, 16, 16, true>(out, &n);    // 512 threads, 64 KiB LDS
+    add_shape<float, 512, 16, 8, true>(out, &n);     // 256 threads, 32 KiB LDS, 64 B rows
+    add_shape<float, 512, 8, 8, true>(out, &n);      // 512 threads, 32 KiB LDS
+    add_shape<float, 512, 32, 32, true>(out, &n);    // 512 threads, 128 KiB LDS, 256 B rows
+    add_shape<float, 512, 8, 4, true>(out, &n);      // 256 threads, 16 KiB LDS, 32 B rows
+    add_shape<float, 512, 8, 4, false>(out, &n);     // 256 threads, 16 KiB LDS
+    add_shape<float, 512, 16, 16, false>(out, &n);   // 512 threads, 64 KiB LDS
+    add_shape<float, 512, 32, 16, false>(out, &n);   // 256 threads, 64 KiB LDS
+    add_shape<float, 256, 16, 8, true>(out, &n);
+    add_shape<float, 256, 16, 32, true>(out, &n);
+    add_shape<float, 256, 32, 32, true>(out, &n);
+    add_shape<float, 2048, 16, 4, false>(out, &n);   // 512 threads, 64 KiB LDS
+    add_shape<float, 2048, 32, 4, false>(out, &n);   // 256 threads, 64 KiB LDS
+    add_shape<float, 2048, 16, 1, false>(out, &n);   // 128 threads, 16 KiB LDS
+    return n;
+}
+
+}  // namespace a2a

@@ -1,0 +1,33 @@
+// complex128 Stockham kernel variants.
+#include "stockham.h"
+
+namespace a2a {
+
+int register_kernels_f64(KEntry* out) {
+    int n = 0;
+    add_defaults<double, 2>(out, &n);
+    add_defaults<double, 4>(out, &n);
+    add_defaults<double, 8>(out, &n);
+    add_defaults<double, 16>(out, &n);
+    add_defaults<double, 32>(out, &n);
+    add_defaults<double, 64>(out, &n);
+    add_defaults<double, 128>(out, &n);
+    add_defaults<double, 256>(out, &n);
+    add_defaults<double, 512>(out, &n);
+    add_defaults<double, 1024>(out, &n);
+    add_defaults<double, 2048>(out, &n);
+    add_defaults<double, 4096>(out, &n);
+    // tuning candidates for the benchmark lengths
+    add_shape<double, 256, 16, 16, true>(out, &n);   // 256 threads, 64 KiB LDS
+    add_shape<double, 256, 16, 4, true>(out, &n);    // 64 threads, 16 KiB LDS
+    add_shape<double, 256, 32, 16, true>(out, &n);   // 128 threads
+    add_shape<double, 256, 16, 8, false>(out, &n);   // 128 threads, 32 KiB LDS
+    add_shape<double, 256, 32, 16, false>(out, &n);  // 128 threads, 64 KiB LDS
+    add_shape<double, 512, 8, 8, true>(out, &n);     // 512 threads
+    add_shape<double, 512, 8, 4, true>(out, &n);     // 256 threads, 32 KiB LDS
+    add_shape<double, 512, 8, 4, false>(out, &n);
+    add_shape<double, 512, 16, 4, false>(out, &n);
+    return n;
+}
+
+}  // namespace a2a

@@ -82,6 +82,14 @@ A2A_HD int64_t line_base(const FftGeom& g, bool strided, int T, int TPL, uint32_
     return base + g.fft.off(uint32_t(t));
 }
 
+// Workgroup renumbering for XCD-aware tile order: hardware hands workgroup b
+// to XCD b % 8; this gives XCD x the contiguous logical range
+// [x*per + min(x, rem), ...) of the G workgroups.  A bijection on [0, G).
+A2A_HD uint32_t xcd_block(uint32_t b, uint32_t G) {
+    const uint32_t x = b & 7u, l = b >> 3, per = G >> 3, rem = G & 7u;
+    return x < rem ? x * (per + 1) + l : rem * (per + 1) + (x - rem) * per + l;
+}
+
 A2A_HD int64_t elem_off(const FftGeom& g, int n0) { return g.fft.off(uint32_t(n0)); }
 
 // ---------------------------------------------------------------------------
@@ -99,38 +107,40 @@ A2A_HD constexpr int str_E(int N, bool dbl) {
     return (N * str_T(N, dbl)) / 256 > rmax_of(N) ? (N * str_T(N, dbl)) / 256 : rmax_of(N);
 }
 
-// variant 0: contiguous; 1: strided wide tile; 2: strided single column.
-struct Shape {
-    int variant, T, E, NT, lds;
+inline bool fft_len_ok(int N) { return N >= 2 && N <= 4096 && (N & (N - 1)) == 0; }
+
+// Kernel choice for one pass: tile width T, elements per thread E, XCD-aware
+// tile order and LDS padding.  NT = T * N / E threads per workgroup.
+struct KSel {
+    int T, E, xcd, pad;
 };
 
-inline Shape shape_of(int N, bool dbl, int variant) {
-    Shape s;
-    s.variant = variant;
-    if (variant == 0) { s.T = cont_T(N); s.E = cont_E(N); }
-    else if (variant == 1) { s.T = str_T(N, dbl); s.E = str_E(N, dbl); }
-    else { s.T = 1; s.E = rmax_of(N); }
-    s.NT = s.T * (N / s.E);
-    s.lds = npass_of(N) > 1 ? N * s.T * (dbl ? 16 : 8) : 0;
+// True when shape (T, E) can run this pass: E holds whole butterflies of every
+// radix, the axis block extent and N/E divide one another (additive offsets),
+// and a strided tile of T columns stays inside one rank's z extent.
+inline bool shape_fits(const AxisMap& fft, const AxisMap& a2, int strided, int N, int T, int E) {
+    if (E < rmax_of(N) || E > N || N % E || E % rmax_of(N)) return false;
+    const uint32_t TPL = uint32_t(N / E);
+    if (!(fft.l % TPL == 0 || TPL % fft.l == 0)) return false;
+    if (T * (N / E) > 1024) return false;
+    if (strided && T > 1 && (a2.s_elem != 1 || a2.l % uint32_t(T) || a2.n % uint32_t(T))) return false;
+    return true;
+}
+
+// Default shape of a pass (before tuning).
+inline KSel default_sel(int N, bool dbl, int strided, const AxisMap& fft, const AxisMap& a2) {
+    if (!strided) return KSel{cont_T(N), cont_E(N), 0, 0};
+    KSel s{str_T(N, dbl), str_E(N, dbl), 0, 0};
+    if (!shape_fits(fft, a2, 1, N, s.T, s.E)) s = KSel{1, rmax_of(N), 0, 0};
     return s;
 }
 
-inline bool fft_len_ok(int N) { return N >= 2 && N <= 4096 && (N & (N - 1)) == 0; }
-
-// Resolve the shape and the final geometry of one pass; returns 0 or a
-// negative status (the caller attaches the message).
-inline int resolve_pass(const AxisMap& fft, const AxisMap& a1, const AxisMap& a2, int strided, int N, bool dbl,
-                        Shape* sh, FftGeom* g, uint32_t* grid) {
+// Final geometry of a pass for a chosen shape; returns 0 or a negative status.
+inline int resolve_with(const AxisMap& fft, const AxisMap& a1, const AxisMap& a2, int strided, int N, int T, int E,
+                        FftGeom* g, uint32_t* grid) {
     if (!fft_len_ok(N)) return A2AFFT_E_UNSUPPORTED;
     if (fft.n != uint32_t(N)) return A2AFFT_E_INVALID_ARG;
-    int variant = strided ? 1 : 0;
-    if (strided) {
-        Shape wide = shape_of(N, dbl, 1);
-        if (a2.s_elem != 1 || a2.l % wide.T || a2.n % wide.T) variant = 2;
-    }
-    *sh = shape_of(N, dbl, variant);
-    const uint32_t TPL = uint32_t(N / sh->E);
-    if (!(fft.l % TPL == 0 || TPL % fft.l == 0)) return A2AFFT_E_UNSUPPORTED;
+    if (!shape_fits(fft, a2, strided, N, T, E)) return A2AFFT_E_UNSUPPORTED;
     g->fft = fft;
     g->a1 = a1;
     g->a2 = a2;
@@ -138,12 +148,12 @@ inline int resolve_pass(const AxisMap& fft, const AxisMap& a1, const AxisMap& a2
     g->scale = 1.0;
     g->tstride = a2.s_elem;
     uint64_t units, blocks;
-    if (variant == 0) {
+    if (!strided) {
         g->n2 = a2.n;
         units = uint64_t(a1.n) * a2.n;
-        blocks = (units + sh->T - 1) / sh->T;
+        blocks = (units + T - 1) / T;
     } else {
-        g->n2 = a2.n / uint32_t(sh->T);
+        g->n2 = a2.n / uint32_t(T);
         units = uint64_t(a1.n) * g->n2;
         blocks = units;
     }

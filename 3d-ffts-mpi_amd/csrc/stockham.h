@@ -152,8 +152,22 @@ template <int R, int DIR, typename V> __device__ __forceinline__ void dft(V* x) 
     else dft16<DIR>(x);
 }
 
+// LDS slot of element d of line `line`.  Contiguous tiles are line-major,
+// strided tiles column-interleaved (the T lines side by side).  PAD inserts
+// one slot every 16 (contiguous) or widens a row to T+1 (strided) so that
+// the strided write pattern of the early passes spreads over more banks.
+template <int N, int T, bool STRIDED, int PAD>
+__device__ __forceinline__ constexpr int lds_slot(int line, int d) {
+    if constexpr (STRIDED) return PAD ? d * (T + 1) + line : d * T + line;
+    else return PAD ? line * (N + N / 16) + d + (d >> 4) : line * N + d;
+}
+template <int N, int T, bool STRIDED, int PAD>
+constexpr int lds_slots() {
+    return npass_of(N) < 2 ? 1 : (STRIDED ? (PAD ? N * (T + 1) : N * T) : (PAD ? T * (N + N / 16) : T * N));
+}
+
 // Passes p..NP-1 of the Stockham schedule on the thread's registers.
-template <typename real, int N, int E, int T, bool STRIDED, int DIR, int p>
+template <typename real, int N, int E, int T, bool STRIDED, int DIR, int PAD, int p>
 __device__ __forceinline__ void stockham_passes(typename CT<real>::V (&v)[E], typename CT<real>::V* lds,
                                                 const typename CT<real>::V* __restrict__ tw, int t, int line) {
     using V = typename CT<real>::V;
@@ -187,36 +201,32 @@ __device__ __forceinline__ void stockham_passes(typename CT<real>::V (&v)[E], ty
                 const int j = t + b * TPL;
                 const int d0 = (j / Ns) * (Ns * R) + (j & (Ns - 1));
 #pragma unroll
-                for (int q = 0; q < R; ++q) {
-                    const int d = d0 + q * Ns;
-                    lds[STRIDED ? d * T + line : line * N + d] = v[b + q * NB];
-                }
+                for (int q = 0; q < R; ++q) lds[lds_slot<N, T, STRIDED, PAD>(line, d0 + q * Ns)] = v[b + q * NB];
             }
             __syncthreads();
 #pragma unroll
-            for (int e = 0; e < E; ++e) {
-                const int n = t + e * TPL;
-                v[e] = lds[STRIDED ? n * T + line : line * N + n];
-            }
+            for (int e = 0; e < E; ++e) v[e] = lds[lds_slot<N, T, STRIDED, PAD>(line, t + e * TPL)];
         }
-        stockham_passes<real, N, E, T, STRIDED, DIR, p + 1>(v, lds, tw, t, line);
+        stockham_passes<real, N, E, T, STRIDED, DIR, PAD, p + 1>(v, lds, tw, t, line);
     }
 }
 
-template <typename real, int N, int E, int T, bool STRIDED, int DIR>
+// One workgroup = T lines (contiguous) or one tile of T columns (strided).
+// XCD = 1 renumbers workgroups so each of the 8 XCDs (which receive
+// workgroups round-robin) works through a contiguous range of tiles.
+template <typename real, int N, int E, int T, bool STRIDED, int DIR, int XCD, int PAD>
 __global__ __launch_bounds__(T * (N / E)) void stockham_kernel(typename CT<real>::V* __restrict__ data, FftGeom g,
                                                                const typename CT<real>::V* __restrict__ tw) {
     using V = typename CT<real>::V;
     constexpr int TPL = N / E;
-    constexpr int NP = npass_of(N);
-    constexpr int LDS = NP > 1 ? N * T : 1;
-    __shared__ V lds[LDS];
+    __shared__ V lds[lds_slots<N, T, STRIDED, PAD>()];
 
     const int tid = threadIdx.x;
     const int t = STRIDED ? tid / T : tid % TPL;
     const int line = STRIDED ? tid % T : tid / TPL;
+    const uint32_t block = XCD ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
     bool valid;
-    const int64_t base = line_base(g, STRIDED, T, TPL, blockIdx.x, tid, &valid);
+    const int64_t base = line_base(g, STRIDED, T, TPL, block, tid, &valid);
 
     V v[E];
 #pragma unroll
@@ -225,7 +235,7 @@ __global__ __launch_bounds__(T * (N / E)) void stockham_kernel(typename CT<real>
         else { v[e].x = 0; v[e].y = 0; }
     }
 
-    stockham_passes<real, N, E, T, STRIDED, DIR, 0>(v, lds, tw, t, line);
+    stockham_passes<real, N, E, T, STRIDED, DIR, PAD, 0>(v, lds, tw, t, line);
 
     if (!valid) return;
     if (g.scale_mode == 1) {
@@ -239,6 +249,39 @@ __global__ __launch_bounds__(T * (N / E)) void stockham_kernel(typename CT<real>
     }
 #pragma unroll
     for (int e = 0; e < E; ++e) data[base + elem_off(g, e * TPL)] = v[e];
+}
+
+// Registry of built kernel variants (filled by the per-precision files).
+struct KEntry {
+    int N, dbl, strided, T, E, xcd, pad, dir;
+    const void* fn;
+};
+
+template <typename real, int N, int E, int T, bool S, int XCD, int PAD>
+inline void add_variant(KEntry* out, int* n) {
+    constexpr int dbl = sizeof(real) == 8;
+    out[(*n)++] = KEntry{N, dbl, S, T, E, XCD, PAD, -1, (const void*)&stockham_kernel<real, N, E, T, S, -1, XCD, PAD>};
+    out[(*n)++] = KEntry{N, dbl, S, T, E, XCD, PAD, 1, (const void*)&stockham_kernel<real, N, E, T, S, 1, XCD, PAD>};
+}
+
+// Both tile orders and both LDS layouts of one shape.
+template <typename real, int N, int E, int T, bool S>
+inline void add_shape(KEntry* out, int* n) {
+    add_variant<real, N, E, T, S, 0, 0>(out, n);
+    if constexpr (npass_of(N) > 1) {
+        add_variant<real, N, E, T, S, 1, 0>(out, n);
+        add_variant<real, N, E, T, S, 0, 1>(out, n);
+        add_variant<real, N, E, T, S, 1, 1>(out, n);
+    }
+}
+
+// Default contiguous / strided shapes and the single-column fallback.
+template <typename real, int N>
+inline void add_defaults(KEntry* out, int* n) {
+    constexpr bool dbl = sizeof(real) == 8;
+    add_shape<real, N, cont_E(N), cont_T(N), false>(out, n);
+    add_shape<real, N, str_E(N, dbl), str_T(N, dbl), true>(out, n);
+    add_variant<real, N, rmax_of(N), 1, true, 0, 0>(out, n);
 }
 
 }  // namespace a2a

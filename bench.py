@@ -80,6 +80,7 @@ def main():
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
     ap.add_argument("--ng", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--autotune", action="store_true", help="time every kernel variant per pass and keep the fastest")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has one GPU and the multi-GPU exchange is "
@@ -105,6 +106,12 @@ def main():
     dist = a2afft.Distribution(Ng, P, dims=dims, dtype=dtype)
     plan = a2afft.Dfft(dist, precision=args.precision, exec="fused")
     plan.make_plans(None)
+    if args.autotune:
+        log("[bench] autotuning kernel variants")
+        work = x.clone()
+        plan.autotune(work, iters=5)
+        del work
+    log(plan.describe())
 
     # ---- accuracy vs the FP64 path on the same (already rounded) input -----
     log("[bench] accuracy check vs FP64")
@@ -138,6 +145,20 @@ def main():
     ms_per_step = wall * 1e3 / args.steps
     pass_ms = [sum(ev[s][p].elapsed_time(ev[s][p + 1]) for s in range(args.steps)) / args.steps for p in range(3)]
     ev_ms = sum(ev[s][0].elapsed_time(ev[s][3]) for s in range(args.steps)) / args.steps
+
+    # measured device-copy bandwidth over the same buffer (the library's copy
+    # kernel, fast_copy_fft): the practical ceiling of one read + one write pass
+    copy_dst = torch.empty_like(data)
+    for _ in range(3):
+        a2afft.fast_copy_fft(copy_dst, data)
+    c0, c1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    c0.record()
+    for _ in range(10):
+        a2afft.fast_copy_fft(copy_dst, data)
+    c1.record()
+    torch.cuda.synchronize()
+    copy_ms = c0.elapsed_time(c1) / 10
+    del copy_dst
 
     bytes_pass = 2 * N * esize
     names = ["z", "x", "y"]
@@ -179,11 +200,14 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"{Ng}^3 c2c forward 3D FFT, {'complex64' if dtype == 'c64' else 'complex128'}, "
                                f"16 virtual ranks dims [1,4,4], GRF P(k)~k^-3 seed {seed} + 1e-3 noise",
-                   "precision": args.precision, "exec": "fused", "Ng": Ng, "ranks": P, "dims": list(dims)},
+                   "precision": args.precision, "exec": "fused", "Ng": Ng, "ranks": P, "dims": list(dims),
+                   "autotune": bool(args.autotune)},
         "gflops": round(gflops, 1),
         "hbm_gbs_whole": round(whole_gbs, 1),
         "roofline_frac_whole": round(whole_gbs / HBM_PEAK_GBS, 4),
         "pass_ms": {names[p]: round(pass_ms[p], 4) for p in range(3)},
+        "pass_gbs": {names[p]: round(bytes_pass / (pass_ms[p] * 1e-3) / 1e9, 1) for p in range(3)},
+        "copy_gbs": round(bytes_pass / (copy_ms * 1e-3) / 1e9, 1),
         "event_ms_per_step": round(ev_ms, 4),
         "rel_l2_vs_fp64": rel_l2,
         "max_abs_vs_fp64": max_abs,

@@ -191,6 +191,17 @@ int a2afft_error_metrics(const void* y, a2afft_dtype dtype, const void* ref, int
  * per-kernel timing; the three passes in order equal one transform. */
 int a2afft_execute_pass(a2afft_plan plan, void* d_data, int pass, int direction, void* stream);
 
+/* Kernel variants (no reference counterpart: cuFFT chose its own kernels).
+ * Every variant of a pass computes identical bits; they differ in tile width
+ * T, elements per thread E, XCD-aware tile order and LDS padding.
+ * autotune times each built variant that fits each fused pass, `iters`
+ * launches in place on `work` (a [P][nlocal] buffer whose contents are
+ * destroyed), and keeps the fastest; set_kernel picks one by hand. */
+int a2afft_plan_autotune(a2afft_plan plan, void* work, int iters, void* stream);
+int a2afft_plan_set_kernel(a2afft_plan plan, int pass, int T, int E, int xcd, int pad);
+/* Writes the chosen variant per pass and the last autotune timings as text. */
+int a2afft_plan_describe(a2afft_plan plan, char* buf, int len);
+
 /* Seeded synthetic input (no reference counterpart: the reference's test
  * fills with an unseeded rand(), tests/test.cpp:12-17).  Philox4x32-10 with
  * key (seed lo, seed hi) and counter (g lo, g hi, kind, 0) for global index

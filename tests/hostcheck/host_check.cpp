@@ -36,9 +36,10 @@ int hc_remap_dest(int kind, int P, int64_t nlocal, const int* lgs, const int* di
 // Enumerates (address, n, line) for every element an FFT launch loads and
 // stores.  pass 0..2: the fused passes of a (Ng, P, dims) decomposition;
 // pass -1: forward_1d_fft over P contiguous pencils of length Ng.
-// shape_out: variant, T, E, NT, grid.  Returns the count, or a negative status.
-int64_t hc_fft_enumerate(int Ng, int P, const int* dims, int dbl, int pass, int64_t* addr, int32_t* n_out,
-                         int64_t* line_out, int64_t cap, int64_t* shape_out) {
+// T = E = 0 picks the default shape; xcd = 1 applies the XCD renumbering.
+// shape_out: T, E, NT, grid.  Returns the count, or a negative status.
+int64_t hc_fft_enumerate(int Ng, int P, const int* dims, int dbl, int pass, int T, int E, int xcd, int64_t* addr,
+                         int32_t* n_out, int64_t* line_out, int64_t cap, int64_t* shape_out) {
     PassMaps pm;
     if (pass < 0) {
         pm = batched_pass_maps(Ng, uint32_t(P));
@@ -49,35 +50,59 @@ int64_t hc_fft_enumerate(int Ng, int P, const int* dims, int dbl, int pass, int6
         fused_pass_maps(Ng, dims, lgs, nlocal, all);
         pm = all[pass];
     }
-    Shape sh;
+    if (T == 0) {
+        KSel d = default_sel(Ng, dbl != 0, pm.strided, pm.fft, pm.a2);
+        T = d.T;
+        E = d.E;
+    }
     FftGeom g;
     uint32_t grid;
-    int st = resolve_pass(pm.fft, pm.a1, pm.a2, pm.strided, Ng, dbl != 0, &sh, &g, &grid);
+    int st = resolve_with(pm.fft, pm.a1, pm.a2, pm.strided, Ng, T, E, &g, &grid);
     if (st < 0) return st;
-    shape_out[0] = sh.variant;
-    shape_out[1] = sh.T;
-    shape_out[2] = sh.E;
-    shape_out[3] = sh.NT;
-    shape_out[4] = grid;
-    const bool strided = sh.variant != 0;
-    const int TPL = Ng / sh.E;
+    const int NT = T * (Ng / E);
+    shape_out[0] = T;
+    shape_out[1] = E;
+    shape_out[2] = NT;
+    shape_out[3] = grid;
+    const bool strided = pm.strided != 0;
+    const int TPL = Ng / E;
     int64_t c = 0;
-    for (uint32_t b = 0; b < grid; ++b)
-        for (int tid = 0; tid < sh.NT; ++tid) {
+    for (uint32_t hb = 0; hb < grid; ++hb) {
+        const uint32_t b = xcd ? xcd_block(hb, grid) : hb;
+        for (int tid = 0; tid < NT; ++tid) {
             bool valid;
-            const int64_t base = line_base(g, strided, sh.T, TPL, b, tid, &valid);
+            const int64_t base = line_base(g, strided, T, TPL, b, tid, &valid);
             if (!valid) continue;
-            const int t = strided ? tid / sh.T : tid % TPL;
-            const int line = strided ? tid % sh.T : tid / TPL;
-            for (int e = 0; e < sh.E; ++e) {
+            const int t = strided ? tid / T : tid % TPL;
+            const int line = strided ? tid % T : tid / TPL;
+            for (int e = 0; e < E; ++e) {
                 if (c >= cap) return -100;
                 addr[c] = base + elem_off(g, e * TPL);
                 n_out[c] = t + e * TPL;
-                line_out[c] = int64_t(b) * sh.T + line;
+                line_out[c] = int64_t(b) * T + line;
                 ++c;
             }
         }
+    }
     return c;
+}
+
+int hc_shape_fits(int Ng, int P, const int* dims, int pass, int T, int E) {
+    PassMaps pm;
+    if (pass < 0) {
+        pm = batched_pass_maps(Ng, uint32_t(P));
+    } else {
+        int lgs[3] = {Ng / dims[0], Ng / dims[1], Ng / dims[2]};
+        int64_t nlocal = int64_t(lgs[0]) * lgs[1] * lgs[2];
+        PassMaps all[3];
+        fused_pass_maps(Ng, dims, lgs, nlocal, all);
+        pm = all[pass];
+    }
+    return shape_fits(pm.fft, pm.a2, pm.strided, Ng, T, E) ? 1 : 0;
+}
+
+void hc_xcd_blocks(uint32_t G, uint32_t* out) {
+    for (uint32_t b = 0; b < G; ++b) out[b] = xcd_block(b, G);
 }
 
 }  // extern "C"

@@ -30,9 +30,11 @@ def hc(tmp_path_factory):
     P = ctypes.c_void_p
     L.hc_fastdiv.argtypes = [ctypes.c_uint32, P, P, ctypes.c_int64]
     L.hc_remap_dest.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int64, P, P, P]
-    L.hc_fft_enumerate.argtypes = [ctypes.c_int, ctypes.c_int, P, ctypes.c_int, ctypes.c_int, P, P, P,
-                                   ctypes.c_int64, P]
+    L.hc_fft_enumerate.argtypes = [ctypes.c_int, ctypes.c_int, P, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                   ctypes.c_int, ctypes.c_int, P, P, P, ctypes.c_int64, P]
     L.hc_fft_enumerate.restype = ctypes.c_int64
+    L.hc_shape_fits.argtypes = [ctypes.c_int, ctypes.c_int, P, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    L.hc_xcd_blocks.argtypes = [ctypes.c_uint32, P]
     return L
 
 
@@ -82,14 +84,14 @@ FFT_CONFIGS = [
 ]
 
 
-def _enumerate(hc, Ng, P, dims, dbl, pass_):
+def _enumerate(hc, Ng, P, dims, dbl, pass_, T=0, E=0, xcd=0):
     cap = (P * (Ng // dims[0]) * (Ng // dims[1]) * (Ng // dims[2])) if pass_ >= 0 else P * Ng
     addr = np.empty(cap, dtype=np.int64)
     n = np.empty(cap, dtype=np.int32)
     line = np.empty(cap, dtype=np.int64)
-    shape = np.zeros(5, dtype=np.int64)
+    shape = np.zeros(4, dtype=np.int64)
     dm = np.array(dims, dtype=np.int32)
-    c = hc.hc_fft_enumerate(Ng, P, _p(dm), int(dbl), pass_, _p(addr), _p(n), _p(line), cap, _p(shape))
+    c = hc.hc_fft_enumerate(Ng, P, _p(dm), int(dbl), pass_, T, E, xcd, _p(addr), _p(n), _p(line), cap, _p(shape))
     assert c >= 0, c
     return addr[:c], n[:c], line[:c], shape
 
@@ -127,3 +129,46 @@ def test_batched_pencil_addresses(hc, N, lines):
         assert np.array_equal(np.sort(addr), np.arange(N * lines))
         assert np.array_equal(addr % N, n)
         assert np.array_equal(addr // N, line)
+
+
+def test_xcd_renumbering_is_a_bijection(hc):
+    for G in list(range(1, 300)) + [4095, 4096, 4097, 16384, 32768, 100003]:
+        out = np.empty(G, dtype=np.uint32)
+        hc.hc_xcd_blocks(G, _p(out))
+        assert np.array_equal(np.sort(out), np.arange(G)), G
+        if G >= 16:
+            # each XCD (b % 8) receives a contiguous logical range
+            for x in range(8):
+                r = out[x::8].astype(np.int64)
+                assert np.all(np.diff(r) == 1), (G, x)
+
+
+SHAPE_CONFIGS = [(64, 16, (1, 4, 4)), (32, 8, (1, 2, 4)), (16, 4, (2, 2, 1)), (128, 16, (1, 4, 4))]
+
+
+@pytest.mark.parametrize("cfg", SHAPE_CONFIGS, ids=lambda c: f"Ng{c[0]}_P{c[1]}_{c[2]}")
+def test_every_fitting_shape_addresses(hc, cfg):
+    """Every (T, E) the dispatcher accepts, with and without XCD renumbering:
+    each element exactly once, each line one global line in natural order."""
+    Ng, P, dims = cfg
+    rc, d = O.decompose(Ng, P, dims)
+    total = P * d.nlocal
+    gid = O.global_to_blocks(d, np.arange(Ng ** 3, dtype=np.int64).reshape((Ng,) * 3)).reshape(-1)
+    dm = np.array(dims, dtype=np.int32)
+    checked = 0
+    for pass_, axis in ((0, 2), (1, 0), (2, 1)):
+        for T in (1, 2, 4, 8, 16, 32, 64, 128):
+            for E in (2, 4, 8, 16, 32, 64):
+                if not hc.hc_shape_fits(Ng, P, _p(dm), pass_, T, E):
+                    continue
+                for xcd in (0, 1):
+                    addr, n, line, shape = _enumerate(hc, Ng, P, dims, 0, pass_, T, E, xcd)
+                    assert np.array_equal(np.sort(addr), np.arange(total)), (pass_, T, E, xcd)
+                    coords = np.stack(np.unravel_index(gid[addr], (Ng,) * 3))
+                    assert np.array_equal(coords[axis], n)
+                    other = [a for a in range(3) if a != axis]
+                    key = coords[other[0]] * Ng + coords[other[1]]
+                    k_sorted = key[np.argsort(line, kind="stable")].reshape(-1, Ng)
+                    assert (k_sorted == k_sorted[:, :1]).all()
+                    checked += 1
+    assert checked > 10
