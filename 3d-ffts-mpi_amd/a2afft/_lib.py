@@ -87,6 +87,9 @@ SIGNATURES = {
     "a2afft_plan_autotune": [_P, _P, _I, _P],
     "a2afft_plan_set_kernel": [_P, _I, _I, _I, _I, _I],
     "a2afft_plan_describe": [_P, ctypes.c_char_p, _I],
+    "a2afft_plan_set_chunking": [_P, _I],
+    "a2afft_plan_num_launches": [_P],
+    "a2afft_execute_launch": [_P, _P, _I, _I, _P],
     "a2afft_fill_random": [_P, _I, _I64, ctypes.c_uint64, _I, _I64, _P],
     "a2afft_error_metrics": [_P, _I, _P, _I64, ctypes.POINTER(ctypes.c_double),
                              ctypes.POINTER(ctypes.c_double), _P],

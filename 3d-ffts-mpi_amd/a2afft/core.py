@@ -158,6 +158,18 @@ class Dfft:
     def set_kernel(self, pass_index, T, E, xcd=0, pad=0):
         return L.check(L.lib().a2afft_plan_set_kernel(self._h, pass_index, T, E, xcd, pad), "set_kernel")
 
+    def set_chunking(self, z_columns):
+        """x/y passes alternate over windows of z_columns z columns (0 = whole passes)."""
+        return L.check(L.lib().a2afft_plan_set_chunking(self._h, z_columns), "set_chunking")
+
+    def num_launches(self):
+        return L.check(L.lib().a2afft_plan_num_launches(self._h), "num_launches")
+
+    def execute_launch(self, data, index, direction=-1):
+        """Launch `index` of a fused transform; returns its pass (0 z, 1 x, 2 y)."""
+        return L.check(L.lib().a2afft_execute_launch(self._h, _ptr(data), index, direction, _stream()),
+                       "execute_launch")
+
     def describe(self):
         buf = ctypes.create_string_buffer(8192)
         L.check(L.lib().a2afft_plan_describe(self._h, buf, len(buf)), "describe")

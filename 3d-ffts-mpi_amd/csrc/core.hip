@@ -262,8 +262,44 @@ struct a2afft_plan_s {
     void* scratch;
     AxisPass pass[3];   // fused: z, x, y over the block layout
     KSel sel[3];        // kernel variant per fused pass
+    uint32_t chunk;     // z columns per x/y chunk (0: whole passes)
     char report[4096];  // last autotune table
 };
+
+// One kernel launch of a fused transform: the z pass, then the x and y
+// passes either whole or alternating over windows of `chunk` z columns, so a
+// window's data is still in the 256 MB Infinity Cache when the y pass reads
+// what the x pass wrote.  Every window holds complete x and y lines, so the
+// arithmetic per line -- and the result -- is the same for every chunking.
+struct Launch {
+    int pass;
+    uint32_t start, count;
+};
+
+static int plan_launches(const a2afft_plan_s* p, Launch* out, int max) {
+    int n = 0;
+    out[n++] = Launch{0, 0, 0};
+    if (!p->chunk) {
+        out[n++] = Launch{1, 0, 0};
+        out[n++] = Launch{2, 0, 0};
+        return n;
+    }
+    const uint32_t Ng = uint32_t(p->d.Ng);
+    for (uint32_t z0 = 0; z0 < Ng && n + 2 <= max; z0 += p->chunk) {
+        const uint32_t cnt = Ng - z0 < p->chunk ? Ng - z0 : p->chunk;
+        out[n++] = Launch{1, z0, cnt};
+        out[n++] = Launch{2, z0, cnt};
+    }
+    return n;
+}
+
+static bool chunk_ok(const a2afft_plan_s* p, uint32_t chunk) {
+    if (chunk == 0) return true;
+    if (p->d.Ng % int(chunk)) return false;
+    for (int i = 1; i < 3; ++i)
+        if (p->sel[i].T > 0 && chunk % uint32_t(p->sel[i].T)) return false;
+    return true;
+}
 
 extern "C" int a2afft_plan_create(a2afft_plan* out, a2afft_dist dist, a2afft_precision prec, a2afft_exec exec) {
     if (!out || !dist) return set_error(A2AFFT_E_INVALID_ARG, "plan_create: null");
@@ -290,6 +326,7 @@ extern "C" int a2afft_plan_create(a2afft_plan* out, a2afft_dist dist, a2afft_pre
     fused_pass_maps(dist->Ng, dist->dims, dist->lgs, dist->nlocal, pm);
     for (int i = 0; i < 3; ++i) p->pass[i] = AxisPass{pm[i].fft, pm[i].a1, pm[i].a2, pm[i].strided, 0, 1.0};
     p->report[0] = 0;
+    p->chunk = 0;
     FftShape sh;
     for (int i = 0; i < 3; ++i) {
         p->sel[i] = fft_default_sel(p->pass[i], dist->dtype, dist->Ng);
@@ -343,18 +380,23 @@ static int exec_staged(a2afft_plan_s* p, void* D, int dir, hipStream_t s) {
     return warn;
 }
 
-// Three in-place passes over the block layout: z, x, y (the reference's axis
-// order), with 1/Ng^3 folded into the last inverse pass.
-static int exec_fused(a2afft_plan_s* p, void* D, int dir, hipStream_t s) {
-    const a2afft_dist_s* d = &p->d;
-    for (int i = 0; i < 3; ++i) {
-        AxisPass ps = p->pass[i];
-        if (i == 2 && dir > 0) {
-            ps.scale = double(int64_t(d->Ng) * d->Ng * d->Ng);
-            ps.scale_mode = 2;
-        }
-        A2A_TRY(launch_fft_pass(ps, D, d->dtype, d->Ng, dir, s, &p->sel[i]));
+static int run_launch(a2afft_plan_s* p, const Launch& L, void* D, int dir, hipStream_t s) {
+    AxisPass ps = p->pass[L.pass];
+    ps.a2_start = L.start;
+    ps.a2_count = L.count;
+    if (L.pass == 2 && dir > 0) {
+        ps.scale = double(int64_t(p->d.Ng) * p->d.Ng * p->d.Ng);
+        ps.scale_mode = 2;
     }
+    return launch_fft_pass(ps, D, p->d.dtype, p->d.Ng, dir, s, &p->sel[L.pass]);
+}
+
+// In-place passes over the block layout in the reference's axis order z, x,
+// y, with 1/Ng^3 folded into the last inverse pass.
+static int exec_fused(a2afft_plan_s* p, void* D, int dir, hipStream_t s) {
+    Launch L[1024];
+    const int n = plan_launches(p, L, int(sizeof(L) / sizeof(L[0])));
+    for (int i = 0; i < n; ++i) A2A_TRY(run_launch(p, L[i], D, dir, s));
     return A2AFFT_OK;
 }
 
@@ -376,13 +418,48 @@ extern "C" int a2afft_execute_pass(a2afft_plan p, void* data, int pass, int dire
     if (!p || !data || pass < 0 || pass > 2 || (direction != -1 && direction != 1))
         return set_error(A2AFFT_E_INVALID_ARG, "execute_pass: bad arguments");
     if (p->exec != A2AFFT_EXEC_FUSED) return set_error(A2AFFT_E_INVALID_ARG, "execute_pass: fused plans only");
-    AxisPass ps = p->pass[pass];
-    if (pass == 2 && direction > 0) {
-        ps.scale = double(int64_t(p->d.Ng) * p->d.Ng * p->d.Ng);
-        ps.scale_mode = 2;
-    }
-    return finish_call(stream,
-                       launch_fft_pass(ps, data, p->d.dtype, p->d.Ng, direction, (hipStream_t)stream, &p->sel[pass]));
+    Launch L[1024];
+    const int n = plan_launches(p, L, int(sizeof(L) / sizeof(L[0])));
+    for (int i = 0; i < n; ++i)
+        if (L[i].pass == pass) {
+            int st = run_launch(p, L[i], data, direction, (hipStream_t)stream);
+            if (st < 0) return st;
+        }
+    return finish_call(stream, A2AFFT_OK);
+}
+
+extern "C" int a2afft_plan_num_launches(a2afft_plan p) {
+    if (!p) return set_error(A2AFFT_E_INVALID_ARG, "num_launches: null plan");
+    if (p->exec != A2AFFT_EXEC_FUSED) return set_error(A2AFFT_E_INVALID_ARG, "num_launches: fused plans only");
+    Launch L[1024];
+    return plan_launches(p, L, int(sizeof(L) / sizeof(L[0])));
+}
+
+// Launch `index` of a fused forward / inverse; returns the pass it belongs
+// to (0 z, 1 x, 2 y) or a negative status.
+extern "C" int a2afft_execute_launch(a2afft_plan p, void* data, int index, int direction, void* stream) {
+    if (!p || !data || (direction != -1 && direction != 1))
+        return set_error(A2AFFT_E_INVALID_ARG, "execute_launch: bad arguments");
+    if (p->exec != A2AFFT_EXEC_FUSED) return set_error(A2AFFT_E_INVALID_ARG, "execute_launch: fused plans only");
+    Launch L[1024];
+    const int n = plan_launches(p, L, int(sizeof(L) / sizeof(L[0])));
+    if (index < 0 || index >= n) return set_error(A2AFFT_E_INVALID_ARG, "execute_launch: index out of range");
+    int st = run_launch(p, L[index], data, direction, (hipStream_t)stream);
+    if (st < 0) return st;
+    st = finish_call(stream, A2AFFT_OK);
+    return st < 0 ? st : L[index].pass;
+}
+
+extern "C" int a2afft_plan_set_chunking(a2afft_plan p, int z_columns) {
+    if (!p || z_columns < 0) return set_error(A2AFFT_E_INVALID_ARG, "set_chunking: bad arguments");
+    if (p->exec != A2AFFT_EXEC_FUSED) return set_error(A2AFFT_E_INVALID_ARG, "set_chunking: fused plans only");
+    if (z_columns >= p->d.Ng) z_columns = 0;
+    if (z_columns > 0 && z_columns < 8) return set_error(A2AFFT_E_INVALID_ARG, "set_chunking: at least 8 columns");
+    if (!chunk_ok(p, uint32_t(z_columns)))
+        return set_error(A2AFFT_E_INVALID_ARG, "set_chunking: %d columns must divide Ng and be a multiple of the "
+                         "x and y tile widths", z_columns);
+    p->chunk = uint32_t(z_columns);
+    return A2AFFT_OK;
 }
 
 extern "C" int a2afft_plan_set_kernel(a2afft_plan p, int pass, int T, int E, int xcd, int pad) {
@@ -392,7 +469,13 @@ extern "C" int a2afft_plan_set_kernel(a2afft_plan p, int pass, int T, int E, int
     const int n = fft_candidates(p->pass[pass], p->d.dtype, p->d.Ng, cand, 256);
     for (int i = 0; i < n; ++i)
         if (cand[i].T == s.T && cand[i].E == s.E && cand[i].xcd == s.xcd && cand[i].pad == s.pad) {
+            const KSel old = p->sel[pass];
             p->sel[pass] = s;
+            if (!chunk_ok(p, p->chunk)) {
+                p->sel[pass] = old;
+                return set_error(A2AFFT_E_INVALID_ARG, "set_kernel: T=%d does not divide the chunk width %u", T,
+                                 p->chunk);
+            }
             return A2AFFT_OK;
         }
     return set_error(A2AFFT_E_UNSUPPORTED, "set_kernel: no built variant T=%d E=%d xcd=%d pad=%d fits pass %d", T, E,
@@ -432,7 +515,7 @@ extern "C" int a2afft_plan_autotune(a2afft_plan p, void* work, int iters, void* 
             if (st >= 0) st = hip_check(int(hipEventElapsedTime(&ms, e0, e1)), "autotune elapsed");
             if (st < 0) break;
             ms /= float(iters);
-            if (ms < best) {
+            if (ms < best && (i == 0 || p->chunk == 0 || p->chunk % uint32_t(cand[c].T) == 0)) {
                 best = ms;
                 p->sel[i] = cand[c];
             }
@@ -441,6 +524,35 @@ extern "C" int a2afft_plan_autotune(a2afft_plan p, void* work, int iters, void* 
                                 "pass %s T=%d E=%d NT=%d xcd=%d pad=%d %.4f ms\n", names[i], cand[c].T, cand[c].E,
                                 cand[c].T * (p->d.Ng / cand[c].E), cand[c].xcd, cand[c].pad, ms);
         }
+    }
+    // chunk widths: time whole forwards
+    if (st >= 0) {
+        uint32_t cands[16];
+        int nc = 0;
+        cands[nc++] = 0;
+        for (uint32_t w = 16; w < uint32_t(p->d.Ng) && nc < 16; w *= 2)
+            if (chunk_ok(p, w)) cands[nc++] = w;
+        const uint32_t keep = p->chunk;
+        float best = 1e30f;
+        uint32_t best_w = keep;
+        for (int k = 0; k < nc && st >= 0; ++k) {
+            p->chunk = cands[k];
+            st = exec_fused(p, work, -1, s);
+            if (st < 0) break;
+            st = hip_check(int(hipEventRecord(e0, s)), "autotune record");
+            for (int it = 0; it < iters && st >= 0; ++it) st = exec_fused(p, work, -1, s);
+            if (st < 0) break;
+            st = hip_check(int(hipEventRecord(e1, s)), "autotune record");
+            if (st >= 0) st = hip_check(int(hipEventSynchronize(e1)), "autotune sync");
+            float ms = 0.f;
+            if (st >= 0) st = hip_check(int(hipEventElapsedTime(&ms, e0, e1)), "autotune elapsed");
+            if (st < 0) break;
+            ms /= float(iters);
+            if (ms < best) { best = ms; best_w = cands[k]; }
+            if (len < int(sizeof(p->report)) - 96)
+                len += snprintf(p->report + len, sizeof(p->report) - len, "forward chunk=%u %.4f ms\n", cands[k], ms);
+        }
+        p->chunk = best_w;
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -458,6 +570,7 @@ extern "C" int a2afft_plan_describe(a2afft_plan p, char* buf, int len) {
         n += snprintf(buf + n, len - n, "chosen %s T=%d E=%d NT=%d xcd=%d pad=%d grid=%d\n", names[i], sh.T, sh.E,
                       sh.NT, sh.xcd, sh.pad, sh.grid);
     }
+    if (n < len) n += snprintf(buf + n, len - n, "chunk %u z columns\n", p->chunk);
     if (n < len) snprintf(buf + n, len - n, "%s", p->report);
     return A2AFFT_OK;
 }

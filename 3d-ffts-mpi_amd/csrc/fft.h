@@ -21,6 +21,8 @@ struct AxisPass {
     int strided;
     int scale_mode;   // 0 none, 1 multiply, 2 divide
     double scale;
+    uint32_t a2_start = 0;   // window along a2 (count 0 = whole axis)
+    uint32_t a2_count = 0;
 };
 
 // Supported transform lengths: powers of two 2..4096.

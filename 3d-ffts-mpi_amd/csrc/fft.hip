@@ -56,12 +56,38 @@ int fft_candidates(const AxisPass& p, a2afft_dtype dt, int N, KSel* out, int max
     return n;
 }
 
+// Measured preferences (MI355X, 512^3 over dims [1,4,4]): XCD-aware order
+// everywhere; complex64 strided passes prefer 256-byte rows (T=32).  The
+// first entry that is built and fits the geometry wins; otherwise the
+// generic default with XCD order.
+struct Pref {
+    int N, dbl, strided;
+    KSel s;
+};
+static const Pref kPrefs[] = {
+    {512, 0, 0, {16, 16, 1, 0}},
+    {512, 0, 1, {32, 32, 1, 1}},
+    {512, 0, 1, {16, 32, 1, 1}},
+    {512, 1, 0, {8, 16, 1, 1}},
+    {512, 1, 1, {8, 16, 1, 1}},
+    {256, 0, 1, {32, 32, 1, 1}},
+    {256, 0, 1, {16, 32, 1, 1}},
+};
+
 KSel fft_default_sel(const AxisPass& p, a2afft_dtype dt, int N) {
-    return default_sel(N, dt == A2AFFT_C128, p.strided, p.fft, p.a2);
+    const bool dbl = dt == A2AFFT_C128;
+    for (const Pref& pr : kPrefs) {
+        if (pr.N != N || pr.dbl != int(dbl) || pr.strided != p.strided) continue;
+        if (shape_fits(p.fft, p.a2, p.strided, N, pr.s.T, pr.s.E) && find_kernel(N, dbl, p.strided, pr.s, -1))
+            return pr.s;
+    }
+    KSel s = default_sel(N, dbl, p.strided, p.fft, p.a2);
+    if (npass_of(N) > 1 && s.T > 1) s.xcd = 1;
+    return s;
 }
 
 static int resolve(const AxisPass& p, a2afft_dtype dt, int N, const KSel& s, FftGeom* g, uint32_t* grid) {
-    int st = resolve_with(p.fft, p.a1, p.a2, p.strided, N, s.T, s.E, g, grid);
+    int st = resolve_with(p.fft, p.a1, p.a2, p.strided, N, s.T, s.E, g, grid, p.a2_start, p.a2_count);
     if (st == A2AFFT_E_UNSUPPORTED)
         return set_error(st, "fft pass: length %d (block extent %u, T=%d, E=%d) not supported", N, p.fft.l, s.T, s.E);
     if (st < 0) return set_error(st, "fft pass: bad geometry (N=%d)", N);

@@ -48,6 +48,7 @@ struct FftGeom {
     AxisMap a1;         // outer enumeration axis of lines / tiles
     AxisMap a2;         // inner enumeration axis (z columns for strided tiles)
     uint32_t n2;        // units along a2 (lines, or tiles of T columns)
+    uint32_t a2_start;  // first a2 coordinate of this launch's window
     uint32_t nunits;    // total lines (contiguous) or tiles (strided)
     FastDiv dn2;
     int64_t tstride;    // strided tiles: element step between the T columns
@@ -68,14 +69,14 @@ A2A_HD int64_t line_base(const FftGeom& g, bool strided, int T, int TPL, uint32_
     if (strided) {
         const uint32_t u1 = g.dn2.div(block);
         const uint32_t zt = block - u1 * g.n2;
-        base = g.a1.off(u1) + g.a2.off(zt * uint32_t(T)) + int64_t(line) * g.tstride;
+        base = g.a1.off(u1) + g.a2.off(g.a2_start + zt * uint32_t(T)) + int64_t(line) * g.tstride;
     } else {
         const uint32_t ln = block * uint32_t(T) + uint32_t(line);
         *valid = ln < g.nunits;
         const uint32_t lv = *valid ? ln : 0u;
         const uint32_t u1 = g.dn2.div(lv);
         const uint32_t u2 = lv - u1 * g.n2;
-        base = g.a1.off(u1) + g.a2.off(u2);
+        base = g.a1.off(u1) + g.a2.off(g.a2_start + u2);
     }
     // off(n) = off(e*TPL) + off(t) for n = t + e*TPL: the dispatcher only
     // launches when the axis block extent and TPL divide one another.
@@ -136,11 +137,17 @@ inline KSel default_sel(int N, bool dbl, int strided, const AxisMap& fft, const 
 }
 
 // Final geometry of a pass for a chosen shape; returns 0 or a negative status.
+// The launch covers a2 coordinates [a2_start, a2_start + a2_count) (count 0 =
+// the whole axis); a strided window must start and end on tile boundaries.
 inline int resolve_with(const AxisMap& fft, const AxisMap& a1, const AxisMap& a2, int strided, int N, int T, int E,
-                        FftGeom* g, uint32_t* grid) {
+                        FftGeom* g, uint32_t* grid, uint32_t a2_start = 0, uint32_t a2_count = 0) {
     if (!fft_len_ok(N)) return A2AFFT_E_UNSUPPORTED;
     if (fft.n != uint32_t(N)) return A2AFFT_E_INVALID_ARG;
     if (!shape_fits(fft, a2, strided, N, T, E)) return A2AFFT_E_UNSUPPORTED;
+    if (a2_count == 0) a2_count = a2.n - a2_start;
+    if (uint64_t(a2_start) + a2_count > a2.n) return A2AFFT_E_INVALID_ARG;
+    if (strided && (a2_start % uint32_t(T) || a2_count % uint32_t(T))) return A2AFFT_E_INVALID_ARG;
+    g->a2_start = a2_start;
     g->fft = fft;
     g->a1 = a1;
     g->a2 = a2;
@@ -149,11 +156,11 @@ inline int resolve_with(const AxisMap& fft, const AxisMap& a1, const AxisMap& a2
     g->tstride = a2.s_elem;
     uint64_t units, blocks;
     if (!strided) {
-        g->n2 = a2.n;
-        units = uint64_t(a1.n) * a2.n;
+        g->n2 = a2_count;
+        units = uint64_t(a1.n) * a2_count;
         blocks = (units + T - 1) / T;
     } else {
-        g->n2 = a2.n / uint32_t(T);
+        g->n2 = a2_count / uint32_t(T);
         units = uint64_t(a1.n) * g->n2;
         blocks = units;
     }

@@ -81,6 +81,7 @@ def main():
     ap.add_argument("--ng", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--autotune", action="store_true", help="time every kernel variant per pass and keep the fastest")
+    ap.add_argument("--chunk", type=int, default=-1, help="z columns per x/y chunk (-1: plan default / autotuned)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has one GPU and the multi-GPU exchange is "
@@ -106,6 +107,8 @@ def main():
     dist = a2afft.Distribution(Ng, P, dims=dims, dtype=dtype)
     plan = a2afft.Dfft(dist, precision=args.precision, exec="fused")
     plan.make_plans(None)
+    if args.chunk >= 0:
+        plan.set_chunking(args.chunk)
     if args.autotune:
         log("[bench] autotuning kernel variants")
         work = x.clone()
@@ -132,19 +135,30 @@ def main():
     for _ in range(args.warmup):
         plan.forward(data)
     torch.cuda.synchronize()
-    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(args.steps)]
+    # events between consecutive launches on the launch stream give each
+    # kernel's duration; a step is one whole forward (all launches)
+    nl = plan.num_launches()
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(nl + 1)] for _ in range(args.steps)]
+    kinds = [0] * nl
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for s in range(args.steps):
         ev[s][0].record()
-        for p in range(3):
-            a2afft.execute_pass(plan, data, p, -1)
-            ev[s][p + 1].record()
+        for i in range(nl):
+            kinds[i] = plan.execute_launch(data, i, -1)
+            ev[s][i + 1].record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms_per_step = wall * 1e3 / args.steps
-    pass_ms = [sum(ev[s][p].elapsed_time(ev[s][p + 1]) for s in range(args.steps)) / args.steps for p in range(3)]
-    ev_ms = sum(ev[s][0].elapsed_time(ev[s][3]) for s in range(args.steps)) / args.steps
+    pass_ms = [0.0, 0.0, 0.0]
+    launch_ms = [0.0, 0.0, 0.0]
+    nper = [kinds.count(p) for p in range(3)]
+    for s in range(args.steps):
+        for i in range(nl):
+            pass_ms[kinds[i]] += ev[s][i].elapsed_time(ev[s][i + 1]) / args.steps
+    for p in range(3):
+        launch_ms[p] = pass_ms[p] / max(1, nper[p])
+    ev_ms = sum(ev[s][0].elapsed_time(ev[s][nl]) for s in range(args.steps)) / args.steps
 
     # measured device-copy bandwidth over the same buffer (the library's copy
     # kernel, fast_copy_fft): the practical ceiling of one read + one write pass
@@ -163,7 +177,10 @@ def main():
     bytes_pass = 2 * N * esize
     names = ["z", "x", "y"]
     dom = max(range(3), key=lambda p: pass_ms[p])
-    achieved = bytes_pass / (pass_ms[dom] * 1e-3) / 1e9
+    # algorithmic bytes of one launch of the dominant kernel: the launch's
+    # share of one read + one write of the grid
+    bytes_launch = bytes_pass / max(1, nper[dom])
+    achieved = bytes_launch / (launch_ms[dom] * 1e-3) / 1e9
     gflops = 5.0 * N * math.log2(N) / (ms_per_step * 1e-3) / 1e9
     whole_gbs = 3 * bytes_pass / (ms_per_step * 1e-3) / 1e9
 
@@ -207,11 +224,13 @@ def main():
         "roofline_frac_whole": round(whole_gbs / HBM_PEAK_GBS, 4),
         "pass_ms": {names[p]: round(pass_ms[p], 4) for p in range(3)},
         "pass_gbs": {names[p]: round(bytes_pass / (pass_ms[p] * 1e-3) / 1e9, 1) for p in range(3)},
+        "launches": {names[p]: nper[p] for p in range(3)},
         "copy_gbs": round(bytes_pass / (copy_ms * 1e-3) / 1e9, 1),
         "event_ms_per_step": round(ev_ms, 4),
         "rel_l2_vs_fp64": rel_l2,
         "max_abs_vs_fp64": max_abs,
-        "roofline": {"bound": "hbm", "kernel": f"stockham {names[dom]} pass", "achieved": round(achieved, 1),
+        "roofline": {"bound": "hbm", "kernel": f"stockham {names[dom]} pass", "bytes_per_launch": int(bytes_launch),
+                     "launch_ms": round(launch_ms[dom], 4), "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic},
         "cpu_baseline": cpu,

@@ -199,6 +199,16 @@ int a2afft_execute_pass(a2afft_plan plan, void* d_data, int pass, int direction,
  * destroyed), and keeps the fastest; set_kernel picks one by hand. */
 int a2afft_plan_autotune(a2afft_plan plan, void* work, int iters, void* stream);
 int a2afft_plan_set_kernel(a2afft_plan plan, int pass, int T, int E, int xcd, int pad);
+/* Chunked execution: after the z pass, the x and y passes alternate over
+ * windows of `z_columns` z columns (0 = whole passes), so each window is
+ * re-read from the Infinity Cache.  Results are identical for every width;
+ * the width must divide Ng and be a multiple of both strided tile widths.
+ * autotune also times whole forwards over the valid widths. */
+int a2afft_plan_set_chunking(a2afft_plan plan, int z_columns);
+/* Launches in one fused forward / inverse, and launch `index` alone (returns
+ * its pass: 0 z, 1 x, 2 y), for per-kernel timing. */
+int a2afft_plan_num_launches(a2afft_plan plan);
+int a2afft_execute_launch(a2afft_plan plan, void* d_data, int index, int direction, void* stream);
 /* Writes the chosen variant per pass and the last autotune timings as text. */
 int a2afft_plan_describe(a2afft_plan plan, char* buf, int len);
 

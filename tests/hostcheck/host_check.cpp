@@ -39,7 +39,8 @@ int hc_remap_dest(int kind, int P, int64_t nlocal, const int* lgs, const int* di
 // T = E = 0 picks the default shape; xcd = 1 applies the XCD renumbering.
 // shape_out: T, E, NT, grid.  Returns the count, or a negative status.
 int64_t hc_fft_enumerate(int Ng, int P, const int* dims, int dbl, int pass, int T, int E, int xcd, int64_t* addr,
-                         int32_t* n_out, int64_t* line_out, int64_t cap, int64_t* shape_out) {
+                         int32_t* n_out, int64_t* line_out, int64_t cap, int64_t* shape_out, uint32_t a2_start,
+                         uint32_t a2_count) {
     PassMaps pm;
     if (pass < 0) {
         pm = batched_pass_maps(Ng, uint32_t(P));
@@ -57,7 +58,7 @@ int64_t hc_fft_enumerate(int Ng, int P, const int* dims, int dbl, int pass, int 
     }
     FftGeom g;
     uint32_t grid;
-    int st = resolve_with(pm.fft, pm.a1, pm.a2, pm.strided, Ng, T, E, &g, &grid);
+    int st = resolve_with(pm.fft, pm.a1, pm.a2, pm.strided, Ng, T, E, &g, &grid, a2_start, a2_count);
     if (st < 0) return st;
     const int NT = T * (Ng / E);
     shape_out[0] = T;

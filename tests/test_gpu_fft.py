@@ -135,3 +135,47 @@ def test_3d_matches_numpy_fftn(a2afft_mod, cuda):
     plan.inverse(data)
     inv = inputs.blocks_to_global(data.cpu().numpy(), dims, g.shape)
     assert _relerr(inv, g) < 1e-15
+
+
+@pytest.mark.parametrize("cfg", [(64, 16, (1, 4, 4), "c64"), (128, 16, (1, 4, 4), "c64"), (64, 8, (1, 2, 4), "c128")],
+                         ids=lambda c: f"Ng{c[0]}_{c[2]}_{c[3]}")
+def test_chunked_and_tuned_plans_bit_identical(a2afft_mod, cuda, cfg):
+    """Chunk widths and kernel variants change launch order and shape only:
+    forward and inverse must equal the default plan bit for bit, and the staged
+    reference-order plan."""
+    import torch
+    Ng, P, dims, dtype = cfg
+    rc, d = O.decompose(Ng, P, dims)
+    x = _blocks(d, dtype, seed=3)
+    dist = a2afft_mod.Distribution(Ng, P, dims=dims, dtype=dtype)
+    base = a2afft_mod.Dfft(dist)
+    base.make_plans(None)
+    ref = torch.from_numpy(x).cuda()
+    base.forward(ref)
+    staged = a2afft_mod.Dfft(dist, exec="staged")
+    staged.make_plans(dist.empty())
+    st = torch.from_numpy(x).cuda()
+    staged.forward(st)
+    torch.cuda.synchronize()
+    assert np.array_equal(_bits(ref.cpu().numpy()), _bits(st.cpu().numpy()))
+    tuned = a2afft_mod.Dfft(dist)
+    tuned.make_plans(None)
+    tuned.autotune(dist.empty().copy_(torch.from_numpy(x)), iters=1)
+    for chunk in (0, 16, 32, Ng // 2):
+        try:
+            tuned.set_chunking(chunk)
+        except a2afft_mod.A2AFFTError:
+            continue
+        y = torch.from_numpy(x).cuda()
+        tuned.forward(y)
+        torch.cuda.synchronize()
+        assert np.array_equal(_bits(y.cpu().numpy()), _bits(ref.cpu().numpy())), (chunk, tuned.describe())
+        tuned.inverse(y)
+        torch.cuda.synchronize()
+        assert _relerr(y.cpu().numpy(), x.astype(np.complex128)) < TOL_3D[dtype] * 2
+    # launch-by-launch execution equals forward
+    y = torch.from_numpy(x).cuda()
+    for i in range(tuned.num_launches()):
+        tuned.execute_launch(y, i, -1)
+    torch.cuda.synchronize()
+    assert np.array_equal(_bits(y.cpu().numpy()), _bits(ref.cpu().numpy()))

@@ -31,7 +31,8 @@ def hc(tmp_path_factory):
     L.hc_fastdiv.argtypes = [ctypes.c_uint32, P, P, ctypes.c_int64]
     L.hc_remap_dest.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int64, P, P, P]
     L.hc_fft_enumerate.argtypes = [ctypes.c_int, ctypes.c_int, P, ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                                   ctypes.c_int, ctypes.c_int, P, P, P, ctypes.c_int64, P]
+                                   ctypes.c_int, ctypes.c_int, P, P, P, ctypes.c_int64, P, ctypes.c_uint32,
+                                   ctypes.c_uint32]
     L.hc_fft_enumerate.restype = ctypes.c_int64
     L.hc_shape_fits.argtypes = [ctypes.c_int, ctypes.c_int, P, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     L.hc_xcd_blocks.argtypes = [ctypes.c_uint32, P]
@@ -84,14 +85,15 @@ FFT_CONFIGS = [
 ]
 
 
-def _enumerate(hc, Ng, P, dims, dbl, pass_, T=0, E=0, xcd=0):
+def _enumerate(hc, Ng, P, dims, dbl, pass_, T=0, E=0, xcd=0, start=0, count=0):
     cap = (P * (Ng // dims[0]) * (Ng // dims[1]) * (Ng // dims[2])) if pass_ >= 0 else P * Ng
     addr = np.empty(cap, dtype=np.int64)
     n = np.empty(cap, dtype=np.int32)
     line = np.empty(cap, dtype=np.int64)
     shape = np.zeros(4, dtype=np.int64)
     dm = np.array(dims, dtype=np.int32)
-    c = hc.hc_fft_enumerate(Ng, P, _p(dm), int(dbl), pass_, T, E, xcd, _p(addr), _p(n), _p(line), cap, _p(shape))
+    c = hc.hc_fft_enumerate(Ng, P, _p(dm), int(dbl), pass_, T, E, xcd, _p(addr), _p(n), _p(line), cap, _p(shape),
+                            start, count)
     assert c >= 0, c
     return addr[:c], n[:c], line[:c], shape
 
@@ -172,3 +174,25 @@ def test_every_fitting_shape_addresses(hc, cfg):
                     assert (k_sorted == k_sorted[:, :1]).all()
                     checked += 1
     assert checked > 10
+
+
+@pytest.mark.parametrize("cfg", [(64, 16, (1, 4, 4), 16, 16), (128, 16, (1, 4, 4), 32, 32), (32, 8, (1, 2, 4), 8, 8)],
+                         ids=lambda c: f"Ng{c[0]}_{c[2]}_T{c[3]}")
+def test_chunk_windows_cover_each_element_once(hc, cfg):
+    """x and y launches over z windows (chunked execution): the windows of one
+    pass together touch every element exactly once, each window only its own z
+    columns."""
+    Ng, P, dims, T, E = cfg
+    rc, d = O.decompose(Ng, P, dims)
+    total = P * d.nlocal
+    gid = O.global_to_blocks(d, np.arange(Ng ** 3, dtype=np.int64).reshape((Ng,) * 3)).reshape(-1)
+    for chunk in (T, 2 * T, Ng // 2):
+        for pass_ in (1, 2):
+            seen = []
+            for z0 in range(0, Ng, chunk):
+                addr, n, line, shape = _enumerate(hc, Ng, P, dims, 0, pass_, T, E, 1, z0, chunk)
+                z = np.unravel_index(gid[addr], (Ng,) * 3)[2]
+                assert z.min() >= z0 and z.max() < z0 + chunk
+                seen.append(addr)
+            allv = np.concatenate(seen)
+            assert np.array_equal(np.sort(allv), np.arange(total)), (chunk, pass_)
