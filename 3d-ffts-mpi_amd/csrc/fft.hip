@@ -60,24 +60,28 @@ int fft_candidates(const AxisPass& p, a2afft_dtype dt, int N, KSel* out, int max
 // everywhere; complex64 strided passes prefer 256-byte rows (T=32).  The
 // first entry that is built and fits the geometry wins; otherwise the
 // generic default with XCD order.
+// `far`: 1 = the transform axis steps >= 64 KiB between elements (the x
+// axis of a block), 0 = nearer, -1 = either.
 struct Pref {
-    int N, dbl, strided;
+    int N, dbl, strided, far;
     KSel s;
 };
 static const Pref kPrefs[] = {
-    {512, 0, 0, {16, 16, 1, 0}},
-    {512, 0, 1, {32, 32, 1, 1}},
-    {512, 0, 1, {16, 32, 1, 1}},
-    {512, 1, 0, {8, 16, 1, 1}},
-    {512, 1, 1, {8, 16, 1, 1}},
-    {256, 0, 1, {32, 32, 1, 1}},
-    {256, 0, 1, {16, 32, 1, 1}},
+    {512, 0, 0, -1, {16, 16, 1, 0}},
+    {512, 0, 1, 1, {32, 32, 1, 1}},
+    {512, 0, 1, -1, {16, 32, 1, 0}},
+    {512, 1, 0, -1, {8, 16, 1, 1}},
+    {512, 1, 1, -1, {8, 16, 1, 1}},
+    {256, 0, 1, 1, {32, 32, 1, 1}},
+    {256, 0, 1, -1, {16, 32, 1, 0}},
 };
 
 KSel fft_default_sel(const AxisPass& p, a2afft_dtype dt, int N) {
     const bool dbl = dt == A2AFFT_C128;
+    const int far = p.fft.s_elem * (dbl ? 16 : 8) >= 65536 ? 1 : 0;
     for (const Pref& pr : kPrefs) {
         if (pr.N != N || pr.dbl != int(dbl) || pr.strided != p.strided) continue;
+        if (pr.far >= 0 && pr.far != far) continue;
         if (shape_fits(p.fft, p.a2, p.strided, N, pr.s.T, pr.s.E) && find_kernel(N, dbl, p.strided, pr.s, -1))
             return pr.s;
     }

@@ -82,6 +82,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--autotune", action="store_true", help="time every kernel variant per pass and keep the fastest")
     ap.add_argument("--chunk", type=int, default=-1, help="z columns per x/y chunk (-1: plan default / autotuned)")
+    ap.add_argument("--kernel", action="append", default=[],
+                    help="force a variant: pass:T:E:xcd:pad (pass 0 z, 1 x, 2 y); repeatable")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has one GPU and the multi-GPU exchange is "
@@ -107,6 +109,9 @@ def main():
     dist = a2afft.Distribution(Ng, P, dims=dims, dtype=dtype)
     plan = a2afft.Dfft(dist, precision=args.precision, exec="fused")
     plan.make_plans(None)
+    for kspec in args.kernel:
+        pi, T, E, xcd, pad = (int(v) for v in kspec.split(":"))
+        plan.set_kernel(pi, T, E, xcd, pad)
     if args.chunk >= 0:
         plan.set_chunking(args.chunk)
     if args.autotune:
@@ -135,21 +140,27 @@ def main():
     for _ in range(args.warmup):
         plan.forward(data)
     torch.cuda.synchronize()
-    # events between consecutive launches on the launch stream give each
-    # kernel's duration; a step is one whole forward (all launches)
+    # timed region: K whole forwards, back to back, bracketed by synchronize
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plan.forward(data)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = wall * 1e3 / args.steps
+
+    # per-kernel durations: the same K forwards launch by launch, with HIP
+    # events between consecutive launches on the launch stream
     nl = plan.num_launches()
     ev = [[torch.cuda.Event(enable_timing=True) for _ in range(nl + 1)] for _ in range(args.steps)]
     kinds = [0] * nl
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
     for s in range(args.steps):
         ev[s][0].record()
         for i in range(nl):
             kinds[i] = plan.execute_launch(data, i, -1)
             ev[s][i + 1].record()
     torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms_per_step = wall * 1e3 / args.steps
     pass_ms = [0.0, 0.0, 0.0]
     launch_ms = [0.0, 0.0, 0.0]
     nper = [kinds.count(p) for p in range(3)]
