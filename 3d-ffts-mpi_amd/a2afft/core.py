@@ -155,8 +155,9 @@ class Dfft:
         self.dist._check_buf(work)
         return L.check(L.lib().a2afft_plan_autotune(self._h, _ptr(work), iters, _stream()), "autotune")
 
-    def set_kernel(self, pass_index, T, E, xcd=0, pad=0):
-        return L.check(L.lib().a2afft_plan_set_kernel(self._h, pass_index, T, E, xcd, pad), "set_kernel")
+    def set_kernel(self, pass_index, T, E, flags=0):
+        """Force a kernel variant: flags 1 XCD order, 2 LDS pad, 4 non-temporal, 8 z-tiles outermost."""
+        return L.check(L.lib().a2afft_plan_set_kernel(self._h, pass_index, T, E, flags), "set_kernel")
 
     def set_chunking(self, z_columns):
         """x/y passes alternate over windows of z_columns z columns (0 = whole passes)."""

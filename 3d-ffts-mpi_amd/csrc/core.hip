@@ -462,13 +462,13 @@ extern "C" int a2afft_plan_set_chunking(a2afft_plan p, int z_columns) {
     return A2AFFT_OK;
 }
 
-extern "C" int a2afft_plan_set_kernel(a2afft_plan p, int pass, int T, int E, int xcd, int pad) {
+extern "C" int a2afft_plan_set_kernel(a2afft_plan p, int pass, int T, int E, int flags) {
     if (!p || pass < 0 || pass > 2) return set_error(A2AFFT_E_INVALID_ARG, "set_kernel: bad arguments");
-    KSel s{T, E, xcd ? 1 : 0, pad ? 1 : 0};
+    KSel s{T, E, flags};
     KSel cand[256];
     const int n = fft_candidates(p->pass[pass], p->d.dtype, p->d.Ng, cand, 256);
     for (int i = 0; i < n; ++i)
-        if (cand[i].T == s.T && cand[i].E == s.E && cand[i].xcd == s.xcd && cand[i].pad == s.pad) {
+        if (cand[i].T == s.T && cand[i].E == s.E && cand[i].flags == s.flags) {
             const KSel old = p->sel[pass];
             p->sel[pass] = s;
             if (!chunk_ok(p, p->chunk)) {
@@ -478,8 +478,8 @@ extern "C" int a2afft_plan_set_kernel(a2afft_plan p, int pass, int T, int E, int
             }
             return A2AFFT_OK;
         }
-    return set_error(A2AFFT_E_UNSUPPORTED, "set_kernel: no built variant T=%d E=%d xcd=%d pad=%d fits pass %d", T, E,
-                     xcd, pad, pass);
+    return set_error(A2AFFT_E_UNSUPPORTED, "set_kernel: no built variant T=%d E=%d flags=%d fits pass %d", T, E,
+                     flags, pass);
 }
 
 // Times every built variant that fits each fused pass (forward direction,
@@ -521,8 +521,8 @@ extern "C" int a2afft_plan_autotune(a2afft_plan p, void* work, int iters, void* 
             }
             if (len < int(sizeof(p->report)) - 96)
                 len += snprintf(p->report + len, sizeof(p->report) - len,
-                                "pass %s T=%d E=%d NT=%d xcd=%d pad=%d %.4f ms\n", names[i], cand[c].T, cand[c].E,
-                                cand[c].T * (p->d.Ng / cand[c].E), cand[c].xcd, cand[c].pad, ms);
+                                "pass %s T=%d E=%d NT=%d flags=%d %.4f ms\n", names[i], cand[c].T, cand[c].E,
+                                cand[c].T * (p->d.Ng / cand[c].E), cand[c].flags, ms);
         }
     }
     // chunk widths: time whole forwards
@@ -567,8 +567,8 @@ extern "C" int a2afft_plan_describe(a2afft_plan p, char* buf, int len) {
     for (int i = 0; i < 3 && n < len; ++i) {
         FftShape sh;
         if (fft_pass_shape(p->pass[i], p->d.dtype, p->d.Ng, &p->sel[i], &sh) < 0) continue;
-        n += snprintf(buf + n, len - n, "chosen %s T=%d E=%d NT=%d xcd=%d pad=%d grid=%d\n", names[i], sh.T, sh.E,
-                      sh.NT, sh.xcd, sh.pad, sh.grid);
+        n += snprintf(buf + n, len - n, "chosen %s T=%d E=%d NT=%d flags=%d grid=%d\n", names[i], sh.T, sh.E,
+                      sh.NT, sh.flags, sh.grid);
     }
     if (n < len) n += snprintf(buf + n, len - n, "chunk %u z columns\n", p->chunk);
     if (n < len) snprintf(buf + n, len - n, "%s", p->report);

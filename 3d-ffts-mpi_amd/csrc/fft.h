@@ -39,7 +39,7 @@ int launch_fft_pass(const AxisPass& pass, void* data, a2afft_dtype dtype, int N,
 
 // Launch shape of a pass (for tests and tuning reports).
 struct FftShape {
-    int T, E, NT, grid, xcd, pad;
+    int T, E, NT, grid, flags;
 };
 int fft_pass_shape(const AxisPass& pass, a2afft_dtype dtype, int N, const KSel* sel, FftShape* out);
 

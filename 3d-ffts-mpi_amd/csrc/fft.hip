@@ -36,8 +36,8 @@ const void* find_kernel(int N, bool dbl, int strided, const KSel& s, int dir) {
     init_registry();
     for (int i = 0; i < g_nkernels; ++i) {
         const KEntry& k = g_kernels[i];
-        if (k.N == N && k.dbl == int(dbl) && k.strided == strided && k.T == s.T && k.E == s.E && k.xcd == s.xcd &&
-            k.pad == s.pad && k.dir == (dir < 0 ? -1 : 1))
+        if (k.N == N && k.dbl == int(dbl) && k.strided == strided && k.T == s.T && k.E == s.E &&
+            k.flags == s.flags && k.dir == (dir < 0 ? -1 : 1))
             return k.fn;
     }
     return nullptr;
@@ -51,7 +51,7 @@ int fft_candidates(const AxisPass& p, a2afft_dtype dt, int N, KSel* out, int max
         const KEntry& k = g_kernels[i];
         if (k.N != N || k.dbl != int(dbl) || k.strided != p.strided || k.dir != -1) continue;
         if (!shape_fits(p.fft, p.a2, p.strided, N, k.T, k.E)) continue;
-        out[n++] = KSel{k.T, k.E, k.xcd, k.pad};
+        out[n++] = KSel{k.T, k.E, k.flags};
     }
     return n;
 }
@@ -67,13 +67,13 @@ struct Pref {
     KSel s;
 };
 static const Pref kPrefs[] = {
-    {512, 0, 0, -1, {16, 16, 1, 0}},
-    {512, 0, 1, 1, {32, 32, 1, 1}},
-    {512, 0, 1, -1, {16, 32, 1, 0}},
-    {512, 1, 0, -1, {8, 16, 1, 1}},
-    {512, 1, 1, -1, {8, 16, 1, 1}},
-    {256, 0, 1, 1, {32, 32, 1, 1}},
-    {256, 0, 1, -1, {16, 32, 1, 0}},
+    {512, 0, 0, -1, {16, 16, KF_XCD}},
+    {512, 0, 1, 1, {32, 32, KF_XCD | KF_PAD}},
+    {512, 0, 1, -1, {16, 32, KF_XCD}},
+    {512, 1, 0, -1, {8, 16, KF_XCD | KF_PAD}},
+    {512, 1, 1, -1, {8, 16, KF_XCD | KF_PAD}},
+    {256, 0, 1, 1, {32, 32, KF_XCD | KF_PAD}},
+    {256, 0, 1, -1, {16, 32, KF_XCD}},
 };
 
 KSel fft_default_sel(const AxisPass& p, a2afft_dtype dt, int N) {
@@ -86,7 +86,7 @@ KSel fft_default_sel(const AxisPass& p, a2afft_dtype dt, int N) {
             return pr.s;
     }
     KSel s = default_sel(N, dbl, p.strided, p.fft, p.a2);
-    if (npass_of(N) > 1 && s.T > 1) s.xcd = 1;
+    if (npass_of(N) > 1 && s.T > 1) s.flags |= KF_XCD;
     return s;
 }
 
@@ -109,8 +109,7 @@ int fft_pass_shape(const AxisPass& p, a2afft_dtype dt, int N, const KSel* sel, F
     out->E = s.E;
     out->NT = s.T * (N / s.E);
     out->grid = int(grid);
-    out->xcd = s.xcd;
-    out->pad = s.pad;
+    out->flags = s.flags;
     return A2AFFT_OK;
 }
 
@@ -125,8 +124,8 @@ int launch_fft_pass(const AxisPass& p, void* data, a2afft_dtype dt, int N, int d
     if (!tw) return set_error(A2AFFT_E_ALLOC, "fft: twiddle table for N=%d", N);
     const void* fn = find_kernel(N, dt == A2AFFT_C128, p.strided, ks, dir);
     if (!fn)
-        return set_error(A2AFFT_E_UNSUPPORTED, "fft: no kernel built for N=%d T=%d E=%d xcd=%d pad=%d", N, ks.T, ks.E,
-                         ks.xcd, ks.pad);
+        return set_error(A2AFFT_E_UNSUPPORTED, "fft: no kernel built for N=%d T=%d E=%d flags=%d", N, ks.T, ks.E,
+                         ks.flags);
     void* args[] = {&data, &g, (void*)&tw};
     int st = hip_check(int(hipLaunchKernel(fn, dim3(grid), dim3(ks.T * (N / ks.E)), args, 0, s)), "stockham launch");
     if (st < 0) return st;

@@ -14,6 +14,7 @@ int register_kernels_f32(KEntry* out) {
     add_defaults<float, 128>(out, &n);
     add_defaults<float, 256>(out, &n);
     add_defaults<float, 512>(out, &n);
+    add_extra_flags<float, 512, 32, 16, true>(out, &n);  // the strided default shape
     add_defaults<float, 1024>(out, &n);
     add_defaults<float, 2048>(out, &n);
     add_defaults<float, 4096>(out, &n);
@@ -22,9 +23,11 @@ int register_kernels_f32(KEntry* out) {
     add_shape<float, 512, 16, 8, true>(out, &n);     // 256 threads, 32 KiB LDS, 64 B rows
     add_shape<float, 512, 8, 8, true>(out, &n);      // 512 threads, 32 KiB LDS
     add_shape<float, 512, 32, 32, true>(out, &n);    // 512 threads, 128 KiB LDS, 256 B rows
+    add_extra_flags<float, 512, 32, 32, true>(out, &n);
     add_shape<float, 512, 8, 4, true>(out, &n);      // 256 threads, 16 KiB LDS, 32 B rows
     add_shape<float, 512, 8, 4, false>(out, &n);     // 256 threads, 16 KiB LDS
     add_shape<float, 512, 16, 16, false>(out, &n);   // 512 threads, 64 KiB LDS
+    add_extra_flags<float, 512, 16, 16, false>(out, &n);
     add_shape<float, 512, 32, 16, false>(out, &n);   // 256 threads, 64 KiB LDS
     add_shape<float, 256, 16, 8, true>(out, &n);
     add_shape<float, 256, 16, 32, true>(out, &n);

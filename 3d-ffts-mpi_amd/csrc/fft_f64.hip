@@ -14,6 +14,7 @@ int register_kernels_f64(KEntry* out) {
     add_defaults<double, 128>(out, &n);
     add_defaults<double, 256>(out, &n);
     add_defaults<double, 512>(out, &n);
+    add_extra_flags<double, 512, 16, 8, true>(out, &n);   // the strided default shape
     add_defaults<double, 1024>(out, &n);
     add_defaults<double, 2048>(out, &n);
     add_defaults<double, 4096>(out, &n);

@@ -51,6 +51,7 @@ struct FftGeom {
     uint32_t a2_start;  // first a2 coordinate of this launch's window
     uint32_t nunits;    // total lines (contiguous) or tiles (strided)
     FastDiv dn2;
+    FastDiv dn1;        // a1.n, for the a2-major unit order
     int64_t tstride;    // strided tiles: element step between the T columns
     int scale_mode;     // 0: none, 1: multiply by scale, 2: divide by scale
     double scale;
@@ -61,21 +62,23 @@ struct FftGeom {
 // Contiguous: workgroup `block` owns lines block*T .. block*T+T-1.  Strided:
 // workgroup `block` owns tile (u1, zt) = T adjacent columns.  The kernels
 // call this with compile-time T and TPL.
-A2A_HD int64_t line_base(const FftGeom& g, bool strided, int T, int TPL, uint32_t block, int tid, bool* valid) {
+A2A_HD int64_t line_base(const FftGeom& g, bool strided, int T, int TPL, uint32_t block, int tid, bool* valid,
+                         bool a2_major = false) {
     const int t = strided ? tid / T : tid % TPL;
     const int line = strided ? tid % T : tid / TPL;
     int64_t base;
     *valid = true;
     if (strided) {
-        const uint32_t u1 = g.dn2.div(block);
-        const uint32_t zt = block - u1 * g.n2;
+        // units are numbered a1-major (u1 = block / n2) or a2-major
+        const uint32_t u1 = a2_major ? g.dn1.mod(block) : g.dn2.div(block);
+        const uint32_t zt = a2_major ? g.dn1.div(block) : block - u1 * g.n2;
         base = g.a1.off(u1) + g.a2.off(g.a2_start + zt * uint32_t(T)) + int64_t(line) * g.tstride;
     } else {
         const uint32_t ln = block * uint32_t(T) + uint32_t(line);
         *valid = ln < g.nunits;
         const uint32_t lv = *valid ? ln : 0u;
-        const uint32_t u1 = g.dn2.div(lv);
-        const uint32_t u2 = lv - u1 * g.n2;
+        const uint32_t u1 = a2_major ? g.dn1.mod(lv) : g.dn2.div(lv);
+        const uint32_t u2 = a2_major ? g.dn1.div(lv) : lv - u1 * g.n2;
         base = g.a1.off(u1) + g.a2.off(g.a2_start + u2);
     }
     // off(n) = off(e*TPL) + off(t) for n = t + e*TPL: the dispatcher only
@@ -110,10 +113,18 @@ A2A_HD constexpr int str_E(int N, bool dbl) {
 
 inline bool fft_len_ok(int N) { return N >= 2 && N <= 4096 && (N & (N - 1)) == 0; }
 
-// Kernel choice for one pass: tile width T, elements per thread E, XCD-aware
-// tile order and LDS padding.  NT = T * N / E threads per workgroup.
+// Kernel variant flags.
+enum : int {
+    KF_XCD = 1,    // XCD-aware workgroup numbering
+    KF_PAD = 2,    // padded LDS layout
+    KF_NT = 4,     // non-temporal global loads and stores
+    KF_A2MAJ = 8,  // units numbered a2-major (z tiles outermost)
+};
+
+// Kernel choice for one pass: tile width T, elements per thread E and the
+// variant flags.  NT = T * N / E threads per workgroup.
 struct KSel {
-    int T, E, xcd, pad;
+    int T, E, flags;
 };
 
 // True when shape (T, E) can run this pass: E holds whole butterflies of every
@@ -130,9 +141,9 @@ inline bool shape_fits(const AxisMap& fft, const AxisMap& a2, int strided, int N
 
 // Default shape of a pass (before tuning).
 inline KSel default_sel(int N, bool dbl, int strided, const AxisMap& fft, const AxisMap& a2) {
-    if (!strided) return KSel{cont_T(N), cont_E(N), 0, 0};
-    KSel s{str_T(N, dbl), str_E(N, dbl), 0, 0};
-    if (!shape_fits(fft, a2, 1, N, s.T, s.E)) s = KSel{1, rmax_of(N), 0, 0};
+    if (!strided) return KSel{cont_T(N), cont_E(N), 0};
+    KSel s{str_T(N, dbl), str_E(N, dbl), 0};
+    if (!shape_fits(fft, a2, 1, N, s.T, s.E)) s = KSel{1, rmax_of(N), 0};
     return s;
 }
 
@@ -167,6 +178,7 @@ inline int resolve_with(const AxisMap& fft, const AxisMap& a1, const AxisMap& a2
     if (units >= (uint64_t(1) << 32) || blocks >= (uint64_t(1) << 31)) return A2AFFT_E_SIZE;
     g->nunits = uint32_t(units);
     g->dn2.set(g->n2 ? g->n2 : 1);
+    g->dn1.set(a1.n ? a1.n : 1);
     *grid = uint32_t(blocks);
     return A2AFFT_OK;
 }

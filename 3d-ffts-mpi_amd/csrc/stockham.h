@@ -153,17 +153,49 @@ template <int R, int DIR, typename V> __device__ __forceinline__ void dft(V* x) 
 }
 
 // LDS slot of element d of line `line`.  Contiguous tiles are line-major,
-// strided tiles column-interleaved (the T lines side by side).  PAD inserts
-// one slot every 16 (contiguous) or widens a row to T+1 (strided) so that
-// the strided write pattern of the early passes spreads over more banks.
+// strided tiles column-interleaved (the T lines side by side).  With PAD a
+// contiguous line gets one spare slot per R0 elements (R0 = first radix):
+// the first exchange writes at stride R0, which otherwise lands every lane of
+// a wavefront in a few banks.  A padded strided row is T+1 wide.
 template <int N, int T, bool STRIDED, int PAD>
 __device__ __forceinline__ constexpr int lds_slot(int line, int d) {
+    constexpr int R0 = radix_of(N, 0);
     if constexpr (STRIDED) return PAD ? d * (T + 1) + line : d * T + line;
-    else return PAD ? line * (N + N / 16) + d + (d >> 4) : line * N + d;
+    else return PAD ? line * (N + N / R0) + d + d / R0 : line * N + d;
 }
 template <int N, int T, bool STRIDED, int PAD>
 constexpr int lds_slots() {
-    return npass_of(N) < 2 ? 1 : (STRIDED ? (PAD ? N * (T + 1) : N * T) : (PAD ? T * (N + N / 16) : T * N));
+    return npass_of(N) < 2 ? 1
+                           : (STRIDED ? (PAD ? N * (T + 1) : N * T)
+                                      : (PAD ? T * (N + N / radix_of(N, 0)) : T * N));
+}
+
+// Global access of one complex value; NTMP marks it non-temporal (a native
+// 2-vector, so it stays one 8- or 16-byte access).
+template <bool NTMP, typename V> __device__ __forceinline__ V ld_global(const V* p) {
+    if constexpr (NTMP) {
+        using R = decltype(p->x);
+        typedef R nv2 __attribute__((ext_vector_type(2)));
+        const nv2 t = __builtin_nontemporal_load(reinterpret_cast<const nv2*>(p));
+        V v;
+        v.x = t.x;
+        v.y = t.y;
+        return v;
+    } else {
+        return *p;
+    }
+}
+template <bool NTMP, typename V> __device__ __forceinline__ void st_global(V* p, V v) {
+    if constexpr (NTMP) {
+        using R = decltype(p->x);
+        typedef R nv2 __attribute__((ext_vector_type(2)));
+        nv2 t;
+        t.x = v.x;
+        t.y = v.y;
+        __builtin_nontemporal_store(t, reinterpret_cast<nv2*>(p));
+    } else {
+        *p = v;
+    }
 }
 
 // Passes p..NP-1 of the Stockham schedule on the thread's registers.
@@ -214,11 +246,15 @@ __device__ __forceinline__ void stockham_passes(typename CT<real>::V (&v)[E], ty
 // One workgroup = T lines (contiguous) or one tile of T columns (strided).
 // XCD = 1 renumbers workgroups so each of the 8 XCDs (which receive
 // workgroups round-robin) works through a contiguous range of tiles.
-template <typename real, int N, int E, int T, bool STRIDED, int DIR, int XCD, int PAD>
+template <typename real, int N, int E, int T, bool STRIDED, int DIR, int FLAGS>
 __global__ __launch_bounds__(T * (N / E)) void stockham_kernel(typename CT<real>::V* __restrict__ data, FftGeom g,
                                                                const typename CT<real>::V* __restrict__ tw) {
     using V = typename CT<real>::V;
     constexpr int TPL = N / E;
+    constexpr int XCD = (FLAGS & KF_XCD) ? 1 : 0;
+    constexpr int PAD = (FLAGS & KF_PAD) ? 1 : 0;
+    constexpr bool NTMP = (FLAGS & KF_NT) != 0;
+    constexpr bool A2MAJ = (FLAGS & KF_A2MAJ) != 0;
     __shared__ V lds[lds_slots<N, T, STRIDED, PAD>()];
 
     const int tid = threadIdx.x;
@@ -226,12 +262,12 @@ __global__ __launch_bounds__(T * (N / E)) void stockham_kernel(typename CT<real>
     const int line = STRIDED ? tid % T : tid / TPL;
     const uint32_t block = XCD ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
     bool valid;
-    const int64_t base = line_base(g, STRIDED, T, TPL, block, tid, &valid);
+    const int64_t base = line_base(g, STRIDED, T, TPL, block, tid, &valid, A2MAJ);
 
     V v[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-        if (valid) v[e] = data[base + elem_off(g, e * TPL)];
+        if (valid) v[e] = ld_global<NTMP>(data + base + elem_off(g, e * TPL));
         else { v[e].x = 0; v[e].y = 0; }
     }
 
@@ -248,31 +284,42 @@ __global__ __launch_bounds__(T * (N / E)) void stockham_kernel(typename CT<real>
         for (int e = 0; e < E; ++e) { v[e].x = v[e].x / s; v[e].y = v[e].y / s; }
     }
 #pragma unroll
-    for (int e = 0; e < E; ++e) data[base + elem_off(g, e * TPL)] = v[e];
+    for (int e = 0; e < E; ++e) st_global<NTMP>(data + base + elem_off(g, e * TPL), v[e]);
 }
 
 // Registry of built kernel variants (filled by the per-precision files).
 struct KEntry {
-    int N, dbl, strided, T, E, xcd, pad, dir;
+    int N, dbl, strided, T, E, flags, dir;
     const void* fn;
 };
 
-template <typename real, int N, int E, int T, bool S, int XCD, int PAD>
+template <typename real, int N, int E, int T, bool S, int FLAGS>
 inline void add_variant(KEntry* out, int* n) {
     constexpr int dbl = sizeof(real) == 8;
-    out[(*n)++] = KEntry{N, dbl, S, T, E, XCD, PAD, -1, (const void*)&stockham_kernel<real, N, E, T, S, -1, XCD, PAD>};
-    out[(*n)++] = KEntry{N, dbl, S, T, E, XCD, PAD, 1, (const void*)&stockham_kernel<real, N, E, T, S, 1, XCD, PAD>};
+    out[(*n)++] = KEntry{N, dbl, S, T, E, FLAGS, -1, (const void*)&stockham_kernel<real, N, E, T, S, -1, FLAGS>};
+    out[(*n)++] = KEntry{N, dbl, S, T, E, FLAGS, 1, (const void*)&stockham_kernel<real, N, E, T, S, 1, FLAGS>};
 }
 
-// Both tile orders and both LDS layouts of one shape.
+// Plain, XCD-ordered, padded and both.
 template <typename real, int N, int E, int T, bool S>
 inline void add_shape(KEntry* out, int* n) {
-    add_variant<real, N, E, T, S, 0, 0>(out, n);
+    add_variant<real, N, E, T, S, 0>(out, n);
     if constexpr (npass_of(N) > 1) {
-        add_variant<real, N, E, T, S, 1, 0>(out, n);
-        add_variant<real, N, E, T, S, 0, 1>(out, n);
-        add_variant<real, N, E, T, S, 1, 1>(out, n);
+        add_variant<real, N, E, T, S, KF_XCD>(out, n);
+        add_variant<real, N, E, T, S, KF_PAD>(out, n);
+        add_variant<real, N, E, T, S, KF_XCD | KF_PAD>(out, n);
     }
+}
+
+// Non-temporal and a2-major variants of a shape (benchmark lengths).
+template <typename real, int N, int E, int T, bool S>
+inline void add_extra_flags(KEntry* out, int* n) {
+    add_variant<real, N, E, T, S, KF_XCD | KF_NT>(out, n);
+    add_variant<real, N, E, T, S, KF_XCD | KF_PAD | KF_NT>(out, n);
+    add_variant<real, N, E, T, S, KF_XCD | KF_A2MAJ>(out, n);
+    add_variant<real, N, E, T, S, KF_XCD | KF_PAD | KF_A2MAJ>(out, n);
+    add_variant<real, N, E, T, S, KF_A2MAJ>(out, n);
+    add_variant<real, N, E, T, S, KF_XCD | KF_NT | KF_A2MAJ>(out, n);
 }
 
 // Default contiguous / strided shapes and the single-column fallback.
@@ -281,7 +328,7 @@ inline void add_defaults(KEntry* out, int* n) {
     constexpr bool dbl = sizeof(real) == 8;
     add_shape<real, N, cont_E(N), cont_T(N), false>(out, n);
     add_shape<real, N, str_E(N, dbl), str_T(N, dbl), true>(out, n);
-    add_variant<real, N, rmax_of(N), 1, true, 0, 0>(out, n);
+    add_variant<real, N, rmax_of(N), 1, true, 0>(out, n);
 }
 
 }  // namespace a2a

@@ -53,7 +53,7 @@ def make_grf(a2afft, torch, Ng, seed, slope=-3.0, noise=1e-3):
     return w.reshape(Ng, Ng, Ng)
 
 
-def cpu_baseline(Ng, P, dims, threads):
+def cpu_baseline(Ng, P, dims, threads, seed):
     """The oracle (C restatement of the reference stage order, FP64) on one
     full transform of the same grid shape; OpenMP over `threads` cores."""
     import numpy as np
@@ -62,7 +62,7 @@ def cpu_baseline(Ng, P, dims, threads):
     O.set_threads(threads)
     rc, d = O.decompose(Ng, P, dims)
     assert rc == 0
-    x = O.fill_uniform(2, Ng ** 3).reshape(P, -1)
+    x = O.fill_uniform(seed, Ng ** 3).reshape(P, -1)
     S = np.zeros_like(x)
     import ctypes
     t0 = time.perf_counter()
@@ -77,13 +77,17 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
-    ap.add_argument("--ng", type=int, default=512)
+    ap.add_argument("--config", default="c3", choices=["c3", "c2"],
+                    help="c3: 512^3 complex64 over dims [1,4,4], GRF seed 2 (the headline); "
+                         "c2: 256^3 complex128 over dims [1,2,4], N(0,1) seed 1")
+    ap.add_argument("--precision", default=None, choices=["fp32", "fp64"])
+    ap.add_argument("--ng", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--autotune", action="store_true", help="time every kernel variant per pass and keep the fastest")
     ap.add_argument("--chunk", type=int, default=-1, help="z columns per x/y chunk (-1: plan default / autotuned)")
     ap.add_argument("--kernel", action="append", default=[],
-                    help="force a variant: pass:T:E:xcd:pad (pass 0 z, 1 x, 2 y); repeatable")
+                    help="force a variant: pass:T:E:flags (pass 0 z, 1 x, 2 y; flags 1 XCD, 2 pad, 4 "
+                         "non-temporal, 8 z-tiles outermost); repeatable")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has one GPU and the multi-GPU exchange is "
@@ -93,14 +97,23 @@ def main():
     import torch
     import a2afft
 
-    Ng, P, dims = args.ng, 16, (1, 4, 4)
-    seed = 2
+    if args.config == "c2":
+        Ng, P, dims, seed, field = args.ng or 256, 8, (1, 2, 4), 1, "normal"
+        args.precision = args.precision or "fp64"
+    else:
+        Ng, P, dims, seed, field = args.ng or 512, 16, (1, 4, 4), 2, "grf"
+        args.precision = args.precision or "fp32"
     dtype = "c64" if args.precision == "fp32" else "c128"
     esize = 8 if dtype == "c64" else 16
     N = Ng ** 3
 
-    log(f"[bench] generating {Ng}^3 GRF input on the device")
-    g = make_grf(a2afft, torch, Ng, seed)
+    log(f"[bench] generating {Ng}^3 {field} input on the device")
+    if field == "grf":
+        g = make_grf(a2afft, torch, Ng, seed)
+    else:
+        g = torch.empty(Ng ** 3, dtype=torch.complex128, device="cuda")
+        a2afft.fill_random(g, seed, "normal")
+        g = g.reshape(Ng, Ng, Ng)
     x = a2afft.inputs.global_to_blocks(g.to(torch.complex64 if dtype == "c64" else torch.complex128), dims)
     del g
     x = x.contiguous()
@@ -110,8 +123,8 @@ def main():
     plan = a2afft.Dfft(dist, precision=args.precision, exec="fused")
     plan.make_plans(None)
     for kspec in args.kernel:
-        pi, T, E, xcd, pad = (int(v) for v in kspec.split(":"))
-        plan.set_kernel(pi, T, E, xcd, pad)
+        pi, T, E, flags = (int(v) for v in kspec.split(":"))
+        plan.set_kernel(pi, T, E, flags)
     if args.chunk >= 0:
         plan.set_chunking(args.chunk)
     if args.autotune:
@@ -208,13 +221,13 @@ def main():
     if not args.no_cpu:
         threads = min(16, len(os.sched_getaffinity(0)))
         log(f"[bench] CPU baseline on {threads} threads")
-        cpu_ms, used = cpu_baseline(Ng, P, dims, threads)
+        cpu_ms, used = cpu_baseline(Ng, P, dims, threads, seed)
         cpu = {"value": round(cpu_ms, 1), "unit": "ms", "cores": used, "kind": "port",
-               "sample": f"one full {Ng}^3 forward, complex128, 16 virtual ranks dims [1,4,4], reference stage "
+               "sample": f"one full {Ng}^3 forward, complex128, {P} virtual ranks dims {list(dims)}, reference stage "
                          f"order (oracle C restatement, OpenMP)"}
 
     out = {
-        "metric": "ms/512^3 c2c 3D FFT + % HBM roofline",
+        "metric": "ms/512^3 c2c 3D FFT + % HBM roofline" if args.config == "c3" else f"ms/{Ng}^3 c2c 3D FFT",
         "value": round(ms_per_step, 4),
         "unit": "ms",
         "n_gpus": 1,
@@ -227,8 +240,11 @@ def main():
         "dtype": "f32" if args.precision == "fp32" else "f64",
         "data": "synthetic",
         "config": {"workload": f"{Ng}^3 c2c forward 3D FFT, {'complex64' if dtype == 'c64' else 'complex128'}, "
-                               f"16 virtual ranks dims [1,4,4], GRF P(k)~k^-3 seed {seed} + 1e-3 noise",
-                   "precision": args.precision, "exec": "fused", "Ng": Ng, "ranks": P, "dims": list(dims),
+                               f"{P} virtual ranks dims {list(dims)}, "
+                               + (f"GRF P(k)~k^-3 seed {seed} + 1e-3 noise" if field == "grf"
+                                  else f"N(0,1) re/im seed {seed}"),
+                   "name": args.config, "precision": args.precision, "exec": "fused", "Ng": Ng, "ranks": P,
+                   "dims": list(dims),
                    "autotune": bool(args.autotune)},
         "gflops": round(gflops, 1),
         "hbm_gbs_whole": round(whole_gbs, 1),

@@ -193,12 +193,13 @@ int a2afft_execute_pass(a2afft_plan plan, void* d_data, int pass, int direction,
 
 /* Kernel variants (no reference counterpart: cuFFT chose its own kernels).
  * Every variant of a pass computes identical bits; they differ in tile width
- * T, elements per thread E, XCD-aware tile order and LDS padding.
+ * T, elements per thread E and flags: 1 XCD-aware workgroup order, 2 padded
+ * LDS, 4 non-temporal loads/stores, 8 z-tiles-outermost unit order.
  * autotune times each built variant that fits each fused pass, `iters`
  * launches in place on `work` (a [P][nlocal] buffer whose contents are
  * destroyed), and keeps the fastest; set_kernel picks one by hand. */
 int a2afft_plan_autotune(a2afft_plan plan, void* work, int iters, void* stream);
-int a2afft_plan_set_kernel(a2afft_plan plan, int pass, int T, int E, int xcd, int pad);
+int a2afft_plan_set_kernel(a2afft_plan plan, int pass, int T, int E, int flags);
 /* Chunked execution: after the z pass, the x and y passes alternate over
  * windows of `z_columns` z columns (0 = whole passes), so each window is
  * re-read from the Infinity Cache.  Results are identical for every width;

@@ -36,7 +36,8 @@ int hc_remap_dest(int kind, int P, int64_t nlocal, const int* lgs, const int* di
 // Enumerates (address, n, line) for every element an FFT launch loads and
 // stores.  pass 0..2: the fused passes of a (Ng, P, dims) decomposition;
 // pass -1: forward_1d_fft over P contiguous pencils of length Ng.
-// T = E = 0 picks the default shape; xcd = 1 applies the XCD renumbering.
+// T = E = 0 picks the default shape; xcd bit 0 applies the XCD renumbering,
+// bit 1 the a2-major unit order.
 // shape_out: T, E, NT, grid.  Returns the count, or a negative status.
 int64_t hc_fft_enumerate(int Ng, int P, const int* dims, int dbl, int pass, int T, int E, int xcd, int64_t* addr,
                          int32_t* n_out, int64_t* line_out, int64_t cap, int64_t* shape_out, uint32_t a2_start,
@@ -69,10 +70,10 @@ int64_t hc_fft_enumerate(int Ng, int P, const int* dims, int dbl, int pass, int 
     const int TPL = Ng / E;
     int64_t c = 0;
     for (uint32_t hb = 0; hb < grid; ++hb) {
-        const uint32_t b = xcd ? xcd_block(hb, grid) : hb;
+        const uint32_t b = (xcd & 1) ? xcd_block(hb, grid) : hb;
         for (int tid = 0; tid < NT; ++tid) {
             bool valid;
-            const int64_t base = line_base(g, strided, T, TPL, b, tid, &valid);
+            const int64_t base = line_base(g, strided, T, TPL, b, tid, &valid, (xcd & 2) != 0);
             if (!valid) continue;
             const int t = strided ? tid / T : tid % TPL;
             const int line = strided ? tid % T : tid / TPL;

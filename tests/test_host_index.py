@@ -163,7 +163,7 @@ def test_every_fitting_shape_addresses(hc, cfg):
             for E in (2, 4, 8, 16, 32, 64):
                 if not hc.hc_shape_fits(Ng, P, _p(dm), pass_, T, E):
                     continue
-                for xcd in (0, 1):
+                for xcd in (0, 1, 2, 3):
                     addr, n, line, shape = _enumerate(hc, Ng, P, dims, 0, pass_, T, E, xcd)
                     assert np.array_equal(np.sort(addr), np.arange(total)), (pass_, T, E, xcd)
                     coords = np.stack(np.unravel_index(gid[addr], (Ng,) * 3))
