@@ -56,8 +56,9 @@ int fft_candidates(const AxisPass& p, a2afft_dtype dt, int N, KSel* out, int max
     return n;
 }
 
-// Measured preferences (MI355X, 512^3 over dims [1,4,4]): XCD-aware order
-// everywhere; complex64 strided passes prefer 256-byte rows (T=32).  The
+// Measured preferences (MI355X autotune, 512^3 over dims [1,4,4]): XCD-aware
+// order and non-temporal access for the z and y passes; the far-strided x
+// pass prefers 256-byte rows (T=32, complex64) and z tiles outermost.  The
 // first entry that is built and fits the geometry wins; otherwise the
 // generic default with XCD order.
 // `far`: 1 = the transform axis steps >= 64 KiB between elements (the x
@@ -67,11 +68,12 @@ struct Pref {
     KSel s;
 };
 static const Pref kPrefs[] = {
-    {512, 0, 0, -1, {16, 16, KF_XCD}},
-    {512, 0, 1, 1, {32, 32, KF_XCD | KF_PAD}},
-    {512, 0, 1, -1, {16, 32, KF_XCD}},
+    {512, 0, 0, -1, {16, 16, KF_XCD | KF_PAD | KF_NT}},
+    {512, 0, 1, 1, {32, 32, KF_A2MAJ}},
+    {512, 0, 1, -1, {16, 32, KF_XCD | KF_NT}},
     {512, 1, 0, -1, {8, 16, KF_XCD | KF_PAD}},
-    {512, 1, 1, -1, {8, 16, KF_XCD | KF_PAD}},
+    {512, 1, 1, 1, {8, 16, KF_XCD | KF_PAD | KF_A2MAJ}},
+    {512, 1, 1, -1, {8, 16, KF_XCD | KF_NT}},
     {256, 0, 1, 1, {32, 32, KF_XCD | KF_PAD}},
     {256, 0, 1, -1, {16, 32, KF_XCD}},
 };
