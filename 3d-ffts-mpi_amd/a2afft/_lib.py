@@ -79,6 +79,7 @@ SIGNATURES = {
     "a2afft_launch_scale_fft": [_P, _P, _I, _I64, _I, _I, _P],
     "a2afft_launch_fast_copy_fft": [_P, _P, _I64, _I, _I, _P],
     "a2afft_plan_create": [ctypes.POINTER(_P), _P, _I, _I],
+    "a2afft_grid_plan_create": [ctypes.POINTER(_P), _I, _I, _I, _I],
     "a2afft_make_plans": [_P, _P],
     "a2afft_forward": [_P, _P, _P],
     "a2afft_inverse": [_P, _P, _P],

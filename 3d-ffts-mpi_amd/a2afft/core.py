@@ -120,39 +120,25 @@ class Distribution:
             pass
 
 
-class Dfft:
-    """a2adfft (a2adfft.cpp:36-188) with a precision and an execution strategy."""
+class _Plan:
+    """Execution methods shared by distribution plans and single-grid plans."""
 
-    def __init__(self, dist, precision=None, exec="fused"):
-        if precision is None:
-            precision = "fp64" if dist.dtype_code == L.C128 else "fp32"
-        self.dist = dist
-        self.precision = precision
-        self.exec = exec
-        self._h = ctypes.c_void_p()
-        L.check(L.lib().a2afft_plan_create(ctypes.byref(self._h), dist.handle, PRECISIONS[precision],
-                                           EXECS[exec]), "a2adfft")
-        self._scratch = None
+    _h = ctypes.c_void_p()
 
-    def make_plans(self, d_scratch=None):
-        """a2adfft::make_plans (a2adfft.cpp:47-53)."""
-        if d_scratch is not None:
-            self.dist._check_buf(d_scratch)
-        self._scratch = d_scratch
-        L.check(L.lib().a2afft_make_plans(self._h, _ptr(d_scratch) if d_scratch is not None else None),
-                "make_plans")
+    def _check_buf(self, b):
+        raise NotImplementedError
 
     def forward(self, d_data):
-        self.dist._check_buf(d_data)
+        self._check_buf(d_data)
         return L.check(L.lib().a2afft_forward(self._h, _ptr(d_data), _stream()), "forward")
 
     def inverse(self, d_data):
-        self.dist._check_buf(d_data)
+        self._check_buf(d_data)
         return L.check(L.lib().a2afft_inverse(self._h, _ptr(d_data), _stream()), "inverse")
 
     def autotune(self, work, iters=5):
         """Pick the fastest kernel variant per pass; `work` contents are destroyed."""
-        self.dist._check_buf(work)
+        self._check_buf(work)
         return L.check(L.lib().a2afft_plan_autotune(self._h, _ptr(work), iters, _stream()), "autotune")
 
     def set_kernel(self, pass_index, T, E, flags=0):
@@ -168,6 +154,7 @@ class Dfft:
 
     def execute_launch(self, data, index, direction=-1):
         """Launch `index` of a fused transform; returns its pass (0 z, 1 x, 2 y)."""
+        self._check_buf(data)
         return L.check(L.lib().a2afft_execute_launch(self._h, _ptr(data), index, direction, _stream()),
                        "execute_launch")
 
@@ -186,6 +173,52 @@ class Dfft:
             self.destroy()
         except Exception:
             pass
+
+
+class Dfft(_Plan):
+    """a2adfft (a2adfft.cpp:36-188) with a precision and an execution strategy."""
+
+    def __init__(self, dist, precision=None, exec="fused"):
+        if precision is None:
+            precision = "fp64" if dist.dtype_code == L.C128 else "fp32"
+        self.dist = dist
+        self.precision = precision
+        self.exec = exec
+        self._h = ctypes.c_void_p()
+        L.check(L.lib().a2afft_plan_create(ctypes.byref(self._h), dist.handle, PRECISIONS[precision],
+                                           EXECS[exec]), "a2adfft")
+        self._scratch = None
+
+    def _check_buf(self, b):
+        self.dist._check_buf(b)
+
+    def make_plans(self, d_scratch=None):
+        """a2adfft::make_plans (a2adfft.cpp:47-53)."""
+        if d_scratch is not None:
+            self.dist._check_buf(d_scratch)
+        self._scratch = d_scratch
+        L.check(L.lib().a2afft_make_plans(self._h, _ptr(d_scratch) if d_scratch is not None else None),
+                "make_plans")
+
+
+class GridDfft(_Plan):
+    """Fused plan over one nx x ny x nz C-order grid (no rank decomposition)."""
+
+    def __init__(self, shape, precision="fp32"):
+        self.shape = tuple(int(s) for s in shape)
+        self.precision = precision
+        self.torch_dtype = torch.complex128 if precision == "fp64" else torch.complex64
+        self._h = ctypes.c_void_p()
+        L.check(L.lib().a2afft_grid_plan_create(ctypes.byref(self._h), *self.shape, PRECISIONS[precision]),
+                "grid_plan_create")
+
+    def _check_buf(self, b):
+        n = self.shape[0] * self.shape[1] * self.shape[2]
+        if b.dtype != self.torch_dtype or b.numel() != n or not b.is_contiguous() or not b.is_cuda:
+            raise ValueError(f"buffer must be a contiguous device {self.torch_dtype} tensor of {n} elements")
+
+    def empty(self, device="cuda"):
+        return torch.empty(self.shape, dtype=self.torch_dtype, device=device)
 
 
 # ---- launcher level --------------------------------------------------------

@@ -256,7 +256,8 @@ A2A_STAGE_ENTRY(a2afft_return_y_pencils, 5)
 // ===========================================================================
 
 struct a2afft_plan_s {
-    a2afft_dist_s d;
+    a2afft_dist_s d;    // the distribution (Ng = 0 for a single-grid plan)
+    int n[3];           // global extents x, y, z
     a2afft_precision prec;
     a2afft_exec exec;
     void* scratch;
@@ -284,9 +285,9 @@ static int plan_launches(const a2afft_plan_s* p, Launch* out, int max) {
         out[n++] = Launch{2, 0, 0};
         return n;
     }
-    const uint32_t Ng = uint32_t(p->d.Ng);
-    for (uint32_t z0 = 0; z0 < Ng && n + 2 <= max; z0 += p->chunk) {
-        const uint32_t cnt = Ng - z0 < p->chunk ? Ng - z0 : p->chunk;
+    const uint32_t nz = uint32_t(p->n[2]);
+    for (uint32_t z0 = 0; z0 < nz && n + 2 <= max; z0 += p->chunk) {
+        const uint32_t cnt = nz - z0 < p->chunk ? nz - z0 : p->chunk;
         out[n++] = Launch{1, z0, cnt};
         out[n++] = Launch{2, z0, cnt};
     }
@@ -295,10 +296,41 @@ static int plan_launches(const a2afft_plan_s* p, Launch* out, int max) {
 
 static bool chunk_ok(const a2afft_plan_s* p, uint32_t chunk) {
     if (chunk == 0) return true;
-    if (p->d.Ng % int(chunk)) return false;
+    if (p->n[2] % int(chunk)) return false;
     for (int i = 1; i < 3; ++i)
         if (p->sel[i].T > 0 && chunk % uint32_t(p->sel[i].T)) return false;
     return true;
+}
+
+static int plan_init(a2afft_plan_s* p) {
+    p->scratch = nullptr;
+    p->report[0] = 0;
+    p->chunk = 0;
+    PassMaps pm[3];
+    fused_pass_maps(p->n, p->d.dims, p->d.lgs, p->d.nlocal, pm);
+    for (int i = 0; i < 3; ++i) p->pass[i] = AxisPass{pm[i].fft, pm[i].a1, pm[i].a2, pm[i].strided, 0, 1.0};
+    FftShape sh;
+    for (int i = 0; i < 3; ++i) {
+        const int N = int(p->pass[i].fft.n);
+        if (!fft_length_supported(N)) return set_error(A2AFFT_E_UNSUPPORTED, "plan: length %d not supported", N);
+        p->sel[i] = fft_default_sel(p->pass[i], p->d.dtype, N);
+        A2A_TRY(fft_pass_shape(p->pass[i], p->d.dtype, N, &p->sel[i], &sh));
+        // build the twiddle tables now so execute allocates nothing
+        if (!twiddle_table(N, -1, p->d.dtype) || !twiddle_table(N, 1, p->d.dtype))
+            return set_error(A2AFFT_E_ALLOC, "plan: twiddle tables");
+    }
+    return A2AFFT_OK;
+}
+
+static int check_precision(a2afft_precision prec, a2afft_dtype dt) {
+    if (prec == A2AFFT_FP64 && dt != A2AFFT_C128)
+        return set_error(A2AFFT_E_INVALID_ARG, "plan_create: FP64 needs complex128 data");
+    if (prec != A2AFFT_FP64 && dt != A2AFFT_C64)
+        return set_error(A2AFFT_E_INVALID_ARG, "plan_create: FP32 / split precisions need complex64 data");
+    if (prec == A2AFFT_SPLIT_BF16 || prec == A2AFFT_SPLIT_FP16)
+        return set_error(A2AFFT_E_UNSUPPORTED, "plan_create: split precisions not built");
+    if (prec != A2AFFT_FP64 && prec != A2AFFT_FP32) return set_error(A2AFFT_E_INVALID_ARG, "plan_create: precision");
+    return A2AFFT_OK;
 }
 
 extern "C" int a2afft_plan_create(a2afft_plan* out, a2afft_dist dist, a2afft_precision prec, a2afft_exec exec) {
@@ -306,37 +338,52 @@ extern "C" int a2afft_plan_create(a2afft_plan* out, a2afft_dist dist, a2afft_pre
     *out = nullptr;
     if (exec != A2AFFT_EXEC_FUSED && exec != A2AFFT_EXEC_STAGED)
         return set_error(A2AFFT_E_INVALID_ARG, "plan_create: bad exec");
-    if (prec == A2AFFT_FP64 && dist->dtype != A2AFFT_C128)
-        return set_error(A2AFFT_E_INVALID_ARG, "plan_create: FP64 needs complex128 data");
-    if (prec != A2AFFT_FP64 && dist->dtype != A2AFFT_C64)
-        return set_error(A2AFFT_E_INVALID_ARG, "plan_create: FP32 / split precisions need complex64 data");
-    if (prec == A2AFFT_SPLIT_BF16 || prec == A2AFFT_SPLIT_FP16)
-        return set_error(A2AFFT_E_UNSUPPORTED, "plan_create: split precisions not built");
-    if (prec != A2AFFT_FP64 && prec != A2AFFT_FP32) return set_error(A2AFFT_E_INVALID_ARG, "plan_create: precision");
-    if (!fft_length_supported(dist->Ng)) return set_error(A2AFFT_E_UNSUPPORTED, "plan: Ng=%d", dist->Ng);
+    A2A_TRY(check_precision(prec, dist->dtype));
     if (int64_t(dist->P) * dist->nlocal >= (int64_t(1) << 40)) return set_error(A2AFFT_E_SIZE, "plan: too large");
-
     a2afft_plan_s* p = new (std::nothrow) a2afft_plan_s;
     if (!p) return set_error(A2AFFT_E_ALLOC, "plan_create");
     p->d = *dist;
+    p->n[0] = p->n[1] = p->n[2] = dist->Ng;
     p->prec = prec;
     p->exec = exec;
-    p->scratch = nullptr;
-    PassMaps pm[3];
-    fused_pass_maps(dist->Ng, dist->dims, dist->lgs, dist->nlocal, pm);
-    for (int i = 0; i < 3; ++i) p->pass[i] = AxisPass{pm[i].fft, pm[i].a1, pm[i].a2, pm[i].strided, 0, 1.0};
-    p->report[0] = 0;
-    p->chunk = 0;
-    FftShape sh;
-    for (int i = 0; i < 3; ++i) {
-        p->sel[i] = fft_default_sel(p->pass[i], dist->dtype, dist->Ng);
-        int st = fft_pass_shape(p->pass[i], dist->dtype, dist->Ng, &p->sel[i], &sh);
-        if (st < 0) { delete p; return st; }
-    }
-    // Build the twiddle tables now so execute allocates nothing.
-    if (!twiddle_table(dist->Ng, -1, dist->dtype) || !twiddle_table(dist->Ng, 1, dist->dtype)) {
+    int st = plan_init(p);
+    if (st < 0) {
         delete p;
-        return set_error(A2AFFT_E_ALLOC, "plan: twiddle tables");
+        return st;
+    }
+    *out = p;
+    return A2AFFT_OK;
+}
+
+// One nx x ny x nz grid in C order (z fastest): the fused transform without a
+// rank decomposition (dims [1,1,1]).
+extern "C" int a2afft_grid_plan_create(a2afft_plan* out, int nx, int ny, int nz, a2afft_precision prec) {
+    if (!out || nx <= 0 || ny <= 0 || nz <= 0) return set_error(A2AFFT_E_INVALID_ARG, "grid_plan_create: bad args");
+    *out = nullptr;
+    const a2afft_dtype dt = prec == A2AFFT_FP64 ? A2AFFT_C128 : A2AFFT_C64;
+    A2A_TRY(check_precision(prec, dt));
+    const int64_t total = int64_t(nx) * ny * nz;
+    if (total >= (int64_t(1) << 32)) return set_error(A2AFFT_E_SIZE, "grid_plan_create: grid >= 2^32 points");
+    a2afft_plan_s* p = new (std::nothrow) a2afft_plan_s;
+    if (!p) return set_error(A2AFFT_E_ALLOC, "grid_plan_create");
+    p->d.Ng = 0;
+    p->d.P = 1;
+    p->d.dims[0] = p->d.dims[1] = p->d.dims[2] = 1;
+    p->d.lgs[0] = nx;
+    p->d.lgs[1] = ny;
+    p->d.lgs[2] = nz;
+    p->d.nlocal = total;
+    p->d.blockSize = 64;
+    p->d.dtype = dt;
+    p->n[0] = nx;
+    p->n[1] = ny;
+    p->n[2] = nz;
+    p->prec = prec;
+    p->exec = A2AFFT_EXEC_FUSED;
+    int st = plan_init(p);
+    if (st < 0) {
+        delete p;
+        return st;
     }
     *out = p;
     return A2AFFT_OK;
@@ -351,6 +398,7 @@ extern "C" int a2afft_make_plans(a2afft_plan p, void* scratch) {
 // a2adfft.cpp:55-188 in the reference stage order.
 static int exec_staged(a2afft_plan_s* p, void* D, int dir, hipStream_t s) {
     if (!p->scratch) return set_error(A2AFFT_E_NO_SCRATCH, "staged execute needs scratch (make_plans)");
+    if (p->d.Ng <= 0) return set_error(A2AFFT_E_INVALID_ARG, "staged execute needs a distribution plan");
     const a2afft_dist_s* d = &p->d;
     void* S = p->scratch;
     const int64_t total = int64_t(d->P) * d->nlocal;
@@ -385,14 +433,14 @@ static int run_launch(a2afft_plan_s* p, const Launch& L, void* D, int dir, hipSt
     ps.a2_start = L.start;
     ps.a2_count = L.count;
     if (L.pass == 2 && dir > 0) {
-        ps.scale = double(int64_t(p->d.Ng) * p->d.Ng * p->d.Ng);
+        ps.scale = double(int64_t(p->n[0]) * p->n[1] * p->n[2]);
         ps.scale_mode = 2;
     }
-    return launch_fft_pass(ps, D, p->d.dtype, p->d.Ng, dir, s, &p->sel[L.pass]);
+    return launch_fft_pass(ps, D, p->d.dtype, int(ps.fft.n), dir, s, &p->sel[L.pass]);
 }
 
 // In-place passes over the block layout in the reference's axis order z, x,
-// y, with 1/Ng^3 folded into the last inverse pass.
+// y, with 1/(nx*ny*nz) folded into the last inverse pass.
 static int exec_fused(a2afft_plan_s* p, void* D, int dir, hipStream_t s) {
     Launch L[1024];
     const int n = plan_launches(p, L, int(sizeof(L) / sizeof(L[0])));
@@ -453,10 +501,10 @@ extern "C" int a2afft_execute_launch(a2afft_plan p, void* data, int index, int d
 extern "C" int a2afft_plan_set_chunking(a2afft_plan p, int z_columns) {
     if (!p || z_columns < 0) return set_error(A2AFFT_E_INVALID_ARG, "set_chunking: bad arguments");
     if (p->exec != A2AFFT_EXEC_FUSED) return set_error(A2AFFT_E_INVALID_ARG, "set_chunking: fused plans only");
-    if (z_columns >= p->d.Ng) z_columns = 0;
+    if (z_columns >= p->n[2]) z_columns = 0;
     if (z_columns > 0 && z_columns < 8) return set_error(A2AFFT_E_INVALID_ARG, "set_chunking: at least 8 columns");
     if (!chunk_ok(p, uint32_t(z_columns)))
-        return set_error(A2AFFT_E_INVALID_ARG, "set_chunking: %d columns must divide Ng and be a multiple of the "
+        return set_error(A2AFFT_E_INVALID_ARG, "set_chunking: %d columns must divide nz and be a multiple of the "
                          "x and y tile widths", z_columns);
     p->chunk = uint32_t(z_columns);
     return A2AFFT_OK;
@@ -466,7 +514,7 @@ extern "C" int a2afft_plan_set_kernel(a2afft_plan p, int pass, int T, int E, int
     if (!p || pass < 0 || pass > 2) return set_error(A2AFFT_E_INVALID_ARG, "set_kernel: bad arguments");
     KSel s{T, E, flags};
     KSel cand[256];
-    const int n = fft_candidates(p->pass[pass], p->d.dtype, p->d.Ng, cand, 256);
+    const int n = fft_candidates(p->pass[pass], p->d.dtype, int(p->pass[pass].fft.n), cand, 256);
     for (int i = 0; i < n; ++i)
         if (cand[i].T == s.T && cand[i].E == s.E && cand[i].flags == s.flags) {
             const KSel old = p->sel[pass];
@@ -500,14 +548,15 @@ extern "C" int a2afft_plan_autotune(a2afft_plan p, void* work, int iters, void* 
     int st = A2AFFT_OK;
     for (int i = 0; i < 3 && st >= 0; ++i) {
         KSel cand[256];
-        const int n = fft_candidates(p->pass[i], p->d.dtype, p->d.Ng, cand, 256);
+        const int N = int(p->pass[i].fft.n);
+        const int n = fft_candidates(p->pass[i], p->d.dtype, N, cand, 256);
         float best = 1e30f;
         for (int c = 0; c < n && st >= 0; ++c) {
-            st = launch_fft_pass(p->pass[i], work, p->d.dtype, p->d.Ng, -1, s, &cand[c]);   // warm
+            st = launch_fft_pass(p->pass[i], work, p->d.dtype, N, -1, s, &cand[c]);   // warm
             if (st < 0) break;
             st = hip_check(int(hipEventRecord(e0, s)), "autotune record");
             for (int it = 0; it < iters && st >= 0; ++it)
-                st = launch_fft_pass(p->pass[i], work, p->d.dtype, p->d.Ng, -1, s, &cand[c]);
+                st = launch_fft_pass(p->pass[i], work, p->d.dtype, N, -1, s, &cand[c]);
             if (st < 0) break;
             st = hip_check(int(hipEventRecord(e1, s)), "autotune record");
             if (st >= 0) st = hip_check(int(hipEventSynchronize(e1)), "autotune sync");
@@ -522,7 +571,7 @@ extern "C" int a2afft_plan_autotune(a2afft_plan p, void* work, int iters, void* 
             if (len < int(sizeof(p->report)) - 96)
                 len += snprintf(p->report + len, sizeof(p->report) - len,
                                 "pass %s T=%d E=%d NT=%d flags=%d %.4f ms\n", names[i], cand[c].T, cand[c].E,
-                                cand[c].T * (p->d.Ng / cand[c].E), cand[c].flags, ms);
+                                cand[c].T * (N / cand[c].E), cand[c].flags, ms);
         }
     }
     // chunk widths: time whole forwards
@@ -530,7 +579,7 @@ extern "C" int a2afft_plan_autotune(a2afft_plan p, void* work, int iters, void* 
         uint32_t cands[16];
         int nc = 0;
         cands[nc++] = 0;
-        for (uint32_t w = 16; w < uint32_t(p->d.Ng) && nc < 16; w *= 2)
+        for (uint32_t w = 16; w < uint32_t(p->n[2]) && nc < 16; w *= 2)
             if (chunk_ok(p, w)) cands[nc++] = w;
         const uint32_t keep = p->chunk;
         float best = 1e30f;
@@ -566,7 +615,7 @@ extern "C" int a2afft_plan_describe(a2afft_plan p, char* buf, int len) {
     int n = 0;
     for (int i = 0; i < 3 && n < len; ++i) {
         FftShape sh;
-        if (fft_pass_shape(p->pass[i], p->d.dtype, p->d.Ng, &p->sel[i], &sh) < 0) continue;
+        if (fft_pass_shape(p->pass[i], p->d.dtype, int(p->pass[i].fft.n), &p->sel[i], &sh) < 0) continue;
         n += snprintf(buf + n, len - n, "chosen %s T=%d E=%d NT=%d flags=%d grid=%d\n", names[i], sh.T, sh.E,
                       sh.NT, sh.flags, sh.grid);
     }

@@ -35,6 +35,7 @@ int register_kernels_f32(KEntry* out) {
     add_shape<float, 2048, 16, 4, false>(out, &n);   // 512 threads, 64 KiB LDS
     add_shape<float, 2048, 32, 4, false>(out, &n);   // 256 threads, 64 KiB LDS
     add_shape<float, 2048, 16, 1, false>(out, &n);   // 128 threads, 16 KiB LDS
+    add_shape<float, 2048, 32, 8, true>(out, &n);    // 512 threads, 128 KiB LDS, 64 B rows
     return n;
 }
 

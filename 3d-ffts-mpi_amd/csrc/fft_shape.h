@@ -191,11 +191,12 @@ struct PassMaps {
     int strided;
 };
 
-inline void fused_pass_maps(int Ng, const int dims[3], const int lgs[3], int64_t nlocal, PassMaps out[3]) {
+// n: global extents (x, y, z); a single C-order grid is dims [1,1,1].
+inline void fused_pass_maps(const int n[3], const int dims[3], const int lgs[3], int64_t nlocal, PassMaps out[3]) {
     const int64_t lx = lgs[0], ly = lgs[1], lz = lgs[2];
-    const AxisMap ax = make_axis(uint32_t(lx), uint32_t(Ng), int64_t(dims[1]) * dims[2] * nlocal, ly * lz);
-    const AxisMap ay = make_axis(uint32_t(ly), uint32_t(Ng), int64_t(dims[2]) * nlocal, lz);
-    const AxisMap az = make_axis(uint32_t(lz), uint32_t(Ng), nlocal, 1);
+    const AxisMap ax = make_axis(uint32_t(lx), uint32_t(n[0]), int64_t(dims[1]) * dims[2] * nlocal, ly * lz);
+    const AxisMap ay = make_axis(uint32_t(ly), uint32_t(n[1]), int64_t(dims[2]) * nlocal, lz);
+    const AxisMap az = make_axis(uint32_t(lz), uint32_t(n[2]), nlocal, 1);
     out[0] = PassMaps{az, ax, ay, 0};
     out[1] = PassMaps{ax, ay, az, 1};
     out[2] = PassMaps{ay, ax, az, 1};

@@ -27,22 +27,24 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def make_grf(a2afft, torch, Ng, seed, slope=-3.0, noise=1e-3):
-    """The GRF input on the device (complex128 global C-order grid)."""
+def make_grf(a2afft, torch, shape, seed, slope=-3.0, noise=1e-3):
+    """The GRF input on the device (complex128 global C-order grid): white
+    noise times |k|^(slope/2) on integer wavenumbers, inverse FFT in FP64 with
+    the library's own FP64 path, unit RMS, plus noise*(U + iU)."""
     dev = "cuda"
-    n = Ng ** 3
+    if isinstance(shape, int):
+        shape = (shape,) * 3
+    n = shape[0] * shape[1] * shape[2]
     w = torch.empty(n, dtype=torch.complex128, device=dev)
     a2afft.fill_random(w, seed, "normal")
-    k = torch.fft.fftfreq(Ng, 1.0 / Ng, dtype=torch.float64, device=dev)
-    k2 = (k[:, None, None] ** 2 + k[None, :, None] ** 2 + k[None, None, :] ** 2).reshape(-1)
+    ks = [torch.arange(m, dtype=torch.float64, device=dev) for m in shape]
+    ks = [torch.where(k < (m + 1) // 2, k, k - m) for k, m in zip(ks, shape)]   # integer wavenumbers
+    k2 = (ks[0][:, None, None] ** 2 + ks[1][None, :, None] ** 2 + ks[2][None, None, :] ** 2).reshape(-1)
     amp = torch.where(k2 >= 1.0, k2.clamp(min=1.0) ** (slope / 4.0), torch.zeros_like(k2))
     w *= amp
     del amp, k2
-    # inverse FFT in FP64 with the library (one rank, dims [1,1,1]); includes 1/Ng^3
-    dist = a2afft.Distribution(Ng, 1, dims=(1, 1, 1), dtype="c128")
-    plan = a2afft.Dfft(dist, precision="fp64", exec="fused")
-    plan.make_plans(None)
-    w = w.reshape(1, n)
+    plan = a2afft.GridDfft(shape, "fp64")
+    w = w.reshape(shape)
     plan.inverse(w)
     rms = torch.sqrt(torch.mean(w.real ** 2 + w.imag ** 2))
     w /= rms
@@ -50,7 +52,7 @@ def make_grf(a2afft, torch, Ng, seed, slope=-3.0, noise=1e-3):
     a2afft.fill_random(u, seed, "uniform")
     w += noise * u
     del u
-    return w.reshape(Ng, Ng, Ng)
+    return w.reshape(shape)
 
 
 def cpu_baseline(Ng, P, dims, threads, seed):
@@ -72,14 +74,98 @@ def cpu_baseline(Ng, P, dims, threads, seed):
     return t * 1e3, O.get_threads()
 
 
+def _events_ms(torch, fn, steps):
+    """Average ms of fn() over `steps` calls, HIP events on the launch stream."""
+    fn()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(steps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps
+
+
+def run_c3b(args, torch, a2afft):
+    """One rank's share of a 2048^3 grid over 64 ranks, dims [1,8,8]: the
+    block [2048][256][256] complex64 (GRF, seed 2, on that shape) through the
+    x stage.  Staged as the reference: pack (x_a2a_to_x_pencils) -> batched
+    2048-point FFT -> unpack (x_pencils_to_x_a2a), on the pencil layout the
+    exchange delivers; fused: the FFT kernel alone on those pencils (with
+    dims[0] = 1 pack and unpack are the identity), and the strided x pass
+    straight on the block."""
+    shape, P, dims, seed = (2048, 256, 256), 64, (1, 8, 8), 2
+    lgs = shape
+    nloc = shape[0] * shape[1] * shape[2]
+    log("[bench] generating 2048x256x256 GRF share")
+    g = make_grf(a2afft, torch, shape, seed)
+    block = g.to(torch.complex64)
+    del g
+    # pencils as the exchange delivers them: fast_z_to_x ([lx][ly][lz] -> [lz][ly][lx])
+    pencils = torch.empty_like(block)
+    a2afft.launch_remap(6, block, pencils, P, nloc, lgs, dims, nbuffers=1)
+    torch.cuda.synchronize()
+    # FP64 references
+    ref_p = pencils.to(torch.complex128)
+    a2afft.forward_1d_fft(ref_p, 2048)
+    work = pencils.clone()
+    a2afft.forward_1d_fft(work, 2048)
+    rel_p, max_p = a2afft.error_metrics(work, ref_p)
+    del ref_p
+    plan = a2afft.GridDfft(shape, "fp32")
+    ref_b = block.to(torch.complex128)
+    plan64 = a2afft.GridDfft(shape, "fp64")
+    a2afft.execute_pass(plan64, ref_b, 1, -1)
+    wb = block.clone()
+    a2afft.execute_pass(plan, wb, 1, -1)
+    rel_b, max_b = a2afft.error_metrics(wb, ref_b)
+    del ref_b, plan64
+    if args.autotune:
+        plan.autotune(wb, iters=3)
+    log(plan.describe())
+    tmp = torch.empty_like(pencils)
+
+    def staged():
+        a2afft.launch_remap(2, pencils, tmp, P, nloc, lgs, dims, nbuffers=1)
+        a2afft.forward_1d_fft(tmp, 2048)
+        a2afft.launch_remap(3, tmp, work, P, nloc, lgs, dims, nbuffers=1)
+
+    t_staged = _events_ms(torch, staged, args.steps)
+    t_pencil = _events_ms(torch, lambda: a2afft.forward_1d_fft(work, 2048), args.steps)
+    t_block = _events_ms(torch, lambda: a2afft.execute_pass(plan, wb, 1, -1), args.steps)
+    best = min(t_pencil, t_block)
+    bytes_fft = 2 * nloc * 8
+    ach = bytes_fft / (t_pencil * 1e-3) / 1e9
+    out = {
+        "metric": "ms per x stage of one 2048x256x256 rank share (c2c, 2048-point lines)",
+        "value": round(best, 4), "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": 1,
+        "ms_per_step": round(best, 4), "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "one rank's 2048x256x256 complex64 share of 2048^3 over dims [1,8,8], "
+                               "GRF P(k)~k^-3 seed 2 + 1e-3 noise, x stage", "name": "c3b"},
+        "ms": {"staged_pack_fft_unpack": round(t_staged, 4), "fused_pencil_fft": round(t_pencil, 4),
+               "fused_block_x_pass": round(t_block, 4)},
+        "gflops": round(5.0 * nloc * math.log2(2048) / (best * 1e-3) / 1e9, 1),
+        "rel_l2_vs_fp64": {"pencils": rel_p, "block": rel_b},
+        "max_abs_vs_fp64": {"pencils": max_p, "block": max_b},
+        "roofline": {"bound": "hbm", "kernel": "stockham 2048 contiguous", "bytes_per_launch": bytes_fft,
+                     "launch_ms": round(t_pencil, 4), "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--config", default="c3", choices=["c3", "c2"],
+    ap.add_argument("--config", default="c3", choices=["c3", "c2", "c3b"],
                     help="c3: 512^3 complex64 over dims [1,4,4], GRF seed 2 (the headline); "
-                         "c2: 256^3 complex128 over dims [1,2,4], N(0,1) seed 1")
+                         "c2: 256^3 complex128 over dims [1,2,4], N(0,1) seed 1; "
+                         "c3b: one rank's 2048x256x256 complex64 share of 2048^3 over dims [1,8,8], x stage")
     ap.add_argument("--precision", default=None, choices=["fp32", "fp64"])
     ap.add_argument("--ng", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -97,6 +183,8 @@ def main():
     import torch
     import a2afft
 
+    if args.config == "c3b":
+        return run_c3b(args, torch, a2afft)
     if args.config == "c2":
         Ng, P, dims, seed, field = args.ng or 256, 8, (1, 2, 4), 1, "normal"
         args.precision = args.precision or "fp64"

@@ -168,6 +168,13 @@ int a2afft_launch_fast_copy_fft(void* out, const void* data, int64_t n_elems,
  * execution strategy.  Builds twiddle tables and kernel choices. */
 int a2afft_plan_create(a2afft_plan* out, a2afft_dist dist, a2afft_precision precision,
                        a2afft_exec exec);
+/* A fused plan over one nx x ny x nz grid in C order (z fastest), without a
+ * rank decomposition; no reference counterpart (the reference is cubic, one
+ * grid per distribution, a2adistribution.h:9).  Each extent a power of two
+ * in 2..4096; complex128 for FP64, complex64 otherwise.  make_plans is not
+ * needed; forward / inverse / autotune / execute_* apply as for other fused
+ * plans, and the inverse scales by 1/(nx*ny*nz). */
+int a2afft_grid_plan_create(a2afft_plan* out, int nx, int ny, int nz, a2afft_precision precision);
 /* a2adfft::make_plans(d_scratch) (a2adfft.cpp:47-53).  scratch is a
  * [P][nlocal] device buffer; the staged strategy needs it, the fused one
  * accepts NULL. */

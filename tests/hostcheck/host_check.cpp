@@ -49,7 +49,8 @@ int64_t hc_fft_enumerate(int Ng, int P, const int* dims, int dbl, int pass, int 
         int lgs[3] = {Ng / dims[0], Ng / dims[1], Ng / dims[2]};
         int64_t nlocal = int64_t(lgs[0]) * lgs[1] * lgs[2];
         PassMaps all[3];
-        fused_pass_maps(Ng, dims, lgs, nlocal, all);
+        const int n3[3] = {Ng, Ng, Ng};
+        fused_pass_maps(n3, dims, lgs, nlocal, all);
         pm = all[pass];
     }
     if (T == 0) {
@@ -97,7 +98,8 @@ int hc_shape_fits(int Ng, int P, const int* dims, int pass, int T, int E) {
         int lgs[3] = {Ng / dims[0], Ng / dims[1], Ng / dims[2]};
         int64_t nlocal = int64_t(lgs[0]) * lgs[1] * lgs[2];
         PassMaps all[3];
-        fused_pass_maps(Ng, dims, lgs, nlocal, all);
+        const int n3[3] = {Ng, Ng, Ng};
+        fused_pass_maps(n3, dims, lgs, nlocal, all);
         pm = all[pass];
     }
     return shape_fits(pm.fft, pm.a2, pm.strided, Ng, T, E) ? 1 : 0;

@@ -179,3 +179,26 @@ def test_chunked_and_tuned_plans_bit_identical(a2afft_mod, cuda, cfg):
         tuned.execute_launch(y, i, -1)
     torch.cuda.synchronize()
     assert np.array_equal(_bits(y.cpu().numpy()), _bits(ref.cpu().numpy()))
+
+
+@pytest.mark.parametrize("shape,prec", [((32, 64, 16), "fp64"), ((128, 256, 64), "fp32"), ((64, 32, 128), "fp64"),
+                                        ((16, 512, 8), "fp32"), ((2, 4, 2048), "fp64")],
+                         ids=lambda v: str(v))
+def test_grid_plan_non_cubic_vs_numpy(a2afft_mod, cuda, shape, prec):
+    """Single-grid plans (no decomposition) of non-cubic shapes, the shapes of
+    the batched configuration, against numpy's fftn in FP64."""
+    import torch
+    from a2afft import inputs
+    n = int(np.prod(shape))
+    g = inputs.uniform(11, n).reshape(shape)
+    x = g.astype(np.complex128 if prec == "fp64" else np.complex64)
+    plan = a2afft_mod.GridDfft(shape, prec)
+    d = torch.from_numpy(x).cuda()
+    plan.forward(d)
+    torch.cuda.synchronize()
+    ref = np.fft.fftn(x.astype(np.complex128))
+    tol = 2e-15 if prec == "fp64" else 6e-7
+    assert _relerr(d.cpu().numpy(), ref) < tol
+    plan.inverse(d)
+    torch.cuda.synchronize()
+    assert _relerr(d.cpu().numpy(), x.astype(np.complex128)) < 2 * tol
