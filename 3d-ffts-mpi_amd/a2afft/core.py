@@ -221,6 +221,70 @@ class GridDfft(_Plan):
         return torch.empty(self.shape, dtype=self.torch_dtype, device=device)
 
 
+class Batch:
+    """Many independent complex64 C-order grids, one persistent launch per transform."""
+
+    def __init__(self, shapes, precision="fp32"):
+        self.shapes = [tuple(int(v) for v in s) for s in shapes]
+        flat = [v for s in self.shapes for v in s]
+        self._h = ctypes.c_void_p()
+        L.check(L.lib().a2afft_batch_create(ctypes.byref(self._h), len(self.shapes), L.ints(flat),
+                                            PRECISIONS[precision]), "batch_create")
+        offs = (ctypes.c_int64 * (len(self.shapes) + 1))()
+        L.check(L.lib().a2afft_batch_offsets(self._h, offs), "batch_offsets")
+        self.offsets = list(offs)
+        self.total = self.offsets[-1]
+
+    def empty(self, device="cuda"):
+        return torch.empty(self.total, dtype=torch.complex64, device=device)
+
+    def views(self, buf):
+        """Per-grid (nx, ny, nz) views of a batch buffer."""
+        return [buf[self.offsets[i]:self.offsets[i + 1]].view(*s) for i, s in enumerate(self.shapes)]
+
+    def _check(self, b):
+        if b.dtype != torch.complex64 or b.numel() != self.total or not b.is_contiguous() or not b.is_cuda:
+            raise ValueError(f"batch buffer must be a contiguous device complex64 tensor of {self.total} elements")
+
+    def execute(self, data, direction=-1):
+        self._check(data)
+        return L.check(L.lib().a2afft_batch_execute(self._h, _ptr(data), direction, _stream()), "batch_execute")
+
+    def forward(self, data):
+        return self.execute(data, -1)
+
+    def inverse(self, data):
+        return self.execute(data, 1)
+
+    def graph_launch(self, data, direction=-1):
+        """Replay (capturing on first use) the hipGraph of this transform for `data`."""
+        self._check(data)
+        return L.check(L.lib().a2afft_batch_graph_launch(self._h, _ptr(data), direction, _stream()),
+                       "batch_graph_launch")
+
+    def status(self):
+        return L.check(L.lib().a2afft_batch_status(self._h, _stream()), "batch_status")
+
+    def set_workgroups(self, n):
+        return L.check(L.lib().a2afft_batch_set_workgroups(self._h, n), "batch_set_workgroups")
+
+    def info(self):
+        u, w, m = ctypes.c_int64(), ctypes.c_int(), ctypes.c_int()
+        L.check(L.lib().a2afft_batch_info(self._h, ctypes.byref(u), ctypes.byref(w), ctypes.byref(m)), "batch_info")
+        return {"units": u.value, "workgroups": w.value, "max_workgroups": m.value}
+
+    def destroy(self):
+        if self._h:
+            L.lib().a2afft_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
 # ---- launcher level --------------------------------------------------------
 
 _REMAP_FNS = [

@@ -1,0 +1,356 @@
+// Batched transform of many independent grids of uneven shapes in ONE
+// persistent launch (no reference counterpart: the reference transforms one
+// cubic grid per a2aDistribution, a2adistribution.h:9).
+//
+// Work queue.  Every (pass, grid) pair is split into the units its Stockham
+// kernel would use (lines or column tiles) and the units are numbered
+// pass-major: all z units of all grids, then all x units, then all y units.
+// Workgroups take units in that order from an atomic counter.  A unit of
+// pass p of grid g first waits until every unit of pass p-1 of grid g has
+// finished.  Those units were taken earlier, by workgroups that are running
+// and wait on nothing later, so every wait ends; the wait is still bounded,
+// and a bound that trips sets an error flag instead of hanging.
+//
+// Completion counters are released after a device-scope fence by the
+// finishing workgroup and acquired with a device-scope fence (which also
+// invalidates the CU's L1) before the waiting workgroup loads any data.
+#include "fft.h"
+#include "stockham.h"
+
+#include <algorithm>
+#include <new>
+#include <vector>
+
+namespace a2a {
+
+namespace {
+
+constexpr int kBatchThreads = 256;
+constexpr uint32_t kSpinLimit = 1u << 24;
+
+// Kernel shape per (N, strided): primary and a single-column fallback.
+struct BShape {
+    int N, strided, T, E;
+};
+constexpr BShape kShapes[] = {
+    {16, 0, 256, 16}, {32, 0, 64, 8}, {64, 0, 32, 8}, {128, 0, 32, 16}, {256, 0, 16, 16}, {512, 0, 8, 16},
+    {16, 1, 32, 16},  {32, 1, 32, 8}, {64, 1, 32, 8}, {128, 1, 32, 16}, {256, 1, 16, 16}, {512, 1, 8, 16},
+    {16, 1, 1, 16},   {32, 1, 1, 8},  {64, 1, 1, 8},  {128, 1, 1, 16},  {256, 1, 1, 16},  {512, 1, 1, 8},
+};
+constexpr int kNumShapes = int(sizeof(kShapes) / sizeof(kShapes[0]));
+constexpr int kFlags = KF_PAD;
+
+}  // namespace
+
+struct BatchItem {
+    FftGeom g;
+    int64_t base;        // element offset of the grid in the batch buffer
+    const float2* tw;    // twiddles of this pass's length and direction
+    uint32_t units;
+    int shape;           // index into kShapes
+};
+
+template <int DIR>
+__device__ __forceinline__ void run_unit(int shape, float2* data, const FftGeom& g, const float2* tw, uint32_t u,
+                                         float2* lds, int tid) {
+    switch (shape) {
+#define A2A_B(I)                                                                                                \
+    case I:                                                                                                    \
+        stockham_unit<float, kShapes[I].N, kShapes[I].E, kShapes[I].T, kShapes[I].strided != 0, DIR, kFlags>( \
+            data, g, tw, u, lds, tid);                                                                         \
+        break;
+        A2A_B(0) A2A_B(1) A2A_B(2) A2A_B(3) A2A_B(4) A2A_B(5) A2A_B(6) A2A_B(7) A2A_B(8)
+        A2A_B(9) A2A_B(10) A2A_B(11) A2A_B(12) A2A_B(13) A2A_B(14) A2A_B(15) A2A_B(16) A2A_B(17)
+#undef A2A_B
+    default:
+        break;
+    }
+}
+
+template <int I>
+constexpr int batch_lds_max() {
+    if constexpr (I == kNumShapes) {
+        return 1;
+    } else {
+        constexpr int here = lds_slots<kShapes[I].N, kShapes[I].T, kShapes[I].strided != 0, 1>();
+        constexpr int rest = batch_lds_max<I + 1>();
+        return here > rest ? here : rest;
+    }
+}
+
+// ctrl[0]: next unit, ctrl[1]: error flag, ctrl[2 + k]: finished units of
+// (pass, grid) pair k = pass * G + grid.
+template <int DIR>
+__global__ __launch_bounds__(kBatchThreads) void batch_kernel(float2* __restrict__ data,
+                                                              const BatchItem* __restrict__ items,
+                                                              const uint32_t* __restrict__ prefix, int K, int G,
+                                                              uint32_t total, uint32_t* __restrict__ ctrl) {
+    __shared__ float2 lds[batch_lds_max<0>()];
+    __shared__ uint32_t s_unit;
+    __shared__ int s_k;
+    const int tid = int(threadIdx.x);
+    for (;;) {
+        __syncthreads();   // the previous unit is done with LDS and s_unit
+        if (tid == 0) {
+            const uint32_t unit = atomicAdd(&ctrl[0], 1u);
+            int k = -1;
+            if (unit < total) {
+                int lo = 0, hi = K;   // largest k with prefix[k] <= unit
+                while (hi - lo > 1) {
+                    const int mid = (lo + hi) >> 1;
+                    if (prefix[mid] <= unit) lo = mid;
+                    else hi = mid;
+                }
+                k = lo;
+                if (k >= G) {   // wait for the previous pass of this grid
+                    const uint32_t need = items[k - G].units;
+                    uint32_t spins = 0;
+                    while (__hip_atomic_load(&ctrl[2 + k - G], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
+                        __builtin_amdgcn_s_sleep(4);
+                        if (++spins > kSpinLimit) {
+                            __hip_atomic_store(&ctrl[1], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            break;
+                        }
+                    }
+                }
+            }
+            s_unit = unit;
+            s_k = k;
+        }
+        __syncthreads();
+        const int k = __builtin_amdgcn_readfirstlane(s_k);
+        if (k < 0) break;
+        __threadfence();   // acquire: this workgroup's loads see the finished pass
+        const BatchItem& it = items[k];
+        const uint32_t u = uint32_t(__builtin_amdgcn_readfirstlane(int(s_unit - prefix[k])));
+        run_unit<DIR>(it.shape, data + it.base, it.g, it.tw, u, lds, tid);
+        __threadfence();   // release this thread's stores
+        __syncthreads();
+        if (tid == 0) atomicAdd(&ctrl[2 + k], 1u);
+    }
+}
+
+__global__ void batch_reset_kernel(uint32_t* ctrl, int n) {
+    for (int i = int(threadIdx.x); i < n; i += int(blockDim.x)) ctrl[i] = 0u;
+}
+
+}  // namespace a2a
+
+using namespace a2a;
+
+struct a2afft_batch_s {
+    int G = 0;
+    std::vector<int> shapes;          // 3 per grid
+    std::vector<int64_t> offsets;     // G + 1
+    BatchItem* d_items[2] = {nullptr, nullptr};   // forward, inverse
+    uint32_t* d_prefix = nullptr;
+    uint32_t* d_ctrl = nullptr;
+    uint32_t total = 0;
+    int ctrl_len = 0;
+    int grid = 0;                     // persistent workgroups
+    int max_grid = 0;
+    hipStream_t cap_stream = nullptr;
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    void* graph_data[2] = {nullptr, nullptr};
+};
+
+static void batch_free(a2afft_batch_s* b) {
+    if (!b) return;
+    for (int d = 0; d < 2; ++d) {
+        if (b->d_items[d]) (void)hipFree(b->d_items[d]);
+        if (b->graph[d]) (void)hipGraphExecDestroy(b->graph[d]);
+    }
+    if (b->d_prefix) (void)hipFree(b->d_prefix);
+    if (b->d_ctrl) (void)hipFree(b->d_ctrl);
+    if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
+    delete b;
+}
+
+static int pick_shape(const AxisPass& p, int N) {
+    for (int i = 0; i < kNumShapes; ++i) {
+        const BShape& s = kShapes[i];
+        if (s.N == N && s.strided == p.strided && shape_fits(p.fft, p.a2, p.strided, N, s.T, s.E)) return i;
+    }
+    return -1;
+}
+
+extern "C" int a2afft_batch_create(a2afft_batch* out, int ngrids, const int* shapes, a2afft_precision prec) {
+    if (!out || ngrids <= 0 || !shapes) return set_error(A2AFFT_E_INVALID_ARG, "batch_create: bad arguments");
+    *out = nullptr;
+    if (prec != A2AFFT_FP32) return set_error(A2AFFT_E_UNSUPPORTED, "batch_create: FP32 (complex64) only");
+    a2afft_batch_s* b = new (std::nothrow) a2afft_batch_s;
+    if (!b) return set_error(A2AFFT_E_ALLOC, "batch_create");
+    b->G = ngrids;
+    b->shapes.assign(shapes, shapes + 3 * ngrids);
+    b->offsets.assign(ngrids + 1, 0);
+    const int K = 3 * ngrids;
+    std::vector<BatchItem> items[2];
+    items[0].resize(K);
+    items[1].resize(K);
+    std::vector<uint32_t> prefix(K + 1, 0);
+    for (int g = 0; g < ngrids; ++g) {
+        const int* n = shapes + 3 * g;
+        const int64_t total = int64_t(n[0]) * n[1] * n[2];
+        for (int a = 0; a < 3; ++a)
+            if (n[a] < 16 || n[a] > 512 || (n[a] & (n[a] - 1))) {
+                batch_free(b);
+                return set_error(A2AFFT_E_UNSUPPORTED, "batch: grid %d extent %d (powers of two 16..512)", g, n[a]);
+            }
+        b->offsets[g + 1] = b->offsets[g] + total;
+        const int dims[3] = {1, 1, 1};
+        PassMaps pm[3];
+        fused_pass_maps(n, dims, n, total, pm);
+        for (int p = 0; p < 3; ++p) {
+            const int k = p * ngrids + g;
+            AxisPass ps{pm[p].fft, pm[p].a1, pm[p].a2, pm[p].strided, 0, 1.0};
+            const int N = int(ps.fft.n);
+            const int si = pick_shape(ps, N);
+            if (si < 0) {
+                batch_free(b);
+                return set_error(A2AFFT_E_UNSUPPORTED, "batch: no kernel shape for grid %d pass %d", g, p);
+            }
+            for (int d = 0; d < 2; ++d) {
+                BatchItem& it = items[d][k];
+                uint32_t units = 0;
+                int st = resolve_with(ps.fft, ps.a1, ps.a2, ps.strided, N, kShapes[si].T, kShapes[si].E, &it.g, &units);
+                if (st < 0) {
+                    batch_free(b);
+                    return set_error(st, "batch: geometry of grid %d pass %d", g, p);
+                }
+                it.g.scale_mode = (d == 1 && p == 2) ? 2 : 0;
+                it.g.scale = (d == 1 && p == 2) ? double(total) : 1.0;
+                it.base = b->offsets[g];
+                it.units = units;
+                it.shape = si;
+                it.tw = (const float2*)twiddle_table(N, d == 0 ? -1 : 1, A2AFFT_C64);
+                if (!it.tw) {
+                    batch_free(b);
+                    return set_error(A2AFFT_E_ALLOC, "batch: twiddle table N=%d", N);
+                }
+            }
+        }
+    }
+    for (int k = 0; k < K; ++k) {
+        const uint64_t next = uint64_t(prefix[k]) + items[0][k].units;
+        if (next >= (uint64_t(1) << 32)) {
+            batch_free(b);
+            return set_error(A2AFFT_E_SIZE, "batch: too many work units");
+        }
+        prefix[k + 1] = uint32_t(next);
+    }
+    b->total = prefix[K];
+    b->ctrl_len = 2 + K;
+    int st = A2AFFT_OK;
+    for (int d = 0; d < 2 && st == A2AFFT_OK; ++d) {
+        st = hip_check(int(hipMalloc(&b->d_items[d], sizeof(BatchItem) * K)), "batch alloc");
+        if (st == A2AFFT_OK)
+            st = hip_check(int(hipMemcpy(b->d_items[d], items[d].data(), sizeof(BatchItem) * K,
+                                         hipMemcpyHostToDevice)), "batch copy");
+    }
+    if (st == A2AFFT_OK) st = hip_check(int(hipMalloc(&b->d_prefix, sizeof(uint32_t) * (K + 1))), "batch alloc");
+    if (st == A2AFFT_OK)
+        st = hip_check(int(hipMemcpy(b->d_prefix, prefix.data(), sizeof(uint32_t) * (K + 1), hipMemcpyHostToDevice)),
+                       "batch copy");
+    if (st == A2AFFT_OK) st = hip_check(int(hipMalloc(&b->d_ctrl, sizeof(uint32_t) * b->ctrl_len)), "batch alloc");
+    int dev = 0, cus = 0, occ = 0;
+    if (st == A2AFFT_OK) st = hip_check(int(hipGetDevice(&dev)), "batch device");
+    if (st == A2AFFT_OK)
+        st = hip_check(int(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)), "batch CUs");
+    if (st == A2AFFT_OK)
+        st = hip_check(int(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, batch_kernel<-1>, kBatchThreads, 0)),
+                       "batch occupancy");
+    if (st == A2AFFT_OK) st = hip_check(int(hipStreamCreate(&b->cap_stream)), "batch stream");
+    if (st < 0) {
+        batch_free(b);
+        return st;
+    }
+    b->max_grid = cus * (occ > 0 ? occ : 1);
+    b->grid = b->max_grid;
+    *out = b;
+    return A2AFFT_OK;
+}
+
+extern "C" int a2afft_batch_offsets(a2afft_batch b, int64_t* offsets) {
+    if (!b || !offsets) return set_error(A2AFFT_E_INVALID_ARG, "batch_offsets: null");
+    for (int g = 0; g <= b->G; ++g) offsets[g] = b->offsets[g];
+    return A2AFFT_OK;
+}
+
+static int batch_launch(a2afft_batch_s* b, void* data, int dir, hipStream_t s) {
+    hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(256), 0, s, b->d_ctrl, b->ctrl_len);
+    A2A_TRY(hip_check(int(hipGetLastError()), "batch reset launch"));
+    const int K = 3 * b->G;
+    if (dir < 0)
+        hipLaunchKernelGGL(batch_kernel<-1>, dim3(b->grid), dim3(kBatchThreads), 0, s, (float2*)data, b->d_items[0],
+                           b->d_prefix, K, b->G, b->total, b->d_ctrl);
+    else
+        hipLaunchKernelGGL(batch_kernel<1>, dim3(b->grid), dim3(kBatchThreads), 0, s, (float2*)data, b->d_items[1],
+                           b->d_prefix, K, b->G, b->total, b->d_ctrl);
+    return hip_check(int(hipGetLastError()), "batch launch");
+}
+
+extern "C" int a2afft_batch_execute(a2afft_batch b, void* data, int direction, void* stream) {
+    if (!b || !data || (direction != -1 && direction != 1))
+        return set_error(A2AFFT_E_INVALID_ARG, "batch_execute: bad arguments");
+    return finish_call(stream, batch_launch(b, data, direction, (hipStream_t)stream));
+}
+
+// Captures reset + persistent launch for `data` into a hipGraph (one per
+// direction); a later capture for another buffer replaces it.
+extern "C" int a2afft_batch_graph_launch(a2afft_batch b, void* data, int direction, void* stream) {
+    if (!b || !data || (direction != -1 && direction != 1))
+        return set_error(A2AFFT_E_INVALID_ARG, "batch_graph_launch: bad arguments");
+    const int d = direction < 0 ? 0 : 1;
+    if (!b->graph[d] || b->graph_data[d] != data) {
+        if (b->graph[d]) (void)hipGraphExecDestroy(b->graph[d]);
+        b->graph[d] = nullptr;
+        hipGraph_t g = nullptr;
+        A2A_TRY(hip_check(int(hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal)), "capture"));
+        int st = batch_launch(b, data, direction, b->cap_stream);
+        int st2 = hip_check(int(hipStreamEndCapture(b->cap_stream, &g)), "end capture");
+        if (st < 0 || st2 < 0) {
+            if (g) (void)hipGraphDestroy(g);
+            return st < 0 ? st : st2;
+        }
+        st = hip_check(int(hipGraphInstantiate(&b->graph[d], g, nullptr, nullptr, 0)), "graph instantiate");
+        (void)hipGraphDestroy(g);
+        if (st < 0) return st;
+        b->graph_data[d] = data;
+    }
+    return finish_call(stream, hip_check(int(hipGraphLaunch(b->graph[d], (hipStream_t)stream)), "graph launch"));
+}
+
+// 0 when the last execution finished normally, 1 when a dependency wait hit
+// its bound.  Synchronises the stream.
+extern "C" int a2afft_batch_status(a2afft_batch b, void* stream) {
+    if (!b) return set_error(A2AFFT_E_INVALID_ARG, "batch_status: null");
+    A2A_TRY(hip_check(int(hipStreamSynchronize((hipStream_t)stream)), "batch sync"));
+    uint32_t h[2] = {0, 0};
+    A2A_TRY(hip_check(int(hipMemcpy(h, b->d_ctrl, sizeof(h), hipMemcpyDeviceToHost)), "batch status copy"));
+    if (h[0] < b->total) return set_error(A2AFFT_E_HIP, "batch: %u of %u units taken", h[0], b->total);
+    return h[1] ? set_error(A2AFFT_E_HIP, "batch: a dependency wait timed out") : A2AFFT_OK;
+}
+
+// Persistent workgroups per launch (0 = one per resident slot on every CU).
+extern "C" int a2afft_batch_set_workgroups(a2afft_batch b, int workgroups) {
+    if (!b || workgroups < 0) return set_error(A2AFFT_E_INVALID_ARG, "batch_set_workgroups: bad arguments");
+    b->grid = workgroups == 0 ? b->max_grid : std::min(workgroups, b->max_grid);
+    for (int d = 0; d < 2; ++d) {
+        if (b->graph[d]) (void)hipGraphExecDestroy(b->graph[d]);
+        b->graph[d] = nullptr;
+    }
+    return A2AFFT_OK;
+}
+
+extern "C" int a2afft_batch_info(a2afft_batch b, int64_t* total_units, int* workgroups, int* max_workgroups) {
+    if (!b) return set_error(A2AFFT_E_INVALID_ARG, "batch_info: null");
+    if (total_units) *total_units = b->total;
+    if (workgroups) *workgroups = b->grid;
+    if (max_workgroups) *max_workgroups = b->max_grid;
+    return A2AFFT_OK;
+}
+
+extern "C" int a2afft_batch_destroy(a2afft_batch b) {
+    batch_free(b);
+    return A2AFFT_OK;
+}

@@ -243,26 +243,28 @@ __device__ __forceinline__ void stockham_passes(typename CT<real>::V (&v)[E], ty
     }
 }
 
-// One workgroup = T lines (contiguous) or one tile of T columns (strided).
-// XCD = 1 renumbers workgroups so each of the 8 XCDs (which receive
-// workgroups round-robin) works through a contiguous range of tiles.
+// One work unit: T lines (contiguous) or one tile of T columns (strided),
+// loaded, transformed and stored by the NT = T*N/E threads of a workgroup.
+// `block` is the unit index; `lds` holds lds_slots() complex values.  Threads
+// with tid >= NT take part in the barriers only.
 template <typename real, int N, int E, int T, bool STRIDED, int DIR, int FLAGS>
-__global__ __launch_bounds__(T * (N / E)) void stockham_kernel(typename CT<real>::V* __restrict__ data, FftGeom g,
-                                                               const typename CT<real>::V* __restrict__ tw) {
+__device__ __forceinline__ void stockham_unit(typename CT<real>::V* __restrict__ data, const FftGeom& g,
+                                              const typename CT<real>::V* __restrict__ tw, uint32_t block,
+                                              typename CT<real>::V* lds, int tid) {
     using V = typename CT<real>::V;
     constexpr int TPL = N / E;
-    constexpr int XCD = (FLAGS & KF_XCD) ? 1 : 0;
+    constexpr int NT = T * TPL;
     constexpr int PAD = (FLAGS & KF_PAD) ? 1 : 0;
     constexpr bool NTMP = (FLAGS & KF_NT) != 0;
     constexpr bool A2MAJ = (FLAGS & KF_A2MAJ) != 0;
-    __shared__ V lds[lds_slots<N, T, STRIDED, PAD>()];
 
-    const int tid = threadIdx.x;
-    const int t = STRIDED ? tid / T : tid % TPL;
-    const int line = STRIDED ? tid % T : tid / TPL;
-    const uint32_t block = XCD ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+    const bool active = tid < NT;
+    const int tl = active ? tid : 0;
+    const int t = STRIDED ? tl / T : tl % TPL;
+    const int line = STRIDED ? tl % T : tl / TPL;
     bool valid;
-    const int64_t base = line_base(g, STRIDED, T, TPL, block, tid, &valid, A2MAJ);
+    const int64_t base = line_base(g, STRIDED, T, TPL, block, tl, &valid, A2MAJ);
+    valid = valid && active;
 
     V v[E];
 #pragma unroll
@@ -285,6 +287,19 @@ __global__ __launch_bounds__(T * (N / E)) void stockham_kernel(typename CT<real>
     }
 #pragma unroll
     for (int e = 0; e < E; ++e) st_global<NTMP>(data + base + elem_off(g, e * TPL), v[e]);
+}
+
+// One workgroup per unit.  XCD = 1 renumbers workgroups so each of the 8
+// XCDs (which receive workgroups round-robin) works through a contiguous
+// range of units.
+template <typename real, int N, int E, int T, bool STRIDED, int DIR, int FLAGS>
+__global__ __launch_bounds__(T * (N / E)) void stockham_kernel(typename CT<real>::V* __restrict__ data, FftGeom g,
+                                                               const typename CT<real>::V* __restrict__ tw) {
+    using V = typename CT<real>::V;
+    constexpr int PAD = (FLAGS & KF_PAD) ? 1 : 0;
+    __shared__ V lds[lds_slots<N, T, STRIDED, PAD>()];
+    const uint32_t block = (FLAGS & KF_XCD) ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+    stockham_unit<real, N, E, T, STRIDED, DIR, FLAGS>(data, g, tw, block, lds, int(threadIdx.x));
 }
 
 // Registry of built kernel variants (filled by the per-precision files).

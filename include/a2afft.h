@@ -185,6 +185,26 @@ int a2afft_forward(a2afft_plan plan, void* d_data, void* stream);
 int a2afft_inverse(a2afft_plan plan, void* d_data, void* stream);
 int a2afft_plan_destroy(a2afft_plan plan);
 
+/* ---- batched grids (no reference counterpart) ---------------------------- */
+/* Many independent C-order grids of uneven shapes, complex64, forward or
+ * inverse in ONE persistent launch (the reference transforms one cubic grid
+ * per distribution, a2adistribution.h:9).  shapes: ngrids x {nx, ny, nz},
+ * powers of two 16..512.  The grids sit back to back in one buffer; offsets
+ * (ngrids + 1 entries, in complex elements) says where.  execute is
+ * stream-ordered; graph_launch captures the same work into a hipGraph on first
+ * use for a given buffer and direction and replays it.  status synchronises
+ * and reports whether every dependency wait finished within its bound. */
+typedef struct a2afft_batch_s* a2afft_batch;
+int a2afft_batch_create(a2afft_batch* out, int ngrids, const int* shapes, a2afft_precision precision);
+int a2afft_batch_offsets(a2afft_batch batch, int64_t* offsets);
+int a2afft_batch_execute(a2afft_batch batch, void* d_data, int direction, void* stream);
+int a2afft_batch_graph_launch(a2afft_batch batch, void* d_data, int direction, void* stream);
+int a2afft_batch_status(a2afft_batch batch, void* stream);
+/* Persistent workgroups per launch (0 = every resident slot on every CU). */
+int a2afft_batch_set_workgroups(a2afft_batch batch, int workgroups);
+int a2afft_batch_info(a2afft_batch batch, int64_t* total_units, int* workgroups, int* max_workgroups);
+int a2afft_batch_destroy(a2afft_batch batch);
+
 /* ---- measurement helpers (no reference counterpart) --------------------- */
 /* rel_l2 = ||y - ref||_2 / ||ref||_2 and max_abs = max |y - ref| over n
  * complex values, reduced on the device in FP64.  y has dtype; ref is

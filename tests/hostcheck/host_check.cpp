@@ -105,6 +105,41 @@ int hc_shape_fits(int Ng, int P, const int* dims, int pass, int T, int E) {
     return shape_fits(pm.fft, pm.a2, pm.strided, Ng, T, E) ? 1 : 0;
 }
 
+// Every address of pass `pass` of a plain nx x ny x nz C-order grid with
+// shape (T, E); returns the count or a negative status.
+int64_t hc_grid_enumerate(int nx, int ny, int nz, int pass, int T, int E, int64_t* addr, int32_t* n_out,
+                          int64_t* line_out, int64_t cap) {
+    const int n3[3] = {nx, ny, nz};
+    const int dims[3] = {1, 1, 1};
+    PassMaps all[3];
+    fused_pass_maps(n3, dims, n3, int64_t(nx) * ny * nz, all);
+    const PassMaps pm = all[pass];
+    const int N = int(pm.fft.n);
+    FftGeom g;
+    uint32_t grid;
+    int st = resolve_with(pm.fft, pm.a1, pm.a2, pm.strided, N, T, E, &g, &grid);
+    if (st < 0) return st;
+    const int NT = T * (N / E), TPL = N / E;
+    const bool strided = pm.strided != 0;
+    int64_t c = 0;
+    for (uint32_t b = 0; b < grid; ++b)
+        for (int tid = 0; tid < NT; ++tid) {
+            bool valid;
+            const int64_t base = line_base(g, strided, T, TPL, b, tid, &valid);
+            if (!valid) continue;
+            const int t = strided ? tid / T : tid % TPL;
+            const int line = strided ? tid % T : tid / TPL;
+            for (int e = 0; e < E; ++e) {
+                if (c >= cap) return -100;
+                addr[c] = base + elem_off(g, e * TPL);
+                n_out[c] = t + e * TPL;
+                line_out[c] = int64_t(b) * T + line;
+                ++c;
+            }
+        }
+    return c;
+}
+
 void hc_xcd_blocks(uint32_t G, uint32_t* out) {
     for (uint32_t b = 0; b < G; ++b) out[b] = xcd_block(b, G);
 }
