@@ -199,9 +199,12 @@ template <bool NTMP, typename V> __device__ __forceinline__ void st_global(V* p,
 }
 
 // Passes p..NP-1 of the Stockham schedule on the thread's registers.
+// `active` = false for threads beyond the unit's NT (a wider workgroup runs
+// the unit): they take part in the barriers but write nothing to LDS.
 template <typename real, int N, int E, int T, bool STRIDED, int DIR, int PAD, int p>
 __device__ __forceinline__ void stockham_passes(typename CT<real>::V (&v)[E], typename CT<real>::V* lds,
-                                                const typename CT<real>::V* __restrict__ tw, int t, int line) {
+                                                const typename CT<real>::V* __restrict__ tw, int t, int line,
+                                                bool active) {
     using V = typename CT<real>::V;
     constexpr int NP = npass_of(N);
     if constexpr (p < NP) {
@@ -228,18 +231,20 @@ __device__ __forceinline__ void stockham_passes(typename CT<real>::V (&v)[E], ty
         }
         if constexpr (p + 1 < NP) {
             if constexpr (p > 0) __syncthreads();
+            if (active) {
 #pragma unroll
-            for (int b = 0; b < NB; ++b) {
-                const int j = t + b * TPL;
-                const int d0 = (j / Ns) * (Ns * R) + (j & (Ns - 1));
+                for (int b = 0; b < NB; ++b) {
+                    const int j = t + b * TPL;
+                    const int d0 = (j / Ns) * (Ns * R) + (j & (Ns - 1));
 #pragma unroll
-                for (int q = 0; q < R; ++q) lds[lds_slot<N, T, STRIDED, PAD>(line, d0 + q * Ns)] = v[b + q * NB];
+                    for (int q = 0; q < R; ++q) lds[lds_slot<N, T, STRIDED, PAD>(line, d0 + q * Ns)] = v[b + q * NB];
+                }
             }
             __syncthreads();
 #pragma unroll
             for (int e = 0; e < E; ++e) v[e] = lds[lds_slot<N, T, STRIDED, PAD>(line, t + e * TPL)];
         }
-        stockham_passes<real, N, E, T, STRIDED, DIR, PAD, p + 1>(v, lds, tw, t, line);
+        stockham_passes<real, N, E, T, STRIDED, DIR, PAD, p + 1>(v, lds, tw, t, line, active);
     }
 }
 
@@ -273,7 +278,7 @@ __device__ __forceinline__ void stockham_unit(typename CT<real>::V* __restrict__
         else { v[e].x = 0; v[e].y = 0; }
     }
 
-    stockham_passes<real, N, E, T, STRIDED, DIR, PAD, 0>(v, lds, tw, t, line);
+    stockham_passes<real, N, E, T, STRIDED, DIR, PAD, 0>(v, lds, tw, t, line, active);
 
     if (!valid) return;
     if (g.scale_mode == 1) {
