@@ -5,15 +5,19 @@
 // Work queue.  Every (pass, grid) pair is split into the units its Stockham
 // kernel would use (lines or column tiles) and the units are numbered
 // pass-major: all z units of all grids, then all x units, then all y units.
-// Workgroups take units in that order from an atomic counter.  A unit of
-// pass p of grid g first waits until every unit of pass p-1 of grid g has
-// finished.  Those units were taken earlier, by workgroups that are running
-// and wait on nothing later, so every wait ends; the wait is still bounded,
-// and a bound that trips sets an error flag instead of hanging.
+// Workgroups take units in that order from an atomic counter.
 //
-// Completion counters are released after a device-scope fence by the
-// finishing workgroup and acquired with a device-scope fence (which also
-// invalidates the CU's L1) before the waiting workgroup loads any data.
+// Synchronisation is per pass, not per unit: the 8 XCDs of the device have
+// L2 caches that are not coherent with one another, so a device-scope fence
+// writes back or invalidates a whole L2.  A workgroup therefore counts the
+// units it finishes and, only when the next unit it takes belongs to a later
+// pass (or the queue is empty), releases its writes with one fence, adds its
+// count to that pass's total, waits until the previous pass's total is
+// complete and acquires with one fence.  That is at most three transitions
+// per workgroup.  Every unit of a pass is taken before any unit of the next
+// pass, by a running workgroup that finishes it without waiting on anything
+// later, so every wait ends; the wait is still bounded, and a bound that
+// trips sets an error flag instead of hanging.
 #include "fft.h"
 #include "stockham.h"
 
@@ -78,17 +82,20 @@ constexpr int batch_lds_max() {
     }
 }
 
-// ctrl[0]: next unit, ctrl[1]: error flag, ctrl[2 + k]: finished units of
-// (pass, grid) pair k = pass * G + grid.
+// ctrl[0]: next unit, ctrl[1]: error flag, ctrl[2 + p]: units of pass p
+// finished and released; ptotal[p]: units of pass p.
 template <int DIR>
 __global__ __launch_bounds__(kBatchThreads) void batch_kernel(float2* __restrict__ data,
                                                               const BatchItem* __restrict__ items,
                                                               const uint32_t* __restrict__ prefix, int K, int G,
-                                                              uint32_t total, uint32_t* __restrict__ ctrl) {
+                                                              uint32_t total, uint32_t* __restrict__ ctrl,
+                                                              uint32_t pt0, uint32_t pt1) {
     __shared__ float2 lds[batch_lds_max<0>()];
     __shared__ uint32_t s_unit;
     __shared__ int s_k;
     const int tid = int(threadIdx.x);
+    int cur = 0;          // pass of the units this workgroup is finishing
+    uint32_t count = 0;   // units of pass `cur` it has finished
     for (;;) {
         __syncthreads();   // the previous unit is done with LDS and s_unit
         if (tid == 0) {
@@ -102,11 +109,23 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(float2* __restrict
                     else hi = mid;
                 }
                 k = lo;
-                if (k >= G) {   // wait for the previous pass of this grid
-                    const uint32_t need = items[k - G].units;
+            }
+            s_unit = unit;
+            s_k = k;
+        }
+        __syncthreads();
+        const int k = __builtin_amdgcn_readfirstlane(s_k);
+        const int q = k < 0 ? 3 : k / G;   // pass of the new unit; 3 = queue empty
+        if (q != cur) {
+            __threadfence();   // release this workgroup's writes of pass `cur`
+            __syncthreads();
+            if (tid == 0) {
+                atomicAdd(&ctrl[2 + cur], count);
+                if (q < 3) {   // wait until every unit of pass q-1 is finished and released
+                    const uint32_t need = q == 1 ? pt0 : pt1;
                     uint32_t spins = 0;
-                    while (__hip_atomic_load(&ctrl[2 + k - G], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
-                        __builtin_amdgcn_s_sleep(4);
+                    while (__hip_atomic_load(&ctrl[2 + q - 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
+                        __builtin_amdgcn_s_sleep(8);
                         if (++spins > kSpinLimit) {
                             __hip_atomic_store(&ctrl[1], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                             break;
@@ -114,19 +133,16 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(float2* __restrict
                     }
                 }
             }
-            s_unit = unit;
-            s_k = k;
+            __syncthreads();
+            __threadfence();   // acquire: this workgroup's loads see pass q-1
+            cur = q;
+            count = 0;
         }
-        __syncthreads();
-        const int k = __builtin_amdgcn_readfirstlane(s_k);
-        if (k < 0) break;
-        __threadfence();   // acquire: this workgroup's loads see the finished pass
+        if (q == 3) break;
         const BatchItem& it = items[k];
         const uint32_t u = uint32_t(__builtin_amdgcn_readfirstlane(int(s_unit - prefix[k])));
         run_unit<DIR>(it.shape, data + it.base, it.g, it.tw, u, lds, tid);
-        __threadfence();   // release this thread's stores
-        __syncthreads();
-        if (tid == 0) atomicAdd(&ctrl[2 + k], 1u);
+        ++count;
     }
 }
 
@@ -146,6 +162,7 @@ struct a2afft_batch_s {
     uint32_t* d_prefix = nullptr;
     uint32_t* d_ctrl = nullptr;
     uint32_t total = 0;
+    uint32_t pass_units[2] = {0, 0};   // units of the z and x passes
     int ctrl_len = 0;
     int grid = 0;                     // persistent workgroups
     int max_grid = 0;
@@ -239,7 +256,9 @@ extern "C" int a2afft_batch_create(a2afft_batch* out, int ngrids, const int* sha
         prefix[k + 1] = uint32_t(next);
     }
     b->total = prefix[K];
-    b->ctrl_len = 2 + K;
+    b->ctrl_len = 2 + 3;
+    b->pass_units[0] = prefix[ngrids] - prefix[0];
+    b->pass_units[1] = prefix[2 * ngrids] - prefix[ngrids];
     int st = A2AFFT_OK;
     for (int d = 0; d < 2 && st == A2AFFT_OK; ++d) {
         st = hip_check(int(hipMalloc(&b->d_items[d], sizeof(BatchItem) * K)), "batch alloc");
@@ -282,10 +301,10 @@ static int batch_launch(a2afft_batch_s* b, void* data, int dir, hipStream_t s) {
     const int K = 3 * b->G;
     if (dir < 0)
         hipLaunchKernelGGL(batch_kernel<-1>, dim3(b->grid), dim3(kBatchThreads), 0, s, (float2*)data, b->d_items[0],
-                           b->d_prefix, K, b->G, b->total, b->d_ctrl);
+                           b->d_prefix, K, b->G, b->total, b->d_ctrl, b->pass_units[0], b->pass_units[1]);
     else
         hipLaunchKernelGGL(batch_kernel<1>, dim3(b->grid), dim3(kBatchThreads), 0, s, (float2*)data, b->d_items[1],
-                           b->d_prefix, K, b->G, b->total, b->d_ctrl);
+                           b->d_prefix, K, b->G, b->total, b->d_ctrl, b->pass_units[0], b->pass_units[1]);
     return hip_check(int(hipGetLastError()), "batch launch");
 }
 
