@@ -265,8 +265,9 @@ class Batch:
     def status(self):
         return L.check(L.lib().a2afft_batch_status(self._h, _stream()), "batch_status")
 
-    def set_workgroups(self, n):
-        return L.check(L.lib().a2afft_batch_set_workgroups(self._h, n), "batch_set_workgroups")
+    def set_workgroups(self, n, xcd_queues=True):
+        return L.check(L.lib().a2afft_batch_set_workgroups(self._h, n, 1 if xcd_queues else 0),
+                       "batch_set_workgroups")
 
     def info(self):
         u, w, m = ctypes.c_int64(), ctypes.c_int(), ctypes.c_int()

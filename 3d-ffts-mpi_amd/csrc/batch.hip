@@ -82,33 +82,66 @@ constexpr int batch_lds_max() {
     }
 }
 
-// ctrl[0]: next unit, ctrl[1]: error flag, ctrl[2 + p]: units of pass p
-// finished and released; ptotal[p]: units of pass p.
+// Unit queues.  Pass p owns units [pstart[p], pstart[p+1]) (pass-major
+// numbering); with xcd_queues they are split into 8 contiguous sub-ranges,
+// one per XCD (the hardware hands workgroup b to XCD b % 8), taken through 8
+// counters; a workgroup whose own sub-range is empty takes from the others.
+// A workgroup moves to pass p+1 only after every counter of pass p is
+// exhausted, so all units of a pass are taken before any unit of the next.
+//
+// ctrl[0..23]: take counters (pass * 8 + queue; queue 0 only without
+// xcd_queues), ctrl[24]: error flag, ctrl[25 + p]: units of pass p finished
+// and released.
+constexpr int kCtrlTake = 0, kCtrlErr = 24, kCtrlDone = 25, kCtrlLen = 28;
+
+struct BatchQueues {
+    uint32_t pstart[4];
+    int xcd_queues;
+};
+
+__device__ __forceinline__ uint32_t queue_start(const BatchQueues& qs, int p, int x) {
+    const uint32_t len = qs.pstart[p + 1] - qs.pstart[p];
+    return qs.pstart[p] + uint32_t((uint64_t(len) * uint32_t(x)) / 8u);
+}
+
 template <int DIR>
 __global__ __launch_bounds__(kBatchThreads) void batch_kernel(float2* __restrict__ data,
                                                               const BatchItem* __restrict__ items,
                                                               const uint32_t* __restrict__ prefix, int K, int G,
-                                                              uint32_t total, uint32_t* __restrict__ ctrl,
-                                                              uint32_t pt0, uint32_t pt1) {
+                                                              BatchQueues qs, uint32_t* __restrict__ ctrl) {
     __shared__ float2 lds[batch_lds_max<0>()];
     __shared__ uint32_t s_unit;
     __shared__ int s_k;
     const int tid = int(threadIdx.x);
+    const int myq = qs.xcd_queues ? int(blockIdx.x & 7u) : 0;
+    const int nq = qs.xcd_queues ? 8 : 1;
     int cur = 0;          // pass of the units this workgroup is finishing
+    int scan = 0;         // pass this workgroup is taking units from
     uint32_t count = 0;   // units of pass `cur` it has finished
     for (;;) {
         __syncthreads();   // the previous unit is done with LDS and s_unit
         if (tid == 0) {
-            const uint32_t unit = atomicAdd(&ctrl[0], 1u);
             int k = -1;
-            if (unit < total) {
-                int lo = 0, hi = K;   // largest k with prefix[k] <= unit
-                while (hi - lo > 1) {
-                    const int mid = (lo + hi) >> 1;
-                    if (prefix[mid] <= unit) lo = mid;
-                    else hi = mid;
+            uint32_t unit = 0;
+            while (k < 0 && scan < 3) {
+                for (int i = 0; i < nq && k < 0; ++i) {
+                    const int x = (myq + i) % nq;
+                    const uint32_t lo = nq == 1 ? qs.pstart[scan] : queue_start(qs, scan, x);
+                    const uint32_t hi = nq == 1 ? qs.pstart[scan + 1] : queue_start(qs, scan, x + 1);
+                    if (lo >= hi) continue;
+                    const uint32_t got = atomicAdd(&ctrl[kCtrlTake + scan * 8 + x], 1u);
+                    if (got < hi - lo) {
+                        unit = lo + got;
+                        int a = 0, b = K;   // largest k with prefix[k] <= unit
+                        while (b - a > 1) {
+                            const int mid = (a + b) >> 1;
+                            if (prefix[mid] <= unit) a = mid;
+                            else b = mid;
+                        }
+                        k = a;
+                    }
                 }
-                k = lo;
+                if (k < 0) ++scan;
             }
             s_unit = unit;
             s_k = k;
@@ -120,14 +153,15 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(float2* __restrict
             __threadfence();   // release this workgroup's writes of pass `cur`
             __syncthreads();
             if (tid == 0) {
-                atomicAdd(&ctrl[2 + cur], count);
+                atomicAdd(&ctrl[kCtrlDone + cur], count);
                 if (q < 3) {   // wait until every unit of pass q-1 is finished and released
-                    const uint32_t need = q == 1 ? pt0 : pt1;
+                    const uint32_t need = qs.pstart[q] - qs.pstart[q - 1];
                     uint32_t spins = 0;
-                    while (__hip_atomic_load(&ctrl[2 + q - 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
+                    while (__hip_atomic_load(&ctrl[kCtrlDone + q - 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
+                           need) {
                         __builtin_amdgcn_s_sleep(8);
                         if (++spins > kSpinLimit) {
-                            __hip_atomic_store(&ctrl[1], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            __hip_atomic_store(&ctrl[kCtrlErr], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                             break;
                         }
                     }
@@ -162,7 +196,7 @@ struct a2afft_batch_s {
     uint32_t* d_prefix = nullptr;
     uint32_t* d_ctrl = nullptr;
     uint32_t total = 0;
-    uint32_t pass_units[2] = {0, 0};   // units of the z and x passes
+    BatchQueues qs;
     int ctrl_len = 0;
     int grid = 0;                     // persistent workgroups
     int max_grid = 0;
@@ -256,9 +290,10 @@ extern "C" int a2afft_batch_create(a2afft_batch* out, int ngrids, const int* sha
         prefix[k + 1] = uint32_t(next);
     }
     b->total = prefix[K];
-    b->ctrl_len = 2 + 3;
-    b->pass_units[0] = prefix[ngrids] - prefix[0];
-    b->pass_units[1] = prefix[2 * ngrids] - prefix[ngrids];
+    b->ctrl_len = kCtrlLen;
+    for (int p = 0; p < 3; ++p) b->qs.pstart[p] = prefix[p * ngrids];
+    b->qs.pstart[3] = prefix[K];
+    b->qs.xcd_queues = 1;
     int st = A2AFFT_OK;
     for (int d = 0; d < 2 && st == A2AFFT_OK; ++d) {
         st = hip_check(int(hipMalloc(&b->d_items[d], sizeof(BatchItem) * K)), "batch alloc");
@@ -301,10 +336,10 @@ static int batch_launch(a2afft_batch_s* b, void* data, int dir, hipStream_t s) {
     const int K = 3 * b->G;
     if (dir < 0)
         hipLaunchKernelGGL(batch_kernel<-1>, dim3(b->grid), dim3(kBatchThreads), 0, s, (float2*)data, b->d_items[0],
-                           b->d_prefix, K, b->G, b->total, b->d_ctrl, b->pass_units[0], b->pass_units[1]);
+                           b->d_prefix, K, b->G, b->qs, b->d_ctrl);
     else
         hipLaunchKernelGGL(batch_kernel<1>, dim3(b->grid), dim3(kBatchThreads), 0, s, (float2*)data, b->d_items[1],
-                           b->d_prefix, K, b->G, b->total, b->d_ctrl, b->pass_units[0], b->pass_units[1]);
+                           b->d_prefix, K, b->G, b->qs, b->d_ctrl);
     return hip_check(int(hipGetLastError()), "batch launch");
 }
 
@@ -344,16 +379,23 @@ extern "C" int a2afft_batch_graph_launch(a2afft_batch b, void* data, int directi
 extern "C" int a2afft_batch_status(a2afft_batch b, void* stream) {
     if (!b) return set_error(A2AFFT_E_INVALID_ARG, "batch_status: null");
     A2A_TRY(hip_check(int(hipStreamSynchronize((hipStream_t)stream)), "batch sync"));
-    uint32_t h[2] = {0, 0};
+    uint32_t h[kCtrlLen];
     A2A_TRY(hip_check(int(hipMemcpy(h, b->d_ctrl, sizeof(h), hipMemcpyDeviceToHost)), "batch status copy"));
-    if (h[0] < b->total) return set_error(A2AFFT_E_HIP, "batch: %u of %u units taken", h[0], b->total);
-    return h[1] ? set_error(A2AFFT_E_HIP, "batch: a dependency wait timed out") : A2AFFT_OK;
+    if (h[kCtrlErr]) return set_error(A2AFFT_E_HIP, "batch: a dependency wait timed out");
+    for (int p = 0; p < 3; ++p) {
+        const uint32_t need = b->qs.pstart[p + 1] - b->qs.pstart[p];
+        if (h[kCtrlDone + p] != need)
+            return set_error(A2AFFT_E_HIP, "batch: pass %d finished %u of %u units", p, h[kCtrlDone + p], need);
+    }
+    return A2AFFT_OK;
 }
 
-// Persistent workgroups per launch (0 = one per resident slot on every CU).
-extern "C" int a2afft_batch_set_workgroups(a2afft_batch b, int workgroups) {
+// Persistent workgroups per launch (0 = one per resident slot on every CU)
+// and the queue layout (1 = one queue range per XCD, 0 = one global queue).
+extern "C" int a2afft_batch_set_workgroups(a2afft_batch b, int workgroups, int xcd_queues) {
     if (!b || workgroups < 0) return set_error(A2AFFT_E_INVALID_ARG, "batch_set_workgroups: bad arguments");
     b->grid = workgroups == 0 ? b->max_grid : std::min(workgroups, b->max_grid);
+    b->qs.xcd_queues = xcd_queues ? 1 : 0;
     for (int d = 0; d < 2; ++d) {
         if (b->graph[d]) (void)hipGraphExecDestroy(b->graph[d]);
         b->graph[d] = nullptr;

@@ -189,14 +189,15 @@ def run_c5(args, torch, a2afft):
         del ref
     # workgroup-count tuning (every setting computes the same bits)
     info = b.info()
-    best_w, best_t = 0, 1e30
-    for w in sorted({info["max_workgroups"], info["max_workgroups"] // 2, 256 * 3, 256}):
-        b.set_workgroups(w)
-        t = _events_ms(torch, lambda: b.forward(y), 3)
-        log(f"[bench] batch workgroups {w}: {t:.4f} ms")
-        if t < best_t:
-            best_w, best_t = w, t
-    b.set_workgroups(best_w)
+    best_w, best_x, best_t = 0, 1, 1e30
+    for xq in (1, 0):
+        for w in sorted({info["max_workgroups"], info["max_workgroups"] // 2, 512, 256}):
+            b.set_workgroups(w, bool(xq))
+            t = _events_ms(torch, lambda: b.forward(y), 3)
+            log(f"[bench] batch workgroups {w} xcd_queues {xq}: {t:.4f} ms")
+            if t < best_t:
+                best_w, best_x, best_t = w, xq, t
+    b.set_workgroups(best_w, bool(best_x))
     t_eager = _events_ms(torch, lambda: b.forward(y), args.steps)
     t_graph = _events_ms(torch, lambda: b.graph_launch(y, -1), args.steps)
     assert b.status() == 0
@@ -221,7 +222,7 @@ def run_c5(args, torch, a2afft):
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "64 complex64 grids drawn with seed 4 from 32^3, 64^3, 128^3, 256^3, 128x256x64, "
                                "256x64x512; U[-1,1) values", "name": "c5", "counts": counts,
-                   "points": npts, "workgroups": best_w},
+                   "points": npts, "workgroups": best_w, "xcd_queues": bool(best_x)},
         "ms": {"persistent_eager": round(t_eager, 4), "persistent_graph": round(t_graph, 4),
                "separate_launches_192": round(t_sep, 4)},
         "rel_l2_vs_fp64": {"max": max(errs), "mean": sum(errs) / len(errs)},

@@ -200,8 +200,10 @@ int a2afft_batch_offsets(a2afft_batch batch, int64_t* offsets);
 int a2afft_batch_execute(a2afft_batch batch, void* d_data, int direction, void* stream);
 int a2afft_batch_graph_launch(a2afft_batch batch, void* d_data, int direction, void* stream);
 int a2afft_batch_status(a2afft_batch batch, void* stream);
-/* Persistent workgroups per launch (0 = every resident slot on every CU). */
-int a2afft_batch_set_workgroups(a2afft_batch batch, int workgroups);
+/* Persistent workgroups per launch (0 = every resident slot on every CU) and
+ * the queue layout: 1 = each pass's units split into one contiguous range per
+ * XCD (the default), 0 = one global queue.  Results are identical. */
+int a2afft_batch_set_workgroups(a2afft_batch batch, int workgroups, int xcd_queues);
 int a2afft_batch_info(a2afft_batch batch, int64_t* total_units, int* workgroups, int* max_workgroups);
 int a2afft_batch_destroy(a2afft_batch batch);
 

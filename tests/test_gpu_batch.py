@@ -22,11 +22,11 @@ def _fill(a2afft_mod, torch, batch, seed):
     return buf
 
 
-@pytest.mark.parametrize("workgroups", [0, 1, 7])
-def test_batch_matches_grid_plans_and_numpy(a2afft_mod, cuda, workgroups):
+@pytest.mark.parametrize("workgroups,xcdq", [(0, 1), (0, 0), (1, 1), (7, 1), (7, 0), (16, 1)])
+def test_batch_matches_grid_plans_and_numpy(a2afft_mod, cuda, workgroups, xcdq):
     import torch
     b = a2afft_mod.Batch(SHAPES)
-    b.set_workgroups(workgroups)
+    b.set_workgroups(workgroups, bool(xcdq))
     x = _fill(a2afft_mod, torch, b, 4)
     x0 = x.cpu().numpy()
     y = x.clone()
