@@ -265,9 +265,11 @@ class Batch:
     def status(self):
         return L.check(L.lib().a2afft_batch_status(self._h, _stream()), "batch_status")
 
-    def set_workgroups(self, n, xcd_queues=True):
-        return L.check(L.lib().a2afft_batch_set_workgroups(self._h, n, 1 if xcd_queues else 0),
-                       "batch_set_workgroups")
+    def set_schedule(self, workgroups=0, xcd_queues=False, chunk=1):
+        """Persistent workgroups (0 = all resident slots), per-XCD queue ranges
+        and units per queue take; every schedule computes the same bits."""
+        return L.check(L.lib().a2afft_batch_set_schedule(self._h, workgroups, 1 if xcd_queues else 0, chunk),
+                       "batch_set_schedule")
 
     def info(self):
         u, w, m = ctypes.c_int64(), ctypes.c_int(), ctypes.c_int()

@@ -5,19 +5,24 @@
 // Work queue.  Every (pass, grid) pair is split into the units its Stockham
 // kernel would use (lines or column tiles) and the units are numbered
 // pass-major: all z units of all grids, then all x units, then all y units.
-// Workgroups take units in that order from an atomic counter.
+// Workgroups take units in that order from atomic counters; a table maps each
+// unit to its (pass, grid) item.  The take of the next unit is issued before
+// the current unit is computed and examined after it, so the latency of the
+// device-scope atomic hides behind the unit's own loads.
 //
 // Synchronisation is per pass, not per unit: the 8 XCDs of the device have
 // L2 caches that are not coherent with one another, so a device-scope fence
 // writes back or invalidates a whole L2.  A workgroup therefore counts the
 // units it finishes and, only when the next unit it takes belongs to a later
-// pass (or the queue is empty), releases its writes with one fence, adds its
-// count to that pass's total, waits until the previous pass's total is
+// pass (or the queues are empty), releases its writes with one fence, adds
+// its count to that pass's total, waits until the previous pass's total is
 // complete and acquires with one fence.  That is at most three transitions
-// per workgroup.  Every unit of a pass is taken before any unit of the next
-// pass, by a running workgroup that finishes it without waiting on anything
-// later, so every wait ends; the wait is still bounded, and a bound that
-// trips sets an error flag instead of hanging.
+// per workgroup.  Every wait ends: every unit of pass p is taken before any
+// unit of pass p+1, and the workgroup holding it finishes it after at most a
+// wait for pass p-1 (it may hold one taken unit of pass p+1 meanwhile, which
+// nobody waits on yet), so by induction over p every pass completes.  The
+// wait is still bounded, and a bound that trips sets an error flag instead of
+// hanging.
 #include "fft.h"
 #include "stockham.h"
 
@@ -31,6 +36,7 @@ namespace {
 
 constexpr int kBatchThreads = 256;
 constexpr uint32_t kSpinLimit = 1u << 24;
+constexpr uint32_t kMaxUnits = 1u << 28;   // the unit -> item table holds 4 bytes per unit
 
 // Kernel shape per (N, strided): primary and a single-column fallback.
 struct BShape {
@@ -51,23 +57,24 @@ struct BatchItem {
     int64_t base;        // element offset of the grid in the batch buffer
     const float2* tw;    // twiddles of this pass's length and direction
     uint32_t units;
+    uint32_t first;      // number of this item's first unit in the queue
     int shape;           // index into kShapes
 };
 
-template <int DIR>
+template <int DIR, typename Hook>
 __device__ __forceinline__ void run_unit(int shape, float2* data, const FftGeom& g, const float2* tw, uint32_t u,
-                                         float2* lds, int tid) {
+                                         float2* lds, int tid, Hook&& hook) {
     switch (shape) {
 #define A2A_B(I)                                                                                                \
     case I:                                                                                                    \
         stockham_unit<float, kShapes[I].N, kShapes[I].E, kShapes[I].T, kShapes[I].strided != 0, DIR, kFlags>( \
-            data, g, tw, u, lds, tid);                                                                         \
+            data, g, tw, u, lds, tid, hook);                                                                   \
         break;
         A2A_B(0) A2A_B(1) A2A_B(2) A2A_B(3) A2A_B(4) A2A_B(5) A2A_B(6) A2A_B(7) A2A_B(8)
         A2A_B(9) A2A_B(10) A2A_B(11) A2A_B(12) A2A_B(13) A2A_B(14) A2A_B(15) A2A_B(16) A2A_B(17)
 #undef A2A_B
-    default:
-        break;
+    default:   // the host only stores valid shape indices
+        __builtin_unreachable();
     }
 }
 
@@ -86,82 +93,130 @@ constexpr int batch_lds_max() {
 // numbering); with xcd_queues they are split into 8 contiguous sub-ranges,
 // one per XCD (the hardware hands workgroup b to XCD b % 8), taken through 8
 // counters; a workgroup whose own sub-range is empty takes from the others.
-// A workgroup moves to pass p+1 only after every counter of pass p is
-// exhausted, so all units of a pass are taken before any unit of the next.
+// One take hands out `chunk` consecutive units.  A workgroup moves to pass
+// p+1 only after every counter of pass p is exhausted, so all units of a pass
+// are taken before any unit of the next.
 //
-// ctrl[0..23]: take counters (pass * 8 + queue; queue 0 only without
-// xcd_queues), ctrl[24]: error flag, ctrl[25 + p]: units of pass p finished
-// and released.
-constexpr int kCtrlTake = 0, kCtrlErr = 24, kCtrlDone = 25, kCtrlLen = 28;
+// Control words, each in its own 256-byte slot so that takes on different
+// counters do not serialise on one cache line: slot 0..23 take counters
+// (pass * 8 + queue; queue 0 only without xcd_queues), slot 24 error flag,
+// slot 25 + p units of pass p finished and released.
+constexpr int kCtrlTake = 0, kCtrlErr = 24, kCtrlDone = 25, kCtrlSlots = 28, kCtrlStride = 64;
+constexpr uint32_t kNoUnit = 0xffffffffu;
+
+__host__ __device__ constexpr int ctrl_word(int slot) { return slot * kCtrlStride; }
 
 struct BatchQueues {
     uint32_t pstart[4];
     int xcd_queues;
+    uint32_t chunk;
 };
 
-__device__ __forceinline__ uint32_t queue_start(const BatchQueues& qs, int p, int x) {
-    const uint32_t len = qs.pstart[p + 1] - qs.pstart[p];
-    return qs.pstart[p] + uint32_t((uint64_t(len) * uint32_t(x)) / 8u);
+// Where thread 0 of a workgroup takes its next unit: the rest [next, end) of
+// its current chunk, else pass `scan`, the i-th queue counted from its own.
+// Exhausted queues stay exhausted (counters only grow), so the cursor only
+// moves forward.
+struct Cursor {
+    int scan, i;
+    uint32_t next, end;
+};
+
+// Pass bounds held in registers.  Thread 0 must not load them from memory
+// inside the unit loop: the wait for such a load would also wait for its own
+// outstanding stores of the previous unit.
+struct PassBounds {
+    uint32_t s0, s1, s2, s3, chunk;
+    __device__ __forceinline__ explicit PassBounds(const BatchQueues& qs)
+        : s0(uint32_t(__builtin_amdgcn_readfirstlane(int(qs.pstart[0])))),
+          s1(uint32_t(__builtin_amdgcn_readfirstlane(int(qs.pstart[1])))),
+          s2(uint32_t(__builtin_amdgcn_readfirstlane(int(qs.pstart[2])))),
+          s3(uint32_t(__builtin_amdgcn_readfirstlane(int(qs.pstart[3])))),
+          chunk(uint32_t(__builtin_amdgcn_readfirstlane(int(qs.chunk)))) {}
+};
+
+// Units of pass p (0..2): [start(p), start(p + 1)).
+__device__ __forceinline__ uint32_t pass_start(const PassBounds& qs, int p) {
+    const uint32_t m1 = p >= 1 ? ~0u : 0u, m2 = p >= 2 ? ~0u : 0u, m3 = p >= 3 ? ~0u : 0u;
+    return qs.s0 + ((qs.s1 - qs.s0) & m1) + ((qs.s2 - qs.s1) & m2) + ((qs.s3 - qs.s2) & m3);
+}
+
+__device__ __forceinline__ void queue_range(const PassBounds& qs, int p, int x, int nq, uint32_t& lo, uint32_t& hi) {
+    const uint32_t a = pass_start(qs, p);
+    const uint32_t len = pass_start(qs, p + 1) - a;
+    lo = a + uint32_t((uint64_t(len) * uint32_t(x)) / uint32_t(nq));
+    hi = a + uint32_t((uint64_t(len) * uint32_t(x + 1)) / uint32_t(nq));
+}
+
+// Issues the take from the cursor's queue and returns the raw ticket without
+// examining it, so the atomic's latency overlaps the unit being computed.  No
+// take while the current chunk has units left.
+__device__ __forceinline__ uint32_t claim_issue(const Cursor& c, const PassBounds& qs, int myq, int nq,
+                                                uint32_t* ctrl) {
+    if (c.next < c.end || c.scan >= 3) return kNoUnit;
+    const int x = (myq + c.i) % nq;
+    uint32_t lo, hi;
+    queue_range(qs, c.scan, x, nq, lo, hi);
+    return lo < hi ? atomicAdd(&ctrl[ctrl_word(kCtrlTake + c.scan * 8 + x)], 1u) : kNoUnit;
+}
+
+// Turns a ticket into a unit number, moving on through the queues (with
+// further takes) while the ticket falls past the end of its queue; kNoUnit
+// when every queue of every pass is exhausted.
+__device__ __forceinline__ uint32_t claim_finish(Cursor& c, uint32_t ticket, const PassBounds& qs, int myq, int nq,
+                                                 uint32_t* ctrl) {
+    if (c.next < c.end) return c.next++;
+    for (;;) {
+        if (c.scan >= 3) return kNoUnit;
+        const int x = (myq + c.i) % nq;
+        uint32_t lo, hi;
+        queue_range(qs, c.scan, x, nq, lo, hi);
+        if (lo < hi && ticket < (hi - lo + qs.chunk - 1) / qs.chunk) {
+            const uint32_t first = lo + ticket * qs.chunk;
+            c.next = first + 1;
+            c.end = min(first + qs.chunk, hi);
+            return first;
+        }
+        if (++c.i == nq) {
+            c.i = 0;
+            ++c.scan;
+        }
+        ticket = claim_issue(c, qs, myq, nq, ctrl);
+    }
 }
 
 template <int DIR>
 __global__ __launch_bounds__(kBatchThreads) void batch_kernel(float2* __restrict__ data,
                                                               const BatchItem* __restrict__ items,
-                                                              const uint32_t* __restrict__ prefix, int K, int G,
+                                                              const uint32_t* __restrict__ unit_item, int G,
                                                               BatchQueues qs, uint32_t* __restrict__ ctrl) {
     __shared__ float2 lds[batch_lds_max<0>()];
-    __shared__ uint32_t s_unit;
-    __shared__ int s_k;
+    __shared__ uint32_t s_unit[2];   // alternate per unit: one barrier per unit suffices
     const int tid = int(threadIdx.x);
-    const int myq = qs.xcd_queues ? int(blockIdx.x & 7u) : 0;
     const int nq = qs.xcd_queues ? 8 : 1;
+    const int myq = qs.xcd_queues ? int(blockIdx.x & 7u) : 0;
+    const PassBounds pb(qs);
+    Cursor c{0, 0, 0u, 0u};
+    if (tid == 0) s_unit[0] = claim_finish(c, claim_issue(c, pb, myq, nq, ctrl), pb, myq, nq, ctrl);
+    __syncthreads();
+    uint32_t unit = uint32_t(__builtin_amdgcn_readfirstlane(int(s_unit[0])));
     int cur = 0;          // pass of the units this workgroup is finishing
-    int scan = 0;         // pass this workgroup is taking units from
     uint32_t count = 0;   // units of pass `cur` it has finished
-    for (;;) {
-        __syncthreads();   // the previous unit is done with LDS and s_unit
-        if (tid == 0) {
-            int k = -1;
-            uint32_t unit = 0;
-            while (k < 0 && scan < 3) {
-                for (int i = 0; i < nq && k < 0; ++i) {
-                    const int x = (myq + i) % nq;
-                    const uint32_t lo = nq == 1 ? qs.pstart[scan] : queue_start(qs, scan, x);
-                    const uint32_t hi = nq == 1 ? qs.pstart[scan + 1] : queue_start(qs, scan, x + 1);
-                    if (lo >= hi) continue;
-                    const uint32_t got = atomicAdd(&ctrl[kCtrlTake + scan * 8 + x], 1u);
-                    if (got < hi - lo) {
-                        unit = lo + got;
-                        int a = 0, b = K;   // largest k with prefix[k] <= unit
-                        while (b - a > 1) {
-                            const int mid = (a + b) >> 1;
-                            if (prefix[mid] <= unit) a = mid;
-                            else b = mid;
-                        }
-                        k = a;
-                    }
-                }
-                if (k < 0) ++scan;
-            }
-            s_unit = unit;
-            s_k = k;
-        }
-        __syncthreads();
-        const int k = __builtin_amdgcn_readfirstlane(s_k);
-        const int q = k < 0 ? 3 : k / G;   // pass of the new unit; 3 = queue empty
+    for (int it_no = 1;; ++it_no) {
+        const int k = unit == kNoUnit ? -1 : int(unit_item[unit]);
+        const int q = k < 0 ? 3 : k / G;   // pass of this unit; 3 = queues empty
         if (q != cur) {
             __threadfence();   // release this workgroup's writes of pass `cur`
             __syncthreads();
             if (tid == 0) {
-                atomicAdd(&ctrl[kCtrlDone + cur], count);
+                atomicAdd(&ctrl[ctrl_word(kCtrlDone + cur)], count);
                 if (q < 3) {   // wait until every unit of pass q-1 is finished and released
-                    const uint32_t need = qs.pstart[q] - qs.pstart[q - 1];
+                    const uint32_t need = pass_start(pb, q) - pass_start(pb, q - 1);
                     uint32_t spins = 0;
-                    while (__hip_atomic_load(&ctrl[kCtrlDone + q - 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
+                    while (__hip_atomic_load(&ctrl[ctrl_word(kCtrlDone + q - 1)], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
                            need) {
                         __builtin_amdgcn_s_sleep(8);
                         if (++spins > kSpinLimit) {
-                            __hip_atomic_store(&ctrl[kCtrlErr], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            __hip_atomic_store(&ctrl[ctrl_word(kCtrlErr)], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                             break;
                         }
                     }
@@ -173,10 +228,17 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(float2* __restrict
             count = 0;
         }
         if (q == 3) break;
+        uint32_t ticket;
+        if (tid == 0) ticket = claim_issue(c, pb, myq, nq, ctrl);
         const BatchItem& it = items[k];
-        const uint32_t u = uint32_t(__builtin_amdgcn_readfirstlane(int(s_unit - prefix[k])));
-        run_unit<DIR>(it.shape, data + it.base, it.g, it.tw, u, lds, tid);
+        // The ticket is examined once the unit's loads are consumed and before
+        // its stores are issued, so thread 0 never waits for its own stores.
+        run_unit<DIR>(it.shape, data + it.base, it.g, it.tw, unit - it.first, lds, tid, [&]() {
+            if (tid == 0) s_unit[it_no & 1] = claim_finish(c, ticket, pb, myq, nq, ctrl);
+        });
         ++count;
+        __syncthreads();   // the unit is done with LDS; s_unit is written
+        unit = uint32_t(__builtin_amdgcn_readfirstlane(int(s_unit[it_no & 1])));
     }
 }
 
@@ -193,7 +255,7 @@ struct a2afft_batch_s {
     std::vector<int> shapes;          // 3 per grid
     std::vector<int64_t> offsets;     // G + 1
     BatchItem* d_items[2] = {nullptr, nullptr};   // forward, inverse
-    uint32_t* d_prefix = nullptr;
+    uint32_t* d_unit_item = nullptr;   // item of every unit
     uint32_t* d_ctrl = nullptr;
     uint32_t total = 0;
     BatchQueues qs;
@@ -211,7 +273,7 @@ static void batch_free(a2afft_batch_s* b) {
         if (b->d_items[d]) (void)hipFree(b->d_items[d]);
         if (b->graph[d]) (void)hipGraphExecDestroy(b->graph[d]);
     }
-    if (b->d_prefix) (void)hipFree(b->d_prefix);
+    if (b->d_unit_item) (void)hipFree(b->d_unit_item);
     if (b->d_ctrl) (void)hipFree(b->d_ctrl);
     if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
     delete b;
@@ -283,17 +345,22 @@ extern "C" int a2afft_batch_create(a2afft_batch* out, int ngrids, const int* sha
     }
     for (int k = 0; k < K; ++k) {
         const uint64_t next = uint64_t(prefix[k]) + items[0][k].units;
-        if (next >= (uint64_t(1) << 32)) {
+        if (next > kMaxUnits) {
             batch_free(b);
-            return set_error(A2AFFT_E_SIZE, "batch: too many work units");
+            return set_error(A2AFFT_E_SIZE, "batch: more than %u work units", kMaxUnits);
         }
         prefix[k + 1] = uint32_t(next);
+        items[0][k].first = items[1][k].first = prefix[k];
     }
+    std::vector<uint32_t> unit_item(prefix[K]);
+    for (int k = 0; k < K; ++k)
+        for (uint32_t u = prefix[k]; u < prefix[k + 1]; ++u) unit_item[u] = uint32_t(k);
     b->total = prefix[K];
-    b->ctrl_len = kCtrlLen;
+    b->ctrl_len = ctrl_word(kCtrlSlots);
     for (int p = 0; p < 3; ++p) b->qs.pstart[p] = prefix[p * ngrids];
     b->qs.pstart[3] = prefix[K];
-    b->qs.xcd_queues = 1;
+    b->qs.xcd_queues = 0;
+    b->qs.chunk = 1;
     int st = A2AFFT_OK;
     for (int d = 0; d < 2 && st == A2AFFT_OK; ++d) {
         st = hip_check(int(hipMalloc(&b->d_items[d], sizeof(BatchItem) * K)), "batch alloc");
@@ -301,10 +368,11 @@ extern "C" int a2afft_batch_create(a2afft_batch* out, int ngrids, const int* sha
             st = hip_check(int(hipMemcpy(b->d_items[d], items[d].data(), sizeof(BatchItem) * K,
                                          hipMemcpyHostToDevice)), "batch copy");
     }
-    if (st == A2AFFT_OK) st = hip_check(int(hipMalloc(&b->d_prefix, sizeof(uint32_t) * (K + 1))), "batch alloc");
     if (st == A2AFFT_OK)
-        st = hip_check(int(hipMemcpy(b->d_prefix, prefix.data(), sizeof(uint32_t) * (K + 1), hipMemcpyHostToDevice)),
-                       "batch copy");
+        st = hip_check(int(hipMalloc(&b->d_unit_item, sizeof(uint32_t) * unit_item.size())), "batch alloc");
+    if (st == A2AFFT_OK)
+        st = hip_check(int(hipMemcpy(b->d_unit_item, unit_item.data(), sizeof(uint32_t) * unit_item.size(),
+                                     hipMemcpyHostToDevice)), "batch copy");
     if (st == A2AFFT_OK) st = hip_check(int(hipMalloc(&b->d_ctrl, sizeof(uint32_t) * b->ctrl_len)), "batch alloc");
     int dev = 0, cus = 0, occ = 0;
     if (st == A2AFFT_OK) st = hip_check(int(hipGetDevice(&dev)), "batch device");
@@ -331,15 +399,14 @@ extern "C" int a2afft_batch_offsets(a2afft_batch b, int64_t* offsets) {
 }
 
 static int batch_launch(a2afft_batch_s* b, void* data, int dir, hipStream_t s) {
-    hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(256), 0, s, b->d_ctrl, b->ctrl_len);
+    hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(1024), 0, s, b->d_ctrl, b->ctrl_len);
     A2A_TRY(hip_check(int(hipGetLastError()), "batch reset launch"));
-    const int K = 3 * b->G;
     if (dir < 0)
         hipLaunchKernelGGL(batch_kernel<-1>, dim3(b->grid), dim3(kBatchThreads), 0, s, (float2*)data, b->d_items[0],
-                           b->d_prefix, K, b->G, b->qs, b->d_ctrl);
+                           b->d_unit_item, b->G, b->qs, b->d_ctrl);
     else
         hipLaunchKernelGGL(batch_kernel<1>, dim3(b->grid), dim3(kBatchThreads), 0, s, (float2*)data, b->d_items[1],
-                           b->d_prefix, K, b->G, b->qs, b->d_ctrl);
+                           b->d_unit_item, b->G, b->qs, b->d_ctrl);
     return hip_check(int(hipGetLastError()), "batch launch");
 }
 
@@ -379,23 +446,27 @@ extern "C" int a2afft_batch_graph_launch(a2afft_batch b, void* data, int directi
 extern "C" int a2afft_batch_status(a2afft_batch b, void* stream) {
     if (!b) return set_error(A2AFFT_E_INVALID_ARG, "batch_status: null");
     A2A_TRY(hip_check(int(hipStreamSynchronize((hipStream_t)stream)), "batch sync"));
-    uint32_t h[kCtrlLen];
-    A2A_TRY(hip_check(int(hipMemcpy(h, b->d_ctrl, sizeof(h), hipMemcpyDeviceToHost)), "batch status copy"));
-    if (h[kCtrlErr]) return set_error(A2AFFT_E_HIP, "batch: a dependency wait timed out");
+    std::vector<uint32_t> h(size_t(b->ctrl_len));
+    A2A_TRY(hip_check(int(hipMemcpy(h.data(), b->d_ctrl, sizeof(uint32_t) * h.size(), hipMemcpyDeviceToHost)),
+                      "batch status copy"));
+    if (h[ctrl_word(kCtrlErr)]) return set_error(A2AFFT_E_HIP, "batch: a dependency wait timed out");
     for (int p = 0; p < 3; ++p) {
         const uint32_t need = b->qs.pstart[p + 1] - b->qs.pstart[p];
-        if (h[kCtrlDone + p] != need)
-            return set_error(A2AFFT_E_HIP, "batch: pass %d finished %u of %u units", p, h[kCtrlDone + p], need);
+        const uint32_t done = h[ctrl_word(kCtrlDone + p)];
+        if (done != need) return set_error(A2AFFT_E_HIP, "batch: pass %d finished %u of %u units", p, done, need);
     }
     return A2AFFT_OK;
 }
 
-// Persistent workgroups per launch (0 = one per resident slot on every CU)
-// and the queue layout (1 = one queue range per XCD, 0 = one global queue).
-extern "C" int a2afft_batch_set_workgroups(a2afft_batch b, int workgroups, int xcd_queues) {
-    if (!b || workgroups < 0) return set_error(A2AFFT_E_INVALID_ARG, "batch_set_workgroups: bad arguments");
+// Persistent workgroups per launch (0 = one per resident slot on every CU),
+// the queue layout (1 = one queue range per XCD, 0 = one global queue) and
+// the units handed out per take.
+extern "C" int a2afft_batch_set_schedule(a2afft_batch b, int workgroups, int xcd_queues, int chunk) {
+    if (!b || workgroups < 0 || chunk < 1 || chunk > 64)
+        return set_error(A2AFFT_E_INVALID_ARG, "batch_set_schedule: bad arguments");
     b->grid = workgroups == 0 ? b->max_grid : std::min(workgroups, b->max_grid);
     b->qs.xcd_queues = xcd_queues ? 1 : 0;
+    b->qs.chunk = uint32_t(chunk);
     for (int d = 0; d < 2; ++d) {
         if (b->graph[d]) (void)hipGraphExecDestroy(b->graph[d]);
         b->graph[d] = nullptr;

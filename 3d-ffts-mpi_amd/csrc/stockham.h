@@ -248,14 +248,20 @@ __device__ __forceinline__ void stockham_passes(typename CT<real>::V (&v)[E], ty
     }
 }
 
+struct NoHook {
+    __device__ __forceinline__ void operator()() const {}
+};
+
 // One work unit: T lines (contiguous) or one tile of T columns (strided),
 // loaded, transformed and stored by the NT = T*N/E threads of a workgroup.
 // `block` is the unit index; `lds` holds lds_slots() complex values.  Threads
-// with tid >= NT take part in the barriers only.
-template <typename real, int N, int E, int T, bool STRIDED, int DIR, int FLAGS>
+// with tid >= NT take part in the barriers only.  Every thread calls `hook`
+// once, after the unit's loads have been consumed and before its stores are
+// issued.
+template <typename real, int N, int E, int T, bool STRIDED, int DIR, int FLAGS, typename Hook = NoHook>
 __device__ __forceinline__ void stockham_unit(typename CT<real>::V* __restrict__ data, const FftGeom& g,
                                               const typename CT<real>::V* __restrict__ tw, uint32_t block,
-                                              typename CT<real>::V* lds, int tid) {
+                                              typename CT<real>::V* lds, int tid, Hook&& hook = Hook{}) {
     using V = typename CT<real>::V;
     constexpr int TPL = N / E;
     constexpr int NT = T * TPL;
@@ -279,6 +285,7 @@ __device__ __forceinline__ void stockham_unit(typename CT<real>::V* __restrict__
     }
 
     stockham_passes<real, N, E, T, STRIDED, DIR, PAD, 0>(v, lds, tw, t, line, active);
+    hook();
 
     if (!valid) return;
     if (g.scale_mode == 1) {

@@ -200,10 +200,11 @@ int a2afft_batch_offsets(a2afft_batch batch, int64_t* offsets);
 int a2afft_batch_execute(a2afft_batch batch, void* d_data, int direction, void* stream);
 int a2afft_batch_graph_launch(a2afft_batch batch, void* d_data, int direction, void* stream);
 int a2afft_batch_status(a2afft_batch batch, void* stream);
-/* Persistent workgroups per launch (0 = every resident slot on every CU) and
- * the queue layout: 1 = each pass's units split into one contiguous range per
- * XCD (the default), 0 = one global queue.  Results are identical. */
-int a2afft_batch_set_workgroups(a2afft_batch batch, int workgroups, int xcd_queues);
+/* Schedule of the persistent launch: workgroups (0 = every resident slot on
+ * every CU); queue layout (1 = each pass's units split into one contiguous
+ * range per XCD, 0 = one global queue, the default); units handed out per
+ * take of the queue (1..64).  Results are identical for every schedule. */
+int a2afft_batch_set_schedule(a2afft_batch batch, int workgroups, int xcd_queues, int chunk);
 int a2afft_batch_info(a2afft_batch batch, int64_t* total_units, int* workgroups, int* max_workgroups);
 int a2afft_batch_destroy(a2afft_batch batch);
 

@@ -22,11 +22,12 @@ def _fill(a2afft_mod, torch, batch, seed):
     return buf
 
 
-@pytest.mark.parametrize("workgroups,xcdq", [(0, 1), (0, 0), (1, 1), (7, 1), (7, 0), (16, 1)])
-def test_batch_matches_grid_plans_and_numpy(a2afft_mod, cuda, workgroups, xcdq):
+@pytest.mark.parametrize("workgroups,xcdq,chunk", [(0, 0, 1), (0, 1, 1), (0, 0, 4), (1, 1, 3), (7, 1, 1),
+                                                   (7, 0, 2), (16, 1, 5), (3, 0, 64)])
+def test_batch_matches_grid_plans_and_numpy(a2afft_mod, cuda, workgroups, xcdq, chunk):
     import torch
     b = a2afft_mod.Batch(SHAPES)
-    b.set_workgroups(workgroups, bool(xcdq))
+    b.set_schedule(workgroups, bool(xcdq), chunk)
     x = _fill(a2afft_mod, torch, b, 4)
     x0 = x.cpu().numpy()
     y = x.clone()
