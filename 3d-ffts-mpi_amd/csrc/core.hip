@@ -302,6 +302,8 @@ static bool chunk_ok(const a2afft_plan_s* p, uint32_t chunk) {
     return true;
 }
 
+static bool is_split(a2afft_precision prec) { return prec == A2AFFT_SPLIT_BF16 || prec == A2AFFT_SPLIT_FP16; }
+
 static int plan_init(a2afft_plan_s* p) {
     p->scratch = nullptr;
     p->report[0] = 0;
@@ -312,6 +314,13 @@ static int plan_init(a2afft_plan_s* p) {
     FftShape sh;
     for (int i = 0; i < 3; ++i) {
         const int N = int(p->pass[i].fft.n);
+        if (is_split(p->prec)) {
+            A2A_TRY(mfma_pass_check(p->pass[i], N));
+            p->sel[i] = KSel{mf_tile_T(N, p->pass[i].strided), N, 0};
+            if (!mfma_twiddle_table(N, -1) || !mfma_twiddle_table(N, 1))
+                return set_error(A2AFFT_E_ALLOC, "plan: twiddle tables");
+            continue;
+        }
         if (!fft_length_supported(N)) return set_error(A2AFFT_E_UNSUPPORTED, "plan: length %d not supported", N);
         p->sel[i] = fft_default_sel(p->pass[i], p->d.dtype, N);
         A2A_TRY(fft_pass_shape(p->pass[i], p->d.dtype, N, &p->sel[i], &sh));
@@ -327,9 +336,8 @@ static int check_precision(a2afft_precision prec, a2afft_dtype dt) {
         return set_error(A2AFFT_E_INVALID_ARG, "plan_create: FP64 needs complex128 data");
     if (prec != A2AFFT_FP64 && dt != A2AFFT_C64)
         return set_error(A2AFFT_E_INVALID_ARG, "plan_create: FP32 / split precisions need complex64 data");
-    if (prec == A2AFFT_SPLIT_BF16 || prec == A2AFFT_SPLIT_FP16)
-        return set_error(A2AFFT_E_UNSUPPORTED, "plan_create: split precisions not built");
-    if (prec != A2AFFT_FP64 && prec != A2AFFT_FP32) return set_error(A2AFFT_E_INVALID_ARG, "plan_create: precision");
+    if (prec != A2AFFT_FP64 && prec != A2AFFT_FP32 && prec != A2AFFT_SPLIT_BF16 && prec != A2AFFT_SPLIT_FP16)
+        return set_error(A2AFFT_E_INVALID_ARG, "plan_create: precision");
     return A2AFFT_OK;
 }
 
@@ -339,6 +347,8 @@ extern "C" int a2afft_plan_create(a2afft_plan* out, a2afft_dist dist, a2afft_pre
     if (exec != A2AFFT_EXEC_FUSED && exec != A2AFFT_EXEC_STAGED)
         return set_error(A2AFFT_E_INVALID_ARG, "plan_create: bad exec");
     A2A_TRY(check_precision(prec, dist->dtype));
+    if (exec == A2AFFT_EXEC_STAGED && is_split(prec))
+        return set_error(A2AFFT_E_UNSUPPORTED, "plan_create: split precisions run the fused path only");
     if (int64_t(dist->P) * dist->nlocal >= (int64_t(1) << 40)) return set_error(A2AFFT_E_SIZE, "plan: too large");
     a2afft_plan_s* p = new (std::nothrow) a2afft_plan_s;
     if (!p) return set_error(A2AFFT_E_ALLOC, "plan_create");
@@ -436,6 +446,7 @@ static int run_launch(a2afft_plan_s* p, const Launch& L, void* D, int dir, hipSt
         ps.scale = double(int64_t(p->n[0]) * p->n[1] * p->n[2]);
         ps.scale_mode = 2;
     }
+    if (is_split(p->prec)) return launch_mfma_pass(ps, D, int(ps.fft.n), dir, p->prec, s);
     return launch_fft_pass(ps, D, p->d.dtype, int(ps.fft.n), dir, s, &p->sel[L.pass]);
 }
 
@@ -512,6 +523,7 @@ extern "C" int a2afft_plan_set_chunking(a2afft_plan p, int z_columns) {
 
 extern "C" int a2afft_plan_set_kernel(a2afft_plan p, int pass, int T, int E, int flags) {
     if (!p || pass < 0 || pass > 2) return set_error(A2AFFT_E_INVALID_ARG, "set_kernel: bad arguments");
+    if (is_split(p->prec)) return set_error(A2AFFT_E_UNSUPPORTED, "set_kernel: split plans have one kernel per pass");
     KSel s{T, E, flags};
     KSel cand[256];
     const int n = fft_candidates(p->pass[pass], p->d.dtype, int(p->pass[pass].fft.n), cand, 256);
@@ -546,7 +558,7 @@ extern "C" int a2afft_plan_autotune(a2afft_plan p, void* work, int iters, void* 
     int len = 0;
     const char* names[3] = {"z", "x", "y"};
     int st = A2AFFT_OK;
-    for (int i = 0; i < 3 && st >= 0; ++i) {
+    for (int i = 0; i < 3 && st >= 0 && !is_split(p->prec); ++i) {
         KSel cand[256];
         const int N = int(p->pass[i].fft.n);
         const int n = fft_candidates(p->pass[i], p->d.dtype, N, cand, 256);
@@ -614,6 +626,11 @@ extern "C" int a2afft_plan_describe(a2afft_plan p, char* buf, int len) {
     const char* names[3] = {"z", "x", "y"};
     int n = 0;
     for (int i = 0; i < 3 && n < len; ++i) {
+        if (is_split(p->prec)) {
+            n += snprintf(buf + n, len - n, "chosen %s split-%s MFMA radix-16 T=%d NT=256\n", names[i],
+                          p->prec == A2AFFT_SPLIT_BF16 ? "bf16" : "fp16", p->sel[i].T);
+            continue;
+        }
         FftShape sh;
         if (fft_pass_shape(p->pass[i], p->d.dtype, int(p->pass[i].fft.n), &p->sel[i], &sh) < 0) continue;
         n += snprintf(buf + n, len - n, "chosen %s T=%d E=%d NT=%d flags=%d grid=%d\n", names[i], sh.T, sh.E,

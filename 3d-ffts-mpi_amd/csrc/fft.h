@@ -47,6 +47,13 @@ int fft_pass_shape(const AxisPass& pass, a2afft_dtype dtype, int N, const KSel* 
 int fft_candidates(const AxisPass& pass, a2afft_dtype dtype, int N, KSel* out, int max);
 KSel fft_default_sel(const AxisPass& pass, a2afft_dtype dtype, int N);
 
+// Split-precision passes (radix-16 stages on MFMA, fft_mfma.hip).
+// mfma_pass_check: E_UNSUPPORTED unless the pass can run split.
+int mfma_pass_check(const AxisPass& pass, int N);
+int launch_mfma_pass(const AxisPass& pass, void* data, int N, int dir, a2afft_precision prec, hipStream_t s);
+const void* mfma_twiddle_table(int N, int dir);
+int mf_tile_T(int N, int strided);
+
 // exp(dir*2*pi*i*m/M) in long double (quadrant-reduced), for table building.
 void twiddle_ld(int64_t m, int64_t M, int dir, long double* re, long double* im);
 

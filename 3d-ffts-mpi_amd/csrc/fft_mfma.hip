@@ -1,0 +1,606 @@
+// Split-precision Stockham passes with the radix-16 DFT stages on MFMA.
+//
+// A radix-16 stage applies the 16-point DFT matrix F (F[q][r] = W16^(DIR*q*r))
+// to every butterfly of the line.  Sixteen butterflies form the 16 columns of
+// one matrix product on v_mfma_f32_16x16x32_{bf16,f16}, in the real form
+//   [Re y]   [Re F  -Im F] [Re x]
+//   [Im y] = [Im F   Re F] [Im x]      (K = 32: real parts, then imaginary)
+// computed as two 16x16x32 products (real rows, imaginary rows).  Operands
+// are split into hi + lo halves of the 16-bit type (hi = round(x),
+// lo = round(x - hi)) and each product is hi*hi + hi*lo + lo*hi with FP32
+// accumulation.  The twiddles between stages are FP32 complex products,
+// applied to the stage's outputs (for the next stage), from tables built in
+// long double and rounded once.  The FP16 split scales each butterfly by a
+// power of two before the split (FP16's range) and undoes it exactly after.
+//
+// Schedule: a radix-16 stages, then one VALU stage of radix R = N / 16^a
+// (2, 4 or 8) when N is not a power of 16 -- 512 = 16 * 16 * 2.
+//
+// Generic kernel (lengths 16..128): 256 threads, a tile of T lines (contiguous lines, or T adjacent
+// columns for strided passes) staged in LDS as a real and an imaginary
+// plane of floats.  Each wavefront owns T/4 whole lines during the MFMA
+// stages, so those need no barriers; the load, the final stage and the store
+// use a coalescing thread map and are separated by two barriers.
+//
+// The operand layouts used: A (16 x 32) lane l holds row l % 16, B (32 x 16)
+// lane l holds column l % 16, each with 8 K values chosen by l / 16 -- the
+// same (l / 16, element) -> k map for A and B, so the sum over k does not
+// depend on it; the result lane l holds column l % 16, rows 4 * (l / 16) + i.
+#include "fft.h"
+#include "stockham.h"
+
+#include <cmath>
+#include <map>
+#include <mutex>
+#include <vector>
+
+namespace a2a {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+template <typename H> struct MfTraits;
+template <> struct MfTraits<__bf16> {
+    using V8 = bf16x8;
+    static constexpr bool kScale = false;
+    __device__ static f32x4 mma(V8 a, V8 b, f32x4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
+};
+template <> struct MfTraits<_Float16> {
+    using V8 = f16x8;
+    static constexpr bool kScale = true;
+    __device__ static f32x4 mma(V8 a, V8 b, f32x4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0); }
+};
+
+constexpr int kMfThreads = 256;
+
+A2A_HD constexpr int mf_stages16(int N) { return N >= 4096 ? 3 : N >= 256 ? 2 : N >= 16 ? 1 : 0; }
+A2A_HD constexpr int mf_pow16(int a) { return a == 0 ? 1 : 16 * mf_pow16(a - 1); }
+A2A_HD constexpr int mf_rlast(int N) { return N / mf_pow16(mf_stages16(N)); }   // 1, 2, 4 or 8
+A2A_HD constexpr int mf_nstages(int N) { return mf_stages16(N) + (mf_rlast(N) > 1 ? 1 : 0); }
+A2A_HD constexpr int mf_radix(int N, int p) { return p < mf_stages16(N) ? 16 : mf_rlast(N); }
+A2A_HD constexpr int mf_ns(int N, int p) { return mf_pow16(p < mf_stages16(N) ? p : mf_stages16(N)); }
+// Table: 16 roots W16^t, then for stage p >= 1 entries [k*R + r], k < Ns, r < R.
+A2A_HD constexpr int mf_twoff(int N, int p) {
+    int off = 16;
+    for (int i = 1; i < p; ++i) off += mf_ns(N, i) * mf_radix(N, i);
+    return off;
+}
+A2A_HD constexpr int mf_twsize(int N) { return mf_twoff(N, mf_nstages(N)); }
+// Tile width: 128 B rows for strided tiles; enough lines that each of the 4
+// wavefronts owns whole lines holding a multiple of 16 butterflies.
+A2A_HD constexpr int mf_T(int N, bool strided) {
+    return N >= 256 ? 16 : ((1024 / N) > (strided ? 16 : 4) ? (1024 / N) : (strided ? 16 : 4));
+}
+A2A_HD constexpr int mf_ld(int N) { return N + N / 16; }   // padded plane row
+A2A_HD constexpr int mf_slot(int d) { return d + (d >> 4); }
+
+bool mf_length_ok(int N) { return N >= 16 && N <= 512 && (N & (N - 1)) == 0; }
+
+template <typename H>
+__device__ __forceinline__ void split2(float x, H& hi, H& lo) {
+    hi = H(x);
+    lo = H(x - float(hi));
+}
+
+template <typename H, int N, int T, bool STRIDED, int DIR>
+__global__ __launch_bounds__(kMfThreads) void mfma_fft_kernel(float2* __restrict__ data, FftGeom g,
+                                                               const float2* __restrict__ tw) {
+    using Tr = MfTraits<H>;
+    using V8 = typename Tr::V8;
+    constexpr int LD = mf_ld(N);
+    constexpr int A16 = mf_stages16(N);
+    constexpr int RL = mf_rlast(N);
+    constexpr int NST = mf_nstages(N);
+    constexpr int LW = T / 4;            // lines per wavefront
+    constexpr int BPL = N / 16;          // radix-16 butterflies per line
+    constexpr int CL = BPL >= 16 ? 1 : 16 / BPL;   // lines per chunk of whole MFMA column groups
+    constexpr int GC = CL * BPL / 16;              // column groups per chunk
+    constexpr int TPLC = kMfThreads / T;           // contiguous map: threads per line
+    static_assert(T % 4 == 0 && LW % CL == 0, "tile does not split into MFMA groups");
+    __shared__ float lds[2 * T * LD];
+    float* const re = lds;
+    float* const im = lds + T * LD;
+
+    const int tid = int(threadIdx.x);
+    const uint32_t block = xcd_block(blockIdx.x, gridDim.x);
+
+    // ---- load: global -> LDS planes (coalescing map) ----
+    const int mline = STRIDED ? tid % T : tid / TPLC;   // line of this thread in the load / store map
+    const int m0 = STRIDED ? tid / T : tid % TPLC;      // first element
+    constexpr int MSTEP = STRIDED ? kMfThreads / T : TPLC;
+    bool valid;
+    const int64_t lbase = line_base(g, STRIDED, T, 1, block, mline, &valid, false);
+#pragma unroll 4
+    for (int d = m0; d < N; d += MSTEP) {
+        float2 v = make_float2(0.f, 0.f);
+        if (valid) v = data[lbase + g.fft.off(uint32_t(d))];
+        re[mline * LD + mf_slot(d)] = v.x;
+        im[mline * LD + mf_slot(d)] = v.y;
+    }
+    __syncthreads();
+
+    // ---- radix-16 stages on MFMA, each wavefront on its own lines ----
+    const int lane = tid & 63;
+    const int cn = lane & 15;   // column (butterfly) of this lane; row of A
+    const int kg = lane >> 4;   // K group: 0,1 real parts x[0..7], x[8..15]; 2,3 imaginary parts
+    const int L0 = (tid >> 6) * LW;
+    // A operands: rows of [Re F, -Im F] and [Im F, Re F], split hi + lo.
+    V8 arh, arl, aih, ail;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int r = 8 * (kg & 1) + i;
+        const float2 w = tw[(cn * r) & 15];
+        const float ar = kg < 2 ? w.x : -w.y;
+        const float ai = kg < 2 ? w.y : w.x;
+        H h, l;
+        split2<H>(ar, h, l);
+        arh[i] = h;
+        arl[i] = l;
+        split2<H>(ai, h, l);
+        aih[i] = h;
+        ail[i] = l;
+    }
+    const float* const src = kg < 2 ? re : im;
+#pragma unroll
+    for (int p = 0; p < A16; ++p) {
+        const int Ns = mf_pow16(p);
+        // One chunk of CL lines (GC column groups) at a time: all of a chunk's
+        // operands are read before any of its outputs is written, and chunks
+        // (whole lines) do not overlap.
+#pragma unroll 1
+        for (int c = 0; c < LW / CL; ++c) {
+            V8 bh[GC], bl[GC];
+            float sc[GC];
+#pragma unroll
+            for (int G = 0; G < GC; ++G) {
+                const int b = G * 16 + cn;
+                const int line = L0 + c * CL + b / BPL, j = b % BPL;
+                float x[8];
+#pragma unroll
+                for (int i = 0; i < 8; ++i) x[i] = src[line * LD + mf_slot(j + (8 * (kg & 1) + i) * BPL)];
+                float s = 1.f;
+                if constexpr (Tr::kScale) {
+                    // power of two that brings the butterfly's largest part near 2^14
+                    float mx = 0.f;
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) mx = fmaxf(mx, fabsf(x[i]));
+                    mx = fmaxf(mx, __shfl_xor(mx, 16));
+                    mx = fmaxf(mx, __shfl_xor(mx, 32));
+                    int e = 0;
+                    (void)frexpf(mx, &e);
+                    const int sh = 14 - e;
+                    s = mx > 0.f ? ldexpf(1.f, sh < -126 ? -126 : (sh > 126 ? 126 : sh)) : 1.f;
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) x[i] *= s;
+                }
+                sc[G] = s;
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    H h, l;
+                    split2<H>(x[i], h, l);
+                    bh[G][i] = h;
+                    bl[G][i] = l;
+                }
+            }
+#pragma unroll
+            for (int G = 0; G < GC; ++G) {
+                f32x4 yr = {0.f, 0.f, 0.f, 0.f}, yi = {0.f, 0.f, 0.f, 0.f};
+                yr = Tr::mma(arh, bh[G], yr);
+                yr = Tr::mma(arh, bl[G], yr);
+                yr = Tr::mma(arl, bh[G], yr);
+                yi = Tr::mma(aih, bh[G], yi);
+                yi = Tr::mma(aih, bl[G], yi);
+                yi = Tr::mma(ail, bh[G], yi);
+                const int b = G * 16 + cn;
+                const int line = L0 + c * CL + b / BPL, j = b % BPL;
+                const float inv = 1.f / sc[G];   // exact: a power of two
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int q = 4 * kg + i;
+                    const int d = (j / Ns) * (Ns * 16) + (j % Ns) + q * Ns;
+                    float2 y = make_float2(yr[i] * inv, yi[i] * inv);
+                    if (p + 1 < NST) {   // twiddle of the next stage for the element it will read at d
+                        const int R2 = mf_radix(N, p + 1), Ns2 = mf_ns(N, p + 1);
+                        const int j2 = d % (N / R2), r2 = d / (N / R2), k2 = j2 % Ns2;
+                        y = c_mul(y, tw[mf_twoff(N, p + 1) + k2 * R2 + r2]);
+                    }
+                    re[line * LD + mf_slot(d)] = y.x;
+                    im[line * LD + mf_slot(d)] = y.y;
+                }
+            }
+        }
+    }
+    __syncthreads();
+
+    // ---- last stage (radix RL on VALU) fused with the store ----
+    const int scale_mode = g.scale_mode;
+    const float fs = float(g.scale);
+    auto put = [&](int d, float2 v) {
+        if (scale_mode == 1) { v.x *= fs; v.y *= fs; }
+        else if (scale_mode == 2) { v.x = v.x / fs; v.y = v.y / fs; }
+        data[lbase + g.fft.off(uint32_t(d))] = v;
+    };
+    if (!valid) return;
+    if constexpr (RL == 1) {
+#pragma unroll 4
+        for (int d = m0; d < N; d += MSTEP) put(d, make_float2(re[mline * LD + mf_slot(d)], im[mline * LD + mf_slot(d)]));
+    } else {
+        constexpr int NB = N / RL;   // butterflies of the last stage per line
+#pragma unroll 4
+        for (int j = m0; j < NB; j += MSTEP) {
+            float2 x[RL];
+#pragma unroll
+            for (int r = 0; r < RL; ++r)
+                x[r] = make_float2(re[mline * LD + mf_slot(j + r * NB)], im[mline * LD + mf_slot(j + r * NB)]);
+            dft<RL, DIR>(x);
+#pragma unroll
+            for (int q = 0; q < RL; ++q) put(j + q * NB, x[q]);
+        }
+    }
+}
+
+
+// Splits 8 floats into hi and lo operand vectors, converting in pairs
+// (one v_cvt_pk per pair for each half).
+template <typename H> struct Split8;
+template <> struct Split8<__bf16> {
+    __device__ static void run(const float (&x)[8], bf16x8& hi, bf16x8& lo) {
+        typedef __bf16 b2 __attribute__((ext_vector_type(2)));
+        typedef float f2 __attribute__((ext_vector_type(2)));
+        uint32_t hp[4], lp[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const f2 v = {x[2 * k], x[2 * k + 1]};
+            hp[k] = __builtin_bit_cast(uint32_t, __builtin_convertvector(v, b2));
+            const f2 r = {x[2 * k] - __uint_as_float(hp[k] << 16), x[2 * k + 1] - __uint_as_float(hp[k] & 0xffff0000u)};
+            lp[k] = __builtin_bit_cast(uint32_t, __builtin_convertvector(r, b2));
+        }
+        typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+        const u4 h = {hp[0], hp[1], hp[2], hp[3]}, l = {lp[0], lp[1], lp[2], lp[3]};
+        hi = __builtin_bit_cast(bf16x8, h);
+        lo = __builtin_bit_cast(bf16x8, l);
+    }
+};
+template <> struct Split8<_Float16> {
+    __device__ static void run(const float (&x)[8], f16x8& hi, f16x8& lo) {
+        typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+        typedef float f2 __attribute__((ext_vector_type(2)));
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const f2 v = {x[2 * k], x[2 * k + 1]};
+            const h2 h = __builtin_convertvector(v, h2);
+            const f2 r = {x[2 * k] - float(h[0]), x[2 * k + 1] - float(h[1])};
+            const h2 l = __builtin_convertvector(r, h2);
+            hi[2 * k] = h[0];
+            hi[2 * k + 1] = h[1];
+            lo[2 * k] = l[0];
+            lo[2 * k + 1] = l[1];
+        }
+    }
+};
+
+// Lines of 256 = 16 * 16 and 512 = 16 * 16 * 2 in tiles of 16 lines, with one
+// LDS exchange, like the FP32 kernels: the first radix-16 stage takes its
+// operands straight from global memory and the last stage (radix 16, or the
+// radix-2 stage of 512) stores straight to it.  A stage-0 column group is
+// one butterfly index across the 16 lines of a strided tile (lanes l % 16 =
+// the 16 adjacent columns) or 16 consecutive butterflies of one contiguous
+// line, so each load and store instruction covers whole 128-byte rows; each
+// lane loads the real or the imaginary float its K slot needs.  For 512 the
+// two stage-1 groups whose outputs meet in a radix-2 butterfly (butterflies j
+// and j + 16) are computed by the same wavefront, so the radix-2 stage runs
+// in registers.
+//
+// Index algebra (BPL = N/16 butterflies per line, q = 4*(l/16) + i):
+//   stage 0 butterfly j: inputs j + r*BPL, output q at d = 16 j + q, LDS slot
+//     17 j + q; stage-1 twiddle W_{16 BPL}^{(d mod 16) (d / BPL)} = table entry
+//     16 + 16 q + j / (BPL/16).
+//   stage 1 butterfly j1: inputs at slots slot(j1) + r * 17 BPL/16; output q
+//     at d = j1 + 16 q (+ 256 for j1 >= 16); radix-2 twiddle of d: 272 + 2 d + 1.
+template <typename H, int N, bool STRIDED, int DIR>
+__global__ __launch_bounds__(kMfThreads) void mfma_fft_direct(float2* __restrict__ data, FftGeom g,
+                                                               const float2* __restrict__ tw) {
+    using Tr = MfTraits<H>;
+    using V8 = typename Tr::V8;
+    static_assert(N == 256 || N == 512, "direct kernel lengths");
+    constexpr int T = 16;
+    constexpr int BPL = N / 16;       // radix-16 butterflies per line
+    constexpr int JB = BPL / 16;      // contiguous: column groups per line (1 or 2)
+    constexpr int GW = N / 64;        // column groups per wavefront and stage
+    constexpr int LD = N + N / 16 + 1;   // odd row: the 16 lines of a strided group hit distinct banks
+    constexpr int RS = 17 * BPL / 16;    // LDS slot step between the inputs of a stage-1 butterfly
+    __shared__ float lds[2 * T * LD];
+
+    const int tid = int(threadIdx.x);
+    const int lane = tid & 63, cn = lane & 15, kg = lane >> 4, kh = kg & 1, ri = kg >> 1, w = tid >> 6;
+    const uint32_t block = xcd_block(blockIdx.x, gridDim.x);
+    const float* const dataf = reinterpret_cast<const float*>(data);
+
+    // line bases: strided lanes stay on column cn; contiguous wavefront w owns lines 4w .. 4w+3
+    int64_t base[4];
+    bool valid[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        const int line = STRIDED ? cn : 4 * w + a;
+        base[a] = line_base(g, STRIDED, T, 1, block, line, &valid[a], false);
+    }
+
+    // ---- stage 0 operands from global memory ----
+    // off(j + r*BPL) = off(j) + off(r*BPL): the axis block extent and BPL are
+    // powers of two, so one divides the other and j < BPL never carries.
+    int64_t offr[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) offr[i] = g.fft.off(uint32_t((8 * kh + i) * BPL));
+    float x[GW][8];
+#pragma unroll
+    for (int gi = 0; gi < GW; ++gi) {
+        const int a = STRIDED ? 0 : gi / JB;
+        const int j = STRIDED ? w * GW + gi : (gi % JB) * 16 + cn;
+        const float* const pl = dataf + ri + 2 * (base[a] + g.fft.off(uint32_t(j)));
+#pragma unroll
+        for (int i = 0; i < 8; ++i) x[gi][i] = valid[a] ? pl[2 * offr[i]] : 0.f;
+    }
+
+    V8 arh, arl, aih, ail;
+    {
+        float fr[8], fi[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const float2 wv = tw[(cn * (8 * kh + i)) & 15];
+            fr[i] = kg < 2 ? wv.x : -wv.y;
+            fi[i] = kg < 2 ? wv.y : wv.x;
+        }
+        Split8<H>::run(fr, arh, arl);
+        Split8<H>::run(fi, aih, ail);
+    }
+
+    // one radix-16 DFT of the lane's column group: y[i] = output 4*kg + i
+    auto dft16_group = [&](float (&xv)[8], float2 (&y)[4]) {
+        float s = 1.f;
+        if constexpr (Tr::kScale) {
+            float mx = 0.f;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) mx = fmaxf(mx, fabsf(xv[i]));
+            mx = fmaxf(mx, __shfl_xor(mx, 16));
+            mx = fmaxf(mx, __shfl_xor(mx, 32));
+            int e = 0;
+            (void)frexpf(mx, &e);
+            const int sh = 14 - e;
+            s = mx > 0.f ? ldexpf(1.f, sh < -126 ? -126 : (sh > 126 ? 126 : sh)) : 1.f;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) xv[i] *= s;
+        }
+        V8 bh, bl;
+        Split8<H>::run(xv, bh, bl);
+        f32x4 yr = {0.f, 0.f, 0.f, 0.f}, yi = {0.f, 0.f, 0.f, 0.f};
+        yr = Tr::mma(arh, bh, yr);
+        yr = Tr::mma(arh, bl, yr);
+        yr = Tr::mma(arl, bh, yr);
+        yi = Tr::mma(aih, bh, yi);
+        yi = Tr::mma(aih, bl, yi);
+        yi = Tr::mma(ail, bh, yi);
+        if constexpr (Tr::kScale) {
+            const float inv = 1.f / s;   // exact: a power of two
+#pragma unroll
+            for (int i = 0; i < 4; ++i) y[i] = make_float2(yr[i] * inv, yi[i] * inv);
+        } else {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) y[i] = make_float2(yr[i], yi[i]);
+        }
+    };
+
+    // ---- stage 0 (Ns = 1): outputs to LDS with the stage-1 twiddles ----
+    {
+        const int lrow = STRIDED ? cn : 4 * w;   // + gi / JB for contiguous
+        float* const wre = lds + lrow * LD + 4 * kg;
+        float* const wim = wre + T * LD;
+        const float2* const twq = tw + 16 + 16 * 4 * kg;   // + 16 i + j / JB
+#pragma unroll
+        for (int gi = 0; gi < GW; ++gi) {
+            const int lo = STRIDED ? 0 : (gi / JB) * LD;
+            const int j = STRIDED ? w * GW + gi : (gi % JB) * 16 + cn;
+            float2 y[4];
+            dft16_group(x[gi], y);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const float2 v = c_mul(y[i], twq[16 * i + j / JB]);
+                wre[lo + 17 * j + i] = v.x;
+                wim[lo + 17 * j + i] = v.y;
+            }
+        }
+    }
+    __syncthreads();
+
+    // ---- stage 1 (Ns = 16), then the radix-2 stage of 512, to global memory ----
+    float xb[GW][8];
+    {
+        const float* const rp = lds + ri * T * LD + 8 * kh * RS;
+#pragma unroll
+        for (int gi = 0; gi < GW; ++gi) {
+            // pair index a = gi % 4; for 512 the second half (gi >= 4) is butterfly j1 + 16
+            const int a = gi % 4, hi = gi / 4;
+            const int line = STRIDED ? cn : 4 * w + a;
+            const int j1 = STRIDED ? 16 * hi + 4 * w + a : 16 * hi + cn;
+            const float* const q = rp + line * LD + j1 + j1 / 16;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) xb[gi][i] = q[i * RS];
+        }
+    }
+    // the inverse's 1/(nx*ny*nz) as one multiplication (exact for powers of two)
+    const bool scaled = g.scale_mode != 0;
+    const float fs = g.scale_mode == 2 ? float(1.0 / g.scale) : float(g.scale);
+    // outputs d = jlo + 16*q (+ 256): off(d) = off(jlo) + off(16*q) (+ off(256))
+    int64_t offq[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) offq[i] = g.fft.off(uint32_t(16 * (4 * kg + i)));
+    const int64_t off256 = N == 512 ? g.fft.off(256u) : 0;
+    auto put = [&](float2* p, float2 v) {
+        if (scaled) { v.x *= fs; v.y *= fs; }
+        *p = v;
+    };
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        const int ab = STRIDED ? 0 : a;
+        const int jlo = STRIDED ? 4 * w + a : cn;   // j1 % 16
+        float2* const po = data + base[ab] + g.fft.off(uint32_t(jlo));
+        float2 y0[4];
+        dft16_group(xb[a], y0);
+        if constexpr (N == 256) {
+            if (valid[ab]) {
+#pragma unroll
+                for (int i = 0; i < 4; ++i) put(po + offq[i], y0[i]);
+            }
+        } else {
+            float2 y1[4];
+            dft16_group(xb[a + 4], y1);
+            const float2* const t2 = tw + mf_twoff(N, 2) + 1 + 2 * jlo + 32 * 4 * kg;   // + 32 i
+            if (valid[ab]) {
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const float2 t = c_mul(y1[i], t2[32 * i]);
+                    put(po + offq[i], c_add(y0[i], t));
+                    put(po + offq[i] + off256, c_sub(y0[i], t));
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Registry and launch.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct MfEntry {
+    int N, strided, dir, fp16;
+    const void* fn;
+};
+
+template <typename H, int N, bool S>
+void mf_add(std::vector<MfEntry>& v) {
+    constexpr int T = mf_T(N, S);
+    constexpr int f16 = std::is_same<H, _Float16>::value ? 1 : 0;
+    if constexpr (N == 256 || N == 512) {
+        v.push_back(MfEntry{N, S, -1, f16, (const void*)&mfma_fft_direct<H, N, S, -1>});
+        v.push_back(MfEntry{N, S, 1, f16, (const void*)&mfma_fft_direct<H, N, S, 1>});
+    } else {
+        v.push_back(MfEntry{N, S, -1, f16, (const void*)&mfma_fft_kernel<H, N, T, S, -1>});
+        v.push_back(MfEntry{N, S, 1, f16, (const void*)&mfma_fft_kernel<H, N, T, S, 1>});
+    }
+}
+
+template <typename H, int N>
+void mf_add_len(std::vector<MfEntry>& v) {
+    mf_add<H, N, false>(v);
+    mf_add<H, N, true>(v);
+}
+
+const std::vector<MfEntry>& mf_registry() {
+    static const std::vector<MfEntry> reg = [] {
+        std::vector<MfEntry> v;
+        mf_add_len<__bf16, 16>(v);
+        mf_add_len<__bf16, 32>(v);
+        mf_add_len<__bf16, 64>(v);
+        mf_add_len<__bf16, 128>(v);
+        mf_add_len<__bf16, 256>(v);
+        mf_add_len<__bf16, 512>(v);
+        mf_add_len<_Float16, 16>(v);
+        mf_add_len<_Float16, 32>(v);
+        mf_add_len<_Float16, 64>(v);
+        mf_add_len<_Float16, 128>(v);
+        mf_add_len<_Float16, 256>(v);
+        mf_add_len<_Float16, 512>(v);
+        return v;
+    }();
+    return reg;
+}
+
+struct MfKey {
+    int dev, N, dir;
+    bool operator<(const MfKey& o) const {
+        if (dev != o.dev) return dev < o.dev;
+        if (N != o.N) return N < o.N;
+        return dir < o.dir;
+    }
+};
+std::mutex g_mf_mu;
+std::map<MfKey, void*> g_mf_tw;
+
+}  // namespace
+
+int mf_tile_T(int N, int strided) { return mf_T(N, strided != 0); }
+
+// Twiddles of the MFMA schedule (FP32, rounded once from long double).
+const void* mfma_twiddle_table(int N, int dir) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+    const MfKey key{dev, N, dir < 0 ? -1 : 1};
+    std::lock_guard<std::mutex> lk(g_mf_mu);
+    auto it = g_mf_tw.find(key);
+    if (it != g_mf_tw.end()) return it->second;
+    std::vector<float2> h(size_t(mf_twsize(N)));
+    for (int t = 0; t < 16; ++t) {
+        long double r, i;
+        twiddle_ld(t, 16, key.dir, &r, &i);
+        h[size_t(t)] = make_float2(float(r), float(i));
+    }
+    for (int p = 1; p < mf_nstages(N); ++p) {
+        const int R = mf_radix(N, p), Ns = mf_ns(N, p), off = mf_twoff(N, p);
+        for (int k = 0; k < Ns; ++k)
+            for (int r = 0; r < R; ++r) {
+                long double re, im;
+                twiddle_ld(int64_t(r) * k, int64_t(Ns) * R, key.dir, &re, &im);
+                h[size_t(off + k * R + r)] = make_float2(float(re), float(im));
+            }
+    }
+    void* d = nullptr;
+    if (hipMalloc(&d, h.size() * sizeof(float2)) != hipSuccess) return nullptr;
+    if (hipMemcpy(d, h.data(), h.size() * sizeof(float2), hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(d);
+        return nullptr;
+    }
+    g_mf_tw[key] = d;
+    return d;
+}
+
+// Geometry check for a split-precision pass: N in 16..512 and a tile width
+// the axis admits.
+int mfma_pass_check(const AxisPass& p, int N) {
+    if (!mf_length_ok(N)) return set_error(A2AFFT_E_UNSUPPORTED, "split precision: length %d (16..512)", N);
+    const int T = mf_T(N, p.strided != 0);
+    FftGeom g;
+    uint32_t grid = 0;
+    const int st = resolve_with(p.fft, p.a1, p.a2, p.strided, N, T, N, &g, &grid, p.a2_start, p.a2_count);
+    if (st < 0)
+        return set_error(A2AFFT_E_UNSUPPORTED, "split precision: pass of length %d does not admit %d-line tiles", N,
+                         T);
+    return A2AFFT_OK;
+}
+
+int launch_mfma_pass(const AxisPass& p, void* data, int N, int dir, a2afft_precision prec, hipStream_t s) {
+    if (!data) return set_error(A2AFFT_E_INVALID_ARG, "fft: null data");
+    if (!mf_length_ok(N)) return set_error(A2AFFT_E_UNSUPPORTED, "split precision: length %d (16..512)", N);
+    const int T = mf_T(N, p.strided != 0);
+    FftGeom g;
+    uint32_t grid = 0;
+    int st = resolve_with(p.fft, p.a1, p.a2, p.strided, N, T, N, &g, &grid, p.a2_start, p.a2_count);
+    if (st < 0) return set_error(A2AFFT_E_UNSUPPORTED, "split precision: geometry (N=%d, T=%d)", N, T);
+    g.scale_mode = p.scale_mode;
+    g.scale = p.scale;
+    if (grid == 0) return A2AFFT_OK;
+    const void* tw = mfma_twiddle_table(N, dir);
+    if (!tw) return set_error(A2AFFT_E_ALLOC, "split precision: twiddle table N=%d", N);
+    const int f16 = prec == A2AFFT_SPLIT_FP16 ? 1 : 0;
+    const void* fn = nullptr;
+    for (const MfEntry& e : mf_registry())
+        if (e.N == N && e.strided == (p.strided != 0) && e.dir == (dir < 0 ? -1 : 1) && e.fp16 == f16) fn = e.fn;
+    if (!fn) return set_error(A2AFFT_E_UNSUPPORTED, "split precision: no kernel for N=%d", N);
+    float2* dp = (float2*)data;
+    void* args[] = {&dp, &g, (void*)&tw};
+    st = hip_check(int(hipLaunchKernel(fn, dim3(grid), dim3(kMfThreads), args, 0, s)), "mfma fft launch");
+    if (st < 0) return st;
+    return hip_check(int(hipGetLastError()), "mfma fft launch");
+}
+
+}  // namespace a2a

@@ -245,7 +245,7 @@ def main():
                          "c2: 256^3 complex128 over dims [1,2,4], N(0,1) seed 1; "
                          "c3b: one rank's 2048x256x256 complex64 share of 2048^3 over dims [1,8,8], x stage; "
                          "c5: 64 uneven complex64 grids (seed 4) in one persistent launch")
-    ap.add_argument("--precision", default=None, choices=["fp32", "fp64"])
+    ap.add_argument("--precision", default=None, choices=["fp32", "fp64", "split_bf16", "split_fp16"])
     ap.add_argument("--ng", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--autotune", action="store_true", help="time every kernel variant per pass and keep the fastest")
@@ -272,7 +272,7 @@ def main():
     else:
         Ng, P, dims, seed, field = args.ng or 512, 16, (1, 4, 4), 2, "grf"
         args.precision = args.precision or "fp32"
-    dtype = "c64" if args.precision == "fp32" else "c128"
+    dtype = "c128" if args.precision == "fp64" else "c64"
     esize = 8 if dtype == "c64" else 16
     N = Ng ** 3
 
@@ -406,7 +406,8 @@ def main():
         "higher_is_better": False,
         "scaling": "strong",
         "vs_baseline": None,
-        "dtype": "f32" if args.precision == "fp32" else "f64",
+        "dtype": {"fp32": "f32", "fp64": "f64", "split_bf16": "bf16x2 (MFMA) + f32",
+                  "split_fp16": "fp16x2 (MFMA) + f32"}[args.precision],
         "data": "synthetic",
         "config": {"workload": f"{Ng}^3 c2c forward 3D FFT, {'complex64' if dtype == 'c64' else 'complex128'}, "
                                f"{P} virtual ranks dims {list(dims)}, "

@@ -1,0 +1,105 @@
+"""GPU tests of the split-precision passes (radix-16 DFT stages as matrix
+products on MFMA with hi + lo operand halves, FP32 accumulation).
+
+The bar is the error against a complex128 reference of the same transform:
+rel-L2 <= 1e-5 for the BF16 split (hi*hi + hi*lo + lo*hi keeps about 16 of
+FP32's 24 mantissa bits per product) and <= 1e-6 for the FP16 split (about
+22 bits, with a power-of-two scale per butterfly for FP16's range).  A wrong
+lane layout, twiddle or output position gives O(1) errors, so these bounds
+also pin the index arithmetic."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"split_bf16": 1e-5, "split_fp16": 1e-6}
+GRIDS = [(64, 64, 64), (32, 32, 32), (128, 32, 256), (512, 64, 32), (16, 256, 64), (32, 128, 64)]
+
+
+def _rand(torch, shape, seed):
+    g = np.random.default_rng(seed)
+    x = (g.uniform(-1, 1, shape) + 1j * g.uniform(-1, 1, shape)).astype(np.complex64)
+    return x, torch.from_numpy(x.reshape(-1).copy()).cuda()
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.mark.parametrize("prec", ["split_bf16", "split_fp16"])
+@pytest.mark.parametrize("shape", GRIDS)
+def test_split_grid_forward_inverse_vs_numpy(a2afft_mod, cuda, prec, shape):
+    import torch
+    x, d = _rand(torch, shape, 11)
+    plan = a2afft_mod.GridDfft(shape, prec)
+    plan.forward(d)
+    got = d.cpu().numpy().reshape(shape)
+    ref = np.fft.fftn(x.astype(np.complex128))
+    err = _rel(got, ref)
+    assert err <= TOL[prec], (prec, shape, err)
+    plan.inverse(d)
+    back = d.cpu().numpy().reshape(shape)
+    err_i = _rel(back, x.astype(np.complex128))
+    assert err_i <= 2 * TOL[prec], (prec, shape, err_i)
+
+
+@pytest.mark.parametrize("prec", ["split_bf16", "split_fp16"])
+def test_split_impulses_give_exact_phases(a2afft_mod, cuda, prec):
+    """A unit impulse at (a, b, c) transforms to exp(-2 pi i (ka/nx + lb/ny + mc/nz));
+    a permuted lane layout or output index moves the phases."""
+    import torch
+    shape = (32, 64, 128)
+    for pos in [(0, 0, 0), (1, 0, 0), (0, 3, 0), (0, 0, 5), (7, 11, 13)]:
+        x = np.zeros(shape, np.complex64)
+        x[pos] = 1.0
+        d = torch.from_numpy(x.reshape(-1).copy()).cuda()
+        a2afft_mod.GridDfft(shape, prec).forward(d)
+        got = d.cpu().numpy().reshape(shape)
+        k = [np.arange(n) for n in shape]
+        ph = (np.exp(-2j * np.pi * k[0] * pos[0] / shape[0])[:, None, None]
+              * np.exp(-2j * np.pi * k[1] * pos[1] / shape[1])[None, :, None]
+              * np.exp(-2j * np.pi * k[2] * pos[2] / shape[2])[None, None, :])
+        assert np.max(np.abs(got - ph)) < 30 * TOL[prec], (prec, pos)
+
+
+@pytest.mark.parametrize("prec", ["split_bf16", "split_fp16"])
+def test_split_distribution_plan_matches_fftn(a2afft_mod, cuda, prec):
+    """Rank-decomposed block layout (64^3 over dims [1,2,4]): stitched output
+    equals fftn of the stitched input."""
+    import torch
+    dist = a2afft_mod.Distribution(64, 8, dims=(1, 2, 4), dtype="c64")
+    buf = dist.empty()
+    a2afft_mod.fill_random(buf, 5, "uniform")
+    x = buf.cpu().numpy()
+    plan = a2afft_mod.Dfft(dist, prec)
+    plan.forward(buf)
+    from a2afft import inputs
+    glob_in = inputs.blocks_to_global(x, dist.dims, (64, 64, 64))
+    glob_out = inputs.blocks_to_global(buf.cpu().numpy(), dist.dims, (64, 64, 64))
+    err = _rel(glob_out, np.fft.fftn(glob_in.astype(np.complex128)))
+    assert err <= TOL[prec], (prec, err)
+
+
+@pytest.mark.parametrize("prec", ["split_bf16", "split_fp16"])
+def test_split_close_to_fp32_on_wide_dynamic_range(a2afft_mod, cuda, prec):
+    """Inputs spanning 2^-20 .. 2^20: the FP16 split's per-butterfly scaling
+    keeps the error relative to the transform's norm."""
+    import torch
+    shape = (64, 32, 128)
+    g = np.random.default_rng(3)
+    mag = 2.0 ** g.uniform(-20, 20, shape)
+    x = (mag * np.exp(2j * np.pi * g.uniform(0, 1, shape))).astype(np.complex64)
+    d = torch.from_numpy(x.reshape(-1).copy()).cuda()
+    a2afft_mod.GridDfft(shape, prec).forward(d)
+    got = d.cpu().numpy().reshape(shape)
+    err = _rel(got, np.fft.fftn(x.astype(np.complex128)))
+    assert err <= TOL[prec], (prec, err)
+
+
+def test_split_rejects_staged_and_short_lines(a2afft_mod, cuda):
+    from a2afft import _lib as L
+    dist = a2afft_mod.Distribution(32, 8, dtype="c64")
+    with pytest.raises(L.A2AFFTError):
+        a2afft_mod.Dfft(dist, "split_bf16", exec="staged")
+    with pytest.raises(L.A2AFFTError):
+        a2afft_mod.GridDfft((8, 64, 64), "split_bf16")
