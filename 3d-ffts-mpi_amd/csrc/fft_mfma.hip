@@ -30,6 +30,7 @@
 #include "stockham.h"
 
 #include <cmath>
+#include <type_traits>
 #include <map>
 #include <mutex>
 #include <vector>
@@ -298,16 +299,21 @@ template <> struct Split8<_Float16> {
 //     16 + 16 q + j / (BPL/16).
 //   stage 1 butterfly j1: inputs at slots slot(j1) + r * 17 BPL/16; output q
 //     at d = j1 + 16 q (+ 256 for j1 >= 16); radix-2 twiddle of d: 272 + 2 d + 1.
-template <typename H, int N, bool STRIDED, int DIR>
-__global__ __launch_bounds__(kMfThreads) void mfma_fft_direct(float2* __restrict__ data, FftGeom g,
-                                                               const float2* __restrict__ tw) {
+template <typename H, int N, bool STRIDED, int DIR, int NW, bool O32>
+__global__ __launch_bounds__(64 * NW, NW == 8 ? 4 : 2) void mfma_fft_direct(float2* __restrict__ data, FftGeom g,
+                                                            const float2* __restrict__ tw) {
+    // O32: every byte offset of the pass fits in 32 bits, so addresses are the
+    // buffer pointer plus a 32-bit lane offset.
+    using Off = typename std::conditional<O32, uint32_t, int64_t>::type;
     using Tr = MfTraits<H>;
     using V8 = typename Tr::V8;
     static_assert(N == 256 || N == 512, "direct kernel lengths");
+    static_assert(NW == 4 || NW == 8, "4 or 8 wavefronts");
     constexpr int T = 16;
     constexpr int BPL = N / 16;       // radix-16 butterflies per line
     constexpr int JB = BPL / 16;      // contiguous: column groups per line (1 or 2)
-    constexpr int GW = N / 64;        // column groups per wavefront and stage
+    constexpr int A = 16 / NW;        // contiguous: lines per wavefront; strided: stage-1 pairs per wavefront
+    constexpr int GW = BPL / NW;      // column groups per wavefront and stage
     constexpr int LD = N + N / 16 + 1;   // odd row: the 16 lines of a strided group hit distinct banks
     constexpr int RS = 17 * BPL / 16;    // LDS slot step between the inputs of a stage-1 butterfly
     __shared__ float lds[2 * T * LD];
@@ -315,31 +321,33 @@ __global__ __launch_bounds__(kMfThreads) void mfma_fft_direct(float2* __restrict
     const int tid = int(threadIdx.x);
     const int lane = tid & 63, cn = lane & 15, kg = lane >> 4, kh = kg & 1, ri = kg >> 1, w = tid >> 6;
     const uint32_t block = xcd_block(blockIdx.x, gridDim.x);
-    const float* const dataf = reinterpret_cast<const float*>(data);
+    const char* const datab = reinterpret_cast<const char*>(data);
+    char* const outb = reinterpret_cast<char*>(data);
 
-    // line bases: strided lanes stay on column cn; contiguous wavefront w owns lines 4w .. 4w+3
-    int64_t base[4];
-    bool valid[4];
+    // line bases (bytes): strided lanes stay on column cn; contiguous wavefront w owns lines A*w .. A*w+A-1
+    Off base[A];
+    bool valid[A];
 #pragma unroll
-    for (int a = 0; a < 4; ++a) {
-        const int line = STRIDED ? cn : 4 * w + a;
-        base[a] = line_base(g, STRIDED, T, 1, block, line, &valid[a], false);
+    for (int a = 0; a < A; ++a) {
+        const int line = STRIDED ? cn : A * w + a;
+        base[a] = Off(8 * line_base(g, STRIDED, T, 1, block, line, &valid[a], false));
     }
 
     // ---- stage 0 operands from global memory ----
     // off(j + r*BPL) = off(j) + off(r*BPL): the axis block extent and BPL are
     // powers of two, so one divides the other and j < BPL never carries.
-    int64_t offr[8];
+    Off offr[8];
 #pragma unroll
-    for (int i = 0; i < 8; ++i) offr[i] = g.fft.off(uint32_t((8 * kh + i) * BPL));
+    for (int i = 0; i < 8; ++i) offr[i] = Off(8 * g.fft.off(uint32_t((8 * kh + i) * BPL)));
     float x[GW][8];
 #pragma unroll
     for (int gi = 0; gi < GW; ++gi) {
         const int a = STRIDED ? 0 : gi / JB;
         const int j = STRIDED ? w * GW + gi : (gi % JB) * 16 + cn;
-        const float* const pl = dataf + ri + 2 * (base[a] + g.fft.off(uint32_t(j)));
+        const Off pl = base[a] + Off(8 * g.fft.off(uint32_t(j))) + Off(4 * ri);
 #pragma unroll
-        for (int i = 0; i < 8; ++i) x[gi][i] = valid[a] ? pl[2 * offr[i]] : 0.f;
+        for (int i = 0; i < 8; ++i)
+            x[gi][i] = valid[a] ? *reinterpret_cast<const float*>(datab + (pl + offr[i])) : 0.f;
     }
 
     V8 arh, arl, aih, ail;
@@ -392,7 +400,7 @@ __global__ __launch_bounds__(kMfThreads) void mfma_fft_direct(float2* __restrict
 
     // ---- stage 0 (Ns = 1): outputs to LDS with the stage-1 twiddles ----
     {
-        const int lrow = STRIDED ? cn : 4 * w;   // + gi / JB for contiguous
+        const int lrow = STRIDED ? cn : A * w;   // + gi / JB for contiguous
         float* const wre = lds + lrow * LD + 4 * kg;
         float* const wim = wre + T * LD;
         const float2* const twq = tw + 16 + 16 * 4 * kg;   // + 16 i + j / JB
@@ -418,10 +426,10 @@ __global__ __launch_bounds__(kMfThreads) void mfma_fft_direct(float2* __restrict
         const float* const rp = lds + ri * T * LD + 8 * kh * RS;
 #pragma unroll
         for (int gi = 0; gi < GW; ++gi) {
-            // pair index a = gi % 4; for 512 the second half (gi >= 4) is butterfly j1 + 16
-            const int a = gi % 4, hi = gi / 4;
-            const int line = STRIDED ? cn : 4 * w + a;
-            const int j1 = STRIDED ? 16 * hi + 4 * w + a : 16 * hi + cn;
+            // pair index a = gi % A; for 512 the second half (gi >= A) is butterfly j1 + 16
+            const int a = gi % A, hi = gi / A;
+            const int line = STRIDED ? cn : A * w + a;
+            const int j1 = STRIDED ? 16 * hi + A * w + a : 16 * hi + cn;
             const float* const q = rp + line * LD + j1 + j1 / 16;
 #pragma unroll
             for (int i = 0; i < 8; ++i) xb[gi][i] = q[i * RS];
@@ -431,19 +439,19 @@ __global__ __launch_bounds__(kMfThreads) void mfma_fft_direct(float2* __restrict
     const bool scaled = g.scale_mode != 0;
     const float fs = g.scale_mode == 2 ? float(1.0 / g.scale) : float(g.scale);
     // outputs d = jlo + 16*q (+ 256): off(d) = off(jlo) + off(16*q) (+ off(256))
-    int64_t offq[4];
+    Off offq[4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) offq[i] = g.fft.off(uint32_t(16 * (4 * kg + i)));
-    const int64_t off256 = N == 512 ? g.fft.off(256u) : 0;
-    auto put = [&](float2* p, float2 v) {
+    for (int i = 0; i < 4; ++i) offq[i] = Off(8 * g.fft.off(uint32_t(16 * (4 * kg + i))));
+    const Off off256 = N == 512 ? Off(8 * g.fft.off(256u)) : Off(0);
+    auto put = [&](Off o, float2 v) {
         if (scaled) { v.x *= fs; v.y *= fs; }
-        *p = v;
+        *reinterpret_cast<float2*>(outb + o) = v;
     };
 #pragma unroll
-    for (int a = 0; a < 4; ++a) {
+    for (int a = 0; a < A; ++a) {
         const int ab = STRIDED ? 0 : a;
-        const int jlo = STRIDED ? 4 * w + a : cn;   // j1 % 16
-        float2* const po = data + base[ab] + g.fft.off(uint32_t(jlo));
+        const int jlo = STRIDED ? A * w + a : cn;   // j1 % 16
+        const Off po = base[ab] + Off(8 * g.fft.off(uint32_t(jlo)));
         float2 y0[4];
         dft16_group(xb[a], y0);
         if constexpr (N == 256) {
@@ -453,7 +461,7 @@ __global__ __launch_bounds__(kMfThreads) void mfma_fft_direct(float2* __restrict
             }
         } else {
             float2 y1[4];
-            dft16_group(xb[a + 4], y1);
+            dft16_group(xb[a + A], y1);
             const float2* const t2 = tw + mf_twoff(N, 2) + 1 + 2 * jlo + 32 * 4 * kg;   // + 32 i
             if (valid[ab]) {
 #pragma unroll
@@ -473,7 +481,7 @@ __global__ __launch_bounds__(kMfThreads) void mfma_fft_direct(float2* __restrict
 namespace {
 
 struct MfEntry {
-    int N, strided, dir, fp16;
+    int N, strided, dir, fp16, nw, o32;   // nw: wavefronts per workgroup; o32: 32-bit byte offsets
     const void* fn;
 };
 
@@ -482,11 +490,17 @@ void mf_add(std::vector<MfEntry>& v) {
     constexpr int T = mf_T(N, S);
     constexpr int f16 = std::is_same<H, _Float16>::value ? 1 : 0;
     if constexpr (N == 256 || N == 512) {
-        v.push_back(MfEntry{N, S, -1, f16, (const void*)&mfma_fft_direct<H, N, S, -1>});
-        v.push_back(MfEntry{N, S, 1, f16, (const void*)&mfma_fft_direct<H, N, S, 1>});
+        v.push_back(MfEntry{N, S, -1, f16, 4, 0, (const void*)&mfma_fft_direct<H, N, S, -1, 4, false>});
+        v.push_back(MfEntry{N, S, 1, f16, 4, 0, (const void*)&mfma_fft_direct<H, N, S, 1, 4, false>});
+        v.push_back(MfEntry{N, S, -1, f16, 8, 0, (const void*)&mfma_fft_direct<H, N, S, -1, 8, false>});
+        v.push_back(MfEntry{N, S, 1, f16, 8, 0, (const void*)&mfma_fft_direct<H, N, S, 1, 8, false>});
+        v.push_back(MfEntry{N, S, -1, f16, 4, 1, (const void*)&mfma_fft_direct<H, N, S, -1, 4, true>});
+        v.push_back(MfEntry{N, S, 1, f16, 4, 1, (const void*)&mfma_fft_direct<H, N, S, 1, 4, true>});
+        v.push_back(MfEntry{N, S, -1, f16, 8, 1, (const void*)&mfma_fft_direct<H, N, S, -1, 8, true>});
+        v.push_back(MfEntry{N, S, 1, f16, 8, 1, (const void*)&mfma_fft_direct<H, N, S, 1, 8, true>});
     } else {
-        v.push_back(MfEntry{N, S, -1, f16, (const void*)&mfma_fft_kernel<H, N, T, S, -1>});
-        v.push_back(MfEntry{N, S, 1, f16, (const void*)&mfma_fft_kernel<H, N, T, S, 1>});
+        v.push_back(MfEntry{N, S, -1, f16, 4, 0, (const void*)&mfma_fft_kernel<H, N, T, S, -1>});
+        v.push_back(MfEntry{N, S, 1, f16, 4, 0, (const void*)&mfma_fft_kernel<H, N, T, S, 1>});
     }
 }
 
@@ -530,6 +544,9 @@ std::map<MfKey, void*> g_mf_tw;
 }  // namespace
 
 int mf_tile_T(int N, int strided) { return mf_T(N, strided != 0); }
+
+// Wavefronts per workgroup of the 256/512 kernels (timing experiments).
+int mf_nw_override = 0;
 
 // Twiddles of the MFMA schedule (FP32, rounded once from long double).
 const void* mfma_twiddle_table(int N, int dir) {
@@ -592,13 +609,22 @@ int launch_mfma_pass(const AxisPass& p, void* data, int N, int dir, a2afft_preci
     const void* tw = mfma_twiddle_table(N, dir);
     if (!tw) return set_error(A2AFFT_E_ALLOC, "split precision: twiddle table N=%d", N);
     const int f16 = prec == A2AFFT_SPLIT_FP16 ? 1 : 0;
+    const int want_nw = mf_nw_override > 0 ? mf_nw_override : 4;
+    // largest byte offset the pass touches (all strides are positive)
+    const uint64_t last = 8ull * uint64_t(g.a1.off(g.a1.n - 1) + g.a2.off(g.a2.n - 1) + g.fft.off(uint32_t(N - 1))) + 8;
+    const int want_o32 = last < (1ull << 32) ? 1 : 0;
     const void* fn = nullptr;
+    int nw = 4;
     for (const MfEntry& e : mf_registry())
-        if (e.N == N && e.strided == (p.strided != 0) && e.dir == (dir < 0 ? -1 : 1) && e.fp16 == f16) fn = e.fn;
+        if (e.N == N && e.strided == (p.strided != 0) && e.dir == (dir < 0 ? -1 : 1) && e.fp16 == f16 &&
+            (!fn || (e.nw == want_nw && (e.o32 == want_o32 || N < 256)))) {
+            fn = e.fn;
+            nw = e.nw;
+        }
     if (!fn) return set_error(A2AFFT_E_UNSUPPORTED, "split precision: no kernel for N=%d", N);
     float2* dp = (float2*)data;
     void* args[] = {&dp, &g, (void*)&tw};
-    st = hip_check(int(hipLaunchKernel(fn, dim3(grid), dim3(kMfThreads), args, 0, s)), "mfma fft launch");
+    st = hip_check(int(hipLaunchKernel(fn, dim3(grid), dim3(64 * nw), args, 0, s)), "mfma fft launch");
     if (st < 0) return st;
     return hip_check(int(hipGetLastError()), "mfma fft launch");
 }

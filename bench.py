@@ -235,16 +235,68 @@ def run_c5(args, torch, a2afft):
     print(json.dumps(out), flush=True)
 
 
+def precision_variants(a2afft, torch, Ng, P, dims, seed, steps):
+    """The same Ng^3 GRF (complex64, block layout over dims) through the FP32,
+    split-BF16 and split-FP16 plans: ms per forward (events, back to back) and
+    rel-L2 / max-abs against the FP64 plan on the same rounded input."""
+    g = make_grf(a2afft, torch, Ng, seed)
+    x = a2afft.inputs.global_to_blocks(g.to(torch.complex64), dims).contiguous()
+    del g
+    dist64 = a2afft.Distribution(Ng, P, dims=dims, dtype="c128")
+    ref = x.to(torch.complex128)
+    a2afft.Dfft(dist64, precision="fp64", exec="fused").forward(ref)
+    dist = a2afft.Distribution(Ng, P, dims=dims, dtype="c64")
+    out = {}
+    data = torch.empty_like(x)
+    for prec in ("fp32", "split_bf16", "split_fp16"):
+        plan = a2afft.Dfft(dist, precision=prec, exec="fused")
+        data.copy_(x)
+        plan.forward(data)
+        rel, mx = a2afft.error_metrics(data, ref)
+        data.copy_(x)
+        ms = _events_ms(torch, lambda: plan.forward(data), steps)
+        out[prec] = {"ms": round(ms, 4), "rel_l2_vs_fp64": rel, "max_abs_vs_fp64": mx,
+                     "hbm_gbs_whole": round(6 * Ng ** 3 * 8 / (ms * 1e-3) / 1e9, 1)}
+        log(f"[bench] {prec}: {ms:.4f} ms rel-L2 {rel:.3e}")
+    del ref, x, data
+    torch.cuda.synchronize()
+    return out
+
+
+def run_c4(args, torch, a2afft):
+    """512^3 complex64 GRF (seed 3) over dims [1,4,4]: split-BF16 MFMA radix-16
+    passes beside plain FP32 and split-FP16, error vs the FP64 plan per variant."""
+    Ng, P, dims, seed = args.ng or 512, 16, (1, 4, 4), 3
+    v = precision_variants(a2afft, torch, Ng, P, dims, seed, args.steps)
+    bf = v["split_bf16"]
+    out = {
+        "metric": "ms per 512^3 c2c forward, split-BF16 MFMA radix-16 vs plain FP32 (rel-L2 vs FP64)",
+        "value": bf["ms"], "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": 1,
+        "ms_per_step": bf["ms"], "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
+        "dtype": "bf16x2 (MFMA) + f32", "data": "synthetic",
+        "config": {"workload": f"{Ng}^3 c2c forward, complex64, {P} virtual ranks dims {list(dims)}, "
+                               f"GRF P(k)~k^-3 seed {seed} + 1e-3 noise", "name": "c4"},
+        "variants": v,
+        "split_bf16_speedup_vs_fp32": round(v["fp32"]["ms"] / bf["ms"], 4),
+        "split_bf16_within_1e-5": bf["rel_l2_vs_fp64"] <= 1e-5,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--config", default="c3", choices=["c3", "c2", "c3b", "c5"],
+    ap.add_argument("--config", default="c3", choices=["c3", "c2", "c3b", "c4", "c5"],
                     help="c3: 512^3 complex64 over dims [1,4,4], GRF seed 2 (the headline); "
                          "c2: 256^3 complex128 over dims [1,2,4], N(0,1) seed 1; "
                          "c3b: one rank's 2048x256x256 complex64 share of 2048^3 over dims [1,8,8], x stage; "
+                         "c4: 512^3 GRF seed 3, FP32 vs split-BF16 vs split-FP16; "
                          "c5: 64 uneven complex64 grids (seed 4) in one persistent launch")
+    ap.add_argument("--no-variants", action="store_true",
+                    help="c3: skip the FP32 / split-BF16 / split-FP16 comparison on the seed-3 field")
     ap.add_argument("--precision", default=None, choices=["fp32", "fp64", "split_bf16", "split_fp16"])
     ap.add_argument("--ng", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -266,6 +318,8 @@ def main():
         return run_c3b(args, torch, a2afft)
     if args.config == "c5":
         return run_c5(args, torch, a2afft)
+    if args.config == "c4":
+        return run_c4(args, torch, a2afft)
     if args.config == "c2":
         Ng, P, dims, seed, field = args.ng or 256, 8, (1, 2, 4), 1, "normal"
         args.precision = args.precision or "fp64"
@@ -432,6 +486,9 @@ def main():
                      "traffic": traffic},
         "cpu_baseline": cpu,
     }
+    if args.config == "c3" and not args.no_variants and args.ng in (None, 512):
+        log("[bench] precision variants on the seed-3 field")
+        out["precision_variants_seed3"] = precision_variants(a2afft, torch, Ng, P, dims, 3, args.steps)
     print(json.dumps(out), flush=True)
 
 
