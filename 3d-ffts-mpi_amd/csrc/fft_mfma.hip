@@ -316,7 +316,7 @@ __global__ __launch_bounds__(64 * NW, NW == 8 ? 4 : 2) void mfma_fft_direct(floa
     constexpr int GW = BPL / NW;      // column groups per wavefront and stage
     constexpr int LD = N + N / 16 + 1;   // odd row: the 16 lines of a strided group hit distinct banks
     constexpr int RS = 17 * BPL / 16;    // LDS slot step between the inputs of a stage-1 butterfly
-    __shared__ float lds[2 * T * LD];
+    __shared__ float lds[2 * T * LD + 2 * (N == 512 ? 768 : 256)];
 
     const int tid = int(threadIdx.x);
     const int lane = tid & 63, cn = lane & 15, kg = lane >> 4, kh = kg & 1, ri = kg >> 1, w = tid >> 6;
@@ -336,18 +336,36 @@ __global__ __launch_bounds__(64 * NW, NW == 8 ? 4 : 2) void mfma_fft_direct(floa
     // ---- stage 0 operands from global memory ----
     // off(j + r*BPL) = off(j) + off(r*BPL): the axis block extent and BPL are
     // powers of two, so one divides the other and j < BPL never carries.
-    Off offr[8];
+    // Lanes l and l ^ 32 need the real and the imaginary parts of the same 8
+    // inputs: each loads 4 of them whole and they trade halves.
+    Off offr[4];
 #pragma unroll
-    for (int i = 0; i < 8; ++i) offr[i] = Off(8 * g.fft.off(uint32_t((8 * kh + i) * BPL)));
-    float x[GW][8];
+    for (int i = 0; i < 4; ++i) offr[i] = Off(8 * g.fft.off(uint32_t((8 * kh + 4 * ri + i) * BPL)));
+    float2 c[GW][4];
 #pragma unroll
     for (int gi = 0; gi < GW; ++gi) {
         const int a = STRIDED ? 0 : gi / JB;
         const int j = STRIDED ? w * GW + gi : (gi % JB) * 16 + cn;
-        const Off pl = base[a] + Off(8 * g.fft.off(uint32_t(j))) + Off(4 * ri);
+        const Off pl = base[a] + Off(8 * g.fft.off(uint32_t(j)));
 #pragma unroll
-        for (int i = 0; i < 8; ++i)
-            x[gi][i] = valid[a] ? *reinterpret_cast<const float*>(datab + (pl + offr[i])) : 0.f;
+        for (int i = 0; i < 4; ++i)
+            c[gi][i] = valid[a] ? *reinterpret_cast<const float2*>(datab + (pl + offr[i])) : make_float2(0.f, 0.f);
+    }
+    // twiddles of stage 1 (and of the radix-2 stage of 512) into LDS
+    constexpr int NTW = (N == 512 ? 768 : 256);
+    float2* const ltw = reinterpret_cast<float2*>(lds + 2 * T * LD);
+    for (int t = tid; t < NTW; t += 64 * NW) ltw[t] = tw[16 + t];
+    float x[GW][8];
+#pragma unroll
+    for (int gi = 0; gi < GW; ++gi) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const float give = ri ? c[gi][i].x : c[gi][i].y;
+            const auto sw = __builtin_amdgcn_permlane32_swap(__float_as_uint(give), __float_as_uint(give), false, false);
+            const float got = __uint_as_float(ri ? sw[0] : sw[1]);   // the partner's value
+            x[gi][i] = ri ? got : c[gi][i].x;
+            x[gi][4 + i] = ri ? c[gi][i].y : got;
+        }
     }
 
     V8 arh, arl, aih, ail;
@@ -398,12 +416,14 @@ __global__ __launch_bounds__(64 * NW, NW == 8 ? 4 : 2) void mfma_fft_direct(floa
         }
     };
 
+    __syncthreads();   // twiddles in LDS
+
     // ---- stage 0 (Ns = 1): outputs to LDS with the stage-1 twiddles ----
     {
         const int lrow = STRIDED ? cn : A * w;   // + gi / JB for contiguous
         float* const wre = lds + lrow * LD + 4 * kg;
         float* const wim = wre + T * LD;
-        const float2* const twq = tw + 16 + 16 * 4 * kg;   // + 16 i + j / JB
+        const float2* const twq = ltw + 16 * 4 * kg;   // + 16 i + j / JB
 #pragma unroll
         for (int gi = 0; gi < GW; ++gi) {
             const int lo = STRIDED ? 0 : (gi / JB) * LD;
@@ -421,20 +441,14 @@ __global__ __launch_bounds__(64 * NW, NW == 8 ? 4 : 2) void mfma_fft_direct(floa
     __syncthreads();
 
     // ---- stage 1 (Ns = 16), then the radix-2 stage of 512, to global memory ----
-    float xb[GW][8];
-    {
-        const float* const rp = lds + ri * T * LD + 8 * kh * RS;
+    // Stage 1 only reads LDS, so it runs pair by pair: pair a is butterfly
+    // j1 = jlo (and, for 512, its radix-2 partner jlo + 16).
+    const float* const rp = lds + ri * T * LD + 8 * kh * RS;
+    auto read_group = [&](int line, int j1, float (&xv)[8]) {
+        const float* const q = rp + line * LD + j1 + j1 / 16;
 #pragma unroll
-        for (int gi = 0; gi < GW; ++gi) {
-            // pair index a = gi % A; for 512 the second half (gi >= A) is butterfly j1 + 16
-            const int a = gi % A, hi = gi / A;
-            const int line = STRIDED ? cn : A * w + a;
-            const int j1 = STRIDED ? 16 * hi + A * w + a : 16 * hi + cn;
-            const float* const q = rp + line * LD + j1 + j1 / 16;
-#pragma unroll
-            for (int i = 0; i < 8; ++i) xb[gi][i] = q[i * RS];
-        }
-    }
+        for (int i = 0; i < 8; ++i) xv[i] = q[i * RS];
+    };
     // the inverse's 1/(nx*ny*nz) as one multiplication (exact for powers of two)
     const bool scaled = g.scale_mode != 0;
     const float fs = g.scale_mode == 2 ? float(1.0 / g.scale) : float(g.scale);
@@ -450,19 +464,23 @@ __global__ __launch_bounds__(64 * NW, NW == 8 ? 4 : 2) void mfma_fft_direct(floa
 #pragma unroll
     for (int a = 0; a < A; ++a) {
         const int ab = STRIDED ? 0 : a;
+        const int line = STRIDED ? cn : A * w + a;
         const int jlo = STRIDED ? A * w + a : cn;   // j1 % 16
         const Off po = base[ab] + Off(8 * g.fft.off(uint32_t(jlo)));
+        float xv[8];
+        read_group(line, jlo, xv);
         float2 y0[4];
-        dft16_group(xb[a], y0);
+        dft16_group(xv, y0);
         if constexpr (N == 256) {
             if (valid[ab]) {
 #pragma unroll
                 for (int i = 0; i < 4; ++i) put(po + offq[i], y0[i]);
             }
         } else {
+            read_group(line, jlo + 16, xv);
             float2 y1[4];
-            dft16_group(xb[a + A], y1);
-            const float2* const t2 = tw + mf_twoff(N, 2) + 1 + 2 * jlo + 32 * 4 * kg;   // + 32 i
+            dft16_group(xv, y1);
+            const float2* const t2 = ltw + (mf_twoff(N, 2) - 16) + 1 + 2 * jlo + 32 * 4 * kg;   // + 32 i
             if (valid[ab]) {
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
@@ -490,12 +508,8 @@ void mf_add(std::vector<MfEntry>& v) {
     constexpr int T = mf_T(N, S);
     constexpr int f16 = std::is_same<H, _Float16>::value ? 1 : 0;
     if constexpr (N == 256 || N == 512) {
-        v.push_back(MfEntry{N, S, -1, f16, 4, 0, (const void*)&mfma_fft_direct<H, N, S, -1, 4, false>});
-        v.push_back(MfEntry{N, S, 1, f16, 4, 0, (const void*)&mfma_fft_direct<H, N, S, 1, 4, false>});
         v.push_back(MfEntry{N, S, -1, f16, 8, 0, (const void*)&mfma_fft_direct<H, N, S, -1, 8, false>});
         v.push_back(MfEntry{N, S, 1, f16, 8, 0, (const void*)&mfma_fft_direct<H, N, S, 1, 8, false>});
-        v.push_back(MfEntry{N, S, -1, f16, 4, 1, (const void*)&mfma_fft_direct<H, N, S, -1, 4, true>});
-        v.push_back(MfEntry{N, S, 1, f16, 4, 1, (const void*)&mfma_fft_direct<H, N, S, 1, 4, true>});
         v.push_back(MfEntry{N, S, -1, f16, 8, 1, (const void*)&mfma_fft_direct<H, N, S, -1, 8, true>});
         v.push_back(MfEntry{N, S, 1, f16, 8, 1, (const void*)&mfma_fft_direct<H, N, S, 1, 8, true>});
     } else {
@@ -544,9 +558,6 @@ std::map<MfKey, void*> g_mf_tw;
 }  // namespace
 
 int mf_tile_T(int N, int strided) { return mf_T(N, strided != 0); }
-
-// Wavefronts per workgroup of the 256/512 kernels (timing experiments).
-int mf_nw_override = 0;
 
 // Twiddles of the MFMA schedule (FP32, rounded once from long double).
 const void* mfma_twiddle_table(int N, int dir) {
@@ -609,7 +620,7 @@ int launch_mfma_pass(const AxisPass& p, void* data, int N, int dir, a2afft_preci
     const void* tw = mfma_twiddle_table(N, dir);
     if (!tw) return set_error(A2AFFT_E_ALLOC, "split precision: twiddle table N=%d", N);
     const int f16 = prec == A2AFFT_SPLIT_FP16 ? 1 : 0;
-    const int want_nw = mf_nw_override > 0 ? mf_nw_override : 4;
+    const int want_nw = 8;
     // largest byte offset the pass touches (all strides are positive)
     const uint64_t last = 8ull * uint64_t(g.a1.off(g.a1.n - 1) + g.a2.off(g.a2.n - 1) + g.fft.off(uint32_t(N - 1))) + 8;
     const int want_o32 = last < (1ull << 32) ? 1 : 0;
