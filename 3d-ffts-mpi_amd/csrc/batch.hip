@@ -10,13 +10,14 @@
 // the current unit is computed and examined after it, so the latency of the
 // device-scope atomic hides behind the unit's own loads.
 //
-// Synchronisation is per pass, not per unit: the 8 XCDs of the device have
-// L2 caches that are not coherent with one another, so a device-scope fence
-// writes back or invalidates a whole L2.  A workgroup therefore counts the
-// units it finishes and, only when the next unit it takes belongs to a later
-// pass (or the queues are empty), releases its writes with one fence, adds
-// its count to that pass's total, waits until the previous pass's total is
-// complete and acquires with one fence.  That is at most three transitions
+// Synchronisation is per pass, not per unit.  The 8 XCDs of the device have
+// L2 caches that are not coherent with one another, and a device-scope fence
+// writes back or invalidates a whole L2, so the kernel uses none: every unit
+// load and store is a relaxed device-scope access (sc1), coherent across the
+// XCDs by itself.  A workgroup counts the units it finishes and, only when
+// the next unit it takes belongs to a later pass (or the queues are empty),
+// waits for its own stores to complete, adds its count to that pass's total
+// and waits until the previous pass's total is complete.  That is at most three transitions
 // per workgroup.  Every wait ends: every unit of pass p is taken before any
 // unit of pass p+1, and the workgroup holding it finishes it after at most a
 // wait for pass p-1 (it may hold one taken unit of pass p+1 meanwhile, which
@@ -48,7 +49,7 @@ constexpr BShape kShapes[] = {
     {16, 1, 1, 16},   {32, 1, 1, 8},  {64, 1, 1, 8},  {128, 1, 1, 16},  {256, 1, 1, 16},  {512, 1, 1, 8},
 };
 constexpr int kNumShapes = int(sizeof(kShapes) / sizeof(kShapes[0]));
-constexpr int kFlags = KF_PAD;
+constexpr int kFlags = KF_PAD | KF_COH;
 
 }  // namespace
 
@@ -185,7 +186,7 @@ __device__ __forceinline__ uint32_t claim_finish(Cursor& c, uint32_t ticket, con
 }
 
 template <int DIR>
-__global__ __launch_bounds__(kBatchThreads) void batch_kernel(float2* __restrict__ data,
+__global__ __launch_bounds__(kBatchThreads, 3) void batch_kernel(float2* __restrict__ data,
                                                               const BatchItem* __restrict__ items,
                                                               const uint32_t* __restrict__ unit_item, int G,
                                                               BatchQueues qs, uint32_t* __restrict__ ctrl) {
@@ -205,7 +206,10 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(float2* __restrict
         const int k = unit == kNoUnit ? -1 : int(unit_item[unit]);
         const int q = k < 0 ? 3 : k / G;   // pass of this unit; 3 = queues empty
         if (q != cur) {
-            __threadfence();   // release this workgroup's writes of pass `cur`
+            // Release: the unit data is written with device-coherent stores, so
+            // waiting for this wavefront's stores to complete is enough (no L2
+            // write-back); the barrier then covers every wavefront.
+            __builtin_amdgcn_s_waitcnt(0);
             __syncthreads();
             if (tid == 0) {
                 atomicAdd(&ctrl[ctrl_word(kCtrlDone + cur)], count);
@@ -222,8 +226,7 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(float2* __restrict
                     }
                 }
             }
-            __syncthreads();
-            __threadfence();   // acquire: this workgroup's loads see pass q-1
+            __syncthreads();   // acquire: the unit loads that follow are device-coherent too
             cur = q;
             count = 0;
         }

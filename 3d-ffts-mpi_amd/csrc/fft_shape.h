@@ -119,6 +119,7 @@ enum : int {
     KF_PAD = 2,    // padded LDS layout
     KF_NT = 4,     // non-temporal global loads and stores
     KF_A2MAJ = 8,  // units numbered a2-major (z tiles outermost)
+    KF_COH = 16,   // device-coherent (L2-bypassing) global accesses, for in-launch hand-offs
 };
 
 // Kernel choice for one pass: tile width T, elements per thread E and the

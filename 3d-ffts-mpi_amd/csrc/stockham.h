@@ -171,9 +171,17 @@ constexpr int lds_slots() {
 }
 
 // Global access of one complex value; NTMP marks it non-temporal (a native
-// 2-vector, so it stays one 8- or 16-byte access).
-template <bool NTMP, typename V> __device__ __forceinline__ V ld_global(const V* p) {
-    if constexpr (NTMP) {
+// 2-vector, so it stays one 8- or 16-byte access).  COH makes it a relaxed
+// device-scope atomic access (complex64 only): coherent across the XCDs'
+// separate L2 caches without cache write-back or invalidation, for kernels
+// that hand data between workgroups inside one launch.
+template <bool NTMP, bool COH = false, typename V> __device__ __forceinline__ V ld_global(const V* p) {
+    if constexpr (COH) {
+        static_assert(sizeof(V) == 8, "coherent access is for complex64");
+        const uint64_t u = __hip_atomic_load(reinterpret_cast<const uint64_t*>(p), __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+        return __builtin_bit_cast(V, u);
+    } else if constexpr (NTMP) {
         using R = decltype(p->x);
         typedef R nv2 __attribute__((ext_vector_type(2)));
         const nv2 t = __builtin_nontemporal_load(reinterpret_cast<const nv2*>(p));
@@ -185,8 +193,12 @@ template <bool NTMP, typename V> __device__ __forceinline__ V ld_global(const V*
         return *p;
     }
 }
-template <bool NTMP, typename V> __device__ __forceinline__ void st_global(V* p, V v) {
-    if constexpr (NTMP) {
+template <bool NTMP, bool COH = false, typename V> __device__ __forceinline__ void st_global(V* p, V v) {
+    if constexpr (COH) {
+        static_assert(sizeof(V) == 8, "coherent access is for complex64");
+        __hip_atomic_store(reinterpret_cast<uint64_t*>(p), __builtin_bit_cast(uint64_t, v), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    } else if constexpr (NTMP) {
         using R = decltype(p->x);
         typedef R nv2 __attribute__((ext_vector_type(2)));
         nv2 t;
@@ -267,6 +279,7 @@ __device__ __forceinline__ void stockham_unit(typename CT<real>::V* __restrict__
     constexpr int NT = T * TPL;
     constexpr int PAD = (FLAGS & KF_PAD) ? 1 : 0;
     constexpr bool NTMP = (FLAGS & KF_NT) != 0;
+    constexpr bool COH = (FLAGS & KF_COH) != 0;
     constexpr bool A2MAJ = (FLAGS & KF_A2MAJ) != 0;
 
     const bool active = tid < NT;
@@ -280,7 +293,7 @@ __device__ __forceinline__ void stockham_unit(typename CT<real>::V* __restrict__
     V v[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-        if (valid) v[e] = ld_global<NTMP>(data + base + elem_off(g, e * TPL));
+        if (valid) v[e] = ld_global<NTMP, COH>(data + base + elem_off(g, e * TPL));
         else { v[e].x = 0; v[e].y = 0; }
     }
 
@@ -298,7 +311,7 @@ __device__ __forceinline__ void stockham_unit(typename CT<real>::V* __restrict__
         for (int e = 0; e < E; ++e) { v[e].x = v[e].x / s; v[e].y = v[e].y / s; }
     }
 #pragma unroll
-    for (int e = 0; e < E; ++e) st_global<NTMP>(data + base + elem_off(g, e * TPL), v[e]);
+    for (int e = 0; e < E; ++e) st_global<NTMP, COH>(data + base + elem_off(g, e * TPL), v[e]);
 }
 
 // One workgroup per unit.  XCD = 1 renumbers workgroups so each of the 8

@@ -189,16 +189,16 @@ def run_c5(args, torch, a2afft):
         del ref
     # workgroup-count tuning (every setting computes the same bits)
     info = b.info()
-    best, best_t = (0, False, 1), 1e30
+    sched, sched_t = (0, False, 1), 1e30
     for xq in (False, True):
         for chunk in (1, 2, 4):
             for w in sorted({info["max_workgroups"], info["max_workgroups"] // 2, 512, 256}):
                 b.set_schedule(w, xq, chunk)
                 t = _events_ms(torch, lambda: b.forward(y), 3)
                 log(f"[bench] batch workgroups {w} xcd_queues {int(xq)} chunk {chunk}: {t:.4f} ms")
-                if t < best_t:
-                    best, best_t = (w, xq, chunk), t
-    b.set_schedule(*best)
+                if t < sched_t:
+                    sched, sched_t = (w, xq, chunk), t
+    b.set_schedule(*sched)
     t_eager = _events_ms(torch, lambda: b.forward(y), args.steps)
     t_graph = _events_ms(torch, lambda: b.graph_launch(y, -1), args.steps)
     assert b.status() == 0
@@ -223,7 +223,7 @@ def run_c5(args, torch, a2afft):
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "64 complex64 grids drawn with seed 4 from 32^3, 64^3, 128^3, 256^3, 128x256x64, "
                                "256x64x512; U[-1,1) values", "name": "c5", "counts": counts,
-                   "points": npts, "schedule": {"workgroups": best[0], "xcd_queues": best[1], "chunk": best[2]}},
+                   "points": npts, "schedule": {"workgroups": sched[0], "xcd_queues": sched[1], "chunk": sched[2]}},
         "ms": {"persistent_eager": round(t_eager, 4), "persistent_graph": round(t_graph, 4),
                "separate_launches_192": round(t_sep, 4)},
         "rel_l2_vs_fp64": {"max": max(errs), "mean": sum(errs) / len(errs)},
