@@ -324,6 +324,8 @@ static int plan_init(a2afft_plan_s* p) {
         if (!fft_length_supported(N)) return set_error(A2AFFT_E_UNSUPPORTED, "plan: length %d not supported", N);
         p->sel[i] = fft_default_sel(p->pass[i], p->d.dtype, N);
         A2A_TRY(fft_pass_shape(p->pass[i], p->d.dtype, N, &p->sel[i], &sh));
+        if (!find_kernel(N, p->d.dtype == A2AFFT_C128, p->pass[i].strided, p->sel[i], -1))
+            return set_error(A2AFFT_E_UNSUPPORTED, "plan: no kernel built for length %d in this precision", N);
         // build the twiddle tables now so execute allocates nothing
         if (!twiddle_table(N, -1, p->d.dtype) || !twiddle_table(N, 1, p->d.dtype))
             return set_error(A2AFFT_E_ALLOC, "plan: twiddle tables");

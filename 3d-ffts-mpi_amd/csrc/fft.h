@@ -25,7 +25,9 @@ struct AxisPass {
     uint32_t a2_count = 0;
 };
 
-// Supported transform lengths: powers of two 2..4096.
+// Supported transform lengths: powers of two 2..4096 and 2^a 3^b 5^c
+// (2^a <= 16) up to 60; a precision may lack kernels for some of the latter
+// (find_kernel says which).
 bool fft_length_supported(int N);
 
 // Device twiddle table for (N, dir, dtype); created on first use, cached for
@@ -42,6 +44,9 @@ struct FftShape {
     int T, E, NT, grid, flags;
 };
 int fft_pass_shape(const AxisPass& pass, a2afft_dtype dtype, int N, const KSel* sel, FftShape* out);
+
+// The built kernel for (N, precision, strided, shape, direction), or null.
+const void* find_kernel(int N, bool dbl, int strided, const KSel& s, int dir);
 
 // The built variants that can run this pass; returns the count.
 int fft_candidates(const AxisPass& pass, a2afft_dtype dtype, int N, KSel* out, int max);

@@ -88,7 +88,7 @@ KSel fft_default_sel(const AxisPass& p, a2afft_dtype dt, int N) {
             return pr.s;
     }
     KSel s = default_sel(N, dbl, p.strided, p.fft, p.a2);
-    if (npass_of(N) > 1 && s.T > 1) s.flags |= KF_XCD;
+    if (npass_of(N) > 1 && s.T > 1 && is_pow2(N)) s.flags |= KF_XCD;   // mixed lengths: plain variants only
     return s;
 }
 

@@ -18,6 +18,7 @@ int register_kernels_f32(KEntry* out) {
     add_defaults<float, 1024>(out, &n);
     add_defaults<float, 2048>(out, &n);
     add_defaults<float, 4096>(out, &n);
+    add_mixed_all<float, 3, 5, 6, 9, 10, 12, 15, 18, 20, 24, 25, 27, 30, 36, 40, 45, 48, 50, 54, 60>(out, &n);
     // tuning candidates for the benchmark lengths
     add_shape<float, 512, 16, 16, true>(out, &n);    // 512 threads, 64 KiB LDS
     add_shape<float, 512, 16, 8, true>(out, &n);     // 256 threads, 32 KiB LDS, 64 B rows

@@ -8,10 +8,40 @@
 namespace a2a {
 
 // Radix schedules.  Pass p has radix radix_of(N,p); Ns = product of earlier radices.
+// Powers of two use radix 2..16 passes.  Mixed lengths 2^a * 3^b * 5^c
+// (2^a <= 16, b + c >= 1, N <= 60) use the power-of-two factor as one radix,
+// then the 3s, then the 5s, with one thread holding a whole line (E = N).
+A2A_HD constexpr bool is_pow2(int N) { return N > 0 && (N & (N - 1)) == 0; }
+A2A_HD constexpr int pow2_part(int N) {
+    int p = 1;
+    while (N > 1 && N % 2 == 0) { N /= 2; p *= 2; }
+    return p;
+}
+A2A_HD constexpr int count_factor(int N, int f) {
+    int c = 0;
+    while (N > 1 && N % f == 0) { N /= f; ++c; }
+    return c;
+}
+A2A_HD constexpr bool mixed_len_ok(int N) {
+    if (N < 3 || N > 60 || is_pow2(N)) return false;
+    int m = N / pow2_part(N);
+    while (m % 3 == 0) m /= 3;
+    while (m % 5 == 0) m /= 5;
+    return m == 1 && pow2_part(N) <= 16;
+}
 A2A_HD constexpr int npass_of(int N) {
+    if (!is_pow2(N)) return (pow2_part(N) > 1 ? 1 : 0) + count_factor(N, 3) + count_factor(N, 5);
     return N <= 16 ? 1 : (N <= 256 ? 2 : 3);
 }
 A2A_HD constexpr int radix_of(int N, int p) {
+    if (!is_pow2(N)) {
+        const int p2 = pow2_part(N);
+        if (p2 > 1) {
+            if (p == 0) return p2;
+            --p;
+        }
+        return p < count_factor(N, 3) ? 3 : 5;
+    }
     // 2:[2] 4:[4] 8:[8] 16:[16] 32:[8,4] 64:[8,8] 128:[16,8] 256:[16,16]
     // 512:[8,8,8] 1024:[16,8,8] 2048:[16,16,8] 4096:[16,16,16]
     return N <= 16 ? N
@@ -24,7 +54,9 @@ A2A_HD constexpr int radix_of(int N, int p) {
          : N == 2048 ? (p == 2 ? 8 : 16)
          : 16;
 }
+// Granule of E (elements per thread): every radix must divide it.
 A2A_HD constexpr int rmax_of(int N) {
+    if (!is_pow2(N)) return N;
     return N <= 16 ? N : (N == 32 || N == 64 || N == 512) ? 8 : 16;
 }
 A2A_HD constexpr int ns_of(int N, int p) {
@@ -111,7 +143,7 @@ A2A_HD constexpr int str_E(int N, bool dbl) {
     return (N * str_T(N, dbl)) / 256 > rmax_of(N) ? (N * str_T(N, dbl)) / 256 : rmax_of(N);
 }
 
-inline bool fft_len_ok(int N) { return N >= 2 && N <= 4096 && (N & (N - 1)) == 0; }
+inline bool fft_len_ok(int N) { return (N >= 2 && N <= 4096 && is_pow2(N)) || mixed_len_ok(N); }
 
 // Kernel variant flags.
 enum : int {
@@ -142,6 +174,12 @@ inline bool shape_fits(const AxisMap& fft, const AxisMap& a2, int strided, int N
 
 // Default shape of a pass (before tuning).
 inline KSel default_sel(int N, bool dbl, int strided, const AxisMap& fft, const AxisMap& a2) {
+    if (!is_pow2(N)) {   // mixed lengths: one thread per line
+        if (!strided) return KSel{64, N, 0};
+        if (shape_fits(fft, a2, 1, N, 16, N)) return KSel{16, N, 0};
+        if (shape_fits(fft, a2, 1, N, 4, N)) return KSel{4, N, 0};
+        return KSel{1, N, 0};
+    }
     if (!strided) return KSel{cont_T(N), cont_E(N), 0};
     KSel s{str_T(N, dbl), str_E(N, dbl), 0};
     if (!shape_fits(fft, a2, 1, N, s.T, s.E)) s = KSel{1, rmax_of(N), 0};

@@ -57,11 +57,21 @@ template <> struct K<float> {
     static constexpr float r2 = 0.70710678118654752440f;   // sqrt(2)/2
     static constexpr float c8 = 0.92387953251128675613f;   // cos(pi/8)
     static constexpr float s8 = 0.38268343236508977173f;   // sin(pi/8)
+    static constexpr float s3 = 0.86602540378443864676f;   // sin(2pi/3)
+    static constexpr float c51 = 0.30901699437494742410f;  // cos(2pi/5)
+    static constexpr float c52 = -0.80901699437494742410f; // cos(4pi/5)
+    static constexpr float s51 = 0.95105651629515357212f;  // sin(2pi/5)
+    static constexpr float s52 = 0.58778525229247312917f;  // sin(4pi/5)
 };
 template <> struct K<double> {
     static constexpr double r2 = 0.70710678118654752440;
     static constexpr double c8 = 0.92387953251128675613;
     static constexpr double s8 = 0.38268343236508977173;
+    static constexpr double s3 = 0.86602540378443864676;
+    static constexpr double c51 = 0.30901699437494742410;
+    static constexpr double c52 = -0.80901699437494742410;
+    static constexpr double s51 = 0.95105651629515357212;
+    static constexpr double s52 = 0.58778525229247312917;
 };
 
 // a * exp(DIR * i*pi/4)
@@ -145,8 +155,50 @@ template <int DIR, typename V> __device__ __forceinline__ void dft16(V* x) {
         x[k1] = y0; x[k1 + 4] = y1; x[k1 + 8] = y2; x[k1 + 12] = y3;
     }
 }
+// 3- and 5-point DFTs: y_k = sum_n x_n W^{nk}, W = exp(DIR*2*pi*i/R); the
+// sums over conjugate pairs share their real parts.
+template <int DIR, typename V> __device__ __forceinline__ void dft3(V* x) {
+    using real = decltype(x[0].x);
+    const real s = DIR < 0 ? -K<real>::s3 : K<real>::s3;
+    const V a = c_add(x[1], x[2]), d = c_sub(x[1], x[2]);
+    V m;   // x0 - (x1 + x2) / 2
+    m.x = x[0].x - a.x * real(0.5);
+    m.y = x[0].y - a.y * real(0.5);
+    V r;   // i * s * (x1 - x2)
+    r.x = -(d.y * s);
+    r.y = d.x * s;
+    x[0] = c_add(x[0], a);
+    x[1] = c_add(m, r);
+    x[2] = c_sub(m, r);
+}
+template <int DIR, typename V> __device__ __forceinline__ void dft5(V* x) {
+    using real = decltype(x[0].x);
+    const real c1 = K<real>::c51, c2 = K<real>::c52;
+    const real s1 = DIR < 0 ? -K<real>::s51 : K<real>::s51;
+    const real s2 = DIR < 0 ? -K<real>::s52 : K<real>::s52;
+    const V a1 = c_add(x[1], x[4]), d1 = c_sub(x[1], x[4]);
+    const V a2 = c_add(x[2], x[3]), d2 = c_sub(x[2], x[3]);
+    V m1, m2, r1, r2;
+    m1.x = x[0].x + (a1.x * c1 + a2.x * c2);
+    m1.y = x[0].y + (a1.y * c1 + a2.y * c2);
+    m2.x = x[0].x + (a1.x * c2 + a2.x * c1);
+    m2.y = x[0].y + (a1.y * c2 + a2.y * c1);
+    // r1 = i * (s1 d1 + s2 d2), r2 = i * (s2 d1 - s1 d2)
+    r1.x = -(d1.y * s1 + d2.y * s2);
+    r1.y = d1.x * s1 + d2.x * s2;
+    r2.x = -(d1.y * s2 - d2.y * s1);
+    r2.y = d1.x * s2 - d2.x * s1;
+    x[0].x = x[0].x + (a1.x + a2.x);
+    x[0].y = x[0].y + (a1.y + a2.y);
+    x[1] = c_add(m1, r1);
+    x[4] = c_sub(m1, r1);
+    x[2] = c_add(m2, r2);
+    x[3] = c_sub(m2, r2);
+}
 template <int R, int DIR, typename V> __device__ __forceinline__ void dft(V* x) {
-    if constexpr (R == 2) dft2<DIR>(x[0], x[1]);
+    if constexpr (R == 3) dft3<DIR>(x);
+    else if constexpr (R == 5) dft5<DIR>(x);
+    else if constexpr (R == 2) dft2<DIR>(x[0], x[1]);
     else if constexpr (R == 4) dft4<DIR>(x[0], x[1], x[2], x[3]);
     else if constexpr (R == 8) dft8<DIR>(x);
     else dft16<DIR>(x);
@@ -233,7 +285,7 @@ __device__ __forceinline__ void stockham_passes(typename CT<real>::V (&v)[E], ty
             for (int r = 0; r < R; ++r) x[r] = v[b + r * NB];
             if constexpr (p > 0) {
                 const int j = t + b * TPL;
-                const int k = j & (Ns - 1);
+                const int k = int(unsigned(j) % unsigned(Ns));
 #pragma unroll
                 for (int r = 1; r < R; ++r) x[r] = c_mul(x[r], tw[toff + k * R + r]);
             }
@@ -247,7 +299,7 @@ __device__ __forceinline__ void stockham_passes(typename CT<real>::V (&v)[E], ty
 #pragma unroll
                 for (int b = 0; b < NB; ++b) {
                     const int j = t + b * TPL;
-                    const int d0 = (j / Ns) * (Ns * R) + (j & (Ns - 1));
+                    const int d0 = int(unsigned(j) / unsigned(Ns)) * (Ns * R) + int(unsigned(j) % unsigned(Ns));
 #pragma unroll
                     for (int q = 0; q < R; ++q) lds[lds_slot<N, T, STRIDED, PAD>(line, d0 + q * Ns)] = v[b + q * NB];
                 }
@@ -360,6 +412,21 @@ inline void add_extra_flags(KEntry* out, int* n) {
     add_variant<real, N, E, T, S, KF_XCD | KF_PAD | KF_A2MAJ>(out, n);
     add_variant<real, N, E, T, S, KF_A2MAJ>(out, n);
     add_variant<real, N, E, T, S, KF_XCD | KF_NT | KF_A2MAJ>(out, n);
+}
+
+// Mixed lengths 2^a 3^b 5^c: one thread per line (E = N), contiguous tiles of
+// 64 lines, strided tiles of 16 or 4 columns and the single-column fallback.
+template <typename real, int N>
+inline void add_mixed(KEntry* out, int* n) {
+    static_assert(!is_pow2(N) && mixed_len_ok(N), "mixed length");
+    add_variant<real, N, N, 64, false, 0>(out, n);
+    add_variant<real, N, N, 16, true, 0>(out, n);
+    add_variant<real, N, N, 4, true, 0>(out, n);
+    add_variant<real, N, N, 1, true, 0>(out, n);
+}
+template <typename real, int... Ns>
+inline void add_mixed_all(KEntry* out, int* n) {
+    (add_mixed<real, Ns>(out, n), ...);
 }
 
 // Default contiguous / strided shapes and the single-column fallback.

@@ -153,7 +153,9 @@ int a2afft_alltoall(const void* send, void* recv, int world_size, int64_t nlocal
 
 /* ---- FFT launchers (dfft/kernels/kernels.h:5-11) ------------------------ */
 /* forward_1d_fft / inverse_1d_fft (a2affts.cu:7-76): n_elems/Ng in-place
- * unnormalised DFTs of contiguous length-Ng pencils (replaces cuFFT Z2Z/C2C). */
+ * unnormalised DFTs of contiguous length-Ng pencils (replaces cuFFT Z2Z/C2C).
+ * Lengths: powers of two 2..4096, and 2^a*3^b*5^c with 2^a <= 16 up to 60
+ * (complex64) or 30 (complex128); others return A2AFFT_E_UNSUPPORTED. */
 int a2afft_forward_1d_fft(void* data, int Ng, int64_t n_elems, a2afft_dtype dtype, void* stream);
 int a2afft_inverse_1d_fft(void* data, int Ng, int64_t n_elems, a2afft_dtype dtype, void* stream);
 /* launch_scale_fft (a2affts.cu:78-92,108-116): out = data / Ng^3 (64-bit Ng^3). */
@@ -170,8 +172,8 @@ int a2afft_plan_create(a2afft_plan* out, a2afft_dist dist, a2afft_precision prec
                        a2afft_exec exec);
 /* A fused plan over one nx x ny x nz grid in C order (z fastest), without a
  * rank decomposition; no reference counterpart (the reference is cubic, one
- * grid per distribution, a2adistribution.h:9).  Each extent a power of two
- * in 2..4096; complex128 for FP64, complex64 otherwise.  make_plans is not
+ * grid per distribution, a2adistribution.h:9).  Each extent a transform
+ * length of forward_1d_fft; complex128 for FP64, complex64 otherwise.  make_plans is not
  * needed; forward / inverse / autotune / execute_* apply as for other fused
  * plans, and the inverse scales by 1/(nx*ny*nz). */
 int a2afft_grid_plan_create(a2afft_plan* out, int nx, int ny, int nz, a2afft_precision precision);

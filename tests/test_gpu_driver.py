@@ -12,12 +12,13 @@ pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 DRIVER = os.path.join(ROOT, "3d-ffts-mpi_amd", "a2afft_driver")
 
-# (P, Ng, blockSize, dims, exec): run_tests.py:18-21 (Ng=12 needs radix 3, see
-# test_unsupported_length_is_an_error) plus the C1 and C2 decompositions.
+# (P, Ng, blockSize, dims, exec): run_tests.py:18-21 -- (8,8), (8,32), (6,12),
+# (4,4) -- plus the C1 and C2 decompositions and a mixed-radix 24^3 over 12.
 CASES = [
     (8, 8, 64, None, "fused"), (8, 32, 128, None, "fused"), (4, 4, 64, None, "staged"),
+    (6, 12, 64, None, "fused"), (6, 12, 64, None, "staged"),
     (4, 64, 64, (2, 2, 1), "staged"), (4, 64, 64, (2, 2, 1), "fused"), (8, 32, 64, (1, 2, 4), "fused"),
-    (64, 32, 64, (4, 4, 4), "staged"), (12, 24, 64, None, "fused"),
+    (64, 32, 64, (4, 4, 4), "staged"), (12, 24, 64, None, "fused"), (12, 24, 64, None, "staged"),
 ]
 
 
@@ -25,8 +26,6 @@ CASES = [
 def test_driver_reference_check(cuda, tmp_path, case):
     from a2afft import stitch
     P, Ng, bs, dims, ex = case
-    if Ng % 3 == 0:
-        pytest.skip("length needs radix 3")
     cmd = [DRIVER, "-Ng", str(Ng), "-P", str(P), "-blockSize", str(bs), "-exec", ex, "-out", str(tmp_path)]
     if dims:
         cmd += ["-dims", *map(str, dims)]
@@ -39,6 +38,7 @@ def test_driver_reference_check(cuda, tmp_path, case):
 
 
 def test_unsupported_length_is_an_error(cuda, tmp_path):
-    r = subprocess.run([DRIVER, "-Ng", "12", "-P", "6", "-out", str(tmp_path)], capture_output=True, text=True,
+    """A length with a factor 7 has no kernel: a clean error, exit status 3."""
+    r = subprocess.run([DRIVER, "-Ng", "14", "-P", "2", "-out", str(tmp_path)], capture_output=True, text=True,
                        timeout=120)
-    assert r.returncode == 3 and "unsupported" in r.stderr
+    assert r.returncode == 3 and "not supported" in r.stderr

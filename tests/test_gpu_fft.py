@@ -13,6 +13,10 @@ from oracle import oracle as O
 pytestmark = pytest.mark.gpu
 
 LENGTHS = [2, 4, 8, 16, 32, 64, 128, 256, 512, 1024, 2048, 4096]
+# mixed radix 2^a 3^b 5^c: (N, dtypes built for it)
+MIXED = [(3, ("c128", "c64")), (5, ("c128", "c64")), (6, ("c128", "c64")), (12, ("c128", "c64")),
+         (15, ("c128", "c64")), (27, ("c128", "c64")), (30, ("c128", "c64")), (45, ("c64",)), (48, ("c64",)),
+         (60, ("c64",))]
 # rel-L2 bounds against the FP64 oracle
 TOL_1D = {"c128": 1e-15, "c64": 1.5e-7}
 
@@ -27,8 +31,10 @@ def _bits(a):
 
 
 @pytest.mark.parametrize("dtype", ["c128", "c64"])
-@pytest.mark.parametrize("N", LENGTHS)
+@pytest.mark.parametrize("N", LENGTHS + [n for n, _ in MIXED])
 def test_batched_1d_fft_vs_oracle(a2afft_mod, cuda, N, dtype):
+    if N not in LENGTHS and dtype not in dict(MIXED)[N]:
+        pytest.skip("no kernel of this length in this precision (a line must fit in registers)")
     import torch
     rng = np.random.default_rng(N)
     lines = max(3, 4096 // N) + 1          # ragged batch: not a multiple of the lines per block
@@ -72,6 +78,10 @@ def test_1d_known_answers(a2afft_mod, cuda, N):
 # decompositions of the benchmark configs at reduced Ng.
 CONFIGS_3D = [
     (64, 4, (2, 2, 1), "c128"),
+    (12, 6, None, "c128"),
+    (12, 6, None, "c64"),
+    (24, 12, None, "c128"),
+    (60, 4, (1, 2, 2), "c64"),
     (8, 8, None, "c128"),
     (32, 8, None, "c128"),
     (4, 4, None, "c128"),

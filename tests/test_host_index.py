@@ -84,6 +84,8 @@ FFT_CONFIGS = [
     (64, 4, (2, 2, 1)), (8, 8, (2, 2, 2)), (32, 8, (2, 2, 2)), (4, 4, (2, 2, 1)), (32, 8, (1, 2, 4)),
     (64, 16, (1, 4, 4)), (64, 16, (4, 2, 2)), (128, 64, (1, 8, 8)), (32, 64, (4, 4, 4)), (256, 8, (1, 2, 4)),
     (16, 1, (1, 1, 1)), (128, 16, (1, 4, 4)), (2, 1, (1, 1, 1)), (512, 16, (1, 4, 4)),
+    # mixed radix: the reference's (P=6, Ng=12) and two more
+    (12, 6, (3, 2, 1)), (24, 12, (3, 2, 2)), (60, 4, (1, 2, 2)),
 ]
 
 
@@ -125,7 +127,7 @@ def test_fused_pass_addresses(hc, cfg, dbl):
         assert (k_sorted == k_sorted[:, :1]).all(), f"pass {pass_}: a line mixes global lines"
 
 
-@pytest.mark.parametrize("N", [2, 4, 8, 16, 32, 64, 128, 256, 512, 1024, 2048, 4096])
+@pytest.mark.parametrize("N", [2, 4, 8, 16, 32, 64, 128, 256, 512, 1024, 2048, 4096, 3, 5, 6, 12, 15, 30, 45, 60])
 @pytest.mark.parametrize("lines", [1, 7, 33])
 def test_batched_pencil_addresses(hc, N, lines):
     for dbl in (0, 1):
@@ -147,7 +149,8 @@ def test_xcd_renumbering_is_a_bijection(hc):
                 assert np.all(np.diff(r) == 1), (G, x)
 
 
-SHAPE_CONFIGS = [(64, 16, (1, 4, 4)), (32, 8, (1, 2, 4)), (16, 4, (2, 2, 1)), (128, 16, (1, 4, 4))]
+SHAPE_CONFIGS = [(64, 16, (1, 4, 4)), (32, 8, (1, 2, 4)), (16, 4, (2, 2, 1)), (128, 16, (1, 4, 4)),
+                 (12, 6, (3, 2, 1)), (60, 4, (1, 2, 2))]
 
 
 @pytest.mark.parametrize("cfg", SHAPE_CONFIGS, ids=lambda c: f"Ng{c[0]}_P{c[1]}_{c[2]}")
@@ -162,7 +165,7 @@ def test_every_fitting_shape_addresses(hc, cfg):
     checked = 0
     for pass_, axis in ((0, 2), (1, 0), (2, 1)):
         for T in (1, 2, 4, 8, 16, 32, 64, 128):
-            for E in (2, 4, 8, 16, 32, 64):
+            for E in sorted({2, 4, 8, 16, 32, 64, Ng}):
                 if not hc.hc_shape_fits(Ng, P, _p(dm), pass_, T, E):
                     continue
                 for xcd in (0, 1, 2, 3):
