@@ -51,6 +51,7 @@ SIGNATURES = {
     "a2afft_status_string": [_I],
     "a2afft_last_error": [],
     "a2afft_set_sync": [_I],
+    "a2afft_set_tuning_cache": [ctypes.c_char_p],
     "a2afft_version": [],
     "a2afft_dims_create": [_I, _IA],
     "a2afft_dist_create": [ctypes.POINTER(_P), _I, _I, _IA, _I, _I],

@@ -34,6 +34,11 @@ def dims_create(nnodes):
     return tuple(d)
 
 
+def set_tuning_cache(path):
+    """Tuning cache file for every plan of the process (None: no cache)."""
+    return L.check(L.lib().a2afft_set_tuning_cache(path.encode() if path else None), "set_tuning_cache")
+
+
 def set_sync(on):
     """Block after every call, as the reference's launchers do."""
     L.lib().a2afft_set_sync(1 if on else 0)

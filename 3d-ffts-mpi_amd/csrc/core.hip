@@ -10,7 +10,11 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <mutex>
 #include <new>
+#include <string>
+#include <vector>
+#include <unistd.h>
 
 namespace a2a {
 
@@ -343,6 +347,123 @@ static int check_precision(a2afft_precision prec, a2afft_dtype dt) {
     return A2AFFT_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Tuning cache (no reference counterpart: the reference re-plans on every call,
+// a2affts.cu:15).  Autotune results are kept in a text file, one line per plan
+// key (device architecture, grid, decomposition, data type, precision and
+// strategy):  "<key> | z=T,E,F x=T,E,F y=T,E,F chunk=C".  A plan created
+// while a cache file is set takes its entry when every recorded kernel is
+// built and fits; anything else is ignored.
+// ---------------------------------------------------------------------------
+static std::mutex g_cache_mu;
+static std::string g_cache_path;
+
+extern "C" int a2afft_set_tuning_cache(const char* path) {
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    g_cache_path = path ? path : "";
+    return A2AFFT_OK;
+}
+
+static std::string plan_key(const a2afft_plan_s* p) {
+    int dev = 0;
+    hipDeviceProp_t prop;
+    std::string arch = "unknown";
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess) {
+        arch = prop.gcnArchName;
+        const size_t colon = arch.find(':');
+        if (colon != std::string::npos) arch.resize(colon);
+    }
+    char buf[256];
+    snprintf(buf, sizeof(buf), "a2afft-tune-v1 arch=%s grid=%dx%dx%d P=%d dims=%dx%dx%d dtype=%d prec=%d exec=%d",
+             arch.c_str(), p->n[0], p->n[1], p->n[2], p->d.P, p->d.dims[0], p->d.dims[1], p->d.dims[2],
+             int(p->d.dtype), int(p->prec), int(p->exec));
+    return buf;
+}
+
+static std::vector<std::string> cache_lines(const std::string& path) {
+    std::vector<std::string> out;
+    FILE* f = fopen(path.c_str(), "r");
+    if (!f) return out;
+    char line[1024];
+    while (fgets(line, sizeof(line), f)) {
+        std::string l(line);
+        while (!l.empty() && (l.back() == '\n' || l.back() == '\r')) l.pop_back();
+        if (!l.empty()) out.push_back(l);
+    }
+    fclose(f);
+    return out;
+}
+
+// Applies the cached entry of this plan, if any and valid; 1 when applied.
+static int cache_apply(a2afft_plan_s* p) {
+    std::string path;
+    {
+        std::lock_guard<std::mutex> lk(g_cache_mu);
+        path = g_cache_path;
+    }
+    if (path.empty() || p->exec != A2AFFT_EXEC_FUSED || is_split(p->prec)) return 0;
+    const std::string key = plan_key(p) + " | ";
+    for (const std::string& l : cache_lines(path)) {
+        if (l.compare(0, key.size(), key) != 0) continue;
+        KSel sel[3];
+        unsigned chunk = 0;
+        if (sscanf(l.c_str() + key.size(), "z=%d,%d,%d x=%d,%d,%d y=%d,%d,%d chunk=%u", &sel[0].T, &sel[0].E,
+                   &sel[0].flags, &sel[1].T, &sel[1].E, &sel[1].flags, &sel[2].T, &sel[2].E, &sel[2].flags,
+                   &chunk) != 10)
+            return 0;
+        const KSel keep[3] = {p->sel[0], p->sel[1], p->sel[2]};
+        const uint32_t keep_chunk = p->chunk;
+        bool ok = true;
+        for (int i = 0; i < 3 && ok; ++i) {
+            const int N = int(p->pass[i].fft.n);
+            FftShape sh;
+            ok = find_kernel(N, p->d.dtype == A2AFFT_C128, p->pass[i].strided, sel[i], -1) != nullptr &&
+                 fft_pass_shape(p->pass[i], p->d.dtype, N, &sel[i], &sh) == A2AFFT_OK;
+            if (ok) p->sel[i] = sel[i];
+        }
+        if (ok) {
+            p->chunk = chunk;
+            ok = chunk_ok(p, chunk);
+        }
+        if (!ok) {   // leave the defaults
+            for (int i = 0; i < 3; ++i) p->sel[i] = keep[i];
+            p->chunk = keep_chunk;
+            return 0;
+        }
+        snprintf(p->report, sizeof(p->report), "tuning from cache %s\n", path.c_str());
+        return 1;
+    }
+    return 0;
+}
+
+// Records this plan's selection in the cache file (replacing its old line).
+static int cache_store(const a2afft_plan_s* p) {
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    if (g_cache_path.empty()) return A2AFFT_OK;
+    const std::string key = plan_key(p) + " | ";
+    char val[160];
+    snprintf(val, sizeof(val), "z=%d,%d,%d x=%d,%d,%d y=%d,%d,%d chunk=%u", p->sel[0].T, p->sel[0].E,
+             p->sel[0].flags, p->sel[1].T, p->sel[1].E, p->sel[1].flags, p->sel[2].T, p->sel[2].E, p->sel[2].flags,
+             p->chunk);
+    std::vector<std::string> lines = cache_lines(g_cache_path);
+    std::vector<std::string> keep;
+    for (const std::string& l : lines)
+        if (l.compare(0, key.size(), key) != 0) keep.push_back(l);
+    keep.push_back(key + val);
+    // write a private temporary file, then rename it over the cache
+    const std::string tmp = g_cache_path + ".tmp." + std::to_string(long(getpid()));
+    FILE* f = fopen(tmp.c_str(), "w");
+    if (!f) return set_error(A2AFFT_E_INVALID_ARG, "tuning cache: cannot write %s", tmp.c_str());
+    bool ok = true;
+    for (const std::string& l : keep) ok = ok && fprintf(f, "%s\n", l.c_str()) > 0;
+    ok = (fclose(f) == 0) && ok;
+    if (!ok || rename(tmp.c_str(), g_cache_path.c_str()) != 0) {
+        (void)remove(tmp.c_str());
+        return set_error(A2AFFT_E_INVALID_ARG, "tuning cache: cannot update %s", g_cache_path.c_str());
+    }
+    return A2AFFT_OK;
+}
+
 extern "C" int a2afft_plan_create(a2afft_plan* out, a2afft_dist dist, a2afft_precision prec, a2afft_exec exec) {
     if (!out || !dist) return set_error(A2AFFT_E_INVALID_ARG, "plan_create: null");
     *out = nullptr;
@@ -363,6 +484,7 @@ extern "C" int a2afft_plan_create(a2afft_plan* out, a2afft_dist dist, a2afft_pre
         delete p;
         return st;
     }
+    (void)cache_apply(p);
     *out = p;
     return A2AFFT_OK;
 }
@@ -397,6 +519,7 @@ extern "C" int a2afft_grid_plan_create(a2afft_plan* out, int nx, int ny, int nz,
         delete p;
         return st;
     }
+    (void)cache_apply(p);
     *out = p;
     return A2AFFT_OK;
 }
@@ -619,7 +742,9 @@ extern "C" int a2afft_plan_autotune(a2afft_plan p, void* work, int iters, void* 
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    return st < 0 ? st : A2AFFT_OK;
+    if (st < 0) return st;
+    if (!is_split(p->prec)) A2A_TRY(cache_store(p));
+    return A2AFFT_OK;
 }
 
 // Text description of the chosen kernel per pass and the last autotune table.

@@ -231,6 +231,14 @@ int a2afft_execute_pass(a2afft_plan plan, void* d_data, int pass, int direction,
  * launches in place on `work` (a [P][nlocal] buffer whose contents are
  * destroyed), and keeps the fastest; set_kernel picks one by hand. */
 int a2afft_plan_autotune(a2afft_plan plan, void* work, int iters, void* stream);
+/* Process-wide tuning cache file (NULL: none); no reference counterpart (the
+ * reference re-plans on every call, a2affts.cu:15).  a2afft_plan_autotune
+ * records its plan's kernels and chunk width under a key of device
+ * architecture, grid, decomposition, data type, precision and strategy;
+ * fused plans created later with the same key start from that choice when
+ * every recorded kernel is built and fits.  The file is text, one line per
+ * key, rewritten through a temporary file and a rename. */
+int a2afft_set_tuning_cache(const char* path);
 int a2afft_plan_set_kernel(a2afft_plan plan, int pass, int T, int E, int flags);
 /* Chunked execution: after the z pass, the x and y passes alternate over
  * windows of `z_columns` z columns (0 = whole passes), so each window is
