@@ -186,6 +186,7 @@ __device__ __forceinline__ uint32_t claim_finish(Cursor& c, uint32_t ticket, con
 }
 
 template <int DIR>
+// At most 168 VGPRs (3 wavefronts per SIMD): three workgroups per CU measured fastest.
 __global__ __launch_bounds__(kBatchThreads, 3) void batch_kernel(float2* __restrict__ data,
                                                               const BatchItem* __restrict__ items,
                                                               const uint32_t* __restrict__ unit_item, int G,
