@@ -74,6 +74,15 @@ def cpu_baseline(Ng, P, dims, threads, seed):
     return t * 1e3, O.get_threads()
 
 
+def _traffic(key):
+    """Measured HBM bytes per launch (profiles/traffic.json), or None."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+            return json.load(f).get(key)
+    except Exception:
+        return None
+
+
 def _events_ms(torch, fn, steps):
     """Average ms of fn() over `steps` calls, HIP events on the launch stream."""
     fn()
@@ -229,7 +238,7 @@ def run_c5(args, torch, a2afft):
         "rel_l2_vs_fp64": {"max": max(errs), "mean": sum(errs) / len(errs)},
         "roofline": {"bound": "hbm", "kernel": "batch_kernel (persistent)", "bytes_per_launch": bytes_alg,
                      "launch_ms": round(best, 4), "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None},
+                     "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": _traffic("c5_batch")},
         "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
@@ -431,14 +440,7 @@ def main():
     gflops = 5.0 * N * math.log2(N) / (ms_per_step * 1e-3) / 1e9
     whole_gbs = 3 * bytes_pass / (ms_per_step * 1e-3) / 1e9
 
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic.json")
-    if os.path.exists(tpath):
-        try:
-            with open(tpath) as f:
-                traffic = json.load(f).get(f"{Ng}_{args.precision}_{names[dom]}")
-        except Exception:
-            traffic = None
+    traffic = _traffic(f"{Ng}_{args.precision}_{names[dom]}")
 
     cpu = None
     if not args.no_cpu:
