@@ -58,7 +58,7 @@ int fft_candidates(const AxisPass& p, a2afft_dtype dt, int N, KSel* out, int max
 
 // Measured preferences (MI355X autotune, 512^3 over dims [1,4,4]): XCD-aware
 // order and non-temporal access for the z and y passes; the far-strided x
-// pass prefers 256-byte rows (T=32, complex64) and z tiles outermost.  The
+// pass prefers z tiles outermost.  The
 // first entry that is built and fits the geometry wins; otherwise the
 // generic default with XCD order.
 // `far`: 1 = the transform axis steps >= 64 KiB between elements (the x
@@ -69,8 +69,8 @@ struct Pref {
 };
 static const Pref kPrefs[] = {
     {512, 0, 0, -1, {16, 16, KF_XCD | KF_PAD | KF_NT}},
-    {512, 0, 1, 1, {32, 32, KF_A2MAJ}},
-    {512, 0, 1, -1, {16, 32, KF_XCD | KF_NT}},
+    {512, 0, 1, 1, {16, 32, KF_A2MAJ}},
+    {512, 0, 1, -1, {16, 32, KF_XCD | KF_PAD | KF_NT}},
     {512, 1, 0, -1, {8, 16, KF_XCD | KF_PAD}},
     {512, 1, 1, 1, {8, 16, KF_XCD | KF_PAD | KF_A2MAJ}},
     {512, 1, 1, -1, {8, 16, KF_XCD | KF_NT}},
