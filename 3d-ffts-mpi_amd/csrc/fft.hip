@@ -56,11 +56,12 @@ int fft_candidates(const AxisPass& p, a2afft_dtype dt, int N, KSel* out, int max
     return n;
 }
 
-// Measured preferences (MI355X autotune, 512^3 over dims [1,4,4]): XCD-aware
-// order and non-temporal access for the z and y passes; the far-strided x
-// pass prefers z tiles outermost.  The
-// first entry that is built and fits the geometry wins; otherwise the
-// generic default with XCD order.
+// Measured preferences (MI355X autotune: 512 on 512^3 over dims [1,4,4], the
+// other lengths on single grids).  XCD-aware order and non-temporal access
+// help the 512 z and y passes; the far-strided x pass prefers z tiles
+// outermost; padded LDS helps the long contiguous lines (2048: 0.40 vs
+// 0.48 ms over 256x256 lines).  The first entry that is built and fits the
+// geometry wins; otherwise the generic default with XCD order.
 // `far`: 1 = the transform axis steps >= 64 KiB between elements (the x
 // axis of a block), 0 = nearer, -1 = either.
 struct Pref {
@@ -71,11 +72,14 @@ static const Pref kPrefs[] = {
     {512, 0, 0, -1, {16, 16, KF_XCD | KF_PAD | KF_NT}},
     {512, 0, 1, 1, {16, 32, KF_A2MAJ}},
     {512, 0, 1, -1, {16, 32, KF_XCD | KF_PAD | KF_NT}},
-    {512, 1, 0, -1, {8, 16, KF_XCD | KF_PAD}},
-    {512, 1, 1, 1, {8, 16, KF_XCD | KF_PAD | KF_A2MAJ}},
+    {512, 1, 0, -1, {8, 16, KF_PAD}},
+    {512, 1, 1, 1, {8, 16, KF_XCD | KF_NT | KF_A2MAJ}},
     {512, 1, 1, -1, {8, 16, KF_XCD | KF_NT}},
     {256, 0, 1, 1, {32, 32, KF_XCD | KF_PAD}},
     {256, 0, 1, -1, {16, 32, KF_XCD}},
+    {256, 1, 0, -1, {16, 16, KF_PAD}},
+    {2048, 0, 0, -1, {4, 32, KF_PAD}},
+    {4096, 0, 0, -1, {1, 16, KF_PAD}},
 };
 
 KSel fft_default_sel(const AxisPass& p, a2afft_dtype dt, int N) {
