@@ -16,11 +16,15 @@
 // Schedule: a radix-16 stages, then one VALU stage of radix R = N / 16^a
 // (2, 4 or 8) when N is not a power of 16 -- 512 = 16 * 16 * 2.
 //
-// Generic kernel (lengths 16..128): 256 threads, a tile of T lines (contiguous lines, or T adjacent
-// columns for strided passes) staged in LDS as a real and an imaginary
-// plane of floats.  Each wavefront owns T/4 whole lines during the MFMA
-// stages, so those need no barriers; the load, the final stage and the store
-// use a coalescing thread map and are separated by two barriers.
+// Two kernels:
+//   mfma_fft_direct (256 and 512): one LDS exchange, operands straight from
+//     and results straight to global memory (described at the kernel).
+//   mfma_fft_kernel (16..128): 256 threads, a tile of T lines (contiguous
+//     lines, or T adjacent columns for strided passes) staged in LDS as a
+//     real and an imaginary plane of floats.  Each wavefront owns T/4 whole
+//     lines during the MFMA stages, so those need no barriers; the load, the
+//     final stage and the store use a coalescing thread map and are
+//     separated by two barriers.
 //
 // The operand layouts used: A (16 x 32) lane l holds row l % 16, B (32 x 16)
 // lane l holds column l % 16, each with 8 K values chosen by l / 16 -- the
@@ -30,9 +34,9 @@
 #include "stockham.h"
 
 #include <cmath>
-#include <type_traits>
 #include <map>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 namespace a2a {
@@ -68,8 +72,9 @@ A2A_HD constexpr int mf_twoff(int N, int p) {
     return off;
 }
 A2A_HD constexpr int mf_twsize(int N) { return mf_twoff(N, mf_nstages(N)); }
-// Tile width: 128 B rows for strided tiles; enough lines that each of the 4
-// wavefronts owns whole lines holding a multiple of 16 butterflies.
+// Tile width: 16 lines for 256 and 512 (the direct kernel); otherwise 128 B
+// rows for strided tiles and enough lines that each of the 4 wavefronts owns
+// whole lines holding a multiple of 16 butterflies.
 A2A_HD constexpr int mf_T(int N, bool strided) {
     return N >= 256 ? 16 : ((1024 / N) > (strided ? 16 : 4) ? (1024 / N) : (strided ? 16 : 4));
 }
