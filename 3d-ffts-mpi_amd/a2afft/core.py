@@ -276,6 +276,13 @@ class Batch:
         return L.check(L.lib().a2afft_batch_set_schedule(self._h, workgroups, 1 if xcd_queues else 0, chunk),
                        "batch_set_schedule")
 
+    def set_grouping(self, group_points=0, skewed=False):
+        """Groups of consecutive grids of at most group_points points whose
+        passes run group by group (0 = all grids in one group), optionally
+        skewed; every order computes the same bits."""
+        return L.check(L.lib().a2afft_batch_set_grouping(self._h, int(group_points), 1 if skewed else 0),
+                       "batch_set_grouping")
+
     def info(self):
         u, w, m = ctypes.c_int64(), ctypes.c_int(), ctypes.c_int()
         L.check(L.lib().a2afft_batch_info(self._h, ctypes.byref(u), ctypes.byref(w), ctypes.byref(m)), "batch_info")

@@ -3,27 +3,35 @@
 // cubic grid per a2aDistribution, a2adistribution.h:9).
 //
 // Work queue.  Every (pass, grid) pair is split into the units its Stockham
-// kernel would use (lines or column tiles) and the units are numbered
-// pass-major: all z units of all grids, then all x units, then all y units.
-// Workgroups take units in that order from atomic counters; a table maps each
-// unit to its (pass, grid) item.  The take of the next unit is issued before
-// the current unit is computed and examined after it, so the latency of the
-// device-scope atomic hides behind the unit's own loads.
+// kernel would use (lines or column tiles).  The grids are packed into groups
+// (one group of all grids by default) and the work is cut into phases: one
+// phase per (group, pass).  Units are numbered phase by phase, in an order
+// the host chooses: group-major (z, x, y of group 0, then of group 1, ...) or
+// skewed along diagonals, so that the phase between a pass and the next pass
+// of the same group belongs to another group.  Groups small enough to stay in
+// the device's memory-side cache between their passes read back what their
+// previous pass wrote from that cache instead of HBM.  Workgroups take units
+// in queue order from atomic counters; a table maps each unit to its
+// (phase, grid) item.  The take of the next unit is issued before the current
+// unit is computed and examined after it, so the latency of the device-scope
+// atomic hides behind the unit's own loads.
 //
-// Synchronisation is per pass, not per unit.  The 8 XCDs of the device have
+// Synchronisation is per phase, not per unit.  The 8 XCDs of the device have
 // L2 caches that are not coherent with one another, and a device-scope fence
 // writes back or invalidates a whole L2, so the kernel uses none: every unit
 // load and store is a relaxed device-scope access (sc1), coherent across the
 // XCDs by itself.  A workgroup counts the units it finishes and, only when
-// the next unit it takes belongs to a later pass (or the queues are empty),
-// waits for its own stores to complete, adds its count to that pass's total
-// and waits until the previous pass's total is complete.  That is at most three transitions
-// per workgroup.  Every wait ends: every unit of pass p is taken before any
-// unit of pass p+1, and the workgroup holding it finishes it after at most a
-// wait for pass p-1 (it may hold one taken unit of pass p+1 meanwhile, which
-// nobody waits on yet), so by induction over p every pass completes.  The
-// wait is still bounded, and a bound that trips sets an error flag instead of
-// hanging.
+// the next unit it takes belongs to another phase (or the queues are empty),
+// waits for its own stores to complete, adds its count to that phase's total
+// and, if the new phase is a later pass of a group, waits until the phase of
+// the previous pass of that group is complete.  Every wait ends: every unit of
+// phase q is taken before any unit of a later phase, a phase depends only on
+// an earlier one, and the workgroup holding a unit of phase q finishes it
+// after at most a wait for that earlier phase (it may hold one taken unit of
+// a later phase meanwhile, which nobody waits on yet), so by induction over q
+// every phase completes.  The wait is still bounded, and a bound that trips
+// sets an error flag, after which every other wait gives up at once, instead
+// of hanging.
 #include "fft.h"
 #include "stockham.h"
 
@@ -36,8 +44,9 @@ namespace a2a {
 namespace {
 
 constexpr int kBatchThreads = 256;
-constexpr uint32_t kSpinLimit = 1u << 24;
+constexpr uint32_t kSpinLimit = 1u << 20;   // polls of ~8K cycles each: seconds
 constexpr uint32_t kMaxUnits = 1u << 28;   // the unit -> item table holds 4 bytes per unit
+constexpr int kMaxPhases = 384;            // (group, pass) phases: at most 128 groups
 
 // Kernel shape per (N, strided): primary and a single-column fallback.
 struct BShape {
@@ -60,6 +69,7 @@ struct BatchItem {
     uint32_t units;
     uint32_t first;      // number of this item's first unit in the queue
     int shape;           // index into kShapes
+    int phase;           // the (group, pass) phase the item belongs to
 };
 
 template <int DIR, typename Hook>
@@ -90,31 +100,32 @@ constexpr int batch_lds_max() {
     }
 }
 
-// Unit queues.  Pass p owns units [pstart[p], pstart[p+1]) (pass-major
-// numbering); with xcd_queues they are split into 8 contiguous sub-ranges,
-// one per XCD (the hardware hands workgroup b to XCD b % 8), taken through 8
-// counters; a workgroup whose own sub-range is empty takes from the others.
-// One take hands out `chunk` consecutive units.  A workgroup moves to pass
-// p+1 only after every counter of pass p is exhausted, so all units of a pass
-// are taken before any unit of the next.
+// Unit queues.  Phase q owns units [start[q], start[q+1]); with xcd_queues
+// they are split into 8 contiguous sub-ranges, one per XCD (the hardware
+// hands workgroup b to XCD b % 8), taken through 8 counters; a workgroup
+// whose own sub-range is empty takes from the others.  One take hands out
+// `chunk` consecutive units.  A workgroup moves to phase q+1 only after every
+// counter of phase q is exhausted, so all units of a phase are taken before
+// any unit of a later one.
 //
 // Control words, each in its own 256-byte slot so that takes on different
-// counters do not serialise on one cache line: slot 0..23 take counters
-// (pass * 8 + queue; queue 0 only without xcd_queues), slot 24 error flag,
-// slot 25 + p units of pass p finished and released.
-constexpr int kCtrlTake = 0, kCtrlErr = 24, kCtrlDone = 25, kCtrlSlots = 28, kCtrlStride = 64;
+// counters do not serialise on one cache line: slot 0 error flag, slot 1 + q
+// units of phase q finished and released, slot 1 + kMaxPhases + 8 q + x take
+// counter of queue x of phase q (queue 0 only without xcd_queues).
+constexpr int kCtrlErr = 0, kCtrlDone = 1, kCtrlTake = 1 + kMaxPhases, kCtrlSlots = 1 + 9 * kMaxPhases;
+constexpr int kCtrlStride = 64;
 constexpr uint32_t kNoUnit = 0xffffffffu;
 
 __host__ __device__ constexpr int ctrl_word(int slot) { return slot * kCtrlStride; }
 
 struct BatchQueues {
-    uint32_t pstart[4];
+    int nphases;
     int xcd_queues;
     uint32_t chunk;
 };
 
 // Where thread 0 of a workgroup takes its next unit: the rest [next, end) of
-// its current chunk, else pass `scan`, the i-th queue counted from its own.
+// its current chunk, else phase `scan`, the i-th queue counted from its own.
 // Exhausted queues stay exhausted (counters only grow), so the cursor only
 // moves forward.
 struct Cursor {
@@ -122,28 +133,20 @@ struct Cursor {
     uint32_t next, end;
 };
 
-// Pass bounds held in registers.  Thread 0 must not load them from memory
+// The phase table in LDS.  Thread 0 must not read it from global memory
 // inside the unit loop: the wait for such a load would also wait for its own
-// outstanding stores of the previous unit.
-struct PassBounds {
-    uint32_t s0, s1, s2, s3, chunk;
-    __device__ __forceinline__ explicit PassBounds(const BatchQueues& qs)
-        : s0(uint32_t(__builtin_amdgcn_readfirstlane(int(qs.pstart[0])))),
-          s1(uint32_t(__builtin_amdgcn_readfirstlane(int(qs.pstart[1])))),
-          s2(uint32_t(__builtin_amdgcn_readfirstlane(int(qs.pstart[2])))),
-          s3(uint32_t(__builtin_amdgcn_readfirstlane(int(qs.pstart[3])))),
-          chunk(uint32_t(__builtin_amdgcn_readfirstlane(int(qs.chunk)))) {}
+// outstanding stores of the previous unit, while an LDS read waits for LDS
+// only.
+struct PhaseTable {
+    const uint32_t* start;   // nphases + 1 unit numbers
+    const int* dep;          // phase that must be complete first, or -1
+    int n;
+    uint32_t chunk;
 };
 
-// Units of pass p (0..2): [start(p), start(p + 1)).
-__device__ __forceinline__ uint32_t pass_start(const PassBounds& qs, int p) {
-    const uint32_t m1 = p >= 1 ? ~0u : 0u, m2 = p >= 2 ? ~0u : 0u, m3 = p >= 3 ? ~0u : 0u;
-    return qs.s0 + ((qs.s1 - qs.s0) & m1) + ((qs.s2 - qs.s1) & m2) + ((qs.s3 - qs.s2) & m3);
-}
-
-__device__ __forceinline__ void queue_range(const PassBounds& qs, int p, int x, int nq, uint32_t& lo, uint32_t& hi) {
-    const uint32_t a = pass_start(qs, p);
-    const uint32_t len = pass_start(qs, p + 1) - a;
+__device__ __forceinline__ void queue_range(const PhaseTable& pt, int p, int x, int nq, uint32_t& lo, uint32_t& hi) {
+    const uint32_t a = pt.start[p];
+    const uint32_t len = pt.start[p + 1] - a;
     lo = a + uint32_t((uint64_t(len) * uint32_t(x)) / uint32_t(nq));
     hi = a + uint32_t((uint64_t(len) * uint32_t(x + 1)) / uint32_t(nq));
 }
@@ -151,37 +154,55 @@ __device__ __forceinline__ void queue_range(const PassBounds& qs, int p, int x, 
 // Issues the take from the cursor's queue and returns the raw ticket without
 // examining it, so the atomic's latency overlaps the unit being computed.  No
 // take while the current chunk has units left.
-__device__ __forceinline__ uint32_t claim_issue(const Cursor& c, const PassBounds& qs, int myq, int nq,
+__device__ __forceinline__ uint32_t claim_issue(const Cursor& c, const PhaseTable& pt, int myq, int nq,
                                                 uint32_t* ctrl) {
-    if (c.next < c.end || c.scan >= 3) return kNoUnit;
+    if (c.next < c.end || c.scan >= pt.n) return kNoUnit;
     const int x = (myq + c.i) % nq;
     uint32_t lo, hi;
-    queue_range(qs, c.scan, x, nq, lo, hi);
+    queue_range(pt, c.scan, x, nq, lo, hi);
     return lo < hi ? atomicAdd(&ctrl[ctrl_word(kCtrlTake + c.scan * 8 + x)], 1u) : kNoUnit;
 }
 
 // Turns a ticket into a unit number, moving on through the queues (with
 // further takes) while the ticket falls past the end of its queue; kNoUnit
-// when every queue of every pass is exhausted.
-__device__ __forceinline__ uint32_t claim_finish(Cursor& c, uint32_t ticket, const PassBounds& qs, int myq, int nq,
+// when every queue of every phase is exhausted.
+__device__ __forceinline__ uint32_t claim_finish(Cursor& c, uint32_t ticket, const PhaseTable& pt, int myq, int nq,
                                                  uint32_t* ctrl) {
     if (c.next < c.end) return c.next++;
     for (;;) {
-        if (c.scan >= 3) return kNoUnit;
+        if (c.scan >= pt.n) return kNoUnit;
         const int x = (myq + c.i) % nq;
         uint32_t lo, hi;
-        queue_range(qs, c.scan, x, nq, lo, hi);
-        if (lo < hi && ticket < (hi - lo + qs.chunk - 1) / qs.chunk) {
-            const uint32_t first = lo + ticket * qs.chunk;
+        queue_range(pt, c.scan, x, nq, lo, hi);
+        if (lo < hi && ticket < (hi - lo + pt.chunk - 1) / pt.chunk) {
+            const uint32_t first = lo + ticket * pt.chunk;
             c.next = first + 1;
-            c.end = min(first + qs.chunk, hi);
+            c.end = min(first + pt.chunk, hi);
             return first;
         }
         if (++c.i == nq) {
             c.i = 0;
             ++c.scan;
         }
-        ticket = claim_issue(c, qs, myq, nq, ctrl);
+        ticket = claim_issue(c, pt, myq, nq, ctrl);
+    }
+}
+
+// Waits until `need` units of phase q are finished and released, giving up
+// (and raising the error flag) after kSpinLimit polls, or within 16 polls once
+// another wait has given up.
+__device__ __forceinline__ void wait_phase(uint32_t* ctrl, int q, uint32_t need) {
+    uint32_t spins = 0;
+    while (__hip_atomic_load(&ctrl[ctrl_word(kCtrlDone + q)], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
+        __builtin_amdgcn_s_sleep(127);   // long polls measured a little faster: less contention on the word
+        ++spins;
+        if ((spins & 15u) == 0 &&
+            __hip_atomic_load(&ctrl[ctrl_word(kCtrlErr)], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u)
+            return;
+        if (spins > kSpinLimit) {
+            __hip_atomic_store(&ctrl[ctrl_word(kCtrlErr)], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return;
+        }
     }
 }
 
@@ -189,23 +210,30 @@ template <int DIR>
 // At most 168 VGPRs (3 wavefronts per SIMD): three workgroups per CU measured fastest.
 __global__ __launch_bounds__(kBatchThreads, 3) void batch_kernel(float2* __restrict__ data,
                                                               const BatchItem* __restrict__ items,
-                                                              const uint32_t* __restrict__ unit_item, int G,
-                                                              BatchQueues qs, uint32_t* __restrict__ ctrl) {
+                                                              const uint32_t* __restrict__ unit_item,
+                                                              const uint32_t* __restrict__ phases, BatchQueues qs,
+                                                              uint32_t* __restrict__ ctrl) {
     __shared__ float2 lds[batch_lds_max<0>()];
+    __shared__ uint32_t s_start[kMaxPhases + 1];
+    __shared__ int s_dep[kMaxPhases];
     __shared__ uint32_t s_unit[2];   // alternate per unit: one barrier per unit suffices
     const int tid = int(threadIdx.x);
+    const int nph = __builtin_amdgcn_readfirstlane(qs.nphases);
+    for (int i = tid; i <= nph; i += kBatchThreads) s_start[i] = phases[i];
+    for (int i = tid; i < nph; i += kBatchThreads) s_dep[i] = int(phases[nph + 1 + i]);
+    __syncthreads();
     const int nq = qs.xcd_queues ? 8 : 1;
     const int myq = qs.xcd_queues ? int(blockIdx.x & 7u) : 0;
-    const PassBounds pb(qs);
+    const PhaseTable pt{s_start, s_dep, nph, uint32_t(__builtin_amdgcn_readfirstlane(int(qs.chunk)))};
     Cursor c{0, 0, 0u, 0u};
-    if (tid == 0) s_unit[0] = claim_finish(c, claim_issue(c, pb, myq, nq, ctrl), pb, myq, nq, ctrl);
+    if (tid == 0) s_unit[0] = claim_finish(c, claim_issue(c, pt, myq, nq, ctrl), pt, myq, nq, ctrl);
     __syncthreads();
     uint32_t unit = uint32_t(__builtin_amdgcn_readfirstlane(int(s_unit[0])));
-    int cur = 0;          // pass of the units this workgroup is finishing
-    uint32_t count = 0;   // units of pass `cur` it has finished
+    int cur = 0;          // phase of the units this workgroup is finishing
+    uint32_t count = 0;   // units of phase `cur` it has finished
     for (int it_no = 1;; ++it_no) {
         const int k = unit == kNoUnit ? -1 : int(unit_item[unit]);
-        const int q = k < 0 ? 3 : k / G;   // pass of this unit; 3 = queues empty
+        const int q = k < 0 ? nph : __builtin_amdgcn_readfirstlane(items[k].phase);   // nph: queues empty
         if (q != cur) {
             // Release: the unit data is written with device-coherent stores, so
             // waiting for this wavefront's stores to complete is enough (no L2
@@ -213,32 +241,22 @@ __global__ __launch_bounds__(kBatchThreads, 3) void batch_kernel(float2* __restr
             __builtin_amdgcn_s_waitcnt(0);
             __syncthreads();
             if (tid == 0) {
-                atomicAdd(&ctrl[ctrl_word(kCtrlDone + cur)], count);
-                if (q < 3) {   // wait until every unit of pass q-1 is finished and released
-                    const uint32_t need = pass_start(pb, q) - pass_start(pb, q - 1);
-                    uint32_t spins = 0;
-                    while (__hip_atomic_load(&ctrl[ctrl_word(kCtrlDone + q - 1)], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
-                           need) {
-                        __builtin_amdgcn_s_sleep(8);
-                        if (++spins > kSpinLimit) {
-                            __hip_atomic_store(&ctrl[ctrl_word(kCtrlErr)], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                            break;
-                        }
-                    }
-                }
+                if (count) atomicAdd(&ctrl[ctrl_word(kCtrlDone + cur)], count);
+                const int dq = q < nph ? pt.dep[q] : -1;
+                if (dq >= 0) wait_phase(ctrl, dq, pt.start[dq + 1] - pt.start[dq]);
             }
             __syncthreads();   // acquire: the unit loads that follow are device-coherent too
             cur = q;
             count = 0;
         }
-        if (q == 3) break;
+        if (q == nph) break;
         uint32_t ticket;
-        if (tid == 0) ticket = claim_issue(c, pb, myq, nq, ctrl);
+        if (tid == 0) ticket = claim_issue(c, pt, myq, nq, ctrl);
         const BatchItem& it = items[k];
         // The ticket is examined once the unit's loads are consumed and before
         // its stores are issued, so thread 0 never waits for its own stores.
         run_unit<DIR>(it.shape, data + it.base, it.g, it.tw, unit - it.first, lds, tid, [&]() {
-            if (tid == 0) s_unit[it_no & 1] = claim_finish(c, ticket, pb, myq, nq, ctrl);
+            if (tid == 0) s_unit[it_no & 1] = claim_finish(c, ticket, pt, myq, nq, ctrl);
         });
         ++count;
         __syncthreads();   // the unit is done with LDS; s_unit is written
@@ -246,8 +264,9 @@ __global__ __launch_bounds__(kBatchThreads, 3) void batch_kernel(float2* __restr
     }
 }
 
-__global__ void batch_reset_kernel(uint32_t* ctrl, int n) {
-    for (int i = int(threadIdx.x); i < n; i += int(blockDim.x)) ctrl[i] = 0u;
+// Clears the first word of every control slot (the only word used).
+__global__ void batch_reset_kernel(uint32_t* ctrl, int nslots) {
+    for (int i = int(threadIdx.x); i < nslots; i += int(blockDim.x)) ctrl[ctrl_word(i)] = 0u;
 }
 
 }  // namespace a2a
@@ -258,12 +277,16 @@ struct a2afft_batch_s {
     int G = 0;
     std::vector<int> shapes;          // 3 per grid
     std::vector<int64_t> offsets;     // G + 1
-    BatchItem* d_items[2] = {nullptr, nullptr};   // forward, inverse
-    uint32_t* d_unit_item = nullptr;   // item of every unit
+    std::vector<BatchItem> proto[2];  // forward, inverse: item of (pass p, grid g) at p * G + g
+    BatchItem* d_items[2] = {nullptr, nullptr};
+    uint32_t* d_unit_item = nullptr;  // item of every unit
+    uint32_t* d_phases = nullptr;     // phase starts (nphases + 1), then dependencies (nphases)
     uint32_t* d_ctrl = nullptr;
+    std::vector<uint32_t> pstart;     // host copy of the phase starts
     uint32_t total = 0;
-    BatchQueues qs;
-    int ctrl_len = 0;
+    BatchQueues qs{0, 0, 1};
+    int64_t group_points = 0;
+    int skewed = 0;
     int grid = 0;                     // persistent workgroups
     int max_grid = 0;
     hipStream_t cap_stream = nullptr;
@@ -271,13 +294,21 @@ struct a2afft_batch_s {
     void* graph_data[2] = {nullptr, nullptr};
 };
 
+static void drop_graphs(a2afft_batch_s* b) {
+    for (int d = 0; d < 2; ++d) {
+        if (b->graph[d]) (void)hipGraphExecDestroy(b->graph[d]);
+        b->graph[d] = nullptr;
+        b->graph_data[d] = nullptr;
+    }
+}
+
 static void batch_free(a2afft_batch_s* b) {
     if (!b) return;
-    for (int d = 0; d < 2; ++d) {
+    drop_graphs(b);
+    for (int d = 0; d < 2; ++d)
         if (b->d_items[d]) (void)hipFree(b->d_items[d]);
-        if (b->graph[d]) (void)hipGraphExecDestroy(b->graph[d]);
-    }
     if (b->d_unit_item) (void)hipFree(b->d_unit_item);
+    if (b->d_phases) (void)hipFree(b->d_phases);
     if (b->d_ctrl) (void)hipFree(b->d_ctrl);
     if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
     delete b;
@@ -291,6 +322,98 @@ static int pick_shape(const AxisPass& p, int N) {
     return -1;
 }
 
+template <typename T>
+static int upload(T** dst, const std::vector<T>& v) {
+    *dst = nullptr;
+    A2A_TRY(hip_check(int(hipMalloc(dst, sizeof(T) * v.size())), "batch alloc"));
+    return hip_check(int(hipMemcpy(*dst, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice)), "batch copy");
+}
+
+// Builds the queue: groups of consecutive grids of at most group_points
+// points (0 = one group; a larger grid forms a group of its own), one phase
+// per (group, pass), phases group-major or skewed along the diagonals
+// group + pass (the later pass first), units numbered phase by phase.  The
+// device tables are replaced only when every new one is in place.
+static int batch_tables(a2afft_batch_s* b, int64_t group_points, int skewed) {
+    const int G = b->G;
+    std::vector<int> first{0};
+    if (group_points > 0) {
+        int64_t acc = 0;
+        for (int g = 0; g < G; ++g) {
+            const int64_t n = b->offsets[g + 1] - b->offsets[g];
+            if (g > first.back() && acc + n > group_points) {
+                first.push_back(g);
+                acc = 0;
+            }
+            acc += n;
+        }
+    }
+    first.push_back(G);
+    const int W = int(first.size()) - 1;
+    if (3 * W > kMaxPhases)
+        return set_error(A2AFFT_E_SIZE, "batch: %d groups of at most %lld points (at most %d groups)", W,
+                         (long long)group_points, kMaxPhases / 3);
+    std::vector<int> ph_group, ph_pass;
+    for (int d = 0; d < (skewed ? W + 2 : W); ++d)
+        for (int i = 0; i < 3; ++i) {
+            const int p = skewed ? 2 - i : i;
+            const int w = skewed ? d - p : d;
+            if (w < 0 || w >= W) continue;
+            ph_group.push_back(w);
+            ph_pass.push_back(p);
+        }
+    const int nph = int(ph_group.size());
+    std::vector<int> index(size_t(3 * W), -1);
+    for (int q = 0; q < nph; ++q) index[size_t(3 * ph_group[q] + ph_pass[q])] = q;
+    std::vector<BatchItem> items[2];
+    std::vector<uint32_t> phases(size_t(2 * nph + 1)), unit_item;
+    uint64_t total = 0;
+    for (int q = 0; q < nph; ++q) {
+        const int w = ph_group[q], p = ph_pass[q];
+        phases[size_t(q)] = uint32_t(total);
+        phases[size_t(nph + 1 + q)] = p > 0 ? uint32_t(index[size_t(3 * w + p - 1)]) : 0xffffffffu;
+        for (int g = first[size_t(w)]; g < first[size_t(w) + 1]; ++g) {
+            const uint32_t units = b->proto[0][size_t(p * G + g)].units;
+            if (total + units > kMaxUnits) return set_error(A2AFFT_E_SIZE, "batch: more than %u work units", kMaxUnits);
+            for (int d = 0; d < 2; ++d) {
+                BatchItem it = b->proto[d][size_t(p * G + g)];
+                it.first = uint32_t(total);
+                it.phase = q;
+                items[d].push_back(it);
+            }
+            unit_item.insert(unit_item.end(), units, uint32_t(items[0].size() - 1));
+            total += units;
+        }
+    }
+    phases[size_t(nph)] = uint32_t(total);
+    BatchItem* d_items[2] = {nullptr, nullptr};
+    uint32_t *d_unit_item = nullptr, *d_phases = nullptr;
+    int st = upload(&d_items[0], items[0]);
+    if (st == A2AFFT_OK) st = upload(&d_items[1], items[1]);
+    if (st == A2AFFT_OK) st = upload(&d_unit_item, unit_item);
+    if (st == A2AFFT_OK) st = upload(&d_phases, phases);
+    if (st < 0) {
+        for (void* q : {(void*)d_items[0], (void*)d_items[1], (void*)d_unit_item, (void*)d_phases})
+            if (q) (void)hipFree(q);
+        return st;
+    }
+    drop_graphs(b);   // they hold the old tables
+    for (int d = 0; d < 2; ++d) {
+        if (b->d_items[d]) (void)hipFree(b->d_items[d]);
+        b->d_items[d] = d_items[d];
+    }
+    if (b->d_unit_item) (void)hipFree(b->d_unit_item);
+    if (b->d_phases) (void)hipFree(b->d_phases);
+    b->d_unit_item = d_unit_item;
+    b->d_phases = d_phases;
+    b->pstart.assign(phases.begin(), phases.begin() + nph + 1);
+    b->total = uint32_t(total);
+    b->qs.nphases = nph;
+    b->group_points = group_points;
+    b->skewed = skewed;
+    return A2AFFT_OK;
+}
+
 extern "C" int a2afft_batch_create(a2afft_batch* out, int ngrids, const int* shapes, a2afft_precision prec) {
     if (!out || ngrids <= 0 || !shapes) return set_error(A2AFFT_E_INVALID_ARG, "batch_create: bad arguments");
     *out = nullptr;
@@ -301,10 +424,8 @@ extern "C" int a2afft_batch_create(a2afft_batch* out, int ngrids, const int* sha
     b->shapes.assign(shapes, shapes + 3 * ngrids);
     b->offsets.assign(ngrids + 1, 0);
     const int K = 3 * ngrids;
-    std::vector<BatchItem> items[2];
-    items[0].resize(K);
-    items[1].resize(K);
-    std::vector<uint32_t> prefix(K + 1, 0);
+    b->proto[0].resize(K);
+    b->proto[1].resize(K);
     for (int g = 0; g < ngrids; ++g) {
         const int* n = shapes + 3 * g;
         const int64_t total = int64_t(n[0]) * n[1] * n[2];
@@ -327,7 +448,7 @@ extern "C" int a2afft_batch_create(a2afft_batch* out, int ngrids, const int* sha
                 return set_error(A2AFFT_E_UNSUPPORTED, "batch: no kernel shape for grid %d pass %d", g, p);
             }
             for (int d = 0; d < 2; ++d) {
-                BatchItem& it = items[d][k];
+                BatchItem& it = b->proto[d][k];
                 uint32_t units = 0;
                 int st = resolve_with(ps.fft, ps.a1, ps.a2, ps.strided, N, kShapes[si].T, kShapes[si].E, &it.g, &units);
                 if (st < 0) {
@@ -338,7 +459,9 @@ extern "C" int a2afft_batch_create(a2afft_batch* out, int ngrids, const int* sha
                 it.g.scale = (d == 1 && p == 2) ? double(total) : 1.0;
                 it.base = b->offsets[g];
                 it.units = units;
+                it.first = 0;
                 it.shape = si;
+                it.phase = 0;
                 it.tw = (const float2*)twiddle_table(N, d == 0 ? -1 : 1, A2AFFT_C64);
                 if (!it.tw) {
                     batch_free(b);
@@ -347,37 +470,9 @@ extern "C" int a2afft_batch_create(a2afft_batch* out, int ngrids, const int* sha
             }
         }
     }
-    for (int k = 0; k < K; ++k) {
-        const uint64_t next = uint64_t(prefix[k]) + items[0][k].units;
-        if (next > kMaxUnits) {
-            batch_free(b);
-            return set_error(A2AFFT_E_SIZE, "batch: more than %u work units", kMaxUnits);
-        }
-        prefix[k + 1] = uint32_t(next);
-        items[0][k].first = items[1][k].first = prefix[k];
-    }
-    std::vector<uint32_t> unit_item(prefix[K]);
-    for (int k = 0; k < K; ++k)
-        for (uint32_t u = prefix[k]; u < prefix[k + 1]; ++u) unit_item[u] = uint32_t(k);
-    b->total = prefix[K];
-    b->ctrl_len = ctrl_word(kCtrlSlots);
-    for (int p = 0; p < 3; ++p) b->qs.pstart[p] = prefix[p * ngrids];
-    b->qs.pstart[3] = prefix[K];
-    b->qs.xcd_queues = 0;
-    b->qs.chunk = 1;
-    int st = A2AFFT_OK;
-    for (int d = 0; d < 2 && st == A2AFFT_OK; ++d) {
-        st = hip_check(int(hipMalloc(&b->d_items[d], sizeof(BatchItem) * K)), "batch alloc");
-        if (st == A2AFFT_OK)
-            st = hip_check(int(hipMemcpy(b->d_items[d], items[d].data(), sizeof(BatchItem) * K,
-                                         hipMemcpyHostToDevice)), "batch copy");
-    }
+    int st = batch_tables(b, 0, 0);
     if (st == A2AFFT_OK)
-        st = hip_check(int(hipMalloc(&b->d_unit_item, sizeof(uint32_t) * unit_item.size())), "batch alloc");
-    if (st == A2AFFT_OK)
-        st = hip_check(int(hipMemcpy(b->d_unit_item, unit_item.data(), sizeof(uint32_t) * unit_item.size(),
-                                     hipMemcpyHostToDevice)), "batch copy");
-    if (st == A2AFFT_OK) st = hip_check(int(hipMalloc(&b->d_ctrl, sizeof(uint32_t) * b->ctrl_len)), "batch alloc");
+        st = hip_check(int(hipMalloc(&b->d_ctrl, sizeof(uint32_t) * size_t(ctrl_word(kCtrlSlots)))), "batch alloc");
     int dev = 0, cus = 0, occ = 0;
     if (st == A2AFFT_OK) st = hip_check(int(hipGetDevice(&dev)), "batch device");
     if (st == A2AFFT_OK)
@@ -403,14 +498,14 @@ extern "C" int a2afft_batch_offsets(a2afft_batch b, int64_t* offsets) {
 }
 
 static int batch_launch(a2afft_batch_s* b, void* data, int dir, hipStream_t s) {
-    hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(1024), 0, s, b->d_ctrl, b->ctrl_len);
+    hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(1024), 0, s, b->d_ctrl, kCtrlSlots);
     A2A_TRY(hip_check(int(hipGetLastError()), "batch reset launch"));
     if (dir < 0)
         hipLaunchKernelGGL(batch_kernel<-1>, dim3(b->grid), dim3(kBatchThreads), 0, s, (float2*)data, b->d_items[0],
-                           b->d_unit_item, b->G, b->qs, b->d_ctrl);
+                           b->d_unit_item, b->d_phases, b->qs, b->d_ctrl);
     else
         hipLaunchKernelGGL(batch_kernel<1>, dim3(b->grid), dim3(kBatchThreads), 0, s, (float2*)data, b->d_items[1],
-                           b->d_unit_item, b->G, b->qs, b->d_ctrl);
+                           b->d_unit_item, b->d_phases, b->qs, b->d_ctrl);
     return hip_check(int(hipGetLastError()), "batch launch");
 }
 
@@ -450,14 +545,15 @@ extern "C" int a2afft_batch_graph_launch(a2afft_batch b, void* data, int directi
 extern "C" int a2afft_batch_status(a2afft_batch b, void* stream) {
     if (!b) return set_error(A2AFFT_E_INVALID_ARG, "batch_status: null");
     A2A_TRY(hip_check(int(hipStreamSynchronize((hipStream_t)stream)), "batch sync"));
-    std::vector<uint32_t> h(size_t(b->ctrl_len));
+    const int nph = b->qs.nphases;
+    std::vector<uint32_t> h(size_t(ctrl_word(kCtrlDone + nph)));
     A2A_TRY(hip_check(int(hipMemcpy(h.data(), b->d_ctrl, sizeof(uint32_t) * h.size(), hipMemcpyDeviceToHost)),
                       "batch status copy"));
     if (h[ctrl_word(kCtrlErr)]) return set_error(A2AFFT_E_HIP, "batch: a dependency wait timed out");
-    for (int p = 0; p < 3; ++p) {
-        const uint32_t need = b->qs.pstart[p + 1] - b->qs.pstart[p];
-        const uint32_t done = h[ctrl_word(kCtrlDone + p)];
-        if (done != need) return set_error(A2AFFT_E_HIP, "batch: pass %d finished %u of %u units", p, done, need);
+    for (int q = 0; q < nph; ++q) {
+        const uint32_t need = b->pstart[size_t(q) + 1] - b->pstart[size_t(q)];
+        const uint32_t done = h[ctrl_word(kCtrlDone + q)];
+        if (done != need) return set_error(A2AFFT_E_HIP, "batch: phase %d finished %u of %u units", q, done, need);
     }
     return A2AFFT_OK;
 }
@@ -471,11 +567,17 @@ extern "C" int a2afft_batch_set_schedule(a2afft_batch b, int workgroups, int xcd
     b->grid = workgroups == 0 ? b->max_grid : std::min(workgroups, b->max_grid);
     b->qs.xcd_queues = xcd_queues ? 1 : 0;
     b->qs.chunk = uint32_t(chunk);
-    for (int d = 0; d < 2; ++d) {
-        if (b->graph[d]) (void)hipGraphExecDestroy(b->graph[d]);
-        b->graph[d] = nullptr;
-    }
+    drop_graphs(b);
     return A2AFFT_OK;
+}
+
+// Groups of consecutive grids of at most group_points points each (0 = all
+// grids in one group: every pass of every grid before the next pass), and
+// the phase order (1 = skewed along the diagonals).
+extern "C" int a2afft_batch_set_grouping(a2afft_batch b, int64_t group_points, int skewed) {
+    if (!b || group_points < 0 || (skewed != 0 && skewed != 1))
+        return set_error(A2AFFT_E_INVALID_ARG, "batch_set_grouping: bad arguments");
+    return batch_tables(b, group_points, skewed);
 }
 
 extern "C" int a2afft_batch_info(a2afft_batch b, int64_t* total_units, int* workgroups, int* max_workgroups) {

@@ -207,6 +207,21 @@ def run_c5(args, torch, a2afft):
                 log(f"[bench] batch workgroups {w} xcd_queues {int(xq)} chunk {chunk}: {t:.4f} ms")
                 if t < sched_t:
                     sched, sched_t = (w, xq, chunk), t
+    # grouping: pass-major (one group), or groups of about one / two of the
+    # largest grids with skewed phases, each with the queue layouts and takes
+    nmax = max(s[0] * s[1] * s[2] for s in shapes)
+    group = (0, False)
+    for gp, sk in ((nmax, True), (2 * nmax, True)):
+        b.set_grouping(gp, sk)
+        for xq in (False, True):
+            for chunk in (1, 2):
+                b.set_schedule(sched[0], xq, chunk)
+                t = _events_ms(torch, lambda: b.forward(y), 3)
+                log(f"[bench] batch groups of {gp} points skewed {int(sk)} xcd_queues {int(xq)} chunk {chunk}: "
+                    f"{t:.4f} ms")
+                if t < sched_t:
+                    sched, sched_t, group = (sched[0], xq, chunk), t, (gp, sk)
+    b.set_grouping(*group)
     b.set_schedule(*sched)
     t_eager = _events_ms(torch, lambda: b.forward(y), args.steps)
     t_graph = _events_ms(torch, lambda: b.graph_launch(y, -1), args.steps)
@@ -232,7 +247,8 @@ def run_c5(args, torch, a2afft):
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "64 complex64 grids drawn with seed 4 from 32^3, 64^3, 128^3, 256^3, 128x256x64, "
                                "256x64x512; U[-1,1) values", "name": "c5", "counts": counts,
-                   "points": npts, "schedule": {"workgroups": sched[0], "xcd_queues": sched[1], "chunk": sched[2]}},
+                   "points": npts, "schedule": {"workgroups": sched[0], "xcd_queues": sched[1], "chunk": sched[2],
+                                                 "group_points": group[0], "skewed": group[1]}},
         "ms": {"persistent_eager": round(t_eager, 4), "persistent_graph": round(t_graph, 4),
                "separate_launches_192": round(t_sep, 4)},
         "rel_l2_vs_fp64": {"max": max(errs), "mean": sum(errs) / len(errs)},

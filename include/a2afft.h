@@ -207,6 +207,15 @@ int a2afft_batch_status(a2afft_batch batch, void* stream);
  * range per XCD, 0 = one global queue, the default); units handed out per
  * take of the queue (1..64).  Results are identical for every schedule. */
 int a2afft_batch_set_schedule(a2afft_batch batch, int workgroups, int xcd_queues, int chunk);
+/* Order of the work: grids are packed, in order, into groups of at most
+ * group_points points (0 = one group of all grids, the default: each pass of
+ * every grid before the next pass; a grid larger than group_points forms a
+ * group of its own).  A group's next pass waits only for the same group's
+ * previous pass, so a group that fits the device's memory-side cache re-reads
+ * its data from there.  skewed = 1 interleaves the (group, pass) phases along
+ * diagonals, so that another group's work runs while a pass completes.  At
+ * most 128 groups.  Results are identical for every order. */
+int a2afft_batch_set_grouping(a2afft_batch batch, int64_t group_points, int skewed);
 int a2afft_batch_info(a2afft_batch batch, int64_t* total_units, int* workgroups, int* max_workgroups);
 int a2afft_batch_destroy(a2afft_batch batch);
 

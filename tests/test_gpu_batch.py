@@ -50,6 +50,55 @@ def test_batch_matches_grid_plans_and_numpy(a2afft_mod, cuda, workgroups, xcdq, 
     assert np.linalg.norm(back - x0) / np.linalg.norm(x0) < 1e-6
 
 
+@pytest.mark.parametrize("group_points,skewed,xcdq,chunk", [(1, 0, 0, 1), (1, 1, 1, 1), (100000, 1, 0, 2),
+                                                            (300000, 0, 1, 1), (2500000, 1, 1, 3), (1 << 40, 1, 0, 1)])
+def test_batch_grouped_orders_match_default(a2afft_mod, cuda, group_points, skewed, xcdq, chunk):
+    """Group-by-group and skewed phase orders compute the bits of the default
+    pass-major order, eagerly and through the graph."""
+    import torch
+    b = a2afft_mod.Batch(SHAPES)
+    x = _fill(a2afft_mod, torch, b, 6)
+    ref = x.clone()
+    b.forward(ref)
+    assert b.status() == 0
+    ref = ref.cpu().numpy()
+    b.set_grouping(group_points, bool(skewed))
+    b.set_schedule(0, bool(xcdq), chunk)
+    y = x.clone()
+    b.forward(y)
+    assert b.status() == 0
+    assert np.array_equal(_bits(y.cpu().numpy()), _bits(ref))
+    for rep in range(2):
+        y.copy_(x)
+        torch.cuda.synchronize()
+        b.graph_launch(y, -1)
+        assert b.status() == 0
+        assert np.array_equal(_bits(y.cpu().numpy()), _bits(ref)), rep
+    b.inverse(y)
+    assert b.status() == 0
+    x0 = x.cpu().numpy()
+    assert np.linalg.norm(y.cpu().numpy() - x0) / np.linalg.norm(x0) < 1e-6
+
+
+def test_batch_grouping_limits(a2afft_mod, cuda):
+    """At most 128 groups; a rejected grouping leaves the previous one in force."""
+    import torch
+    shapes = [(16, 16, 16)] * 130
+    b = a2afft_mod.Batch(shapes)
+    x = _fill(a2afft_mod, torch, b, 8)
+    ref = x.clone()
+    b.forward(ref)
+    with pytest.raises(a2afft_mod.A2AFFTError):
+        b.set_grouping(1, False)
+    with pytest.raises(a2afft_mod.A2AFFTError):
+        b.set_grouping(-1, False)
+    b.set_grouping(2 * 16 ** 3, True)   # 65 groups
+    y = x.clone()
+    b.forward(y)
+    assert b.status() == 0
+    assert np.array_equal(_bits(y.cpu().numpy()), _bits(ref.cpu().numpy()))
+
+
 def test_batch_graph_replays_match_eager(a2afft_mod, cuda):
     import torch
     b = a2afft_mod.Batch(SHAPES)
