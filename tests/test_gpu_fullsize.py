@@ -1,0 +1,192 @@
+"""GPU tests at the full sizes of the benchmark workloads.
+
+The oracle cannot transform these grids in a test's time, so the checks are
+size-independent: the FP32 result against the library's FP64 result (whose
+kernels the oracle pins at smaller sizes), and the FP64 result against closed
+forms computed directly from the input -- Parseval's identity, the DC bin and
+one off-axis bin as explicit sums in FP64, numpy's fftn where it runs in a few
+seconds -- plus bit equality between strategies and launch layouts that must
+compute the same bits, and round trips."""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _same_bits(a, b):
+    import torch
+    if a.dtype == torch.complex128:
+        return bool(torch.equal(a.view(torch.float64), b.view(torch.float64)))
+    return bool(torch.equal(a.view(torch.float32), b.view(torch.float32)))
+
+
+def _norm2(t):
+    import torch
+    return float(torch.sum(torch.abs(t.to(torch.complex128)) ** 2))
+
+
+def _bin(torch, g, k):
+    """sum_n g[n] exp(-2 pi i k.n / N) in FP64, axis by axis (g: global grid, complex128)."""
+    out = g
+    for axis in (2, 1, 0):
+        n = g.shape[axis]
+        ph = torch.exp(torch.arange(n, device=g.device, dtype=torch.float64) * (-2j * math.pi * k[axis] / n))
+        shape = [1] * out.dim()
+        shape[-1] = n
+        out = (out * ph.view(shape)).sum(-1)
+    return complex(out.item())
+
+
+def test_c3_512cube_c64_dims144(a2afft_mod, cuda):
+    """512^3 complex64 over 16 virtual ranks, dims [1,4,4]: FP32 within 1e-6 of
+    FP64 (and the split variants within their bounds), FP64 against Parseval and
+    two explicit bins, fused == staged bit for bit, FP32 round trip."""
+    import torch
+    from a2afft import inputs
+    Ng, P, dims = 512, 16, (1, 4, 4)
+    n3 = Ng ** 3
+    dist = a2afft_mod.Distribution(Ng, P, dims=dims, dtype="c64")
+    x = dist.empty()
+    a2afft_mod.fill_random(x, 2, "uniform")
+    x128 = x.to(torch.complex128)
+    ref = x128.clone()
+    dist64 = a2afft_mod.Distribution(Ng, P, dims=dims, dtype="c128")
+    a2afft_mod.Dfft(dist64, "fp64").forward(ref)
+    # closed forms on the FP64 result
+    par = _norm2(ref) / (n3 * _norm2(x128))
+    assert abs(par - 1.0) < 1e-12, par
+    g = inputs.blocks_to_global(x128, dims, (Ng,) * 3)
+    yg = inputs.blocks_to_global(ref, dims, (Ng,) * 3)
+    scale = math.sqrt(n3)
+    for k in ((0, 0, 0), (3, 200, 511)):
+        want = _bin(torch, g, k)
+        got = complex(yg[k].item())
+        assert abs(got - want) < 1e-10 * scale, (k, got, want)
+    del g, yg
+    # FP32 and split variants against FP64
+    y = x.clone()
+    a2afft_mod.Dfft(dist, "fp32").forward(y)
+    rel, _ = a2afft_mod.error_metrics(y, ref)
+    assert rel <= 1e-6, rel
+    for prec, tol in (("split_bf16", 1e-5), ("split_fp16", 1e-6)):
+        s = x.clone()
+        a2afft_mod.Dfft(dist, prec).forward(s)
+        r, _ = a2afft_mod.error_metrics(s, ref)
+        assert r <= tol, (prec, r)
+        del s
+    del ref
+    # the reference's stage order computes the same bits
+    st = x.clone()
+    staged = a2afft_mod.Dfft(dist, "fp32", exec="staged")
+    staged.make_plans(dist.empty())
+    staged.forward(st)
+    assert _same_bits(st, y)
+    del st, staged
+    # round trip
+    a2afft_mod.Dfft(dist, "fp32").inverse(y)
+    rel, _ = a2afft_mod.error_metrics(y, x128)
+    assert rel <= 1e-6, rel
+
+
+@pytest.mark.parametrize("dims", [(1, 2, 4), (2, 2, 2)])
+def test_c2_256cube_c128(a2afft_mod, cuda, dims):
+    """256^3 complex128 over 8 virtual ranks: forward against numpy's fftn of
+    the stitched grid (rel-L2 <= 1e-14), fused == staged bit for bit, inverse
+    (with 1/N) back to the input."""
+    import torch
+    from a2afft import inputs
+    Ng, P = 256, 8
+    dist = a2afft_mod.Distribution(Ng, P, dims=dims, dtype="c128")
+    x = dist.empty()
+    a2afft_mod.fill_random(x, 1, "normal")
+    y = x.clone()
+    plan = a2afft_mod.Dfft(dist, "fp64")
+    plan.forward(y)
+    ref = np.fft.fftn(inputs.blocks_to_global(x.cpu().numpy(), dims, (Ng,) * 3))
+    got = inputs.blocks_to_global(y.cpu().numpy(), dims, (Ng,) * 3)
+    rel = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+    assert rel <= 1e-14, rel
+    del ref, got
+    st = x.clone()
+    staged = a2afft_mod.Dfft(dist, "fp64", exec="staged")
+    staged.make_plans(dist.empty())
+    staged.forward(st)
+    assert _same_bits(st, y)
+    plan.inverse(y)
+    rel, _ = a2afft_mod.error_metrics(y, x)
+    assert rel <= 1e-14, rel
+
+
+def test_c3b_2048_rank_share(a2afft_mod, cuda):
+    """One rank's 2048x256x256 complex64 share of 2048^3 over 64 ranks, dims
+    [1,8,8]: the batched 2048-point FFT on the pencils the exchange delivers,
+    against the FP64 kernels (rel-L2 <= 1e-6) and numpy on sampled pencils;
+    the strided x pass on the block gives the same bits as the pencil FFT."""
+    import torch
+    shape, P, dims = (2048, 256, 256), 64, (1, 8, 8)
+    nloc = shape[0] * shape[1] * shape[2]
+    block = torch.empty(nloc, dtype=torch.complex64, device="cuda")
+    a2afft_mod.fill_random(block, 2, "uniform")
+    pencils = torch.empty_like(block)
+    a2afft_mod.launch_remap(6, block, pencils, P, nloc, shape, dims, nbuffers=1)   # [lz][ly][lx]
+    ref = pencils.to(torch.complex128)
+    a2afft_mod.forward_1d_fft(ref, 2048)
+    y = pencils.clone()
+    a2afft_mod.forward_1d_fft(y, 2048)
+    rel, _ = a2afft_mod.error_metrics(y, ref)
+    assert rel <= 1e-6, rel
+    rows = np.random.default_rng(0).choice(nloc // 2048, 64, replace=False)
+    src = pencils.view(-1, 2048)[torch.from_numpy(rows).cuda()].cpu().numpy().astype(np.complex128)
+    got = ref.view(-1, 2048)[torch.from_numpy(rows).cuda()].cpu().numpy()
+    want = np.fft.fft(src, axis=1)
+    assert np.linalg.norm(got - want) / np.linalg.norm(want) < 1e-14
+    del ref
+    # the fused x pass on the block layout, transposed the same way, is the pencil FFT
+    plan = a2afft_mod.GridDfft(shape, "fp32")
+    wb = block.clone()
+    a2afft_mod.execute_pass(plan, wb, 1, -1)
+    back = torch.empty_like(wb)
+    a2afft_mod.launch_remap(6, wb, back, P, nloc, shape, dims, nbuffers=1)
+    assert _same_bits(back, y)
+
+
+def _c5_shapes():
+    from a2afft import inputs
+    cand = [(32, 32, 32), (64, 64, 64), (128, 128, 128), (256, 256, 256), (128, 256, 64), (256, 64, 512)]
+    u = (inputs.uniform(4, 64).real + 1.0) / 2.0
+    return [cand[min(5, int(v * 6))] for v in u]
+
+
+def test_c5_batch_of_64_grids(a2afft_mod, cuda):
+    """The 64-grid batch (shapes drawn with seed 4): every grid equals its
+    single-grid plan bit for bit, for the default and a grouped, skewed order;
+    one grid of each shape within 6e-7 of FP64."""
+    import torch
+    shapes = _c5_shapes()
+    b = a2afft_mod.Batch(shapes)
+    x = b.empty()
+    a2afft_mod.fill_random(x, 4, "uniform")
+    for group_points, skewed in ((0, False), (1 << 25, True)):
+        b.set_grouping(group_points, skewed)
+        y = x.clone()
+        b.forward(y)
+        assert b.status() == 0
+        plans = {}
+        checked = set()
+        for i, s in enumerate(shapes):
+            lo, hi = b.offsets[i], b.offsets[i + 1]
+            if s not in plans:
+                plans[s] = a2afft_mod.GridDfft(s, "fp32")
+            plan = plans[s]
+            g = x[lo:hi].clone()
+            plan.forward(g)
+            assert _same_bits(y[lo:hi], g), (i, s, group_points)
+            if s not in checked:
+                checked.add(s)
+                r = x[lo:hi].to(torch.complex128)
+                a2afft_mod.GridDfft(s, "fp64").forward(r)
+                rel, _ = a2afft_mod.error_metrics(y[lo:hi], r)
+                assert rel < 6e-7, (s, rel)
+        del y
