@@ -30,8 +30,8 @@
 // after at most a wait for that earlier phase (it may hold one taken unit of
 // a later phase meanwhile, which nobody waits on yet), so by induction over q
 // every phase completes.  The wait is still bounded, and a bound that trips
-// sets an error flag, after which every other wait gives up at once, instead
-// of hanging.
+// sets an error flag, after which every other wait gives up within 16 polls,
+// instead of hanging.
 #include "fft.h"
 #include "stockham.h"
 
@@ -207,7 +207,8 @@ __device__ __forceinline__ void wait_phase(uint32_t* ctrl, int q, uint32_t need)
 }
 
 template <int DIR>
-// At most 168 VGPRs (3 wavefronts per SIMD): three workgroups per CU measured fastest.
+// At least 3 wavefronts per SIMD (at most 168 VGPRs); the kernel needs 103, so LDS
+// (39.9 KB per workgroup) is what limits a CU to four workgroups.
 __global__ __launch_bounds__(kBatchThreads, 3) void batch_kernel(float2* __restrict__ data,
                                                               const BatchItem* __restrict__ items,
                                                               const uint32_t* __restrict__ unit_item,
@@ -253,9 +254,16 @@ __global__ __launch_bounds__(kBatchThreads, 3) void batch_kernel(float2* __restr
         uint32_t ticket;
         if (tid == 0) ticket = claim_issue(c, pt, myq, nq, ctrl);
         const BatchItem& it = items[k];
+        // The thread index as the compiler must re-read it every unit: left
+        // visible, the per-thread LDS addresses of every shape are hoisted out
+        // of the loop and kept live across all 18 shapes (168 VGPRs with
+        // spills, whose reloads wait for every outstanding global access)
+        // instead of 103 VGPRs recomputed per unit.
+        int utid = tid;
+        asm volatile("" : "+v"(utid));
         // The ticket is examined once the unit's loads are consumed and before
         // its stores are issued, so thread 0 never waits for its own stores.
-        run_unit<DIR>(it.shape, data + it.base, it.g, it.tw, unit - it.first, lds, tid, [&]() {
+        run_unit<DIR>(it.shape, data + it.base, it.g, it.tw, unit - it.first, lds, utid, [&]() {
             if (tid == 0) s_unit[it_no & 1] = claim_finish(c, ticket, pt, myq, nq, ctrl);
         });
         ++count;

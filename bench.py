@@ -201,7 +201,8 @@ def run_c5(args, torch, a2afft):
     sched, sched_t = (0, False, 1), 1e30
     for xq in (False, True):
         for chunk in (1, 2, 4):
-            for w in sorted({info["max_workgroups"], info["max_workgroups"] // 2, 512, 256}):
+            for w in sorted({info["max_workgroups"], 3 * info["max_workgroups"] // 4, info["max_workgroups"] // 2,
+                             512, 256}):
                 b.set_schedule(w, xq, chunk)
                 t = _events_ms(torch, lambda: b.forward(y), 3)
                 log(f"[bench] batch workgroups {w} xcd_queues {int(xq)} chunk {chunk}: {t:.4f} ms")
