@@ -35,6 +35,8 @@ int register_kernels_f32(KEntry* out) {
     add_shape<float, 256, 32, 32, true>(out, &n);
     add_shape<float, 2048, 16, 4, false>(out, &n);   // 512 threads, 64 KiB LDS
     add_shape<float, 2048, 32, 4, false>(out, &n);   // 256 threads, 64 KiB LDS
+    add_extra_flags<float, 2048, 32, 4, false>(out, &n);
+    add_extra_flags<float, 2048, 16, 2, false>(out, &n);   // the contiguous default shape
     add_shape<float, 2048, 16, 1, false>(out, &n);   // 128 threads, 16 KiB LDS
     add_shape<float, 2048, 32, 8, true>(out, &n);    // 512 threads, 128 KiB LDS, 64 B rows
     return n;
