@@ -283,6 +283,21 @@ class Batch:
         return L.check(L.lib().a2afft_batch_set_grouping(self._h, int(group_points), 1 if skewed else 0),
                        "batch_set_grouping")
 
+    def autotune(self, work, iters=3):
+        """Time schedules and phase orders on `work` (overwritten) and keep the fastest."""
+        self._check(work)
+        return L.check(L.lib().a2afft_batch_autotune(self._h, _ptr(work), iters, _stream()), "batch_autotune")
+
+    def describe(self):
+        buf = ctypes.create_string_buffer(16384)
+        L.check(L.lib().a2afft_batch_describe(self._h, buf, len(buf)), "batch_describe")
+        return buf.value.decode()
+
+    def schedule(self):
+        """The current schedule: workgroups, xcd_queues, chunk, group_points, skewed (and sizes)."""
+        first = self.describe().splitlines()[0]
+        return {k: int(v) for k, v in (kv.split("=") for kv in first.split())}
+
     def info(self):
         u, w, m = ctypes.c_int64(), ctypes.c_int(), ctypes.c_int()
         L.check(L.lib().a2afft_batch_info(self._h, ctypes.byref(u), ctypes.byref(w), ctypes.byref(m)), "batch_info")

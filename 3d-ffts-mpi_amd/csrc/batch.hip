@@ -8,9 +8,10 @@
 // phase per (group, pass).  Units are numbered phase by phase, in an order
 // the host chooses: group-major (z, x, y of group 0, then of group 1, ...) or
 // skewed along diagonals, so that the phase between a pass and the next pass
-// of the same group belongs to another group.  Groups small enough to stay in
-// the device's memory-side cache between their passes read back what their
-// previous pass wrote from that cache instead of HBM.  Workgroups take units
+// of the same group belongs to another group.  A group's next pass then reads
+// data its previous pass wrote shortly before, which may still sit in the
+// device's memory-side cache (measured: 8 grids of 256^3 take 1.14 ms one grid
+// per group, skewed, against 1.29 ms pass-major).  Workgroups take units
 // in queue order from atomic counters; a table maps each unit to its
 // (phase, grid) item.  The take of the next unit is issued before the current
 // unit is computed and examined after it, so the latency of the device-scope
@@ -36,7 +37,9 @@
 #include "stockham.h"
 
 #include <algorithm>
+#include <cstdio>
 #include <new>
+#include <string>
 #include <vector>
 
 namespace a2a {
@@ -300,6 +303,7 @@ struct a2afft_batch_s {
     hipStream_t cap_stream = nullptr;
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     void* graph_data[2] = {nullptr, nullptr};
+    std::string report;               // the last autotune's timings
 };
 
 static void drop_graphs(a2afft_batch_s* b) {
@@ -586,6 +590,120 @@ extern "C" int a2afft_batch_set_grouping(a2afft_batch b, int64_t group_points, i
     if (!b || group_points < 0 || (skewed != 0 && skewed != 1))
         return set_error(A2AFFT_E_INVALID_ARG, "batch_set_grouping: bad arguments");
     return batch_tables(b, group_points, skewed);
+}
+
+namespace {
+
+struct EventPair {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int create() {
+        A2A_TRY(hip_check(int(hipEventCreate(&e0)), "batch autotune event"));
+        return hip_check(int(hipEventCreate(&e1)), "batch autotune event");
+    }
+    ~EventPair() {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+    }
+};
+
+// Mean ms of `iters` forward executions with the current schedule (after one
+// untimed run); fails if a dependency wait hit its bound.
+int batch_time(a2afft_batch_s* b, void* data, int iters, hipStream_t s, const EventPair& ev, float* ms) {
+    A2A_TRY(batch_launch(b, data, -1, s));
+    A2A_TRY(hip_check(int(hipEventRecord(ev.e0, s)), "batch autotune record"));
+    for (int it = 0; it < iters; ++it) A2A_TRY(batch_launch(b, data, -1, s));
+    A2A_TRY(hip_check(int(hipEventRecord(ev.e1, s)), "batch autotune record"));
+    A2A_TRY(hip_check(int(hipEventSynchronize(ev.e1)), "batch autotune sync"));
+    A2A_TRY(hip_check(int(hipEventElapsedTime(ms, ev.e0, ev.e1)), "batch autotune elapsed"));
+    *ms /= float(iters);
+    uint32_t err = 0;
+    A2A_TRY(hip_check(int(hipMemcpy(&err, b->d_ctrl + ctrl_word(kCtrlErr), sizeof(err), hipMemcpyDeviceToHost)),
+                      "batch autotune status"));
+    return err ? set_error(A2AFFT_E_HIP, "batch autotune: a dependency wait timed out") : A2AFFT_OK;
+}
+
+struct Schedule {
+    int grid, xcd_queues;
+    uint32_t chunk;
+    int64_t group_points;
+    int skewed;
+};
+
+}  // namespace
+
+// Times forward executions over schedules and phase orders and keeps the
+// fastest: workgroup counts (all resident slots, 3/4, 1/2, 512, 256) x queue
+// layouts x takes of 1, 2, 4 units in pass-major order, then groups of about
+// one and two of the largest grids, skewed, at the best workgroup count.
+// Every candidate computes the same bits; `data` is overwritten.
+extern "C" int a2afft_batch_autotune(a2afft_batch b, void* data, int iters, void* stream) {
+    if (!b || !data || iters <= 0) return set_error(A2AFFT_E_INVALID_ARG, "batch_autotune: bad arguments");
+    hipStream_t s = (hipStream_t)stream;
+    EventPair ev;
+    A2A_TRY(ev.create());
+    const Schedule keep{b->grid, b->qs.xcd_queues, b->qs.chunk, b->group_points, b->skewed};
+    Schedule best = keep;
+    float best_ms = 1e30f;
+    std::string rep;
+    char line[160];
+    auto apply = [&](const Schedule& c) -> int {
+        if (c.group_points != b->group_points || c.skewed != b->skewed)
+            A2A_TRY(batch_tables(b, c.group_points, c.skewed));
+        b->grid = c.grid;
+        b->qs.xcd_queues = c.xcd_queues;
+        b->qs.chunk = c.chunk;
+        return A2AFFT_OK;
+    };
+    auto trial = [&](const Schedule& c) -> int {
+        A2A_TRY(apply(c));
+        float ms = 0.f;
+        A2A_TRY(batch_time(b, data, iters, s, ev, &ms));
+        snprintf(line, sizeof(line), "workgroups=%d xcd_queues=%d chunk=%u group_points=%lld skewed=%d %.4f ms\n",
+                 c.grid, c.xcd_queues, c.chunk, (long long)c.group_points, c.skewed, ms);
+        rep += line;
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = c;
+        }
+        return A2AFFT_OK;
+    };
+    std::vector<int> ws{b->max_grid, 3 * b->max_grid / 4, b->max_grid / 2, 512, 256};
+    std::sort(ws.begin(), ws.end());
+    ws.erase(std::unique(ws.begin(), ws.end()), ws.end());
+    int st = A2AFFT_OK;
+    for (int w : ws) {
+        if (w <= 0 || w > b->max_grid) continue;
+        for (int xq = 0; xq < 2 && st >= 0; ++xq)
+            for (uint32_t chunk : {1u, 2u, 4u})
+                if (st >= 0) st = trial(Schedule{w, xq, chunk, 0, 0});
+    }
+    int64_t nmax = 0;
+    for (int g = 0; g < b->G; ++g) nmax = std::max(nmax, b->offsets[size_t(g) + 1] - b->offsets[size_t(g)]);
+    const int w = best.grid;
+    for (int64_t gp : {nmax, 2 * nmax}) {
+        if (st < 0) break;
+        if (batch_tables(b, gp, 1) != A2AFFT_OK) continue;   // too many groups: not a candidate
+        for (int xq = 0; xq < 2 && st >= 0; ++xq)
+            for (uint32_t chunk : {1u, 2u})
+                if (st >= 0) st = trial(Schedule{w, xq, chunk, gp, 1});
+    }
+    const int st2 = apply(st >= 0 ? best : keep);
+    drop_graphs(b);
+    if (st >= 0) b->report = rep;
+    return st < 0 ? st : st2;
+}
+
+// The current schedule on the first line ("key=value" pairs), then the last
+// autotune's timings.
+extern "C" int a2afft_batch_describe(a2afft_batch b, char* buf, int len) {
+    if (!b || !buf || len <= 0) return set_error(A2AFFT_E_INVALID_ARG, "batch_describe: bad arguments");
+    const int n = snprintf(buf, size_t(len),
+                           "grids=%d units=%u phases=%d workgroups=%d max_workgroups=%d xcd_queues=%d chunk=%u "
+                           "group_points=%lld skewed=%d\n",
+                           b->G, b->total, b->qs.nphases, b->grid, b->max_grid, b->qs.xcd_queues, b->qs.chunk,
+                           (long long)b->group_points, b->skewed);
+    if (n >= 0 && n < len) snprintf(buf + n, size_t(len - n), "%s", b->report.c_str());
+    return A2AFFT_OK;
 }
 
 extern "C" int a2afft_batch_info(a2afft_batch b, int64_t* total_units, int* workgroups, int* max_workgroups) {

@@ -196,34 +196,12 @@ def run_c5(args, torch, a2afft):
         a2afft.GridDfft(s, "fp64").forward(ref)
         errs.append(a2afft.error_metrics(y[lo:hi], ref.reshape(-1))[0])
         del ref
-    # workgroup-count tuning (every setting computes the same bits)
-    info = b.info()
-    sched, sched_t = (0, False, 1), 1e30
-    for xq in (False, True):
-        for chunk in (1, 2, 4):
-            for w in sorted({info["max_workgroups"], 3 * info["max_workgroups"] // 4, info["max_workgroups"] // 2,
-                             512, 256}):
-                b.set_schedule(w, xq, chunk)
-                t = _events_ms(torch, lambda: b.forward(y), 3)
-                log(f"[bench] batch workgroups {w} xcd_queues {int(xq)} chunk {chunk}: {t:.4f} ms")
-                if t < sched_t:
-                    sched, sched_t = (w, xq, chunk), t
-    # grouping: pass-major (one group), or groups of about one / two of the
-    # largest grids with skewed phases, each with the queue layouts and takes
-    nmax = max(s[0] * s[1] * s[2] for s in shapes)
-    group = (0, False)
-    for gp, sk in ((nmax, True), (2 * nmax, True)):
-        b.set_grouping(gp, sk)
-        for xq in (False, True):
-            for chunk in (1, 2):
-                b.set_schedule(sched[0], xq, chunk)
-                t = _events_ms(torch, lambda: b.forward(y), 3)
-                log(f"[bench] batch groups of {gp} points skewed {int(sk)} xcd_queues {int(xq)} chunk {chunk}: "
-                    f"{t:.4f} ms")
-                if t < sched_t:
-                    sched, sched_t, group = (sched[0], xq, chunk), t, (gp, sk)
-    b.set_grouping(*group)
-    b.set_schedule(*sched)
+    # the library's autotune: workgroup counts, queue layouts, takes and phase
+    # orders (every setting computes the same bits)
+    b.autotune(y, iters=3)
+    for line in b.describe().splitlines():
+        log(f"[bench] batch {line}")
+    sched = b.schedule()
     t_eager = _events_ms(torch, lambda: b.forward(y), args.steps)
     t_graph = _events_ms(torch, lambda: b.graph_launch(y, -1), args.steps)
     assert b.status() == 0
@@ -248,8 +226,8 @@ def run_c5(args, torch, a2afft):
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "64 complex64 grids drawn with seed 4 from 32^3, 64^3, 128^3, 256^3, 128x256x64, "
                                "256x64x512; U[-1,1) values", "name": "c5", "counts": counts,
-                   "points": npts, "schedule": {"workgroups": sched[0], "xcd_queues": sched[1], "chunk": sched[2],
-                                                 "group_points": group[0], "skewed": group[1]}},
+                   "points": npts, "schedule": {k: sched[k] for k in ("workgroups", "xcd_queues", "chunk", "group_points",
+                                                                  "skewed")}},
         "ms": {"persistent_eager": round(t_eager, 4), "persistent_graph": round(t_graph, 4),
                "separate_launches_192": round(t_sep, 4)},
         "rel_l2_vs_fp64": {"max": max(errs), "mean": sum(errs) / len(errs)},

@@ -216,6 +216,14 @@ int a2afft_batch_set_schedule(a2afft_batch batch, int workgroups, int xcd_queues
  * diagonals, so that another group's work runs while a pass completes.  At
  * most 128 groups.  Results are identical for every order. */
 int a2afft_batch_set_grouping(a2afft_batch batch, int64_t group_points, int skewed);
+/* Times `iters` forward executions per candidate on `stream` and keeps the
+ * fastest schedule and order: workgroup counts x queue layouts x takes of
+ * 1, 2, 4 units (pass-major), then skewed groups of about one and two of the
+ * largest grids.  d_data (the batch buffer) is overwritten.  Synchronises. */
+int a2afft_batch_autotune(a2afft_batch batch, void* d_data, int iters, void* stream);
+/* First line: the current schedule as key=value pairs; then the last
+ * autotune's timings. */
+int a2afft_batch_describe(a2afft_batch batch, char* buf, int len);
 int a2afft_batch_info(a2afft_batch batch, int64_t* total_units, int* workgroups, int* max_workgroups);
 int a2afft_batch_destroy(a2afft_batch batch);
 

@@ -80,6 +80,30 @@ def test_batch_grouped_orders_match_default(a2afft_mod, cuda, group_points, skew
     assert np.linalg.norm(y.cpu().numpy() - x0) / np.linalg.norm(x0) < 1e-6
 
 
+def test_batch_autotune_keeps_bits(a2afft_mod, cuda):
+    """The library's autotune picks a schedule among equivalent ones: results
+    equal the default schedule's bit for bit, and describe reports it."""
+    import torch
+    b = a2afft_mod.Batch(SHAPES)
+    x = _fill(a2afft_mod, torch, b, 12)
+    ref = x.clone()
+    b.forward(ref)
+    work = b.empty()
+    a2afft_mod.fill_random(work, 1, "uniform")
+    b.autotune(work, iters=1)
+    sched = b.schedule()
+    assert 1 <= sched["workgroups"] <= sched["max_workgroups"] and sched["chunk"] in (1, 2, 4)
+    assert len(b.describe().splitlines()) > 20   # one timing line per candidate
+    y = x.clone()
+    b.forward(y)
+    assert b.status() == 0
+    assert np.array_equal(_bits(y.cpu().numpy()), _bits(ref.cpu().numpy()))
+    b.graph_launch(y, 1)
+    assert b.status() == 0
+    x0 = x.cpu().numpy()
+    assert np.linalg.norm(y.cpu().numpy() - x0) / np.linalg.norm(x0) < 1e-6
+
+
 def test_batch_grouping_limits(a2afft_mod, cuda):
     """At most 128 groups; a rejected grouping leaves the previous one in force."""
     import torch
