@@ -56,8 +56,9 @@ int fft_candidates(const AxisPass& p, a2afft_dtype dt, int N, KSel* out, int max
     return n;
 }
 
-// Measured preferences (MI355X autotune: 512 on 512^3 over dims [1,4,4], the
-// other lengths on single grids).  XCD-aware order and non-temporal access
+// Measured preferences (MI355X autotune: 512 on 512^3 over dims [1,4,4],
+// complex128 256 on 256^3 over dims [1,2,4], the other lengths on single
+// grids).  XCD-aware order and non-temporal access
 // help the 512 z and y passes; the far-strided x pass prefers z tiles
 // outermost; padded LDS helps the long contiguous lines (2048 over 256x256
 // lines: 0.384 ms padded, XCD-ordered and non-temporal, 0.474 ms plain).  The first entry that is built and fits the
@@ -77,7 +78,9 @@ static const Pref kPrefs[] = {
     {512, 1, 1, -1, {8, 16, KF_XCD | KF_NT}},
     {256, 0, 1, 1, {32, 32, KF_XCD | KF_PAD}},
     {256, 0, 1, -1, {16, 32, KF_XCD}},
-    {256, 1, 0, -1, {16, 16, KF_PAD}},
+    {256, 1, 0, -1, {8, 16, KF_XCD | KF_PAD}},
+    {256, 1, 1, 1, {8, 16, KF_XCD | KF_A2MAJ}},
+    {256, 1, 1, -1, {4, 16, KF_XCD | KF_PAD}},
     {2048, 0, 0, -1, {2, 16, KF_XCD | KF_PAD | KF_NT}},
     {4096, 0, 0, -1, {1, 16, KF_PAD}},
 };

@@ -15,6 +15,10 @@ int register_kernels_f32(KEntry* out) {
     add_defaults<float, 256>(out, &n);
     add_defaults<float, 512>(out, &n);
     add_extra_flags<float, 512, 32, 16, true>(out, &n);  // the strided default shape
+    add_variant<float, 512, 32, 16, true, KF_A2MAJ | KF_PAD>(out, &n);
+    add_variant<float, 512, 32, 16, true, KF_A2MAJ | KF_NT>(out, &n);
+    add_variant<float, 512, 32, 16, true, KF_A2MAJ | KF_PAD | KF_NT>(out, &n);
+    add_variant<float, 512, 32, 16, true, KF_XCD | KF_A2MAJ | KF_PAD | KF_NT>(out, &n);
     add_defaults<float, 1024>(out, &n);
     add_defaults<float, 2048>(out, &n);
     add_defaults<float, 4096>(out, &n);

@@ -20,6 +20,8 @@ int register_kernels_f64(KEntry* out) {
     add_defaults<double, 4096>(out, &n);
     add_mixed_all<double, 3, 5, 6, 9, 10, 12, 15, 18, 20, 24, 25, 27, 30>(out, &n);   // a line fits in registers
     // tuning candidates for the benchmark lengths
+    add_extra_flags<double, 256, 16, 8, true>(out, &n);    // the strided default shape
+    add_extra_flags<double, 256, 16, 16, false>(out, &n);  // the contiguous default shape
     add_shape<double, 256, 16, 16, true>(out, &n);   // 256 threads, 64 KiB LDS
     add_shape<double, 256, 16, 4, true>(out, &n);    // 64 threads, 16 KiB LDS
     add_shape<double, 256, 32, 16, true>(out, &n);   // 128 threads
