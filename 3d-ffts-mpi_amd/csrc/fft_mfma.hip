@@ -60,7 +60,7 @@ template <> struct MfTraits<_Float16> {
 constexpr int kMfThreads = 256;
 
 A2A_HD constexpr int mf_stages16(int N) { return N >= 4096 ? 3 : N >= 256 ? 2 : N >= 16 ? 1 : 0; }
-A2A_HD constexpr int mf_pow16(int a) { return a == 0 ? 1 : 16 * mf_pow16(a - 1); }
+A2A_HD constexpr int mf_pow16(int a) { return 1 << (4 * a); }   // not recursive: inlined in kernels
 A2A_HD constexpr int mf_rlast(int N) { return N / mf_pow16(mf_stages16(N)); }   // 1, 2, 4 or 8
 A2A_HD constexpr int mf_nstages(int N) { return mf_stages16(N) + (mf_rlast(N) > 1 ? 1 : 0); }
 A2A_HD constexpr int mf_radix(int N, int p) { return p < mf_stages16(N) ? 16 : mf_rlast(N); }
