@@ -15,14 +15,15 @@ DRIVER = os.path.join(ROOT, "3d-ffts-mpi_amd", "a2afft_driver")
 # (P, Ng, blockSize, dims, exec): run_tests.py:18-21 -- (8,8), (8,32), (6,12),
 # (4,4) -- plus the C1 and C2 decompositions and a mixed-radix 24^3 over 12.
 CASES = [
-    (8, 8, 64, None, "fused"), (8, 32, 128, None, "fused"), (4, 4, 64, None, "staged"),
-    (6, 12, 64, None, "fused"), (6, 12, 64, None, "staged"),
+    (8, 8, 64, None, "fused"), (8, 8, 64, None, "staged"), (8, 32, 128, None, "fused"), (8, 32, 128, None, "staged"),
+    (4, 4, 64, None, "staged"), (4, 4, 64, None, "fused"), (6, 12, 128, None, "fused"), (6, 12, 128, None, "staged"),
+    (6, 12, 64, None, "fused"),
     (4, 64, 64, (2, 2, 1), "staged"), (4, 64, 64, (2, 2, 1), "fused"), (8, 32, 64, (1, 2, 4), "fused"),
     (64, 32, 64, (4, 4, 4), "staged"), (12, 24, 64, None, "fused"), (12, 24, 64, None, "staged"),
 ]
 
 
-@pytest.mark.parametrize("case", CASES, ids=lambda c: f"P{c[0]}_Ng{c[1]}_{c[3]}_{c[4]}")
+@pytest.mark.parametrize("case", CASES, ids=lambda c: f"P{c[0]}_Ng{c[1]}_bs{c[2]}_{c[3]}_{c[4]}")
 def test_driver_reference_check(cuda, tmp_path, case):
     from a2afft import stitch
     P, Ng, bs, dims, ex = case
