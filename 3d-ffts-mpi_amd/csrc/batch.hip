@@ -346,13 +346,13 @@ static int upload(T** dst, const std::vector<T>& v) {
 // per (group, pass), phases group-major or skewed along the diagonals
 // group + pass (the later pass first), units numbered phase by phase.  The
 // device tables are replaced only when every new one is in place.
-static int batch_tables(a2afft_batch_s* b, int64_t group_points, int skewed) {
-    const int G = b->G;
+// First grid of every group, then G.
+static std::vector<int> group_starts(const a2afft_batch_s* b, int64_t group_points) {
     std::vector<int> first{0};
     if (group_points > 0) {
         int64_t acc = 0;
-        for (int g = 0; g < G; ++g) {
-            const int64_t n = b->offsets[g + 1] - b->offsets[g];
+        for (int g = 0; g < b->G; ++g) {
+            const int64_t n = b->offsets[size_t(g) + 1] - b->offsets[size_t(g)];
             if (g > first.back() && acc + n > group_points) {
                 first.push_back(g);
                 acc = 0;
@@ -360,7 +360,13 @@ static int batch_tables(a2afft_batch_s* b, int64_t group_points, int skewed) {
             acc += n;
         }
     }
-    first.push_back(G);
+    first.push_back(b->G);
+    return first;
+}
+
+static int batch_tables(a2afft_batch_s* b, int64_t group_points, int skewed) {
+    const int G = b->G;
+    const std::vector<int> first = group_starts(b, group_points);
     const int W = int(first.size()) - 1;
     if (3 * W > kMaxPhases)
         return set_error(A2AFFT_E_SIZE, "batch: %d groups of at most %lld points (at most %d groups)", W,
@@ -682,7 +688,8 @@ extern "C" int a2afft_batch_autotune(a2afft_batch b, void* data, int iters, void
     const int w = best.grid;
     for (int64_t gp : {nmax, 2 * nmax}) {
         if (st < 0) break;
-        if (batch_tables(b, gp, 1) != A2AFFT_OK) continue;   // too many groups: not a candidate
+        if (3 * (int(group_starts(b, gp).size()) - 1) > kMaxPhases) continue;   // too many groups
+        st = batch_tables(b, gp, 1);
         for (int xq = 0; xq < 2 && st >= 0; ++xq)
             for (uint32_t chunk : {1u, 2u})
                 if (st >= 0) st = trial(Schedule{w, xq, chunk, gp, 1});
