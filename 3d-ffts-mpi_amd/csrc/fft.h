@@ -25,14 +25,16 @@ struct AxisPass {
     uint32_t a2_count = 0;
 };
 
-// Supported transform lengths: powers of two 2..4096 and 2^a 3^b 5^c
-// (2^a <= 16) up to 60; a precision may lack kernels for some of the latter
-// (find_kernel says which).
+// Supported transform lengths: every N in 2..4096.  Stockham kernels cover
+// powers of two and 2^a 3^b 5^c (2^a <= 16) up to 60 (fft_len_ok; a
+// precision may lack some of the latter); the direct DFT (KF_DFT) the rest.
 bool fft_length_supported(int N);
 
 // Device twiddle table for (N, dir, dtype); created on first use, cached for
 // the life of the process (per device).  Returns null on failure.
 const void* twiddle_table(int N, int dir, a2afft_dtype dtype);
+// The table a pass with kernel choice `s` reads (Stockham or direct DFT).
+const void* pass_twiddles(int N, int dir, a2afft_dtype dtype, const KSel& s);
 
 // Launch one pass.  data: complex of dtype.  dir: -1 forward, +1 inverse.
 // sel: the kernel variant (null = the default for this geometry).

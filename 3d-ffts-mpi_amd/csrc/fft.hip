@@ -15,7 +15,73 @@ namespace a2a {
 int register_kernels_f32(KEntry* out);
 int register_kernels_f64(KEntry* out);
 
-bool fft_length_supported(int N) { return fft_len_ok(N); }
+// Every length 2..kDftMaxN runs: the Stockham kernels where their schedules
+// and builds cover it, the direct DFT otherwise.
+bool fft_length_supported(int N) { return N >= 2 && N <= kDftMaxN; }
+
+static bool is_dft(const KSel& s) { return (s.flags & KF_DFT) != 0; }
+static KSel dft_sel(int N) { return KSel{1, N, KF_DFT}; }
+
+// Direct DFT for the lengths no Stockham kernel covers (prime factors other
+// than 2, 3 and 5, mixed lengths above the one-thread-per-line limit, mixed
+// lengths not built in a precision).  A workgroup reads one line into LDS;
+// thread t forms outputs k = t, t + 256, ... as sum over n = 0..N-1, in that
+// order, of x[n] * w[(n k) mod N] with explicit fma, so every layout of the
+// same line gives the same bits, and writes them in place (all reads of the
+// line precede the barrier).  w[m] = exp(DIR 2 pi i m / N), rounded once from
+// long double, is staged in LDS when line and table fit in 64 KiB.
+template <typename real, int DIR>
+__global__ __launch_bounds__(256) void dft_kernel(typename CT<real>::V* __restrict__ data, FftGeom g,
+                                                  const typename CT<real>::V* __restrict__ w, int N) {
+    using V = typename CT<real>::V;
+    extern __shared__ __align__(16) unsigned char dft_smem[];
+    V* const x = reinterpret_cast<V*>(dft_smem);
+    V* const wl = x + N;
+    const bool stage_w = 2 * N * int(sizeof(V)) <= 65536;
+    const int tid = int(threadIdx.x);
+    const uint32_t ln = blockIdx.x;
+    const uint32_t u1 = g.dn2.div(ln);
+    const uint32_t u2 = ln - u1 * g.n2;
+    const int64_t base = g.a1.off(u1) + g.a2.off(g.a2_start + u2);
+    for (int n = tid; n < N; n += 256) {
+        x[n] = data[base + g.fft.off(uint32_t(n))];
+        if (stage_w) wl[n] = w[n];
+    }
+    __syncthreads();
+    auto run = [&](const V* tw) {
+        for (int k = tid; k < N; k += 256) {
+            real sr = 0, si = 0;
+            int m = 0;
+            for (int n = 0; n < N; ++n) {
+                const V a = x[n], t = tw[m];
+                sr = fma(a.x, t.x, sr);
+                sr = fma(-a.y, t.y, sr);
+                si = fma(a.x, t.y, si);
+                si = fma(a.y, t.x, si);
+                m += k;
+                if (m >= N) m -= N;
+            }
+            if (g.scale_mode == 1) {
+                sr *= real(g.scale);
+                si *= real(g.scale);
+            } else if (g.scale_mode == 2) {
+                sr = sr / real(g.scale);
+                si = si / real(g.scale);
+            }
+            V o;
+            o.x = sr;
+            o.y = si;
+            data[base + g.fft.off(uint32_t(k))] = o;
+        }
+    };
+    if (stage_w) run(wl);
+    else run(w);
+}
+
+static const void* dft_kernel_for(bool dbl, int dir) {
+    if (dbl) return dir < 0 ? (const void*)&dft_kernel<double, -1> : (const void*)&dft_kernel<double, 1>;
+    return dir < 0 ? (const void*)&dft_kernel<float, -1> : (const void*)&dft_kernel<float, 1>;
+}
 
 namespace {
 constexpr int kMaxKernels = 4096;
@@ -33,6 +99,8 @@ void init_registry() {
 }  // namespace
 
 const void* find_kernel(int N, bool dbl, int strided, const KSel& s, int dir) {
+    if (is_dft(s)) return (s.T == 1 && s.E == N && s.flags == KF_DFT && fft_length_supported(N)) ? dft_kernel_for(dbl, dir)
+                                                                                                    : nullptr;
     init_registry();
     for (int i = 0; i < g_nkernels; ++i) {
         const KEntry& k = g_kernels[i];
@@ -53,6 +121,7 @@ int fft_candidates(const AxisPass& p, a2afft_dtype dt, int N, KSel* out, int max
         if (!shape_fits(p.fft, p.a2, p.strided, N, k.T, k.E)) continue;
         out[n++] = KSel{k.T, k.E, k.flags};
     }
+    if (n == 0 && max > 0 && fft_length_supported(N)) out[n++] = dft_sel(N);
     return n;
 }
 
@@ -86,6 +155,7 @@ static const Pref kPrefs[] = {
 };
 
 KSel fft_default_sel(const AxisPass& p, a2afft_dtype dt, int N) {
+    if (!fft_len_ok(N)) return dft_sel(N);
     const bool dbl = dt == A2AFFT_C128;
     const int far = p.fft.s_elem * (dbl ? 16 : 8) >= 65536 ? 1 : 0;
     for (const Pref& pr : kPrefs) {
@@ -96,10 +166,40 @@ KSel fft_default_sel(const AxisPass& p, a2afft_dtype dt, int N) {
     }
     KSel s = default_sel(N, dbl, p.strided, p.fft, p.a2);
     if (npass_of(N) > 1 && s.T > 1 && is_pow2(N)) s.flags |= KF_XCD;   // mixed lengths: plain variants only
+    if (!shape_fits(p.fft, p.a2, p.strided, N, s.T, s.E) || !find_kernel(N, dbl, p.strided, s, -1))
+        return dft_sel(N);   // no Stockham kernel for this length, precision or block extent
     return s;
 }
 
+// Geometry of a direct-DFT pass: one line per workgroup, lines enumerated
+// (a1, a2) with a2 fastest over the launch's a2 window.
+static int resolve_dft(const AxisPass& p, int N, FftGeom* g, uint32_t* grid) {
+    if (p.fft.n != uint32_t(N)) return set_error(A2AFFT_E_INVALID_ARG, "fft pass: bad geometry (N=%d)", N);
+    const uint32_t count = p.a2_count ? p.a2_count : p.a2.n - p.a2_start;
+    if (uint64_t(p.a2_start) + count > p.a2.n) return set_error(A2AFFT_E_INVALID_ARG, "fft pass: bad window");
+    const uint64_t units = uint64_t(p.a1.n) * count;
+    if (units >= (uint64_t(1) << 31)) return set_error(A2AFFT_E_SIZE, "fft pass: too many lines");
+    g->fft = p.fft;
+    g->a1 = p.a1;
+    g->a2 = p.a2;
+    g->a2_start = p.a2_start;
+    g->n2 = count;
+    g->nunits = uint32_t(units);
+    g->dn2.set(count ? count : 1);
+    g->dn1.set(p.a1.n ? p.a1.n : 1);
+    g->tstride = p.a2.s_elem;
+    g->scale_mode = p.scale_mode;
+    g->scale = p.scale;
+    *grid = uint32_t(units);
+    return A2AFFT_OK;
+}
+
 static int resolve(const AxisPass& p, a2afft_dtype dt, int N, const KSel& s, FftGeom* g, uint32_t* grid) {
+    if (is_dft(s)) {
+        if (s.T != 1 || s.E != N || !fft_length_supported(N))
+            return set_error(A2AFFT_E_UNSUPPORTED, "fft pass: length %d not supported", N);
+        return resolve_dft(p, N, g, grid);
+    }
     int st = resolve_with(p.fft, p.a1, p.a2, p.strided, N, s.T, s.E, g, grid, p.a2_start, p.a2_count);
     if (st == A2AFFT_E_UNSUPPORTED)
         return set_error(st, "fft pass: length %d (block extent %u, T=%d, E=%d) not supported", N, p.fft.l, s.T, s.E);
@@ -116,7 +216,7 @@ int fft_pass_shape(const AxisPass& p, a2afft_dtype dt, int N, const KSel* sel, F
     A2A_TRY(resolve(p, dt, N, s, &g, &grid));
     out->T = s.T;
     out->E = s.E;
-    out->NT = s.T * (N / s.E);
+    out->NT = is_dft(s) ? 256 : s.T * (N / s.E);
     out->grid = int(grid);
     out->flags = s.flags;
     return A2AFFT_OK;
@@ -129,16 +229,25 @@ int launch_fft_pass(const AxisPass& p, void* data, a2afft_dtype dt, int N, int d
     uint32_t grid;
     A2A_TRY(resolve(p, dt, N, ks, &g, &grid));
     if (grid == 0) return A2AFFT_OK;
-    const void* tw = twiddle_table(N, dir, dt);
-    if (!tw) return set_error(A2AFFT_E_ALLOC, "fft: twiddle table for N=%d", N);
     const void* fn = find_kernel(N, dt == A2AFFT_C128, p.strided, ks, dir);
     if (!fn)
         return set_error(A2AFFT_E_UNSUPPORTED, "fft: no kernel built for N=%d T=%d E=%d flags=%d", N, ks.T, ks.E,
                          ks.flags);
-    void* args[] = {&data, &g, (void*)&tw};
-    int st = hip_check(int(hipLaunchKernel(fn, dim3(grid), dim3(ks.T * (N / ks.E)), args, 0, s)), "stockham launch");
+    const void* tw = pass_twiddles(N, dir, dt, ks);
+    if (!tw) return set_error(A2AFFT_E_ALLOC, "fft: twiddle table for N=%d", N);
+    int st;
+    if (is_dft(ks)) {
+        const size_t line = size_t(N) * size_t(esize_of(dt));
+        const size_t lds = 2 * line <= 65536 ? 2 * line : line;
+        int n = N;
+        void* args[] = {&data, &g, (void*)&tw, &n};
+        st = hip_check(int(hipLaunchKernel(fn, dim3(grid), dim3(256), args, lds, s)), "dft launch");
+    } else {
+        void* args[] = {&data, &g, (void*)&tw};
+        st = hip_check(int(hipLaunchKernel(fn, dim3(grid), dim3(ks.T * (N / ks.E)), args, 0, s)), "stockham launch");
+    }
     if (st < 0) return st;
-    return hip_check(int(hipGetLastError()), "stockham launch");
+    return hip_check(int(hipGetLastError()), "fft launch");
 }
 
 // ---------------------------------------------------------------------------
@@ -195,15 +304,22 @@ std::mutex g_tw_mu;
 std::map<TwKey, void*> g_tw;
 }  // namespace
 
-const void* twiddle_table(int N, int dir, a2afft_dtype dt) {
+// Stockham tables (kind 1) or the N roots of unity of the direct DFT (kind 2).
+static const void* cached_table(int N, int dir, a2afft_dtype dt, int kind) {
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return nullptr;
-    TwKey key{dev, N, dir < 0 ? -1 : 1, int(dt)};
+    TwKey key{dev, N, (dir < 0 ? -1 : 1) * kind, int(dt)};
     std::lock_guard<std::mutex> lk(g_tw_mu);
     auto it = g_tw.find(key);
     if (it != g_tw.end()) return it->second;
     std::vector<long double> re, im;
-    build_table(N, dir, re, im);
+    if (kind == 1) {
+        build_table(N, dir, re, im);
+    } else {
+        re.resize(size_t(N));
+        im.resize(size_t(N));
+        for (int m = 0; m < N; ++m) twiddle_ld(m, N, dir, &re[size_t(m)], &im[size_t(m)]);
+    }
     const size_t n = re.size();
     void* d = nullptr;
     if (dt == A2AFFT_C128) {
@@ -219,6 +335,12 @@ const void* twiddle_table(int N, int dir, a2afft_dtype dt) {
     }
     g_tw[key] = d;
     return d;
+}
+
+const void* twiddle_table(int N, int dir, a2afft_dtype dt) { return cached_table(N, dir, dt, 1); }
+
+const void* pass_twiddles(int N, int dir, a2afft_dtype dt, const KSel& s) {
+    return cached_table(N, dir, dt, is_dft(s) ? 2 : 1);
 }
 
 }  // namespace a2a

@@ -152,7 +152,9 @@ enum : int {
     KF_NT = 4,     // non-temporal global loads and stores
     KF_A2MAJ = 8,  // units numbered a2-major (z tiles outermost)
     KF_COH = 16,   // device-coherent (L2-bypassing) global accesses, for in-launch hand-offs
+    KF_DFT = 64,   // direct O(N^2) DFT, one line per workgroup: the lengths no Stockham kernel covers
 };
+constexpr int kDftMaxN = 4096;   // longest direct-DFT line (N complex values in LDS)
 
 // Kernel choice for one pass: tile width T, elements per thread E and the
 // variant flags.  NT = T * N / E threads per workgroup.

@@ -346,8 +346,28 @@ int orc_fft_batched(double* data, int N, int64_t nlines, int sign) {
     if (N <= 0) return -1;
     int rad[64];
     int nrad = factorize(N, rad);
+    int naive = 0;
     for (int p = 0; p < nrad; ++p)
-        if (rad[p] > 64) return -2;               /* butterfly scratch bound */
+        if (rad[p] > 64) naive = 1;               /* a prime factor above the butterfly scratch bound */
+    if (naive) {                                  /* the textbook sum, line by line */
+        int err = 0;
+        #pragma omp parallel
+        {
+            double* out = (double*)malloc(sizeof(double) * 2 * N);
+            if (!out) {
+                #pragma omp atomic write
+                err = -3;
+            } else {
+                #pragma omp for schedule(static)
+                for (int64_t l = 0; l < nlines; ++l) {
+                    orc_dft_naive(data + 2 * l * (int64_t)N, out, N, sign);
+                    memcpy(data + 2 * l * (int64_t)N, out, sizeof(double) * 2 * N);
+                }
+                free(out);
+            }
+        }
+        return err;
+    }
     double* tw = (double*)malloc(sizeof(double) * 2 * N);
     if (!tw) return -3;
     for (int m = 0; m < N; ++m) orc_twiddle(m, N, sign, &tw[2 * m], &tw[2 * m + 1]);

@@ -85,7 +85,8 @@ void orc_stage(int stage, const orc_decomp* d, void* b1, void* b2, int esize);
 /* --- 1D FFT (the published algorithm behind dfft/kernels/a2affts.cu:7-76) -- */
 /* In-place batched complex128 DFT of `nlines` contiguous lines of length N:
  * X[k] = sum_n x[n] exp(sign*2*pi*i*n*k/N), unnormalised.  Mixed-radix
- * Stockham over the factors of N with long-double twiddles. */
+ * Stockham over the factors of N with long-double twiddles; a length with a
+ * prime factor above 64 takes orc_dft_naive line by line. */
 int orc_fft_batched(double* data, int N, int64_t nlines, int sign);
 /* O(N^2) long-double DFT of one line (tests pin orc_fft_batched with it). */
 void orc_dft_naive(const double* in, double* out, int N, int sign);

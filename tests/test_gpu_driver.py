@@ -13,13 +13,15 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 DRIVER = os.path.join(ROOT, "3d-ffts-mpi_amd", "a2afft_driver")
 
 # (P, Ng, blockSize, dims, exec): run_tests.py:18-21 -- (8,8), (8,32), (6,12),
-# (4,4) -- plus the C1 and C2 decompositions and a mixed-radix 24^3 over 12.
+# (4,4) -- plus the C1 and C2 decompositions, a mixed-radix 24^3 over 12 and
+# 14^3 over 2 (direct DFT).
 CASES = [
     (8, 8, 64, None, "fused"), (8, 8, 64, None, "staged"), (8, 32, 128, None, "fused"), (8, 32, 128, None, "staged"),
     (4, 4, 64, None, "staged"), (4, 4, 64, None, "fused"), (6, 12, 128, None, "fused"), (6, 12, 128, None, "staged"),
     (6, 12, 64, None, "fused"),
     (4, 64, 64, (2, 2, 1), "staged"), (4, 64, 64, (2, 2, 1), "fused"), (8, 32, 64, (1, 2, 4), "fused"),
     (64, 32, 64, (4, 4, 4), "staged"), (12, 24, 64, None, "fused"), (12, 24, 64, None, "staged"),
+    (2, 14, 64, None, "fused"), (2, 14, 64, None, "staged"),   # a factor 7: the direct DFT
 ]
 
 
@@ -39,7 +41,9 @@ def test_driver_reference_check(cuda, tmp_path, case):
 
 
 def test_unsupported_length_is_an_error(cuda, tmp_path):
-    """A length with a factor 7 has no kernel: a clean error, exit status 3."""
-    r = subprocess.run([DRIVER, "-Ng", "14", "-P", "2", "-out", str(tmp_path)], capture_output=True, text=True,
-                       timeout=120)
-    assert r.returncode == 3 and "not supported" in r.stderr
+    """Lines longer than 4096 have no kernel: a clean error, not a launch."""
+    import torch
+    import a2afft
+    d = torch.zeros(2 * 4099, dtype=torch.complex128, device="cuda")
+    with pytest.raises(a2afft.A2AFFTError, match="not supported"):
+        a2afft.forward_1d_fft(d, 4099)

@@ -176,7 +176,7 @@ def test_alltoall_closed_form():
 
 # ---- FFT ---------------------------------------------------------------
 
-@pytest.mark.parametrize("N", [2, 3, 4, 5, 6, 8, 12, 16, 30, 32, 60, 64])
+@pytest.mark.parametrize("N", [2, 3, 4, 5, 6, 7, 8, 12, 14, 16, 30, 32, 49, 60, 63, 64])
 def test_fft_matches_naive_dft(N):
     rng = np.random.default_rng(N)
     x = rng.standard_normal((3, N)) + 1j * rng.standard_normal((3, N))
@@ -193,6 +193,17 @@ def test_fft_matches_numpy(N):
     assert np.linalg.norm(O.fft_batched(x, -1) - np.fft.fft(x)) <= 1e-15 * np.log2(N) * np.linalg.norm(np.fft.fft(x))
     assert np.linalg.norm(O.fft_batched(x, 1) - N * np.fft.ifft(x)) <= 1e-15 * np.log2(N) * N * \
         np.linalg.norm(np.fft.ifft(x))
+
+
+@pytest.mark.parametrize("N", [67, 97, 194, 1009])
+def test_fft_large_prime_factor_matches_numpy(N):
+    """Lengths with a prime factor above 64 go through the O(N^2) sum; numpy's
+    pocketfft (Bluestein for large primes) is the independent check."""
+    rng = np.random.default_rng(N)
+    x = rng.standard_normal((3, N)) + 1j * rng.standard_normal((3, N))
+    ref = np.fft.fft(x)
+    assert np.linalg.norm(O.fft_batched(x, -1) - ref) <= 1e-15 * np.sqrt(N) * np.linalg.norm(ref)
+    assert np.linalg.norm(O.fft_batched(x, 1) - N * np.fft.ifft(x)) <= 1e-15 * np.sqrt(N) * np.linalg.norm(ref)
 
 
 def test_fft_known_answers_and_identities():
