@@ -101,6 +101,7 @@ SIGNATURES = {
     "a2afft_batch_set_schedule": [_P, _I, _I, _I],
     "a2afft_batch_set_grouping": [_P, _I64, _I],
     "a2afft_batch_autotune": [_P, _P, _I, _P],
+    "a2afft_batch_set_shape": [_P, _I, _I, _I, _I, _I],
     "a2afft_batch_describe": [_P, ctypes.c_char_p, _I],
     "a2afft_batch_info": [_P, ctypes.POINTER(_I64), ctypes.POINTER(_I), ctypes.POINTER(_I)],
     "a2afft_batch_destroy": [_P],

@@ -288,6 +288,11 @@ class Batch:
         self._check(work)
         return L.check(L.lib().a2afft_batch_autotune(self._h, _ptr(work), iters, _stream()), "batch_autotune")
 
+    def set_shape(self, N, strided, T=0, E=0, pad=True):
+        """Tile shape of the (N, strided) class of passes (T=0: the primary)."""
+        return L.check(L.lib().a2afft_batch_set_shape(self._h, N, 1 if strided else 0, T, E, 1 if pad else 0),
+                       "batch_set_shape")
+
     def describe(self):
         buf = ctypes.create_string_buffer(16384)
         L.check(L.lib().a2afft_batch_describe(self._h, buf, len(buf)), "batch_describe")

@@ -51,17 +51,33 @@ constexpr uint32_t kSpinLimit = 1u << 20;   // polls of ~8K cycles each: seconds
 constexpr uint32_t kMaxUnits = 1u << 28;   // the unit -> item table holds 4 bytes per unit
 constexpr int kMaxPhases = 384;            // (group, pass) phases: at most 128 groups
 
-// Kernel shape per (N, strided): primary and a single-column fallback.
+// Kernel shapes per (N, strided): the primary (first of its class in this
+// table), alternatives the autotuner may choose instead (the same tile
+// unpadded, half the tile), and a single-column fallback for z extents below
+// the tile width.  Alternatives keep the primary's elements per thread and
+// stay within its LDS, so the kernel's registers (103) and LDS (39.9 KB) are
+// those of the primaries.  Twice the elements per thread was measured slower
+// for every class (the 64-grid batch, 8 x 256^3, 15 x 256x64x512) and made
+// the kernel need 164 VGPRs.
 struct BShape {
-    int N, strided, T, E;
+    int N, strided, T, E, pad;
 };
 constexpr BShape kShapes[] = {
-    {16, 0, 256, 16}, {32, 0, 64, 8}, {64, 0, 32, 8}, {128, 0, 32, 16}, {256, 0, 16, 16}, {512, 0, 8, 16},
-    {16, 1, 32, 16},  {32, 1, 32, 8}, {64, 1, 32, 8}, {128, 1, 32, 16}, {256, 1, 16, 16}, {512, 1, 8, 16},
-    {16, 1, 1, 16},   {32, 1, 1, 8},  {64, 1, 1, 8},  {128, 1, 1, 16},  {256, 1, 1, 16},  {512, 1, 1, 8},
+    // primary
+    {16, 0, 256, 16, 1}, {32, 0, 64, 8, 1}, {64, 0, 32, 8, 1}, {128, 0, 32, 16, 1}, {256, 0, 16, 16, 1},
+    {512, 0, 8, 16, 1}, {16, 1, 32, 16, 1}, {32, 1, 32, 8, 1}, {64, 1, 32, 8, 1}, {128, 1, 32, 16, 1},
+    {256, 1, 16, 16, 1}, {512, 1, 8, 16, 1},
+    // single-column fallbacks
+    {16, 1, 1, 16, 1}, {32, 1, 1, 8, 1}, {64, 1, 1, 8, 1}, {128, 1, 1, 16, 1}, {256, 1, 1, 16, 1}, {512, 1, 1, 8, 1},
+    // alternatives: unpadded
+    {32, 0, 64, 8, 0}, {64, 0, 32, 8, 0}, {128, 0, 32, 16, 0}, {256, 0, 16, 16, 0}, {512, 0, 8, 16, 0},
+    {32, 1, 32, 8, 0}, {64, 1, 32, 8, 0}, {128, 1, 32, 16, 0}, {256, 1, 16, 16, 0}, {512, 1, 8, 16, 0},
+    // alternatives: half the tile (lines or columns), 128 threads
+    {32, 0, 32, 8, 1}, {64, 0, 16, 8, 1}, {128, 0, 16, 16, 1}, {256, 0, 8, 16, 1}, {512, 0, 4, 16, 1},
+    {64, 1, 16, 8, 1}, {128, 1, 16, 16, 1}, {256, 1, 8, 16, 1}, {512, 1, 4, 16, 1},
 };
 constexpr int kNumShapes = int(sizeof(kShapes) / sizeof(kShapes[0]));
-constexpr int kFlags = KF_PAD | KF_COH;
+constexpr int kFirstAlt = 18;
 
 }  // namespace
 
@@ -75,17 +91,24 @@ struct BatchItem {
     int phase;           // the (group, pass) phase the item belongs to
 };
 
+static_assert(kNumShapes == 37, "run_unit lists every shape");
+
+constexpr int shape_flags(int i) { return (kShapes[i].pad ? KF_PAD : 0) | KF_COH; }
+
 template <int DIR, typename Hook>
 __device__ __forceinline__ void run_unit(int shape, float2* data, const FftGeom& g, const float2* tw, uint32_t u,
                                          float2* lds, int tid, Hook&& hook) {
     switch (shape) {
 #define A2A_B(I)                                                                                                \
     case I:                                                                                                    \
-        stockham_unit<float, kShapes[I].N, kShapes[I].E, kShapes[I].T, kShapes[I].strided != 0, DIR, kFlags>( \
-            data, g, tw, u, lds, tid, hook);                                                                   \
+        stockham_unit<float, kShapes[I].N, kShapes[I].E, kShapes[I].T, kShapes[I].strided != 0, DIR,          \
+                      shape_flags(I)>(data, g, tw, u, lds, tid, hook);                                         \
         break;
         A2A_B(0) A2A_B(1) A2A_B(2) A2A_B(3) A2A_B(4) A2A_B(5) A2A_B(6) A2A_B(7) A2A_B(8)
         A2A_B(9) A2A_B(10) A2A_B(11) A2A_B(12) A2A_B(13) A2A_B(14) A2A_B(15) A2A_B(16) A2A_B(17)
+        A2A_B(18) A2A_B(19) A2A_B(20) A2A_B(21) A2A_B(22) A2A_B(23) A2A_B(24) A2A_B(25) A2A_B(26)
+        A2A_B(27) A2A_B(28) A2A_B(29) A2A_B(30) A2A_B(31) A2A_B(32) A2A_B(33) A2A_B(34) A2A_B(35)
+        A2A_B(36)
 #undef A2A_B
     default:   // the host only stores valid shape indices
         __builtin_unreachable();
@@ -97,7 +120,7 @@ constexpr int batch_lds_max() {
     if constexpr (I == kNumShapes) {
         return 1;
     } else {
-        constexpr int here = lds_slots<kShapes[I].N, kShapes[I].T, kShapes[I].strided != 0, 1>();
+        constexpr int here = lds_slots<kShapes[I].N, kShapes[I].T, kShapes[I].strided != 0, kShapes[I].pad>();
         constexpr int rest = batch_lds_max<I + 1>();
         return here > rest ? here : rest;
     }
@@ -288,6 +311,8 @@ struct a2afft_batch_s {
     int G = 0;
     std::vector<int> shapes;          // 3 per grid
     std::vector<int64_t> offsets;     // G + 1
+    std::vector<AxisPass> passes;     // pass p of grid g at p * G + g
+    int choice[2][10];                // [strided][log2 N]: chosen shape of the class, -1 = the primary
     std::vector<BatchItem> proto[2];  // forward, inverse: item of (pass p, grid g) at p * G + g
     BatchItem* d_items[2] = {nullptr, nullptr};
     uint32_t* d_unit_item = nullptr;  // item of every unit
@@ -326,12 +351,56 @@ static void batch_free(a2afft_batch_s* b) {
     delete b;
 }
 
-static int pick_shape(const AxisPass& p, int N) {
-    for (int i = 0; i < kNumShapes; ++i) {
-        const BShape& s = kShapes[i];
-        if (s.N == N && s.strided == p.strided && shape_fits(p.fft, p.a2, p.strided, N, s.T, s.E)) return i;
-    }
+static int log2i(int n) {
+    int l = 0;
+    while ((1 << l) < n) ++l;
+    return l;
+}
+
+static bool shape_ok(int i, const AxisPass& p, int N) {
+    const BShape& s = kShapes[i];
+    return s.N == N && s.strided == p.strided && shape_fits(p.fft, p.a2, p.strided, N, s.T, s.E);
+}
+
+// The class's chosen shape if it fits the pass, else the primary, else the
+// single-column fallback.
+static int pick_shape(const a2afft_batch_s* b, const AxisPass& p, int N) {
+    const int c = b->choice[p.strided][log2i(N)];
+    if (c >= 0 && shape_ok(c, p, N)) return c;
+    for (int i = 0; i < kFirstAlt; ++i)
+        if (shape_ok(i, p, N)) return i;
     return -1;
+}
+
+// Items of every (pass, grid) for the current shape choices.
+static int build_protos(a2afft_batch_s* b) {
+    const int G = b->G;
+    for (int p = 0; p < 3; ++p)
+        for (int g = 0; g < G; ++g) {
+            const int k = p * G + g;
+            const AxisPass& ps = b->passes[size_t(k)];
+            const int N = int(ps.fft.n);
+            const int si = pick_shape(b, ps, N);
+            if (si < 0) return set_error(A2AFFT_E_UNSUPPORTED, "batch: no kernel shape for grid %d pass %d", g, p);
+            const int64_t total = b->offsets[size_t(g) + 1] - b->offsets[size_t(g)];
+            for (int d = 0; d < 2; ++d) {
+                BatchItem& it = b->proto[d][size_t(k)];
+                uint32_t units = 0;
+                const int st = resolve_with(ps.fft, ps.a1, ps.a2, ps.strided, N, kShapes[si].T, kShapes[si].E, &it.g,
+                                            &units);
+                if (st < 0) return set_error(st, "batch: geometry of grid %d pass %d", g, p);
+                it.g.scale_mode = (d == 1 && p == 2) ? 2 : 0;
+                it.g.scale = (d == 1 && p == 2) ? double(total) : 1.0;
+                it.base = b->offsets[size_t(g)];
+                it.units = units;
+                it.first = 0;
+                it.shape = si;
+                it.phase = 0;
+                it.tw = (const float2*)twiddle_table(N, d == 0 ? -1 : 1, A2AFFT_C64);
+                if (!it.tw) return set_error(A2AFFT_E_ALLOC, "batch: twiddle table N=%d", N);
+            }
+        }
+    return A2AFFT_OK;
 }
 
 template <typename T>
@@ -444,6 +513,9 @@ extern "C" int a2afft_batch_create(a2afft_batch* out, int ngrids, const int* sha
     const int K = 3 * ngrids;
     b->proto[0].resize(K);
     b->proto[1].resize(K);
+    b->passes.resize(K);
+    for (int s = 0; s < 2; ++s)
+        for (int l = 0; l < 10; ++l) b->choice[s][l] = -1;
     for (int g = 0; g < ngrids; ++g) {
         const int* n = shapes + 3 * g;
         const int64_t total = int64_t(n[0]) * n[1] * n[2];
@@ -456,37 +528,12 @@ extern "C" int a2afft_batch_create(a2afft_batch* out, int ngrids, const int* sha
         const int dims[3] = {1, 1, 1};
         PassMaps pm[3];
         fused_pass_maps(n, dims, n, total, pm);
-        for (int p = 0; p < 3; ++p) {
-            const int k = p * ngrids + g;
-            AxisPass ps{pm[p].fft, pm[p].a1, pm[p].a2, pm[p].strided, 0, 1.0};
-            const int N = int(ps.fft.n);
-            const int si = pick_shape(ps, N);
-            if (si < 0) {
-                batch_free(b);
-                return set_error(A2AFFT_E_UNSUPPORTED, "batch: no kernel shape for grid %d pass %d", g, p);
-            }
-            for (int d = 0; d < 2; ++d) {
-                BatchItem& it = b->proto[d][k];
-                uint32_t units = 0;
-                int st = resolve_with(ps.fft, ps.a1, ps.a2, ps.strided, N, kShapes[si].T, kShapes[si].E, &it.g, &units);
-                if (st < 0) {
-                    batch_free(b);
-                    return set_error(st, "batch: geometry of grid %d pass %d", g, p);
-                }
-                it.g.scale_mode = (d == 1 && p == 2) ? 2 : 0;
-                it.g.scale = (d == 1 && p == 2) ? double(total) : 1.0;
-                it.base = b->offsets[g];
-                it.units = units;
-                it.first = 0;
-                it.shape = si;
-                it.phase = 0;
-                it.tw = (const float2*)twiddle_table(N, d == 0 ? -1 : 1, A2AFFT_C64);
-                if (!it.tw) {
-                    batch_free(b);
-                    return set_error(A2AFFT_E_ALLOC, "batch: twiddle table N=%d", N);
-                }
-            }
-        }
+        for (int p = 0; p < 3; ++p)
+            b->passes[size_t(p * ngrids + g)] = AxisPass{pm[p].fft, pm[p].a1, pm[p].a2, pm[p].strided, 0, 1.0};
+    }
+    if (const int st0 = build_protos(b); st0 < 0) {
+        batch_free(b);
+        return st0;
     }
     int st = batch_tables(b, 0, 0);
     if (st == A2AFFT_OK)
@@ -637,11 +684,13 @@ struct Schedule {
 
 }  // namespace
 
-// Times forward executions over schedules and phase orders and keeps the
-// fastest: workgroup counts (all resident slots, 3/4, 1/2, 512, 256) x queue
-// layouts x takes of 1, 2, 4 units in pass-major order, then groups of about
-// one and two of the largest grids, skewed, at the best workgroup count.
-// Every candidate computes the same bits; `data` is overwritten.
+// Times forward executions over schedules, phase orders and tile shapes and
+// keeps the fastest: workgroup counts (all resident slots, 3/4, 1/2, 512,
+// 256) x queue layouts x takes of 1, 2, 4 units in pass-major order, then
+// groups of about one and two of the largest grids, skewed, at the best
+// workgroup count, then each (length, strided) class's tile shapes (elements
+// per thread, LDS padding) one class at a time.  Every candidate computes the
+// same bits; `data` is overwritten.
 extern "C" int a2afft_batch_autotune(a2afft_batch b, void* data, int iters, void* stream) {
     if (!b || !data || iters <= 0) return set_error(A2AFFT_E_INVALID_ARG, "batch_autotune: bad arguments");
     hipStream_t s = (hipStream_t)stream;
@@ -694,14 +743,83 @@ extern "C" int a2afft_batch_autotune(a2afft_batch b, void* data, int iters, void
             for (uint32_t chunk : {1u, 2u})
                 if (st >= 0) st = trial(Schedule{w, xq, chunk, gp, 1});
     }
-    const int st2 = apply(st >= 0 ? best : keep);
+    int keep_choice[2][10];
+    std::copy(&b->choice[0][0], &b->choice[0][0] + 20, &keep_choice[0][0]);
+    int st2 = apply(st >= 0 ? best : keep);
+    // Tile shapes, one (N, strided) class at a time at the best schedule:
+    // every shape of the class that fits one of its passes, kept if faster.
+    for (int sd = 0; sd < 2 && st >= 0 && st2 >= 0; ++sd)
+        for (int l = 4; l <= 9 && st >= 0 && st2 >= 0; ++l) {
+            const int N = 1 << l;
+            std::vector<int> cands;
+            for (int i = 0; i < kNumShapes; ++i) {
+                if (kShapes[i].N != N || kShapes[i].strided != sd || kShapes[i].T == 1) continue;
+                bool fits = false;   // fits at least one pass of the class
+                for (const AxisPass& ps : b->passes) fits = fits || shape_ok(i, ps, N);
+                if (fits) cands.push_back(i);
+            }
+            if (cands.size() < 2) continue;
+            int best_shape = b->choice[sd][l];
+            for (int c : cands) {
+                b->choice[sd][l] = c;
+                st = build_protos(b);
+                if (st >= 0) st = batch_tables(b, b->group_points, b->skewed);
+                float ms = 0.f;
+                if (st >= 0) st = batch_time(b, data, iters, s, ev, &ms);
+                if (st < 0) break;
+                snprintf(line, sizeof(line), "shape N=%d strided=%d T=%d E=%d pad=%d %.4f ms\n", N, sd, kShapes[c].T,
+                         kShapes[c].E, kShapes[c].pad, ms);
+                rep += line;
+                if (ms < best_ms) {
+                    best_ms = ms;
+                    best_shape = c;
+                }
+            }
+            b->choice[sd][l] = best_shape;
+        }
+    if (st < 0) std::copy(&keep_choice[0][0], &keep_choice[0][0] + 20, &b->choice[0][0]);
+    if (st2 >= 0) st2 = build_protos(b);
+    if (st2 >= 0) st2 = batch_tables(b, b->group_points, b->skewed);
+    if (st < 0 && st2 >= 0) st2 = apply(keep);
     drop_graphs(b);
     if (st >= 0) b->report = rep;
     return st < 0 ? st : st2;
 }
 
-// The current schedule on the first line ("key=value" pairs), then the last
-// autotune's timings.
+// Tile shape of one (length, strided) class: T lines or columns, E elements
+// per thread, LDS padding.  The shape must be in the kernel's table; passes
+// of the class it does not fit keep the primary (or the single-column
+// fallback).  T = 0 restores the primary.
+extern "C" int a2afft_batch_set_shape(a2afft_batch b, int N, int strided, int T, int E, int pad) {
+    if (!b || N < 16 || N > 512 || (N & (N - 1)) || (strided != 0 && strided != 1))
+        return set_error(A2AFFT_E_INVALID_ARG, "batch_set_shape: bad arguments");
+    const int l = log2i(N);
+    int pick = -1;
+    if (T != 0) {
+        for (int i = 0; i < kNumShapes && pick < 0; ++i) {
+            const BShape& k = kShapes[i];
+            if (k.N == N && k.strided == strided && k.T == T && k.E == E && k.pad == (pad ? 1 : 0)) pick = i;
+        }
+        if (pick < 0)
+            return set_error(A2AFFT_E_UNSUPPORTED, "batch_set_shape: no shape N=%d strided=%d T=%d E=%d pad=%d", N,
+                             strided, T, E, pad);
+    }
+    const int keep = b->choice[strided][l];
+    b->choice[strided][l] = pick;
+    int st = build_protos(b);
+    if (st >= 0) st = batch_tables(b, b->group_points, b->skewed);
+    if (st < 0) {
+        b->choice[strided][l] = keep;
+        (void)build_protos(b);
+        (void)batch_tables(b, b->group_points, b->skewed);
+    }
+    drop_graphs(b);
+    return st;
+}
+
+// The current schedule on the first line ("key=value" pairs), the tile shape
+// of every (length, strided) class on the second, then the last autotune's
+// timings.
 extern "C" int a2afft_batch_describe(a2afft_batch b, char* buf, int len) {
     if (!b || !buf || len <= 0) return set_error(A2AFFT_E_INVALID_ARG, "batch_describe: bad arguments");
     const int n = snprintf(buf, size_t(len),
@@ -709,7 +827,23 @@ extern "C" int a2afft_batch_describe(a2afft_batch b, char* buf, int len) {
                            "group_points=%lld skewed=%d\n",
                            b->G, b->total, b->qs.nphases, b->grid, b->max_grid, b->qs.xcd_queues, b->qs.chunk,
                            (long long)b->group_points, b->skewed);
-    if (n >= 0 && n < len) snprintf(buf + n, size_t(len - n), "%s", b->report.c_str());
+    std::string shapes = "shapes:";
+    for (int sd = 0; sd < 2; ++sd)
+        for (int l = 4; l <= 9; ++l) {
+            bool used = false;
+            for (const AxisPass& ps : b->passes) used = used || (int(ps.fft.n) == (1 << l) && ps.strided == sd);
+            if (!used) continue;
+            const int c = b->choice[sd][l];
+            char t[64];
+            if (c >= 0)
+                snprintf(t, sizeof(t), " %d%s=T%dE%d%s", 1 << l, sd ? "s" : "c", kShapes[c].T, kShapes[c].E,
+                         kShapes[c].pad ? "p" : "");
+            else
+                snprintf(t, sizeof(t), " %d%s=primary", 1 << l, sd ? "s" : "c");
+            shapes += t;
+        }
+    if (n >= 0 && n < len)
+        snprintf(buf + n, size_t(len - n), "%s\n%s", shapes.c_str(), b->report.c_str());
     return A2AFFT_OK;
 }
 

@@ -219,12 +219,20 @@ int a2afft_batch_set_schedule(a2afft_batch batch, int workgroups, int xcd_queues
  * most 128 groups.  Results are identical for every order. */
 int a2afft_batch_set_grouping(a2afft_batch batch, int64_t group_points, int skewed);
 /* Times `iters` forward executions per candidate on `stream` and keeps the
- * fastest schedule and order: workgroup counts x queue layouts x takes of
- * 1, 2, 4 units (pass-major), then skewed groups of about one and two of the
- * largest grids.  d_data (the batch buffer) is overwritten.  Synchronises. */
+ * fastest schedule, order and tile shapes: workgroup counts x queue layouts x
+ * takes of 1, 2, 4 units (pass-major), then skewed groups of about one and
+ * two of the largest grids, then per (length, strided) class the tile shapes
+ * (elements per thread, LDS padding).  d_data (the batch buffer) is
+ * overwritten.  Synchronises. */
 int a2afft_batch_autotune(a2afft_batch batch, void* d_data, int iters, void* stream);
-/* First line: the current schedule as key=value pairs; then the last
- * autotune's timings. */
+/* Tile shape of one (length N, strided 0/1) class of passes: T lines or
+ * columns per unit, E elements per thread, LDS padding (the kernel holds a
+ * primary shape per class, the same unpadded, and half the tile); passes it
+ * does not fit keep the primary.  T = 0 restores the primary.  Results are
+ * identical for every shape. */
+int a2afft_batch_set_shape(a2afft_batch batch, int N, int strided, int T, int E, int pad);
+/* First line: the current schedule as key=value pairs; second line: the tile
+ * shape of every (length, strided) class; then the last autotune's timings. */
 int a2afft_batch_describe(a2afft_batch batch, char* buf, int len);
 int a2afft_batch_info(a2afft_batch batch, int64_t* total_units, int* workgroups, int* max_workgroups);
 int a2afft_batch_destroy(a2afft_batch batch);

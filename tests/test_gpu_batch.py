@@ -104,6 +104,34 @@ def test_batch_autotune_keeps_bits(a2afft_mod, cuda):
     assert np.linalg.norm(y.cpu().numpy() - x0) / np.linalg.norm(x0) < 1e-6
 
 
+# (N, strided, T, E, pad): every alternative tile shape of the kernel's table
+ALT_SHAPES = [(32, 0, 64, 8, 0), (64, 0, 32, 8, 0), (128, 0, 32, 16, 0), (32, 0, 32, 8, 1), (64, 0, 16, 8, 1),
+              (128, 0, 16, 16, 1), (32, 1, 32, 8, 0), (64, 1, 32, 8, 0), (128, 1, 32, 16, 0), (64, 1, 16, 8, 1),
+              (128, 1, 16, 16, 1)]
+ALT_SHAPES_BIG = [(256, 0, 16, 16, 0), (512, 0, 8, 16, 0), (256, 0, 8, 16, 1), (512, 0, 4, 16, 1),
+                  (256, 1, 16, 16, 0), (512, 1, 8, 16, 0), (256, 1, 8, 16, 1), (512, 1, 4, 16, 1)]
+
+
+def test_batch_tile_shapes_keep_bits(a2afft_mod, cuda):
+    """Every alternative tile shape computes the primary's bits."""
+    import torch
+    for shapes, alts in ((SHAPES, ALT_SHAPES), ([(512, 256, 512), (256, 512, 256)], ALT_SHAPES_BIG)):
+        b = a2afft_mod.Batch(shapes)
+        x = _fill(a2afft_mod, torch, b, 13)
+        ref = x.clone()
+        b.forward(ref)
+        ref = ref.cpu().numpy()
+        for N, strided, T, E, pad in alts:
+            b.set_shape(N, strided, T, E, bool(pad))
+            y = x.clone()
+            b.forward(y)
+            assert b.status() == 0
+            assert np.array_equal(_bits(y.cpu().numpy()), _bits(ref)), (N, strided, T, E, pad)
+            b.set_shape(N, strided)   # back to the primary
+    with pytest.raises(a2afft_mod.A2AFFTError):
+        b.set_shape(512, 1, 3, 16)   # not in the table
+
+
 def test_batch_grouping_limits(a2afft_mod, cuda):
     """At most 128 groups; a rejected grouping leaves the previous one in force."""
     import torch
