@@ -331,7 +331,8 @@ static int plan_init(a2afft_plan_s* p) {
         if (!find_kernel(N, p->d.dtype == A2AFFT_C128, p->pass[i].strided, p->sel[i], -1))
             return set_error(A2AFFT_E_UNSUPPORTED, "plan: no kernel built for length %d in this precision", N);
         // build the twiddle tables now so execute allocates nothing
-        if (!pass_twiddles(N, -1, p->d.dtype, p->sel[i]) || !pass_twiddles(N, 1, p->d.dtype, p->sel[i]))
+        if (!pass_twiddles(p->pass[i], N, -1, p->d.dtype, p->sel[i]) ||
+            !pass_twiddles(p->pass[i], N, 1, p->d.dtype, p->sel[i]))
             return set_error(A2AFFT_E_ALLOC, "plan: twiddle tables");
     }
     return A2AFFT_OK;

@@ -25,21 +25,37 @@ struct AxisPass {
     uint32_t a2_count = 0;
 };
 
-// Supported transform lengths: every N in 2..4096.  Stockham kernels cover
-// powers of two and 2^a 3^b 5^c (2^a <= 16) up to 60 (fft_len_ok; a
-// precision may lack some of the latter); the direct DFT (KF_DFT) the rest.
+// Supported transform lengths: every N in 2..4096.  Fixed-schedule Stockham
+// kernels cover powers of two and 2^a 3^b 5^c (2^a <= 16) up to 60
+// (fft_len_ok; a precision may lack some of the latter), the runtime-schedule
+// Stockham kernel (KF_MIX) the other 2^a 3^b 5^c lengths, the direct DFT
+// (KF_DFT) the rest.
 bool fft_length_supported(int N);
 
 // Device twiddle table for (N, dir, dtype); created on first use, cached for
 // the life of the process (per device).  Returns null on failure.
 const void* twiddle_table(int N, int dir, a2afft_dtype dtype);
-// The table a pass with kernel choice `s` reads (Stockham or direct DFT).
-const void* pass_twiddles(int N, int dir, a2afft_dtype dtype, const KSel& s);
+// The table a pass with kernel choice `s` reads (Stockham, runtime-schedule
+// Stockham or direct DFT).
+struct AxisPass;
+const void* pass_twiddles(const AxisPass& pass, int N, int dir, a2afft_dtype dtype, const KSel& s);
 
 // Launch one pass.  data: complex of dtype.  dir: -1 forward, +1 inverse.
 // sel: the kernel variant (null = the default for this geometry).
 int launch_fft_pass(const AxisPass& pass, void* data, a2afft_dtype dtype, int N, int dir, hipStream_t s,
                     const KSel* sel);
+
+// Stockham FFT with the radix schedule in the kernel arguments (KF_MIX,
+// fft_mixed.hip) for 2^a 3^b 5^c lengths the fixed schedules do not cover.
+constexpr int kMixMaxPasses = 16;
+struct MixedPlan {
+    int N, E, npass, T, TPL, twsize;
+    int radix[kMixMaxPasses], ns[kMixMaxPasses], toff[kMixMaxPasses];
+};
+bool mixed_plan(const AxisPass& pass, int N, bool dbl, MixedPlan* mp);
+const void* mixed_kernel_for(int E, bool dbl, int strided, int dir);
+int mixed_lds_bytes(const MixedPlan& mp, bool dbl);
+const void* mixed_twiddles(const MixedPlan& mp, int dir, a2afft_dtype dtype);
 
 // Launch shape of a pass (for tests and tuning reports).
 struct FftShape {

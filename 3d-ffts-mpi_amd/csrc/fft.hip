@@ -20,7 +20,16 @@ int register_kernels_f64(KEntry* out);
 bool fft_length_supported(int N) { return N >= 2 && N <= kDftMaxN; }
 
 static bool is_dft(const KSel& s) { return (s.flags & KF_DFT) != 0; }
+static bool is_mix(const KSel& s) { return (s.flags & KF_MIX) != 0; }
 static KSel dft_sel(int N) { return KSel{1, N, KF_DFT}; }
+
+// Kernel choice for a length no fixed-schedule kernel covers: the
+// runtime-schedule Stockham when its shape fits the pass, else the direct DFT.
+static KSel fallback_sel(const AxisPass& p, a2afft_dtype dt, int N) {
+    MixedPlan mp;
+    if (mixed_plan(p, N, dt == A2AFFT_C128, &mp)) return KSel{mp.T, mp.E, KF_MIX};
+    return dft_sel(N);
+}
 
 // Direct DFT for the lengths no Stockham kernel covers (prime factors other
 // than 2, 3 and 5, mixed lengths above the one-thread-per-line limit, mixed
@@ -99,6 +108,7 @@ void init_registry() {
 }  // namespace
 
 const void* find_kernel(int N, bool dbl, int strided, const KSel& s, int dir) {
+    if (is_mix(s)) return s.flags == KF_MIX ? mixed_kernel_for(s.E, dbl, strided, dir) : nullptr;
     if (is_dft(s)) return (s.T == 1 && s.E == N && s.flags == KF_DFT && fft_length_supported(N)) ? dft_kernel_for(dbl, dir)
                                                                                                     : nullptr;
     init_registry();
@@ -121,7 +131,11 @@ int fft_candidates(const AxisPass& p, a2afft_dtype dt, int N, KSel* out, int max
         if (!shape_fits(p.fft, p.a2, p.strided, N, k.T, k.E)) continue;
         out[n++] = KSel{k.T, k.E, k.flags};
     }
-    if (n == 0 && max > 0 && fft_length_supported(N)) out[n++] = dft_sel(N);
+    if (n == 0 && fft_length_supported(N)) {
+        const KSel f = fallback_sel(p, dt, N);
+        if (is_mix(f) && n < max) out[n++] = f;
+        if (n < max) out[n++] = dft_sel(N);
+    }
     return n;
 }
 
@@ -155,7 +169,7 @@ static const Pref kPrefs[] = {
 };
 
 KSel fft_default_sel(const AxisPass& p, a2afft_dtype dt, int N) {
-    if (!fft_len_ok(N)) return dft_sel(N);
+    if (!fft_len_ok(N)) return fallback_sel(p, dt, N);
     const bool dbl = dt == A2AFFT_C128;
     const int far = p.fft.s_elem * (dbl ? 16 : 8) >= 65536 ? 1 : 0;
     for (const Pref& pr : kPrefs) {
@@ -167,7 +181,7 @@ KSel fft_default_sel(const AxisPass& p, a2afft_dtype dt, int N) {
     KSel s = default_sel(N, dbl, p.strided, p.fft, p.a2);
     if (npass_of(N) > 1 && s.T > 1 && is_pow2(N)) s.flags |= KF_XCD;   // mixed lengths: plain variants only
     if (!shape_fits(p.fft, p.a2, p.strided, N, s.T, s.E) || !find_kernel(N, dbl, p.strided, s, -1))
-        return dft_sel(N);   // no Stockham kernel for this length, precision or block extent
+        return fallback_sel(p, dt, N);   // no fixed-schedule kernel for this length, precision or block extent
     return s;
 }
 
@@ -194,7 +208,20 @@ static int resolve_dft(const AxisPass& p, int N, FftGeom* g, uint32_t* grid) {
     return A2AFFT_OK;
 }
 
-static int resolve(const AxisPass& p, a2afft_dtype dt, int N, const KSel& s, FftGeom* g, uint32_t* grid) {
+static int resolve(const AxisPass& p, a2afft_dtype dt, int N, const KSel& s, FftGeom* g, uint32_t* grid,
+                   MixedPlan* mp_out = nullptr) {
+    if (is_mix(s)) {
+        MixedPlan mp;
+        if (!mixed_plan(p, N, dt == A2AFFT_C128, &mp) || mp.T != s.T || mp.E != s.E)
+            return set_error(A2AFFT_E_UNSUPPORTED, "fft pass: length %d (T=%d, E=%d) not supported", N, s.T, s.E);
+        if (p.fft.n != uint32_t(N)) return set_error(A2AFFT_E_INVALID_ARG, "fft pass: bad geometry (N=%d)", N);
+        const int st = resolve_geom(p.fft, p.a1, p.a2, p.strided, mp.T, g, grid, p.a2_start, p.a2_count);
+        if (st < 0) return set_error(st, "fft pass: bad geometry (N=%d)", N);
+        g->scale_mode = p.scale_mode;
+        g->scale = p.scale;
+        if (mp_out) *mp_out = mp;
+        return A2AFFT_OK;
+    }
     if (is_dft(s)) {
         if (s.T != 1 || s.E != N || !fft_length_supported(N))
             return set_error(A2AFFT_E_UNSUPPORTED, "fft pass: length %d not supported", N);
@@ -227,16 +254,22 @@ int launch_fft_pass(const AxisPass& p, void* data, a2afft_dtype dt, int N, int d
     const KSel ks = sel ? *sel : fft_default_sel(p, dt, N);
     FftGeom g;
     uint32_t grid;
-    A2A_TRY(resolve(p, dt, N, ks, &g, &grid));
+    MixedPlan mp{};
+    A2A_TRY(resolve(p, dt, N, ks, &g, &grid, &mp));
     if (grid == 0) return A2AFFT_OK;
     const void* fn = find_kernel(N, dt == A2AFFT_C128, p.strided, ks, dir);
     if (!fn)
         return set_error(A2AFFT_E_UNSUPPORTED, "fft: no kernel built for N=%d T=%d E=%d flags=%d", N, ks.T, ks.E,
                          ks.flags);
-    const void* tw = pass_twiddles(N, dir, dt, ks);
+    const void* tw = pass_twiddles(p, N, dir, dt, ks);
     if (!tw) return set_error(A2AFFT_E_ALLOC, "fft: twiddle table for N=%d", N);
     int st;
-    if (is_dft(ks)) {
+    if (is_mix(ks)) {
+        void* args[] = {&data, &g, (void*)&tw, &mp};
+        st = hip_check(int(hipLaunchKernel(fn, dim3(grid), dim3(256), args,
+                                           size_t(mixed_lds_bytes(mp, dt == A2AFFT_C128)), s)),
+                       "mixed-radix launch");
+    } else if (is_dft(ks)) {
         const size_t line = size_t(N) * size_t(esize_of(dt));
         const size_t lds = 2 * line <= 65536 ? 2 * line : line;
         int n = N;
@@ -339,7 +372,12 @@ static const void* cached_table(int N, int dir, a2afft_dtype dt, int kind) {
 
 const void* twiddle_table(int N, int dir, a2afft_dtype dt) { return cached_table(N, dir, dt, 1); }
 
-const void* pass_twiddles(int N, int dir, a2afft_dtype dt, const KSel& s) {
+const void* pass_twiddles(const AxisPass& p, int N, int dir, a2afft_dtype dt, const KSel& s) {
+    if (is_mix(s)) {
+        MixedPlan mp;
+        if (!mixed_plan(p, N, dt == A2AFFT_C128, &mp)) return nullptr;
+        return mixed_twiddles(mp, dir, dt);
+    }
     return cached_table(N, dir, dt, is_dft(s) ? 2 : 1);
 }
 

@@ -1,0 +1,262 @@
+// Stockham FFTs for the 2^a 3^b 5^c lengths the fixed-schedule kernels do not
+// cover (above the one-thread-per-line limit of fft_len_ok, up to 4096): the
+// radix schedule travels in the kernel arguments and only the elements per
+// thread E is a template parameter, so nine E values cover every such length.
+//
+// Schedule: the power-of-two factor as radix-4 passes (one radix 2 for an odd
+// exponent) or as radix-2 passes, whichever allows the smaller E, then the
+// 3s, then the 5s.  E is the smallest built value (at most 60 complex64 or 30
+// complex128 values) that every radix divides and that divides N; a thread holds
+// the elements n = t + e*TPL (TPL = N/E) of one line, exactly as in the
+// fixed kernels, and each pass's body is unrolled for its radix behind a
+// switch, so the registers are indexed with constants only.
+#include "fft.h"
+#include "stockham.h"
+
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <vector>
+
+namespace a2a {
+
+namespace {
+
+constexpr int kMixE[] = {9, 10, 12, 15, 18, 20, 25, 30, 60};
+constexpr int kMixThreads = 256;
+
+int gcd_i(int a, int b) { return b ? gcd_i(b, a % b) : a; }
+
+template <bool STRIDED>
+__device__ __forceinline__ int mix_slot(int line, int d, int N, int T) {
+    return STRIDED ? d * T + line : line * N + d;
+}
+
+// Pass p of radix R on the thread's registers, then (unless it is the last
+// pass) the exchange through LDS that puts the next pass's inputs in place.
+template <int R, int E, bool STRIDED, int DIR, typename V>
+__device__ __forceinline__ void mix_pass(V (&v)[E], V* lds, const V* __restrict__ tw, const MixedPlan& mp, int p,
+                                         int t, int line, bool active) {
+    if constexpr (E % R == 0) {
+        constexpr int NB = E / R;
+        const int TPL = mp.TPL, Ns = mp.ns[p], toff = mp.toff[p];
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            V x[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) x[r] = v[b + r * NB];
+            if (p > 0) {
+                const int j = t + b * TPL;
+                const int k = j % Ns;
+#pragma unroll
+                for (int r = 1; r < R; ++r) x[r] = c_mul(x[r], tw[toff + k * R + r]);
+            }
+            dft<R, DIR>(x);
+#pragma unroll
+            for (int q = 0; q < R; ++q) v[b + q * NB] = x[q];
+        }
+        if (p + 1 < mp.npass) {
+            if (p > 0) __syncthreads();   // the previous exchange's reads are done
+            if (active) {
+#pragma unroll
+                for (int b = 0; b < NB; ++b) {
+                    const int j = t + b * TPL;
+                    const int d0 = (j / Ns) * (Ns * R) + j % Ns;
+#pragma unroll
+                    for (int q = 0; q < R; ++q) lds[mix_slot<STRIDED>(line, d0 + q * Ns, mp.N, mp.T)] = v[b + q * NB];
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int e = 0; e < E; ++e) v[e] = lds[mix_slot<STRIDED>(line, t + e * TPL, mp.N, mp.T)];
+        }
+    }
+}
+
+template <typename real, int E, bool STRIDED, int DIR>
+__global__ __launch_bounds__(kMixThreads) void mixed_kernel(typename CT<real>::V* __restrict__ data, FftGeom g,
+                                                            const typename CT<real>::V* __restrict__ tw, MixedPlan mp) {
+    using V = typename CT<real>::V;
+    extern __shared__ __align__(16) unsigned char mix_smem[];
+    V* const lds = reinterpret_cast<V*>(mix_smem);
+    const int TPL = mp.TPL, T = mp.T;
+    const int tid = int(threadIdx.x);
+    const bool active = tid < T * TPL;
+    const int tl = active ? tid : 0;
+    const int t = STRIDED ? tl / T : tl % TPL;
+    const int line = STRIDED ? tl % T : tl / TPL;
+    bool valid;
+    const int64_t base = line_base(g, STRIDED, T, TPL, blockIdx.x, tl, &valid, false);
+    valid = valid && active;
+    V v[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        if (valid) v[e] = data[base + elem_off(g, e * TPL)];
+        else { v[e].x = 0; v[e].y = 0; }
+    }
+    for (int p = 0; p < mp.npass; ++p) {
+        // t and line as the compiler must re-read them every pass: left
+        // visible, the LDS addresses of every radix's exchange are hoisted
+        // out of the loop and held in registers across all four bodies.
+        int tp = t, lp = line;
+        asm volatile("" : "+v"(tp), "+v"(lp));
+        switch (mp.radix[p]) {
+        case 2: mix_pass<2, E, STRIDED, DIR>(v, lds, tw, mp, p, tp, lp, active); break;
+        case 3: mix_pass<3, E, STRIDED, DIR>(v, lds, tw, mp, p, tp, lp, active); break;
+        case 4: mix_pass<4, E, STRIDED, DIR>(v, lds, tw, mp, p, tp, lp, active); break;
+        default: mix_pass<5, E, STRIDED, DIR>(v, lds, tw, mp, p, tp, lp, active); break;
+        }
+    }
+    if (!valid) return;
+    if (g.scale_mode == 1) {
+        const real s = real(g.scale);
+#pragma unroll
+        for (int e = 0; e < E; ++e) { v[e].x *= s; v[e].y *= s; }
+    } else if (g.scale_mode == 2) {
+        const real s = real(g.scale);
+#pragma unroll
+        for (int e = 0; e < E; ++e) { v[e].x = v[e].x / s; v[e].y = v[e].y / s; }
+    }
+#pragma unroll
+    for (int e = 0; e < E; ++e) data[base + elem_off(g, e * TPL)] = v[e];
+}
+
+template <typename real, int E>
+const void* mixed_fn(bool strided, int dir) {
+    if (strided)
+        return dir < 0 ? (const void*)&mixed_kernel<real, E, true, -1> : (const void*)&mixed_kernel<real, E, true, 1>;
+    return dir < 0 ? (const void*)&mixed_kernel<real, E, false, -1> : (const void*)&mixed_kernel<real, E, false, 1>;
+}
+
+template <typename real>
+const void* mixed_fn_e(int E, bool strided, int dir) {
+    switch (E) {
+    case 9: return mixed_fn<real, 9>(strided, dir);
+    case 10: return mixed_fn<real, 10>(strided, dir);
+    case 12: return mixed_fn<real, 12>(strided, dir);
+    case 15: return mixed_fn<real, 15>(strided, dir);
+    case 18: return mixed_fn<real, 18>(strided, dir);
+    case 20: return mixed_fn<real, 20>(strided, dir);
+    case 25: return mixed_fn<real, 25>(strided, dir);
+    case 30: return mixed_fn<real, 30>(strided, dir);
+    default: return nullptr;
+    }
+}
+
+}  // namespace
+
+// The schedule and E depend on (N, precision) only, so every pass and every
+// layout of a line computes the same bits; the pass decides the tile (T) and
+// whether the shape fits at all (additive offsets: the axis block extent and
+// TPL divide one another).
+bool mixed_plan(const AxisPass& ps, int N, bool dbl, MixedPlan* mp) {
+    if (N < 2 || N > kDftMaxN || is_pow2(N)) return false;
+    int m = N, a = 0, b3 = 0, c5 = 0;
+    while (m % 2 == 0) { m /= 2; ++a; }
+    while (m % 3 == 0) { m /= 3; ++b3; }
+    while (m % 5 == 0) { m /= 5; ++c5; }
+    if (m != 1) return false;
+    const int emax = dbl ? 30 : 60;
+    const int esize = dbl ? 16 : 8;
+    MixedPlan q{};
+    bool found = false;
+    for (int twos = 0; twos < 2; ++twos) {   // the variant with the smaller E (fewer registers)
+        MixedPlan c{};
+        c.N = N;
+        int np = 0;
+        if (!twos) {
+            for (int i = 0; i < a / 2; ++i) c.radix[np++] = 4;
+            if (a % 2) c.radix[np++] = 2;
+        } else {
+            if (a < 2) continue;   // the same schedule as above
+            for (int i = 0; i < a; ++i) c.radix[np++] = 2;
+        }
+        for (int i = 0; i < b3; ++i) c.radix[np++] = 3;
+        for (int i = 0; i < c5; ++i) c.radix[np++] = 5;
+        if (np > kMixMaxPasses) continue;
+        c.npass = np;
+        int G = 1;
+        for (int i = 0; i < np; ++i) G = G / gcd_i(G, c.radix[i]) * c.radix[i];
+        for (int e : kMixE)
+            if (e <= emax && e % G == 0 && N % e == 0) {
+                c.E = e;
+                break;
+            }
+        if (c.E == 0 || N / c.E > kMixThreads) continue;
+        if (!found || c.E < q.E) q = c;
+        found = true;
+    }
+    if (!found) return false;
+    q.TPL = N / q.E;
+    if (!(ps.fft.l % uint32_t(q.TPL) == 0 || uint32_t(q.TPL) % ps.fft.l == 0)) return false;
+    const int lds_lines = 65536 / (N * esize);
+    if (lds_lines < 1) return false;
+    if (!ps.strided) {
+        q.T = std::min(kMixThreads / q.TPL, lds_lines);
+    } else {
+        q.T = 1;
+        for (int tt : {16, 8, 4, 2})
+            if (tt * q.TPL <= kMixThreads && tt <= lds_lines && ps.a2.s_elem == 1 && ps.a2.l % uint32_t(tt) == 0 &&
+                ps.a2.n % uint32_t(tt) == 0) {
+                q.T = tt;
+                break;
+            }
+    }
+    int ns = 1, off = 0;
+    for (int i = 0; i < q.npass; ++i) {
+        q.ns[i] = ns;
+        q.toff[i] = i > 0 ? off : 0;
+        if (i > 0) off += ns * q.radix[i];
+        ns *= q.radix[i];
+    }
+    q.twsize = off;
+    *mp = q;
+    return true;
+}
+
+const void* mixed_kernel_for(int E, bool dbl, int strided, int dir) {
+    if (dbl) return E == 60 ? nullptr : mixed_fn_e<double>(E, strided != 0, dir);
+    if (E == 60) return mixed_fn<float, 60>(strided != 0, dir);
+    return mixed_fn_e<float>(E, strided != 0, dir);
+}
+
+int mixed_lds_bytes(const MixedPlan& mp, bool dbl) { return mp.T * mp.N * (dbl ? 16 : 8); }
+
+// exp(dir 2 pi i r k / (Ns R)) at toff[p] + k R + r for passes p >= 1,
+// long double rounded once; cached per (device, N, dir, dtype).
+const void* mixed_twiddles(const MixedPlan& mp, int dir, a2afft_dtype dt) {
+    static std::mutex mu;
+    static std::map<std::vector<int>, void*> cache;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+    std::vector<int> key{dev, mp.N, dir < 0 ? -1 : 1, int(dt), mp.npass};
+    for (int i = 0; i < mp.npass; ++i) key.push_back(mp.radix[i]);
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    const size_t n = size_t(mp.twsize > 0 ? mp.twsize : 1);
+    std::vector<long double> re(n, 1.0L), im(n, 0.0L);
+    for (int p = 1; p < mp.npass; ++p) {
+        const int R = mp.radix[p], Ns = mp.ns[p];
+        for (int k = 0; k < Ns; ++k)
+            for (int r = 0; r < R; ++r)
+                twiddle_ld(int64_t(r) * k, int64_t(Ns) * R, dir, &re[size_t(mp.toff[p] + k * R + r)],
+                           &im[size_t(mp.toff[p] + k * R + r)]);
+    }
+    void* d = nullptr;
+    if (dt == A2AFFT_C128) {
+        std::vector<double2> h(n);
+        for (size_t i = 0; i < n; ++i) { h[i].x = double(re[i]); h[i].y = double(im[i]); }
+        if (hipMalloc(&d, n * sizeof(double2)) != hipSuccess) return nullptr;
+        if (hipMemcpy(d, h.data(), n * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+    } else {
+        std::vector<float2> h(n);
+        for (size_t i = 0; i < n; ++i) { h[i].x = float(re[i]); h[i].y = float(im[i]); }
+        if (hipMalloc(&d, n * sizeof(float2)) != hipSuccess) return nullptr;
+        if (hipMemcpy(d, h.data(), n * sizeof(float2), hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+    }
+    cache[key] = d;
+    return d;
+}
+
+}  // namespace a2a

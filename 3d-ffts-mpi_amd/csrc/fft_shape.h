@@ -153,6 +153,7 @@ enum : int {
     KF_A2MAJ = 8,  // units numbered a2-major (z tiles outermost)
     KF_COH = 16,   // device-coherent (L2-bypassing) global accesses, for in-launch hand-offs
     KF_DFT = 64,   // direct O(N^2) DFT, one line per workgroup: the lengths no Stockham kernel covers
+    KF_MIX = 128,  // Stockham FFT with the radix schedule in the arguments: long 2^a 3^b 5^c lengths
 };
 constexpr int kDftMaxN = 4096;   // longest direct-DFT line (N complex values in LDS)
 
@@ -191,11 +192,9 @@ inline KSel default_sel(int N, bool dbl, int strided, const AxisMap& fft, const 
 // Final geometry of a pass for a chosen shape; returns 0 or a negative status.
 // The launch covers a2 coordinates [a2_start, a2_start + a2_count) (count 0 =
 // the whole axis); a strided window must start and end on tile boundaries.
-inline int resolve_with(const AxisMap& fft, const AxisMap& a1, const AxisMap& a2, int strided, int N, int T, int E,
-                        FftGeom* g, uint32_t* grid, uint32_t a2_start = 0, uint32_t a2_count = 0) {
-    if (!fft_len_ok(N)) return A2AFFT_E_UNSUPPORTED;
-    if (fft.n != uint32_t(N)) return A2AFFT_E_INVALID_ARG;
-    if (!shape_fits(fft, a2, strided, N, T, E)) return A2AFFT_E_UNSUPPORTED;
+// The geometry once the shape is known to fit (any length).
+inline int resolve_geom(const AxisMap& fft, const AxisMap& a1, const AxisMap& a2, int strided, int T, FftGeom* g,
+                        uint32_t* grid, uint32_t a2_start, uint32_t a2_count) {
     if (a2_count == 0) a2_count = a2.n - a2_start;
     if (uint64_t(a2_start) + a2_count > a2.n) return A2AFFT_E_INVALID_ARG;
     if (strided && (a2_start % uint32_t(T) || a2_count % uint32_t(T))) return A2AFFT_E_INVALID_ARG;
@@ -222,6 +221,14 @@ inline int resolve_with(const AxisMap& fft, const AxisMap& a1, const AxisMap& a2
     g->dn1.set(a1.n ? a1.n : 1);
     *grid = uint32_t(blocks);
     return A2AFFT_OK;
+}
+
+inline int resolve_with(const AxisMap& fft, const AxisMap& a1, const AxisMap& a2, int strided, int N, int T, int E,
+                        FftGeom* g, uint32_t* grid, uint32_t a2_start = 0, uint32_t a2_count = 0) {
+    if (!fft_len_ok(N)) return A2AFFT_E_UNSUPPORTED;
+    if (fft.n != uint32_t(N)) return A2AFFT_E_INVALID_ARG;
+    if (!shape_fits(fft, a2, strided, N, T, E)) return A2AFFT_E_UNSUPPORTED;
+    return resolve_geom(fft, a1, a2, strided, T, g, grid, a2_start, a2_count);
 }
 
 // Axis maps of the three fused passes over the virtual-rank block layout:

@@ -154,10 +154,11 @@ int a2afft_alltoall(const void* send, void* recv, int world_size, int64_t nlocal
 /* ---- FFT launchers (dfft/kernels/kernels.h:5-11) ------------------------ */
 /* forward_1d_fft / inverse_1d_fft (a2affts.cu:7-76): n_elems/Ng in-place
  * unnormalised DFTs of contiguous length-Ng pencils (replaces cuFFT Z2Z/C2C).
- * Lengths: every Ng in 2..4096 (cuFFT's domain up to that length).  Powers of
- * two and 2^a*3^b*5^c with 2^a <= 16 up to 60 (complex64) or 30 (complex128)
- * run as Stockham FFTs; the others as a direct O(Ng^2) DFT per line.  Longer
- * lines return A2AFFT_E_UNSUPPORTED. */
+ * Lengths: every Ng in 2..4096 (cuFFT's domain up to that length).  Every
+ * 2^a*3^b*5^c length runs as a Stockham FFT (fixed schedules for powers of
+ * two and short mixed lengths, the radix schedule in the kernel arguments for
+ * the rest); lengths with other prime factors as a direct O(Ng^2) DFT per
+ * line.  Longer lines return A2AFFT_E_UNSUPPORTED. */
 int a2afft_forward_1d_fft(void* data, int Ng, int64_t n_elems, a2afft_dtype dtype, void* stream);
 int a2afft_inverse_1d_fft(void* data, int Ng, int64_t n_elems, a2afft_dtype dtype, void* stream);
 /* launch_scale_fft (a2affts.cu:78-92,108-116): out = data / Ng^3 (64-bit Ng^3). */

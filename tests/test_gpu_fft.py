@@ -51,10 +51,51 @@ def test_batched_1d_fft_vs_oracle(a2afft_mod, cuda, N, dtype):
     assert _relerr(d.cpu().numpy(), ref_i) < TOL_1D[dtype] * max(1, np.log2(N))
 
 
-# Lengths no Stockham schedule covers: prime factors 7, 11, 97, and smooth
-# lengths above the mixed-radix limit.  They run as the direct O(N^2) DFT,
+# 2^a 3^b 5^c lengths above the one-thread-per-line limit: the Stockham kernel
+# with the radix schedule in its arguments (radix 4/2, then 3s, then 5s).
+MIXED_LONG = [72, 96, 100, 120, 192, 243, 384, 625, 768, 1000, 1536, 2187, 3072, 3125, 3840, 4000]
+
+
+def _smooth(n):
+    for f in (2, 3, 5):
+        while n % f == 0:
+            n //= f
+    return n == 1
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+@pytest.mark.parametrize("N", MIXED_LONG)
+def test_long_mixed_lengths_vs_oracle(a2afft_mod, cuda, N, dtype):
+    import torch
+    rng = np.random.default_rng(N + 1)
+    lines = 7
+    x = rng.standard_normal((lines, N)) + 1j * rng.standard_normal((lines, N))
+    x = x.astype(np.complex128 if dtype == "c128" else np.complex64)
+    for sign, fn in ((-1, a2afft_mod.forward_1d_fft), (1, a2afft_mod.inverse_1d_fft)):
+        ref = O.fft_batched(x.astype(np.complex128), sign)
+        d = torch.from_numpy(x).cuda()
+        fn(d, N)
+        torch.cuda.synchronize()
+        err = _relerr(d.cpu().numpy(), ref)
+        assert err < TOL_1D[dtype] * np.log2(N), (N, dtype, sign, err)
+
+
+def test_long_mixed_impulses(a2afft_mod, cuda):
+    """N = 384 = 2^7 * 3 and 3125 = 5^5: an impulse at n0 transforms to exp(-2 pi i k n0 / N)."""
+    import torch
+    for N, n0 in ((384, 7), (3125, 1234)):
+        x = np.zeros((1, N), dtype=np.complex128)
+        x[0, n0] = 1.0
+        d = torch.from_numpy(x).cuda()
+        a2afft_mod.forward_1d_fft(d, N)
+        k = np.arange(N)
+        want = np.exp(-2j * np.pi * ((k * n0) % N) / N)   # reduced first: exact phases for numpy's exp
+        assert np.abs(d.cpu().numpy()[0] - want).max() < 1e-13, N
+
+
+# Lengths with other prime factors (7, 11, 13, 97): the direct O(N^2) DFT,
 # whose rounding grows like sqrt(N) rather than log2(N).
-DIRECT = [7, 11, 14, 49, 63, 97, 100, 120, 243, 1000, 4095]
+DIRECT = [7, 11, 14, 49, 63, 97, 4095]
 
 
 @pytest.mark.parametrize("dtype", ["c128", "c64"])
@@ -87,7 +128,7 @@ def test_direct_dft_known_answers(a2afft_mod, cuda):
     a2afft_mod.forward_1d_fft(d, N)
     y = d.cpu().numpy()
     k = np.arange(N)
-    assert np.abs(y[0] - np.exp(-2j * np.pi * k * n0 / N)).max() < 1e-14
+    assert np.abs(y[0] - np.exp(-2j * np.pi * ((k * n0) % N) / N)).max() < 1e-14
     e = np.zeros(N, dtype=np.complex128)
     e[k0] = N
     assert np.abs(y[1] - e).max() < 1e-12
@@ -122,8 +163,11 @@ CONFIGS_3D = [
     (14, 2, None, "c128"),        # the direct DFT (a factor 7) on every axis, blocks of 7 x 14 x 14
     (14, 2, None, "c64"),
     (21, 3, None, "c128"),        # 3 * 7 over dims [3,1,1]
-    (100, 4, (2, 2, 1), "c64"),   # 2^2 * 5^2, above the mixed-radix limit
-    (45, 9, None, "c128"),        # 45 has no complex128 Stockham kernel: the direct DFT
+    (100, 4, (2, 2, 1), "c64"),   # 2^2 * 5^2, above the one-thread-per-line limit: runtime schedule
+    (45, 9, None, "c128"),        # no fixed complex128 kernel for 45: runtime schedule
+    (96, 8, (1, 2, 4), "c128"),   # 2^5 * 3 over blocks of 96 x 48 x 24
+    (96, 8, (1, 2, 4), "c64"),
+    (120, 8, (2, 2, 2), "c128"),  # 2^3 * 3 * 5: radix-2 schedule in complex128
     (12, 6, None, "c128"),
     (12, 6, None, "c64"),
     (24, 12, None, "c128"),
@@ -141,9 +185,8 @@ TOL_3D = {"c128": 2e-15, "c64": 6e-7}
 
 
 def _tol3d(Ng, dtype):
-    """Stockham lengths: TOL_3D; direct-DFT lengths round like sqrt(Ng)."""
-    stockham = Ng in LENGTHS or (Ng in dict(MIXED) and dtype in dict(MIXED)[Ng])
-    return TOL_3D[dtype] * (1.0 if stockham else max(1.0, np.sqrt(Ng) / 2))
+    """Stockham lengths (2^a 3^b 5^c): TOL_3D; direct-DFT lengths round like sqrt(Ng)."""
+    return TOL_3D[dtype] * (1.0 if _smooth(Ng) else max(1.0, np.sqrt(Ng) / 2))
 
 
 def _blocks(d, dtype, seed):
