@@ -84,6 +84,7 @@ SIGNATURES = {
     "a2afft_make_plans": [_P, _P],
     "a2afft_forward": [_P, _P, _P],
     "a2afft_inverse": [_P, _P, _P],
+    "a2afft_graph_launch": [_P, _P, _I, _P],
     "a2afft_plan_destroy": [_P],
     "a2afft_execute_pass": [_P, _P, _I, _I, _P],
     "a2afft_plan_autotune": [_P, _P, _I, _P],

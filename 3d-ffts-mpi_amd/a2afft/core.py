@@ -141,6 +141,11 @@ class _Plan:
         self._check_buf(d_data)
         return L.check(L.lib().a2afft_inverse(self._h, _ptr(d_data), _stream()), "inverse")
 
+    def graph_launch(self, d_data, direction=-1):
+        """forward (-1) / inverse (+1) replayed from a hipGraph captured on first use for this buffer."""
+        self._check_buf(d_data)
+        return L.check(L.lib().a2afft_graph_launch(self._h, _ptr(d_data), direction, _stream()), "graph_launch")
+
     def autotune(self, work, iters=5):
         """Pick the fastest kernel variant per pass; `work` contents are destroyed."""
         self._check_buf(work)

@@ -269,7 +269,20 @@ struct a2afft_plan_s {
     KSel sel[3];        // kernel variant per fused pass
     uint32_t chunk;     // z columns per x/y chunk (0: whole passes)
     char report[4096];  // last autotune table
+    hipStream_t cap_stream = nullptr;                // graph capture stream (graph_launch)
+    hipGraphExec_t graph[2] = {nullptr, nullptr};    // forward, inverse
+    void* graph_data[2] = {nullptr, nullptr};        // the buffer each graph was captured for
 };
+
+// Graphs hold the kernel choices and arguments of their capture: any change
+// to the plan drops them, and the next graph_launch captures again.
+static void drop_plan_graphs(a2afft_plan_s* p) {
+    for (int d = 0; d < 2; ++d) {
+        if (p->graph[d]) (void)hipGraphExecDestroy(p->graph[d]);
+        p->graph[d] = nullptr;
+        p->graph_data[d] = nullptr;
+    }
+}
 
 // One kernel launch of a fused transform: the z pass, then the x and y
 // passes either whole or alternating over windows of `chunk` z columns, so a
@@ -528,6 +541,7 @@ extern "C" int a2afft_grid_plan_create(a2afft_plan* out, int nx, int ny, int nz,
 extern "C" int a2afft_make_plans(a2afft_plan p, void* scratch) {
     if (!p) return set_error(A2AFFT_E_INVALID_ARG, "make_plans: null plan");
     p->scratch = scratch;
+    drop_plan_graphs(p);
     return A2AFFT_OK;
 }
 
@@ -644,6 +658,7 @@ extern "C" int a2afft_plan_set_chunking(a2afft_plan p, int z_columns) {
         return set_error(A2AFFT_E_INVALID_ARG, "set_chunking: %d columns must divide nz and be a multiple of the "
                          "x and y tile widths", z_columns);
     p->chunk = uint32_t(z_columns);
+    drop_plan_graphs(p);
     return A2AFFT_OK;
 }
 
@@ -662,6 +677,7 @@ extern "C" int a2afft_plan_set_kernel(a2afft_plan p, int pass, int T, int E, int
                 return set_error(A2AFFT_E_INVALID_ARG, "set_kernel: T=%d does not divide the chunk width %u", T,
                                  p->chunk);
             }
+            drop_plan_graphs(p);
             return A2AFFT_OK;
         }
     return set_error(A2AFFT_E_UNSUPPORTED, "set_kernel: no built variant T=%d E=%d flags=%d fits pass %d", T, E,
@@ -741,6 +757,7 @@ extern "C" int a2afft_plan_autotune(a2afft_plan p, void* work, int iters, void* 
         }
         p->chunk = best_w;
     }
+    drop_plan_graphs(p);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (st < 0) return st;
@@ -769,7 +786,41 @@ extern "C" int a2afft_plan_describe(a2afft_plan p, char* buf, int len) {
     return A2AFFT_OK;
 }
 
+// Captures forward or inverse for `data` into a hipGraph on first use (on an
+// internal stream) and replays it on `stream`; a call for another buffer, or
+// after the plan's kernels or chunking change, captures again.
+extern "C" int a2afft_graph_launch(a2afft_plan p, void* data, int direction, void* stream) {
+    if (!p || !data || (direction != -1 && direction != 1))
+        return set_error(A2AFFT_E_INVALID_ARG, "graph_launch: bad arguments");
+    const int d = direction < 0 ? 0 : 1;
+    int warn = 0;
+    if (!p->graph[d] || p->graph_data[d] != data) {
+        if (!p->cap_stream) A2A_TRY(hip_check(int(hipStreamCreate(&p->cap_stream)), "graph stream"));
+        if (p->graph[d]) (void)hipGraphExecDestroy(p->graph[d]);
+        p->graph[d] = nullptr;
+        hipGraph_t g = nullptr;
+        A2A_TRY(hip_check(int(hipStreamBeginCapture(p->cap_stream, hipStreamCaptureModeThreadLocal)), "capture"));
+        const int st = execute(p, data, direction, p->cap_stream);
+        const int st2 = hip_check(int(hipStreamEndCapture(p->cap_stream, &g)), "end capture");
+        if (st < 0 || st2 < 0) {
+            if (g) (void)hipGraphDestroy(g);
+            return st < 0 ? st : st2;
+        }
+        warn = st;
+        const int st3 = hip_check(int(hipGraphInstantiate(&p->graph[d], g, nullptr, nullptr, 0)), "graph instantiate");
+        (void)hipGraphDestroy(g);
+        if (st3 < 0) return st3;
+        p->graph_data[d] = data;
+    }
+    const int st = finish_call(stream, hip_check(int(hipGraphLaunch(p->graph[d], (hipStream_t)stream)), "graph launch"));
+    return st < 0 ? st : warn;
+}
+
 extern "C" int a2afft_plan_destroy(a2afft_plan p) {
+    if (p) {
+        drop_plan_graphs(p);
+        if (p->cap_stream) (void)hipStreamDestroy(p->cap_stream);
+    }
     delete p;
     return A2AFFT_OK;
 }

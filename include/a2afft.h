@@ -188,6 +188,13 @@ int a2afft_make_plans(a2afft_plan plan, void* d_scratch);
  * block layout; inverse applies the 1/Ng^3 of scale_fft. */
 int a2afft_forward(a2afft_plan plan, void* d_data, void* stream);
 int a2afft_inverse(a2afft_plan plan, void* d_data, void* stream);
+/* forward (direction -1) or inverse (+1) as a hipGraph replay (no reference
+ * counterpart: launch overhead matters for small grids).  The first call for
+ * a buffer and direction captures the plan's launches on an internal stream;
+ * later calls replay it on `stream`.  A call for another buffer, or after
+ * make_plans / set_kernel / set_chunking / autotune, captures again.  Same
+ * bits as forward / inverse. */
+int a2afft_graph_launch(a2afft_plan plan, void* d_data, int direction, void* stream);
 int a2afft_plan_destroy(a2afft_plan plan);
 
 /* ---- batched grids (no reference counterpart) ---------------------------- */
