@@ -53,8 +53,8 @@ constexpr int kMaxPhases = 384;            // (group, pass) phases: at most 128 
 
 // Kernel shapes per (N, strided): the primary (first of its class in this
 // table), alternatives the autotuner may choose instead (the same tile
-// unpadded, half the tile), and a single-column fallback for z extents below
-// the tile width.  Alternatives keep the primary's elements per thread and
+// unpadded, half the tile, wider tiles for short lines), and a single-column
+// fallback for z extents below the tile width.  Alternatives keep the primary's elements per thread and
 // stay within its LDS, so the kernel's registers (103) and LDS (39.9 KB) are
 // those of the primaries.  Twice the elements per thread was measured slower
 // for every class (the 64-grid batch, 8 x 256^3, 15 x 256x64x512) and made
@@ -75,6 +75,9 @@ constexpr BShape kShapes[] = {
     // alternatives: half the tile (lines or columns), 128 threads
     {32, 0, 32, 8, 1}, {64, 0, 16, 8, 1}, {128, 0, 16, 16, 1}, {256, 0, 8, 16, 1}, {512, 0, 4, 16, 1},
     {64, 1, 16, 8, 1}, {128, 1, 16, 16, 1}, {256, 1, 8, 16, 1}, {512, 1, 4, 16, 1},
+    // alternatives: wider tiles for short lines (more columns or lines per unit)
+    {32, 1, 64, 8, 1}, {64, 1, 64, 16, 1}, {32, 1, 128, 16, 1}, {16, 1, 256, 16, 1}, {64, 0, 64, 16, 1},
+    {32, 0, 128, 16, 1},
 };
 constexpr int kNumShapes = int(sizeof(kShapes) / sizeof(kShapes[0]));
 constexpr int kFirstAlt = 18;
@@ -91,7 +94,7 @@ struct BatchItem {
     int phase;           // the (group, pass) phase the item belongs to
 };
 
-static_assert(kNumShapes == 37, "run_unit lists every shape");
+static_assert(kNumShapes == 43, "run_unit lists every shape");
 
 constexpr int shape_flags(int i) { return (kShapes[i].pad ? KF_PAD : 0) | KF_COH; }
 
@@ -108,7 +111,7 @@ __device__ __forceinline__ void run_unit(int shape, float2* data, const FftGeom&
         A2A_B(9) A2A_B(10) A2A_B(11) A2A_B(12) A2A_B(13) A2A_B(14) A2A_B(15) A2A_B(16) A2A_B(17)
         A2A_B(18) A2A_B(19) A2A_B(20) A2A_B(21) A2A_B(22) A2A_B(23) A2A_B(24) A2A_B(25) A2A_B(26)
         A2A_B(27) A2A_B(28) A2A_B(29) A2A_B(30) A2A_B(31) A2A_B(32) A2A_B(33) A2A_B(34) A2A_B(35)
-        A2A_B(36)
+        A2A_B(36) A2A_B(37) A2A_B(38) A2A_B(39) A2A_B(40) A2A_B(41) A2A_B(42)
 #undef A2A_B
     default:   // the host only stores valid shape indices
         __builtin_unreachable();
@@ -359,7 +362,7 @@ static int log2i(int n) {
 
 static bool shape_ok(int i, const AxisPass& p, int N) {
     const BShape& s = kShapes[i];
-    return s.N == N && s.strided == p.strided && shape_fits(p.fft, p.a2, p.strided, N, s.T, s.E);
+    return s.N == N && int(p.fft.n) == N && s.strided == p.strided && shape_fits(p.fft, p.a2, p.strided, N, s.T, s.E);
 }
 
 // The class's chosen shape if it fits the pass, else the primary, else the

@@ -107,7 +107,8 @@ def test_batch_autotune_keeps_bits(a2afft_mod, cuda):
 # (N, strided, T, E, pad): every alternative tile shape of the kernel's table
 ALT_SHAPES = [(32, 0, 64, 8, 0), (64, 0, 32, 8, 0), (128, 0, 32, 16, 0), (32, 0, 32, 8, 1), (64, 0, 16, 8, 1),
               (128, 0, 16, 16, 1), (32, 1, 32, 8, 0), (64, 1, 32, 8, 0), (128, 1, 32, 16, 0), (64, 1, 16, 8, 1),
-              (128, 1, 16, 16, 1)]
+              (128, 1, 16, 16, 1), (32, 1, 64, 8, 1), (64, 1, 64, 16, 1), (32, 1, 128, 16, 1), (16, 1, 256, 16, 1),
+              (64, 0, 64, 16, 1), (32, 0, 128, 16, 1)]
 ALT_SHAPES_BIG = [(256, 0, 16, 16, 0), (512, 0, 8, 16, 0), (256, 0, 8, 16, 1), (512, 0, 4, 16, 1),
                   (256, 1, 16, 16, 0), (512, 1, 8, 16, 0), (256, 1, 8, 16, 1), (512, 1, 4, 16, 1)]
 
@@ -115,7 +116,8 @@ ALT_SHAPES_BIG = [(256, 0, 16, 16, 0), (512, 0, 8, 16, 0), (256, 0, 8, 16, 1), (
 def test_batch_tile_shapes_keep_bits(a2afft_mod, cuda):
     """Every alternative tile shape computes the primary's bits."""
     import torch
-    for shapes, alts in ((SHAPES, ALT_SHAPES), ([(512, 256, 512), (256, 512, 256)], ALT_SHAPES_BIG)):
+    wide = [(16, 32, 256), (64, 64, 256), (32, 128, 256)]   # z extents that the wide strided tiles fit
+    for shapes, alts in ((SHAPES + wide, ALT_SHAPES), ([(512, 256, 512), (256, 512, 256)], ALT_SHAPES_BIG)):
         b = a2afft_mod.Batch(shapes)
         x = _fill(a2afft_mod, torch, b, 13)
         ref = x.clone()
