@@ -685,8 +685,10 @@ extern "C" int a2afft_plan_set_kernel(a2afft_plan p, int pass, int T, int E, int
 }
 
 // Times every built variant that fits each fused pass (forward direction,
-// `iters` launches each, in place on `work`, whose contents are destroyed)
-// and keeps the fastest.  Every variant computes identical bits.
+// `iters` launches each, in place on `work`, whose contents are destroyed),
+// keeps the fastest per pass, then the fastest chunk width, and keeps the
+// result only if whole forwards beat the plan's choices before the call.
+// Every variant computes identical bits.
 extern "C" int a2afft_plan_autotune(a2afft_plan p, void* work, int iters, void* stream) {
     if (!p || !work || iters <= 0) return set_error(A2AFFT_E_INVALID_ARG, "autotune: bad arguments");
     if (p->exec != A2AFFT_EXEC_FUSED) return set_error(A2AFFT_E_INVALID_ARG, "autotune: fused plans only");
@@ -700,6 +702,19 @@ extern "C" int a2afft_plan_autotune(a2afft_plan p, void* work, int iters, void* 
     int len = 0;
     const char* names[3] = {"z", "x", "y"};
     int st = A2AFFT_OK;
+    const KSel start_sel[3] = {p->sel[0], p->sel[1], p->sel[2]};
+    const uint32_t start_chunk = p->chunk;
+    // mean ms of `reps` whole forwards with the plan's current choices
+    auto time_forward = [&](int reps, float* ms) -> int {
+        A2A_TRY(exec_fused(p, work, -1, s));
+        A2A_TRY(hip_check(int(hipEventRecord(e0, s)), "autotune record"));
+        for (int it = 0; it < reps; ++it) A2A_TRY(exec_fused(p, work, -1, s));
+        A2A_TRY(hip_check(int(hipEventRecord(e1, s)), "autotune record"));
+        A2A_TRY(hip_check(int(hipEventSynchronize(e1)), "autotune sync"));
+        A2A_TRY(hip_check(int(hipEventElapsedTime(ms, e0, e1)), "autotune elapsed"));
+        *ms /= float(reps);
+        return A2AFFT_OK;
+    };
     for (int i = 0; i < 3 && st >= 0 && !is_split(p->prec); ++i) {
         KSel cand[256];
         const int N = int(p->pass[i].fft.n);
@@ -756,6 +771,28 @@ extern "C" int a2afft_plan_autotune(a2afft_plan p, void* work, int iters, void* 
                 len += snprintf(p->report + len, sizeof(p->report) - len, "forward chunk=%u %.4f ms\n", cands[k], ms);
         }
         p->chunk = best_w;
+    }
+    // Passes timed one by one can mislead where launches, not bytes, dominate
+    // (small grids): keep the tuned choice only if whole forwards confirm it.
+    if (st >= 0) {
+        float tuned = 0.f, start = 0.f;
+        const KSel tuned_sel[3] = {p->sel[0], p->sel[1], p->sel[2]};
+        const uint32_t tuned_chunk = p->chunk;
+        st = time_forward(2 * iters, &tuned);
+        for (int i = 0; i < 3 && st >= 0; ++i) p->sel[i] = start_sel[i];
+        p->chunk = start_chunk;
+        if (st >= 0) st = time_forward(2 * iters, &start);
+        if (st >= 0 && tuned < start) {
+            for (int i = 0; i < 3; ++i) p->sel[i] = tuned_sel[i];
+            p->chunk = tuned_chunk;
+        }
+        if (st >= 0 && len < int(sizeof(p->report)) - 96)
+            len += snprintf(p->report + len, sizeof(p->report) - len, "forward tuned %.4f ms, before %.4f ms: kept %s\n",
+                            tuned, start, tuned < start ? "tuned" : "before");
+    }
+    if (st < 0) {   // leave the plan as it was
+        for (int i = 0; i < 3; ++i) p->sel[i] = start_sel[i];
+        p->chunk = start_chunk;
     }
     drop_plan_graphs(p);
     (void)hipEventDestroy(e0);

@@ -34,6 +34,9 @@ int register_kernels_f32(KEntry* out) {
     add_shape<float, 512, 16, 16, false>(out, &n);   // 512 threads, 64 KiB LDS
     add_extra_flags<float, 512, 16, 16, false>(out, &n);
     add_shape<float, 512, 32, 16, false>(out, &n);   // 256 threads, 64 KiB LDS
+    add_shape<float, 64, 16, 64, true>(out, &n);    // 256 threads, 512 B rows
+    add_shape<float, 32, 16, 128, true>(out, &n);   // 256 threads, 1 KiB rows
+    add_shape<float, 128, 16, 32, true>(out, &n);   // 256 threads, 256 B rows
     add_shape<float, 256, 16, 8, true>(out, &n);
     add_shape<float, 256, 16, 32, true>(out, &n);
     add_shape<float, 256, 32, 32, true>(out, &n);

@@ -264,7 +264,9 @@ int a2afft_execute_pass(a2afft_plan plan, void* d_data, int pass, int direction,
  * LDS, 4 non-temporal loads/stores, 8 z-tiles-outermost unit order.
  * autotune times each built variant that fits each fused pass, `iters`
  * launches in place on `work` (a [P][nlocal] buffer whose contents are
- * destroyed), and keeps the fastest; set_kernel picks one by hand. */
+ * destroyed), keeps the fastest per pass and then the fastest chunk width,
+ * and keeps the result only if whole forwards beat the choices it started
+ * from; set_kernel picks one by hand. */
 int a2afft_plan_autotune(a2afft_plan plan, void* work, int iters, void* stream);
 /* Process-wide tuning cache file (NULL: none); no reference counterpart (the
  * reference re-plans on every call, a2affts.cu:15).  a2afft_plan_autotune
