@@ -90,6 +90,7 @@ SIGNATURES = {
     "a2afft_plan_autotune": [_P, _P, _I, _P],
     "a2afft_plan_set_kernel": [_P, _I, _I, _I, _I],
     "a2afft_plan_describe": [_P, ctypes.c_char_p, _I],
+    "a2afft_plan_pattern_copy": [_P, _P, _I, _I, ctypes.POINTER(ctypes.c_float), _P],
     "a2afft_plan_set_chunking": [_P, _I],
     "a2afft_plan_num_launches": [_P],
     "a2afft_execute_launch": [_P, _P, _I, _I, _P],

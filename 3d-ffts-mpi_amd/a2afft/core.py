@@ -173,6 +173,16 @@ class _Plan:
         L.check(L.lib().a2afft_plan_describe(self._h, buf, len(buf)), "describe")
         return buf.value.decode()
 
+    def pattern_copy(self, d_data, pass_index, iters=10):
+        """Mean ms of a copy kernel with fused pass `pass_index`'s (0 z, 1 x,
+        2 y) access pattern, tile order, cache policy and LDS footprint; the
+        data is left unchanged."""
+        self._check_buf(d_data)
+        ms = ctypes.c_float()
+        L.check(L.lib().a2afft_plan_pattern_copy(self._h, _ptr(d_data), pass_index, iters, ctypes.byref(ms),
+                                                 _stream()), "pattern_copy")
+        return ms.value
+
     def destroy(self):
         if self._h:
             L.lib().a2afft_plan_destroy(self._h)

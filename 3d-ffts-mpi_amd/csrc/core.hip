@@ -823,6 +823,35 @@ extern "C" int a2afft_plan_describe(a2afft_plan p, char* buf, int len) {
     return A2AFFT_OK;
 }
 
+// Pattern copy of one fused pass: mean ms of `iters` launches of a copy
+// kernel with the pass kernel's addresses, tile order, cache policy and LDS
+// footprint, in place and leaving `data` unchanged.
+extern "C" int a2afft_plan_pattern_copy(a2afft_plan p, void* data, int pass, int iters, float* ms, void* stream) {
+    if (!p || !data || !ms || pass < 0 || pass > 2 || iters <= 0)
+        return set_error(A2AFFT_E_INVALID_ARG, "pattern copy: bad arguments");
+    if (p->exec != A2AFFT_EXEC_FUSED || is_split(p->prec))
+        return set_error(A2AFFT_E_UNSUPPORTED, "pattern copy: fused FP32/FP64 plans only");
+    hipStream_t s = (hipStream_t)stream;
+    const AxisPass& ap = p->pass[pass];
+    const int N = int(ap.fft.n);
+    A2A_TRY(launch_pattern_copy(ap, data, p->d.dtype, N, p->sel[pass], s));   // warm-up
+    hipEvent_t e0, e1;
+    A2A_TRY(hip_check(int(hipEventCreate(&e0)), "pattern copy event"));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return set_error(A2AFFT_E_HIP, "pattern copy event");
+    }
+    int st = hip_check(int(hipEventRecord(e0, s)), "pattern copy record");
+    for (int it = 0; it < iters && st >= 0; ++it) st = launch_pattern_copy(ap, data, p->d.dtype, N, p->sel[pass], s);
+    if (st >= 0) st = hip_check(int(hipEventRecord(e1, s)), "pattern copy record");
+    if (st >= 0) st = hip_check(int(hipEventSynchronize(e1)), "pattern copy sync");
+    if (st >= 0) st = hip_check(int(hipEventElapsedTime(ms, e0, e1)), "pattern copy elapsed");
+    if (st >= 0) *ms /= float(iters);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return st;
+}
+
 // Captures forward or inverse for `data` into a hipGraph on first use (on an
 // internal stream) and replays it on `stream`; a call for another buffer, or
 // after the plan's kernels or chunking change, captures again.

@@ -45,6 +45,11 @@ const void* pass_twiddles(const AxisPass& pass, int N, int dir, a2afft_dtype dty
 int launch_fft_pass(const AxisPass& pass, void* data, a2afft_dtype dtype, int N, int dir, hipStream_t s,
                     const KSel* sel);
 
+// A copy kernel with the access pattern of the Stockham pass `sel` would run
+// (pattern.hip): that pass's memory traffic without its arithmetic.
+// Fixed-schedule Stockham selections only.
+int launch_pattern_copy(const AxisPass& pass, void* data, a2afft_dtype dtype, int N, const KSel& sel, hipStream_t s);
+
 // Stockham FFT with the radix schedule in the kernel arguments (KF_MIX,
 // fft_mixed.hip) for 2^a 3^b 5^c lengths the fixed schedules do not cover.
 constexpr int kMixMaxPasses = 16;

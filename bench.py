@@ -425,6 +425,12 @@ def main():
     copy_ms = c0.elapsed_time(c1) / 10
     del copy_dst
 
+    # each pass's access pattern alone: a copy kernel with the pass kernel's
+    # addresses, tile order, cache policy and LDS footprint (no arithmetic)
+    pattern_ms = None
+    if args.precision in ("fp32", "fp64"):
+        pattern_ms = [plan.pattern_copy(data, p, 10) for p in range(3)]
+
     bytes_pass = 2 * N * esize
     names = ["z", "x", "y"]
     dom = max(range(3), key=lambda p: pass_ms[p])
@@ -474,6 +480,10 @@ def main():
         "pass_gbs": {names[p]: round(bytes_pass / (pass_ms[p] * 1e-3) / 1e9, 1) for p in range(3)},
         "launches": {names[p]: nper[p] for p in range(3)},
         "copy_gbs": round(bytes_pass / (copy_ms * 1e-3) / 1e9, 1),
+        "pattern_copy_gbs": ({names[p]: round(bytes_pass / (pattern_ms[p] * 1e-3) / 1e9, 1) for p in range(3)}
+                             if pattern_ms else None),
+        "pass_vs_pattern_copy": ({names[p]: round(pattern_ms[p] / pass_ms[p], 3) for p in range(3)}
+                                 if pattern_ms else None),
         "event_ms_per_step": round(ev_ms, 4),
         "rel_l2_vs_fp64": rel_l2,
         "max_abs_vs_fp64": max_abs,

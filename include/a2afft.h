@@ -289,6 +289,15 @@ int a2afft_plan_num_launches(a2afft_plan plan);
 int a2afft_execute_launch(a2afft_plan plan, void* d_data, int index, int direction, void* stream);
 /* Writes the chosen variant per pass and the last autotune timings as text. */
 int a2afft_plan_describe(a2afft_plan plan, char* buf, int len);
+/* Pattern copy of fused pass `pass` (0 = z, 1 = x, 2 = y): the mean
+ * milliseconds over `iters` launches of a copy kernel that reads and writes
+ * every element of the pass in place with the pass kernel's addresses, tile
+ * order, cache policy and LDS footprint, but no arithmetic; `data` is left
+ * unchanged.  Fused FP32/FP64 plans whose pass runs a fixed-schedule Stockham
+ * kernel; A2AFFT_E_UNSUPPORTED otherwise.  No counterpart in the reference: a
+ * measurement aid that sets each pass's time beside the copy time of its own
+ * access pattern. */
+int a2afft_plan_pattern_copy(a2afft_plan plan, void* d_data, int pass, int iters, float* ms, void* stream);
 
 /* Seeded synthetic input (no reference counterpart: the reference's test
  * fills with an unseeded rand(), tests/test.cpp:12-17).  Philox4x32-10 with

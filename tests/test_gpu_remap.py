@@ -95,18 +95,28 @@ def test_stages_bit_exact(a2afft_mod, cuda, cfg, dtype):
             assert st == (0 if dist.reference_domain else 1)
 
 
-def test_copy_and_scale(a2afft_mod, cuda):
+@pytest.mark.parametrize("n,off", [(1000, 0), (1001, 0), (1, 0), (4099, 1), (1000, 1), (3, 1)])
+def test_copy_and_scale(a2afft_mod, cuda, n, off):
+    """Counts and offsets that take the 16-byte lanes (complex128, even
+    complex64 counts at aligned addresses) and the 8-byte forms (odd complex64
+    counts, views starting 8 bytes into an allocation)."""
     import torch
-    rng = np.random.default_rng(3)
+    rng = np.random.default_rng(3 + n + off)
     for dt in (np.complex128, np.complex64):
-        a = (rng.standard_normal(1000) + 1j * rng.standard_normal(1000)).astype(dt)
-        x = _to_dev(a)
-        y = torch.zeros_like(x)
+        a = (rng.standard_normal(n + off) + 1j * rng.standard_normal(n + off)).astype(dt)
+        xb = _to_dev(a)
+        yb = torch.zeros_like(xb)
+        x, y = xb[off:], yb[off:]
         a2afft_mod.fast_copy_fft(y, x)
         torch.cuda.synchronize()
-        assert np.array_equal(_bits(y.cpu().numpy()), _bits(a))
+        out = yb.cpu().numpy()
+        assert np.array_equal(_bits(out[off:]), _bits(a[off:]))
+        assert not out[:off].any()                   # nothing written before the view
+        yb.zero_()
         a2afft_mod.scale_fft(y, x, 12)
         torch.cuda.synchronize()
         # scale_fft divides each component by (double)(Ng^3) (a2affts.cu:83-88)
         ref = (a.real / dt(0).real.dtype.type(12 ** 3)) + 1j * (a.imag / dt(0).real.dtype.type(12 ** 3))
-        assert np.array_equal(_bits(y.cpu().numpy()), _bits(ref.astype(dt)))
+        out = yb.cpu().numpy()
+        assert np.array_equal(_bits(out[off:]), _bits(ref[off:].astype(dt)))
+        assert not out[:off].any()
