@@ -154,16 +154,16 @@ struct Pref {
 };
 static const Pref kPrefs[] = {
     {512, 0, 0, -1, {16, 16, KF_XCD | KF_PAD | KF_NT}},
-    {512, 0, 1, 1, {16, 32, KF_A2MAJ}},
-    {512, 0, 1, -1, {16, 32, KF_XCD | KF_PAD | KF_NT}},
-    {512, 1, 0, -1, {8, 16, KF_PAD}},
-    {512, 1, 1, 1, {8, 16, KF_XCD | KF_NT | KF_A2MAJ}},
+    {512, 0, 1, 1, {16, 32, KF_A2MAJ | KF_HALFX}},
+    {512, 0, 1, -1, {16, 32, KF_XCD | KF_PAD | KF_NT | KF_HALFX}},
+    {512, 1, 0, -1, {8, 16, KF_XCD | KF_PAD | KF_HALFX}},
+    {512, 1, 1, 1, {8, 16, KF_A2MAJ | KF_HALFX}},
     {512, 1, 1, -1, {8, 16, KF_XCD | KF_NT}},
     {256, 0, 1, 1, {32, 32, KF_XCD | KF_PAD}},
     {256, 0, 1, -1, {16, 32, KF_XCD}},
-    {256, 1, 0, -1, {8, 16, KF_XCD | KF_PAD}},
-    {256, 1, 1, 1, {8, 16, KF_XCD | KF_A2MAJ}},
-    {256, 1, 1, -1, {4, 16, KF_XCD | KF_PAD}},
+    {256, 1, 0, -1, {16, 16, KF_XCD | KF_PAD | KF_HALFX}},
+    {256, 1, 1, 1, {8, 16, KF_XCD | KF_A2MAJ | KF_HALFX}},
+    {256, 1, 1, -1, {8, 16, KF_XCD | KF_PAD | KF_HALFX}},
     {2048, 0, 0, -1, {2, 16, KF_XCD | KF_PAD | KF_NT}},
     {4096, 0, 0, -1, {1, 16, KF_PAD}},
 };

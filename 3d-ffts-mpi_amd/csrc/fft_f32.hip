@@ -46,6 +46,10 @@ int register_kernels_f32(KEntry* out) {
     add_extra_flags<float, 2048, 16, 2, false>(out, &n);   // the contiguous default shape
     add_shape<float, 2048, 16, 1, false>(out, &n);   // 128 threads, 16 KiB LDS
     add_shape<float, 2048, 32, 8, true>(out, &n);    // 512 threads, 128 KiB LDS, 64 B rows
+    // half-width exchanges for the 512 and 2048 default shapes
+    add_halfx<float, 512, 32, 16, true>(out, &n);
+    add_halfx<float, 512, 16, 16, false>(out, &n);
+    add_halfx<float, 2048, 16, 2, false>(out, &n);
     return n;
 }
 

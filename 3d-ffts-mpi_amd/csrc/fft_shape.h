@@ -152,6 +152,7 @@ enum : int {
     KF_NT = 4,     // non-temporal global loads and stores
     KF_A2MAJ = 8,  // units numbered a2-major (z tiles outermost)
     KF_COH = 16,   // device-coherent (L2-bypassing) global accesses, for in-launch hand-offs
+    KF_HALFX = 32, // LDS exchanges move one component at a time: half the LDS, two more barriers
     KF_DFT = 64,   // direct O(N^2) DFT, one line per workgroup: the lengths no Stockham kernel covers
     KF_MIX = 128,  // Stockham FFT with the radix schedule in the arguments: long 2^a 3^b 5^c lengths
 };

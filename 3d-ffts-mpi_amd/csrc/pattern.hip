@@ -75,7 +75,9 @@ int launch_pattern_copy(const AxisPass& p, void* data, a2afft_dtype dt, int N, c
                          : (nt ? pattern_kernel_for<CT<float>::V, true>(ks.E, threads)
                                : pattern_kernel_for<CT<float>::V, false>(ks.E, threads));
     if (!fn) return set_error(A2AFFT_E_UNSUPPORTED, "pattern copy: E=%d, %d threads", ks.E, threads);
-    const size_t lds = size_t(lds_slots_rt(N, ks.T, p.strided, (ks.flags & KF_PAD) ? 1 : 0)) * (dbl ? 16u : 8u);
+    // a complex value per slot, or one component with KF_HALFX
+    const size_t slot_bytes = (dbl ? 16u : 8u) / ((ks.flags & KF_HALFX) ? 2u : 1u);
+    const size_t lds = size_t(lds_slots_rt(N, ks.T, p.strided, (ks.flags & KF_PAD) ? 1 : 0)) * slot_bytes;
     int T = ks.T, strided = p.strided, flags = ks.flags;
     void* args[] = {&data, &g, &T, (void*)&TPL, &strided, &flags};
     A2A_TRY(hip_check(int(hipLaunchKernel(fn, dim3(grid), dim3(threads), args, lds, s)), "pattern copy launch"));

@@ -24,6 +24,8 @@
 // so a line is computed bit-identically whichever shape or launch computes it.
 #pragma once
 
+#include <type_traits>
+
 #include "common.h"
 #include "fft_shape.h"
 
@@ -262,11 +264,48 @@ template <bool NTMP, bool COH = false, typename V> __device__ __forceinline__ vo
     }
 }
 
+// One slot of an LDS exchange: the complex value (C = 2), its real part
+// (C = 0) or its imaginary part (C = 1).
+template <int C, typename L, typename V> __device__ __forceinline__ void lds_put(L& slot, const V& v) {
+    if constexpr (C == 2) slot = v;
+    else if constexpr (C == 0) slot = v.x;
+    else slot = v.y;
+}
+template <int C, typename L, typename V> __device__ __forceinline__ void lds_get(const L& slot, V& v) {
+    if constexpr (C == 2) v = slot;
+    else if constexpr (C == 0) v.x = slot;
+    else v.y = slot;
+}
+
+// The Stockham write of a pass's outputs (radix R, Ns = product of the
+// earlier radices) and the read of the next pass's inputs, through LDS, for
+// component C of every value.
+template <int C, int N, int E, int T, bool STRIDED, int PAD, int R, int Ns, typename V, typename L>
+__device__ __forceinline__ void stockham_exchange(V (&v)[E], L* lds, int t, int line, bool active) {
+    constexpr int TPL = N / E;
+    constexpr int NB = E / R;
+    if (active) {
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            const int j = t + b * TPL;
+            const int d0 = int(unsigned(j) / unsigned(Ns)) * (Ns * R) + int(unsigned(j) % unsigned(Ns));
+#pragma unroll
+            for (int q = 0; q < R; ++q) lds_put<C>(lds[lds_slot<N, T, STRIDED, PAD>(line, d0 + q * Ns)], v[b + q * NB]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < E; ++e) lds_get<C>(lds[lds_slot<N, T, STRIDED, PAD>(line, t + e * TPL)], v[e]);
+}
+
 // Passes p..NP-1 of the Stockham schedule on the thread's registers.
 // `active` = false for threads beyond the unit's NT (a wider workgroup runs
-// the unit): they take part in the barriers but write nothing to LDS.
-template <typename real, int N, int E, int T, bool STRIDED, int DIR, int PAD, int p>
-__device__ __forceinline__ void stockham_passes(typename CT<real>::V (&v)[E], typename CT<real>::V* lds,
+// the unit): they take part in the barriers but write nothing to LDS.  `lds`
+// holds complex values (L = V), or one real component per slot (L = real):
+// then each exchange moves the real parts, then the imaginary parts, through
+// half as much LDS, so more workgroups fit a CU.
+template <typename real, int N, int E, int T, bool STRIDED, int DIR, int PAD, int p, typename L>
+__device__ __forceinline__ void stockham_passes(typename CT<real>::V (&v)[E], L* lds,
                                                 const typename CT<real>::V* __restrict__ tw, int t, int line,
                                                 bool active) {
     using V = typename CT<real>::V;
@@ -295,18 +334,13 @@ __device__ __forceinline__ void stockham_passes(typename CT<real>::V (&v)[E], ty
         }
         if constexpr (p + 1 < NP) {
             if constexpr (p > 0) __syncthreads();
-            if (active) {
-#pragma unroll
-                for (int b = 0; b < NB; ++b) {
-                    const int j = t + b * TPL;
-                    const int d0 = int(unsigned(j) / unsigned(Ns)) * (Ns * R) + int(unsigned(j) % unsigned(Ns));
-#pragma unroll
-                    for (int q = 0; q < R; ++q) lds[lds_slot<N, T, STRIDED, PAD>(line, d0 + q * Ns)] = v[b + q * NB];
-                }
+            if constexpr (sizeof(L) == sizeof(V)) {
+                stockham_exchange<2, N, E, T, STRIDED, PAD, R, Ns>(v, lds, t, line, active);
+            } else {
+                stockham_exchange<0, N, E, T, STRIDED, PAD, R, Ns>(v, lds, t, line, active);
+                __syncthreads();
+                stockham_exchange<1, N, E, T, STRIDED, PAD, R, Ns>(v, lds, t, line, active);
             }
-            __syncthreads();
-#pragma unroll
-            for (int e = 0; e < E; ++e) v[e] = lds[lds_slot<N, T, STRIDED, PAD>(line, t + e * TPL)];
         }
         stockham_passes<real, N, E, T, STRIDED, DIR, PAD, p + 1>(v, lds, tw, t, line, active);
     }
@@ -322,10 +356,10 @@ struct NoHook {
 // with tid >= NT take part in the barriers only.  Every thread calls `hook`
 // once, after the unit's loads have been consumed and before its stores are
 // issued.
-template <typename real, int N, int E, int T, bool STRIDED, int DIR, int FLAGS, typename Hook = NoHook>
+template <typename real, int N, int E, int T, bool STRIDED, int DIR, int FLAGS, typename L, typename Hook = NoHook>
 __device__ __forceinline__ void stockham_unit(typename CT<real>::V* __restrict__ data, const FftGeom& g,
-                                              const typename CT<real>::V* __restrict__ tw, uint32_t block,
-                                              typename CT<real>::V* lds, int tid, Hook&& hook = Hook{}) {
+                                              const typename CT<real>::V* __restrict__ tw, uint32_t block, L* lds,
+                                              int tid, Hook&& hook = Hook{}) {
     using V = typename CT<real>::V;
     constexpr int TPL = N / E;
     constexpr int NT = T * TPL;
@@ -374,7 +408,9 @@ __global__ __launch_bounds__(T * (N / E)) void stockham_kernel(typename CT<real>
                                                                const typename CT<real>::V* __restrict__ tw) {
     using V = typename CT<real>::V;
     constexpr int PAD = (FLAGS & KF_PAD) ? 1 : 0;
-    __shared__ V lds[lds_slots<N, T, STRIDED, PAD>()];
+    // complex slots, or one real component per slot with KF_HALFX
+    using L = typename std::conditional<(FLAGS & KF_HALFX) != 0, real, V>::type;
+    __shared__ L lds[lds_slots<N, T, STRIDED, PAD>()];
     const uint32_t block = (FLAGS & KF_XCD) ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
     stockham_unit<real, N, E, T, STRIDED, DIR, FLAGS>(data, g, tw, block, lds, int(threadIdx.x));
 }
@@ -412,6 +448,17 @@ inline void add_extra_flags(KEntry* out, int* n) {
     add_variant<real, N, E, T, S, KF_XCD | KF_PAD | KF_A2MAJ>(out, n);
     add_variant<real, N, E, T, S, KF_A2MAJ>(out, n);
     add_variant<real, N, E, T, S, KF_XCD | KF_NT | KF_A2MAJ>(out, n);
+}
+
+// Half-width LDS exchange variants of a shape (KF_HALFX): the same tiles with
+// twice the workgroups per CU where LDS limits occupancy.
+template <typename real, int N, int E, int T, bool S>
+inline void add_halfx(KEntry* out, int* n) {
+    add_variant<real, N, E, T, S, KF_XCD | KF_HALFX>(out, n);
+    add_variant<real, N, E, T, S, KF_XCD | KF_PAD | KF_HALFX>(out, n);
+    add_variant<real, N, E, T, S, KF_XCD | KF_PAD | KF_NT | KF_HALFX>(out, n);
+    add_variant<real, N, E, T, S, KF_A2MAJ | KF_HALFX>(out, n);
+    add_variant<real, N, E, T, S, KF_XCD | KF_A2MAJ | KF_HALFX>(out, n);
 }
 
 // Mixed lengths 2^a 3^b 5^c: one thread per line (E = N), contiguous tiles of

@@ -307,3 +307,39 @@ def test_grid_plan_non_cubic_vs_numpy(a2afft_mod, cuda, shape, prec):
     plan.inverse(d)
     torch.cuda.synchronize()
     assert _relerr(d.cpu().numpy(), x.astype(np.complex128)) < 2 * tol
+
+
+# (grid shape, precision, [(pass, T, E)]): the shapes built with half-width
+# LDS exchanges (KF_HALFX) on every pass of a grid that has them
+HALFX_CASES = [((256, 256, 256), "fp64", [(0, 8, 16), (0, 16, 16), (1, 8, 16), (1, 4, 16), (2, 8, 16)]),
+               ((512, 16, 512), "fp64", [(0, 8, 16), (1, 8, 16)]),
+               ((512, 16, 512), "fp32", [(0, 16, 16), (1, 16, 32)]),
+               ((16, 16, 2048), "fp32", [(0, 2, 16)])]
+HALFX_FLAGS = (32 | 1, 32 | 1 | 2, 32 | 1 | 2 | 4, 32 | 8, 32 | 8 | 1)
+
+
+@pytest.mark.parametrize("case", HALFX_CASES, ids=lambda c: f"{c[0]}_{c[1]}")
+def test_halfx_variants_bit_identical(a2afft_mod, cuda, case):
+    """Half-width LDS exchanges move the same values in two steps: every such
+    variant, forced on its pass, reproduces the default plan bit for bit."""
+    import torch
+    shape, prec, shapes = case
+    dt = torch.complex128 if prec == "fp64" else torch.complex64
+    x = torch.empty(shape, dtype=dt, device="cuda")
+    a2afft_mod.fill_random(x, 11, "uniform")
+    base = a2afft_mod.GridDfft(shape, prec)
+    ref = x.clone()
+    base.forward(ref)
+    torch.cuda.synchronize()
+    ref_bits = _bits(ref.cpu().numpy())
+    ran = 0
+    for (p, T, E) in shapes:
+        for flags in HALFX_FLAGS:
+            plan = a2afft_mod.GridDfft(shape, prec)
+            plan.set_kernel(p, T, E, flags)
+            y = x.clone()
+            plan.forward(y)
+            torch.cuda.synchronize()
+            assert np.array_equal(_bits(y.cpu().numpy()), ref_bits), (p, T, E, flags)
+            ran += 1
+    assert ran == len(shapes) * len(HALFX_FLAGS)
