@@ -152,7 +152,8 @@ class _Plan:
         return L.check(L.lib().a2afft_plan_autotune(self._h, _ptr(work), iters, _stream()), "autotune")
 
     def set_kernel(self, pass_index, T, E, flags=0):
-        """Force a kernel variant: flags 1 XCD order, 2 LDS pad, 4 non-temporal, 8 z-tiles outermost."""
+        """Force a kernel variant: flags 1 XCD order, 2 LDS pad, 4 non-temporal, 8 z-tiles outermost,
+        32 half-width LDS exchanges."""
         return L.check(L.lib().a2afft_plan_set_kernel(self._h, pass_index, T, E, flags), "set_kernel")
 
     def set_chunking(self, z_columns):

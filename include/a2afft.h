@@ -261,7 +261,8 @@ int a2afft_execute_pass(a2afft_plan plan, void* d_data, int pass, int direction,
 /* Kernel variants (no reference counterpart: cuFFT chose its own kernels).
  * Every variant of a pass computes identical bits; they differ in tile width
  * T, elements per thread E and flags: 1 XCD-aware workgroup order, 2 padded
- * LDS, 4 non-temporal loads/stores, 8 z-tiles-outermost unit order.
+ * LDS, 4 non-temporal loads/stores, 8 z-tiles-outermost unit order, 32 LDS
+ * exchanges one component at a time (half the LDS per workgroup).
  * autotune times each built variant that fits each fused pass, `iters`
  * launches in place on `work` (a [P][nlocal] buffer whose contents are
  * destroyed), keeps the fastest per pass and then the fastest chunk width,
@@ -281,7 +282,10 @@ int a2afft_plan_set_kernel(a2afft_plan plan, int pass, int T, int E, int flags);
  * windows of `z_columns` z columns (0 = whole passes), so each window is
  * re-read from the Infinity Cache.  Results are identical for every width;
  * the width must divide Ng and be a multiple of both strided tile widths.
- * autotune also times whole forwards over the valid widths. */
+ * A new plan starts at 0, or, for complex64 grids above 512 MiB whose x and
+ * y passes run the 512-point half-width-exchange tiles, at the widest window
+ * of at most 256 MiB.  autotune also times whole forwards over the valid
+ * widths. */
 int a2afft_plan_set_chunking(a2afft_plan plan, int z_columns);
 /* Launches in one fused forward / inverse, and launch `index` alone (returns
  * its pass: 0 z, 1 x, 2 y), for per-kernel timing. */
