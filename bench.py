@@ -83,9 +83,11 @@ def _traffic(key):
         return None
 
 
-def _events_ms(torch, fn, steps):
-    """Average ms of fn() over `steps` calls, HIP events on the launch stream."""
-    fn()
+def _events_ms(torch, fn, steps, warmup=1):
+    """Average ms of fn() over `steps` calls after `warmup` untimed ones, HIP
+    events on the launch stream."""
+    for _ in range(max(1, warmup)):
+        fn()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     e0.record()
@@ -140,9 +142,9 @@ def run_c3b(args, torch, a2afft):
         a2afft.forward_1d_fft(tmp, 2048)
         a2afft.launch_remap(3, tmp, work, P, nloc, lgs, dims, nbuffers=1)
 
-    t_staged = _events_ms(torch, staged, args.steps)
-    t_pencil = _events_ms(torch, lambda: a2afft.forward_1d_fft(work, 2048), args.steps)
-    t_block = _events_ms(torch, lambda: a2afft.execute_pass(plan, wb, 1, -1), args.steps)
+    t_staged = _events_ms(torch, staged, args.steps, args.warmup)
+    t_pencil = _events_ms(torch, lambda: a2afft.forward_1d_fft(work, 2048), args.steps, args.warmup)
+    t_block = _events_ms(torch, lambda: a2afft.execute_pass(plan, wb, 1, -1), args.steps, args.warmup)
     best = min(t_pencil, t_block)
     bytes_fft = 2 * nloc * 8
     # the same access patterns without arithmetic: the contiguous 2048-point
@@ -153,7 +155,7 @@ def run_c3b(args, torch, a2afft):
     ach = bytes_fft / (t_pencil * 1e-3) / 1e9
     out = {
         "metric": "ms per x stage of one 2048x256x256 rank share (c2c, 2048-point lines)",
-        "value": round(best, 4), "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": 1,
+        "value": round(best, 4), "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(best, 4), "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic",
         "config": {"workload": "one rank's 2048x256x256 complex64 share of 2048^3 over dims [1,8,8], "
@@ -210,8 +212,8 @@ def run_c5(args, torch, a2afft):
     for line in b.describe().splitlines():
         log(f"[bench] batch {line}")
     sched = b.schedule()
-    t_eager = _events_ms(torch, lambda: b.forward(y), args.steps)
-    t_graph = _events_ms(torch, lambda: b.graph_launch(y, -1), args.steps)
+    t_eager = _events_ms(torch, lambda: b.forward(y), args.steps, args.warmup)
+    t_graph = _events_ms(torch, lambda: b.graph_launch(y, -1), args.steps, args.warmup)
     assert b.status() == 0
     plans = {s: a2afft.GridDfft(s, "fp32") for s in set(shapes)}
     views = b.views(y)
@@ -220,7 +222,7 @@ def run_c5(args, torch, a2afft):
         for v, s in zip(views, shapes):
             plans[s].forward(v)
 
-    t_sep = _events_ms(torch, separate, args.steps)
+    t_sep = _events_ms(torch, separate, args.steps, args.warmup)
     npts = sum(s[0] * s[1] * s[2] for s in shapes)
     flops = sum(5.0 * (s[0] * s[1] * s[2]) * math.log2(s[0] * s[1] * s[2]) for s in shapes)
     bytes_alg = 6 * npts * 8
@@ -230,7 +232,7 @@ def run_c5(args, torch, a2afft):
     out = {
         "metric": "aggregate GFLOP/s, 64 uneven c2c 3D FFTs in one persistent launch",
         "value": round(flops / (best * 1e-3) / 1e9, 1), "unit": "GFLOP/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": 1, "ms_per_step": round(best, 4), "higher_is_better": True, "scaling": "strong",
+        "warmup": args.warmup, "ms_per_step": round(best, 4), "higher_is_better": True, "scaling": "strong",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "64 complex64 grids drawn with seed 4 from 32^3, 64^3, 128^3, 256^3, 128x256x64, "
                                "256x64x512; U[-1,1) values", "name": "c5", "counts": counts,
@@ -247,7 +249,7 @@ def run_c5(args, torch, a2afft):
     print(json.dumps(out), flush=True)
 
 
-def precision_variants(a2afft, torch, Ng, P, dims, seed, steps):
+def precision_variants(a2afft, torch, Ng, P, dims, seed, steps, warmup=30):
     """The same Ng^3 GRF (complex64, block layout over dims) through the FP32,
     split-BF16 and split-FP16 plans: ms per forward (events, back to back) and
     rel-L2 / max-abs against the FP64 plan on the same rounded input."""
@@ -266,7 +268,7 @@ def precision_variants(a2afft, torch, Ng, P, dims, seed, steps):
         plan.forward(data)
         rel, mx = a2afft.error_metrics(data, ref)
         data.copy_(x)
-        ms = _events_ms(torch, lambda: plan.forward(data), steps)
+        ms = _events_ms(torch, lambda: plan.forward(data), steps, warmup)
         out[prec] = {"ms": round(ms, 4), "rel_l2_vs_fp64": rel, "max_abs_vs_fp64": mx,
                      "hbm_gbs_whole": round(6 * Ng ** 3 * 8 / (ms * 1e-3) / 1e9, 1)}
         log(f"[bench] {prec}: {ms:.4f} ms rel-L2 {rel:.3e}")
@@ -279,11 +281,11 @@ def run_c4(args, torch, a2afft):
     """512^3 complex64 GRF (seed 3) over dims [1,4,4]: split-BF16 MFMA radix-16
     passes beside plain FP32 and split-FP16, error vs the FP64 plan per variant."""
     Ng, P, dims, seed = args.ng or 512, 16, (1, 4, 4), 3
-    v = precision_variants(a2afft, torch, Ng, P, dims, seed, args.steps)
+    v = precision_variants(a2afft, torch, Ng, P, dims, seed, args.steps, args.warmup)
     bf = v["split_bf16"]
     out = {
         "metric": "ms per 512^3 c2c forward, split-BF16 MFMA radix-16 vs plain FP32 (rel-L2 vs FP64)",
-        "value": bf["ms"], "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": 1,
+        "value": bf["ms"], "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": bf["ms"], "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
         "dtype": "bf16x2 (MFMA) + f32", "data": "synthetic",
         "config": {"workload": f"{Ng}^3 c2c forward, complex64, {P} virtual ranks dims {list(dims)}, "
@@ -519,7 +521,7 @@ def main():
     }
     if args.config == "c3" and not args.no_variants and args.ng in (None, 512):
         log("[bench] precision variants on the seed-3 field")
-        out["precision_variants_seed3"] = precision_variants(a2afft, torch, Ng, P, dims, 3, args.steps)
+        out["precision_variants_seed3"] = precision_variants(a2afft, torch, Ng, P, dims, 3, args.steps, args.warmup)
     print(json.dumps(out), flush=True)
 
 
