@@ -321,16 +321,21 @@ static bool chunk_ok(const a2afft_plan_s* p, uint32_t chunk) {
 
 static bool is_split(a2afft_precision prec) { return prec == A2AFFT_SPLIT_BF16 || prec == A2AFFT_SPLIT_FP16; }
 
-// Default chunk width.  complex64 x and y passes on the 512-point tiles with
-// half-width LDS exchanges run faster alternating over windows of z columns
-// that fit the 256 MB memory-side cache, once the grid is larger than 512 MiB
-// (512^3: 1.163 -> 1.134 ms with 128-column windows).  The full-exchange
-// tiles, and smaller grids, measured slower chunked: whole passes otherwise.
+// Default chunk width.  x and y passes on the 512-point tiles run faster
+// alternating over windows of z columns that fit the 256 MB memory-side
+// cache, once the grid is larger than 512 MiB, when the x pass runs the
+// half-width-exchange tile (and, for complex64, the y pass too):
+//   512^3 complex64:  1.163 -> 1.134 ms with 128-column windows,
+//   512^3 complex128: 2.40 -> 2.23 ms with 64-column windows.
+// complex64 with full-exchange tiles (1.163 -> 1.189 ms) and smaller grids
+// measured slower chunked: whole passes otherwise.
 static uint32_t default_chunk(const a2afft_plan_s* p) {
-    if (p->prec != A2AFFT_FP32) return 0;
+    if (p->prec != A2AFFT_FP32 && p->prec != A2AFFT_FP64) return 0;
+    const bool c64 = p->prec == A2AFFT_FP32;
     for (int i = 1; i < 3; ++i)
-        if (p->pass[i].fft.n != 512u || !p->pass[i].strided || !(p->sel[i].flags & KF_HALFX)) return 0;
-    const uint64_t column = uint64_t(p->n[0]) * uint64_t(p->n[1]) * 8u;   // bytes per z column
+        if (p->pass[i].fft.n != 512u || !p->pass[i].strided) return 0;
+    if (!(p->sel[1].flags & KF_HALFX) || (c64 && !(p->sel[2].flags & KF_HALFX))) return 0;
+    const uint64_t column = uint64_t(p->n[0]) * uint64_t(p->n[1]) * (c64 ? 8u : 16u);   // bytes per z column
     if (column * uint64_t(p->n[2]) <= (uint64_t(512) << 20)) return 0;
     uint32_t w = 0;
     for (uint32_t c = 16; c < uint32_t(p->n[2]) && column * c <= (uint64_t(256) << 20); c *= 2)

@@ -282,10 +282,10 @@ int a2afft_plan_set_kernel(a2afft_plan plan, int pass, int T, int E, int flags);
  * windows of `z_columns` z columns (0 = whole passes), so each window is
  * re-read from the Infinity Cache.  Results are identical for every width;
  * the width must divide Ng and be a multiple of both strided tile widths.
- * A new plan starts at 0, or, for complex64 grids above 512 MiB whose x and
- * y passes run the 512-point half-width-exchange tiles, at the widest window
- * of at most 256 MiB.  autotune also times whole forwards over the valid
- * widths. */
+ * A new plan starts at 0, or, for grids above 512 MiB whose x and y passes
+ * are 512-point lines on half-width-exchange tiles (for complex128 the x pass
+ * alone), at the widest window of at most 256 MiB.  autotune also times
+ * whole forwards over the valid widths. */
 int a2afft_plan_set_chunking(a2afft_plan plan, int z_columns);
 /* Launches in one fused forward / inverse, and launch `index` alone (returns
  * its pass: 0 z, 1 x, 2 y), for per-kernel timing. */
