@@ -1,9 +1,9 @@
 // Pattern copy: the memory traffic of one Stockham pass without its
-// arithmetic.  Every
-// thread computes its line's address exactly as stockham_unit does (the same
-// line_base / elem_off, workgroup order, tile shape and cache policy),
-// loads its E elements and stores them back unchanged; the workgroup holds
-// the pass kernel's LDS footprint, so as many workgroups share a CU.  The
+// arithmetic.  Every thread computes its line's address exactly as
+// stockham_unit does (the same line_base / elem_off, workgroup order, tile
+// shape and cache policy), loads its E elements and stores them back
+// unchanged; the workgroup holds the pass kernel's LDS footprint, so as many
+// workgroups share a CU.  The
 // pass kernel's time beside this kernel's time shows how much of it the
 // arithmetic and LDS exchanges add to the memory traffic of its own access
 // pattern.  It is not a strict bound: a pure copy issues all of a
