@@ -343,3 +343,18 @@ def test_halfx_variants_bit_identical(a2afft_mod, cuda, case):
             assert np.array_equal(_bits(y.cpu().numpy()), ref_bits), (p, T, E, flags)
             ran += 1
     assert ran == len(shapes) * len(HALFX_FLAGS)
+
+
+@pytest.mark.parametrize("case", [((512, 16, (1, 4, 4), "c64", "fp32"), 128), ((512, 16, (1, 4, 4), "c128", "fp64"), 64),
+                                  ((256, 8, (1, 2, 4), "c128", "fp64"), 0), ((256, 8, (1, 2, 4), "c64", "fp32"), 0),
+                                  ((512, 16, (1, 4, 4), "c64", "split_bf16"), 0)],
+                         ids=lambda c: str(c))
+def test_default_chunk_width(a2afft_mod, cuda, case):
+    """New plans alternate the x and y passes over windows of at most 256 MiB
+    only for grids above 512 MiB on the 512-point half-width-exchange tiles;
+    every other plan runs whole passes."""
+    (Ng, P, dims, dtype, prec), width = case
+    dist = a2afft_mod.Distribution(Ng, P, dims=dims, dtype=dtype)
+    plan = a2afft_mod.Dfft(dist, precision=prec)
+    assert f"chunk {width} z columns" in plan.describe()
+    assert plan.num_launches() == (3 if width == 0 else 1 + 2 * (Ng // width))
