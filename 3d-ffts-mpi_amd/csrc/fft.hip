@@ -391,6 +391,7 @@ static int fft_1d(void* data, int Ng, int64_t n_elems, a2afft_dtype dtype, int d
         return set_error(A2AFFT_E_INVALID_ARG, "1d fft: bad arguments");
     const int64_t lines = n_elems / Ng;
     if (lines >= (int64_t(1) << 32)) return set_error(A2AFFT_E_SIZE, "1d fft: too many lines");
+    if (Ng > kDftMaxN) return launch_long_fft(data, Ng, lines, dtype, dir, (hipStream_t)stream);
     const PassMaps pm = batched_pass_maps(Ng, uint32_t(lines));
     const AxisPass p{pm.fft, pm.a1, pm.a2, 0, 0, 1.0};
     return launch_fft_pass(p, data, dtype, Ng, dir, (hipStream_t)stream, nullptr);

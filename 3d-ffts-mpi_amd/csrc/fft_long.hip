@@ -1,0 +1,179 @@
+// Batched FFTs of contiguous lines longer than one kernel's 4096 points: the
+// four-step decomposition N = N1 * N2 (both <= 4096), the lines' counterpart
+// of cuFFT's large-length plans behind forward_1d_fft / inverse_1d_fft
+// (a2affts.cu:7-76).  With n = N2*n1 + n2 and k = k1 + N1*k2,
+//   X[k1 + N1 k2] = sum_n2 W_N2^(n2 k2) W_N^(n2 k1) sum_n1 W_N1^(n1 k1) x[N2 n1 + n2]
+// 1. length-N1 FFTs down the columns of each line seen as [N1][N2], in place
+//    (the strided Stockham pass, T adjacent columns per workgroup);
+// 2. multiply by W_N^(n2 k1) and transpose to [N2][N1] into a workspace;
+// 3. length-N2 FFTs down the columns of the workspace lines, which leaves
+//    X[k1 + N1 k2] at k2*N1 + k1, natural order;
+// 4. copy the workspace back.
+// Lines are processed in chunks of at most 256 MiB; the workspace comes from
+// the stream-ordered allocator on the caller's stream (allocated and freed in
+// stream order, so calls on different streams do not share it).
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <vector>
+
+#include "fft.h"
+#include "remap.h"
+
+namespace a2a {
+
+namespace {
+
+constexpr int64_t kLongChunkBytes = int64_t(256) << 20;
+
+// W_N^m = hi[m / B] * lo[m % B]: two tables of about sqrt(N) entries, each
+// entry from long double rounded once to double.
+struct LongTw {
+    uint32_t B = 0, nhi = 0;
+    double2* hi = nullptr;
+    double2* lo = nullptr;
+};
+
+const LongTw* long_twiddles(int64_t N, int dir) {
+    static std::mutex mu;
+    static std::map<std::vector<int64_t>, LongTw> cache;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+    const std::vector<int64_t> key{dev, N, dir < 0 ? -1 : 1};
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) return &it->second;
+    LongTw t;
+    t.B = 1;
+    while (int64_t(t.B) * t.B < N) ++t.B;
+    t.nhi = uint32_t((N + t.B - 1) / t.B);
+    std::vector<double2> lo(t.B), hi(t.nhi);
+    long double re, im;
+    for (uint32_t i = 0; i < t.B; ++i) {
+        twiddle_ld(int64_t(i), N, dir, &re, &im);
+        lo[i].x = double(re);
+        lo[i].y = double(im);
+    }
+    for (uint32_t j = 0; j < t.nhi; ++j) {
+        twiddle_ld(int64_t(j) * t.B, N, dir, &re, &im);
+        hi[j].x = double(re);
+        hi[j].y = double(im);
+    }
+    if (hipMalloc(&t.lo, lo.size() * sizeof(double2)) != hipSuccess) return nullptr;
+    if (hipMalloc(&t.hi, hi.size() * sizeof(double2)) != hipSuccess) {
+        (void)hipFree(t.lo);
+        return nullptr;
+    }
+    if (hipMemcpy(t.lo, lo.data(), lo.size() * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(t.hi, hi.data(), hi.size() * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(t.lo);
+        (void)hipFree(t.hi);
+        return nullptr;
+    }
+    return &(cache[key] = t);
+}
+
+// dst[line][n2][k1] = src[line][k1][n2] * W_N^(n2 k1), through a 32 x 33 LDS
+// tile; 256 threads as 32 x 8.  Workgroup b covers tile (tn2, tk1) of line
+// `line`, b = (line * tk1s + tk1) * tn2s + tn2.
+template <typename R>
+__global__ __launch_bounds__(256) void twiddle_transpose_kernel(const HIP_vector_type<R, 2>* __restrict__ src,
+                                                                HIP_vector_type<R, 2>* __restrict__ dst,
+                                                                uint32_t N1, uint32_t N2, uint32_t tn2s,
+                                                                uint32_t tk1s, const double2* __restrict__ hi,
+                                                                const double2* __restrict__ lo, uint32_t B) {
+    using V = HIP_vector_type<R, 2>;
+    __shared__ V tile[32][33];
+    const uint32_t b = blockIdx.x;
+    const uint32_t tn2 = b % tn2s, rest = b / tn2s;
+    const uint32_t tk1 = rest % tk1s, line = rest / tk1s;
+    const uint32_t tx = threadIdx.x & 31u, ty = threadIdx.x >> 5;
+    const int64_t base = int64_t(line) * N1 * N2;
+#pragma unroll
+    for (uint32_t r = 0; r < 32; r += 8) {
+        const uint32_t k1 = tk1 * 32u + ty + r, n2 = tn2 * 32u + tx;
+        if (k1 < N1 && n2 < N2) {
+            const V v = src[base + int64_t(k1) * N2 + n2];
+            const uint32_t m = n2 * k1;   // < N1 * N2 <= 2^24
+            const double2 h = hi[m / B], l = lo[m % B];
+            const double wr = h.x * l.x - h.y * l.y, wi = h.x * l.y + h.y * l.x;
+            const double vr = double(v.x), vi = double(v.y);
+            V o;
+            o.x = R(vr * wr - vi * wi);
+            o.y = R(vr * wi + vi * wr);
+            tile[ty + r][tx] = o;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t r = 0; r < 32; r += 8) {
+        const uint32_t n2 = tn2 * 32u + ty + r, k1 = tk1 * 32u + tx;
+        if (k1 < N1 && n2 < N2) dst[base + int64_t(n2) * N1 + k1] = tile[tx][ty + r];
+    }
+}
+
+}  // namespace
+
+// N1 * N2 = N with both factors in 2..4096, N1 >= N2 and as close to sqrt(N)
+// as the divisors allow; false when N has no such split (a prime above 4096).
+bool long_fft_split(int64_t N, int* n1, int* n2) {
+    if (N <= kDftMaxN || N > int64_t(kDftMaxN) * kDftMaxN) return false;
+    int best = 0;
+    for (int64_t d = 2; d <= kDftMaxN; ++d)
+        if (N % d == 0 && N / d <= kDftMaxN && N / d >= 2 && d >= N / d && (best == 0 || d < best)) best = int(d);
+    if (!best) return false;
+    *n1 = best;
+    *n2 = int(N / best);
+    return true;
+}
+
+int launch_long_fft(void* data, int N, int64_t lines, a2afft_dtype dt, int dir, hipStream_t s) {
+    int N1 = 0, N2 = 0;
+    if (!long_fft_split(N, &N1, &N2))
+        return set_error(A2AFFT_E_UNSUPPORTED, "1d fft: length %d is not a product of two lengths <= %d", N,
+                         kDftMaxN);
+    const size_t esize = dt == A2AFFT_C128 ? 16 : 8;
+    const LongTw* tw = long_twiddles(N, dir);
+    if (!tw) return set_error(A2AFFT_E_ALLOC, "1d fft: twiddles for length %d", N);
+    const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(lines, kLongChunkBytes / (int64_t(N) * esize)));
+    void* work = nullptr;
+    A2A_TRY(hip_check(int(hipMallocAsync(&work, size_t(chunk) * N * esize, s)), "1d fft: workspace"));
+    const uint32_t tn2s = uint32_t((N2 + 31) / 32), tk1s = uint32_t((N1 + 31) / 32);
+    int st = A2AFFT_OK;
+    for (int64_t l0 = 0; l0 < lines && st >= 0; l0 += chunk) {
+        const uint32_t cl = uint32_t(std::min<int64_t>(chunk, lines - l0));
+        char* D = static_cast<char*>(data) + size_t(l0) * N * esize;
+        // 1. columns of [N1][N2], in place
+        const AxisPass p1{make_axis(uint32_t(N1), uint32_t(N1), 0, N2), make_axis(cl, cl, 0, N),
+                          make_axis(uint32_t(N2), uint32_t(N2), 0, 1), 1, 0, 1.0};
+        st = launch_fft_pass(p1, D, dt, N1, dir, s, nullptr);
+        if (st < 0) break;
+        // 2. twiddle and transpose into the workspace
+        const uint64_t blocks = uint64_t(tn2s) * tk1s * cl;
+        if (blocks >= (uint64_t(1) << 31)) {
+            st = set_error(A2AFFT_E_SIZE, "1d fft: too many transpose tiles");
+            break;
+        }
+        if (dt == A2AFFT_C128)
+            hipLaunchKernelGGL(twiddle_transpose_kernel<double>, dim3(uint32_t(blocks)), dim3(256), 0, s,
+                               reinterpret_cast<const double2*>(D), static_cast<double2*>(work), uint32_t(N1),
+                               uint32_t(N2), tn2s, tk1s, tw->hi, tw->lo, tw->B);
+        else
+            hipLaunchKernelGGL(twiddle_transpose_kernel<float>, dim3(uint32_t(blocks)), dim3(256), 0, s,
+                               reinterpret_cast<const float2*>(D), static_cast<float2*>(work), uint32_t(N1),
+                               uint32_t(N2), tn2s, tk1s, tw->hi, tw->lo, tw->B);
+        st = hip_check(int(hipGetLastError()), "1d fft: transpose launch");
+        if (st < 0) break;
+        // 3. columns of [N2][N1] in the workspace: natural order
+        const AxisPass p3{make_axis(uint32_t(N2), uint32_t(N2), 0, N1), make_axis(cl, cl, 0, N),
+                          make_axis(uint32_t(N1), uint32_t(N1), 0, 1), 1, 0, 1.0};
+        st = launch_fft_pass(p3, work, dt, N2, dir, s, nullptr);
+        if (st < 0) break;
+        // 4. back in place
+        st = launch_copy(D, work, int64_t(cl) * N, dt, s);
+    }
+    const int fs = hip_check(int(hipFreeAsync(work, s)), "1d fft: workspace free");
+    return st < 0 ? st : fs;
+}
+
+}  // namespace a2a

@@ -158,7 +158,13 @@ int a2afft_alltoall(const void* send, void* recv, int world_size, int64_t nlocal
  * 2^a*3^b*5^c length runs as a Stockham FFT (fixed schedules for powers of
  * two and short mixed lengths, the radix schedule in the kernel arguments for
  * the rest); lengths with other prime factors as a direct O(Ng^2) DFT per
- * line.  Longer lines return A2AFFT_E_UNSUPPORTED. */
+ * line.  Longer lines, up to 4096^2, run as four steps over Ng = N1 * N2
+ * (both factors <= 4096): N1-point FFTs in place, a twiddle-and-transpose
+ * into a workspace, N2-point FFTs there, a copy back.  The workspace (at most
+ * 256 MiB, lines are processed in chunks) comes from the stream-ordered
+ * allocator on `stream` and is released in stream order.  Lengths above 4096
+ * with no such split (primes, lengths above 4096^2) return
+ * A2AFFT_E_UNSUPPORTED. */
 int a2afft_forward_1d_fft(void* data, int Ng, int64_t n_elems, a2afft_dtype dtype, void* stream);
 int a2afft_inverse_1d_fft(void* data, int Ng, int64_t n_elems, a2afft_dtype dtype, void* stream);
 /* launch_scale_fft (a2affts.cu:78-92,108-116): out = data / Ng^3 (64-bit Ng^3). */

@@ -358,3 +358,52 @@ def test_default_chunk_width(a2afft_mod, cuda, case):
     plan = a2afft_mod.Dfft(dist, precision=prec)
     assert f"chunk {width} z columns" in plan.describe()
     assert plan.num_launches() == (3 if width == 0 else 1 + 2 * (Ng // width))
+
+
+# Lines longer than 4096 points: four-step over N = N1 * N2 (both <= 4096).
+# Splits: 8192 = 128*64, 12288 = 128*96 (a runtime-schedule pass), 16384 =
+# 128*128, 65536 = 256*256, 10000 = 100*100, 8194 = 241*34 (direct DFTs),
+# 2^20 = 1024*1024.
+LONG = [8192, 12288, 16384, 65536, 10000, 8194, 1 << 20]
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+@pytest.mark.parametrize("N", LONG)
+def test_long_lines_vs_oracle(a2afft_mod, cuda, N, dtype):
+    import torch
+    rng = np.random.default_rng(N + 1)
+    # two lines for 8194: the oracle sums its 241-point factor directly, O(N^2) per line
+    lines = 2 if N == 8194 else (1 << 18) // N + 1
+    x = rng.standard_normal((lines, N)) + 1j * rng.standard_normal((lines, N))
+    x = x.astype(np.complex128 if dtype == "c128" else np.complex64)
+    # direct-DFT factors grow the error like sqrt(N1) + sqrt(N2), Stockham ones like log2 N
+    scale = max(np.log2(N), np.sqrt(241) + np.sqrt(34)) if N == 8194 else np.log2(N)
+    for direction, fn in ((-1, a2afft_mod.forward_1d_fft), (+1, a2afft_mod.inverse_1d_fft)):
+        ref = O.fft_batched(x.astype(np.complex128), direction)
+        d = torch.from_numpy(x).cuda()
+        fn(d, N)
+        torch.cuda.synchronize()
+        assert _relerr(d.cpu().numpy(), ref) < TOL_1D[dtype] * scale, direction
+
+
+def test_long_line_impulse(a2afft_mod, cuda):
+    """delta(n - n0) -> exp(-2 pi i k n0 / N), phases reduced mod N exactly."""
+    import torch
+    N, n0 = 1 << 16, 12345
+    x = np.zeros(N, np.complex128)
+    x[n0] = 1.0
+    d = torch.from_numpy(x).cuda()
+    a2afft_mod.forward_1d_fft(d, N)
+    torch.cuda.synchronize()
+    k = np.arange(N, dtype=np.int64)
+    ref = np.exp(-2j * np.pi * ((k * n0) % N) / N)
+    assert np.max(np.abs(d.cpu().numpy() - ref)) < 1e-14
+
+
+@pytest.mark.parametrize("N", [8191, 4099 * 4097])
+def test_long_line_without_split_is_an_error(a2afft_mod, cuda, N):
+    """A prime above 4096, or a length above 4096^2, has no two-factor split."""
+    import torch
+    d = torch.zeros(N, dtype=torch.complex64, device="cuda")
+    with pytest.raises(a2afft_mod.A2AFFTError, match="not a product"):
+        a2afft_mod.forward_1d_fft(d, N)
