@@ -170,7 +170,7 @@ class _Plan:
                        "execute_launch")
 
     def describe(self):
-        buf = ctypes.create_string_buffer(8192)
+        buf = ctypes.create_string_buffer(32768)
         L.check(L.lib().a2afft_plan_describe(self._h, buf, len(buf)), "describe")
         return buf.value.decode()
 

@@ -268,7 +268,7 @@ struct a2afft_plan_s {
     AxisPass pass[3];   // fused: z, x, y over the block layout
     KSel sel[3];        // kernel variant per fused pass
     uint32_t chunk;     // z columns per x/y chunk (0: whole passes)
-    char report[4096];  // last autotune table
+    char report[16384]; // last autotune table
     hipStream_t cap_stream = nullptr;                // graph capture stream (graph_launch)
     hipGraphExec_t graph[2] = {nullptr, nullptr};    // forward, inverse
     void* graph_data[2] = {nullptr, nullptr};        // the buffer each graph was captured for
