@@ -41,9 +41,10 @@ def test_driver_reference_check(cuda, tmp_path, case):
 
 
 def test_unsupported_length_is_an_error(cuda, tmp_path):
-    """Lines longer than 4096 have no kernel: a clean error, not a launch."""
+    """A prime line length above 4096 has no kernel and no split into two
+    kernel lengths: a clean error, not a launch."""
     import torch
     import a2afft
     d = torch.zeros(2 * 4099, dtype=torch.complex128, device="cuda")
-    with pytest.raises(a2afft.A2AFFTError, match="not supported"):
+    with pytest.raises(a2afft.A2AFFTError, match="not a product of two lengths"):
         a2afft.forward_1d_fft(d, 4099)
