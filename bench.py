@@ -170,7 +170,7 @@ def run_c3b(args, torch, a2afft):
         "max_abs_vs_fp64": {"pencils": max_p, "block": max_b},
         "roofline": {"bound": "hbm", "kernel": "stockham 2048 contiguous", "bytes_per_launch": bytes_fft,
                      "launch_ms": round(t_pencil, 4), "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None},
+                     "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": _traffic("c3b_pencil_2048")},
         "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
