@@ -51,10 +51,8 @@ int launch_fft_pass(const AxisPass& pass, void* data, a2afft_dtype dtype, int N,
 int launch_pattern_copy(const AxisPass& pass, void* data, a2afft_dtype dtype, int N, const KSel& sel, hipStream_t s);
 
 // Batched contiguous lines longer than 4096 points (fft_long.hip): four-step
-// over N = N1 * N2 with both factors <= 4096, through a workspace from the
-// stream-ordered allocator.  long_fft_split picks the factors (false: no
-// such split).
-bool long_fft_split(int64_t N, int* n1, int* n2);
+// over N = N1 * N2 with both factors <= 4096 (long_fft_split), through a
+// workspace from the stream-ordered allocator.
 int launch_long_fft(void* data, int N, int64_t lines, a2afft_dtype dtype, int dir, hipStream_t s);
 
 // Stockham FFT with the radix schedule in the kernel arguments (KF_MIX,

@@ -74,8 +74,7 @@ const LongTw* long_twiddles(int64_t N, int dir) {
 }
 
 // dst[line][n2][k1] = src[line][k1][n2] * W_N^(n2 k1), through a 32 x 33 LDS
-// tile; 256 threads as 32 x 8.  Workgroup b covers tile (tn2, tk1) of line
-// `line`, b = (line * tk1s + tk1) * tn2s + tn2.
+// tile; 256 threads as 32 x 8, one tile per workgroup (long_tile).
 template <typename R>
 __global__ __launch_bounds__(256) void twiddle_transpose_kernel(const HIP_vector_type<R, 2>* __restrict__ src,
                                                                 HIP_vector_type<R, 2>* __restrict__ dst,
@@ -84,9 +83,8 @@ __global__ __launch_bounds__(256) void twiddle_transpose_kernel(const HIP_vector
                                                                 const double2* __restrict__ lo, uint32_t B) {
     using V = HIP_vector_type<R, 2>;
     __shared__ V tile[32][33];
-    const uint32_t b = blockIdx.x;
-    const uint32_t tn2 = b % tn2s, rest = b / tn2s;
-    const uint32_t tk1 = rest % tk1s, line = rest / tk1s;
+    uint32_t tn2, tk1, line;
+    long_tile(blockIdx.x, tn2s, tk1s, &tn2, &tk1, &line);
     const uint32_t tx = threadIdx.x & 31u, ty = threadIdx.x >> 5;
     const int64_t base = int64_t(line) * N1 * N2;
 #pragma unroll
@@ -113,19 +111,6 @@ __global__ __launch_bounds__(256) void twiddle_transpose_kernel(const HIP_vector
 }
 
 }  // namespace
-
-// N1 * N2 = N with both factors in 2..4096, N1 >= N2 and as close to sqrt(N)
-// as the divisors allow; false when N has no such split (a prime above 4096).
-bool long_fft_split(int64_t N, int* n1, int* n2) {
-    if (N <= kDftMaxN || N > int64_t(kDftMaxN) * kDftMaxN) return false;
-    int best = 0;
-    for (int64_t d = 2; d <= kDftMaxN; ++d)
-        if (N % d == 0 && N / d <= kDftMaxN && N / d >= 2 && d >= N / d && (best == 0 || d < best)) best = int(d);
-    if (!best) return false;
-    *n1 = best;
-    *n2 = int(N / best);
-    return true;
-}
 
 int launch_long_fft(void* data, int N, int64_t lines, a2afft_dtype dt, int dir, hipStream_t s) {
     int N1 = 0, N2 = 0;

@@ -232,6 +232,31 @@ inline int resolve_with(const AxisMap& fft, const AxisMap& a1, const AxisMap& a2
     return resolve_geom(fft, a1, a2, strided, T, g, grid, a2_start, a2_count);
 }
 
+// Batched lines longer than kDftMaxN (fft_long.hip): N = N1 * N2 with both
+// factors in 2..kDftMaxN, N1 >= N2 and N1 the smallest such divisor (as close
+// to sqrt(N) as the divisors allow); false when N has no such split (a prime
+// above kDftMaxN, or N above kDftMaxN^2).
+inline bool long_fft_split(int64_t N, int* n1, int* n2) {
+    if (N <= kDftMaxN || N > int64_t(kDftMaxN) * kDftMaxN) return false;
+    for (int64_t d = 2; d <= kDftMaxN; ++d)
+        if (N % d == 0 && N / d <= kDftMaxN && N / d >= 2 && d >= N / d) {
+            *n1 = int(d);
+            *n2 = int(N / d);
+            return true;
+        }
+    return false;
+}
+
+// The four-step's twiddle-transpose of each line [N1][N2] -> [N2][N1] in
+// 32 x 32 tiles: workgroup b covers tile (tn2, tk1) of line `line`, with
+// b = (line * tk1s + tk1) * tn2s + tn2, tn2s = ceil(N2 / 32), tk1s = ceil(N1 / 32).
+A2A_HD void long_tile(uint32_t b, uint32_t tn2s, uint32_t tk1s, uint32_t* tn2, uint32_t* tk1, uint32_t* line) {
+    *tn2 = b % tn2s;
+    const uint32_t rest = b / tn2s;
+    *tk1 = rest % tk1s;
+    *line = rest / tk1s;
+}
+
 // Axis maps of the three fused passes over the virtual-rank block layout:
 // z pass = lines (x, y), contiguous; x pass = tiles (y, z-columns); y pass =
 // tiles (x, z-columns).  The reference's axis order z, x, y (a2adfft.cpp:60-114).

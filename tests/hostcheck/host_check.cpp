@@ -144,4 +144,33 @@ void hc_xcd_blocks(uint32_t G, uint32_t* out) {
     for (uint32_t b = 0; b < G; ++b) out[b] = xcd_block(b, G);
 }
 
+// The four-step of lines longer than 4096: its factor split, and for every
+// workgroup of a chunk of `lines` lines every (source, destination) element
+// pair its twiddle-transpose moves, with the tile slot the pair passes through
+// (the writer thread (tx, ty + r) of phase 1 and the reader of phase 2 meet at
+// tile[ty + r][tx]).  Returns the count, or a negative status.
+int hc_long_split(int64_t N, int* n1, int* n2) { return long_fft_split(N, n1, n2) ? 1 : 0; }
+
+int64_t hc_long_transpose(int N1, int N2, int lines, int64_t* src, int64_t* dst, int64_t cap) {
+    const uint32_t tn2s = uint32_t((N2 + 31) / 32), tk1s = uint32_t((N1 + 31) / 32);
+    const uint64_t blocks = uint64_t(tn2s) * tk1s * uint32_t(lines);
+    const int64_t N = int64_t(N1) * N2;
+    int64_t c = 0;
+    for (uint64_t b = 0; b < blocks; ++b) {
+        uint32_t tn2, tk1, line;
+        long_tile(uint32_t(b), tn2s, tk1s, &tn2, &tk1, &line);
+        for (uint32_t i = 0; i < 32; ++i)        // tile row = ty + r of the phase-1 writer
+            for (uint32_t j = 0; j < 32; ++j) {  // tile column = tx
+                const uint32_t k1 = tk1 * 32u + i, n2 = tn2 * 32u + j;
+                if (!(k1 < uint32_t(N1) && n2 < uint32_t(N2))) continue;
+                if (c >= cap) return -100;
+                src[c] = int64_t(line) * N + int64_t(k1) * N2 + n2;
+                // phase 2: thread (tx = i, ty + r = j) reads tile[i][j], n2' = tn2*32 + j, k1' = tk1*32 + i
+                dst[c] = int64_t(line) * N + int64_t(tn2 * 32u + j) * N1 + (tk1 * 32u + i);
+                ++c;
+            }
+    }
+    return c;
+}
+
 }  // extern "C"

@@ -38,6 +38,9 @@ def hc(tmp_path_factory):
     L.hc_xcd_blocks.argtypes = [ctypes.c_uint32, P]
     L.hc_grid_enumerate.argtypes = [ctypes.c_int] * 6 + [P, P, P, ctypes.c_int64]
     L.hc_grid_enumerate.restype = ctypes.c_int64
+    L.hc_long_split.argtypes = [ctypes.c_int64, P, P]
+    L.hc_long_transpose.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P, ctypes.c_int64]
+    L.hc_long_transpose.restype = ctypes.c_int64
     return L
 
 
@@ -235,3 +238,36 @@ def test_non_cubic_grid_addresses(hc, shape):
             key = coords[other[0]] * 4096 + coords[other[1]]
             k_sorted = key[np.argsort(line, kind="stable")].reshape(-1, N)
             assert (k_sorted == k_sorted[:, :1]).all()
+
+
+def test_long_line_split(hc):
+    """Lines above 4096 split into N1 * N2, both in 2..4096, N1 >= N2; primes
+    above 4096 and lengths above 4096^2 have no split."""
+    n1, n2 = ctypes.c_int(), ctypes.c_int()
+    for N in [4097 * 2, 8192, 10000, 12288, 8194, 1 << 20, 4096 * 4096, 4095 * 4093, 6000, 4100]:
+        assert hc.hc_long_split(N, ctypes.byref(n1), ctypes.byref(n2)) == 1, N
+        assert n1.value * n2.value == N and 2 <= n2.value <= n1.value <= 4096, (N, n1.value, n2.value)
+        # no divisor between sqrt(N) and N1: the most balanced split
+        assert not any(N % d == 0 and N // d <= 4096 for d in range(int(np.ceil(np.sqrt(N))), n1.value)), N
+    for N in [4096, 4099, 8191, 4099 * 4097, 4096 * 4096 + 2, 1 << 25]:
+        assert hc.hc_long_split(N, ctypes.byref(n1), ctypes.byref(n2)) == 0, N
+
+
+@pytest.mark.parametrize("case", [(128, 64, 3), (241, 34, 2), (100, 100, 1), (128, 96, 2), (1024, 1024, 1),
+                                  (33, 31, 5)], ids=lambda c: f"{c[0]}x{c[1]}x{c[2]}")
+def test_long_line_transpose_addresses(hc, case):
+    """The twiddle-transpose's workgroups move every element of the chunk
+    exactly once, in bounds, as the per-line transposition [N1][N2] ->
+    [N2][N1] (numpy's transpose as the reference)."""
+    N1, N2, lines = case
+    total = lines * N1 * N2
+    src = np.empty(total + 1, np.int64)
+    dst = np.empty(total + 1, np.int64)
+    c = hc.hc_long_transpose(N1, N2, lines, _p(src), _p(dst), total + 1)
+    assert c == total
+    src, dst = src[:c], dst[:c]
+    assert np.array_equal(np.sort(src), np.arange(total)) and np.array_equal(np.sort(dst), np.arange(total))
+    moved = np.empty(total, np.int64)
+    moved[dst] = src                           # out[dst] = in[src]
+    ref = np.arange(total).reshape(lines, N1, N2).transpose(0, 2, 1).reshape(-1)
+    assert np.array_equal(moved, ref)
