@@ -11,7 +11,9 @@
 // 4. copy the workspace back.
 // Lines are processed in chunks of at most 256 MiB; the workspace comes from
 // the stream-ordered allocator on the caller's stream (allocated and freed in
-// stream order, so calls on different streams do not share it).
+// stream order, so calls on different streams do not share it).  When N1 =
+// N2 the transpose runs in place over tile pairs: three passes, no
+// workspace.
 #include <algorithm>
 #include <map>
 #include <mutex>
@@ -110,6 +112,55 @@ __global__ __launch_bounds__(256) void twiddle_transpose_kernel(const HIP_vector
     }
 }
 
+// The square case in place: line [M][M], workgroup = tile pair (a, c),
+// a <= c (long_pair).  Both tiles are read, multiplied by W_N^(k1 n2) (the
+// same for (k1, n2) and (n2, k1)) and staged in LDS before either is written,
+// and every element belongs to exactly one pair, so no two workgroups touch
+// the same element.
+template <typename R>
+__global__ __launch_bounds__(256) void twiddle_transpose_square_kernel(HIP_vector_type<R, 2>* __restrict__ data,
+                                                                       uint32_t M, uint32_t tiles,
+                                                                       const double2* __restrict__ hi,
+                                                                       const double2* __restrict__ lo, uint32_t B) {
+    using V = HIP_vector_type<R, 2>;
+    __shared__ V t0[32][33];
+    __shared__ V t1[32][33];
+    uint32_t a, c, line;
+    long_pair(blockIdx.x, tiles, &a, &c, &line);
+    const uint32_t tx = threadIdx.x & 31u, ty = threadIdx.x >> 5;
+    V* const L = data + int64_t(line) * M * M;
+    auto load = [&](uint32_t row0, uint32_t col0, V (&t)[32][33]) {
+#pragma unroll
+        for (uint32_t r = 0; r < 32; r += 8) {
+            const uint32_t k1 = row0 + ty + r, n2 = col0 + tx;
+            if (k1 < M && n2 < M) {
+                const V v = L[int64_t(k1) * M + n2];
+                const uint32_t m = n2 * k1;
+                const double2 h = hi[m / B], l = lo[m % B];
+                const double wr = h.x * l.x - h.y * l.y, wi = h.x * l.y + h.y * l.x;
+                const double vr = double(v.x), vi = double(v.y);
+                V o;
+                o.x = R(vr * wr - vi * wi);
+                o.y = R(vr * wi + vi * wr);
+                t[ty + r][tx] = o;
+            }
+        }
+    };
+    // t[i][j] holds element (row0 + i, col0 + j); it goes to (col0 + j, row0 + i)
+    auto store = [&](uint32_t row0, uint32_t col0, V (&t)[32][33]) {
+#pragma unroll
+        for (uint32_t r = 0; r < 32; r += 8) {
+            const uint32_t n2 = col0 + ty + r, k1 = row0 + tx;
+            if (k1 < M && n2 < M) L[int64_t(n2) * M + k1] = t[tx][ty + r];
+        }
+    };
+    load(a * 32u, c * 32u, t0);
+    if (a != c) load(c * 32u, a * 32u, t1);
+    __syncthreads();
+    store(a * 32u, c * 32u, t0);
+    if (a != c) store(c * 32u, a * 32u, t1);
+}
+
 }  // namespace
 
 int launch_long_fft(void* data, int N, int64_t lines, a2afft_dtype dt, int dir, hipStream_t s) {
@@ -120,6 +171,26 @@ int launch_long_fft(void* data, int N, int64_t lines, a2afft_dtype dt, int dir, 
     const size_t esize = dt == A2AFFT_C128 ? 16 : 8;
     const LongTw* tw = long_twiddles(N, dir);
     if (!tw) return set_error(A2AFFT_E_ALLOC, "1d fft: twiddles for length %d", N);
+    if (N1 == N2) {   // square: three passes in place, no workspace
+        const uint32_t M = uint32_t(N1), tiles = (M + 31) / 32, pairs = tiles * (tiles + 1) / 2;
+        const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(lines, ((int64_t(1) << 31) - 1) / pairs));
+        for (int64_t l0 = 0; l0 < lines; l0 += chunk) {
+            const uint32_t cl = uint32_t(std::min<int64_t>(chunk, lines - l0));
+            char* D = static_cast<char*>(data) + size_t(l0) * N * esize;
+            const AxisPass p{make_axis(M, M, 0, M), make_axis(cl, cl, 0, N), make_axis(M, M, 0, 1), 1, 0, 1.0};
+            A2A_TRY(launch_fft_pass(p, D, dt, N1, dir, s, nullptr));
+            const uint32_t blocks = uint32_t(pairs) * cl;
+            if (dt == A2AFFT_C128)
+                hipLaunchKernelGGL(twiddle_transpose_square_kernel<double>, dim3(blocks), dim3(256), 0, s,
+                                   reinterpret_cast<double2*>(D), M, tiles, tw->hi, tw->lo, tw->B);
+            else
+                hipLaunchKernelGGL(twiddle_transpose_square_kernel<float>, dim3(blocks), dim3(256), 0, s,
+                                   reinterpret_cast<float2*>(D), M, tiles, tw->hi, tw->lo, tw->B);
+            A2A_TRY(hip_check(int(hipGetLastError()), "1d fft: transpose launch"));
+            A2A_TRY(launch_fft_pass(p, D, dt, N2, dir, s, nullptr));
+        }
+        return A2AFFT_OK;
+    }
     const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(lines, kLongChunkBytes / (int64_t(N) * esize)));
     void* work = nullptr;
     A2A_TRY(hip_check(int(hipMallocAsync(&work, size_t(chunk) * N * esize, s)), "1d fft: workspace"));

@@ -257,6 +257,21 @@ A2A_HD void long_tile(uint32_t b, uint32_t tn2s, uint32_t tk1s, uint32_t* tn2, u
     *line = rest / tk1s;
 }
 
+// The square case N1 = N2 = M transposes in place: workgroup b handles the
+// tile pair (a, c), a <= c, of line `line` (rows of the upper triangle of the
+// tiles x tiles grid in order), with b = line * pairs + pair.
+A2A_HD void long_pair(uint32_t b, uint32_t tiles, uint32_t* a, uint32_t* c, uint32_t* line) {
+    const uint32_t pairs = tiles * (tiles + 1) / 2;
+    *line = b / pairs;
+    uint32_t p = b % pairs, row = 0;
+    while (p >= tiles - row) {
+        p -= tiles - row;
+        ++row;
+    }
+    *a = row;
+    *c = row + p;
+}
+
 // Axis maps of the three fused passes over the virtual-rank block layout:
 // z pass = lines (x, y), contiguous; x pass = tiles (y, z-columns); y pass =
 // tiles (x, z-columns).  The reference's axis order z, x, y (a2adfft.cpp:60-114).

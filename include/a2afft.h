@@ -162,7 +162,8 @@ int a2afft_alltoall(const void* send, void* recv, int world_size, int64_t nlocal
  * (both factors <= 4096): N1-point FFTs in place, a twiddle-and-transpose
  * into a workspace, N2-point FFTs there, a copy back.  The workspace (at most
  * 256 MiB, lines are processed in chunks) comes from the stream-ordered
- * allocator on `stream` and is released in stream order.  Lengths above 4096
+ * allocator on `stream` and is released in stream order; square lengths
+ * (N1 = N2) transpose in place and take none.  Lengths above 4096
  * with no such split (primes, lengths above 4096^2) return
  * A2AFFT_E_UNSUPPORTED. */
 int a2afft_forward_1d_fft(void* data, int Ng, int64_t n_elems, a2afft_dtype dtype, void* stream);

@@ -173,4 +173,32 @@ int64_t hc_long_transpose(int N1, int N2, int lines, int64_t* src, int64_t* dst,
     return c;
 }
 
+// The square in-place case: for every workgroup (tile pair) of `lines` lines
+// of [M][M], every (source, destination) element pair, and the workgroup that
+// moves it.  Returns the count, or a negative status.
+int64_t hc_long_square(int M, int lines, int64_t* src, int64_t* dst, int64_t* wg, int64_t cap) {
+    const uint32_t tiles = uint32_t((M + 31) / 32), pairs = tiles * (tiles + 1) / 2;
+    const uint64_t blocks = uint64_t(pairs) * uint32_t(lines);
+    const int64_t N = int64_t(M) * M;
+    int64_t c = 0;
+    for (uint64_t b = 0; b < blocks; ++b) {
+        uint32_t a, cc, line;
+        long_pair(uint32_t(b), tiles, &a, &cc, &line);
+        for (int k = 0; k < (a == cc ? 1 : 2); ++k) {
+            const uint32_t row0 = (k == 0 ? a : cc) * 32u, col0 = (k == 0 ? cc : a) * 32u;
+            for (uint32_t i = 0; i < 32; ++i)
+                for (uint32_t j = 0; j < 32; ++j) {
+                    const uint32_t k1 = row0 + i, n2 = col0 + j;
+                    if (!(k1 < uint32_t(M) && n2 < uint32_t(M))) continue;
+                    if (c >= cap) return -100;
+                    src[c] = int64_t(line) * N + int64_t(k1) * M + n2;
+                    dst[c] = int64_t(line) * N + int64_t(n2) * M + k1;
+                    wg[c] = int64_t(b);
+                    ++c;
+                }
+        }
+    }
+    return c;
+}
+
 }  // extern "C"

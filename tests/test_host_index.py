@@ -41,6 +41,8 @@ def hc(tmp_path_factory):
     L.hc_long_split.argtypes = [ctypes.c_int64, P, P]
     L.hc_long_transpose.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P, ctypes.c_int64]
     L.hc_long_transpose.restype = ctypes.c_int64
+    L.hc_long_square.argtypes = [ctypes.c_int, ctypes.c_int, P, P, P, ctypes.c_int64]
+    L.hc_long_square.restype = ctypes.c_int64
     return L
 
 
@@ -271,3 +273,29 @@ def test_long_line_transpose_addresses(hc, case):
     moved[dst] = src                           # out[dst] = in[src]
     ref = np.arange(total).reshape(lines, N1, N2).transpose(0, 2, 1).reshape(-1)
     assert np.array_equal(moved, ref)
+
+
+@pytest.mark.parametrize("case", [(128, 3), (100, 2), (33, 4), (256, 1), (1024, 1), (31, 2)],
+                         ids=lambda c: f"{c[0]}sq_x{c[1]}")
+def test_long_line_square_transpose_in_place(hc, case):
+    """The in-place transpose of square lines: every element moved once, as
+    numpy's transpose, and each workgroup writes exactly the elements it read
+    (no two workgroups touch the same element)."""
+    M, lines = case
+    total = lines * M * M
+    src = np.empty(total + 1, np.int64)
+    dst = np.empty(total + 1, np.int64)
+    wg = np.empty(total + 1, np.int64)
+    c = hc.hc_long_square(M, lines, _p(src), _p(dst), _p(wg), total + 1)
+    assert c == total
+    src, dst, wg = src[:c], dst[:c], wg[:c]
+    assert np.array_equal(np.sort(src), np.arange(total)) and np.array_equal(np.sort(dst), np.arange(total))
+    moved = np.empty(total, np.int64)
+    moved[dst] = src
+    ref = np.arange(total).reshape(lines, M, M).transpose(0, 2, 1).reshape(-1)
+    assert np.array_equal(moved, ref)
+    order = np.argsort(wg, kind="stable")
+    w, s_, d_ = wg[order], src[order], dst[order]
+    cuts = np.flatnonzero(np.diff(w)) + 1
+    for ss, dd in zip(np.split(s_, cuts), np.split(d_, cuts)):
+        assert np.array_equal(np.sort(ss), np.sort(dd))
