@@ -407,3 +407,31 @@ def test_long_line_without_split_is_an_error(a2afft_mod, cuda, N):
     d = torch.zeros(N, dtype=torch.complex64, device="cuda")
     with pytest.raises(a2afft_mod.A2AFFTError, match="not a product"):
         a2afft_mod.forward_1d_fft(d, N)
+
+
+def test_long_lines_on_two_streams(a2afft_mod, cuda):
+    """Two rectangular-split transforms (8192 = 128 * 64, each with its own
+    stream-ordered workspace) issued on two streams without synchronizing in
+    between: each equals the same transform run alone."""
+    import torch
+    N = 8192
+    rng = np.random.default_rng(77)
+    xa = (rng.standard_normal((64, N)) + 1j * rng.standard_normal((64, N))).astype(np.complex64)
+    xb = (rng.standard_normal((96, N)) + 1j * rng.standard_normal((96, N))).astype(np.complex64)
+    ref_a = torch.from_numpy(xa).cuda()
+    ref_b = torch.from_numpy(xb).cuda()
+    a2afft_mod.forward_1d_fft(ref_a, N)
+    a2afft_mod.forward_1d_fft(ref_b, N)
+    torch.cuda.synchronize()
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    for rep in range(3):
+        a = torch.from_numpy(xa).cuda()
+        b = torch.from_numpy(xb).cuda()
+        torch.cuda.synchronize()
+        with torch.cuda.stream(s1):
+            a2afft_mod.forward_1d_fft(a, N)
+        with torch.cuda.stream(s2):
+            a2afft_mod.forward_1d_fft(b, N)
+        torch.cuda.synchronize()
+        assert np.array_equal(_bits(a.cpu().numpy()), _bits(ref_a.cpu().numpy())), rep
+        assert np.array_equal(_bits(b.cpu().numpy()), _bits(ref_b.cpu().numpy())), rep
