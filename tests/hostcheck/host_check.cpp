@@ -201,4 +201,54 @@ int64_t hc_long_square(int M, int lines, int64_t* src, int64_t* dst, int64_t* wg
     return c;
 }
 
+// Every address the four-step's first FFT pass touches: `lines` lines of
+// N1 * N2 seen as [N1][N2], N1-point FFTs down the N2 columns, shape (T, E)
+// (T = E = 0: the default).  n_out = the element's index along its column,
+// col_out = line * N2 + column.  Returns the count, or a negative status.
+int64_t hc_long_pass_enumerate(int N1, int N2, int lines, int dbl, int T, int E, int64_t* addr, int32_t* n_out,
+                               int64_t* col_out, int64_t cap, int64_t* shape_out) {
+    const int64_t N = int64_t(N1) * N2;
+    const AxisMap fft = make_axis(uint32_t(N1), uint32_t(N1), 0, N2);
+    const AxisMap a1 = make_axis(uint32_t(lines), uint32_t(lines), 0, N);
+    const AxisMap a2 = make_axis(uint32_t(N2), uint32_t(N2), 0, 1);
+    if (T == 0) {
+        const KSel d = default_sel(N1, dbl != 0, 1, fft, a2);
+        T = d.T;
+        E = d.E;
+    }
+    FftGeom g;
+    uint32_t grid;
+    int st;
+    if (fft_len_ok(N1)) {
+        st = resolve_with(fft, a1, a2, 1, N1, T, E, &g, &grid);
+    } else {
+        // the runtime-schedule kernel and the direct DFT: the geometry of
+        // their chosen (T, E), resolved as the dispatcher does
+        if (T <= 0 || E <= 0 || N1 % E) return A2AFFT_E_UNSUPPORTED;
+        const uint32_t tpl = uint32_t(N1 / E);
+        if (!(fft.l % tpl == 0 || tpl % fft.l == 0) || T * int(tpl) > 1024) return A2AFFT_E_UNSUPPORTED;
+        st = resolve_geom(fft, a1, a2, 1, T, &g, &grid, 0, 0);
+    }
+    if (st < 0) return st;
+    const int NT = T * (N1 / E), TPL = N1 / E;
+    shape_out[0] = T;
+    shape_out[1] = E;
+    int64_t c = 0;
+    for (uint32_t b = 0; b < grid; ++b)
+        for (int tid = 0; tid < NT; ++tid) {
+            bool valid;
+            const int64_t base = line_base(g, true, T, TPL, b, tid, &valid);
+            if (!valid) continue;
+            const int t = tid / T;
+            for (int e = 0; e < E; ++e) {
+                if (c >= cap) return -100;
+                addr[c] = base + elem_off(g, e * TPL);
+                n_out[c] = t + e * TPL;
+                col_out[c] = int64_t(b) * T + tid % T;
+                ++c;
+            }
+        }
+    return c;
+}
+
 }  // extern "C"

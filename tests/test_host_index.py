@@ -43,6 +43,8 @@ def hc(tmp_path_factory):
     L.hc_long_transpose.restype = ctypes.c_int64
     L.hc_long_square.argtypes = [ctypes.c_int, ctypes.c_int, P, P, P, ctypes.c_int64]
     L.hc_long_square.restype = ctypes.c_int64
+    L.hc_long_pass_enumerate.argtypes = [ctypes.c_int] * 6 + [P, P, P, ctypes.c_int64, P]
+    L.hc_long_pass_enumerate.restype = ctypes.c_int64
     return L
 
 
@@ -299,3 +301,39 @@ def test_long_line_square_transpose_in_place(hc, case):
     cuts = np.flatnonzero(np.diff(w)) + 1
     for ss, dd in zip(np.split(s_, cuts), np.split(d_, cuts)):
         assert np.array_equal(np.sort(ss), np.sort(dd))
+
+
+@pytest.mark.parametrize("case", [(128, 64, 3), (128, 128, 2), (256, 256, 1), (100, 100, 2), (241, 34, 2),
+                                  (128, 96, 2), (1024, 1024, 1), (64, 128, 2)], ids=lambda c: f"{c[0]}x{c[1]}x{c[2]}")
+def test_long_line_column_pass_addresses(hc, case):
+    """The four-step's column FFT pass (N1-point FFTs down the N2 columns of
+    each line), for the default shape and every power-of-two tile that fits:
+    each element of the lines exactly once, and each column its N1 elements
+    in natural order."""
+    N1, N2, lines = case
+    total = lines * N1 * N2
+    pow2 = N1 & (N1 - 1) == 0
+    # power-of-two lengths: the default and the Stockham shapes; others: the
+    # (T, E) shapes of the runtime-schedule kernel (E dividing N1) and the
+    # direct DFT's one line per workgroup (T = 1, E = N1)
+    Es = (2, 4, 8, 16, 32) if pow2 else [e for e in range(2, 65) if N1 % e == 0] + [N1]
+    shapes = ([(0, 0)] if pow2 else []) + [(T, E) for T in (1, 2, 4, 8, 16, 32, 64) for E in Es]
+    checked = 0
+    for dbl in (0, 1):
+        for T, E in shapes:
+            addr = np.empty(total + 1, np.int64)
+            n = np.empty(total + 1, np.int32)
+            col = np.empty(total + 1, np.int64)
+            shape = np.zeros(2, np.int64)
+            c = hc.hc_long_pass_enumerate(N1, N2, lines, dbl, T, E, _p(addr), _p(n), _p(col), total + 1, _p(shape))
+            if c < 0:
+                assert (T, E) != (0, 0), (N1, N2, dbl)   # the default shape always fits
+                continue
+            assert c == total, (T, E)
+            addr, n, col = addr[:c], n[:c], col[:c]
+            assert np.array_equal(np.sort(addr), np.arange(total))
+            line, r = addr // (N1 * N2), addr % (N1 * N2)
+            assert np.array_equal(r // N2, n)                    # element n1 of its column
+            assert np.array_equal(line * N2 + r % N2, col)      # the column it belongs to
+            checked += 1
+    assert checked >= 2
