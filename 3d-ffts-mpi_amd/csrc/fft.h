@@ -50,9 +50,15 @@ int launch_fft_pass(const AxisPass& pass, void* data, a2afft_dtype dtype, int N,
 // Fixed-schedule Stockham selections only.
 int launch_pattern_copy(const AxisPass& pass, void* data, a2afft_dtype dtype, int N, const KSel& sel, hipStream_t s);
 
+// In-place DFTs of `lines` contiguous lines of N points, any supported N
+// (the batched 1D launchers' dispatcher).
+int fft_lines(void* data, int N, int64_t lines, a2afft_dtype dtype, int dir, hipStream_t s);
+
 // Batched contiguous lines longer than 4096 points (fft_long.hip): four-step
 // over N = N1 * N2 with both factors <= 4096 (long_fft_split), through a
-// workspace from the stream-ordered allocator.
+// workspace from the stream-ordered allocator; lengths with no such split up
+// to kBluesteinMaxN as Bluestein's chirp-z convolution of power-of-two length.
+constexpr int kBluesteinMaxN = 1 << 23;
 int launch_long_fft(void* data, int N, int64_t lines, a2afft_dtype dtype, int dir, hipStream_t s);
 
 // Stockham FFT with the radix schedule in the kernel arguments (KF_MIX,

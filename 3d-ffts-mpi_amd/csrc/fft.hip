@@ -385,16 +385,21 @@ const void* pass_twiddles(const AxisPass& p, int N, int dir, a2afft_dtype dt, co
 
 using namespace a2a;
 
+namespace a2a {
+int fft_lines(void* data, int N, int64_t lines, a2afft_dtype dtype, int dir, hipStream_t s) {
+    if (lines >= (int64_t(1) << 32)) return set_error(A2AFFT_E_SIZE, "1d fft: too many lines");
+    if (N > kDftMaxN) return launch_long_fft(data, N, lines, dtype, dir, s);
+    const PassMaps pm = batched_pass_maps(N, uint32_t(lines));
+    const AxisPass p{pm.fft, pm.a1, pm.a2, 0, 0, 1.0};
+    return launch_fft_pass(p, data, dtype, N, dir, s, nullptr);
+}
+}  // namespace a2a
+
 // Batched in-place DFTs over n_elems/Ng contiguous pencils (a2affts.cu:7-76).
 static int fft_1d(void* data, int Ng, int64_t n_elems, a2afft_dtype dtype, int dir, void* stream) {
     if (!data || Ng <= 0 || n_elems <= 0 || n_elems % Ng)
         return set_error(A2AFFT_E_INVALID_ARG, "1d fft: bad arguments");
-    const int64_t lines = n_elems / Ng;
-    if (lines >= (int64_t(1) << 32)) return set_error(A2AFFT_E_SIZE, "1d fft: too many lines");
-    if (Ng > kDftMaxN) return launch_long_fft(data, Ng, lines, dtype, dir, (hipStream_t)stream);
-    const PassMaps pm = batched_pass_maps(Ng, uint32_t(lines));
-    const AxisPass p{pm.fft, pm.a1, pm.a2, 0, 0, 1.0};
-    return launch_fft_pass(p, data, dtype, Ng, dir, (hipStream_t)stream, nullptr);
+    return fft_lines(data, Ng, n_elems / Ng, dtype, dir, (hipStream_t)stream);
 }
 
 extern "C" int a2afft_forward_1d_fft(void* data, int Ng, int64_t n_elems, a2afft_dtype dtype, void* stream) {

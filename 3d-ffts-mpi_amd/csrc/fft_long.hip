@@ -161,13 +161,167 @@ __global__ __launch_bounds__(256) void twiddle_transpose_square_kernel(HIP_vecto
     if (a != c) store(c * 32u, a * 32u, t1);
 }
 
+// Bluestein's chirp-z form for lengths with no two-factor split:
+//   X[k] = c[k] sum_n (x[n] c[n]) conj(c[k - n]),  c[m] = exp(dir pi i m^2 / N),
+// a cyclic convolution of length M (a power of two >= 2N - 1) done with two
+// M-point FFTs.  chirp: c[0..N) (m^2 reduced mod 2N exactly, long double,
+// rounded once); bhat: the forward M-point FFT of b, b[m] = conj(c[|m|]) for
+// |m| < N (indices mod M), 0 elsewhere, computed in FP64 on the device.
+struct BluePlan {
+    int M = 0;
+    double2* chirp = nullptr;
+    double2* bhat = nullptr;
+};
+
+template <typename R>
+__global__ __launch_bounds__(256) void blue_pre_kernel(const HIP_vector_type<R, 2>* __restrict__ x,
+                                                       HIP_vector_type<R, 2>* __restrict__ a, int N, int M,
+                                                       uint64_t total, const double2* __restrict__ c) {
+    const uint64_t i = uint64_t(blockIdx.x) * 256u + threadIdx.x;
+    if (i >= total) return;
+    const uint64_t line = i / uint64_t(M);
+    const int m = int(i - line * uint64_t(M));
+    HIP_vector_type<R, 2> o;
+    o.x = R(0);
+    o.y = R(0);
+    if (m < N) {
+        const HIP_vector_type<R, 2> v = x[line * uint64_t(N) + uint64_t(m)];
+        const double2 w = c[m];
+        const double vr = double(v.x), vi = double(v.y);
+        o.x = R(vr * w.x - vi * w.y);
+        o.y = R(vr * w.y + vi * w.x);
+    }
+    a[i] = o;
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void blue_mul_kernel(HIP_vector_type<R, 2>* __restrict__ a, int M, uint64_t total,
+                                                       const double2* __restrict__ bhat, double inv_m) {
+    const uint64_t i = uint64_t(blockIdx.x) * 256u + threadIdx.x;
+    if (i >= total) return;
+    const double2 b = bhat[i % uint64_t(M)];
+    const HIP_vector_type<R, 2> v = a[i];
+    const double vr = double(v.x), vi = double(v.y);
+    HIP_vector_type<R, 2> o;
+    o.x = R((vr * b.x - vi * b.y) * inv_m);
+    o.y = R((vr * b.y + vi * b.x) * inv_m);
+    a[i] = o;
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void blue_post_kernel(const HIP_vector_type<R, 2>* __restrict__ a,
+                                                        HIP_vector_type<R, 2>* __restrict__ x, int N, int M,
+                                                        uint64_t total, const double2* __restrict__ c) {
+    const uint64_t i = uint64_t(blockIdx.x) * 256u + threadIdx.x;
+    if (i >= total) return;
+    const uint64_t line = i / uint64_t(N);
+    const int k = int(i - line * uint64_t(N));
+    const HIP_vector_type<R, 2> v = a[line * uint64_t(M) + uint64_t(k)];
+    const double2 w = c[k];
+    const double vr = double(v.x), vi = double(v.y);
+    HIP_vector_type<R, 2> o;
+    o.x = R(vr * w.x - vi * w.y);
+    o.y = R(vr * w.y + vi * w.x);
+    x[i] = o;
+}
+
+// The cached plan for (device, N, dir); built once, synchronously on `s`.
+const BluePlan* blue_plan(int N, int dir, hipStream_t s) {
+    static std::mutex mu;
+    static std::map<std::vector<int64_t>, BluePlan> cache;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+    const std::vector<int64_t> key{dev, N, dir < 0 ? -1 : 1};
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) return &it->second;
+    BluePlan bp;
+    bp.M = 1;
+    while (bp.M < 2 * N - 1) bp.M *= 2;
+    std::vector<double2> c(static_cast<size_t>(N)), b(static_cast<size_t>(bp.M));
+    long double re, im;
+    for (int m = 0; m < N; ++m) {
+        const int64_t sq = int64_t(m) * m % (2 * int64_t(N));
+        twiddle_ld(sq, 2 * int64_t(N), dir, &re, &im);   // exp(dir 2 pi i sq / 2N)
+        c[size_t(m)].x = double(re);
+        c[size_t(m)].y = double(im);
+    }
+    for (int m = 0; m < bp.M; ++m) b[size_t(m)] = make_double2(0.0, 0.0);
+    for (int m = 0; m < N; ++m) {
+        const double2 cc = make_double2(c[size_t(m)].x, -c[size_t(m)].y);
+        b[size_t(m)] = cc;
+        if (m > 0) b[size_t(bp.M - m)] = cc;
+    }
+    if (hipMalloc(&bp.chirp, c.size() * sizeof(double2)) != hipSuccess) return nullptr;
+    if (hipMalloc(&bp.bhat, b.size() * sizeof(double2)) != hipSuccess) {
+        (void)hipFree(bp.chirp);
+        return nullptr;
+    }
+    bool ok = hipMemcpy(bp.chirp, c.data(), c.size() * sizeof(double2), hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(bp.bhat, b.data(), b.size() * sizeof(double2), hipMemcpyHostToDevice) == hipSuccess;
+    ok = ok && fft_lines(bp.bhat, bp.M, 1, A2AFFT_C128, -1, s) >= 0 && hipStreamSynchronize(s) == hipSuccess;
+    if (!ok) {
+        (void)hipFree(bp.chirp);
+        (void)hipFree(bp.bhat);
+        return nullptr;
+    }
+    return &(cache[key] = bp);
+}
+
+int launch_bluestein(void* data, int N, int64_t lines, a2afft_dtype dt, int dir, hipStream_t s) {
+    const BluePlan* bp = blue_plan(N, dir, s);
+    if (!bp) return set_error(A2AFFT_E_ALLOC, "1d fft: Bluestein tables for length %d", N);
+    const int M = bp->M;
+    const size_t esize = dt == A2AFFT_C128 ? 16 : 8;
+    const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(lines, kLongChunkBytes / (int64_t(M) * esize)));
+    void* work = nullptr;
+    A2A_TRY(hip_check(int(hipMallocAsync(&work, size_t(chunk) * M * esize, s)), "1d fft: workspace"));
+    int st = A2AFFT_OK;
+    for (int64_t l0 = 0; l0 < lines && st >= 0; l0 += chunk) {
+        const int64_t cl = std::min<int64_t>(chunk, lines - l0);
+        char* D = static_cast<char*>(data) + size_t(l0) * N * esize;
+        const uint64_t tm = uint64_t(cl) * M, tn = uint64_t(cl) * N;
+        const dim3 gm(uint32_t((tm + 255) / 256)), gn(uint32_t((tn + 255) / 256));
+        if (dt == A2AFFT_C128)
+            hipLaunchKernelGGL(blue_pre_kernel<double>, gm, dim3(256), 0, s, reinterpret_cast<const double2*>(D),
+                               static_cast<double2*>(work), N, M, tm, bp->chirp);
+        else
+            hipLaunchKernelGGL(blue_pre_kernel<float>, gm, dim3(256), 0, s, reinterpret_cast<const float2*>(D),
+                               static_cast<float2*>(work), N, M, tm, bp->chirp);
+        st = hip_check(int(hipGetLastError()), "1d fft: Bluestein pre");
+        if (st >= 0) st = fft_lines(work, M, cl, dt, -1, s);
+        if (st < 0) break;
+        if (dt == A2AFFT_C128)
+            hipLaunchKernelGGL(blue_mul_kernel<double>, gm, dim3(256), 0, s, static_cast<double2*>(work), M, tm,
+                               bp->bhat, 1.0 / double(M));
+        else
+            hipLaunchKernelGGL(blue_mul_kernel<float>, gm, dim3(256), 0, s, static_cast<float2*>(work), M, tm,
+                               bp->bhat, 1.0 / double(M));
+        st = hip_check(int(hipGetLastError()), "1d fft: Bluestein product");
+        if (st >= 0) st = fft_lines(work, M, cl, dt, +1, s);
+        if (st < 0) break;
+        if (dt == A2AFFT_C128)
+            hipLaunchKernelGGL(blue_post_kernel<double>, gn, dim3(256), 0, s, static_cast<const double2*>(work),
+                               reinterpret_cast<double2*>(D), N, M, tn, bp->chirp);
+        else
+            hipLaunchKernelGGL(blue_post_kernel<float>, gn, dim3(256), 0, s, static_cast<const float2*>(work),
+                               reinterpret_cast<float2*>(D), N, M, tn, bp->chirp);
+        st = hip_check(int(hipGetLastError()), "1d fft: Bluestein post");
+    }
+    const int fs = hip_check(int(hipFreeAsync(work, s)), "1d fft: workspace free");
+    return st < 0 ? st : fs;
+}
+
 }  // namespace
 
 int launch_long_fft(void* data, int N, int64_t lines, a2afft_dtype dt, int dir, hipStream_t s) {
     int N1 = 0, N2 = 0;
-    if (!long_fft_split(N, &N1, &N2))
-        return set_error(A2AFFT_E_UNSUPPORTED, "1d fft: length %d is not a product of two lengths <= %d", N,
-                         kDftMaxN);
+    if (!long_fft_split(N, &N1, &N2)) {
+        if (N <= kBluesteinMaxN) return launch_bluestein(data, N, lines, dt, dir, s);
+        return set_error(A2AFFT_E_UNSUPPORTED,
+                         "1d fft: length %d is neither a product of two lengths <= %d nor at most %d", N, kDftMaxN,
+                         kBluesteinMaxN);
+    }
     const size_t esize = dt == A2AFFT_C128 ? 16 : 8;
     const LongTw* tw = long_twiddles(N, dir);
     if (!tw) return set_error(A2AFFT_E_ALLOC, "1d fft: twiddles for length %d", N);

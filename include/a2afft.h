@@ -163,9 +163,12 @@ int a2afft_alltoall(const void* send, void* recv, int world_size, int64_t nlocal
  * into a workspace, N2-point FFTs there, a copy back.  The workspace (at most
  * 256 MiB, lines are processed in chunks) comes from the stream-ordered
  * allocator on `stream` and is released in stream order; square lengths
- * (N1 = N2) transpose in place and take none.  Lengths above 4096
- * with no such split (primes, lengths above 4096^2) return
- * A2AFFT_E_UNSUPPORTED. */
+ * (N1 = N2) transpose in place and take none.  Lines above 4096 with no such
+ * split (a prime factor above 4096), up to 2^23, run as Bluestein's chirp-z
+ * convolution over two power-of-two FFTs of length M >= 2Ng - 1, with a
+ * workspace from the same allocator; their chirp and filter tables are built
+ * on the first call for (Ng, direction), which synchronizes `stream` once.
+ * Longer lines with no split return A2AFFT_E_UNSUPPORTED. */
 int a2afft_forward_1d_fft(void* data, int Ng, int64_t n_elems, a2afft_dtype dtype, void* stream);
 int a2afft_inverse_1d_fft(void* data, int Ng, int64_t n_elems, a2afft_dtype dtype, void* stream);
 /* launch_scale_fft (a2affts.cu:78-92,108-116): out = data / Ng^3 (64-bit Ng^3). */

@@ -41,10 +41,11 @@ def test_driver_reference_check(cuda, tmp_path, case):
 
 
 def test_unsupported_length_is_an_error(cuda, tmp_path):
-    """A prime line length above 4096 has no kernel and no split into two
-    kernel lengths: a clean error, not a launch."""
+    """A line longer than 4096^2 with no split into two kernel lengths, and
+    above the chirp-z limit (2^23): a clean error, not a launch."""
     import torch
     import a2afft
-    d = torch.zeros(2 * 4099, dtype=torch.complex128, device="cuda")
-    with pytest.raises(a2afft.A2AFFTError, match="not a product of two lengths"):
-        a2afft.forward_1d_fft(d, 4099)
+    N = (1 << 24) + 1
+    d = torch.zeros(N, dtype=torch.complex64, device="cuda")
+    with pytest.raises(a2afft.A2AFFTError, match="neither a product of two lengths"):
+        a2afft.forward_1d_fft(d, N)

@@ -400,13 +400,51 @@ def test_long_line_impulse(a2afft_mod, cuda):
     assert np.max(np.abs(d.cpu().numpy() - ref)) < 1e-14
 
 
-@pytest.mark.parametrize("N", [8191, 4099 * 4097])
+@pytest.mark.parametrize("N", [(1 << 24) + 1, 4099 * 4097])
 def test_long_line_without_split_is_an_error(a2afft_mod, cuda, N):
-    """A prime above 4096, or a length above 4096^2, has no two-factor split."""
+    """Above 2^23 a length needs a split into two factors <= 4096; these have
+    none (one is above 4096^2, the other has the prime factor 4099)."""
     import torch
     d = torch.zeros(N, dtype=torch.complex64, device="cuda")
-    with pytest.raises(a2afft_mod.A2AFFTError, match="not a product"):
+    with pytest.raises(a2afft_mod.A2AFFTError, match="neither a product"):
         a2afft_mod.forward_1d_fft(d, N)
+
+
+# Lines above 4096 with no two-factor split (a prime factor above 4096): Bluestein's
+# chirp-z convolution over power-of-two FFTs.  The error of two M-point FFTs
+# (M < 4N) plus the chirp products: bounded here by 3 log2(N) times the
+# per-length tolerance.
+BLUESTEIN = [(4099, 2), (8191, 2), (12297, 1)]
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+@pytest.mark.parametrize("case", BLUESTEIN, ids=lambda c: f"N{c[0]}")
+def test_chirp_z_lines_vs_oracle(a2afft_mod, cuda, case, dtype):
+    import torch
+    N, lines = case
+    rng = np.random.default_rng(N)
+    x = rng.standard_normal((lines, N)) + 1j * rng.standard_normal((lines, N))
+    x = x.astype(np.complex128 if dtype == "c128" else np.complex64)
+    for direction, fn in ((-1, a2afft_mod.forward_1d_fft), (+1, a2afft_mod.inverse_1d_fft)):
+        ref = O.fft_batched(x.astype(np.complex128), direction)
+        d = torch.from_numpy(x).cuda()
+        fn(d, N)
+        torch.cuda.synchronize()
+        assert _relerr(d.cpu().numpy(), ref) < TOL_1D[dtype] * 3 * np.log2(N), direction
+
+
+def test_chirp_z_impulse(a2afft_mod, cuda):
+    """delta(n - n0) -> exp(-2 pi i k n0 / N) at the prime N = 8191."""
+    import torch
+    N, n0 = 8191, 4321
+    x = np.zeros(N, np.complex128)
+    x[n0] = 1.0
+    d = torch.from_numpy(x).cuda()
+    a2afft_mod.forward_1d_fft(d, N)
+    torch.cuda.synchronize()
+    k = np.arange(N, dtype=np.int64)
+    ref = np.exp(-2j * np.pi * ((k * n0) % N) / N)
+    assert np.max(np.abs(d.cpu().numpy() - ref)) < 1e-13
 
 
 def test_long_lines_on_two_streams(a2afft_mod, cuda):
