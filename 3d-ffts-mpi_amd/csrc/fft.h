@@ -59,6 +59,10 @@ int fft_lines(void* data, int N, int64_t lines, a2afft_dtype dtype, int dir, hip
 // workspace from the stream-ordered allocator; lengths with no such split up
 // to kBluesteinMaxN as Bluestein's chirp-z convolution of power-of-two length.
 constexpr int kBluesteinMaxN = 1 << 23;
+// The chirp-z form for any N <= kBluesteinMaxN, and whether it beats the
+// direct DFT at a length the Stockham kernels do not cover.
+int launch_chirp_z(void* data, int N, int64_t lines, a2afft_dtype dtype, int dir, hipStream_t s);
+bool chirp_z_preferred(int N);
 int launch_long_fft(void* data, int N, int64_t lines, a2afft_dtype dtype, int dir, hipStream_t s);
 
 // Stockham FFT with the radix schedule in the kernel arguments (KF_MIX,

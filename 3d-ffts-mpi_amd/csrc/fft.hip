@@ -391,6 +391,9 @@ int fft_lines(void* data, int N, int64_t lines, a2afft_dtype dtype, int dir, hip
     if (N > kDftMaxN) return launch_long_fft(data, N, lines, dtype, dir, s);
     const PassMaps pm = batched_pass_maps(N, uint32_t(lines));
     const AxisPass p{pm.fft, pm.a1, pm.a2, 0, 0, 1.0};
+    // lengths only the direct DFT covers: the chirp-z form where it is faster
+    if ((fft_default_sel(p, dtype, N).flags & KF_DFT) && chirp_z_preferred(N))
+        return launch_chirp_z(data, N, lines, dtype, dir, s);
     return launch_fft_pass(p, data, dtype, N, dir, s, nullptr);
 }
 }  // namespace a2a

@@ -158,7 +158,9 @@ int a2afft_alltoall(const void* send, void* recv, int world_size, int64_t nlocal
  * 2^a*3^b*5^c length runs as a Stockham FFT (fixed schedules for powers of
  * two and short mixed lengths, the radix schedule in the kernel arguments for
  * the rest); lengths with other prime factors as a direct O(Ng^2) DFT per
- * line.  Longer lines, up to 4096^2, run as four steps over Ng = N1 * N2
+ * line, or, once Ng^2 >= 108 M (M the power of two >= 2Ng - 1, where it is
+ * measured faster), as the chirp-z convolution described below.  Longer
+ * lines, up to 4096^2, run as four steps over Ng = N1 * N2
  * (both factors <= 4096): N1-point FFTs in place, a twiddle-and-transpose
  * into a workspace, N2-point FFTs there, a copy back.  The workspace (at most
  * 256 MiB, lines are processed in chunks) comes from the stream-ordered

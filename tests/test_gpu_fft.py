@@ -410,11 +410,12 @@ def test_long_line_without_split_is_an_error(a2afft_mod, cuda, N):
         a2afft_mod.forward_1d_fft(d, N)
 
 
-# Lines above 4096 with no two-factor split (a prime factor above 4096): Bluestein's
+# Lines above 4096 with no two-factor split (a prime factor above 4096), and
+# lines up to 4096 that only the direct DFT covers once N^2 >= 108 M: Bluestein's
 # chirp-z convolution over power-of-two FFTs.  The error of two M-point FFTs
 # (M < 4N) plus the chirp products: bounded here by 3 log2(N) times the
 # per-length tolerance.
-BLUESTEIN = [(4099, 2), (8191, 2), (12297, 1)]
+BLUESTEIN = [(4099, 2), (8191, 2), (12297, 1), (251, 8), (509, 4), (2039, 2), (4093, 2), (4095, 2)]
 _CHIRP_REF = {}
 
 
@@ -484,3 +485,26 @@ def test_long_lines_on_two_streams(a2afft_mod, cuda):
         torch.cuda.synchronize()
         assert np.array_equal(_bits(a.cpu().numpy()), _bits(ref_a.cpu().numpy())), rep
         assert np.array_equal(_bits(b.cpu().numpy()), _bits(ref_b.cpu().numpy())), rep
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("case", [((3, 5, 4095), 0, 2), ((4093, 2, 4), 1, 0), ((6, 2039, 4), 2, 1)],
+                         ids=lambda c: f"{c[0]}_pass{c[1]}")
+def test_direct_dft_in_plan_passes(a2afft_mod, cuda, case, prec):
+    """Plan passes keep the direct DFT for lengths no Stockham kernel covers
+    (the 1D launchers take the chirp-z form there), contiguous and strided:
+    one pass against the oracle's FFT along that axis."""
+    import torch
+    shape, pass_, axis = case
+    N = shape[axis]
+    rng = np.random.default_rng(N + pass_)
+    x = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    dt = np.complex128 if prec == "fp64" else np.complex64
+    x = x.astype(dt)
+    plan = a2afft_mod.GridDfft(shape, prec)
+    d = torch.from_numpy(x).cuda()
+    a2afft_mod.execute_pass(plan, d, pass_, -1)
+    torch.cuda.synchronize()
+    ref = np.moveaxis(O.fft_batched(np.moveaxis(x.astype(np.complex128), axis, -1), -1), -1, axis)
+    tol = TOL_1D["c128" if prec == "fp64" else "c64"] * np.sqrt(N)
+    assert _relerr(d.cpu().numpy(), ref) < tol
