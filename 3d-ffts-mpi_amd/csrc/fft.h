@@ -69,11 +69,12 @@ int launch_long_fft(void* data, int N, int64_t lines, a2afft_dtype dtype, int di
 // fft_mixed.hip) for 2^a 3^b 5^c lengths the fixed schedules do not cover.
 constexpr int kMixMaxPasses = 16;
 struct MixedPlan {
-    int N, E, npass, T, TPL, twsize;
+    int N, E, npass, T, TPL, twsize, NT;   // NT: threads per workgroup
     int radix[kMixMaxPasses], ns[kMixMaxPasses], toff[kMixMaxPasses];
 };
 bool mixed_plan(const AxisPass& pass, int N, bool dbl, MixedPlan* mp);
-const void* mixed_kernel_for(int E, bool dbl, int strided, int dir);
+const void* mixed_kernel_for(int N, int T, int E, bool dbl, int strided, int dir);
+bool mixed_is_wide(int N, int T, int E);   // more than 256 threads: strided tiles only
 int mixed_lds_bytes(const MixedPlan& mp, bool dbl);
 const void* mixed_twiddles(const MixedPlan& mp, int dir, a2afft_dtype dtype);
 

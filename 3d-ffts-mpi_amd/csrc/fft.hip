@@ -108,7 +108,7 @@ void init_registry() {
 }  // namespace
 
 const void* find_kernel(int N, bool dbl, int strided, const KSel& s, int dir) {
-    if (is_mix(s)) return s.flags == KF_MIX ? mixed_kernel_for(s.E, dbl, strided, dir) : nullptr;
+    if (is_mix(s)) return s.flags == KF_MIX ? mixed_kernel_for(N, s.T, s.E, dbl, strided, dir) : nullptr;
     if (is_dft(s)) return (s.T == 1 && s.E == N && s.flags == KF_DFT && fft_length_supported(N)) ? dft_kernel_for(dbl, dir)
                                                                                                     : nullptr;
     init_registry();
@@ -266,7 +266,7 @@ int launch_fft_pass(const AxisPass& p, void* data, a2afft_dtype dt, int N, int d
     int st;
     if (is_mix(ks)) {
         void* args[] = {&data, &g, (void*)&tw, &mp};
-        st = hip_check(int(hipLaunchKernel(fn, dim3(grid), dim3(256), args,
+        st = hip_check(int(hipLaunchKernel(fn, dim3(grid), dim3(mp.NT), args,
                                            size_t(mixed_lds_bytes(mp, dt == A2AFFT_C128)), s)),
                        "mixed-radix launch");
     } else if (is_dft(ks)) {

@@ -24,6 +24,13 @@ namespace {
 
 constexpr int kMixE[] = {9, 10, 12, 15, 18, 20, 25, 30, 60};
 constexpr int kMixThreads = 256;
+// Strided tiles of long lines: with 256 threads a 768-point line (TPL 64)
+// leaves room for 4 lines per tile, i.e. 32-byte row segments.  The wide
+// kernels take up to 1024 threads and 128 KiB of LDS so such tiles reach 16
+// lines; they hold at most 48 data registers per thread (E <= 24 complex64,
+// E <= 12 complex128) to stay within the 128 VGPRs that 16 waves allow.
+constexpr int kMixWideThreads = 1024;
+constexpr int kMixWideLds = 131072;
 
 int gcd_i(int a, int b) { return b ? gcd_i(b, a % b) : a; }
 
@@ -73,8 +80,8 @@ __device__ __forceinline__ void mix_pass(V (&v)[E], V* lds, const V* __restrict_
     }
 }
 
-template <typename real, int E, bool STRIDED, int DIR>
-__global__ __launch_bounds__(kMixThreads) void mixed_kernel(typename CT<real>::V* __restrict__ data, FftGeom g,
+template <typename real, int E, bool STRIDED, int DIR, int NTB>
+__global__ __launch_bounds__(NTB) void mixed_kernel(typename CT<real>::V* __restrict__ data, FftGeom g,
                                                             const typename CT<real>::V* __restrict__ tw, MixedPlan mp) {
     using V = typename CT<real>::V;
     extern __shared__ __align__(16) unsigned char mix_smem[];
@@ -124,8 +131,38 @@ __global__ __launch_bounds__(kMixThreads) void mixed_kernel(typename CT<real>::V
 template <typename real, int E>
 const void* mixed_fn(bool strided, int dir) {
     if (strided)
-        return dir < 0 ? (const void*)&mixed_kernel<real, E, true, -1> : (const void*)&mixed_kernel<real, E, true, 1>;
-    return dir < 0 ? (const void*)&mixed_kernel<real, E, false, -1> : (const void*)&mixed_kernel<real, E, false, 1>;
+        return dir < 0 ? (const void*)&mixed_kernel<real, E, true, -1, kMixThreads>
+                       : (const void*)&mixed_kernel<real, E, true, 1, kMixThreads>;
+    return dir < 0 ? (const void*)&mixed_kernel<real, E, false, -1, kMixThreads>
+                   : (const void*)&mixed_kernel<real, E, false, 1, kMixThreads>;
+}
+
+template <typename real, int E>
+const void* mixed_wide_fn(int dir) {
+    return dir < 0 ? (const void*)&mixed_kernel<real, E, true, -1, kMixWideThreads>
+                   : (const void*)&mixed_kernel<real, E, true, 1, kMixWideThreads>;
+}
+
+bool wide_ok(int E, bool dbl) { return E * (dbl ? 4 : 2) <= 48; }
+
+template <typename real>
+const void* mixed_wide_fn_e(int E, int dir) {
+    if (sizeof(real) == 8 && E > 12) return nullptr;   // beyond wide_ok's register budget
+    switch (E) {
+    case 9: return mixed_wide_fn<real, 9>(dir);
+    case 10: return mixed_wide_fn<real, 10>(dir);
+    case 12: return mixed_wide_fn<real, 12>(dir);
+    default: break;
+    }
+    if constexpr (sizeof(real) == 4) {
+        switch (E) {
+        case 15: return mixed_wide_fn<real, 15>(dir);
+        case 18: return mixed_wide_fn<real, 18>(dir);
+        case 20: return mixed_wide_fn<real, 20>(dir);
+        default: break;
+        }
+    }
+    return nullptr;
 }
 
 template <typename real>
@@ -201,7 +238,24 @@ bool mixed_plan(const AxisPass& ps, int N, bool dbl, MixedPlan* mp) {
                 q.T = tt;
                 break;
             }
+        // Wide tiles only where the row segments are shorter than 128 bytes:
+        // beyond that they cost occupancy (one 96 KiB workgroup per CU) and
+        // gain nothing (384-point complex128 lines: T=16 was slower than 8).
+        const int seg_lines = 128 / esize;
+        if (q.T < seg_lines && wide_ok(q.E, dbl)) {
+            const int wide_lines = kMixWideLds / (N * esize);
+            for (int tt : {16, 8, 4, 2}) {
+                if (tt > seg_lines) continue;
+                if (tt <= q.T) break;
+                if (tt * q.TPL <= kMixWideThreads && tt <= wide_lines && ps.a2.s_elem == 1 &&
+                    ps.a2.l % uint32_t(tt) == 0 && ps.a2.n % uint32_t(tt) == 0) {
+                    q.T = tt;
+                    break;
+                }
+            }
+        }
     }
+    q.NT = mixed_is_wide(N, q.T, q.E) ? (q.T * q.TPL + 63) / 64 * 64 : kMixThreads;
     int ns = 1, off = 0;
     for (int i = 0; i < q.npass; ++i) {
         q.ns[i] = ns;
@@ -214,7 +268,13 @@ bool mixed_plan(const AxisPass& ps, int N, bool dbl, MixedPlan* mp) {
     return true;
 }
 
-const void* mixed_kernel_for(int E, bool dbl, int strided, int dir) {
+bool mixed_is_wide(int N, int T, int E) { return E > 0 && T * (N / E) > kMixThreads; }
+
+const void* mixed_kernel_for(int N, int T, int E, bool dbl, int strided, int dir) {
+    if (mixed_is_wide(N, T, E)) {
+        if (!strided || !wide_ok(E, dbl) || T * (N / E) > kMixWideThreads) return nullptr;
+        return dbl ? mixed_wide_fn_e<double>(E, dir) : mixed_wide_fn_e<float>(E, dir);
+    }
     if (dbl) return E == 60 ? nullptr : mixed_fn_e<double>(E, strided != 0, dir);
     if (E == 60) return mixed_fn<float, 60>(strided != 0, dir);
     return mixed_fn_e<float>(E, strided != 0, dir);

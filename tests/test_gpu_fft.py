@@ -309,6 +309,39 @@ def test_grid_plan_non_cubic_vs_numpy(a2afft_mod, cuda, shape, prec):
     assert _relerr(d.cpu().numpy(), x.astype(np.complex128)) < 2 * tol
 
 
+# (grid shape, precision, a tile the plan must choose): strided passes of long
+# 2^a 3^b 5^c lines whose tiles need more than 256 threads (wide runtime-schedule
+# kernels, up to 1024 threads and 128 KiB of LDS)
+WIDE_MIX_CASES = [((384, 384, 16), "fp32", "x T=16 E=12 NT=512"), ((768, 32, 16), "fp32", "x T=16 E=12 NT=1024"),
+                  ((768, 4, 16), "fp64", "x T=8 E=12 NT=512"), ((2560, 2, 16), "fp32", "x T=4 E=10 NT=1024"),
+                  ((1536, 2, 16), "fp32", "x T=8 E=12 NT=1024"), ((640, 8, 16), "fp32", "x T=16 E=10 NT=1024"),
+                  ((16, 768, 32), "fp32", "y T=16 E=12 NT=1024")]
+
+
+@pytest.mark.parametrize("case", WIDE_MIX_CASES, ids=lambda c: f"{c[0]}_{c[1]}")
+def test_wide_mixed_tiles_vs_numpy(a2afft_mod, cuda, case):
+    """Wide strided tiles of the runtime-schedule Stockham kernel: the plan
+    picks them, and forward and inverse match numpy's fftn in FP64 to the
+    precision's bound."""
+    import torch
+    from a2afft import inputs
+    shape, prec, chosen = case
+    n = int(np.prod(shape))
+    x = inputs.uniform(13, n).reshape(shape).astype(np.complex128 if prec == "fp64" else np.complex64)
+    plan = a2afft_mod.GridDfft(shape, prec)
+    assert chosen in plan.describe(), plan.describe()
+    d = torch.from_numpy(x).cuda()
+    plan.forward(d)
+    torch.cuda.synchronize()
+    ref = np.fft.fftn(x.astype(np.complex128))
+    lg = np.log2(max(shape))
+    tol = 1e-15 * lg if prec == "fp64" else 1.5e-7 * lg
+    assert _relerr(d.cpu().numpy(), ref) < tol
+    plan.inverse(d)
+    torch.cuda.synchronize()
+    assert _relerr(d.cpu().numpy(), x.astype(np.complex128)) < 2 * tol
+
+
 # (grid shape, precision, [(pass, T, E)]): the shapes built with half-width
 # LDS exchanges (KF_HALFX) on every pass of a grid that has them
 HALFX_CASES = [((256, 256, 256), "fp64", [(0, 8, 16), (0, 16, 16), (1, 8, 16), (1, 4, 16), (2, 8, 16)]),
