@@ -4,7 +4,8 @@
 // thread E is a template parameter, so nine E values cover every such length.
 //
 // Schedule: the power-of-two factor as radix-4 passes (one radix 2 for an odd
-// exponent) or as radix-2 passes, whichever allows the smaller E, then the
+// exponent), or as radix-2 passes where those allow a smaller E and the
+// radix-4 E would need more than 48 data registers, then the
 // 3s, then the 5s.  E is the smallest built value (at most 60 complex64 or 30
 // complex128 values) that every radix divides and that divides N; a thread holds
 // the elements n = t + e*TPL (TPL = N/E) of one line, exactly as in the
@@ -197,7 +198,7 @@ bool mixed_plan(const AxisPass& ps, int N, bool dbl, MixedPlan* mp) {
     const int esize = dbl ? 16 : 8;
     MixedPlan q{};
     bool found = false;
-    for (int twos = 0; twos < 2; ++twos) {   // the variant with the smaller E (fewer registers)
+    for (int twos = 0; twos < 2; ++twos) {   // radix-4 passes first, then radix-2 passes
         MixedPlan c{};
         c.N = N;
         int np = 0;
@@ -220,7 +221,10 @@ bool mixed_plan(const AxisPass& ps, int N, bool dbl, MixedPlan* mp) {
                 break;
             }
         if (c.E == 0 || N / c.E > kMixThreads) continue;
-        if (!found || c.E < q.E) q = c;
+        // The radix-4 schedule (fewer passes, fewer LDS exchanges) unless its
+        // data needs more than 48 registers and the radix-2 one needs fewer
+        // (640-point complex64 lines: E=20 in 5 passes, not E=10 in 8).
+        if (!found || (c.E < q.E && q.E * (dbl ? 4 : 2) > 48)) q = c;
         found = true;
     }
     if (!found) return false;

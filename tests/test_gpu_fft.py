@@ -313,8 +313,8 @@ def test_grid_plan_non_cubic_vs_numpy(a2afft_mod, cuda, shape, prec):
 # 2^a 3^b 5^c lines whose tiles need more than 256 threads (wide runtime-schedule
 # kernels, up to 1024 threads and 128 KiB of LDS)
 WIDE_MIX_CASES = [((384, 384, 16), "fp32", "x T=16 E=12 NT=512"), ((768, 32, 16), "fp32", "x T=16 E=12 NT=1024"),
-                  ((768, 4, 16), "fp64", "x T=8 E=12 NT=512"), ((2560, 2, 16), "fp32", "x T=4 E=10 NT=1024"),
-                  ((1536, 2, 16), "fp32", "x T=8 E=12 NT=1024"), ((640, 8, 16), "fp32", "x T=16 E=10 NT=1024"),
+                  ((768, 4, 16), "fp64", "x T=8 E=12 NT=512"), ((2560, 2, 16), "fp32", "x T=4 E=20 NT=512"),
+                  ((1536, 2, 16), "fp32", "x T=8 E=12 NT=1024"), ((640, 8, 16), "fp32", "x T=16 E=20 NT=512"),
                   ((16, 768, 32), "fp32", "y T=16 E=12 NT=1024")]
 
 
