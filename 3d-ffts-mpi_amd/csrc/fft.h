@@ -72,7 +72,8 @@ struct MixedPlan {
     int N, E, npass, T, TPL, twsize, NT;   // NT: threads per workgroup
     int radix[kMixMaxPasses], ns[kMixMaxPasses], toff[kMixMaxPasses];
 };
-bool mixed_plan(const AxisPass& pass, int N, bool dbl, MixedPlan* mp);
+// force_T > 0: that tile (lines per workgroup) if it fits the pass, else false.
+bool mixed_plan(const AxisPass& pass, int N, bool dbl, MixedPlan* mp, int force_T = 0);
 const void* mixed_kernel_for(int N, int T, int E, bool dbl, int strided, int dir);
 bool mixed_is_wide(int N, int T, int E);   // more than 256 threads: strided tiles only
 int mixed_lds_bytes(const MixedPlan& mp, bool dbl);

@@ -133,7 +133,13 @@ int fft_candidates(const AxisPass& p, a2afft_dtype dt, int N, KSel* out, int max
     }
     if (n == 0 && fft_length_supported(N)) {
         const KSel f = fallback_sel(p, dt, N);
-        if (is_mix(f) && n < max) out[n++] = f;
+        if (is_mix(f) && n < max) {
+            out[n++] = f;
+            // every other tile the runtime-schedule kernel accepts on this pass
+            MixedPlan mp;
+            for (int tt : {1, 2, 4, 8, 16, 32})
+                if (tt != f.T && n < max && mixed_plan(p, N, dbl, &mp, tt)) out[n++] = KSel{tt, mp.E, KF_MIX};
+        }
         if (n < max) out[n++] = dft_sel(N);
     }
     return n;
@@ -212,7 +218,7 @@ static int resolve(const AxisPass& p, a2afft_dtype dt, int N, const KSel& s, Fft
                    MixedPlan* mp_out = nullptr) {
     if (is_mix(s)) {
         MixedPlan mp;
-        if (!mixed_plan(p, N, dt == A2AFFT_C128, &mp) || mp.T != s.T || mp.E != s.E)
+        if (!mixed_plan(p, N, dt == A2AFFT_C128, &mp, s.T) || mp.T != s.T || mp.E != s.E)
             return set_error(A2AFFT_E_UNSUPPORTED, "fft pass: length %d (T=%d, E=%d) not supported", N, s.T, s.E);
         if (p.fft.n != uint32_t(N)) return set_error(A2AFFT_E_INVALID_ARG, "fft pass: bad geometry (N=%d)", N);
         const int st = resolve_geom(p.fft, p.a1, p.a2, p.strided, mp.T, g, grid, p.a2_start, p.a2_count);

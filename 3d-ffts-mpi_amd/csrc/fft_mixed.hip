@@ -187,7 +187,7 @@ const void* mixed_fn_e(int E, bool strided, int dir) {
 // layout of a line computes the same bits; the pass decides the tile (T) and
 // whether the shape fits at all (additive offsets: the axis block extent and
 // TPL divide one another).
-bool mixed_plan(const AxisPass& ps, int N, bool dbl, MixedPlan* mp) {
+bool mixed_plan(const AxisPass& ps, int N, bool dbl, MixedPlan* mp, int force_T) {
     if (N < 2 || N > kDftMaxN || is_pow2(N)) return false;
     int m = N, a = 0, b3 = 0, c5 = 0;
     while (m % 2 == 0) { m /= 2; ++a; }
@@ -232,7 +232,20 @@ bool mixed_plan(const AxisPass& ps, int N, bool dbl, MixedPlan* mp) {
     if (!(ps.fft.l % uint32_t(q.TPL) == 0 || uint32_t(q.TPL) % ps.fft.l == 0)) return false;
     const int lds_lines = 65536 / (N * esize);
     if (lds_lines < 1) return false;
-    if (!ps.strided) {
+    if (force_T > 0) {   // a tile the caller asks for (autotune, tuning cache): accept it if it fits
+        const int tt = force_T;
+        const bool narrow = tt * q.TPL <= kMixThreads && tt <= lds_lines;
+        if (!ps.strided) {
+            if (!narrow) return false;
+        } else {
+            const bool wide = wide_ok(q.E, dbl) && tt * q.TPL <= kMixWideThreads &&
+                              tt <= kMixWideLds / (N * esize);
+            if (!narrow && !wide) return false;
+            if (tt > 1 && !(ps.a2.s_elem == 1 && ps.a2.l % uint32_t(tt) == 0 && ps.a2.n % uint32_t(tt) == 0))
+                return false;
+        }
+        q.T = tt;
+    } else if (!ps.strided) {
         q.T = std::min(kMixThreads / q.TPL, lds_lines);
     } else {
         q.T = 1;

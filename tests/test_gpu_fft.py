@@ -342,6 +342,35 @@ def test_wide_mixed_tiles_vs_numpy(a2afft_mod, cuda, case):
     assert _relerr(d.cpu().numpy(), x.astype(np.complex128)) < 2 * tol
 
 
+@pytest.mark.parametrize("case", [((768, 32, 16), "fp32", 1, 12, (1, 2, 4, 8, 16)),
+                                  ((768, 4, 16), "fp64", 1, 12, (1, 2, 4, 8)),
+                                  ((16, 16, 768), "fp32", 0, 12, (1, 2, 4)),
+                                  ((16, 960, 16), "fp32", 2, 30, (1, 2, 4, 8))],
+                         ids=lambda c: f"{c[0]}_{c[1]}_pass{c[2]}")
+def test_mixed_tiles_bit_identical(a2afft_mod, cuda, case):
+    """Every tile the runtime-schedule kernel accepts (narrow and wide; the
+    autotune candidates) computes each line with the same arithmetic: forced on
+    its pass, each reproduces the default plan bit for bit."""
+    import torch
+    shape, prec, pass_, E, tiles = case
+    dt = torch.complex128 if prec == "fp64" else torch.complex64
+    x = torch.empty(shape, dtype=dt, device="cuda")
+    a2afft_mod.fill_random(x, 17, "uniform")
+    base = a2afft_mod.GridDfft(shape, prec)
+    ref = x.clone()
+    base.forward(ref)
+    torch.cuda.synchronize()
+    ref_bits = _bits(ref.cpu().numpy())
+    for T in tiles:
+        plan = a2afft_mod.GridDfft(shape, prec)
+        plan.set_kernel(pass_, T, E, 128)
+        assert f"T={T} E={E} " in plan.describe().splitlines()[pass_], plan.describe()
+        y = x.clone()
+        plan.forward(y)
+        torch.cuda.synchronize()
+        assert np.array_equal(_bits(y.cpu().numpy()), ref_bits), (T, E)
+
+
 # (grid shape, precision, [(pass, T, E)]): the shapes built with half-width
 # LDS exchanges (KF_HALFX) on every pass of a grid that has them
 HALFX_CASES = [((256, 256, 256), "fp64", [(0, 8, 16), (0, 16, 16), (1, 8, 16), (1, 4, 16), (2, 8, 16)]),
