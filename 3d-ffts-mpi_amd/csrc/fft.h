@@ -71,6 +71,7 @@ constexpr int kMixMaxPasses = 16;
 struct MixedPlan {
     int N, E, npass, T, TPL, twsize, NT;   // NT: threads per workgroup
     int radix[kMixMaxPasses], ns[kMixMaxPasses], toff[kMixMaxPasses];
+    FastDiv dns[kMixMaxPasses];   // division by ns[p]
 };
 // force_T > 0: that tile (lines per workgroup) if it fits the pass, else false.
 bool mixed_plan(const AxisPass& pass, int N, bool dbl, MixedPlan* mp, int force_T = 0);

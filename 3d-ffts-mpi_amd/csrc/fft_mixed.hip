@@ -48,6 +48,7 @@ __device__ __forceinline__ void mix_pass(V (&v)[E], V* lds, const V* __restrict_
     if constexpr (E % R == 0) {
         constexpr int NB = E / R;
         const int TPL = mp.TPL, Ns = mp.ns[p], toff = mp.toff[p];
+        const FastDiv dns = mp.dns[p];   // j / Ns as a multiply-shift, not a division sequence
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
             V x[R];
@@ -55,7 +56,7 @@ __device__ __forceinline__ void mix_pass(V (&v)[E], V* lds, const V* __restrict_
             for (int r = 0; r < R; ++r) x[r] = v[b + r * NB];
             if (p > 0) {
                 const int j = t + b * TPL;
-                const int k = j % Ns;
+                const int k = int(dns.mod(uint32_t(j)));
 #pragma unroll
                 for (int r = 1; r < R; ++r) x[r] = c_mul(x[r], tw[toff + k * R + r]);
             }
@@ -69,7 +70,8 @@ __device__ __forceinline__ void mix_pass(V (&v)[E], V* lds, const V* __restrict_
 #pragma unroll
                 for (int b = 0; b < NB; ++b) {
                     const int j = t + b * TPL;
-                    const int d0 = (j / Ns) * (Ns * R) + j % Ns;
+                    const int jq = int(dns.div(uint32_t(j)));
+                    const int d0 = jq * (Ns * R) + (j - jq * Ns);
 #pragma unroll
                     for (int q = 0; q < R; ++q) lds[mix_slot<STRIDED>(line, d0 + q * Ns, mp.N, mp.T)] = v[b + q * NB];
                 }
@@ -276,6 +278,7 @@ bool mixed_plan(const AxisPass& ps, int N, bool dbl, MixedPlan* mp, int force_T)
     int ns = 1, off = 0;
     for (int i = 0; i < q.npass; ++i) {
         q.ns[i] = ns;
+        q.dns[i].set(uint32_t(ns));
         q.toff[i] = i > 0 ? off : 0;
         if (i > 0) off += ns * q.radix[i];
         ns *= q.radix[i];
