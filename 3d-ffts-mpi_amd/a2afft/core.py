@@ -28,7 +28,7 @@ def _ptr(t):
 
 
 def dims_create(nnodes):
-    """MPI_Dims_create(nnodes, 3) (a2adistribution.cpp:100)."""
+    """MPI_Dims_create(nnodes, 3) (a2adistribution.cpp:59)."""
     d = (ctypes.c_int * 3)()
     L.check(L.lib().a2afft_dims_create(nnodes, d), "dims_create")
     return tuple(d)
@@ -45,7 +45,7 @@ def set_sync(on):
 
 
 class Distribution:
-    """a2aDistribution over ``world_size`` virtual ranks (a2adistribution.cpp:85-154)."""
+    """a2aDistribution over ``world_size`` virtual ranks (a2adistribution.cpp:44-113)."""
 
     def __init__(self, Ng, world_size, dims=None, blockSize=64, dtype="c128"):
         self._h = ctypes.c_void_p()
@@ -69,14 +69,14 @@ class Distribution:
         return self._h
 
     def coords(self, rank):
-        """rank2coords + get_local_grid_start (a2adistribution.cpp:55-83)."""
+        """rank2coords + get_local_grid_start (a2adistribution.cpp:14-42)."""
         c = (ctypes.c_int * 3)()
         s = (ctypes.c_int * 3)()
         L.check(L.lib().a2afft_dist_rank_coords(self._h, rank, c, s), "rank_coords")
         return tuple(c), tuple(s)
 
     def write_dims(self, path):
-        """The saveDims header (a2adistribution.cpp:142-152)."""
+        """The saveDims header (a2adistribution.cpp:101-111)."""
         L.check(L.lib().a2afft_dist_write_dims(self._h, str(path).encode()), "write_dims")
 
     def empty(self, device="cuda"):
@@ -113,7 +113,7 @@ class Distribution:
         return self._stage("a2afft_return_y_pencils", d_Buff1, d_Buff2)
 
     def finalize(self):
-        """a2aDistribution::finalize (a2adistribution.cpp:258-265)."""
+        """a2aDistribution::finalize (a2adistribution.cpp:217-223)."""
         if self._h:
             L.lib().a2afft_dist_destroy(self._h)
             self._h = ctypes.c_void_p()
@@ -197,7 +197,7 @@ class _Plan:
 
 
 class Dfft(_Plan):
-    """a2adfft (a2adfft.cpp:36-188) with a precision and an execution strategy."""
+    """a2adfft (a2adfft.cpp:7-158) with a precision and an execution strategy."""
 
     def __init__(self, dist, precision=None, exec="fused"):
         if precision is None:
@@ -214,7 +214,7 @@ class Dfft(_Plan):
         self.dist._check_buf(b)
 
     def make_plans(self, d_scratch=None):
-        """a2adfft::make_plans (a2adfft.cpp:47-53)."""
+        """a2adfft::make_plans (a2adfft.cpp:18-24)."""
         if d_scratch is not None:
             self.dist._check_buf(d_scratch)
         self._scratch = d_scratch

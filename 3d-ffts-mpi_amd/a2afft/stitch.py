@@ -1,9 +1,9 @@
 """Reader and checker for the reference-format outputs of the driver.
 
 Reads the `dims` header and the `proc<r>` dumps (the formats of
-src/a2afft/distribution/a2adistribution.cpp:142-152 and
+src/a2afft/distribution/a2adistribution.cpp:101-111 and
 src/a2afft/tests/test.cpp:69-125), reassembles the global grid of each step
-from the rank blocks (rank2coords, a2adistribution.cpp:55-61) and applies the
+from the rank blocks (rank2coords, a2adistribution.cpp:14-20) and applies the
 reference's pass/fail check, np.fft.fftn(input) ~ forward and input ~ inverse
 with numpy's default allclose tolerances (src/a2afft/tests/stitch.py:89),
 computed here in complex128 and reported with rel-L2 as well.

@@ -48,7 +48,7 @@ public:
     a2afft_dist handle = nullptr;
     void* stream = nullptr;   // hipStream_t; null = the null stream
 
-    // a2aDistribution(MPI_Comm, Ng, blockSize) (a2adistribution.cpp:85-154)
+    // a2aDistribution(MPI_Comm, Ng, blockSize) (a2adistribution.cpp:44-113)
     // with the communicator replaced by `world_size` virtual ranks; `dims`
     // null applies the MPI_Dims_create rule.
     a2aDistribution(int input_world_size, int input_Ng, int input_blockSize, const int* input_dims = nullptr) {
@@ -99,7 +99,7 @@ public:
         a2afft_cpp::check(a2afft_return_y_pencils(handle, d_Buff1[0], d_Buff2[0], stream), "returnYPencils");
     }
 
-    // a2aDistribution::finalize (a2adistribution.cpp:258-265)
+    // a2aDistribution::finalize (a2adistribution.cpp:217-223)
     void finalize() {
         if (handle) a2afft_dist_destroy(handle);
         handle = nullptr;
@@ -117,7 +117,7 @@ public:
     int plans_made = 0;
     a2afft_plan plan = nullptr;
 
-    // a2adfft(a2aDistribution&) (a2adfft.cpp:36-45); the execution strategy
+    // a2adfft(a2aDistribution&) (a2adfft.cpp:7-16); the execution strategy
     // defaults to the fused three-pass transform.
     explicit a2adfft(a2aDistribution& dist, a2afft_exec exec = A2AFFT_EXEC_FUSED)
         : Ng(dist.Ng), nlocal(dist.nlocal), world_size(dist.world_size), blockSize(dist.blockSize),
@@ -133,14 +133,14 @@ public:
         if (plan) a2afft_plan_destroy(plan);
     }
 
-    // a2adfft::make_plans (a2adfft.cpp:47-53)
+    // a2adfft::make_plans (a2adfft.cpp:18-24)
     void make_plans(fftPrecision** input_d_scratch) {
         d_scratch = input_d_scratch;
         a2afft_cpp::check(a2afft_make_plans(plan, input_d_scratch ? input_d_scratch[0] : nullptr), "make_plans");
         plans_made = 1;
     }
 
-    // a2adfft::forward / inverse (a2adfft.cpp:55-188)
+    // a2adfft::forward / inverse (a2adfft.cpp:26-158)
     void forward(fftPrecision** d_data) {
         a2afft_cpp::check(a2afft_forward(plan, d_data[0], distribution.stream), "forward");
     }

@@ -52,7 +52,7 @@ struct FastDiv {
 // axis, s_rank the element distance between neighbouring ranks along it and
 // s_elem the element stride inside a block.  The global element (x, y, z) of
 // a [P][lx][ly][lz] buffer sits at off_x(x) + off_y(y) + off_z(z)
-// (rank2coords / block origin, a2adistribution.cpp:55-83).
+// (rank2coords / block origin, a2adistribution.cpp:14-42).
 struct AxisMap {
     uint32_t l;
     uint32_t n;        // global extent along the axis

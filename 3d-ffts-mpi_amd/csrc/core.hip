@@ -126,12 +126,12 @@ extern "C" int a2afft_dist_create(a2afft_dist* out, int Ng, int world_size, cons
     } else {
         a2afft_dims_create(world_size, d);
     }
-    // a2adistribution.cpp:102-104
+    // a2adistribution.cpp:61-63
     for (int a = 0; a < 3; ++a)
         if (Ng % d[a]) return set_error(A2AFFT_E_DIVISIBILITY, "Ng %% dims[%d] != 0", a);
     int l[3] = {Ng / d[0], Ng / d[1], Ng / d[2]};
     const int64_t nlocal = int64_t(l[0]) * l[1] * l[2];
-    // a2adistribution.cpp:115-121
+    // a2adistribution.cpp:74-80
     for (int a = 0; a < 3; ++a)
         if (world_size % d[a]) return set_error(A2AFFT_E_RANKS, "world_size %% dims[%d] != 0", a);
     if ((int64_t(l[0]) * l[1]) % world_size) return set_error(A2AFFT_E_CHUNK, "(lx*ly) %% world_size != 0");
@@ -167,7 +167,7 @@ extern "C" int a2afft_dist_get_info(a2afft_dist d, a2afft_dist_info* info) {
     return A2AFFT_OK;
 }
 
-// rank2coords (a2adistribution.cpp:55-61), get_local_grid_start (:77-83)
+// rank2coords (a2adistribution.cpp:14-20), get_local_grid_start (:36-42)
 extern "C" int a2afft_dist_rank_coords(a2afft_dist d, int rank, int coords[3], int start[3]) {
     if (!d || rank < 0 || rank >= d->P) return set_error(A2AFFT_E_INVALID_ARG, "rank_coords: bad rank");
     int c[3];
@@ -181,7 +181,7 @@ extern "C" int a2afft_dist_rank_coords(a2afft_dist d, int rank, int coords[3], i
     return A2AFFT_OK;
 }
 
-// a2adistribution.cpp:142-152
+// a2adistribution.cpp:101-111
 extern "C" int a2afft_dist_write_dims(a2afft_dist d, const char* path) {
     if (!d || !path) return set_error(A2AFFT_E_INVALID_ARG, "write_dims: null");
     if (d->nlocal > 0x7fffffff) return set_error(A2AFFT_E_SIZE, "write_dims: nlocal does not fit int32");
@@ -199,7 +199,7 @@ extern "C" int a2afft_dist_destroy(a2afft_dist d) {
     return A2AFFT_OK;
 }
 
-// ---- stages (a2adistribution.cpp:156-256) ---------------------------------
+// ---- stages (a2adistribution.cpp:115-215) ---------------------------------
 
 enum { Z_A2P = 0, Z_P2A = 1, X_A2P = 2, X_P2A = 3, Y_A2P = 4, Y_P2A = 5 };
 enum { T_ZX = 0, T_XY = 1, T_YZ = 2 };
@@ -217,24 +217,24 @@ static int all2all(const a2afft_dist_s* d, const void* s, void* t, hipStream_t s
 static int stage(const a2afft_dist_s* d, int which, void* b1, void* b2, hipStream_t s) {
     if (!d || !b1 || !b2 || b1 == b2) return set_error(A2AFFT_E_INVALID_ARG, "stage: bad buffers");
     switch (which) {
-    case 0:  // getZPencils :156-170
+    case 0:  // getZPencils :115-129
         A2A_TRY(all2all(d, b1, b2, s));
         return remap(d, Z_A2P, b2, b1, s);
-    case 1:  // returnZPencils :172-186
+    case 1:  // returnZPencils :131-145
         A2A_TRY(remap(d, Z_P2A, b1, b2, s));
         return all2all(d, b2, b1, s);
-    case 2:  // getXPencils :188-204
+    case 2:  // getXPencils :147-163
         A2A_TRY(transpose(d, T_ZX, b1, b2, s));
         A2A_TRY(all2all(d, b2, b1, s));
         return remap(d, X_A2P, b1, b2, s);
-    case 3:  // returnXPencils :206-220
+    case 3:  // returnXPencils :165-179
         A2A_TRY(remap(d, X_P2A, b2, b1, s));
         return all2all(d, b1, b2, s);
-    case 4:  // getYPencils :222-238
+    case 4:  // getYPencils :181-197
         A2A_TRY(transpose(d, T_XY, b2, b1, s));
         A2A_TRY(all2all(d, b1, b2, s));
         return remap(d, Y_A2P, b2, b1, s);
-    default: {  // returnYPencils :240-256
+    default: {  // returnYPencils :199-215
         int w = remap(d, Y_P2A, b2, b1, s);
         if (w < 0) return w;
         A2A_TRY(all2all(d, b1, b2, s));
@@ -568,7 +568,7 @@ extern "C" int a2afft_make_plans(a2afft_plan p, void* scratch) {
     return A2AFFT_OK;
 }
 
-// a2adfft.cpp:55-188 in the reference stage order.
+// a2adfft.cpp:26-158 in the reference stage order.
 static int exec_staged(a2afft_plan_s* p, void* D, int dir, hipStream_t s) {
     if (!p->scratch) return set_error(A2AFFT_E_NO_SCRATCH, "staged execute needs scratch (make_plans)");
     if (p->d.Ng <= 0) return set_error(A2AFFT_E_INVALID_ARG, "staged execute needs a distribution plan");
@@ -590,10 +590,10 @@ static int exec_staged(a2afft_plan_s* p, void* D, int dir, hipStream_t s) {
     A2A_TRY(stage(d, 4, D, S, s));
     A2A_TRY(fft(D));
     if (dir < 0) {
-        A2A_TRY(launch_copy(S, D, total, d->dtype, s));   // a2adfft.cpp:108
+        A2A_TRY(launch_copy(S, D, total, d->dtype, s));   // a2adfft.cpp:79
     } else {
         const double scale = double(int64_t(d->Ng) * d->Ng * d->Ng);
-        A2A_TRY(launch_scale(S, D, total, scale, d->dtype, s));   // a2adfft.cpp:176
+        A2A_TRY(launch_scale(S, D, total, scale, d->dtype, s));   // a2adfft.cpp:147
     }
     int w = stage(d, 5, D, S, s);
     if (w < 0) return w;

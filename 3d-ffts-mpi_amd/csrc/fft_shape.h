@@ -274,7 +274,7 @@ A2A_HD void long_pair(uint32_t b, uint32_t tiles, uint32_t* a, uint32_t* c, uint
 
 // Axis maps of the three fused passes over the virtual-rank block layout:
 // z pass = lines (x, y), contiguous; x pass = tiles (y, z-columns); y pass =
-// tiles (x, z-columns).  The reference's axis order z, x, y (a2adfft.cpp:60-114).
+// tiles (x, z-columns).  The reference's axis order z, x, y (a2adfft.cpp:31-85).
 struct PassMaps {
     AxisMap fft, a1, a2;
     int strided;

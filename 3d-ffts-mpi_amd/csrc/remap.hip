@@ -1,7 +1,7 @@
 // Remap kernels of the a2afft hot path on gfx950: the six pack/unpack index
 // maps (src/a2afft/distribution/kernels/a2apencils.cu), the three fast-axis
 // transposes (a2achangefastaxis.cu), the device-local all-to-all that
-// replaces MPI_Alltoall (a2adistribution.cpp:158-166), and the copy / scale
+// replaces MPI_Alltoall (a2adistribution.cpp:117-125), and the copy / scale
 // kernels of dfft/kernels/a2affts.cu:78-126.
 //
 // All remaps are pure permutations of complex values: the index arithmetic is
@@ -116,7 +116,7 @@ static int check_geometry(int world_size, int64_t nlocal, const int lgs[3], cons
     if (nlocal % world_size) return set_error(A2AFFT_E_CHUNK, "remap: nlocal %% world_size != 0");
     if ((int64_t(lgs[0]) * lgs[1]) % world_size || (int64_t(lgs[0]) * lgs[2]) % world_size ||
         (int64_t(lgs[1]) * lgs[2]) % world_size)
-        return set_error(A2AFFT_E_CHUNK, "remap: (l_a*l_b) %% world_size != 0 (a2adistribution.cpp:119-121)");
+        return set_error(A2AFFT_E_CHUNK, "remap: (l_a*l_b) %% world_size != 0 (a2adistribution.cpp:78-80)");
     for (int a = 0; a < 3; ++a)
         if (world_size % dims[a]) return set_error(A2AFFT_E_RANKS, "remap: world_size %% dims != 0");
     return A2AFFT_OK;

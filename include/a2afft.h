@@ -33,13 +33,13 @@ extern "C" {
  * (a2apencils.cu:197) is a bijection; the true inverse was applied instead. */
 #define A2AFFT_W_DIVERGES_FROM_REFERENCE   1
 #define A2AFFT_E_INVALID_ARG              -1
-#define A2AFFT_E_DIVISIBILITY             -2  /* Ng % dims[i] != 0           (a2adistribution.cpp:102-104) */
-#define A2AFFT_E_RANKS                    -3  /* P % dims[i] != 0 / dims != P (a2adistribution.cpp:115-117) */
-#define A2AFFT_E_CHUNK                    -4  /* (l_a*l_b) % P != 0           (a2adistribution.cpp:119-121) */
+#define A2AFFT_E_DIVISIBILITY             -2  /* Ng % dims[i] != 0           (a2adistribution.cpp:61-63) */
+#define A2AFFT_E_RANKS                    -3  /* P % dims[i] != 0 / dims != P (a2adistribution.cpp:74-76) */
+#define A2AFFT_E_CHUNK                    -4  /* (l_a*l_b) % P != 0           (a2adistribution.cpp:78-80) */
 #define A2AFFT_E_HIP                      -5
 #define A2AFFT_E_ALLOC                    -6
 #define A2AFFT_E_UNSUPPORTED              -7  /* length / precision not built */
-#define A2AFFT_E_NO_SCRATCH               -8  /* execute before make_plans    (a2adfft.cpp:47-53) */
+#define A2AFFT_E_NO_SCRATCH               -8  /* execute before make_plans    (a2adfft.cpp:18-24) */
 #define A2AFFT_E_SIZE                     -9  /* index range exceeded */
 
 typedef enum { A2AFFT_C128 = 0, A2AFFT_C64 = 1 } a2afft_dtype;
@@ -53,7 +53,7 @@ typedef enum {
 
 typedef enum {
     A2AFFT_EXEC_FUSED = 0,  /* 3 in-place axis passes over the block layout */
-    A2AFFT_EXEC_STAGED = 1  /* the reference stage order: 18 launches (a2adfft.cpp:55-188) */
+    A2AFFT_EXEC_STAGED = 1  /* the reference stage order: 18 launches (a2adfft.cpp:26-158) */
 } a2afft_exec;
 
 typedef struct a2afft_dist_s* a2afft_dist;
@@ -81,27 +81,27 @@ int a2afft_version(void);
 
 /* ---- decomposition: a2aDistribution ------------------------------------- */
 /* MPI_Dims_create(nnodes, 3, dims) with all dims free
- * (replaces the call at a2adistribution.cpp:100). */
+ * (replaces the call at a2adistribution.cpp:59). */
 int a2afft_dims_create(int nnodes, int dims[3]);
 
 /* a2aDistribution::a2aDistribution(MPI_Comm, Ng, blockSize)
- * (a2adistribution.cpp:85-154).  The communicator becomes `world_size`
+ * (a2adistribution.cpp:44-113).  The communicator becomes `world_size`
  * virtual ranks; dims == NULL applies the MPI_Dims_create rule, otherwise
  * dims[0]*dims[1]*dims[2] must equal world_size.  Checks the reference's
  * asserted preconditions and returns their codes instead of aborting. */
 int a2afft_dist_create(a2afft_dist* out, int Ng, int world_size, const int* dims,
                        int blockSize, a2afft_dtype dtype);
 int a2afft_dist_get_info(a2afft_dist dist, a2afft_dist_info* info);
-/* rank2coords + get_local_grid_start (a2adistribution.cpp:55-83). */
+/* rank2coords + get_local_grid_start (a2adistribution.cpp:14-42). */
 int a2afft_dist_rank_coords(a2afft_dist dist, int rank, int coords[3],
                             int local_coordinates_start[3]);
-/* The `dims` file written under saveDims (a2adistribution.cpp:142-152):
+/* The `dims` file written under saveDims (a2adistribution.cpp:101-111):
  * int32 world_size, Ng, nlocal, dims[3], local_grid_size[3]. */
 int a2afft_dist_write_dims(a2afft_dist dist, const char* path);
-/* a2aDistribution::finalize (a2adistribution.cpp:258-265). */
+/* a2aDistribution::finalize (a2adistribution.cpp:217-223). */
 int a2afft_dist_destroy(a2afft_dist dist);
 
-/* Stage methods (a2adistribution.cpp:156-256) over all P virtual ranks.
+/* Stage methods (a2adistribution.cpp:115-215) over all P virtual ranks.
  * buff1/buff2 play d_Buff1/d_Buff2; each is [P][nlocal] complex. */
 int a2afft_get_z_pencils(a2afft_dist dist, void* buff1, void* buff2, void* stream);
 int a2afft_return_z_pencils(a2afft_dist dist, void* buff1, void* buff2, void* stream);
@@ -147,7 +147,7 @@ int a2afft_launch_d_fast_y_to_z(const void* source, void* dest, const int local_
 
 /* MPI_Alltoall(send, nlocal/P, TYPE_COMPLEX, recv, ...) over the P virtual
  * ranks of [P][nlocal] buffers: recv[s][q*ns+i] = send[q][s*ns+i]
- * (a2adistribution.cpp:158-166 and the five other call sites). */
+ * (a2adistribution.cpp:117-125 and the five other call sites). */
 int a2afft_alltoall(const void* send, void* recv, int world_size, int64_t nlocal,
                     a2afft_dtype dtype, void* stream);
 
@@ -181,7 +181,7 @@ int a2afft_launch_fast_copy_fft(void* out, const void* data, int64_t n_elems,
                                 int blockSize, a2afft_dtype dtype, void* stream);
 
 /* ---- plan / execute: a2adfft -------------------------------------------- */
-/* a2adfft::a2adfft(dist) (a2adfft.cpp:36-45) plus the precision and
+/* a2adfft::a2adfft(dist) (a2adfft.cpp:7-16) plus the precision and
  * execution strategy.  Builds twiddle tables and kernel choices. */
 int a2afft_plan_create(a2afft_plan* out, a2afft_dist dist, a2afft_precision precision,
                        a2afft_exec exec);
@@ -192,11 +192,11 @@ int a2afft_plan_create(a2afft_plan* out, a2afft_dist dist, a2afft_precision prec
  * needed; forward / inverse / autotune / execute_* apply as for other fused
  * plans, and the inverse scales by 1/(nx*ny*nz). */
 int a2afft_grid_plan_create(a2afft_plan* out, int nx, int ny, int nz, a2afft_precision precision);
-/* a2adfft::make_plans(d_scratch) (a2adfft.cpp:47-53).  scratch is a
+/* a2adfft::make_plans(d_scratch) (a2adfft.cpp:18-24).  scratch is a
  * [P][nlocal] device buffer; the staged strategy needs it, the fused one
  * accepts NULL. */
 int a2afft_make_plans(a2afft_plan plan, void* d_scratch);
-/* a2adfft::forward / inverse (a2adfft.cpp:55-188): in place on d_data in the
+/* a2adfft::forward / inverse (a2adfft.cpp:26-158): in place on d_data in the
  * block layout; inverse applies the 1/Ng^3 of scale_fft. */
 int a2afft_forward(a2afft_plan plan, void* d_data, void* stream);
 int a2afft_inverse(a2afft_plan plan, void* d_data, void* stream);
@@ -266,7 +266,7 @@ int a2afft_error_metrics(const void* y, a2afft_dtype dtype, const void* ref, int
 
 /* One axis pass of a fused plan, exactly as a2afft_forward (direction -1) or
  * a2afft_inverse (+1) runs it: pass 0 = z, 1 = x, 2 = y (the reference's axis
- * order, a2adfft.cpp:60-114); the inverse's 1/Ng^3 rides on pass 2.  For
+ * order, a2adfft.cpp:31-85); the inverse's 1/Ng^3 rides on pass 2.  For
  * per-kernel timing; the three passes in order equal one transform. */
 int a2afft_execute_pass(a2afft_plan plan, void* d_data, int pass, int direction, void* stream);
 

@@ -35,7 +35,7 @@ int orc_get_threads(void) {
 /* Decomposition                                                            */
 /* ------------------------------------------------------------------------ */
 
-/* MPI_Dims_create(P, 3, dims) with all dims free (a2adistribution.cpp:100).
+/* MPI_Dims_create(P, 3, dims) with all dims free (a2adistribution.cpp:59).
  * MPI-standard rule: dims as close to each other as possible, non-increasing.
  * Restated as: prime factors of P taken largest first, each multiplied into
  * the currently smallest dimension, then sorted non-increasing.  Pinned by
@@ -59,15 +59,15 @@ void orc_dims_create(int P, int dims[3]) {
             if (dims[b] > dims[a]) { int t = dims[a]; dims[a] = dims[b]; dims[b] = t; }
 }
 
-/* a2adistribution.cpp:55-61 */
+/* a2adistribution.cpp:14-20 */
 void orc_rank2coords(int rank, const int dims[3], int out[3]) {
     out[0] = rank / (dims[1] * dims[2]);
     out[1] = (rank - out[0] * (dims[1] * dims[2])) / dims[2];
     out[2] = rank - out[0] * (dims[1] * dims[2]) - out[1] * dims[2];
 }
 
-/* a2adistribution.cpp:85-121: dims, local grid (topology2localgrid :66-72),
- * nlocal (:110) and the asserted preconditions (:102-104, :115-121). */
+/* a2adistribution.cpp:44-80: dims, local grid (topology2localgrid :25-31),
+ * nlocal (:69) and the asserted preconditions (:61-63, :74-80). */
 int orc_decompose(int Ng, int P, const int* dims_or_null, orc_decomp* d) {
     if (Ng <= 0 || P <= 0 || !d) return -6;
     d->Ng = Ng;
@@ -202,8 +202,8 @@ void orc_remap_apply(int kind, const orc_decomp* d, const void* src, void* dst, 
     }
 }
 
-/* MPI_Alltoall with nsends = nlocal/P per peer (a2adistribution.cpp:158-166;
- * every stage's nsends reduces to nlocal/P, :158,174,190,208,224,242). */
+/* MPI_Alltoall with nsends = nlocal/P per peer (a2adistribution.cpp:117-125;
+ * every stage's nsends reduces to nlocal/P, :117,133,149,167,183,201). */
 void orc_alltoall(const orc_decomp* d, const void* send, void* recv, int esize) {
     const int64_t P = d->P, nlocal = d->nlocal, ns = nlocal / P;
     const size_t chunk = (size_t)ns * esize;
@@ -214,32 +214,32 @@ void orc_alltoall(const orc_decomp* d, const void* send, void* recv, int esize) 
                    (const char*)send + ((size_t)q * nlocal + s * ns) * esize, chunk);
 }
 
-/* a2adistribution.cpp:156-256.  b1 = d_Buff1, b2 = d_Buff2. */
+/* a2adistribution.cpp:115-215.  b1 = d_Buff1, b2 = d_Buff2. */
 void orc_stage(int stage, const orc_decomp* d, void* b1, void* b2, int esize) {
     switch (stage) {
-    case ORC_GET_Z:        /* :156-170 */
+    case ORC_GET_Z:        /* :115-129 */
         orc_alltoall(d, b1, b2, esize);
         orc_remap_apply(ORC_Z_A2A_TO_Z_PENCILS, d, b2, b1, esize);
         break;
-    case ORC_RETURN_Z:     /* :172-186 */
+    case ORC_RETURN_Z:     /* :131-145 */
         orc_remap_apply(ORC_Z_PENCILS_TO_Z_A2A, d, b1, b2, esize);
         orc_alltoall(d, b2, b1, esize);
         break;
-    case ORC_GET_X:        /* :188-204 */
+    case ORC_GET_X:        /* :147-163 */
         orc_remap_apply(ORC_FAST_Z_TO_X, d, b1, b2, esize);
         orc_alltoall(d, b2, b1, esize);
         orc_remap_apply(ORC_X_A2A_TO_X_PENCILS, d, b1, b2, esize);
         break;
-    case ORC_RETURN_X:     /* :206-220 */
+    case ORC_RETURN_X:     /* :165-179 */
         orc_remap_apply(ORC_X_PENCILS_TO_X_A2A, d, b2, b1, esize);
         orc_alltoall(d, b1, b2, esize);
         break;
-    case ORC_GET_Y:        /* :222-238 */
+    case ORC_GET_Y:        /* :181-197 */
         orc_remap_apply(ORC_FAST_X_TO_Y, d, b2, b1, esize);
         orc_alltoall(d, b1, b2, esize);
         orc_remap_apply(ORC_Y_A2A_TO_Y_PENCILS, d, b2, b1, esize);
         break;
-    case ORC_RETURN_Y:     /* :240-256 */
+    case ORC_RETURN_Y:     /* :199-215 */
         orc_remap_apply(ORC_Y_PENCILS_TO_Y_A2A, d, b2, b1, esize);
         orc_alltoall(d, b1, b2, esize);
         orc_remap_apply(ORC_FAST_Y_TO_Z, d, b2, b1, esize);
@@ -393,7 +393,7 @@ int orc_fft_batched(double* data, int N, int64_t nlines, int sign) {
 /* 3D transform, reference stage order                                      */
 /* ------------------------------------------------------------------------ */
 
-/* a2adfft.cpp:55-120 with forward_1d_fft (a2affts.cu:7-40: nlocal/Ng lines
+/* a2adfft.cpp:26-91 with forward_1d_fft (a2affts.cu:7-40: nlocal/Ng lines
  * per rank, in place) and fast_copy_fft (a2affts.cu:94-106). */
 int orc_forward(const orc_decomp* d, double* D, double* S) {
     const int64_t total = (int64_t)d->P * d->nlocal, lines = total / d->Ng;
@@ -412,7 +412,7 @@ int orc_forward(const orc_decomp* d, double* D, double* S) {
     return e;
 }
 
-/* a2adfft.cpp:122-188; scale_fft (a2affts.cu:78-92) divides by Ng^3,
+/* a2adfft.cpp:93-158; scale_fft (a2affts.cu:78-92) divides by Ng^3,
  * evaluated here in 64-bit so it does not overflow for Ng >= 1291. */
 int orc_inverse(const orc_decomp* d, double* D, double* S) {
     const int64_t total = (int64_t)d->P * d->nlocal, lines = total / d->Ng;
@@ -438,7 +438,7 @@ int orc_inverse(const orc_decomp* d, double* D, double* S) {
 /* ------------------------------------------------------------------------ */
 
 /* Block r holds [lx][ly][lz] at global origin coords(r)*lgs
- * (a2adistribution.cpp:77-83, tests/stitch.py:28-47). */
+ * (a2adistribution.cpp:36-42, tests/stitch.py:28-47). */
 static void blocks_global(const orc_decomp* d, void* blocks, void* global, int esize, int to_global) {
     const int64_t Ng = d->Ng, lx = d->lgs[0], ly = d->lgs[1], lz = d->lgs[2];
     for (int r = 0; r < d->P; ++r) {

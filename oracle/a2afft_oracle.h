@@ -47,13 +47,13 @@ enum {
     ORC_FAST_Y_TO_Z = 8           /* a2achangefastaxis.cu:65-84 */
 };
 
-/* Stage kinds: a2adistribution.cpp:156-256 */
+/* Stage kinds: a2adistribution.cpp:115-215 */
 enum {
     ORC_GET_Z = 0, ORC_RETURN_Z = 1, ORC_GET_X = 2,
     ORC_RETURN_X = 3, ORC_GET_Y = 4, ORC_RETURN_Y = 5
 };
 
-/* --- decomposition (a2adistribution.cpp:55-121) --------------------------- */
+/* --- decomposition (a2adistribution.cpp:14-80) --------------------------- */
 void orc_dims_create(int P, int dims[3]);
 void orc_rank2coords(int rank, const int dims[3], int out[3]);
 /* Returns 0 on success, or a negative code naming the failed precondition
@@ -76,10 +76,10 @@ int orc_remap_is_gather(int kind);
 /* Applies a remap to every virtual rank: src,dst are [P][nlocal] elements of
  * esize bytes.  Uses the true inverse for y-return. */
 void orc_remap_apply(int kind, const orc_decomp* d, const void* src, void* dst, int esize);
-/* MPI_Alltoall over P virtual ranks with equal chunks (a2adistribution.cpp:158-166):
+/* MPI_Alltoall over P virtual ranks with equal chunks (a2adistribution.cpp:117-125):
  * recv[s][q*ns+i] = send[q][s*ns+i], ns = nlocal/P. */
 void orc_alltoall(const orc_decomp* d, const void* send, void* recv, int esize);
-/* Stage compositions (a2adistribution.cpp:156-256) on complex buffers of esize bytes. */
+/* Stage compositions (a2adistribution.cpp:115-215) on complex buffers of esize bytes. */
 void orc_stage(int stage, const orc_decomp* d, void* b1, void* b2, int esize);
 
 /* --- 1D FFT (the published algorithm behind dfft/kernels/a2affts.cu:7-76) -- */
@@ -93,7 +93,7 @@ void orc_dft_naive(const double* in, double* out, int N, int sign);
 /* Twiddle exp(sign*2*pi*i*m/M) in long double, rounded once to double. */
 void orc_twiddle(int64_t m, int64_t M, int sign, double* re, double* im);
 
-/* --- 3D transform in the reference stage order (a2adfft.cpp:55-188) ------- */
+/* --- 3D transform in the reference stage order (a2adfft.cpp:26-158) ------- */
 /* D, S: [P][nlocal] complex128. Result in D (block layout), S clobbered. */
 int orc_forward(const orc_decomp* d, double* D, double* S);
 /* Inverse with the 1/Ng^3 scaling of a2affts.cu:78-92 (64-bit Ng^3). */

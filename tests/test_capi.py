@@ -63,10 +63,10 @@ def test_dist_fields_and_preconditions(a2afft_mod):
     assert not A.Distribution(32, 64, dims=(4, 4, 4)).reference_domain
     with pytest.raises(A.A2AFFTError) as e:
         A.Distribution(9, 4)
-    assert e.value.status == -2                        # Ng % dims (a2adistribution.cpp:102-104)
+    assert e.value.status == -2                        # Ng % dims (a2adistribution.cpp:61-63)
     with pytest.raises(A.A2AFFTError) as e:
         A.Distribution(10, 4)
-    assert e.value.status == -4                        # (lx*ly) % P (a2adistribution.cpp:119)
+    assert e.value.status == -4                        # (lx*ly) % P (a2adistribution.cpp:78)
     with pytest.raises(A.A2AFFTError) as e:
         A.Distribution(8, 8, dims=(2, 2, 4))
     assert e.value.status == -3
@@ -75,7 +75,7 @@ def test_dist_fields_and_preconditions(a2afft_mod):
 
 
 def test_write_dims_matches_reference_layout(a2afft_mod, tmp_path):
-    """int32 world_size, Ng, nlocal, dims[3], local_grid_size[3] (a2adistribution.cpp:142-152),
+    """int32 world_size, Ng, nlocal, dims[3], local_grid_size[3] (a2adistribution.cpp:101-111),
     the header tests/stitch.py:59-66 reads."""
     d = a2afft_mod.Distribution(32, 8, dims=(1, 2, 4))
     path = tmp_path / "dims"

@@ -75,7 +75,7 @@ def test_alltoall_bit_exact(a2afft_mod, cuda, cfg, dtype):
 @pytest.mark.parametrize("cfg", CONFIGS, ids=lambda c: f"Ng{c[0]}_P{c[1]}_{c[2]}")
 def test_stages_bit_exact(a2afft_mod, cuda, cfg, dtype):
     """The six get/return stages in the order a2adfft::forward uses them, with
-    the two buffers compared after every stage (a2adistribution.cpp:156-256)."""
+    the two buffers compared after every stage (a2adistribution.cpp:115-215)."""
     import torch
     Ng, P, dims = cfg
     rc, d = O.decompose(Ng, P, dims)

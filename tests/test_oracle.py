@@ -30,7 +30,7 @@ def _golden():
 
 
 def test_dims_create_table():
-    # MPI standard: balanced, non-increasing (a2adistribution.cpp:100)
+    # MPI standard: balanced, non-increasing (a2adistribution.cpp:59)
     table = {1: (1, 1, 1), 2: (2, 1, 1), 4: (2, 2, 1), 6: (3, 2, 1), 8: (2, 2, 2), 12: (3, 2, 2),
              16: (4, 2, 2), 24: (4, 3, 2), 27: (3, 3, 3), 32: (4, 4, 2), 64: (4, 4, 4), 128: (8, 4, 4)}
     for P, dims in table.items():
@@ -39,9 +39,9 @@ def test_dims_create_table():
 
 def test_decompose_preconditions():
     assert O.decompose(64, 4, None)[0] == 0
-    assert O.decompose(9, 4, None)[0] == -1           # Ng % dims (a2adistribution.cpp:102-104)
-    assert O.decompose(10, 4, None)[0] == -3          # (lx*ly) % P (:119)
-    assert O.decompose(32, 64, (1, 8, 8))[0] == -5    # (ly*lz) % P (:119-121)
+    assert O.decompose(9, 4, None)[0] == -1           # Ng % dims (a2adistribution.cpp:61-63)
+    assert O.decompose(10, 4, None)[0] == -3          # (lx*ly) % P (:78)
+    assert O.decompose(32, 64, (1, 8, 8))[0] == -5    # (ly*lz) % P (:78-80)
     assert O.decompose(8, 8, (2, 2, 3))[0] == -6      # dims product != P
     rc, d = O.decompose(256, 8, (1, 2, 4))
     assert rc == 0 and d.lgs_t == (256, 128, 64) and d.nlocal == 256 * 128 * 64
@@ -158,7 +158,7 @@ def test_global_id_trace(cfg):
     O.stage(O.RETURN_X, d, b1, b2)
     O.stage(O.GET_Y, d, b1, b2)
     check_pencils(b1, 1)
-    b2[:] = b1                                       # launch_fast_copy_fft (a2adfft.cpp:108)
+    b2[:] = b1                                       # launch_fast_copy_fft (a2adfft.cpp:79)
     O.stage(O.RETURN_Y, d, b1, b2)
     assert np.array_equal(b1, start)
 
