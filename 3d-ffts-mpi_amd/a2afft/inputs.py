@@ -100,10 +100,42 @@ def gaussian_random_field(shape, seed, slope=-3.0, noise=1e-3, workers=None):
     return f
 
 
+def gaussian_random_field_device(shape, seed, slope=-3.0, noise=1e-3, device="cuda"):
+    """gaussian_random_field computed on the GPU: the same recipe and the same
+    Philox draws (the library's fill_random), the inverse FFT by the library's
+    FP64 grid plan.  Returns a complex128 device tensor of `shape` (global C
+    order).  It agrees with the host generator to FP64 rounding (the two
+    inverse FFTs round differently), which the tests check at 32^3 and 64^3."""
+    import torch
+    from . import core
+    if isinstance(shape, int):
+        shape = (shape,) * 3
+    shape = tuple(int(s) for s in shape)
+    n = shape[0] * shape[1] * shape[2]
+    w = torch.empty(n, dtype=torch.complex128, device=device)
+    core.fill_random(w, seed, "normal")
+    # integer wavenumbers k = N * fftfreq(N) of each axis
+    ks = [torch.arange(m, dtype=torch.float64, device=device) for m in shape]
+    ks = [torch.where(k < (m + 1) // 2, k, k - m) for k, m in zip(ks, shape)]
+    k2 = (ks[0][:, None, None] ** 2 + ks[1][None, :, None] ** 2 + ks[2][None, None, :] ** 2).reshape(-1)
+    amp = torch.where(k2 >= 1.0, k2.clamp(min=1.0) ** (slope / 4.0), torch.zeros_like(k2))
+    w *= amp
+    del amp, k2
+    w = w.reshape(shape)
+    core.GridDfft(shape, "fp64").inverse(w)
+    rms = torch.sqrt(torch.mean(w.real ** 2 + w.imag ** 2))
+    w /= rms
+    u = torch.empty_like(w)
+    core.fill_random(u, seed, "uniform")
+    w += noise * u
+    del u
+    return w
+
+
 # ---- block layout ----------------------------------------------------------
 
 def global_to_blocks(g, dims):
-    """(Nx,Ny,Nz) global grid -> (P, nlocal) rank blocks, rank-major (a2adistribution.cpp:55-83)."""
+    """(Nx,Ny,Nz) global grid -> (P, nlocal) rank blocks, rank-major (a2adistribution.cpp:14-42)."""
     d0, d1, d2 = dims
     nx, ny, nz = g.shape
     lx, ly, lz = nx // d0, ny // d1, nz // d2

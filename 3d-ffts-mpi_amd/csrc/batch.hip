@@ -329,17 +329,22 @@ struct a2afft_batch_s {
     int grid = 0;                     // persistent workgroups
     int max_grid = 0;
     hipStream_t cap_stream = nullptr;
-    hipGraphExec_t graph[2] = {nullptr, nullptr};
-    void* graph_data[2] = {nullptr, nullptr};
+    struct Graph {
+        void* data;
+        int dir;
+        hipGraphExec_t exec;
+    };
+    std::vector<Graph> graphs;        // captured transforms, most recent last
     std::string report;               // the last autotune's timings
 };
 
+// Captured graphs kept per batch: one per (buffer, direction), the least
+// recently used dropped beyond this count.
+constexpr size_t kMaxBatchGraphs = 32;
+
 static void drop_graphs(a2afft_batch_s* b) {
-    for (int d = 0; d < 2; ++d) {
-        if (b->graph[d]) (void)hipGraphExecDestroy(b->graph[d]);
-        b->graph[d] = nullptr;
-        b->graph_data[d] = nullptr;
-    }
+    for (auto& g : b->graphs) (void)hipGraphExecDestroy(g.exec);
+    b->graphs.clear();
 }
 
 static void batch_free(a2afft_batch_s* b) {
@@ -583,15 +588,16 @@ extern "C" int a2afft_batch_execute(a2afft_batch b, void* data, int direction, v
     return finish_call(stream, batch_launch(b, data, direction, (hipStream_t)stream));
 }
 
-// Captures reset + persistent launch for `data` into a hipGraph (one per
-// direction); a later capture for another buffer replaces it.
+// Captures reset + persistent launch for `data` into a hipGraph on first use
+// for that buffer and direction, and replays it; up to kMaxBatchGraphs
+// captures are kept, so callers can rotate over several buffers.
 extern "C" int a2afft_batch_graph_launch(a2afft_batch b, void* data, int direction, void* stream) {
     if (!b || !data || (direction != -1 && direction != 1))
         return set_error(A2AFFT_E_INVALID_ARG, "batch_graph_launch: bad arguments");
     const int d = direction < 0 ? 0 : 1;
-    if (!b->graph[d] || b->graph_data[d] != data) {
-        if (b->graph[d]) (void)hipGraphExecDestroy(b->graph[d]);
-        b->graph[d] = nullptr;
+    size_t k = 0;
+    while (k < b->graphs.size() && !(b->graphs[k].data == data && b->graphs[k].dir == d)) ++k;
+    if (k == b->graphs.size()) {
         hipGraph_t g = nullptr;
         A2A_TRY(hip_check(int(hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal)), "capture"));
         int st = batch_launch(b, data, direction, b->cap_stream);
@@ -600,12 +606,23 @@ extern "C" int a2afft_batch_graph_launch(a2afft_batch b, void* data, int directi
             if (g) (void)hipGraphDestroy(g);
             return st < 0 ? st : st2;
         }
-        st = hip_check(int(hipGraphInstantiate(&b->graph[d], g, nullptr, nullptr, 0)), "graph instantiate");
+        hipGraphExec_t ex = nullptr;
+        st = hip_check(int(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0)), "graph instantiate");
         (void)hipGraphDestroy(g);
         if (st < 0) return st;
-        b->graph_data[d] = data;
+        if (b->graphs.size() >= kMaxBatchGraphs) {
+            (void)hipGraphExecDestroy(b->graphs.front().exec);
+            b->graphs.erase(b->graphs.begin());
+        }
+        b->graphs.push_back(a2afft_batch_s::Graph{data, d, ex});
+        k = b->graphs.size() - 1;
+    } else if (k + 1 != b->graphs.size()) {   // most recently used last
+        const a2afft_batch_s::Graph g = b->graphs[k];
+        b->graphs.erase(b->graphs.begin() + long(k));
+        b->graphs.push_back(g);
+        k = b->graphs.size() - 1;
     }
-    return finish_call(stream, hip_check(int(hipGraphLaunch(b->graph[d], (hipStream_t)stream)), "graph launch"));
+    return finish_call(stream, hip_check(int(hipGraphLaunch(b->graphs[k].exec, (hipStream_t)stream)), "graph launch"));
 }
 
 // 0 when the last execution finished normally, 1 when a dependency wait hit

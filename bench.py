@@ -28,50 +28,59 @@ def log(msg):
 
 
 def make_grf(a2afft, torch, shape, seed, slope=-3.0, noise=1e-3):
-    """The GRF input on the device (complex128 global C-order grid): white
-    noise times |k|^(slope/2) on integer wavenumbers, inverse FFT in FP64 with
-    the library's own FP64 path, unit RMS, plus noise*(U + iU)."""
-    dev = "cuda"
-    if isinstance(shape, int):
-        shape = (shape,) * 3
-    n = shape[0] * shape[1] * shape[2]
-    w = torch.empty(n, dtype=torch.complex128, device=dev)
-    a2afft.fill_random(w, seed, "normal")
-    ks = [torch.arange(m, dtype=torch.float64, device=dev) for m in shape]
-    ks = [torch.where(k < (m + 1) // 2, k, k - m) for k, m in zip(ks, shape)]   # integer wavenumbers
-    k2 = (ks[0][:, None, None] ** 2 + ks[1][None, :, None] ** 2 + ks[2][None, None, :] ** 2).reshape(-1)
-    amp = torch.where(k2 >= 1.0, k2.clamp(min=1.0) ** (slope / 4.0), torch.zeros_like(k2))
-    w *= amp
-    del amp, k2
-    plan = a2afft.GridDfft(shape, "fp64")
-    w = w.reshape(shape)
-    plan.inverse(w)
-    rms = torch.sqrt(torch.mean(w.real ** 2 + w.imag ** 2))
-    w /= rms
-    u = torch.empty_like(w)
-    a2afft.fill_random(u, seed, "uniform")
-    w += noise * u
-    del u
-    return w.reshape(shape)
+    """The GRF input on the device (complex128 global C-order grid): the
+    package's generator, tested against the host recipe at 32^3 and 64^3."""
+    return a2afft.inputs.gaussian_random_field_device(shape, seed, slope, noise)
 
 
-def cpu_baseline(Ng, P, dims, threads, seed):
-    """The oracle (C restatement of the reference stage order, FP64) on one
-    full transform of the same grid shape; OpenMP over `threads` cores."""
+def host_cores():
+    """(cores the process may run on, CPU model): the affinity mask, capped by
+    the cgroup CPU quota when one is set (cpu.max), so every thread counted
+    gets a core of its own."""
+    aff = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, period = f.read().split()[:2]
+        if q != "max":
+            quota = max(1, int(int(q) // int(period)))
+    except Exception:
+        pass
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except Exception:
+        pass
+    return (min(aff, quota) if quota else aff), aff, quota, model
+
+
+def cpu_baseline(Ng, P, dims, threads, seed, samples=3):
+    """The oracle (C restatement of the reference stage order, FP64) on full
+    transforms of the same grid shape, OpenMP over `threads` cores: the median
+    of `samples` timed forwards (ms), every sample's time, and the threads used."""
+    import ctypes
     import numpy as np
     sys.path.insert(0, ROOT)
     from oracle import oracle as O
     O.set_threads(threads)
     rc, d = O.decompose(Ng, P, dims)
     assert rc == 0
-    x = O.fill_uniform(seed, Ng ** 3).reshape(P, -1)
-    S = np.zeros_like(x)
-    import ctypes
-    t0 = time.perf_counter()
-    st = O.lib().orc_forward(ctypes.byref(d), x.ctypes.data_as(ctypes.c_void_p), S.ctypes.data_as(ctypes.c_void_p))
-    t = time.perf_counter() - t0
-    assert st == 0
-    return t * 1e3, O.get_threads()
+    x0 = O.fill_uniform(seed, Ng ** 3).reshape(P, -1)
+    x = np.empty_like(x0)
+    S = np.zeros_like(x0)
+    times = []
+    for _ in range(samples):
+        x[...] = x0
+        t0 = time.perf_counter()
+        st = O.lib().orc_forward(ctypes.byref(d), x.ctypes.data_as(ctypes.c_void_p), S.ctypes.data_as(ctypes.c_void_p))
+        times.append((time.perf_counter() - t0) * 1e3)
+        assert st == 0
+    assert np.isfinite(x).all()
+    return float(np.median(times)), times, O.get_threads()
 
 
 def _traffic(key):
@@ -83,16 +92,47 @@ def _traffic(key):
         return None
 
 
-def _events_ms(torch, fn, steps, warmup=1):
-    """Average ms of fn() over `steps` calls after `warmup` untimed ones, HIP
-    events on the launch stream."""
-    for _ in range(max(1, warmup)):
-        fn()
+class Rotation:
+    """`count` device copies of `src`, refilled outside every timed region, so
+    each timed transform runs on the input itself: a forward multiplies values
+    by up to N, and repeated in-place forwards of one buffer overflow float32
+    within a few calls."""
+
+    def __init__(self, torch, src, count):
+        self.torch, self.src = torch, src
+        self.bufs = [torch.empty_like(src) for _ in range(max(1, count))]
+        self.refill()
+
+    def refill(self):
+        for b in self.bufs:
+            b.copy_(self.src)
+        self.torch.cuda.synchronize()
+
+    def __getitem__(self, i):
+        return self.bufs[i % len(self.bufs)]
+
+    def finite(self):
+        t = self.torch
+        return all(bool(t.isfinite(t.view_as_real(b)).all()) for b in self.bufs)
+
+
+def rotation_count(src, steps, budget_bytes=24 << 30):
+    """Buffers for `steps` timed calls within a memory budget (at least 1)."""
+    per = src.numel() * src.element_size()
+    return max(1, min(steps, budget_bytes // max(1, per)))
+
+
+def _events_ms(torch, fn, rot, steps, warmup=1):
+    """Average ms of fn(buf) over `steps` calls, HIP events on the launch
+    stream; warm-up calls first, then every buffer of `rot` refilled from the
+    input (untimed), then the timed calls on buffers i = 0, 1, ... in turn."""
+    for i in range(max(1, warmup)):
+        fn(rot[i])
+    rot.refill()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
     e0.record()
-    for _ in range(steps):
-        fn()
+    for i in range(steps):
+        fn(rot[i])
     e1.record()
     torch.cuda.synchronize()
     return e0.elapsed_time(e1) / steps
@@ -137,14 +177,21 @@ def run_c3b(args, torch, a2afft):
     log(plan.describe())
     tmp = torch.empty_like(pencils)
 
-    def staged():
-        a2afft.launch_remap(2, pencils, tmp, P, nloc, lgs, dims, nbuffers=1)
+    def staged(buf):
+        a2afft.launch_remap(2, buf, tmp, P, nloc, lgs, dims, nbuffers=1)
         a2afft.forward_1d_fft(tmp, 2048)
         a2afft.launch_remap(3, tmp, work, P, nloc, lgs, dims, nbuffers=1)
 
-    t_staged = _events_ms(torch, staged, args.steps, args.warmup)
-    t_pencil = _events_ms(torch, lambda: a2afft.forward_1d_fft(work, 2048), args.steps, args.warmup)
-    t_block = _events_ms(torch, lambda: a2afft.execute_pass(plan, wb, 1, -1), args.steps, args.warmup)
+    nrot = rotation_count(block, args.steps, 16 << 30)
+    rot = Rotation(torch, pencils, nrot)
+    t_staged = _events_ms(torch, staged, rot, args.steps, args.warmup)
+    t_pencil = _events_ms(torch, lambda buf: a2afft.forward_1d_fft(buf, 2048), rot, args.steps, args.warmup)
+    finite = rot.finite()
+    del rot
+    rot = Rotation(torch, block, nrot)
+    t_block = _events_ms(torch, lambda buf: a2afft.execute_pass(plan, buf, 1, -1), rot, args.steps, args.warmup)
+    finite = finite and rot.finite()
+    del rot
     best = min(t_pencil, t_block)
     bytes_fft = 2 * nloc * 8
     # the same access patterns without arithmetic: the contiguous 2048-point
@@ -171,6 +218,7 @@ def run_c3b(args, torch, a2afft):
         "roofline": {"bound": "hbm", "kernel": "stockham 2048 contiguous", "bytes_per_launch": bytes_fft,
                      "launch_ms": round(t_pencil, 4), "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": _traffic("c3b_pencil_2048")},
+        "finite": finite,
         "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
@@ -212,17 +260,22 @@ def run_c5(args, torch, a2afft):
     for line in b.describe().splitlines():
         log(f"[bench] batch {line}")
     sched = b.schedule()
-    t_eager = _events_ms(torch, lambda: b.forward(y), args.steps, args.warmup)
-    t_graph = _events_ms(torch, lambda: b.graph_launch(y, -1), args.steps, args.warmup)
+    del y
+    rot = Rotation(torch, x, rotation_count(x, args.steps))
+    t_eager = _events_ms(torch, lambda buf: b.forward(buf), rot, args.steps, args.warmup)
+    assert b.status() == 0
+    # the batch keeps one captured graph per buffer and direction
+    t_graph = _events_ms(torch, lambda buf: b.graph_launch(buf, -1), rot, args.steps, args.warmup)
     assert b.status() == 0
     plans = {s: a2afft.GridDfft(s, "fp32") for s in set(shapes)}
-    views = b.views(y)
 
-    def separate():
-        for v, s in zip(views, shapes):
+    def separate(buf):
+        for v, s in zip(b.views(buf), shapes):
             plans[s].forward(v)
 
-    t_sep = _events_ms(torch, separate, args.steps, args.warmup)
+    t_sep = _events_ms(torch, separate, rot, args.steps, args.warmup)
+    finite = rot.finite()
+    del rot
     npts = sum(s[0] * s[1] * s[2] for s in shapes)
     flops = sum(5.0 * (s[0] * s[1] * s[2]) * math.log2(s[0] * s[1] * s[2]) for s in shapes)
     bytes_alg = 6 * npts * 8
@@ -244,15 +297,19 @@ def run_c5(args, torch, a2afft):
         "roofline": {"bound": "hbm", "kernel": "batch_kernel (persistent)", "bytes_per_launch": bytes_alg,
                      "launch_ms": round(best, 4), "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": _traffic("c5_batch")},
+        "finite": finite,
         "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
 
 
-def precision_variants(a2afft, torch, Ng, P, dims, seed, steps, warmup=30):
+def precision_variants(a2afft, torch, Ng, P, dims, seed, steps, warmup=30, rounds=3):
     """The same Ng^3 GRF (complex64, block layout over dims) through the FP32,
-    split-BF16 and split-FP16 plans: ms per forward (events, back to back) and
-    rel-L2 / max-abs against the FP64 plan on the same rounded input."""
+    split-BF16 and split-FP16 plans: ms per forward (events over `steps`
+    forwards on rotating copies of the input, the three variants interleaved
+    over `rounds` rounds, median per variant) and rel-L2 / max-abs against the
+    FP64 plan on the same rounded input."""
+    import statistics
     g = make_grf(a2afft, torch, Ng, seed)
     x = a2afft.inputs.global_to_blocks(g.to(torch.complex64), dims).contiguous()
     del g
@@ -260,19 +317,30 @@ def precision_variants(a2afft, torch, Ng, P, dims, seed, steps, warmup=30):
     ref = x.to(torch.complex128)
     a2afft.Dfft(dist64, precision="fp64", exec="fused").forward(ref)
     dist = a2afft.Distribution(Ng, P, dims=dims, dtype="c64")
+    precs = ("fp32", "split_bf16", "split_fp16")
+    plans = {p: a2afft.Dfft(dist, precision=p, exec="fused") for p in precs}
     out = {}
     data = torch.empty_like(x)
-    for prec in ("fp32", "split_bf16", "split_fp16"):
-        plan = a2afft.Dfft(dist, precision=prec, exec="fused")
+    for prec in precs:
         data.copy_(x)
-        plan.forward(data)
+        plans[prec].forward(data)
         rel, mx = a2afft.error_metrics(data, ref)
-        data.copy_(x)
-        ms = _events_ms(torch, lambda: plan.forward(data), steps, warmup)
-        out[prec] = {"ms": round(ms, 4), "rel_l2_vs_fp64": rel, "max_abs_vs_fp64": mx,
-                     "hbm_gbs_whole": round(6 * Ng ** 3 * 8 / (ms * 1e-3) / 1e9, 1)}
-        log(f"[bench] {prec}: {ms:.4f} ms rel-L2 {rel:.3e}")
-    del ref, x, data
+        out[prec] = {"rel_l2_vs_fp64": rel, "max_abs_vs_fp64": mx}
+    del ref, data
+    rot = Rotation(torch, x, rotation_count(x, steps))
+    times = {p: [] for p in precs}
+    for r in range(rounds):
+        for prec in precs:
+            times[prec].append(_events_ms(torch, lambda buf: plans[prec].forward(buf), rot, steps,
+                                          warmup if r == 0 else max(1, warmup // 5)))
+    finite = rot.finite()
+    del rot, x
+    for prec in precs:
+        ms = statistics.median(times[prec])
+        out[prec].update({"ms": round(ms, 4), "ms_rounds": [round(t, 4) for t in times[prec]],
+                          "hbm_gbs_whole": round(6 * Ng ** 3 * 8 / (ms * 1e-3) / 1e9, 1)})
+        log(f"[bench] {prec}: {ms:.4f} ms (rounds {times[prec]}) rel-L2 {out[prec]['rel_l2_vs_fp64']:.3e}")
+    out["finite"] = finite
     torch.cuda.synchronize()
     return out
 
@@ -387,30 +455,39 @@ def main():
     torch.cuda.synchronize()
 
     # ---- timing ----------------------------------------------------------
+    # Every timed forward runs on its own copy of the input (refilled between
+    # timed loops, outside them), so no step transforms an already transformed
+    # (and, after a few in-place forwards, overflowed) field.
     log(f"[bench] warmup {args.warmup}, timed {args.steps}")
-    data.copy_(x)
-    for _ in range(args.warmup):
-        plan.forward(data)
-    torch.cuda.synchronize()
+    rot = Rotation(torch, x, rotation_count(x, args.steps))
+    for i in range(args.warmup):
+        plan.forward(rot[i])
+    rot.refill()
     # timed region: K whole forwards, back to back, bracketed by synchronize
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        plan.forward(data)
+    for i in range(args.steps):
+        plan.forward(rot[i])
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms_per_step = wall * 1e3 / args.steps
+    finite = rot.finite()
+    # the timed forwards computed the same bits as the accuracy check's
+    same_bits = bool(torch.equal(torch.view_as_real(rot[0]), torch.view_as_real(data)))
+    del data
 
     # per-kernel durations: the same K forwards launch by launch, with HIP
     # events between consecutive launches on the launch stream
+    rot.refill()
     nl = plan.num_launches()
     ev = [[torch.cuda.Event(enable_timing=True) for _ in range(nl + 1)] for _ in range(args.steps)]
     kinds = [0] * nl
     torch.cuda.synchronize()
     for s in range(args.steps):
+        buf = rot[s]
         ev[s][0].record()
         for i in range(nl):
-            kinds[i] = plan.execute_launch(data, i, -1)
+            kinds[i] = plan.execute_launch(buf, i, -1)
             ev[s][i + 1].record()
     torch.cuda.synchronize()
     pass_ms = [0.0, 0.0, 0.0]
@@ -423,11 +500,12 @@ def main():
     # an event between two launches delays the second by a fixed amount:
     # measure it as the difference to the same forwards with events at the
     # ends only, and take it off every launch
+    rot.refill()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     e0.record()
-    for _ in range(args.steps):
-        plan.forward(data)
+    for i in range(args.steps):
+        plan.forward(rot[i])
     e1.record()
     torch.cuda.synchronize()
     ends_ms = e0.elapsed_time(e1) / args.steps
@@ -435,6 +513,9 @@ def main():
     for p in range(3):
         launch_ms[p] = pass_ms[p] / max(1, nper[p]) - gap_ms
         pass_ms[p] = launch_ms[p] * nper[p]
+    finite = finite and rot.finite()
+    data = rot[0]
+    del rot
 
     # measured device-copy bandwidth over the same buffer (the library's copy
     # kernel, fast_copy_fft): the practical ceiling of one read + one write pass
@@ -470,12 +551,14 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        threads = min(16, len(os.sched_getaffinity(0)))
-        log(f"[bench] CPU baseline on {threads} threads")
-        cpu_ms, used = cpu_baseline(Ng, P, dims, threads, seed)
+        threads, aff, quota, model = host_cores()
+        log(f"[bench] CPU baseline on {threads} threads (affinity {aff}, cgroup quota {quota}), {model}")
+        cpu_ms, samples, used = cpu_baseline(Ng, P, dims, threads, seed)
         cpu = {"value": round(cpu_ms, 1), "unit": "ms", "cores": used, "kind": "port",
-               "sample": f"one full {Ng}^3 forward, complex128, {P} virtual ranks dims {list(dims)}, reference stage "
-                         f"order (oracle C restatement, OpenMP)"}
+               "sample": f"median of {len(samples)} full {Ng}^3 forwards, complex128, {P} virtual ranks dims "
+                         f"{list(dims)}, reference stage order (oracle C restatement, OpenMP)",
+               "samples_ms": [round(t, 1) for t in samples], "affinity_cores": aff, "cgroup_quota_cores": quota,
+               "cpu_model": model}
 
     out = {
         "metric": "ms/512^3 c2c 3D FFT + % HBM roofline" if args.config == "c3" else f"ms/{Ng}^3 c2c 3D FFT",
@@ -513,6 +596,9 @@ def main():
         "event_gap_ms": round(gap_ms, 5),
         "rel_l2_vs_fp64": rel_l2,
         "max_abs_vs_fp64": max_abs,
+        "finite": finite,
+        "timed_equals_checked": same_bits,
+        "rotating_inputs": True,
         "roofline": {"bound": "hbm", "kernel": f"stockham {names[dom]} pass", "bytes_per_launch": int(bytes_launch),
                      "launch_ms": round(launch_ms[dom], 4), "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
@@ -523,6 +609,9 @@ def main():
         log("[bench] precision variants on the seed-3 field")
         out["precision_variants_seed3"] = precision_variants(a2afft, torch, Ng, P, dims, 3, args.steps, args.warmup)
     print(json.dumps(out), flush=True)
+    if not finite:
+        log("[bench] the timed transforms produced non-finite values")
+        sys.exit(3)
 
 
 if __name__ == "__main__":

@@ -39,43 +39,55 @@ def _bin(torch, g, k):
     return complex(out.item())
 
 
-def test_c3_512cube_c64_dims144(a2afft_mod, cuda):
-    """512^3 complex64 over 16 virtual ranks, dims [1,4,4]: FP32 within 1e-6 of
-    FP64 (and the split variants within their bounds), FP64 against Parseval and
-    two explicit bins, fused == staged bit for bit, FP32 round trip."""
+def _grf_blocks(a2afft_mod, Ng, dims, seed):
+    """The 512^3 GRF P(k) ~ k^-3 (+ 1e-3 noise) of the benchmark, rounded to
+    complex64, in block layout over dims (the generator bench.py uses)."""
     import torch
     from a2afft import inputs
-    Ng, P, dims = 512, 16, (1, 4, 4)
+    g = inputs.gaussian_random_field_device((Ng,) * 3, seed)
+    x = inputs.global_to_blocks(g.to(torch.complex64), dims).contiguous()
+    del g
+    return x
+
+
+def _fp64_checked(a2afft_mod, x, Ng, P, dims):
+    """FP64 forward of x (complex64 blocks) with Parseval's identity and two
+    bins summed explicitly in FP64 checked on it; returns the complex128 result."""
+    import torch
+    from a2afft import inputs
     n3 = Ng ** 3
-    dist = a2afft_mod.Distribution(Ng, P, dims=dims, dtype="c64")
-    x = dist.empty()
-    a2afft_mod.fill_random(x, 2, "uniform")
     x128 = x.to(torch.complex128)
     ref = x128.clone()
     dist64 = a2afft_mod.Distribution(Ng, P, dims=dims, dtype="c128")
     a2afft_mod.Dfft(dist64, "fp64").forward(ref)
-    # closed forms on the FP64 result
     par = _norm2(ref) / (n3 * _norm2(x128))
     assert abs(par - 1.0) < 1e-12, par
     g = inputs.blocks_to_global(x128, dims, (Ng,) * 3)
     yg = inputs.blocks_to_global(ref, dims, (Ng,) * 3)
-    scale = math.sqrt(n3)
-    for k in ((0, 0, 0), (3, 200, 511)):
+    scale = math.sqrt(_norm2(ref) / n3)   # RMS of the spectrum
+    for k in ((0, 0, 0), (3, 200, 511), (1, 0, 0)):
         want = _bin(torch, g, k)
         got = complex(yg[k].item())
         assert abs(got - want) < 1e-10 * scale, (k, got, want)
-    del g, yg
-    # FP32 and split variants against FP64
+    del g, yg, x128
+    return ref
+
+
+def test_c3_512cube_c64_dims144(a2afft_mod, cuda):
+    """512^3 complex64 GRF (P(k) ~ k^-3, seed 2, + 1e-3 noise) over 16 virtual
+    ranks, dims [1,4,4]: FP32 within 1e-6 of FP64 (rel-L2), the FP64 result
+    against Parseval and explicit bins, fused == staged bit for bit, FP32
+    round trip."""
+    import torch
+    Ng, P, dims = 512, 16, (1, 4, 4)
+    dist = a2afft_mod.Distribution(Ng, P, dims=dims, dtype="c64")
+    x = _grf_blocks(a2afft_mod, Ng, dims, 2).reshape(P, -1)
+    ref = _fp64_checked(a2afft_mod, x, Ng, P, dims)
     y = x.clone()
     a2afft_mod.Dfft(dist, "fp32").forward(y)
-    rel, _ = a2afft_mod.error_metrics(y, ref)
+    rel, mx = a2afft_mod.error_metrics(y, ref)
+    print(f"c3 fp32 rel-L2 {rel:.3e} max-abs {mx:.3e}")
     assert rel <= 1e-6, rel
-    for prec, tol in (("split_bf16", 1e-5), ("split_fp16", 1e-6)):
-        s = x.clone()
-        a2afft_mod.Dfft(dist, prec).forward(s)
-        r, _ = a2afft_mod.error_metrics(s, ref)
-        assert r <= tol, (prec, r)
-        del s
     del ref
     # the reference's stage order computes the same bits
     st = x.clone()
@@ -86,8 +98,54 @@ def test_c3_512cube_c64_dims144(a2afft_mod, cuda):
     del st, staged
     # round trip
     a2afft_mod.Dfft(dist, "fp32").inverse(y)
-    rel, _ = a2afft_mod.error_metrics(y, x128)
+    rel, _ = a2afft_mod.error_metrics(y, x.to(torch.complex128))
     assert rel <= 1e-6, rel
+
+
+def test_c4_512cube_precision_variants(a2afft_mod, cuda):
+    """512^3 complex64 GRF (seed 3) over dims [1,4,4] through FP32 (<= 1e-6),
+    split-BF16 (<= 1e-5) and split-FP16 (<= 1e-6), rel-L2 against the FP64
+    result, which is itself checked against Parseval and explicit bins."""
+    Ng, P, dims = 512, 16, (1, 4, 4)
+    dist = a2afft_mod.Distribution(Ng, P, dims=dims, dtype="c64")
+    x = _grf_blocks(a2afft_mod, Ng, dims, 3).reshape(P, -1)
+    ref = _fp64_checked(a2afft_mod, x, Ng, P, dims)
+    for prec, tol in (("fp32", 1e-6), ("split_bf16", 1e-5), ("split_fp16", 1e-6)):
+        s = x.clone()
+        a2afft_mod.Dfft(dist, prec).forward(s)
+        r, mx = a2afft_mod.error_metrics(s, ref)
+        print(f"c4 {prec} rel-L2 {r:.3e} max-abs {mx:.3e}")
+        assert r <= tol, (prec, r)
+        del s
+
+
+def test_c3_uniform_split_variants(a2afft_mod, cuda):
+    """512^3 complex64 U[-1,1) (seed 2) over dims [1,4,4]: the split variants
+    within their bounds on a flat spectrum too."""
+    import torch
+    Ng, P, dims = 512, 16, (1, 4, 4)
+    dist = a2afft_mod.Distribution(Ng, P, dims=dims, dtype="c64")
+    x = dist.empty()
+    a2afft_mod.fill_random(x, 2, "uniform")
+    ref = x.to(torch.complex128)
+    a2afft_mod.Dfft(a2afft_mod.Distribution(Ng, P, dims=dims, dtype="c128"), "fp64").forward(ref)
+    for prec, tol in (("fp32", 1e-6), ("split_bf16", 1e-5), ("split_fp16", 1e-6)):
+        s = x.clone()
+        a2afft_mod.Dfft(dist, prec).forward(s)
+        r, _ = a2afft_mod.error_metrics(s, ref)
+        assert r <= tol, (prec, r)
+        del s
+
+
+def test_grf_device_matches_host(a2afft_mod, cuda):
+    """The device GRF generator (used by bench.py and the full-size tests)
+    equals the host recipe (numpy Philox + scipy FFT) to FP64 rounding."""
+    from a2afft import inputs
+    for shape, seed in (((32, 32, 32), 2), ((64, 64, 64), 3), ((32, 64, 16), 2)):
+        d = inputs.gaussian_random_field_device(shape, seed).cpu().numpy()
+        h = inputs.gaussian_random_field(shape, seed)
+        rel = np.linalg.norm(d - h) / np.linalg.norm(h)
+        assert rel < 1e-13, (shape, rel)
 
 
 @pytest.mark.parametrize("dims", [(1, 2, 4), (2, 2, 2)])
