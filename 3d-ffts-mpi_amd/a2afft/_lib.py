@@ -109,6 +109,8 @@ SIGNATURES = {
     "a2afft_batch_destroy": [_P],
     "a2afft_error_metrics": [_P, _I, _P, _I64, ctypes.POINTER(ctypes.c_double),
                              ctypes.POINTER(ctypes.c_double), _P],
+    "a2afft_error_metrics_noalloc": [_P, _I, _P, _I64, ctypes.POINTER(ctypes.c_double),
+                                     ctypes.POINTER(ctypes.c_double), _P],
 }
 _RESTYPES = {
     "a2afft_status_string": ctypes.c_char_p,

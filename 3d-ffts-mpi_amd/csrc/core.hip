@@ -981,3 +981,67 @@ extern "C" int a2afft_error_metrics(const void* y, a2afft_dtype dtype, const voi
     *max_abs = mx;
     return A2AFFT_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Allocation-free error reduction.  a2afft_error_metrics allocates its
+// per-workgroup partial sums on every call; this variant keeps them in a
+// file-scope device array instead, for callers that cannot allocate (code
+// under stream capture, serving loops).  Each GPU holds its own copy of the
+// array, found once per device with hipGetSymbolAddress; every call on a
+// device shares it, so calls that overlap on one device race.  Every
+// workgroup writes its three partials on every call, so nothing needs
+// clearing between calls.
+// ---------------------------------------------------------------------------
+constexpr int kErrBlocks = 1024;
+constexpr int64_t kErrNoAllocMaxElems = int64_t(1) << 36;   // 1 TiB of complex128: beyond any one GPU
+static __device__ __align__(256) double scratch[3 * kErrBlocks];
+
+static std::mutex g_scratch_mu;
+static std::vector<void*> g_scratch_ptr;   // per device: the address of `scratch`
+
+static int scratch_address(double** out) {
+    int dev = 0;
+    A2A_TRY(hip_check(int(hipGetDevice(&dev)), "error_metrics_noalloc: device"));
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (size_t(dev) >= g_scratch_ptr.size()) g_scratch_ptr.resize(size_t(dev) + 1, nullptr);
+    if (!g_scratch_ptr[size_t(dev)]) {
+        void* p = nullptr;
+        A2A_TRY(hip_check(int(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch))), "error_metrics_noalloc: scratch"));
+        if (!p) return set_error(A2AFFT_E_HIP, "error_metrics_noalloc: scratch has no device address");
+        g_scratch_ptr[size_t(dev)] = p;
+    }
+    *out = static_cast<double*>(g_scratch_ptr[size_t(dev)]);
+    return A2AFFT_OK;
+}
+
+extern "C" int a2afft_error_metrics_noalloc(const void* y, a2afft_dtype dtype, const void* ref, int64_t n,
+                                            double* rel_l2, double* max_abs, void* stream) {
+    if (!y || !ref || n <= 0 || !rel_l2 || !max_abs || (dtype != A2AFFT_C64 && dtype != A2AFFT_C128))
+        return set_error(A2AFFT_E_INVALID_ARG, "error_metrics_noalloc: bad args");
+    if (n > kErrNoAllocMaxElems)
+        return set_error(A2AFFT_E_SIZE, "error_metrics_noalloc: %lld elements exceed the maximum of 2^36",
+                         (long long)n);
+    double* part = nullptr;
+    A2A_TRY(scratch_address(&part));
+    hipStream_t s = (hipStream_t)stream;
+    if (dtype == A2AFFT_C128)
+        hipLaunchKernelGGL(err_partials<double2>, dim3(kErrBlocks), dim3(256), 0, s, (const double2*)y,
+                           (const double2*)ref, uint64_t(n), part);
+    else
+        hipLaunchKernelGGL(err_partials<float2>, dim3(kErrBlocks), dim3(256), 0, s, (const float2*)y,
+                           (const double2*)ref, uint64_t(n), part);
+    A2A_TRY(hip_check(int(hipGetLastError()), "error_metrics_noalloc launch"));
+    double h[3 * kErrBlocks];
+    A2A_TRY(hip_check(int(hipMemcpyAsync(h, part, sizeof(h), hipMemcpyDeviceToHost, s)), "error_metrics_noalloc copy"));
+    A2A_TRY(hip_check(int(hipStreamSynchronize(s)), "error_metrics_noalloc sync"));
+    long double sd = 0.0L, sr = 0.0L;
+    double mx = 0.0;
+    for (int b = 0; b < kErrBlocks; ++b) {
+        sd += h[3 * b];
+        sr += h[3 * b + 1];
+        mx = h[3 * b + 2] > mx ? h[3 * b + 2] : mx;
+    }
+    *rel_l2 = sr > 0 ? double(sqrtl(sd / sr)) : double(sqrtl(sd));
+    *max_abs = mx;
+    return A2AFFT_OK;
+}

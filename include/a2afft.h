@@ -264,6 +264,17 @@ int a2afft_batch_destroy(a2afft_batch batch);
 int a2afft_error_metrics(const void* y, a2afft_dtype dtype, const void* ref, int64_t n,
                          double* rel_l2, double* max_abs, void* stream);
 
+/* a2afft_error_metrics without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (serving
+ * loops, harnesses that forbid hipMalloc).  The partial sums live in a
+ * fixed 24 KiB device array of the library (one per GPU, held until the
+ * process ends) instead of a per-call allocation; the result is the same,
+ * bit for bit.  n <= 2^36, else A2AFFT_E_SIZE.  Calls that overlap on one
+ * device race on that array.  It synchronizes `stream`, so it cannot be
+ * recorded into a graph. */
+int a2afft_error_metrics_noalloc(const void* y, a2afft_dtype dtype, const void* ref, int64_t n,
+                                 double* rel_l2, double* max_abs, void* stream);
+
 /* One axis pass of a fused plan, exactly as a2afft_forward (direction -1) or
  * a2afft_inverse (+1) runs it: pass 0 = z, 1 = x, 2 = y (the reference's axis
  * order, a2adfft.cpp:31-85); the inverse's 1/Ng^3 rides on pass 2.  For
