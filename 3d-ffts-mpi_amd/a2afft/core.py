@@ -153,7 +153,8 @@ class _Plan:
 
     def set_kernel(self, pass_index, T, E, flags=0):
         """Force a kernel variant: flags 1 XCD order, 2 LDS pad, 4 non-temporal, 8 z-tiles outermost,
-        32 half-width LDS exchanges."""
+        32 half-width LDS exchanges.  Split plans: T=16, E=0 selects the interleaved-K split-BF16
+        kernel (flags 1, 4, 8 as above), E=N the split-K-halves kernels."""
         return L.check(L.lib().a2afft_plan_set_kernel(self._h, pass_index, T, E, flags), "set_kernel")
 
     def set_chunking(self, z_columns):

@@ -259,6 +259,12 @@ A2A_STAGE_ENTRY(a2afft_return_y_pencils, 5)
 // Plan / execute (a2adfft)
 // ===========================================================================
 
+// One kernel launch of a fused transform.
+struct Launch {
+    int pass;
+    uint32_t start, count;
+};
+
 struct a2afft_plan_s {
     a2afft_dist_s d;    // the distribution (Ng = 0 for a single-grid plan)
     int n[3];           // global extents x, y, z
@@ -272,6 +278,8 @@ struct a2afft_plan_s {
     hipStream_t cap_stream = nullptr;                // graph capture stream (graph_launch)
     hipGraphExec_t graph[2] = {nullptr, nullptr};    // forward, inverse
     void* graph_data[2] = {nullptr, nullptr};        // the buffer each graph was captured for
+    std::vector<Launch> launches;             // launch list for `launches_chunk`
+    uint32_t launches_chunk = ~0u;
 };
 
 // Graphs hold the kernel choices and arguments of their capture: any change
@@ -284,31 +292,31 @@ static void drop_plan_graphs(a2afft_plan_s* p) {
     }
 }
 
-// One kernel launch of a fused transform: the z pass, then the x and y
-// passes either whole or alternating over windows of `chunk` z columns, so a
+// The launches of a fused transform: the z pass, then the x and y passes
+// either whole or alternating over windows of `chunk` z columns, so a
 // window's data is still in the 256 MB Infinity Cache when the y pass reads
 // what the x pass wrote.  Every window holds complete x and y lines, so the
 // arithmetic per line -- and the result -- is the same for every chunking.
-struct Launch {
-    int pass;
-    uint32_t start, count;
-};
-
-static int plan_launches(const a2afft_plan_s* p, Launch* out, int max) {
-    int n = 0;
-    out[n++] = Launch{0, 0, 0};
+// The list is rebuilt when the chunk width changes, so execution itself
+// allocates nothing once a plan has run.
+static const std::vector<Launch>& plan_launches(a2afft_plan_s* p) {
+    if (p->launches_chunk == p->chunk) return p->launches;
+    std::vector<Launch>& out = p->launches;
+    out.clear();
+    out.push_back(Launch{0, 0, 0});
     if (!p->chunk) {
-        out[n++] = Launch{1, 0, 0};
-        out[n++] = Launch{2, 0, 0};
-        return n;
+        out.push_back(Launch{1, 0, 0});
+        out.push_back(Launch{2, 0, 0});
+    } else {
+        const uint32_t nz = uint32_t(p->n[2]);
+        for (uint32_t z0 = 0; z0 < nz; z0 += p->chunk) {
+            const uint32_t cnt = nz - z0 < p->chunk ? nz - z0 : p->chunk;
+            out.push_back(Launch{1, z0, cnt});
+            out.push_back(Launch{2, z0, cnt});
+        }
     }
-    const uint32_t nz = uint32_t(p->n[2]);
-    for (uint32_t z0 = 0; z0 < nz && n + 2 <= max; z0 += p->chunk) {
-        const uint32_t cnt = nz - z0 < p->chunk ? nz - z0 : p->chunk;
-        out[n++] = Launch{1, z0, cnt};
-        out[n++] = Launch{2, z0, cnt};
-    }
-    return n;
+    p->launches_chunk = p->chunk;
+    return out;
 }
 
 static bool chunk_ok(const a2afft_plan_s* p, uint32_t chunk) {
@@ -328,19 +336,67 @@ static bool is_split(a2afft_precision prec) { return prec == A2AFFT_SPLIT_BF16 |
 //   512^3 complex64:  1.163 -> 1.134 ms with 128-column windows,
 //   512^3 complex128: 2.40 -> 2.23 ms with 64-column windows.
 // complex64 with full-exchange tiles (1.163 -> 1.189 ms) and smaller grids
-// measured slower chunked: whole passes otherwise.
+// measured slower chunked: whole passes otherwise.  Split-BF16 plans window
+// when both strided passes run the interleaved-K kernel; the split-K-halves
+// kernels (split-FP16) measured slower windowed (1.41-1.48 against 1.30-1.33
+// ms) and run whole passes.
 static uint32_t default_chunk(const a2afft_plan_s* p) {
-    if (p->prec != A2AFFT_FP32 && p->prec != A2AFFT_FP64) return 0;
-    const bool c64 = p->prec == A2AFFT_FP32;
+    if (p->prec == A2AFFT_SPLIT_FP16) return 0;
+    const bool c64 = p->prec != A2AFFT_FP64;
     for (int i = 1; i < 3; ++i)
         if (p->pass[i].fft.n != 512u || !p->pass[i].strided) return 0;
-    if (!(p->sel[1].flags & KF_HALFX) || (c64 && !(p->sel[2].flags & KF_HALFX))) return 0;
+    if (p->prec == A2AFFT_SPLIT_BF16) {
+        // the interleaved-K kernel on both strided passes (split-BF16 512^3
+        // complex64: 1.18-1.19 ms whole, 1.104-1.116 ms in 128-column windows)
+        if (p->sel[1].E != 0 || p->sel[2].E != 0) return 0;
+    } else if (!(p->sel[1].flags & KF_HALFX) || (c64 && !(p->sel[2].flags & KF_HALFX))) {
+        return 0;
+    }
     const uint64_t column = uint64_t(p->n[0]) * uint64_t(p->n[1]) * (c64 ? 8u : 16u);   // bytes per z column
     if (column * uint64_t(p->n[2]) <= (uint64_t(512) << 20)) return 0;
     uint32_t w = 0;
     for (uint32_t c = 16; c < uint32_t(p->n[2]) && column * c <= (uint64_t(256) << 20); c *= 2)
         if (chunk_ok(p, c)) w = c;
     return w;
+}
+
+// The kernel choices that can run fused pass i of the plan.
+static int pass_candidates(const a2afft_plan_s* p, int i, KSel* out, int max) {
+    const int N = int(p->pass[i].fft.n);
+    if (is_split(p->prec)) return mfma_candidates(p->pass[i], N, p->prec, out, max);
+    return fft_candidates(p->pass[i], p->d.dtype, N, out, max);
+}
+
+static bool sel_allowed(const a2afft_plan_s* p, int i, const KSel& s) {
+    KSel cand[256];
+    const int n = pass_candidates(p, i, cand, 256);
+    for (int c = 0; c < n; ++c)
+        if (cand[c].T == s.T && cand[c].E == s.E && cand[c].flags == s.flags) return true;
+    return false;
+}
+
+// Builds the twiddle tables selection `s` of pass i reads, both directions,
+// so that execution allocates nothing.
+static int sel_tables(const a2afft_plan_s* p, int i, const KSel& s) {
+    const int N = int(p->pass[i].fft.n);
+    const bool ok = is_split(p->prec)
+                        ? (mfma_twiddle_table(N, -1) && mfma_twiddle_table(N, 1))
+                        : (pass_twiddles(p->pass[i], N, -1, p->d.dtype, s) && pass_twiddles(p->pass[i], N, 1, p->d.dtype, s));
+    return ok ? A2AFFT_OK : set_error(A2AFFT_E_ALLOC, "plan: twiddle tables");
+}
+
+// Threads per workgroup that selection `s` of pass i launches.
+static int sel_threads(const a2afft_plan_s* p, int i, const KSel& s) {
+    const int N = int(p->pass[i].fft.n);
+    if (is_split(p->prec)) return s.E == 0 ? 256 : (N >= 256 ? 512 : 256);
+    FftShape sh;
+    return fft_pass_shape(p->pass[i], p->d.dtype, N, &s, &sh) == A2AFFT_OK ? sh.NT : 0;
+}
+
+// One launch of fused pass `ps` with selection `sel`.
+static int launch_pass(const a2afft_plan_s* p, const AxisPass& ps, void* D, int dir, hipStream_t s, const KSel* sel) {
+    if (is_split(p->prec)) return launch_mfma_pass(ps, D, int(ps.fft.n), dir, p->prec, s, sel);
+    return launch_fft_pass(ps, D, p->d.dtype, int(ps.fft.n), dir, s, sel);
 }
 
 static int plan_init(a2afft_plan_s* p) {
@@ -355,9 +411,8 @@ static int plan_init(a2afft_plan_s* p) {
         const int N = int(p->pass[i].fft.n);
         if (is_split(p->prec)) {
             A2A_TRY(mfma_pass_check(p->pass[i], N));
-            p->sel[i] = KSel{mf_tile_T(N, p->pass[i].strided), N, 0};
-            if (!mfma_twiddle_table(N, -1) || !mfma_twiddle_table(N, 1))
-                return set_error(A2AFFT_E_ALLOC, "plan: twiddle tables");
+            p->sel[i] = mfma_default_sel(p->pass[i], N, p->prec, false);
+            A2A_TRY(sel_tables(p, i, p->sel[i]));
             continue;
         }
         if (!fft_length_supported(N)) return set_error(A2AFFT_E_UNSUPPORTED, "plan: length %d not supported", N);
@@ -366,11 +421,20 @@ static int plan_init(a2afft_plan_s* p) {
         if (!find_kernel(N, p->d.dtype == A2AFFT_C128, p->pass[i].strided, p->sel[i], -1))
             return set_error(A2AFFT_E_UNSUPPORTED, "plan: no kernel built for length %d in this precision", N);
         // build the twiddle tables now so execute allocates nothing
-        if (!pass_twiddles(p->pass[i], N, -1, p->d.dtype, p->sel[i]) ||
-            !pass_twiddles(p->pass[i], N, 1, p->d.dtype, p->sel[i]))
+        A2A_TRY(sel_tables(p, i, p->sel[i]));
+    }
+    // the staged path's batched pencil pass (its default selection)
+    if (p->exec == A2AFFT_EXEC_STAGED && p->d.Ng > 0) {
+        const int64_t total = int64_t(p->d.P) * p->d.nlocal;
+        const PassMaps bm = batched_pass_maps(p->d.Ng, uint32_t(total / p->d.Ng));
+        const AxisPass bp{bm.fft, bm.a1, bm.a2, 0, 0, 1.0};
+        const KSel bs = fft_default_sel(bp, p->d.dtype, p->d.Ng);
+        if (!pass_twiddles(bp, p->d.Ng, -1, p->d.dtype, bs) || !pass_twiddles(bp, p->d.Ng, 1, p->d.dtype, bs))
             return set_error(A2AFFT_E_ALLOC, "plan: twiddle tables");
     }
     p->chunk = default_chunk(p);
+    if (p->chunk && is_split(p->prec))   // the windowed choice of the strided passes
+        for (int i = 1; i < 3; ++i) p->sel[i] = mfma_default_sel(p->pass[i], int(p->pass[i].fft.n), p->prec, true);
     return A2AFFT_OK;
 }
 
@@ -411,7 +475,7 @@ static std::string plan_key(const a2afft_plan_s* p) {
         if (colon != std::string::npos) arch.resize(colon);
     }
     char buf[256];
-    snprintf(buf, sizeof(buf), "a2afft-tune-v1 arch=%s grid=%dx%dx%d P=%d dims=%dx%dx%d dtype=%d prec=%d exec=%d",
+    snprintf(buf, sizeof(buf), "a2afft-tune-v2 arch=%s grid=%dx%dx%d P=%d dims=%dx%dx%d dtype=%d prec=%d exec=%d",
              arch.c_str(), p->n[0], p->n[1], p->n[2], p->d.P, p->d.dims[0], p->d.dims[1], p->d.dims[2],
              int(p->d.dtype), int(p->prec), int(p->exec));
     return buf;
@@ -438,7 +502,7 @@ static int cache_apply(a2afft_plan_s* p) {
         std::lock_guard<std::mutex> lk(g_cache_mu);
         path = g_cache_path;
     }
-    if (path.empty() || p->exec != A2AFFT_EXEC_FUSED || is_split(p->prec)) return 0;
+    if (path.empty() || p->exec != A2AFFT_EXEC_FUSED) return 0;
     const std::string key = plan_key(p) + " | ";
     for (const std::string& l : cache_lines(path)) {
         if (l.compare(0, key.size(), key) != 0) continue;
@@ -450,18 +514,20 @@ static int cache_apply(a2afft_plan_s* p) {
             return 0;
         const KSel keep[3] = {p->sel[0], p->sel[1], p->sel[2]};
         const uint32_t keep_chunk = p->chunk;
+        // validating an entry must not leave an error message behind
+        char saved_err[sizeof(g_err)];
+        memcpy(saved_err, g_err, sizeof(g_err));
         bool ok = true;
         for (int i = 0; i < 3 && ok; ++i) {
-            const int N = int(p->pass[i].fft.n);
-            FftShape sh;
-            ok = find_kernel(N, p->d.dtype == A2AFFT_C128, p->pass[i].strided, sel[i], -1) != nullptr &&
-                 fft_pass_shape(p->pass[i], p->d.dtype, N, &sel[i], &sh) == A2AFFT_OK;
+            ok = sel_allowed(p, i, sel[i]);
             if (ok) p->sel[i] = sel[i];
         }
         if (ok) {
             p->chunk = chunk;
-            ok = chunk_ok(p, chunk);
+            ok = (chunk == 0 || chunk >= 8) && chunk_ok(p, chunk);
         }
+        for (int i = 0; i < 3 && ok; ++i) ok = sel_tables(p, i, p->sel[i]) == A2AFFT_OK;
+        memcpy(g_err, saved_err, sizeof(g_err));
         if (!ok) {   // leave the defaults
             for (int i = 0; i < 3; ++i) p->sel[i] = keep[i];
             p->chunk = keep_chunk;
@@ -609,16 +675,13 @@ static int run_launch(a2afft_plan_s* p, const Launch& L, void* D, int dir, hipSt
         ps.scale = double(int64_t(p->n[0]) * p->n[1] * p->n[2]);
         ps.scale_mode = 2;
     }
-    if (is_split(p->prec)) return launch_mfma_pass(ps, D, int(ps.fft.n), dir, p->prec, s);
-    return launch_fft_pass(ps, D, p->d.dtype, int(ps.fft.n), dir, s, &p->sel[L.pass]);
+    return launch_pass(p, ps, D, dir, s, &p->sel[L.pass]);
 }
 
 // In-place passes over the block layout in the reference's axis order z, x,
 // y, with 1/(nx*ny*nz) folded into the last inverse pass.
 static int exec_fused(a2afft_plan_s* p, void* D, int dir, hipStream_t s) {
-    Launch L[1024];
-    const int n = plan_launches(p, L, int(sizeof(L) / sizeof(L[0])));
-    for (int i = 0; i < n; ++i) A2A_TRY(run_launch(p, L[i], D, dir, s));
+    for (const Launch& L : plan_launches(p)) A2A_TRY(run_launch(p, L, D, dir, s));
     return A2AFFT_OK;
 }
 
@@ -640,11 +703,9 @@ extern "C" int a2afft_execute_pass(a2afft_plan p, void* data, int pass, int dire
     if (!p || !data || pass < 0 || pass > 2 || (direction != -1 && direction != 1))
         return set_error(A2AFFT_E_INVALID_ARG, "execute_pass: bad arguments");
     if (p->exec != A2AFFT_EXEC_FUSED) return set_error(A2AFFT_E_INVALID_ARG, "execute_pass: fused plans only");
-    Launch L[1024];
-    const int n = plan_launches(p, L, int(sizeof(L) / sizeof(L[0])));
-    for (int i = 0; i < n; ++i)
-        if (L[i].pass == pass) {
-            int st = run_launch(p, L[i], data, direction, (hipStream_t)stream);
+    for (const Launch& L : plan_launches(p))
+        if (L.pass == pass) {
+            int st = run_launch(p, L, data, direction, (hipStream_t)stream);
             if (st < 0) return st;
         }
     return finish_call(stream, A2AFFT_OK);
@@ -653,8 +714,7 @@ extern "C" int a2afft_execute_pass(a2afft_plan p, void* data, int pass, int dire
 extern "C" int a2afft_plan_num_launches(a2afft_plan p) {
     if (!p) return set_error(A2AFFT_E_INVALID_ARG, "num_launches: null plan");
     if (p->exec != A2AFFT_EXEC_FUSED) return set_error(A2AFFT_E_INVALID_ARG, "num_launches: fused plans only");
-    Launch L[1024];
-    return plan_launches(p, L, int(sizeof(L) / sizeof(L[0])));
+    return int(plan_launches(p).size());
 }
 
 // Launch `index` of a fused forward / inverse; returns the pass it belongs
@@ -663,13 +723,12 @@ extern "C" int a2afft_execute_launch(a2afft_plan p, void* data, int index, int d
     if (!p || !data || (direction != -1 && direction != 1))
         return set_error(A2AFFT_E_INVALID_ARG, "execute_launch: bad arguments");
     if (p->exec != A2AFFT_EXEC_FUSED) return set_error(A2AFFT_E_INVALID_ARG, "execute_launch: fused plans only");
-    Launch L[1024];
-    const int n = plan_launches(p, L, int(sizeof(L) / sizeof(L[0])));
-    if (index < 0 || index >= n) return set_error(A2AFFT_E_INVALID_ARG, "execute_launch: index out of range");
-    int st = run_launch(p, L[index], data, direction, (hipStream_t)stream);
+    const std::vector<Launch>& L = plan_launches(p);
+    if (index < 0 || index >= int(L.size())) return set_error(A2AFFT_E_INVALID_ARG, "execute_launch: index out of range");
+    int st = run_launch(p, L[size_t(index)], data, direction, (hipStream_t)stream);
     if (st < 0) return st;
     st = finish_call(stream, A2AFFT_OK);
-    return st < 0 ? st : L[index].pass;
+    return st < 0 ? st : L[size_t(index)].pass;
 }
 
 extern "C" int a2afft_plan_set_chunking(a2afft_plan p, int z_columns) {
@@ -687,24 +746,23 @@ extern "C" int a2afft_plan_set_chunking(a2afft_plan p, int z_columns) {
 
 extern "C" int a2afft_plan_set_kernel(a2afft_plan p, int pass, int T, int E, int flags) {
     if (!p || pass < 0 || pass > 2) return set_error(A2AFFT_E_INVALID_ARG, "set_kernel: bad arguments");
-    if (is_split(p->prec)) return set_error(A2AFFT_E_UNSUPPORTED, "set_kernel: split plans have one kernel per pass");
-    KSel s{T, E, flags};
-    KSel cand[256];
-    const int n = fft_candidates(p->pass[pass], p->d.dtype, int(p->pass[pass].fft.n), cand, 256);
-    for (int i = 0; i < n; ++i)
-        if (cand[i].T == s.T && cand[i].E == s.E && cand[i].flags == s.flags) {
-            const KSel old = p->sel[pass];
-            p->sel[pass] = s;
-            if (!chunk_ok(p, p->chunk)) {
-                p->sel[pass] = old;
-                return set_error(A2AFFT_E_INVALID_ARG, "set_kernel: T=%d does not divide the chunk width %u", T,
-                                 p->chunk);
-            }
-            drop_plan_graphs(p);
-            return A2AFFT_OK;
-        }
-    return set_error(A2AFFT_E_UNSUPPORTED, "set_kernel: no built variant T=%d E=%d flags=%d fits pass %d", T, E,
-                     flags, pass);
+    const KSel s{T, E, flags};
+    if (!sel_allowed(p, pass, s))
+        return set_error(A2AFFT_E_UNSUPPORTED, "set_kernel: no built variant T=%d E=%d flags=%d fits pass %d", T, E,
+                         flags, pass);
+    const KSel old = p->sel[pass];
+    p->sel[pass] = s;
+    if (!chunk_ok(p, p->chunk)) {
+        p->sel[pass] = old;
+        return set_error(A2AFFT_E_INVALID_ARG, "set_kernel: T=%d does not divide the chunk width %u", T, p->chunk);
+    }
+    const int st = sel_tables(p, pass, s);
+    if (st < 0) {
+        p->sel[pass] = old;
+        return st;
+    }
+    drop_plan_graphs(p);
+    return A2AFFT_OK;
 }
 
 // Times every built variant that fits each fused pass (forward direction,
@@ -738,17 +796,15 @@ extern "C" int a2afft_plan_autotune(a2afft_plan p, void* work, int iters, void* 
         *ms /= float(reps);
         return A2AFFT_OK;
     };
-    for (int i = 0; i < 3 && st >= 0 && !is_split(p->prec); ++i) {
+    for (int i = 0; i < 3 && st >= 0; ++i) {
         KSel cand[256];
-        const int N = int(p->pass[i].fft.n);
-        const int n = fft_candidates(p->pass[i], p->d.dtype, N, cand, 256);
+        const int n = pass_candidates(p, i, cand, 256);
         float best = 1e30f;
         for (int c = 0; c < n && st >= 0; ++c) {
-            st = launch_fft_pass(p->pass[i], work, p->d.dtype, N, -1, s, &cand[c]);   // warm
+            st = launch_pass(p, p->pass[i], work, -1, s, &cand[c]);   // warm
             if (st < 0) break;
             st = hip_check(int(hipEventRecord(e0, s)), "autotune record");
-            for (int it = 0; it < iters && st >= 0; ++it)
-                st = launch_fft_pass(p->pass[i], work, p->d.dtype, N, -1, s, &cand[c]);
+            for (int it = 0; it < iters && st >= 0; ++it) st = launch_pass(p, p->pass[i], work, -1, s, &cand[c]);
             if (st < 0) break;
             st = hip_check(int(hipEventRecord(e1, s)), "autotune record");
             if (st >= 0) st = hip_check(int(hipEventSynchronize(e1)), "autotune sync");
@@ -763,7 +819,7 @@ extern "C" int a2afft_plan_autotune(a2afft_plan p, void* work, int iters, void* 
             if (len < int(sizeof(p->report)) - 96)
                 len += snprintf(p->report + len, sizeof(p->report) - len,
                                 "pass %s T=%d E=%d NT=%d flags=%d %.4f ms\n", names[i], cand[c].T, cand[c].E,
-                                cand[c].T * (N / cand[c].E), cand[c].flags, ms);
+                                sel_threads(p, i, cand[c]), cand[c].flags, ms);
         }
     }
     // chunk widths: time whole forwards
@@ -813,6 +869,7 @@ extern "C" int a2afft_plan_autotune(a2afft_plan p, void* work, int iters, void* 
             len += snprintf(p->report + len, sizeof(p->report) - len, "forward tuned %.4f ms, before %.4f ms: kept %s\n",
                             tuned, start, tuned < start ? "tuned" : "before");
     }
+    for (int i = 0; i < 3 && st >= 0; ++i) st = sel_tables(p, i, p->sel[i]);   // both directions
     if (st < 0) {   // leave the plan as it was
         for (int i = 0; i < 3; ++i) p->sel[i] = start_sel[i];
         p->chunk = start_chunk;
@@ -821,7 +878,7 @@ extern "C" int a2afft_plan_autotune(a2afft_plan p, void* work, int iters, void* 
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (st < 0) return st;
-    if (!is_split(p->prec)) A2A_TRY(cache_store(p));
+    A2A_TRY(cache_store(p));
     return A2AFFT_OK;
 }
 
@@ -832,8 +889,10 @@ extern "C" int a2afft_plan_describe(a2afft_plan p, char* buf, int len) {
     int n = 0;
     for (int i = 0; i < 3 && n < len; ++i) {
         if (is_split(p->prec)) {
-            n += snprintf(buf + n, len - n, "chosen %s split-%s MFMA radix-16 T=%d NT=256\n", names[i],
-                          p->prec == A2AFFT_SPLIT_BF16 ? "bf16" : "fp16", p->sel[i].T);
+            n += snprintf(buf + n, len - n, "chosen %s split-%s MFMA radix-16 %s T=%d E=%d NT=%d flags=%d\n", names[i],
+                          p->prec == A2AFFT_SPLIT_BF16 ? "bf16" : "fp16",
+                          p->sel[i].E == 0 ? "interleaved-K" : "split-K-halves", p->sel[i].T, p->sel[i].E,
+                          sel_threads(p, i, p->sel[i]), p->sel[i].flags);
             continue;
         }
         FftShape sh;

@@ -246,10 +246,13 @@ int fft_pass_shape(const AxisPass& p, a2afft_dtype dt, int N, const KSel* sel, F
     const KSel s = sel ? *sel : fft_default_sel(p, dt, N);
     FftGeom g;
     uint32_t grid;
-    A2A_TRY(resolve(p, dt, N, s, &g, &grid));
+    MixedPlan mp{};
+    A2A_TRY(resolve(p, dt, N, s, &g, &grid, &mp));
     out->T = s.T;
     out->E = s.E;
-    out->NT = is_dft(s) ? 256 : s.T * (N / s.E);
+    // the launched thread count: the runtime-schedule kernel rounds its
+    // workgroup to its own width (mp.NT)
+    out->NT = is_dft(s) ? 256 : is_mix(s) ? mp.NT : s.T * (N / s.E);
     out->grid = int(grid);
     out->flags = s.flags;
     return A2AFFT_OK;

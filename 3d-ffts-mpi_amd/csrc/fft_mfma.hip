@@ -503,6 +503,418 @@ __global__ __launch_bounds__(64 * NW, NW == 8 ? 4 : 2) void mfma_fft_direct(floa
 }
 
 // ---------------------------------------------------------------------------
+// Split-BF16 passes of 256 = 16 * 16 and 512 = 16 * 16 * 2 point lines with
+// the complex values interleaved along K, one LDS exchange of operands that
+// are already split, and buffer addressing.
+//
+// K order.  The 32 K slots of a radix-16 product hold the 16 inputs as
+// (re, im) pairs: lane group g (= lane / 16) holds inputs r = 4g .. 4g+3, so
+// a lane's 8 bf16 operand values are 4 whole complex numbers -- the 4 values
+// it loads -- and no lane exchange is needed.  Rows of the DFT operand are
+// (Re F, -Im F) pairs for the real outputs and (Im F, Re F) pairs for the
+// imaginary outputs.  Products are accumulated small to large:
+// lo*hi, hi*lo, hi*hi.
+//
+// Stockham indices (B0 = N/16 radix-16 butterflies per line, R2 = N/256):
+//   stage 0 butterfly j reads x[j + B0 r] and writes d = 16 j + q;
+//   stage 1 butterfly j1 reads d = j1 + B0 r, i.e. output q = j1 % 16 of
+//     stage-0 butterfly j = j1/16 + R2 r, twiddled by W_256^(r q), and writes
+//     256 (j1/16) + j1 % 16 + 16 q';
+//   stage 2 (R2 = 2) butterfly j2 = j1 % 16 + 16 q' combines the stage-1
+//     outputs q' of j1 = j2 % 16 and j2 % 16 + 16, the second twiddled by
+//     W_512^j2, and writes j2 and j2 + 256.
+// So a stage-0 output (j, q) is K slot r = j / R2 of stage-1 butterfly
+// j1 = q + 16 (j % R2).
+//
+// Strided tiles (16 adjacent columns; MFMA columns = the 16 columns):
+//   phase s = 0 .. R2-1: wave w runs the stage-0 butterflies
+//   j = s + R2 (4w + m), m = 0..3, so for each of its outputs q a lane holds
+//   K slots 4w .. 4w+3 of stage-1 butterfly q + 16 s in one column -- a
+//   16-byte hi and a 16-byte lo operand written with one store each; after a
+//   barrier wave w runs stage-1 butterflies 4w .. 4w+3 (+16 s) and either
+//   stores them (R2 = 1), keeps them (s = 0) or completes the radix-2 stage
+//   (s = 1).  The exchange buffer holds one phase: 32 KiB.
+// Contiguous lines (16 lines per workgroup): stage 0 runs with the data as
+//   the A operand (rows = 16 consecutive butterflies), so a lane gets output
+//   q = lane % 16 of butterflies 4g .. 4g+3, which are 4 (R2 = 1) or 2
+//   (R2 = 2) consecutive K slots of one stage-1 butterfly; stage 1 runs with
+//   columns = 16 consecutive butterflies j1 of a line, so the final stores
+//   cover 128-byte rows.  Each wave exchanges only its own lines (8 KiB of
+//   LDS per phase), so no workgroup barrier is needed.
+//
+// Addresses are 32-bit byte offsets from the buffer start (the launcher
+// checks the pass's extent fits), the wave-uniform part in the scalar
+// offset; loads and stores outside the extent are dropped by the buffer
+// range check.
+// ---------------------------------------------------------------------------
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+// Strided-tile variant: each load and store instruction covers the rows of 4
+// neighbouring butterflies (lane groups transposed in registers).
+constexpr int kMfRowAdj = 1 << 8;
+typedef int i32x2 __attribute__((ext_vector_type(2)));
+
+// Packed (a, b) rounded to bf16 (a in the low half).
+__device__ __forceinline__ uint32_t pk_bf16(float a, float b) {
+    typedef float f2 __attribute__((ext_vector_type(2)));
+    typedef __bf16 b2 __attribute__((ext_vector_type(2)));
+    const f2 v = {a, b};
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, b2));
+}
+// One complex value as packed bf16 hi = round(v) and lo = round(v - hi).
+__device__ __forceinline__ void split_c(float2 v, uint32_t& hi, uint32_t& lo) {
+    hi = pk_bf16(v.x, v.y);
+    lo = pk_bf16(v.x - __uint_as_float(hi << 16), v.y - __uint_as_float(hi & 0xffff0000u));
+}
+__device__ __forceinline__ void split4(const float2 (&x)[4], u32x4& h, u32x4& l) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        uint32_t a, b;
+        split_c(x[k], a, b);
+        h[k] = a;
+        l[k] = b;
+    }
+}
+__device__ __forceinline__ bf16x8 as_bf(u32x4 v) { return __builtin_bit_cast(bf16x8, v); }
+__device__ __forceinline__ f32x4 mma_bf(u32x4 a, u32x4 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(a), as_bf(b), c, 0, 0, 0);
+}
+
+// DFT-16 operand rows (real and imaginary output rows, hi and lo).
+struct F16Op {
+    u32x4 rh, rl, ih, il;
+};
+// Normal form: columns = butterflies (data = B operand); lane (c, g) gets
+// outputs 4g .. 4g+3 of column c.
+__device__ __forceinline__ void dft16_cols(const F16Op& F, u32x4 bh, u32x4 bl, f32x4& yr, f32x4& yi) {
+    const f32x4 z = {0.f, 0.f, 0.f, 0.f};
+    yr = mma_bf(F.rl, bh, z);
+    yi = mma_bf(F.il, bh, z);
+    yr = mma_bf(F.rh, bl, yr);
+    yi = mma_bf(F.ih, bl, yi);
+    yr = mma_bf(F.rh, bh, yr);
+    yi = mma_bf(F.ih, bh, yi);
+}
+// Transposed form: rows = butterflies (data = A operand); lane (c, g) gets
+// output c of rows (butterflies) 4g .. 4g+3.
+__device__ __forceinline__ void dft16_rows(const F16Op& F, u32x4 ah, u32x4 al, f32x4& yr, f32x4& yi) {
+    const f32x4 z = {0.f, 0.f, 0.f, 0.f};
+    yr = mma_bf(al, F.rh, z);
+    yi = mma_bf(al, F.ih, z);
+    yr = mma_bf(ah, F.rl, yr);
+    yi = mma_bf(ah, F.il, yi);
+    yr = mma_bf(ah, F.rh, yr);
+    yi = mma_bf(ah, F.ih, yi);
+}
+
+template <bool NTMP>
+__device__ __forceinline__ float2 bld(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    const i32x2 v = __builtin_amdgcn_raw_buffer_load_b64(r, int(voff), int(soff), NTMP ? 2 : 0);
+    return __builtin_bit_cast(float2, v);
+}
+template <bool NTMP>
+__device__ __forceinline__ void bst(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff, float2 v) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(i32x2, v), r, int(voff), int(soff), NTMP ? 2 : 0);
+}
+
+// Workgroup barrier that orders LDS accesses only: global loads and stores
+// in flight stay in flight across it.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// 4 x 4 transpose of 4-complex blocks across the lane groups (lane / 16) of
+// a wave: block b of lane group g moves to block g of lane group b.  Two
+// 2 x 2 block transposes: the high bits of group and block through
+// v_permlane32_swap (upper 32 lanes of the first operand <-> lower 32 lanes
+// of the second), then the low bits through v_permlane16_swap (odd rows of
+// the first operand <-> even rows of the second).
+__device__ __forceinline__ void swap_lanes32(float& x, float& y) {
+    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(x), __float_as_uint(y), false, false);
+    x = __uint_as_float(r[0]);
+    y = __uint_as_float(r[1]);
+}
+__device__ __forceinline__ void swap_lanes16(float& x, float& y) {
+    const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(x), __float_as_uint(y), false, false);
+    x = __uint_as_float(r[0]);
+    y = __uint_as_float(r[1]);
+}
+__device__ __forceinline__ void xpose4(float2 (&B)[4][4]) {
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            swap_lanes32(B[k][e].x, B[k + 2][e].x);
+            swap_lanes32(B[k][e].y, B[k + 2][e].y);
+        }
+#pragma unroll
+    for (int k = 0; k < 4; k += 2)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            swap_lanes16(B[k][e].x, B[k + 1][e].x);
+            swap_lanes16(B[k][e].y, B[k + 1][e].y);
+        }
+}
+
+template <int N, bool STRIDED, int FLAGS>
+__global__ __launch_bounds__(256) void mf2_kernel(float2* __restrict__ data, FftGeom g, const float2* __restrict__ tw,
+                                                  uint32_t extent) {
+    static_assert(N == 256 || N == 512, "mf2 lengths");
+    constexpr int R2 = N / 256;
+    constexpr int B0 = N / 16;
+    constexpr bool NTMP = (FLAGS & KF_NT) != 0;
+    constexpr bool A2MAJ = (FLAGS & KF_A2MAJ) != 0;
+    constexpr bool ROWADJ = (FLAGS & kMfRowAdj) != 0;
+    __shared__ u32x4 xh[1024];
+    __shared__ u32x4 xl[1024];
+    __shared__ float2 tw1[256];                 // W_256^(q r) at [16 q + r]
+    __shared__ float2 tw2[R2 > 1 ? 256 : 1];    // W_512^k, k < 256
+
+    const int tid = int(threadIdx.x);
+    const int lane = tid & 63, cn = lane & 15, gq = lane >> 4;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const uint32_t block = (FLAGS & KF_XCD) ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(data, 0, int(extent), 0x00020000);
+
+    tw1[tid] = tw[16 + tid];
+    if constexpr (R2 > 1) tw2[tid] = tw[16 + 256 + 2 * tid + 1];
+
+    F16Op F;
+    {
+        float2 fr[4], fi[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const float2 wv = tw[(cn * (4 * gq + e)) & 15];
+            fr[e] = make_float2(wv.x, -wv.y);
+            fi[e] = make_float2(wv.y, wv.x);
+        }
+        split4(fr, F.rh, F.rl);
+        split4(fi, F.ih, F.il);
+    }
+    // byte offsets of the K rows n = B0 (4 gq + e), and of the outputs 16 q'
+    uint32_t offr[4], offq[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        offr[e] = 8u * uint32_t(g.fft.off(uint32_t(B0 * (4 * gq + e))));
+        offq[e] = 8u * uint32_t(g.fft.off(uint32_t(16 * (4 * gq + e))));
+    }
+    const uint32_t off256 = R2 > 1 ? 8u * uint32_t(g.fft.off(256u)) : 0u;
+    const bool scaled = g.scale_mode != 0;
+    const float fs = g.scale_mode == 2 ? float(1.0 / g.scale) : float(g.scale);
+    auto sc = [&](float2 v) {
+        if (scaled) { v.x *= fs; v.y *= fs; }
+        return v;
+    };
+
+    if constexpr (STRIDED) {
+        bool valid;
+        const uint32_t lb = 8u * uint32_t(line_base(g, true, 16, 1, block, cn, &valid, A2MAJ));
+        float2 v[R2][4][4];
+        if constexpr (ROWADJ) {
+            // lane group gq loads all 16 K rows of butterfly m = gq (rows
+            // j + B0 r, so one load covers the rows of 4 neighbouring
+            // butterflies), then the 4 x 4 block transpose hands each lane
+            // group its K block of every butterfly
+#pragma unroll
+            for (int s = 0; s < R2; ++s) {
+                const uint32_t vj = lb + 8u * uint32_t(g.fft.off(uint32_t(s + R2 * (4 * w + gq))));
+#pragma unroll
+                for (int r = 0; r < 16; ++r)
+                    v[s][r >> 2][r & 3] = bld<NTMP>(rs, vj, 8u * uint32_t(g.fft.off(uint32_t(B0 * r))));
+            }
+        } else {
+#pragma unroll
+            for (int s = 0; s < R2; ++s)
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    const uint32_t oj = 8u * uint32_t(g.fft.off(uint32_t(s + R2 * (4 * w + m))));
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) v[s][m][e] = bld<NTMP>(rs, lb + offr[e], oj);
+                }
+        }
+        lds_barrier();   // twiddles in LDS
+        // row-adjacent stores: lane group g' stores butterfly 4w + g'
+        const uint32_t vq = ROWADJ ? lb + 8u * uint32_t(g.fft.off(uint32_t(4 * w + gq))) : 0u;
+        float2 y0[4][4];
+#pragma unroll
+        for (int s = 0; s < R2; ++s) {
+            if constexpr (ROWADJ) xpose4(v[s]);
+            f32x4 yr[4], yi[4];
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                u32x4 bh, bl;
+                split4(v[s][m], bh, bl);
+                dft16_cols(F, bh, bl, yr[m], yi[m]);
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int q = 4 * gq + i;
+                float2 c[4];
+#pragma unroll
+                for (int m = 0; m < 4; ++m) c[m] = c_mul(make_float2(yr[m][i], yi[m][i]), tw1[16 * q + 4 * w + m]);
+                u32x4 h, l;
+                split4(c, h, l);
+                xh[(4 * q + w) * 16 + cn] = h;
+                xl[(4 * q + w) * 16 + cn] = l;
+            }
+            lds_barrier();
+            float2 o[2][4][4];   // outputs to store: [half][butterfly a][i]
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                const int q1 = 4 * w + a;
+                f32x4 zr, zi;
+                dft16_cols(F, xh[(4 * q1 + gq) * 16 + cn], xl[(4 * q1 + gq) * 16 + cn], zr, zi);
+                if constexpr (R2 == 1) {
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) o[0][a][i] = sc(make_float2(zr[i], zi[i]));
+                } else if (s == 0) {
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) y0[a][i] = make_float2(zr[i], zi[i]);
+                } else {
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const float2 t = c_mul(make_float2(zr[i], zi[i]), tw2[q1 + 16 * (4 * gq + i)]);
+                        o[0][a][i] = sc(c_add(y0[a][i], t));
+                        o[1][a][i] = sc(c_sub(y0[a][i], t));
+                    }
+                }
+            }
+            if (R2 == 1 || s == 1) {
+#pragma unroll
+                for (int h = 0; h < R2; ++h) {
+                    if constexpr (ROWADJ) {
+                        // lane group g' gets outputs 4k + i of butterfly 4w + g' in block k
+                        xpose4(o[h]);
+#pragma unroll
+                        for (int k = 0; k < 4; ++k)
+#pragma unroll
+                            for (int i = 0; i < 4; ++i)
+                                bst<NTMP>(rs, vq, 8u * uint32_t(g.fft.off(uint32_t(16 * (4 * k + i) + 256 * h))),
+                                          o[h][k][i]);
+                    } else {
+#pragma unroll
+                        for (int a = 0; a < 4; ++a) {
+                            const uint32_t oq1 = 8u * uint32_t(g.fft.off(uint32_t(4 * w + a))) + (h ? off256 : 0u);
+#pragma unroll
+                            for (int i = 0; i < 4; ++i) bst<NTMP>(rs, lb + offq[i], oq1, o[h][a][i]);
+                        }
+                    }
+                }
+            }
+            if (s + 1 < R2) lds_barrier();
+        }
+    } else {
+        constexpr int LP = 16 / R2;   // lines per phase
+        constexpr int LW = LP / 4;    // lines per wave and phase
+        const uint32_t ocn = 8u * uint32_t(g.fft.off(uint32_t(cn)));
+        uint32_t vo[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) vo[e] = ocn + offr[e];
+        // line bases (wave-uniform) and validity of the wave's lines
+        uint32_t lbase[R2][LW];
+        bool lval[R2][LW];
+#pragma unroll
+        for (int ph = 0; ph < R2; ++ph)
+#pragma unroll
+            for (int la = 0; la < LW; ++la) {
+                bool ok;
+                const int64_t b = line_base(g, false, 16, 1, block, ph * LP + w * LW + la, &ok, false);
+                lbase[ph][la] = 8u * uint32_t(ok ? b : 0);
+                lval[ph][la] = ok;
+            }
+        float2 v[R2][LW][R2][4];
+#pragma unroll
+        for (int ph = 0; ph < R2; ++ph)
+#pragma unroll
+            for (int la = 0; la < LW; ++la)
+#pragma unroll
+                for (int jh = 0; jh < R2; ++jh) {
+                    const uint32_t so = lbase[ph][la] + 8u * uint32_t(g.fft.off(uint32_t(16 * jh)));
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)
+                        v[ph][la][jh][e] = lval[ph][la] ? bld<NTMP>(rs, vo[e], so) : make_float2(0.f, 0.f);
+                }
+        lds_barrier();   // twiddles in LDS
+        u32x4* const wh = xh + w * 256;   // this wave's exchange region
+        u32x4* const wl = xl + w * 256;
+#pragma unroll
+        for (int ph = 0; ph < R2; ++ph) {
+            // stage 0: rows = butterflies j = 16 jh + row of line la
+#pragma unroll
+            for (int la = 0; la < LW; ++la)
+#pragma unroll
+                for (int jh = 0; jh < R2; ++jh) {
+                    u32x4 ah, al;
+                    split4(v[ph][la][jh], ah, al);
+                    f32x4 yr, yi;
+                    dft16_rows(F, ah, al, yr, yi);
+                    // lane holds output q = cn of butterflies j = 16 jh + 4 gq + i
+                    float2 c[4];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const int r = (16 * jh + 4 * gq + i) / R2;   // K slot in the stage-1 butterfly
+                        c[i] = c_mul(make_float2(yr[i], yi[i]), tw1[16 * cn + r]);
+                    }
+                    if constexpr (R2 == 1) {
+                        u32x4 h, l;
+                        split4(c, h, l);
+                        wh[(la * 4 + gq) * 16 + cn] = h;
+                        wl[(la * 4 + gq) * 16 + cn] = l;
+                    } else {
+#pragma unroll
+                        for (int s = 0; s < 2; ++s) {   // outputs i = s, s + 2: K slots 8 jh + 2 gq + {0, 1}
+                            uint32_t h0, l0, h1, l1;
+                            split_c(c[s], h0, l0);
+                            split_c(c[s + 2], h1, l1);
+                            const int slot = ((la * 2 + s) * 4 + 2 * jh + (gq >> 1)) * 16 + cn;
+                            u32x2* const ph2 = reinterpret_cast<u32x2*>(wh + slot) + (gq & 1);
+                            u32x2* const pl2 = reinterpret_cast<u32x2*>(wl + slot) + (gq & 1);
+                            *ph2 = u32x2{h0, h1};
+                            *pl2 = u32x2{l0, l1};
+                        }
+                    }
+                }
+            // the wave reads back only what it wrote: LDS accesses of one
+            // wavefront complete in order
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+            for (int la = 0; la < LW; ++la) {
+                if (!lval[ph][la]) continue;
+                const uint32_t so = lbase[ph][la];
+                f32x4 zr[R2], zi[R2];
+#pragma unroll
+                for (int s = 0; s < R2; ++s) {
+                    const int slot = ((la * R2 + s) * 4 + gq) * 16 + cn;
+                    dft16_cols(F, wh[slot], wl[slot], zr[s], zi[s]);
+                }
+                // lane holds outputs q' = 4 gq + i of butterflies j1 = cn + 16 s
+                if constexpr (R2 == 1) {
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) bst<NTMP>(rs, ocn + offq[i], so, sc(make_float2(zr[0][i], zi[0][i])));
+                } else {
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const float2 y0 = make_float2(zr[0][i], zi[0][i]);
+                        const float2 t = c_mul(make_float2(zr[1][i], zi[1][i]), tw2[cn + 16 * (4 * gq + i)]);
+                        bst<NTMP>(rs, ocn + offq[i], so, sc(c_add(y0, t)));
+                        bst<NTMP>(rs, ocn + offq[i], so + off256, sc(c_sub(y0, t)));
+                    }
+                }
+            }
+            if (ph + 1 < R2) {   // the next phase overwrites this wave's region
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Registry and launch.
 // ---------------------------------------------------------------------------
 namespace {
@@ -551,6 +963,59 @@ const std::vector<MfEntry>& mf_registry() {
         return v;
     }();
     return reg;
+}
+
+// The interleaved-K split-BF16 kernel (mf2_kernel): variants by tile order
+// and cache policy, each also launched with extra dynamic LDS that limits a
+// CU to 3 (kMfOcc3) or 2 (kMfOcc2) workgroups.
+constexpr int kMfOcc3 = 1 << 9;
+constexpr int kMfOcc2 = 1 << 10;
+
+struct Mf2Entry {
+    int N, strided, flags;
+    const void* fn;
+};
+
+template <int N, bool S, int F>
+void mf2_add(std::vector<Mf2Entry>& v) {
+    v.push_back(Mf2Entry{N, S, F, (const void*)&mf2_kernel<N, S, F>});
+}
+
+template <int N>
+void mf2_add_len(std::vector<Mf2Entry>& v) {
+    mf2_add<N, false, KF_XCD | KF_NT>(v);
+    mf2_add<N, false, KF_NT>(v);
+    mf2_add<N, false, KF_XCD>(v);
+    mf2_add<N, true, KF_XCD | KF_NT>(v);
+    mf2_add<N, true, KF_NT>(v);
+    mf2_add<N, true, KF_A2MAJ | KF_NT>(v);
+    mf2_add<N, true, KF_XCD | KF_A2MAJ | KF_NT>(v);
+    mf2_add<N, true, KF_XCD>(v);
+    mf2_add<N, true, KF_A2MAJ>(v);
+    mf2_add<N, true, KF_XCD | KF_A2MAJ>(v);
+    mf2_add<N, true, kMfRowAdj | KF_XCD | KF_NT>(v);
+    mf2_add<N, true, kMfRowAdj | KF_XCD | KF_A2MAJ | KF_NT>(v);
+    mf2_add<N, true, kMfRowAdj | KF_A2MAJ>(v);
+    mf2_add<N, true, kMfRowAdj | KF_A2MAJ | KF_NT>(v);
+    mf2_add<N, true, kMfRowAdj | KF_NT>(v);
+    mf2_add<N, true, kMfRowAdj>(v);
+    mf2_add<N, true, kMfRowAdj | KF_XCD>(v);
+}
+
+const std::vector<Mf2Entry>& mf2_registry() {
+    static const std::vector<Mf2Entry> reg = [] {
+        std::vector<Mf2Entry> v;
+        mf2_add_len<256>(v);
+        mf2_add_len<512>(v);
+        return v;
+    }();
+    return reg;
+}
+
+const void* mf2_find(int N, int strided, int flags) {
+    for (const Mf2Entry& e : mf2_registry())
+        if (e.N == N && e.strided == (strided != 0) && e.flags == flags) return e.fn;
+    return nullptr;
 }
 
 struct MfKey {
@@ -615,9 +1080,80 @@ int mfma_pass_check(const AxisPass& p, int N) {
     return A2AFFT_OK;
 }
 
-int launch_mfma_pass(const AxisPass& p, void* data, int N, int dir, a2afft_precision prec, hipStream_t s) {
+// One past the largest byte offset a pass touches over its whole axes (all
+// strides are positive).
+static uint64_t pass_extent(const AxisPass& p, int N) {
+    return 8ull * uint64_t(p.a1.off(p.a1.n - 1) + p.a2.off(p.a2.n - 1) + p.fft.off(uint32_t(N - 1))) + 8;
+}
+
+// The interleaved-K kernel runs split-BF16 passes of 256 and 512 points whose
+// byte offsets fit 32 bits, in 16-line tiles.
+static bool mf2_fits(const AxisPass& p, int N, a2afft_precision prec) {
+    if (prec != A2AFFT_SPLIT_BF16 || (N != 256 && N != 512)) return false;
+    if (pass_extent(p, N) >= (1ull << 32)) return false;
+    FftGeom g;
+    uint32_t grid = 0;
+    return resolve_with(p.fft, p.a1, p.a2, p.strided, N, 16, N, &g, &grid) == A2AFFT_OK;
+}
+
+// Kernel choices of a split pass: T = tile lines; E = 0 for the interleaved-K
+// kernel with variant flags, E = N for the kernels with split K halves.
+int mfma_candidates(const AxisPass& p, int N, a2afft_precision prec, KSel* out, int max) {
+    int n = 0;
+    if (mf2_fits(p, N, prec))
+        for (const Mf2Entry& e : mf2_registry())
+            for (int occ : {0, kMfOcc3, kMfOcc2})
+                if (e.N == N && e.strided == (p.strided != 0) && n < max) out[n++] = KSel{16, 0, e.flags | occ};
+    if (n < max && mfma_pass_check(p, N) == A2AFFT_OK) out[n++] = KSel{mf_T(N, p.strided != 0), N, 0};
+    return n;
+}
+
+// Defaults of the interleaved-K kernel, measured on 512^3 complex64 over dims
+// [1,4,4] (ms per pass, whole passes unless noted):
+//   contiguous (z): XCD order + non-temporal 0.376-0.380 (hardware order 0.408);
+//   far-strided (x, rows 128 KiB apart): row-adjacent lanes + z tiles
+//     outermost 0.448-0.452 (XCD + non-temporal without row-adjacent lanes
+//     0.502-0.528, with them 0.547);
+//   near-strided (y): XCD + non-temporal 0.379-0.394 whole; in 128-column
+//     windows alternating with x, plain non-temporal order gives the fastest
+//     forward (1.104-1.110 ms against 1.126-1.131 with XCD order).
+// `windowed`: the pass runs in windows of z columns (the plan's chunking).
+KSel mfma_default_sel(const AxisPass& p, int N, a2afft_precision prec, bool windowed) {
+    if (mf2_fits(p, N, prec)) {
+        if (!p.strided) return KSel{16, 0, KF_XCD | KF_NT};
+        if (p.fft.s_elem >= 4096) return KSel{16, 0, kMfRowAdj | KF_A2MAJ};
+        return KSel{16, 0, windowed ? KF_NT : KF_XCD | KF_NT};
+    }
+    return KSel{mf_T(N, p.strided != 0), N, 0};
+}
+
+int launch_mfma_pass(const AxisPass& p, void* data, int N, int dir, a2afft_precision prec, hipStream_t s,
+                     const KSel* sel) {
     if (!data) return set_error(A2AFFT_E_INVALID_ARG, "fft: null data");
     if (!mf_length_ok(N)) return set_error(A2AFFT_E_UNSUPPORTED, "split precision: length %d (16..512)", N);
+    const KSel ks = sel ? *sel : mfma_default_sel(p, N, prec, false);
+    const void* tw = mfma_twiddle_table(N, dir);
+    if (!tw) return set_error(A2AFFT_E_ALLOC, "split precision: twiddle table N=%d", N);
+    float2* dp = (float2*)data;
+    if (ks.E == 0) {   // interleaved-K kernel
+        if (!mf2_fits(p, N, prec)) return set_error(A2AFFT_E_UNSUPPORTED, "split precision: no 16-line tile kernel");
+        const void* fn = mf2_find(N, p.strided, ks.flags & ~(kMfOcc3 | kMfOcc2));
+        if (!fn) return set_error(A2AFFT_E_UNSUPPORTED, "split precision: no variant flags=%d", ks.flags);
+        // occupancy variants: extra dynamic LDS leaves room for 3 or 2 workgroups per CU
+        const size_t dyn = (ks.flags & kMfOcc2) ? size_t(28 << 10) : (ks.flags & kMfOcc3) ? size_t(16 << 10) : 0;
+        FftGeom g;
+        uint32_t grid = 0;
+        int st = resolve_with(p.fft, p.a1, p.a2, p.strided, N, 16, N, &g, &grid, p.a2_start, p.a2_count);
+        if (st < 0) return set_error(A2AFFT_E_UNSUPPORTED, "split precision: geometry (N=%d)", N);
+        g.scale_mode = p.scale_mode;
+        g.scale = p.scale;
+        if (grid == 0) return A2AFFT_OK;
+        uint32_t extent = uint32_t(pass_extent(p, N));
+        void* args[] = {&dp, &g, (void*)&tw, &extent};
+        st = hip_check(int(hipLaunchKernel(fn, dim3(grid), dim3(256), args, dyn, s)), "mfma fft launch");
+        if (st < 0) return st;
+        return hip_check(int(hipGetLastError()), "mfma fft launch");
+    }
     const int T = mf_T(N, p.strided != 0);
     FftGeom g;
     uint32_t grid = 0;
@@ -626,13 +1162,9 @@ int launch_mfma_pass(const AxisPass& p, void* data, int N, int dir, a2afft_preci
     g.scale_mode = p.scale_mode;
     g.scale = p.scale;
     if (grid == 0) return A2AFFT_OK;
-    const void* tw = mfma_twiddle_table(N, dir);
-    if (!tw) return set_error(A2AFFT_E_ALLOC, "split precision: twiddle table N=%d", N);
     const int f16 = prec == A2AFFT_SPLIT_FP16 ? 1 : 0;
     const int want_nw = 8;
-    // largest byte offset the pass touches (all strides are positive)
-    const uint64_t last = 8ull * uint64_t(g.a1.off(g.a1.n - 1) + g.a2.off(g.a2.n - 1) + g.fft.off(uint32_t(N - 1))) + 8;
-    const int want_o32 = last < (1ull << 32) ? 1 : 0;
+    const int want_o32 = pass_extent(p, N) < (1ull << 32) ? 1 : 0;
     const void* fn = nullptr;
     int nw = 4;
     for (const MfEntry& e : mf_registry())
@@ -642,7 +1174,6 @@ int launch_mfma_pass(const AxisPass& p, void* data, int N, int dir, a2afft_preci
             nw = e.nw;
         }
     if (!fn) return set_error(A2AFFT_E_UNSUPPORTED, "split precision: no kernel for N=%d", N);
-    float2* dp = (float2*)data;
     void* args[] = {&dp, &g, (void*)&tw};
     st = hip_check(int(hipLaunchKernel(fn, dim3(grid), dim3(64 * nw), args, 0, s)), "mfma fft launch");
     if (st < 0) return st;

@@ -409,12 +409,14 @@ def test_halfx_variants_bit_identical(a2afft_mod, cuda, case):
 
 @pytest.mark.parametrize("case", [((512, 16, (1, 4, 4), "c64", "fp32"), 128), ((512, 16, (1, 4, 4), "c128", "fp64"), 64),
                                   ((256, 8, (1, 2, 4), "c128", "fp64"), 0), ((256, 8, (1, 2, 4), "c64", "fp32"), 0),
-                                  ((512, 16, (1, 4, 4), "c64", "split_bf16"), 0)],
+                                  ((512, 16, (1, 4, 4), "c64", "split_bf16"), 128),
+                                  ((512, 16, (1, 4, 4), "c64", "split_fp16"), 0)],
                          ids=lambda c: str(c))
 def test_default_chunk_width(a2afft_mod, cuda, case):
     """New plans alternate the x and y passes over windows of at most 256 MiB
-    only for grids above 512 MiB on the 512-point half-width-exchange tiles;
-    every other plan runs whole passes."""
+    only for grids above 512 MiB on the 512-point half-width-exchange tiles
+    or the interleaved-K split-BF16 kernel; every other plan runs whole
+    passes."""
     (Ng, P, dims, dtype, prec), width = case
     dist = a2afft_mod.Distribution(Ng, P, dims=dims, dtype=dtype)
     plan = a2afft_mod.Dfft(dist, precision=prec)
@@ -570,3 +572,21 @@ def test_direct_dft_in_plan_passes(a2afft_mod, cuda, case, prec):
     ref = np.moveaxis(O.fft_batched(np.moveaxis(x.astype(np.complex128), axis, -1), -1), -1, axis)
     tol = TOL_1D["c128" if prec == "fp64" else "c64"] * np.sqrt(N)
     assert _relerr(d.cpu().numpy(), ref) < tol
+
+
+def test_many_windows_match_whole_passes(a2afft_mod, cuda):
+    """16 x 16 x 4096 complex128 in 8-column windows: 512 windows, 1025
+    launches per transform, every window's x and y passes run (bit-identical
+    to whole passes)."""
+    import torch
+    shape = (16, 16, 4096)
+    x = torch.empty(shape, dtype=torch.complex128, device="cuda")
+    a2afft_mod.fill_random(x, 23, "uniform")
+    whole = a2afft_mod.GridDfft(shape, "fp64")
+    win = a2afft_mod.GridDfft(shape, "fp64")
+    win.set_chunking(8)
+    assert win.num_launches() == 1 + 2 * 512
+    a, b = x.clone(), x.clone()
+    whole.forward(a)
+    win.forward(b)
+    assert torch.equal(torch.view_as_real(a), torch.view_as_real(b))

@@ -13,7 +13,8 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 TOL = {"split_bf16": 1e-5, "split_fp16": 1e-6}
-GRIDS = [(64, 64, 64), (32, 32, 32), (128, 32, 256), (512, 64, 32), (16, 256, 64), (32, 128, 64)]
+GRIDS = [(64, 64, 64), (32, 32, 32), (128, 32, 256), (512, 64, 32), (16, 256, 64), (32, 128, 64),
+         (64, 32, 512), (32, 512, 64), (16, 64, 512)]
 
 
 def _rand(torch, shape, seed):
@@ -103,3 +104,56 @@ def test_split_rejects_staged_and_short_lines(a2afft_mod, cuda):
         a2afft_mod.Dfft(dist, "split_bf16", exec="staged")
     with pytest.raises(L.A2AFFTError):
         a2afft_mod.GridDfft((8, 64, 64), "split_bf16")
+
+
+# interleaved-K split-BF16 variants (T=16, E=0): contiguous and strided flags
+_MF2_FLAGS = {0: [5, 4, 1], 1: [5, 4, 12, 13, 1, 8, 9, 261, 269, 264, 257], 2: [5, 4, 12, 13, 1, 261, 269]}
+
+
+def test_split_bf16_variants_same_bits(a2afft_mod, cuda):
+    """Every tile order and cache policy of the interleaved-K kernel computes
+    the same bits on every pass; the split-K-halves kernel (E = N) stays
+    within the BF16 bound of FP64."""
+    import torch
+    shape = (256, 512, 256)
+    x, d0 = _rand(torch, shape, 21)
+    ref = np.fft.fftn(x.astype(np.complex128))
+    base = a2afft_mod.GridDfft(shape, "split_bf16")
+    want = d0.clone()
+    base.forward(want)
+    assert _rel(want.cpu().numpy().reshape(shape), ref) <= TOL["split_bf16"]
+    for p in range(3):
+        for fl in _MF2_FLAGS[p]:
+            plan = a2afft_mod.GridDfft(shape, "split_bf16")
+            plan.set_kernel(p, 16, 0, fl)
+            d = d0.clone()
+            plan.forward(d)
+            assert torch.equal(torch.view_as_real(d), torch.view_as_real(want)), (p, fl)
+        n = shape[[2, 0, 1][p]]
+        plan = a2afft_mod.GridDfft(shape, "split_bf16")
+        plan.set_kernel(p, 16, n, 0)
+        d = d0.clone()
+        plan.forward(d)
+        err = _rel(d.cpu().numpy().reshape(shape), ref)
+        assert err <= TOL["split_bf16"], (p, err)
+
+
+def test_split_bf16_rank_segments_256(a2afft_mod, cuda):
+    """256^3 complex64 over dims [1,2,4]: the y and z lines run across rank
+    blocks; stitched output equals numpy's fftn within the BF16 bound, and the
+    inverse returns the input."""
+    import torch
+    from a2afft import inputs
+    Ng, P, dims = 256, 8, (1, 2, 4)
+    dist = a2afft_mod.Distribution(Ng, P, dims=dims, dtype="c64")
+    buf = dist.empty()
+    a2afft_mod.fill_random(buf, 9, "uniform")
+    x = buf.cpu().numpy()
+    plan = a2afft_mod.Dfft(dist, "split_bf16")
+    plan.forward(buf)
+    got = inputs.blocks_to_global(buf.cpu().numpy(), dims, (Ng,) * 3)
+    ref = np.fft.fftn(inputs.blocks_to_global(x, dims, (Ng,) * 3).astype(np.complex128))
+    assert _rel(got, ref) <= TOL["split_bf16"]
+    del got, ref
+    plan.inverse(buf)
+    assert _rel(buf.cpu().numpy(), x.astype(np.complex128)) <= 2 * TOL["split_bf16"]
