@@ -701,10 +701,12 @@ __global__ __launch_bounds__(256) void mf2_kernel(float2* __restrict__ data, Fft
         offq[e] = 8u * uint32_t(g.fft.off(uint32_t(16 * (4 * gq + e))));
     }
     const uint32_t off256 = R2 > 1 ? 8u * uint32_t(g.fft.off(256u)) : 0u;
-    const bool scaled = g.scale_mode != 0;
-    const float fs = g.scale_mode == 2 ? float(1.0 / g.scale) : float(g.scale);
+    // the inverse's 1/(nx*ny*nz) as one multiplication (exact for the
+    // power-of-two lengths these kernels take); 1 when the pass does not scale
+    const float fs = g.scale_mode == 0 ? 1.f : g.scale_mode == 2 ? float(1.0 / g.scale) : float(g.scale);
     auto sc = [&](float2 v) {
-        if (scaled) { v.x *= fs; v.y *= fs; }
+        v.x *= fs;
+        v.y *= fs;
         return v;
     };
 
