@@ -410,7 +410,7 @@ static int plan_init(a2afft_plan_s* p) {
     for (int i = 0; i < 3; ++i) {
         const int N = int(p->pass[i].fft.n);
         if (is_split(p->prec)) {
-            A2A_TRY(mfma_pass_check(p->pass[i], N));
+            A2A_TRY(mfma_pass_check(p->pass[i], N, p->prec));
             p->sel[i] = mfma_default_sel(p->pass[i], N, p->prec, false);
             A2A_TRY(sel_tables(p, i, p->sel[i]));
             continue;

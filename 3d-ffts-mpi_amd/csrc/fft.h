@@ -95,7 +95,7 @@ KSel fft_default_sel(const AxisPass& pass, a2afft_dtype dtype, int N);
 
 // Split-precision passes (radix-16 stages on MFMA, fft_mfma.hip).
 // mfma_pass_check: E_UNSUPPORTED unless the pass can run split.
-int mfma_pass_check(const AxisPass& pass, int N);
+int mfma_pass_check(const AxisPass& pass, int N, a2afft_precision prec);
 // sel: T = tile lines, E = 0 for the interleaved-K split-BF16 kernel (flags =
 // its variant), E = N for the split-K-halves kernels; null = the default.
 int launch_mfma_pass(const AxisPass& pass, void* data, int N, int dir, a2afft_precision prec, hipStream_t s,
@@ -103,7 +103,6 @@ int launch_mfma_pass(const AxisPass& pass, void* data, int N, int dir, a2afft_pr
 int mfma_candidates(const AxisPass& pass, int N, a2afft_precision prec, KSel* out, int max);
 KSel mfma_default_sel(const AxisPass& pass, int N, a2afft_precision prec, bool windowed);
 const void* mfma_twiddle_table(int N, int dir);
-int mf_tile_T(int N, int strided);
 
 // exp(dir*2*pi*i*m/M) in long double (quadrant-reduced), for table building.
 void twiddle_ld(int64_t m, int64_t M, int dir, long double* re, long double* im);

@@ -916,6 +916,129 @@ __global__ __launch_bounds__(256) void mf2_kernel(float2* __restrict__ data, Fft
     }
 }
 
+// Contiguous lines of 1024 = 16 * 16 * 4 and 2048 = 16 * 16 * 8 points,
+// split-BF16: the two radix-16 stages as in mf2_kernel's contiguous form, then
+// a radix-R2 stage (R2 = N / 256) on VALU.  A workgroup takes 8192 / N lines
+// and each wave 8 / R2 whole lines, exchanged through its own 16 KiB of LDS.
+// Stage-0 output q of butterfly j is K slot j / R2 of stage-1 butterfly
+// q + 16 (j % R2); stage-1 output q' of butterflies j2 % 16 + 16 s (s < R2)
+// are the R2 inputs of radix-R2 butterfly j2 = j1 % 16 + 16 q', twiddled by
+// W_N^(s j2), written to j2 + 256 q2.
+template <int N, int DIR, int FLAGS>
+__global__ __launch_bounds__(256) void mf2_long_kernel(float2* __restrict__ data, FftGeom g,
+                                                       const float2* __restrict__ tw, uint32_t extent) {
+    static_assert(N == 1024 || N == 2048, "long lengths");
+    constexpr int R2 = N / 256;
+    constexpr int B0 = N / 16;
+    constexpr int LW = 8 / R2;    // lines per wave
+    constexpr int TL = 4 * LW;    // lines per workgroup
+    constexpr bool NTMP = (FLAGS & KF_NT) != 0;
+    __shared__ u32x4 xh[2048];
+    __shared__ u32x4 xl[2048];
+    __shared__ float2 tw1[256];
+
+    const int tid = int(threadIdx.x);
+    const int lane = tid & 63, cn = lane & 15, gq = lane >> 4;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const uint32_t block = (FLAGS & KF_XCD) ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(data, 0, int(extent), 0x00020000);
+    tw1[tid] = tw[16 + tid];
+
+    F16Op F;
+    {
+        float2 fr[4], fi[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const float2 wv = tw[(cn * (4 * gq + e)) & 15];
+            fr[e] = make_float2(wv.x, -wv.y);
+            fi[e] = make_float2(wv.y, wv.x);
+        }
+        split4(fr, F.rh, F.rl);
+        split4(fi, F.ih, F.il);
+    }
+    const uint32_t ocn = 8u * uint32_t(g.fft.off(uint32_t(cn)));
+    uint32_t vo[4], offq[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        vo[e] = ocn + 8u * uint32_t(g.fft.off(uint32_t(B0 * (4 * gq + e))));
+        offq[e] = 8u * uint32_t(g.fft.off(uint32_t(16 * (4 * gq + e))));
+    }
+    const float fs = g.scale_mode == 0 ? 1.f : g.scale_mode == 2 ? float(1.0 / g.scale) : float(g.scale);
+
+    uint32_t lbase[LW];
+    bool lval[LW];
+#pragma unroll
+    for (int la = 0; la < LW; ++la) {
+        bool ok;
+        const int64_t b = line_base(g, false, TL, 1, block, w * LW + la, &ok, false);
+        lbase[la] = 8u * uint32_t(ok ? b : 0);
+        lval[la] = ok;
+    }
+    float2 v[LW][R2][4];
+#pragma unroll
+    for (int la = 0; la < LW; ++la)
+#pragma unroll
+        for (int jh = 0; jh < R2; ++jh) {
+            const uint32_t so = lbase[la] + 8u * uint32_t(g.fft.off(uint32_t(16 * jh)));
+#pragma unroll
+            for (int e = 0; e < 4; ++e) v[la][jh][e] = lval[la] ? bld<NTMP>(rs, vo[e], so) : make_float2(0.f, 0.f);
+        }
+    lds_barrier();   // twiddles in LDS
+    uint32_t* const wh = reinterpret_cast<uint32_t*>(xh + w * 512);
+    uint32_t* const wl = reinterpret_cast<uint32_t*>(xl + w * 512);
+#pragma unroll
+    for (int la = 0; la < LW; ++la) {
+        // stage 0: rows = butterflies j = 16 jh + row; the lane holds output
+        // q = cn of butterflies 16 jh + 4 gq + i
+#pragma unroll
+        for (int jh = 0; jh < R2; ++jh) {
+            u32x4 ah, al;
+            split4(v[la][jh], ah, al);
+            f32x4 yr, yi;
+            dft16_rows(F, ah, al, yr, yi);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int j = 16 * jh + 4 * gq + i;
+                const int s = j % R2, r = j / R2;
+                uint32_t h, l;
+                split_c(c_mul(make_float2(yr[i], yi[i]), tw1[16 * cn + r]), h, l);
+                const int slot = ((la * R2 + s) * 4 + (r >> 2)) * 16 + cn;
+                wh[4 * slot + (r & 3)] = h;
+                wl[4 * slot + (r & 3)] = l;
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // stage 1: butterflies j1 = cn + 16 s; the lane gets outputs q' = 4 gq + i
+        float2 z[R2][4];
+#pragma unroll
+        for (int s = 0; s < R2; ++s) {
+            const int slot = ((la * R2 + s) * 4 + gq) * 16 + cn;
+            f32x4 zr, zi;
+            dft16_cols(F, xh[w * 512 + slot], xl[w * 512 + slot], zr, zi);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) z[s][i] = make_float2(zr[i], zi[i]);
+        }
+        // radix-R2 stage and stores
+        if (lval[la]) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int j2 = cn + 16 * (4 * gq + i);
+                float2 x[R2];
+                x[0] = z[0][i];
+#pragma unroll
+                for (int s = 1; s < R2; ++s) x[s] = c_mul(z[s][i], tw[16 + 256 + j2 * R2 + s]);
+                dft<R2, DIR>(x);
+#pragma unroll
+                for (int q2 = 0; q2 < R2; ++q2)
+                    bst<NTMP>(rs, ocn + offq[i], lbase[la] + 8u * uint32_t(g.fft.off(uint32_t(256 * q2))),
+                              make_float2(x[q2].x * fs, x[q2].y * fs));
+            }
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Registry and launch.
 // ---------------------------------------------------------------------------
@@ -1004,6 +1127,18 @@ void mf2_add_len(std::vector<Mf2Entry>& v) {
     mf2_add<N, true, kMfRowAdj | KF_XCD>(v);
 }
 
+// Long contiguous lines: one entry per direction (the radix-R2 stage's sign).
+struct Mf2LongEntry {
+    int N, dir, flags;
+    const void* fn;
+};
+
+template <int N, int F>
+void mf2_long_add(std::vector<Mf2LongEntry>& v) {
+    v.push_back(Mf2LongEntry{N, -1, F, (const void*)&mf2_long_kernel<N, -1, F>});
+    v.push_back(Mf2LongEntry{N, 1, F, (const void*)&mf2_long_kernel<N, 1, F>});
+}
+
 const std::vector<Mf2Entry>& mf2_registry() {
     static const std::vector<Mf2Entry> reg = [] {
         std::vector<Mf2Entry> v;
@@ -1014,11 +1149,43 @@ const std::vector<Mf2Entry>& mf2_registry() {
     return reg;
 }
 
-const void* mf2_find(int N, int strided, int flags) {
+const std::vector<Mf2LongEntry>& mf2_long_registry() {
+    static const std::vector<Mf2LongEntry> reg = [] {
+        std::vector<Mf2LongEntry> v;
+        mf2_long_add<1024, KF_XCD | KF_NT>(v);
+        mf2_long_add<1024, KF_NT>(v);
+        mf2_long_add<2048, KF_XCD | KF_NT>(v);
+        mf2_long_add<2048, KF_NT>(v);
+        return v;
+    }();
+    return reg;
+}
+
+// flags of the built variants of length N (any direction)
+void mf2_flags(int N, int strided, std::vector<int>* out) {
+    if (N >= 1024) {
+        if (strided) return;
+        for (const Mf2LongEntry& e : mf2_long_registry())
+            if (e.N == N && e.dir == -1) out->push_back(e.flags);
+        return;
+    }
+    for (const Mf2Entry& e : mf2_registry())
+        if (e.N == N && e.strided == (strided != 0)) out->push_back(e.flags);
+}
+
+const void* mf2_find(int N, int strided, int flags, int dir) {
+    if (N >= 1024) {
+        for (const Mf2LongEntry& e : mf2_long_registry())
+            if (e.N == N && !strided && e.flags == flags && e.dir == (dir < 0 ? -1 : 1)) return e.fn;
+        return nullptr;
+    }
     for (const Mf2Entry& e : mf2_registry())
         if (e.N == N && e.strided == (strided != 0) && e.flags == flags) return e.fn;
     return nullptr;
 }
+
+// Lines per workgroup of the interleaved-K kernels.
+int mf2_tile_lines(int N) { return N <= 512 ? 16 : 8192 / N; }
 
 struct MfKey {
     int dev, N, dir;
@@ -1033,7 +1200,6 @@ std::map<MfKey, void*> g_mf_tw;
 
 }  // namespace
 
-int mf_tile_T(int N, int strided) { return mf_T(N, strided != 0); }
 
 // Twiddles of the MFMA schedule (FP32, rounded once from long double).
 const void* mfma_twiddle_table(int N, int dir) {
@@ -1068,18 +1234,13 @@ const void* mfma_twiddle_table(int N, int dir) {
     return d;
 }
 
-// Geometry check for a split-precision pass: N in 16..512 and a tile width
-// the axis admits.
-int mfma_pass_check(const AxisPass& p, int N) {
-    if (!mf_length_ok(N)) return set_error(A2AFFT_E_UNSUPPORTED, "split precision: length %d (16..512)", N);
+// The split-K-halves kernels: N in 16..512 and a tile width the axis admits.
+static bool legacy_fits(const AxisPass& p, int N) {
+    if (!mf_length_ok(N)) return false;
     const int T = mf_T(N, p.strided != 0);
     FftGeom g;
     uint32_t grid = 0;
-    const int st = resolve_with(p.fft, p.a1, p.a2, p.strided, N, T, N, &g, &grid, p.a2_start, p.a2_count);
-    if (st < 0)
-        return set_error(A2AFFT_E_UNSUPPORTED, "split precision: pass of length %d does not admit %d-line tiles", N,
-                         T);
-    return A2AFFT_OK;
+    return resolve_with(p.fft, p.a1, p.a2, p.strided, N, T, N, &g, &grid, p.a2_start, p.a2_count) == A2AFFT_OK;
 }
 
 // One past the largest byte offset a pass touches over its whole axes (all
@@ -1088,25 +1249,38 @@ static uint64_t pass_extent(const AxisPass& p, int N) {
     return 8ull * uint64_t(p.a1.off(p.a1.n - 1) + p.a2.off(p.a2.n - 1) + p.fft.off(uint32_t(N - 1))) + 8;
 }
 
-// The interleaved-K kernel runs split-BF16 passes of 256 and 512 points whose
-// byte offsets fit 32 bits, in 16-line tiles.
+// The interleaved-K kernels run split-BF16 passes of 256 and 512 points (16-
+// line tiles) and contiguous lines of 1024 and 2048 points (8 and 4 lines),
+// whose byte offsets fit 32 bits.
 static bool mf2_fits(const AxisPass& p, int N, a2afft_precision prec) {
-    if (prec != A2AFFT_SPLIT_BF16 || (N != 256 && N != 512)) return false;
+    if (prec != A2AFFT_SPLIT_BF16 || (N != 256 && N != 512 && N != 1024 && N != 2048)) return false;
+    if (N >= 1024 && p.strided) return false;
     if (pass_extent(p, N) >= (1ull << 32)) return false;
     FftGeom g;
     uint32_t grid = 0;
-    return resolve_with(p.fft, p.a1, p.a2, p.strided, N, 16, N, &g, &grid) == A2AFFT_OK;
+    return resolve_with(p.fft, p.a1, p.a2, p.strided, N, mf2_tile_lines(N), N, &g, &grid) == A2AFFT_OK;
+}
+
+// Geometry check for a split-precision pass.
+int mfma_pass_check(const AxisPass& p, int N, a2afft_precision prec) {
+    if (mf2_fits(p, N, prec) || legacy_fits(p, N)) return A2AFFT_OK;
+    return set_error(A2AFFT_E_UNSUPPORTED,
+                     "split precision: no kernel for a pass of length %d (16..512 in tiles of the axis, or "
+                     "split-BF16 contiguous 1024 and 2048)", N);
 }
 
 // Kernel choices of a split pass: T = tile lines; E = 0 for the interleaved-K
-// kernel with variant flags, E = N for the kernels with split K halves.
+// kernels with variant flags, E = N for the kernels with split K halves.
 int mfma_candidates(const AxisPass& p, int N, a2afft_precision prec, KSel* out, int max) {
     int n = 0;
-    if (mf2_fits(p, N, prec))
-        for (const Mf2Entry& e : mf2_registry())
+    if (mf2_fits(p, N, prec)) {
+        std::vector<int> fl;
+        mf2_flags(N, p.strided, &fl);
+        for (int f : fl)
             for (int occ : {0, kMfOcc3, kMfOcc2})
-                if (e.N == N && e.strided == (p.strided != 0) && n < max) out[n++] = KSel{16, 0, e.flags | occ};
-    if (n < max && mfma_pass_check(p, N) == A2AFFT_OK) out[n++] = KSel{mf_T(N, p.strided != 0), N, 0};
+                if (n < max && (N <= 512 || occ == 0)) out[n++] = KSel{mf2_tile_lines(N), 0, f | occ};
+    }
+    if (n < max && legacy_fits(p, N)) out[n++] = KSel{mf_T(N, p.strided != 0), N, 0};
     return n;
 }
 
@@ -1122,7 +1296,7 @@ int mfma_candidates(const AxisPass& p, int N, a2afft_precision prec, KSel* out, 
 // `windowed`: the pass runs in windows of z columns (the plan's chunking).
 KSel mfma_default_sel(const AxisPass& p, int N, a2afft_precision prec, bool windowed) {
     if (mf2_fits(p, N, prec)) {
-        if (!p.strided) return KSel{16, 0, KF_XCD | KF_NT};
+        if (!p.strided) return KSel{mf2_tile_lines(N), 0, KF_XCD | KF_NT};
         if (p.fft.s_elem >= 4096) return KSel{16, 0, kMfRowAdj | KF_A2MAJ};
         return KSel{16, 0, windowed ? KF_NT : KF_XCD | KF_NT};
     }
@@ -1132,20 +1306,20 @@ KSel mfma_default_sel(const AxisPass& p, int N, a2afft_precision prec, bool wind
 int launch_mfma_pass(const AxisPass& p, void* data, int N, int dir, a2afft_precision prec, hipStream_t s,
                      const KSel* sel) {
     if (!data) return set_error(A2AFFT_E_INVALID_ARG, "fft: null data");
-    if (!mf_length_ok(N)) return set_error(A2AFFT_E_UNSUPPORTED, "split precision: length %d (16..512)", N);
     const KSel ks = sel ? *sel : mfma_default_sel(p, N, prec, false);
     const void* tw = mfma_twiddle_table(N, dir);
     if (!tw) return set_error(A2AFFT_E_ALLOC, "split precision: twiddle table N=%d", N);
     float2* dp = (float2*)data;
     if (ks.E == 0) {   // interleaved-K kernel
-        if (!mf2_fits(p, N, prec)) return set_error(A2AFFT_E_UNSUPPORTED, "split precision: no 16-line tile kernel");
-        const void* fn = mf2_find(N, p.strided, ks.flags & ~(kMfOcc3 | kMfOcc2));
+        if (!mf2_fits(p, N, prec)) return set_error(A2AFFT_E_UNSUPPORTED, "split precision: no interleaved-K kernel");
+        const void* fn = mf2_find(N, p.strided, ks.flags & ~(kMfOcc3 | kMfOcc2), dir);
         if (!fn) return set_error(A2AFFT_E_UNSUPPORTED, "split precision: no variant flags=%d", ks.flags);
         // occupancy variants: extra dynamic LDS leaves room for 3 or 2 workgroups per CU
         const size_t dyn = (ks.flags & kMfOcc2) ? size_t(28 << 10) : (ks.flags & kMfOcc3) ? size_t(16 << 10) : 0;
         FftGeom g;
         uint32_t grid = 0;
-        int st = resolve_with(p.fft, p.a1, p.a2, p.strided, N, 16, N, &g, &grid, p.a2_start, p.a2_count);
+        int st = resolve_with(p.fft, p.a1, p.a2, p.strided, N, mf2_tile_lines(N), N, &g, &grid, p.a2_start,
+                              p.a2_count);
         if (st < 0) return set_error(A2AFFT_E_UNSUPPORTED, "split precision: geometry (N=%d)", N);
         g.scale_mode = p.scale_mode;
         g.scale = p.scale;
@@ -1156,6 +1330,7 @@ int launch_mfma_pass(const AxisPass& p, void* data, int N, int dir, a2afft_preci
         if (st < 0) return st;
         return hip_check(int(hipGetLastError()), "mfma fft launch");
     }
+    if (!mf_length_ok(N)) return set_error(A2AFFT_E_UNSUPPORTED, "split precision: length %d (16..512)", N);
     const int T = mf_T(N, p.strided != 0);
     FftGeom g;
     uint32_t grid = 0;

@@ -210,6 +210,42 @@ def test_c3b_2048_rank_share(a2afft_mod, cuda):
     assert _same_bits(back, y)
 
 
+def test_c3b_2048_rank_share_split_bf16(a2afft_mod, cuda):
+    """The 2048x256x256 rank share's pencil FFT (65536 contiguous lines of
+    2048 points, the z pass of a 256x256x2048 grid plan) in split-BF16, on the
+    GRF share (seed 2): within 1e-5 of the FP64 line FFTs, and the two
+    radix-16 MFMA stages plus the radix-8 stage pinned by numpy on sampled
+    pencils."""
+    import torch
+    from a2afft import inputs
+    shape, P, dims = (2048, 256, 256), 64, (1, 8, 8)
+    nloc = shape[0] * shape[1] * shape[2]
+    g = inputs.gaussian_random_field_device(shape, 2)
+    block = g.to(torch.complex64).reshape(-1)
+    del g
+    pencils = torch.empty_like(block)
+    a2afft_mod.launch_remap(6, block, pencils, P, nloc, shape, dims, nbuffers=1)   # [lz][ly][lx]
+    del block
+    ref = pencils.to(torch.complex128)
+    a2afft_mod.forward_1d_fft(ref, 2048)
+    plan = a2afft_mod.GridDfft((256, 256, 2048), "split_bf16")
+    assert "interleaved-K" in plan.describe().splitlines()[0]
+    y = pencils.clone()
+    a2afft_mod.execute_pass(plan, y, 0, -1)
+    rel, mx = a2afft_mod.error_metrics(y, ref)
+    print(f"c3b split-bf16 pencils rel-L2 {rel:.3e} max-abs {mx:.3e}")
+    assert rel <= 1e-5, rel
+    rows = np.random.default_rng(1).choice(nloc // 2048, 32, replace=False)
+    idx = torch.from_numpy(rows).cuda()
+    want = np.fft.fft(pencils.view(-1, 2048)[idx].cpu().numpy().astype(np.complex128), axis=1)
+    got = y.view(-1, 2048)[idx].cpu().numpy()
+    assert np.linalg.norm(got - want) / np.linalg.norm(want) <= 1e-5
+    # inverse pass back to the input (1/N folded in by the caller here: none)
+    a2afft_mod.execute_pass(plan, y, 0, 1)
+    rel_rt, _ = a2afft_mod.error_metrics(y, pencils.to(torch.complex128) * 2048)
+    assert rel_rt <= 2e-5, rel_rt
+
+
 def _c5_shapes():
     from a2afft import inputs
     cand = [(32, 32, 32), (64, 64, 64), (128, 128, 128), (256, 256, 256), (128, 256, 64), (256, 64, 512)]

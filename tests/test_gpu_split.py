@@ -14,7 +14,7 @@ pytestmark = pytest.mark.gpu
 
 TOL = {"split_bf16": 1e-5, "split_fp16": 1e-6}
 GRIDS = [(64, 64, 64), (32, 32, 32), (128, 32, 256), (512, 64, 32), (16, 256, 64), (32, 128, 64),
-         (64, 32, 512), (32, 512, 64), (16, 64, 512)]
+         (64, 32, 512), (32, 512, 64), (16, 64, 512), (16, 16, 2048), (32, 16, 1024)]
 
 
 def _rand(torch, shape, seed):
@@ -31,6 +31,8 @@ def _rel(a, b):
 @pytest.mark.parametrize("shape", GRIDS)
 def test_split_grid_forward_inverse_vs_numpy(a2afft_mod, cuda, prec, shape):
     import torch
+    if prec == "split_fp16" and shape[2] > 512:
+        pytest.skip("split-FP16 runs lines of 16..512 points; 1024 and 2048 are split-BF16 only")
     x, d = _rand(torch, shape, 11)
     plan = a2afft_mod.GridDfft(shape, prec)
     plan.forward(d)
