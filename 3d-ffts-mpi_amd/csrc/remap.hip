@@ -72,6 +72,60 @@ __global__ __launch_bounds__(256) void transpose_kernel(const V* __restrict__ sr
     }
 }
 
+// The same transpose with 16-byte lanes on both sides: tiles of TA x TC
+// elements (512-byte rows: 64 x 64 complex64, 32 x 32 complex128); a lane
+// loads EW = 16 / sizeof(V) consecutive c of one row and stores EW
+// consecutive a of one output row, both non-temporal (each value is read
+// and written once).  Needs A and C multiples of EW; partial tiles are
+// bounds-checked per vector.
+template <typename V, int TA, int TC>
+__global__ __launch_bounds__(256) void transpose16_kernel(const V* __restrict__ src, V* __restrict__ dst,
+                                                          uint32_t A, uint32_t M, uint32_t C,
+                                                          uint32_t tiles_c, uint64_t rank_elems) {
+    constexpr int EW = 16 / int(sizeof(V));
+    constexpr int LR = TC / EW;        // lanes per loaded row
+    constexpr int RP = 256 / LR;       // rows per load pass
+    constexpr int LW = TA / EW;        // lanes per stored row
+    constexpr int WP = 256 / LW;       // rows per store pass
+    typedef uint32_t v4 __attribute__((ext_vector_type(4)));
+    __shared__ V tile[TA][TC + 1];
+    const uint32_t ta = blockIdx.x / tiles_c, tc = blockIdx.x - ta * tiles_c;
+    const uint32_t bm = blockIdx.y;
+    const uint32_t b = bm / M, m = bm - b * M;
+    const size_t base = size_t(blockIdx.z) * rank_elems + size_t(b) * A * M * C;
+    const uint32_t a0 = ta * TA, c0 = tc * TC;
+    const uint32_t tid = threadIdx.x;
+    {
+        const uint32_t lc = (tid % LR) * EW, lr = tid / LR;
+#pragma unroll
+        for (int r = 0; r < TA; r += RP) {
+            const uint32_t a = a0 + lr + r, c = c0 + lc;
+            if (a < A && c < C) {
+                const v4 u = __builtin_nontemporal_load(
+                    reinterpret_cast<const v4*>(src + base + (size_t(a) * M + m) * C + c));
+                const V* e = reinterpret_cast<const V*>(&u);
+#pragma unroll
+                for (int k = 0; k < EW; ++k) tile[lr + r][lc + k] = e[k];
+            }
+        }
+    }
+    __syncthreads();
+    {
+        const uint32_t la = (tid % LW) * EW, lr = tid / LW;
+#pragma unroll
+        for (int r = 0; r < TC; r += WP) {
+            const uint32_t c = c0 + lr + r, a = a0 + la;
+            if (a < A && c < C) {
+                v4 u;
+                V* e = reinterpret_cast<V*>(&u);
+#pragma unroll
+                for (int k = 0; k < EW; ++k) e[k] = tile[la + k][lr + r];
+                __builtin_nontemporal_store(u, reinterpret_cast<v4*>(dst + base + (size_t(c) * M + m) * A + a));
+            }
+        }
+    }
+}
+
 // One 16-byte (or 8-byte) vector per lane, non-temporal both ways: every value
 // is read once and written once, and 16-byte lanes measured 6.5 TB/s on a
 // 1 GiB copy against 5.8 TB/s for 8-byte lanes.
@@ -175,9 +229,28 @@ int launch_transpose(int which, const void* src, void* dst, const int lgs[3], in
     if (which == 0) { B = 1; A = lx; M = ly; C = lz; }            // fast_z_to_x
     else if (which == 1) { B = 1; A = lz * ly; M = 1; C = lx; }   // fast_x_to_y
     else { B = lx; A = lz; M = 1; C = ly; }                       // fast_y_to_z
+    if (uint64_t(B) * M > 65535) return set_error(A2AFFT_E_SIZE, "transpose: grid too large");
+    // 16-byte lanes both ways when the rows split into them (complex64: A and
+    // C even, and 16-byte aligned buffers): 64 x 64 (complex64) or 32 x 32
+    // (complex128) tiles
+    const bool c128 = dtype == A2AFFT_C128;
+    const bool aligned = ((reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(dst)) % 16) == 0 &&
+                         (c128 || (A % 2 == 0 && C % 2 == 0 && uint64_t(nlocal) % 2 == 0));
+    if (aligned) {
+        const uint32_t T = c128 ? 32 : 64;
+        const uint32_t ta = (A + T - 1) / T, tc = (C + T - 1) / T;
+        if (uint64_t(ta) * tc >= (uint64_t(1) << 31)) return set_error(A2AFFT_E_SIZE, "transpose: grid too large");
+        dim3 grid(ta * tc, B * M, nbuffers);
+        if (c128)
+            hipLaunchKernelGGL((transpose16_kernel<double2, 32, 32>), grid, dim3(256), 0, s, (const double2*)src,
+                               (double2*)dst, A, M, C, tc, uint64_t(nlocal));
+        else
+            hipLaunchKernelGGL((transpose16_kernel<float2, 64, 64>), grid, dim3(256), 0, s, (const float2*)src,
+                               (float2*)dst, A, M, C, tc, uint64_t(nlocal));
+        return hip_check(int(hipGetLastError()), "transpose kernel launch");
+    }
     const uint32_t ta = (A + 31) / 32, tc = (C + 31) / 32;
-    if (uint64_t(ta) * tc >= (uint64_t(1) << 31) || uint64_t(B) * M > 65535)
-        return set_error(A2AFFT_E_SIZE, "transpose: grid too large");
+    if (uint64_t(ta) * tc >= (uint64_t(1) << 31)) return set_error(A2AFFT_E_SIZE, "transpose: grid too large");
     dim3 grid(ta * tc, B * M, nbuffers);
     if (dtype == A2AFFT_C128)
         hipLaunchKernelGGL(transpose_kernel<double2>, grid, dim3(256), 0, s, (const double2*)src, (double2*)dst, A,
