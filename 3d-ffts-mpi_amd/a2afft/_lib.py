@@ -96,6 +96,7 @@ SIGNATURES = {
     "a2afft_execute_launch": [_P, _P, _I, _I, _P],
     "a2afft_fill_random": [_P, _I, _I64, ctypes.c_uint64, _I, _I64, _P],
     "a2afft_batch_create": [ctypes.POINTER(_P), _I, _IA, _I],
+    "a2afft_batch_create_ranks": [ctypes.POINTER(_P), _I, _IA, _IA, _I],
     "a2afft_batch_offsets": [_P, ctypes.POINTER(_I64)],
     "a2afft_batch_execute": [_P, _P, _I, _P],
     "a2afft_batch_graph_launch": [_P, _P, _I, _P],

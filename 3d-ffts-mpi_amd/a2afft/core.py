@@ -246,12 +246,22 @@ class GridDfft(_Plan):
 class Batch:
     """Many independent complex64 C-order grids, one persistent launch per transform."""
 
-    def __init__(self, shapes, precision="fp32"):
+    def __init__(self, shapes, precision="fp32", dims=None):
+        """dims: per grid (d0, d1, d2) virtual-rank decomposition (None: undecomposed C-order grids);
+        a decomposed grid is stored as its rank blocks, rank-major, like a Distribution buffer."""
         self.shapes = [tuple(int(v) for v in s) for s in shapes]
+        self.dims = None if dims is None else [tuple(int(v) for v in d) for d in dims]
         flat = [v for s in self.shapes for v in s]
         self._h = ctypes.c_void_p()
-        L.check(L.lib().a2afft_batch_create(ctypes.byref(self._h), len(self.shapes), L.ints(flat),
-                                            PRECISIONS[precision]), "batch_create")
+        if self.dims is None:
+            L.check(L.lib().a2afft_batch_create(ctypes.byref(self._h), len(self.shapes), L.ints(flat),
+                                                PRECISIONS[precision]), "batch_create")
+        else:
+            if len(self.dims) != len(self.shapes):
+                raise ValueError("one dims triple per grid")
+            L.check(L.lib().a2afft_batch_create_ranks(ctypes.byref(self._h), len(self.shapes), L.ints(flat),
+                                                      L.ints([v for d in self.dims for v in d]),
+                                                      PRECISIONS[precision]), "batch_create_ranks")
         offs = (ctypes.c_int64 * (len(self.shapes) + 1))()
         L.check(L.lib().a2afft_batch_offsets(self._h, offs), "batch_offsets")
         self.offsets = list(offs)
@@ -261,8 +271,14 @@ class Batch:
         return torch.empty(self.total, dtype=torch.complex64, device=device)
 
     def views(self, buf):
-        """Per-grid (nx, ny, nz) views of a batch buffer."""
-        return [buf[self.offsets[i]:self.offsets[i + 1]].view(*s) for i, s in enumerate(self.shapes)]
+        """Per-grid views of a batch buffer: (nx, ny, nz), or (P, nlocal) rank blocks for a decomposed grid."""
+        out = []
+        for i, s in enumerate(self.shapes):
+            v = buf[self.offsets[i]:self.offsets[i + 1]]
+            d = self.dims[i] if self.dims is not None else (1, 1, 1)
+            P = d[0] * d[1] * d[2]
+            out.append(v.view(*s) if P == 1 else v.view(P, -1))
+        return out
 
     def _check(self, b):
         if b.dtype != torch.complex64 or b.numel() != self.total or not b.is_contiguous() or not b.is_cuda:

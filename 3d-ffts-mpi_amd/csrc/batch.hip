@@ -509,7 +509,8 @@ static int batch_tables(a2afft_batch_s* b, int64_t group_points, int skewed) {
     return A2AFFT_OK;
 }
 
-extern "C" int a2afft_batch_create(a2afft_batch* out, int ngrids, const int* shapes, a2afft_precision prec) {
+extern "C" int a2afft_batch_create_ranks(a2afft_batch* out, int ngrids, const int* shapes, const int* dims,
+                                         a2afft_precision prec) {
     if (!out || ngrids <= 0 || !shapes) return set_error(A2AFFT_E_INVALID_ARG, "batch_create: bad arguments");
     *out = nullptr;
     if (prec != A2AFFT_FP32) return set_error(A2AFFT_E_UNSUPPORTED, "batch_create: FP32 (complex64) only");
@@ -532,10 +533,21 @@ extern "C" int a2afft_batch_create(a2afft_batch* out, int ngrids, const int* sha
                 batch_free(b);
                 return set_error(A2AFFT_E_UNSUPPORTED, "batch: grid %d extent %d (powers of two 16..512)", g, n[a]);
             }
+        // the grid's virtual-rank block decomposition: P = d0*d1*d2 blocks of
+        // (nx/d0) x (ny/d1) x (nz/d2), rank-major (a2adistribution.cpp:14-42)
+        const int d[3] = {dims ? dims[3 * g] : 1, dims ? dims[3 * g + 1] : 1, dims ? dims[3 * g + 2] : 1};
+        int lgs[3];
+        for (int a = 0; a < 3; ++a) {
+            if (d[a] <= 0 || n[a] % d[a]) {
+                batch_free(b);
+                return set_error(A2AFFT_E_DIVISIBILITY, "batch: grid %d extent %d not divisible by dims %d", g, n[a],
+                                 d[a]);
+            }
+            lgs[a] = n[a] / d[a];
+        }
         b->offsets[g + 1] = b->offsets[g] + total;
-        const int dims[3] = {1, 1, 1};
         PassMaps pm[3];
-        fused_pass_maps(n, dims, n, total, pm);
+        fused_pass_maps(n, d, lgs, total / (int64_t(d[0]) * d[1] * d[2]), pm);
         for (int p = 0; p < 3; ++p)
             b->passes[size_t(p * ngrids + g)] = AxisPass{pm[p].fft, pm[p].a1, pm[p].a2, pm[p].strided, 0, 1.0};
     }
@@ -562,6 +574,10 @@ extern "C" int a2afft_batch_create(a2afft_batch* out, int ngrids, const int* sha
     b->grid = b->max_grid;
     *out = b;
     return A2AFFT_OK;
+}
+
+extern "C" int a2afft_batch_create(a2afft_batch* out, int ngrids, const int* shapes, a2afft_precision prec) {
+    return a2afft_batch_create_ranks(out, ngrids, shapes, nullptr, prec);
 }
 
 extern "C" int a2afft_batch_offsets(a2afft_batch b, int64_t* offsets) {

@@ -220,6 +220,14 @@ int a2afft_plan_destroy(a2afft_plan plan);
  * and reports whether every dependency wait finished within its bound. */
 typedef struct a2afft_batch_s* a2afft_batch;
 int a2afft_batch_create(a2afft_batch* out, int ngrids, const int* shapes, a2afft_precision precision);
+/* The same with a virtual-rank decomposition per grid (dims: ngrids x {d0, d1,
+ * d2}, NULL = all {1,1,1}): grid g is stored as d0*d1*d2 rank blocks back to
+ * back, each (nx/d0) x (ny/d1) x (nz/d2) in C order, rank-major -- the layout
+ * of a distribution's [P][nlocal] buffer (a2adistribution.cpp:14-42, 59-69)
+ * -- and its passes address the blocks directly, as a fused plan over that
+ * distribution does (no separate all-to-all or pack). */
+int a2afft_batch_create_ranks(a2afft_batch* out, int ngrids, const int* shapes, const int* dims,
+                              a2afft_precision precision);
 int a2afft_batch_offsets(a2afft_batch batch, int64_t* offsets);
 int a2afft_batch_execute(a2afft_batch batch, void* d_data, int direction, void* stream);
 int a2afft_batch_graph_launch(a2afft_batch batch, void* d_data, int direction, void* stream);

@@ -181,3 +181,64 @@ def test_batch_rejects_unsupported_shapes(a2afft_mod, cuda):
         a2afft_mod.Batch([(32, 32, 1024)])
     with pytest.raises(a2afft_mod.A2AFFTError):
         a2afft_mod.Batch([(24, 32, 32)])
+
+
+# (shape, dims): cubic grids over the reference's decompositions and some
+# non-cubic blocks, mixed with undecomposed grids
+RANK_ITEMS = [((64, 64, 64), (1, 2, 4)), ((32, 32, 32), (2, 2, 1)), ((128, 128, 128), (2, 2, 2)),
+              ((64, 64, 64), (1, 1, 1)), ((128, 256, 64), (2, 2, 1)), ((64, 64, 64), (4, 2, 2)),
+              ((32, 32, 32), (1, 4, 4)), ((256, 64, 128), (1, 2, 2))]
+
+
+def test_batch_rank_blocks_match_plans_and_oracle(a2afft_mod, orc, cuda):
+    """Grids decomposed into virtual-rank blocks inside the persistent launch:
+    every cubic item equals, bit for bit, the fused plan over its distribution
+    (same block layout), is within 1e-6 of the oracle's reference-order FP64
+    transform at 32^3 and 64^3, and every item (non-cubic included) matches
+    numpy's fftn of the stitched grid; the inverse returns the input."""
+    import torch
+    from a2afft import inputs
+    shapes = [s for s, _ in RANK_ITEMS]
+    dims = [d for _, d in RANK_ITEMS]
+    b = a2afft_mod.Batch(shapes, dims=dims)
+    x = _fill(a2afft_mod, torch, b, 12)
+    x0 = x.cpu().numpy()
+    y = x.clone()
+    b.forward(y)
+    assert b.status() == 0
+    got = y.cpu().numpy()
+    for i, (s, d) in enumerate(RANK_ITEMS):
+        lo, hi = b.offsets[i], b.offsets[i + 1]
+        P = d[0] * d[1] * d[2]
+        blocks = x0[lo:hi].reshape(P, -1)
+        if s[0] == s[1] == s[2]:
+            dist = a2afft_mod.Distribution(s[0], P, dims=d, dtype="c64")
+            g = torch.from_numpy(blocks.copy()).cuda()
+            a2afft_mod.Dfft(dist, "fp32").forward(g)
+            torch.cuda.synchronize()
+            assert np.array_equal(_bits(got[lo:hi]), _bits(g.cpu().numpy().reshape(-1))), (s, d)
+            if s[0] <= 64:
+                rc, od = orc.decompose(s[0], P, d)
+                assert rc == 0
+                ref = orc.forward(od, blocks.astype(np.complex128))
+                err = np.linalg.norm(got[lo:hi].reshape(P, -1) - ref) / np.linalg.norm(ref)
+                assert err < 1e-6, (s, d, err)
+        glob = inputs.blocks_to_global(got[lo:hi].reshape(P, -1), d, s)
+        ref = np.fft.fftn(inputs.blocks_to_global(blocks, d, s).astype(np.complex128))
+        err = np.linalg.norm(glob - ref) / np.linalg.norm(ref)
+        assert err < 6e-7, (s, d, err)
+    b.inverse(y)
+    assert b.status() == 0
+    back = y.cpu().numpy()
+    assert np.linalg.norm(back - x0) / np.linalg.norm(x0) < 1e-6
+    # views: rank blocks for decomposed grids
+    v = b.views(y)
+    assert tuple(v[0].shape) == (8, 64 * 64 * 64 // 8) and tuple(v[3].shape) == (64, 64, 64)
+
+
+def test_batch_rank_blocks_reject_bad_dims(a2afft_mod, cuda):
+    from a2afft import _lib as L
+    with pytest.raises(L.A2AFFTError):
+        a2afft_mod.Batch([(64, 64, 64)], dims=[(3, 1, 1)])
+    with pytest.raises(ValueError):
+        a2afft_mod.Batch([(64, 64, 64), (32, 32, 32)], dims=[(1, 1, 1)])
