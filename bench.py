@@ -284,6 +284,33 @@ def run_c5(args, torch, a2afft):
     t_sep = _events_ms(torch, separate, rot, args.steps, args.warmup)
     finite = rot.finite()
     del rot
+    # the same 64 grids, each decomposed over 8 virtual ranks, dims [2,2,2]:
+    # the persistent launch addresses the rank blocks directly (the remap
+    # folded into the passes); the input is the same field in block layout
+    dims = [(2, 2, 2)] * len(shapes)
+    br = a2afft.Batch(shapes, dims=dims)
+    xr = torch.empty_like(x)
+    for i, s in enumerate(shapes):
+        lo, hi = b.offsets[i], b.offsets[i + 1]
+        xr[lo:hi] = a2afft.inputs.global_to_blocks(x[lo:hi].view(*s), dims[i]).reshape(-1)
+    yr = xr.clone()
+    br.forward(yr)
+    assert br.status() == 0
+    errs_r = []
+    for i, s in enumerate(shapes):
+        lo, hi = b.offsets[i], b.offsets[i + 1]
+        ref = x[lo:hi].to(torch.complex128).view(*s).contiguous()
+        a2afft.GridDfft(s, "fp64").forward(ref)
+        got = a2afft.inputs.blocks_to_global(yr[lo:hi].view(8, -1), dims[i], s).reshape(-1)
+        errs_r.append(a2afft.error_metrics(got.contiguous(), ref.reshape(-1))[0])
+        del ref, got
+    br.autotune(yr, iters=3)
+    del yr
+    rot = Rotation(torch, xr, rotation_count(xr, args.steps))
+    t_ranks = _events_ms(torch, lambda buf: br.forward(buf), rot, args.steps, args.warmup)
+    assert br.status() == 0
+    finite = finite and rot.finite()
+    del rot, xr
     npts = sum(s[0] * s[1] * s[2] for s in shapes)
     flops = sum(5.0 * (s[0] * s[1] * s[2]) * math.log2(s[0] * s[1] * s[2]) for s in shapes)
     bytes_alg = 6 * npts * 8
@@ -300,8 +327,9 @@ def run_c5(args, torch, a2afft):
                    "points": npts, "schedule": {k: sched[k] for k in ("workgroups", "xcd_queues", "chunk", "group_points",
                                                                   "skewed")}},
         "ms": {"persistent_eager": round(t_eager, 4), "persistent_graph": round(t_graph, 4),
-               "separate_launches_192": round(t_sep, 4)},
+               "separate_launches_192": round(t_sep, 4), "persistent_rank_blocks_dims222": round(t_ranks, 4)},
         "rel_l2_vs_fp64": {"max": max(errs), "mean": sum(errs) / len(errs)},
+        "rank_blocks_rel_l2_vs_fp64": {"max": max(errs_r), "mean": sum(errs_r) / len(errs_r)},
         "roofline": {"bound": "hbm", "kernel": "batch_kernel (persistent)", "bytes_per_launch": bytes_alg,
                      "launch_ms": round(best, 4), "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": _traffic("c5_batch")},
