@@ -669,7 +669,10 @@ __global__ __launch_bounds__(256) void mf2_kernel(float2* __restrict__ data, Fft
     constexpr bool ROWADJ = (FLAGS & kMfRowAdj) != 0;
     __shared__ u32x4 xh[1024];
     __shared__ u32x4 xl[1024];
-    __shared__ float2 tw1[256];                 // W_256^(q r) at [16 q + r]
+    // W_256^(q r) at [16 q + r]; the table is symmetric (each entry is
+    // computed from the product q r), so lanes that vary q read [16 r + q]:
+    // consecutive lanes, consecutive banks
+    __shared__ float2 tw1[256];
     __shared__ float2 tw2[R2 > 1 ? 256 : 1];    // W_512^k, k < 256
 
     const int tid = int(threadIdx.x);
@@ -857,7 +860,7 @@ __global__ __launch_bounds__(256) void mf2_kernel(float2* __restrict__ data, Fft
 #pragma unroll
                     for (int i = 0; i < 4; ++i) {
                         const int r = (16 * jh + 4 * gq + i) / R2;   // K slot in the stage-1 butterfly
-                        c[i] = c_mul(make_float2(yr[i], yi[i]), tw1[16 * cn + r]);
+                        c[i] = c_mul(make_float2(yr[i], yi[i]), tw1[16 * r + cn]);
                     }
                     if constexpr (R2 == 1) {
                         u32x4 h, l;
@@ -1001,7 +1004,7 @@ __global__ __launch_bounds__(256) void mf2_long_kernel(float2* __restrict__ data
                 const int j = 16 * jh + 4 * gq + i;
                 const int s = j % R2, r = j / R2;
                 uint32_t h, l;
-                split_c(c_mul(make_float2(yr[i], yi[i]), tw1[16 * cn + r]), h, l);
+                split_c(c_mul(make_float2(yr[i], yi[i]), tw1[16 * r + cn]), h, l);
                 const int slot = ((la * R2 + s) * 4 + (r >> 2)) * 16 + cn;
                 wh[4 * slot + (r & 3)] = h;
                 wl[4 * slot + (r & 3)] = l;
