@@ -325,6 +325,13 @@ bool chirp_z_preferred(int N) {
     return int64_t(N) * N >= 108 * M;
 }
 
+// True when n has no prime factor other than 2, 3 and 5 (a Stockham length).
+static bool smooth235(int n) {
+    for (int f : {2, 3, 5})
+        while (n % f == 0) n /= f;
+    return n == 1;
+}
+
 int launch_long_fft(void* data, int N, int64_t lines, a2afft_dtype dt, int dir, hipStream_t s) {
     int N1 = 0, N2 = 0;
     if (!long_fft_split(N, &N1, &N2)) {
@@ -333,6 +340,12 @@ int launch_long_fft(void* data, int N, int64_t lines, a2afft_dtype dt, int dir, 
                          "1d fft: length %d is neither a product of two lengths <= %d nor at most %d", N, kDftMaxN,
                          kBluesteinMaxN);
     }
+    // A factor with a large prime would run its column pass as a direct DFT,
+    // O(factor) per element (8186 = 4093 * 2: 122 ms per GiB for the 4093-point
+    // pass alone); where chirp-z beats that DFT, transform the whole line by
+    // chirp-z instead.
+    for (const int f : {N1, N2})
+        if (!smooth235(f) && chirp_z_preferred(f)) return launch_chirp_z(data, N, lines, dt, dir, s);
     const size_t esize = dt == A2AFFT_C128 ? 16 : 8;
     const LongTw* tw = long_twiddles(N, dir);
     if (!tw) return set_error(A2AFFT_E_ALLOC, "1d fft: twiddles for length %d", N);
