@@ -48,6 +48,14 @@ int register_kernels_f32(KEntry* out) {
     add_shape<float, 2048, 32, 8, true>(out, &n);    // 512 threads, 128 KiB LDS, 64 B rows
     // half-width exchanges for the 512 and 2048 default shapes
     add_halfx<float, 512, 32, 16, true>(out, &n);
+    // hardware tile order (no XCD renumbering) with half-width exchanges
+    add_variant<float, 512, 32, 16, true, KF_HALFX>(out, &n);
+    add_variant<float, 512, 32, 16, true, KF_NT | KF_HALFX>(out, &n);
+    add_variant<float, 512, 32, 16, true, KF_PAD | KF_NT | KF_HALFX>(out, &n);
+    add_variant<float, 512, 32, 16, true, KF_PAD | KF_HALFX>(out, &n);
+    add_variant<float, 512, 32, 16, true, KF_A2MAJ | KF_NT | KF_HALFX>(out, &n);
+    add_variant<float, 512, 32, 16, true, KF_A2MAJ | KF_PAD | KF_HALFX>(out, &n);
+    add_variant<float, 512, 32, 16, true, KF_A2MAJ | KF_PAD | KF_NT | KF_HALFX>(out, &n);
     add_halfx<float, 512, 16, 16, false>(out, &n);
     add_halfx<float, 2048, 16, 2, false>(out, &n);
     return n;
