@@ -23,13 +23,15 @@ struct AxisPass {
     double scale;
     uint32_t a2_start = 0;   // window along a2 (count 0 = whole axis)
     uint32_t a2_count = 0;
+    void* work = nullptr;    // chirp-z passes: a workspace the caller owns (null: allocated per call)
+    size_t work_bytes = 0;
 };
 
 // Supported transform lengths: every N in 2..4096.  Fixed-schedule Stockham
 // kernels cover powers of two and 2^a 3^b 5^c (2^a <= 16) up to 60
 // (fft_len_ok; a precision may lack some of the latter), the runtime-schedule
-// Stockham kernel (KF_MIX) the other 2^a 3^b 5^c lengths, the direct DFT
-// (KF_DFT) the rest.
+// Stockham kernel (KF_MIX) the other 2^a 3^b 5^c lengths, the chirp-z pass
+// (KF_CZ) or the direct DFT (KF_DFT) the rest.
 bool fft_length_supported(int N);
 
 // Device twiddle table for (N, dir, dtype); created on first use, cached for
@@ -51,8 +53,10 @@ int launch_fft_pass(const AxisPass& pass, void* data, a2afft_dtype dtype, int N,
 int launch_pattern_copy(const AxisPass& pass, void* data, a2afft_dtype dtype, int N, const KSel& sel, hipStream_t s);
 
 // In-place DFTs of `lines` contiguous lines of N points, any supported N
-// (the batched 1D launchers' dispatcher).
-int fft_lines(void* data, int N, int64_t lines, a2afft_dtype dtype, int dir, hipStream_t s);
+// (the batched 1D launchers' dispatcher).  work/work_bytes: a workspace for
+// the lengths that need one (four-step, chirp-z); null = allocated per call.
+int fft_lines(void* data, int N, int64_t lines, a2afft_dtype dtype, int dir, hipStream_t s, void* work = nullptr,
+              size_t work_bytes = 0);
 
 // Batched contiguous lines longer than 4096 points (fft_long.hip): four-step
 // over N = N1 * N2 with both factors <= 4096 (long_fft_split), through a
@@ -63,7 +67,28 @@ constexpr int kBluesteinMaxN = 1 << 23;
 // direct DFT at a length the Stockham kernels do not cover.
 int launch_chirp_z(void* data, int N, int64_t lines, a2afft_dtype dtype, int dir, hipStream_t s);
 bool chirp_z_preferred(int N);
-int launch_long_fft(void* data, int N, int64_t lines, a2afft_dtype dtype, int dir, hipStream_t s);
+// work/work_bytes: the rectangular four-step's workspace (null = allocated
+// per call from the stream-ordered allocator).
+int launch_long_fft(void* data, int N, int64_t lines, a2afft_dtype dtype, int dir, hipStream_t s,
+                    void* work = nullptr, size_t work_bytes = 0);
+
+// Chirp-z passes (KF_CZ) over any axis pass geometry, N <= kDftMaxN: the
+// lines are gathered into power-of-two rows of a workspace, convolved there
+// with two M-point FFTs and scattered back (fft_long.hip).
+bool chirp_pass_ok(int N);   // N in 9..kDftMaxN (M >= 32)
+// Workspace bytes a chirp-z pass over `lines` lines uses (chunks of at most
+// 256 MiB of rows; twice that where the M-point FFT is a rectangular four-step).
+size_t chirp_pass_work_bytes(int N, a2afft_dtype dtype, uint64_t lines);
+// Builds every table a chirp-z pass of length N reads, both directions, and
+// returns the chirp of direction `dir` (the pass's table), or null.
+const void* chirp_pass_tables(int N, int dir, a2afft_dtype dtype);
+// The gather kernel (the pass's first launch), for dispatch checks.
+const void* chirp_pass_kernel(bool dbl);
+// Runs a resolved chirp-z pass: `units` lines of geometry g (strided: adjacent
+// lines adjacent in memory) in place on `data`, in chunks through `work`
+// (null: allocated from the stream-ordered allocator for this call).
+int launch_chirp_pass(const FftGeom& g, uint32_t units, int strided, void* data, a2afft_dtype dtype, int N, int dir,
+                      hipStream_t s, void* work, size_t work_bytes);
 
 // Stockham FFT with the radix schedule in the kernel arguments (KF_MIX,
 // fft_mixed.hip) for 2^a 3^b 5^c lengths the fixed schedules do not cover.

@@ -21,13 +21,17 @@ bool fft_length_supported(int N) { return N >= 2 && N <= kDftMaxN; }
 
 static bool is_dft(const KSel& s) { return (s.flags & KF_DFT) != 0; }
 static bool is_mix(const KSel& s) { return (s.flags & KF_MIX) != 0; }
+static bool is_cz(const KSel& s) { return (s.flags & KF_CZ) != 0; }
 static KSel dft_sel(int N) { return KSel{1, N, KF_DFT}; }
+static KSel cz_sel(int N) { return KSel{1, N, KF_CZ}; }
 
 // Kernel choice for a length no fixed-schedule kernel covers: the
-// runtime-schedule Stockham when its shape fits the pass, else the direct DFT.
+// runtime-schedule Stockham when its shape fits the pass, else the chirp-z
+// pass where it beats the direct DFT, else the direct DFT.
 static KSel fallback_sel(const AxisPass& p, a2afft_dtype dt, int N) {
     MixedPlan mp;
     if (mixed_plan(p, N, dt == A2AFFT_C128, &mp)) return KSel{mp.T, mp.E, KF_MIX};
+    if (chirp_pass_ok(N) && chirp_z_preferred(N)) return cz_sel(N);
     return dft_sel(N);
 }
 
@@ -111,6 +115,8 @@ const void* find_kernel(int N, bool dbl, int strided, const KSel& s, int dir) {
     if (is_mix(s)) return s.flags == KF_MIX ? mixed_kernel_for(N, s.T, s.E, dbl, strided, dir) : nullptr;
     if (is_dft(s)) return (s.T == 1 && s.E == N && s.flags == KF_DFT && fft_length_supported(N)) ? dft_kernel_for(dbl, dir)
                                                                                                     : nullptr;
+    if (is_cz(s)) return (s.T == 1 && s.E == N && s.flags == KF_CZ && chirp_pass_ok(N)) ? chirp_pass_kernel(dbl)
+                                                                                        : nullptr;
     init_registry();
     for (int i = 0; i < g_nkernels; ++i) {
         const KEntry& k = g_kernels[i];
@@ -140,6 +146,7 @@ int fft_candidates(const AxisPass& p, a2afft_dtype dt, int N, KSel* out, int max
             for (int tt : {1, 2, 4, 8, 16, 32})
                 if (tt != f.T && n < max && mixed_plan(p, N, dbl, &mp, tt)) out[n++] = KSel{tt, mp.E, KF_MIX};
         }
+        if (chirp_pass_ok(N) && n < max) out[n++] = cz_sel(N);
         if (n < max) out[n++] = dft_sel(N);
     }
     return n;
@@ -191,26 +198,16 @@ KSel fft_default_sel(const AxisPass& p, a2afft_dtype dt, int N) {
     return s;
 }
 
-// Geometry of a direct-DFT pass: one line per workgroup, lines enumerated
-// (a1, a2) with a2 fastest over the launch's a2 window.
+// Geometry of a direct-DFT or chirp-z pass: lines enumerated (a1, a2) with a2
+// fastest over the launch's a2 window (one line per direct-DFT workgroup).
 static int resolve_dft(const AxisPass& p, int N, FftGeom* g, uint32_t* grid) {
     if (p.fft.n != uint32_t(N)) return set_error(A2AFFT_E_INVALID_ARG, "fft pass: bad geometry (N=%d)", N);
-    const uint32_t count = p.a2_count ? p.a2_count : p.a2.n - p.a2_start;
-    if (uint64_t(p.a2_start) + count > p.a2.n) return set_error(A2AFFT_E_INVALID_ARG, "fft pass: bad window");
-    const uint64_t units = uint64_t(p.a1.n) * count;
-    if (units >= (uint64_t(1) << 31)) return set_error(A2AFFT_E_SIZE, "fft pass: too many lines");
-    g->fft = p.fft;
-    g->a1 = p.a1;
-    g->a2 = p.a2;
-    g->a2_start = p.a2_start;
-    g->n2 = count;
-    g->nunits = uint32_t(units);
-    g->dn2.set(count ? count : 1);
-    g->dn1.set(p.a1.n ? p.a1.n : 1);
-    g->tstride = p.a2.s_elem;
+    // the contiguous enumeration with one line per unit, whatever the layout
+    const int st = resolve_geom(p.fft, p.a1, p.a2, 0, 1, g, grid, p.a2_start, p.a2_count);
+    if (st == A2AFFT_E_SIZE) return set_error(st, "fft pass: too many lines");
+    if (st < 0) return set_error(st, "fft pass: bad window");
     g->scale_mode = p.scale_mode;
     g->scale = p.scale;
-    *grid = uint32_t(units);
     return A2AFFT_OK;
 }
 
@@ -228,8 +225,8 @@ static int resolve(const AxisPass& p, a2afft_dtype dt, int N, const KSel& s, Fft
         if (mp_out) *mp_out = mp;
         return A2AFFT_OK;
     }
-    if (is_dft(s)) {
-        if (s.T != 1 || s.E != N || !fft_length_supported(N))
+    if (is_dft(s) || is_cz(s)) {
+        if (s.T != 1 || s.E != N || !(is_cz(s) ? chirp_pass_ok(N) : fft_length_supported(N)))
             return set_error(A2AFFT_E_UNSUPPORTED, "fft pass: length %d not supported", N);
         return resolve_dft(p, N, g, grid);
     }
@@ -252,7 +249,7 @@ int fft_pass_shape(const AxisPass& p, a2afft_dtype dt, int N, const KSel* sel, F
     out->E = s.E;
     // the launched thread count: the runtime-schedule kernel rounds its
     // workgroup to its own width (mp.NT)
-    out->NT = is_dft(s) ? 256 : is_mix(s) ? mp.NT : s.T * (N / s.E);
+    out->NT = (is_dft(s) || is_cz(s)) ? 256 : is_mix(s) ? mp.NT : s.T * (N / s.E);
     out->grid = int(grid);
     out->flags = s.flags;
     return A2AFFT_OK;
@@ -278,6 +275,8 @@ int launch_fft_pass(const AxisPass& p, void* data, a2afft_dtype dt, int N, int d
         st = hip_check(int(hipLaunchKernel(fn, dim3(grid), dim3(mp.NT), args,
                                            size_t(mixed_lds_bytes(mp, dt == A2AFFT_C128)), s)),
                        "mixed-radix launch");
+    } else if (is_cz(ks)) {
+        st = launch_chirp_pass(g, grid, p.strided, data, dt, N, dir, s, p.work, p.work_bytes);
     } else if (is_dft(ks)) {
         const size_t line = size_t(N) * size_t(esize_of(dt));
         const size_t lds = 2 * line <= 65536 ? 2 * line : line;
@@ -387,6 +386,7 @@ const void* pass_twiddles(const AxisPass& p, int N, int dir, a2afft_dtype dt, co
         if (!mixed_plan(p, N, dt == A2AFFT_C128, &mp)) return nullptr;
         return mixed_twiddles(mp, dir, dt);
     }
+    if (is_cz(s)) return chirp_pass_tables(N, dir, dt);
     return cached_table(N, dir, dt, is_dft(s) ? 2 : 1);
 }
 
@@ -395,14 +395,14 @@ const void* pass_twiddles(const AxisPass& p, int N, int dir, a2afft_dtype dt, co
 using namespace a2a;
 
 namespace a2a {
-int fft_lines(void* data, int N, int64_t lines, a2afft_dtype dtype, int dir, hipStream_t s) {
+int fft_lines(void* data, int N, int64_t lines, a2afft_dtype dtype, int dir, hipStream_t s, void* work,
+              size_t work_bytes) {
     if (lines >= (int64_t(1) << 32)) return set_error(A2AFFT_E_SIZE, "1d fft: too many lines");
-    if (N > kDftMaxN) return launch_long_fft(data, N, lines, dtype, dir, s);
+    if (N > kDftMaxN) return launch_long_fft(data, N, lines, dtype, dir, s, work, work_bytes);
     const PassMaps pm = batched_pass_maps(N, uint32_t(lines));
-    const AxisPass p{pm.fft, pm.a1, pm.a2, 0, 0, 1.0};
-    // lengths only the direct DFT covers: the chirp-z form where it is faster
-    if ((fft_default_sel(p, dtype, N).flags & KF_DFT) && chirp_z_preferred(N))
-        return launch_chirp_z(data, N, lines, dtype, dir, s);
+    AxisPass p{pm.fft, pm.a1, pm.a2, 0, 0, 1.0};
+    p.work = work;   // lengths only the direct DFT covers run as chirp-z where it is faster
+    p.work_bytes = work_bytes;
     return launch_fft_pass(p, data, dtype, N, dir, s, nullptr);
 }
 }  // namespace a2a

@@ -155,8 +155,41 @@ enum : int {
     KF_HALFX = 32, // LDS exchanges move one component at a time: half the LDS, two more barriers
     KF_DFT = 64,   // direct O(N^2) DFT, one line per workgroup: the lengths no Stockham kernel covers
     KF_MIX = 128,  // Stockham FFT with the radix schedule in the arguments: long 2^a 3^b 5^c lengths
+    KF_CZ = 2048,  // Bluestein chirp-z through a workspace of power-of-two rows: large prime factors
 };
 constexpr int kDftMaxN = 4096;   // longest direct-DFT line (N complex values in LDS)
+
+// Chirp-z passes (KF_CZ, fft_long.hip).  The lines of a launch window, in
+// chunks of `nl` lines from line l0, are gathered into rows of M elements of
+// a workspace (row u - l0: elements 0..N-1 times the chirp, zeros after) and
+// scattered back after the convolution.  Line u of a window is
+// (u1, u2) = (u / n2, u % n2), its element n at
+// a1.off(u1) + a2.off(a2_start + u2) + fft.off(n).  Workgroup b of a chunk
+// moves the tile (b / etiles, b % etiles) of kCzTL lines by kCzTM elements
+// through LDS.  Slot i of a tile is (line, element) (i % kCzTL, i / kCzTL)
+// on the grid side of a strided pass, where adjacent lines are adjacent in
+// memory, and (i / kCzTM, i % kCzTM) on the grid side of a contiguous pass
+// and on the workspace side.
+constexpr int kCzTL = 64, kCzTM = 32;
+A2A_HD int64_t cz_line_base(const FftGeom& g, uint32_t u) {
+    const uint32_t u1 = g.dn2.div(u);
+    return g.a1.off(u1) + g.a2.off(g.a2_start + (u - u1 * g.n2));
+}
+A2A_HD void cz_slot(bool lines_fast, int i, int* tl, int* te) {
+    if (lines_fast) {
+        *tl = i % kCzTL;
+        *te = i / kCzTL;
+    } else {
+        *tl = i / kCzTM;
+        *te = i % kCzTM;
+    }
+}
+// The convolution length: the power of two >= 2N - 1.
+A2A_HD int cz_length(int N) {
+    int M = 1;
+    while (M < 2 * N - 1) M *= 2;
+    return M;
+}
 
 // Kernel choice for one pass: tile width T, elements per thread E and the
 // variant flags.  NT = T * N / E threads per workgroup.

@@ -553,12 +553,25 @@ def test_long_lines_on_two_streams(a2afft_mod, cuda):
         assert np.array_equal(_bits(b.cpu().numpy()), _bits(ref_b.cpu().numpy())), rep
 
 
+KF_DFT, KF_CZ = 64, 2048
+
+
+def _chosen_flags(plan, name):
+    for ln in plan.describe().splitlines():
+        if ln.startswith(f"chosen {name} "):
+            return int(ln.split("flags=")[1].split()[0])
+    raise AssertionError(plan.describe())
+
+
+@pytest.mark.parametrize("kernel", ["default", "dft"])
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
-@pytest.mark.parametrize("case", [((3, 5, 4095), 0, 2), ((4093, 2, 4), 1, 0), ((6, 2039, 4), 2, 1)],
+@pytest.mark.parametrize("case", [((3, 5, 4095), 0, 2), ((4093, 2, 4), 1, 0), ((6, 2039, 4), 2, 1),
+                                  ((4, 6, 509), 0, 2), ((251, 3, 8), 1, 0)],
                          ids=lambda c: f"{c[0]}_pass{c[1]}")
-def test_direct_dft_in_plan_passes(a2afft_mod, cuda, case, prec):
-    """Plan passes keep the direct DFT for lengths no Stockham kernel covers
-    (the 1D launchers take the chirp-z form there), contiguous and strided:
+def test_prime_lengths_in_plan_passes(a2afft_mod, cuda, case, prec, kernel):
+    """Plan passes over lengths no Stockham kernel covers, contiguous (z) and
+    strided (x, y): by default the chirp-z pass (lengths where it beats the
+    direct DFT, all of these), or the direct DFT forced through set_kernel;
     one pass against the oracle's FFT along that axis."""
     import torch
     shape, pass_, axis = case
@@ -568,12 +581,100 @@ def test_direct_dft_in_plan_passes(a2afft_mod, cuda, case, prec):
     dt = np.complex128 if prec == "fp64" else np.complex64
     x = x.astype(dt)
     plan = a2afft_mod.GridDfft(shape, prec)
+    name = "zxy"[pass_]
+    if kernel == "dft":
+        plan.set_kernel(pass_, 1, N, KF_DFT)
+        assert _chosen_flags(plan, name) == KF_DFT
+        tol = TOL_1D["c128" if prec == "fp64" else "c64"] * np.sqrt(N)
+    else:
+        assert _chosen_flags(plan, name) == KF_CZ
+        tol = TOL_1D["c128" if prec == "fp64" else "c64"] * 3 * np.log2(N)
     d = torch.from_numpy(x).cuda()
     a2afft_mod.execute_pass(plan, d, pass_, -1)
     torch.cuda.synchronize()
     ref = np.moveaxis(O.fft_batched(np.moveaxis(x.astype(np.complex128), axis, -1), -1), -1, axis)
-    tol = TOL_1D["c128" if prec == "fp64" else "c64"] * np.sqrt(N)
     assert _relerr(d.cpu().numpy(), ref) < tol
+    # inverse of the same pass: back to N times the input (the y pass, last
+    # of an inverse transform, also divides by the grid size)
+    a2afft_mod.execute_pass(plan, d, pass_, 1)
+    torch.cuda.synchronize()
+    gain = N / (np.prod(shape) if pass_ == 2 else 1)
+    assert _relerr(d.cpu().numpy() / gain, x.astype(np.complex128)) < 2 * tol
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+def test_chirp_3d_fused_staged_numpy(a2afft_mod, cuda, dtype):
+    """254^3 (254 = 2 * 127) over dims [1,1,2]: every axis a chirp-z pass.
+    Fused and staged (reference stage order, batched chirp-z pencils) agree
+    bit for bit, forward and inverse; the forward matches numpy's fftn of the
+    stitched grid; a captured graph replays the eager bits."""
+    import torch
+    from a2afft import inputs
+    Ng, dims = 254, (1, 1, 2)
+    g = inputs.uniform(11, Ng ** 3).reshape((Ng,) * 3)
+    x = inputs.global_to_blocks(g, dims).astype(np.complex128 if dtype == "c128" else np.complex64)
+    dist = a2afft_mod.Distribution(Ng, 2, dims=dims, dtype=dtype)
+    outs = {}
+    for ex in ("staged", "fused"):
+        plan = a2afft_mod.Dfft(dist, exec=ex)
+        plan.make_plans(dist.empty())
+        if ex == "fused":
+            for name in "zxy":
+                assert _chosen_flags(plan, name) == KF_CZ
+        data = torch.from_numpy(x).cuda()
+        plan.forward(data)
+        torch.cuda.synchronize()
+        outs[ex] = data.cpu().numpy()
+        plan.inverse(data)
+        torch.cuda.synchronize()
+        outs[ex + "_inv"] = data.cpu().numpy()
+        if ex == "fused":
+            for rep in range(3):
+                y = torch.from_numpy(x).cuda()
+                plan.graph_launch(y, -1)
+                torch.cuda.synchronize()
+                assert np.array_equal(_bits(y.cpu().numpy()), _bits(outs["fused"])), rep
+                z = torch.from_numpy(x).cuda()
+                plan.forward(z)   # eager work between replays
+                torch.cuda.synchronize()
+    assert np.array_equal(_bits(outs["staged"]), _bits(outs["fused"])), "fused != staged (forward)"
+    assert np.array_equal(_bits(outs["staged_inv"]), _bits(outs["fused_inv"])), "fused != staged (inverse)"
+    tol = TOL_1D[dtype] * 3 * np.log2(Ng) * 2
+    ref = np.fft.fftn(g if dtype == "c128" else g.astype(np.complex64).astype(np.complex128))
+    out = inputs.blocks_to_global(outs["fused"], dims, g.shape)
+    assert _relerr(out, ref) < tol
+    inv = inputs.blocks_to_global(outs["fused_inv"], dims, g.shape)
+    assert _relerr(inv, inputs.blocks_to_global(x, dims, g.shape).astype(np.complex128)) < 2 * tol
+
+
+def test_chirp_pass_windows_match_whole(a2afft_mod, cuda):
+    """A 509-point strided x pass (chirp-z) over 16-column z windows, and
+    every direct-DFT pass forced, against whole passes: same bits per
+    kernel choice; chirp-z and direct DFT within rounding of each other."""
+    import torch
+    shape = (509, 16, 64)
+    x = torch.empty(shape, dtype=torch.complex64, device="cuda")
+    a2afft_mod.fill_random(x, 29, "uniform")
+    whole = a2afft_mod.GridDfft(shape, "fp32")
+    win = a2afft_mod.GridDfft(shape, "fp32")
+    win.set_chunking(16)
+    assert _chosen_flags(whole, "x") == KF_CZ
+    a, b = x.clone(), x.clone()
+    whole.forward(a)
+    win.forward(b)
+    assert torch.equal(torch.view_as_real(a), torch.view_as_real(b))
+    whole.inverse(a)
+    win.inverse(b)
+    assert torch.equal(torch.view_as_real(a), torch.view_as_real(b))
+    dft = a2afft_mod.GridDfft(shape, "fp32")
+    dft.set_kernel(1, 1, 509, KF_DFT)
+    c, e = x.clone(), x.clone()
+    dft.forward(c)
+    whole.forward(e)
+    torch.cuda.synchronize()
+    ref = np.fft.fftn(x.cpu().numpy().astype(np.complex128))
+    assert _relerr(c.cpu().numpy(), ref) < TOL_1D["c64"] * np.sqrt(509) * 2
+    assert _relerr(e.cpu().numpy(), ref) < TOL_1D["c64"] * 3 * np.log2(509) * 2
 
 
 def test_many_windows_match_whole_passes(a2afft_mod, cuda):
