@@ -46,6 +46,14 @@ int register_kernels_f32(KEntry* out) {
     add_extra_flags<float, 2048, 16, 2, false>(out, &n);   // the contiguous default shape
     add_shape<float, 2048, 16, 1, false>(out, &n);   // 128 threads, 16 KiB LDS
     add_shape<float, 2048, 32, 8, true>(out, &n);    // 512 threads, 128 KiB LDS, 64 B rows
+    // far-strided 2048-point columns with half-width exchanges: 64 B rows in
+    // 64 KiB of LDS (16-column tiles spill at 512 or 1024 threads)
+    add_halfx<float, 2048, 32, 8, true>(out, &n);    // 512 threads
+    add_halfx<float, 2048, 64, 8, true>(out, &n);    // 256 threads, 256 VGPRs
+    add_halfx<float, 2048, 64, 16, true>(out, &n);   // 512 threads, 128 B rows, 128 KiB LDS
+    add_variant<float, 2048, 32, 8, true, KF_A2MAJ>(out, &n);
+    add_variant<float, 2048, 32, 8, true, KF_XCD | KF_A2MAJ>(out, &n);
+    add_variant<float, 2048, 32, 8, true, KF_XCD | KF_NT>(out, &n);
     // half-width exchanges for the 512 and 2048 default shapes
     add_halfx<float, 512, 32, 16, true>(out, &n);
     // hardware tile order (no XCD renumbering) with half-width exchanges

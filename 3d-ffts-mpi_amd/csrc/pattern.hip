@@ -15,7 +15,7 @@
 namespace a2a {
 
 // Workgroup bound: 1024 threads leave 128 VGPRs a lane, too few for 32
-// complex128 values.
+// complex128 or 64 complex64 values; 512 threads leave 256.
 template <typename V, int E> struct PatternBound {
     static constexpr int threads = sizeof(V) * E >= 512 ? 512 : 1024;
 };
@@ -47,14 +47,17 @@ static int lds_slots_rt(int N, int T, int strided, int pad) {
 
 template <typename V, bool NTMP>
 static const void* pattern_kernel_for(int E, int threads) {
-    const int lim = E == 32 ? PatternBound<V, 32>::threads : 1024;
-    if (threads > lim) return nullptr;
+    const int lim = E == 64 ? PatternBound<V, 64>::threads : E == 32 ? PatternBound<V, 32>::threads : 1024;
+    if (threads > lim || (E == 64 && sizeof(V) > 8)) return nullptr;
     switch (E) {
     case 2: return reinterpret_cast<const void*>(&pattern_copy_kernel<V, 2, NTMP>);
     case 4: return reinterpret_cast<const void*>(&pattern_copy_kernel<V, 4, NTMP>);
     case 8: return reinterpret_cast<const void*>(&pattern_copy_kernel<V, 8, NTMP>);
     case 16: return reinterpret_cast<const void*>(&pattern_copy_kernel<V, 16, NTMP>);
     case 32: return reinterpret_cast<const void*>(&pattern_copy_kernel<V, 32, NTMP>);
+    case 64:
+        if constexpr (sizeof(V) == 8) return reinterpret_cast<const void*>(&pattern_copy_kernel<V, 64, NTMP>);
+        return nullptr;
     default: return nullptr;
     }
 }

@@ -376,7 +376,9 @@ def test_mixed_tiles_bit_identical(a2afft_mod, cuda, case):
 HALFX_CASES = [((256, 256, 256), "fp64", [(0, 8, 16), (0, 16, 16), (1, 8, 16), (1, 4, 16), (2, 8, 16)]),
                ((512, 16, 512), "fp64", [(0, 8, 16), (1, 8, 16)]),
                ((512, 16, 512), "fp32", [(0, 16, 16), (1, 16, 32)]),
-               ((16, 16, 2048), "fp32", [(0, 2, 16)])]
+               ((16, 16, 2048), "fp32", [(0, 2, 16)]),
+               ((2048, 4, 32), "fp32", [(1, 8, 32), (1, 8, 64), (1, 16, 64)]),
+               ((4, 2048, 32), "fp32", [(2, 16, 64)])]
 HALFX_FLAGS = (32 | 1, 32 | 1 | 2, 32 | 1 | 2 | 4, 32 | 8, 32 | 8 | 1)
 
 
