@@ -21,6 +21,7 @@
 
 #include "fft.h"
 #include "remap.h"
+#include "stockham.h"
 
 namespace a2a {
 
@@ -314,15 +315,17 @@ int launch_chirp_z(void* data, int N, int64_t lines, a2afft_dtype dt, int dir, h
     return st < 0 ? st : fs;
 }
 
-// The direct DFT costs O(N) per element, the chirp-z form about 2.8 ms per GiB
-// times M/N.  Measured per GiB of complex64 lines: DFT 3.2 ms at N = 97,
-// 6.5 ms at 251, 13 ms at 509, 122 ms at 4093; chirp-z 11.0 ms at 4099
-// (M = 16384), 5.8 ms at 8191 (M = 16384).  Crossing: N^2 >= 108 M.
+// Where chirp-z beats the direct DFT.  Per GiB of complex64 lines the direct
+// DFT costs O(N): 7.7 ms at N = 11 (one workgroup per line leaves most of its
+// threads idle), 3.2 ms at 97, 13 ms at 509, 122 ms at 4093.  The one-kernel
+// convolution (M <= 4096) costs 0.9-2.1 ms at every length from 11 to 2039;
+// above that the tiled form costs about 2 ms times M / N (8.1 ms at 4093).
+// FP64 lines agree, except contiguous lines of 37-53 points, where the DFT
+// is up to 1.3x faster.  Lengths below 9 (7) stay on the DFT.
 bool chirp_z_preferred(int N) {
-    if (N < 2 || N > kBluesteinMaxN) return false;
-    int64_t M = 1;
-    while (M < 2 * int64_t(N) - 1) M *= 2;
-    return int64_t(N) * N >= 108 * M;
+    if (N < 9 || N > kBluesteinMaxN) return false;
+    const int64_t M = cz_length(N);
+    return M <= kDftMaxN || int64_t(N) * N >= 108 * M;
 }
 
 // True when n has no prime factor other than 2, 3 and 5 (a Stockham length).
@@ -341,12 +344,12 @@ int launch_long_fft(void* data, int N, int64_t lines, a2afft_dtype dt, int dir, 
                          "1d fft: length %d is neither a product of two lengths <= %d nor at most %d", N, kDftMaxN,
                          kBluesteinMaxN);
     }
-    // A factor with a large prime would run its column pass as a direct DFT,
-    // O(factor) per element (8186 = 4093 * 2: 122 ms per GiB for the 4093-point
-    // pass alone); where chirp-z beats that DFT, transform the whole line by
-    // chirp-z instead.
+    // A factor with a prime above 5 runs its column pass as a one-kernel
+    // chirp-z pass up to 2048 points.  Longer such factors (8186 = 4093 * 2)
+    // would need the tiled chirp-z form inside the four-step, with a workspace
+    // of its own: transform the whole line by chirp-z instead.
     for (const int f : {N1, N2})
-        if (!smooth235(f) && chirp_z_preferred(f)) return launch_chirp_z(data, N, lines, dt, dir, s);
+        if (!smooth235(f) && cz_length(f) > kDftMaxN) return launch_chirp_z(data, N, lines, dt, dir, s);
     const size_t esize = dt == A2AFFT_C128 ? 16 : 8;
     const LongTw* tw = long_twiddles(N, dir);
     if (!tw) return set_error(A2AFFT_E_ALLOC, "1d fft: twiddles for length %d", N);
@@ -524,6 +527,77 @@ __global__ __launch_bounds__(256) void cz_scatter_kernel(const HIP_vector_type<R
     }
 }
 
+// The whole convolution of T = 8192 / M lines per workgroup, for M <= 4096:
+// each line times the chirp into registers (zeros past N), the forward
+// M-point Stockham FFT, the product with the filter spectrum / M, the inverse
+// FFT, then times the chirp and the pass's scale back into place.  The steps
+// and their rounding are those of the tiled path (cz_mul, the Stockham
+// passes, blue_mul_kernel's product), in registers and LDS instead of a
+// workspace: one read and one write of each line.  In place: a workgroup
+// reads all its lines before it writes any, and owns them alone.
+template <typename real, int M, bool STRIDED>
+__global__ __launch_bounds__(kCzFusedThreads) void cz_fused_kernel(
+    typename CT<real>::V* __restrict__ data, FftGeom g, uint32_t units, int N,
+    const typename CT<real>::V* __restrict__ twf, const typename CT<real>::V* __restrict__ twi,
+    const double2* __restrict__ chirp, const double2* __restrict__ bhat, double inv_m) {
+    using V = typename CT<real>::V;
+    constexpr int E = kCzFusedE, T = cz_fused_lines(M), TPL = M / E;
+    static_assert(T * TPL == kCzFusedThreads, "512 threads a workgroup");
+    __shared__ V lds[lds_slots<M, T, STRIDED, 0>()];
+    int t, line;
+    cz_fused_thread(STRIDED, M, int(threadIdx.x), &t, &line);
+    const uint32_t u = blockIdx.x * uint32_t(T) + uint32_t(line);
+    const bool valid = u < units;
+    const int64_t base = valid ? cz_line_base(g, u) : 0;
+    V v[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int n = t + e * TPL;
+        if (valid && n < N) {
+            v[e] = cz_mul<real>(data[base + g.fft.off(uint32_t(n))], chirp[n], 0, 1.0);
+        } else {
+            v[e].x = real(0);
+            v[e].y = real(0);
+        }
+    }
+    stockham_passes<real, M, E, T, STRIDED, -1, 0, 0>(v, lds, twf, t, line, true);
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const double2 b = bhat[t + e * TPL];
+        const double vr = double(v[e].x), vi = double(v[e].y);
+        v[e].x = real((vr * b.x - vi * b.y) * inv_m);
+        v[e].y = real((vr * b.y + vi * b.x) * inv_m);
+    }
+    __syncthreads();   // the forward transform's last LDS reads before the inverse's first writes
+    stockham_passes<real, M, E, T, STRIDED, 1, 0, 0>(v, lds, twi, t, line, true);
+    if (!valid) return;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int k = t + e * TPL;
+        if (k < N) data[base + g.fft.off(uint32_t(k))] = cz_mul<real>(v[e], chirp[k], g.scale_mode, g.scale);
+    }
+}
+
+template <typename real, bool S>
+const void* cz_fused_for(int M) {
+    switch (M) {
+    case 32: return (const void*)&cz_fused_kernel<real, 32, S>;
+    case 64: return (const void*)&cz_fused_kernel<real, 64, S>;
+    case 128: return (const void*)&cz_fused_kernel<real, 128, S>;
+    case 256: return (const void*)&cz_fused_kernel<real, 256, S>;
+    case 512: return (const void*)&cz_fused_kernel<real, 512, S>;
+    case 1024: return (const void*)&cz_fused_kernel<real, 1024, S>;
+    case 2048: return (const void*)&cz_fused_kernel<real, 2048, S>;
+    case 4096: return (const void*)&cz_fused_kernel<real, 4096, S>;
+    default: return nullptr;
+    }
+}
+
+const void* cz_fused_fn(int M, bool dbl, int strided) {
+    if (dbl) return strided ? cz_fused_for<double, true>(M) : cz_fused_for<double, false>(M);
+    return strided ? cz_fused_for<float, true>(M) : cz_fused_for<float, false>(M);
+}
+
 // The M-point FFT is a rectangular four-step (M = 8192 = 128 * 64), which
 // needs a workspace of its own.
 bool long_needs_work(int M) {
@@ -566,6 +640,7 @@ size_t chirp_pass_work_bytes(int N, a2afft_dtype dt, uint64_t lines) {
     if (!chirp_pass_ok(N)) return 0;
     const size_t esize = size_t(esize_of(dt));
     const int M = cz_length(N);
+    if (M <= kDftMaxN) return 0;   // one kernel, no workspace
     const uint64_t chunk = std::min<uint64_t>(lines, uint64_t(chirp_chunk_lines(N, esize)));
     return size_t(chunk) * size_t(M) * esize * (long_needs_work(M) ? 2 : 1);
 }
@@ -589,6 +664,23 @@ int launch_chirp_pass(const FftGeom& g, uint32_t units, int strided, void* data,
     const BluePlan* bp = blue_plan(N, dir, s);
     if (!bp) return set_error(A2AFFT_E_ALLOC, "chirp-z pass: tables for length %d", N);
     const int M = bp->M;
+    const bool dbl = dt == A2AFFT_C128;
+    if (M <= kDftMaxN) {   // the whole convolution in one kernel
+        const void* fn = cz_fused_fn(M, dbl, strided);
+        const void* twf = twiddle_table(M, -1, dt);
+        const void* twi = twiddle_table(M, 1, dt);
+        if (!fn || !twf || !twi) return set_error(A2AFFT_E_ALLOC, "chirp-z pass: tables for length %d", N);
+        const uint32_t T = uint32_t(cz_fused_lines(M)), grid = (units + T - 1) / T;
+        const double2* ch = bp->chirp;
+        const double2* bh = bp->bhat;
+        double inv_m = 1.0 / double(M);
+        int n = N;
+        uint32_t nu = units;
+        void* args[] = {&data, (void*)&g, &nu, &n, (void*)&twf, (void*)&twi, (void*)&ch, (void*)&bh, &inv_m};
+        A2A_TRY(hip_check(int(hipLaunchKernel(fn, dim3(grid), dim3(kCzFusedThreads), args, 0, s)),
+                          "chirp-z pass launch"));
+        return hip_check(int(hipGetLastError()), "chirp-z pass launch");
+    }
     const size_t esize = size_t(esize_of(dt));
     const size_t rows = size_t(M) * esize;                          // bytes of one workspace row
     const size_t per_line = rows * (long_needs_work(M) ? 2 : 1);   // with the four-step's share
@@ -604,7 +696,6 @@ int launch_chirp_pass(const FftGeom& g, uint32_t units, int strided, void* data,
     void* work2 = per_line > rows ? static_cast<char*>(work) + size_t(chunk) * rows : nullptr;
     const size_t work2_bytes = per_line > rows ? size_t(chunk) * rows : 0;
     const uint32_t etiles = uint32_t(M / kCzTM), ktiles = uint32_t((N + kCzTM - 1) / kCzTM);
-    const bool dbl = dt == A2AFFT_C128;
     int st = A2AFFT_OK;
     for (int64_t l0 = 0; l0 < int64_t(units) && st >= 0; l0 += chunk) {
         const uint32_t cl = uint32_t(std::min<int64_t>(chunk, int64_t(units) - l0));

@@ -184,6 +184,17 @@ A2A_HD void cz_slot(bool lines_fast, int i, int* tl, int* te) {
         *te = i % kCzTM;
     }
 }
+// Convolutions of M <= kDftMaxN points run in one kernel (cz_fused_kernel):
+// a workgroup of 512 threads holds kCzFusedE elements of each of
+// T = 8192 / M lines in registers, thread tid taking element t + e * (M / E)
+// of line `line` of its tile as below (the Stockham kernels' lane orders).
+constexpr int kCzFusedE = 16, kCzFusedThreads = 512;
+A2A_HD constexpr int cz_fused_lines(int M) { return (kCzFusedThreads * kCzFusedE) / M; }
+A2A_HD void cz_fused_thread(bool strided, int M, int tid, int* t, int* line) {
+    const int T = cz_fused_lines(M), TPL = M / kCzFusedE;
+    *t = strided ? tid / T : tid % TPL;
+    *line = strided ? tid % T : tid / TPL;
+}
 // The convolution length: the power of two >= 2N - 1.
 A2A_HD int cz_length(int N) {
     int M = 1;

@@ -351,4 +351,45 @@ int64_t hc_cz_enumerate(const int* n3, const int* dims, int pass, int side, uint
     return c;
 }
 
+// Every element the single-kernel chirp-z pass (cz_fused_kernel, M <= 4096)
+// loads and stores: pass `pass` of an n3 grid over `dims` virtual ranks,
+// window [a2_start, a2_start + a2_count) of its a2 axis.  Per element: its
+// grid address, its index n along the line, and the window line u.  Returns
+// the count, or a negative status.
+int64_t hc_cz_fused_enumerate(const int* n3, const int* dims, int pass, uint32_t a2_start, uint32_t a2_count,
+                              int64_t* addr, int32_t* n_out, int64_t* line_out, int64_t cap) {
+    int lgs[3];
+    for (int i = 0; i < 3; ++i) lgs[i] = n3[i] / dims[i];
+    const int64_t nlocal = int64_t(lgs[0]) * lgs[1] * lgs[2];
+    PassMaps all[3];
+    fused_pass_maps(n3, dims, lgs, nlocal, all);
+    const PassMaps& pm = all[pass];
+    const int N = int(pm.fft.n), M = cz_length(N);
+    if (M > kDftMaxN || M < kCzTM) return -101;
+    FftGeom g;
+    uint32_t units;
+    const int st = resolve_geom(pm.fft, pm.a1, pm.a2, 0, 1, &g, &units, a2_start, a2_count);
+    if (st < 0) return st;
+    const uint32_t T = uint32_t(cz_fused_lines(M)), TPL = uint32_t(M / kCzFusedE);
+    const uint32_t grid = (units + T - 1) / T;
+    int64_t c = 0;
+    for (uint32_t b = 0; b < grid; ++b)
+        for (int tid = 0; tid < kCzFusedThreads; ++tid) {
+            int t, line;
+            cz_fused_thread(pm.strided != 0, M, tid, &t, &line);
+            const uint32_t u = b * T + uint32_t(line);
+            if (u >= units) continue;
+            for (int e = 0; e < kCzFusedE; ++e) {
+                const int n = t + e * int(TPL);
+                if (n >= N) continue;
+                if (c >= cap) return -100;
+                addr[c] = cz_line_base(g, u) + g.fft.off(uint32_t(n));
+                n_out[c] = n;
+                line_out[c] = u;
+                ++c;
+            }
+        }
+    return c;
+}
+
 }  // extern "C"

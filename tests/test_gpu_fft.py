@@ -93,8 +93,9 @@ def test_long_mixed_impulses(a2afft_mod, cuda):
         assert np.abs(d.cpu().numpy()[0] - want).max() < 1e-13, N
 
 
-# Lengths with other prime factors (7, 11, 13, 97): the direct O(N^2) DFT,
-# whose rounding grows like sqrt(N) rather than log2(N).
+# Lengths with other prime factors (7, 11, 13, 97): the direct O(N^2) DFT
+# below 9 points, whose rounding grows like sqrt(N), and chirp-z convolutions
+# above (two power-of-two FFTs: log2 N times a few).
 DIRECT = [7, 11, 14, 49, 63, 97, 4095]
 
 
@@ -106,7 +107,7 @@ def test_direct_dft_lengths_vs_oracle(a2afft_mod, cuda, N, dtype):
     lines = 5
     x = rng.standard_normal((lines, N)) + 1j * rng.standard_normal((lines, N))
     x = x.astype(np.complex128 if dtype == "c128" else np.complex64)
-    tol = TOL_1D[dtype] * np.sqrt(N)
+    tol = TOL_1D[dtype] * max(np.sqrt(N), 3 * np.log2(N))
     for sign, fn in ((-1, a2afft_mod.forward_1d_fft), (1, a2afft_mod.inverse_1d_fft)):
         ref = O.fft_batched(x.astype(np.complex128), sign)
         d = torch.from_numpy(x).cuda()

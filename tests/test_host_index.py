@@ -48,6 +48,9 @@ def hc(tmp_path_factory):
     L.hc_cz_enumerate.argtypes = [P, P, ctypes.c_int, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32,
                                   ctypes.c_int64, P, P, P, P, P, ctypes.c_int64]
     L.hc_cz_enumerate.restype = ctypes.c_int64
+    L.hc_cz_fused_enumerate.argtypes = [P, P, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, P, P, P,
+                                        ctypes.c_int64]
+    L.hc_cz_fused_enumerate.restype = ctypes.c_int64
     return L
 
 
@@ -418,3 +421,42 @@ def test_chirp_pass_addresses(hc, cfg):
                             assert (w[full, :r] == 1).all() and (w[full, r:] == 0).all()
                     out[side] = dict(zip(addr.tolist(), work.tolist()))
                 assert out[0] == out[1], (pass_, chunk)
+
+
+@pytest.mark.parametrize("cfg", CZ_GRIDS + [((251, 6, 20), (1, 2, 2)), ((9, 37, 1000), (1, 1, 2))],
+                         ids=lambda c: f"{c[0]}_{c[1]}")
+def test_chirp_fused_addresses(hc, cfg):
+    """The single-kernel chirp-z pass (M <= 4096): every element of the pass
+    (or of a z window) loaded and stored exactly once by the thread the
+    kernel's lane order gives it, element n of a line at coordinate n along the
+    axis, each line one global line."""
+    shape, dims = cfg
+    n3 = np.array(shape, dtype=np.int32)
+    dm = np.array(dims, dtype=np.int32)
+    P = int(np.prod(dims))
+    lgs = [s // d for s, d in zip(shape, dims)]
+    total = P * lgs[0] * lgs[1] * lgs[2]
+    g = np.arange(total, dtype=np.int64).reshape(shape)
+    blocks = g.reshape(dims[0], lgs[0], dims[1], lgs[1], dims[2], lgs[2]).transpose(0, 2, 4, 1, 3, 5).reshape(-1)
+    for pass_, axis in ((0, 2), (1, 0), (2, 1)):
+        N = shape[axis]
+        if N < 9:
+            continue
+        other = [a for a in range(3) if a != axis]
+        for start, count in ([(0, 0)] + ([(2, shape[2] - 3)] if pass_ > 0 else [])):
+            cnt = count if count else shape[2]
+            cap = (total // shape[2]) * cnt if count else total
+            addr = np.empty(cap, np.int64)
+            n = np.empty(cap, np.int32)
+            line = np.empty(cap, np.int64)
+            c = hc.hc_cz_fused_enumerate(_p(n3), _p(dm), pass_, start, count, _p(addr), _p(n), _p(line), cap)
+            assert c == cap, (pass_, c)
+            if not count:
+                assert np.array_equal(np.sort(addr), np.arange(total)), pass_
+            else:
+                assert np.unique(addr).size == cap
+            coords = np.stack(np.unravel_index(blocks[addr], shape))
+            assert np.array_equal(coords[axis], n), pass_
+            key = coords[other[0]] * 4096 + coords[other[1]]
+            ks = key[np.argsort(line, kind="stable")].reshape(-1, N)
+            assert (ks == ks[:, :1]).all(), pass_
