@@ -280,8 +280,6 @@ struct a2afft_plan_s {
     void* graph_data[2] = {nullptr, nullptr};        // the buffer each graph was captured for
     std::vector<Launch> launches;             // launch list for `launches_chunk`
     uint32_t launches_chunk = ~0u;
-    void* cz_work = nullptr;    // chirp-z passes' workspace (allocated at plan time)
-    size_t cz_bytes = 0;
 };
 
 // Graphs hold the kernel choices and arguments of their capture: any change
@@ -377,37 +375,15 @@ static bool sel_allowed(const a2afft_plan_s* p, int i, const KSel& s) {
     return false;
 }
 
-// Grows the chirp-z workspace to `need` bytes and hands it to the fused
-// passes.  A new buffer drops the graphs captured with the old one.
-static int grow_workspace(a2afft_plan_s* p, size_t need) {
-    if (need > p->cz_bytes) {
-        void* w = nullptr;
-        A2A_TRY(hip_check(int(hipMalloc(&w, need)), "plan: chirp-z workspace"));
-        if (p->cz_work) (void)hipFree(p->cz_work);
-        p->cz_work = w;
-        p->cz_bytes = need;
-        drop_plan_graphs(p);
-    }
-    for (int i = 0; i < 3; ++i) {
-        p->pass[i].work = p->cz_work;
-        p->pass[i].work_bytes = p->cz_bytes;
-    }
-    return A2AFFT_OK;
-}
-
-// Builds the twiddle tables selection `s` of pass i reads, both directions,
-// and for a chirp-z pass its workspace, so that execution allocates nothing.
-static int sel_tables(a2afft_plan_s* p, int i, const KSel& s) {
+// Builds the twiddle tables selection `s` of pass i reads (for a chirp-z
+// pass its chirp, filter spectrum and M-point tables), both directions, so
+// that execution allocates nothing.
+static int sel_tables(const a2afft_plan_s* p, int i, const KSel& s) {
     const int N = int(p->pass[i].fft.n);
     const bool ok = is_split(p->prec)
                         ? (mfma_twiddle_table(N, -1) && mfma_twiddle_table(N, 1))
                         : (pass_twiddles(p->pass[i], N, -1, p->d.dtype, s) && pass_twiddles(p->pass[i], N, 1, p->d.dtype, s));
-    if (!ok) return set_error(A2AFFT_E_ALLOC, "plan: twiddle tables");
-    if (!is_split(p->prec) && (s.flags & KF_CZ)) {
-        const AxisPass& ps = p->pass[i];
-        A2A_TRY(grow_workspace(p, chirp_pass_work_bytes(N, p->d.dtype, uint64_t(ps.a1.n) * ps.a2.n)));
-    }
-    return A2AFFT_OK;
+    return ok ? A2AFFT_OK : set_error(A2AFFT_E_ALLOC, "plan: twiddle tables");
 }
 
 // Threads per workgroup that selection `s` of pass i launches.
@@ -456,8 +432,6 @@ static int plan_init(a2afft_plan_s* p) {
         const KSel bs = fft_default_sel(bp, p->d.dtype, p->d.Ng);
         if (!pass_twiddles(bp, p->d.Ng, -1, p->d.dtype, bs) || !pass_twiddles(bp, p->d.Ng, 1, p->d.dtype, bs))
             return set_error(A2AFFT_E_ALLOC, "plan: twiddle tables");
-        if (bs.flags & KF_CZ)
-            A2A_TRY(grow_workspace(p, chirp_pass_work_bytes(p->d.Ng, p->d.dtype, uint64_t(total / p->d.Ng))));
     }
     p->chunk = default_chunk(p);
     if (p->chunk && is_split(p->prec))   // the windowed choice of the strided passes
@@ -684,9 +658,7 @@ static int exec_staged(a2afft_plan_s* p, void* D, int dir, hipStream_t s) {
     const int64_t total = int64_t(d->P) * d->nlocal;
     auto fft = [&](void* buf) -> int {
         const PassMaps pm = batched_pass_maps(d->Ng, uint32_t(total / d->Ng));
-        AxisPass ps{pm.fft, pm.a1, pm.a2, 0, 0, 1.0};
-        ps.work = p->cz_work;   // chirp-z lengths: the plan's workspace
-        ps.work_bytes = p->cz_bytes;
+        const AxisPass ps{pm.fft, pm.a1, pm.a2, 0, 0, 1.0};
         return launch_fft_pass(ps, buf, d->dtype, d->Ng, dir, s, nullptr);
     };
     int warn = 0;
@@ -1011,7 +983,6 @@ extern "C" int a2afft_plan_destroy(a2afft_plan p) {
     if (p) {
         drop_plan_graphs(p);
         if (p->cap_stream) (void)hipStreamDestroy(p->cap_stream);
-        if (p->cz_work) (void)hipFree(p->cz_work);
     }
     delete p;
     return A2AFFT_OK;

@@ -23,8 +23,6 @@ struct AxisPass {
     double scale;
     uint32_t a2_start = 0;   // window along a2 (count 0 = whole axis)
     uint32_t a2_count = 0;
-    void* work = nullptr;    // chirp-z passes: a workspace the caller owns (null: allocated per call)
-    size_t work_bytes = 0;
 };
 
 // Supported transform lengths: every N in 2..4096.  Fixed-schedule Stockham
@@ -53,42 +51,31 @@ int launch_fft_pass(const AxisPass& pass, void* data, a2afft_dtype dtype, int N,
 int launch_pattern_copy(const AxisPass& pass, void* data, a2afft_dtype dtype, int N, const KSel& sel, hipStream_t s);
 
 // In-place DFTs of `lines` contiguous lines of N points, any supported N
-// (the batched 1D launchers' dispatcher).  work/work_bytes: a workspace for
-// the lengths that need one (four-step, chirp-z); null = allocated per call.
-int fft_lines(void* data, int N, int64_t lines, a2afft_dtype dtype, int dir, hipStream_t s, void* work = nullptr,
-              size_t work_bytes = 0);
+// (the batched 1D launchers' dispatcher).
+int fft_lines(void* data, int N, int64_t lines, a2afft_dtype dtype, int dir, hipStream_t s);
 
 // Batched contiguous lines longer than 4096 points (fft_long.hip): four-step
 // over N = N1 * N2 with both factors <= 4096 (long_fft_split), through a
 // workspace from the stream-ordered allocator; lengths with no such split up
 // to kBluesteinMaxN as Bluestein's chirp-z convolution of power-of-two length.
 constexpr int kBluesteinMaxN = 1 << 23;
-// The chirp-z form for any N <= kBluesteinMaxN, and whether it beats the
-// direct DFT at a length the Stockham kernels do not cover.
+// The chirp-z form for lines of any N <= kBluesteinMaxN.
 int launch_chirp_z(void* data, int N, int64_t lines, a2afft_dtype dtype, int dir, hipStream_t s);
-bool chirp_z_preferred(int N);
-// work/work_bytes: the rectangular four-step's workspace (null = allocated
-// per call from the stream-ordered allocator).
-int launch_long_fft(void* data, int N, int64_t lines, a2afft_dtype dtype, int dir, hipStream_t s,
-                    void* work = nullptr, size_t work_bytes = 0);
+int launch_long_fft(void* data, int N, int64_t lines, a2afft_dtype dtype, int dir, hipStream_t s);
 
-// Chirp-z passes (KF_CZ) over any axis pass geometry, N <= kDftMaxN: the
-// lines are gathered into power-of-two rows of a workspace, convolved there
-// with two M-point FFTs and scattered back (fft_long.hip).
-bool chirp_pass_ok(int N);   // N in 9..kDftMaxN (M >= 32)
-// Workspace bytes a chirp-z pass over `lines` lines uses (chunks of at most
-// 256 MiB of rows; twice that where the M-point FFT is a rectangular four-step).
-size_t chirp_pass_work_bytes(int N, a2afft_dtype dtype, uint64_t lines);
+// Chirp-z passes (KF_CZ) over any axis pass geometry, N in 9..kDftMaxN: one
+// kernel per pass convolves each line with two power-of-two FFTs of
+// M = cz_length(N) <= 8192 points in registers and LDS (fft_long.hip).
+bool chirp_pass_ok(int N);
 // Builds every table a chirp-z pass of length N reads, both directions, and
 // returns the chirp of direction `dir` (the pass's table), or null.
 const void* chirp_pass_tables(int N, int dir, a2afft_dtype dtype);
-// The gather kernel (the pass's first launch), for dispatch checks.
-const void* chirp_pass_kernel(bool dbl);
+// The pass's kernel for (N, precision, layout), or null.
+const void* chirp_pass_kernel(int N, bool dbl, int strided);
 // Runs a resolved chirp-z pass: `units` lines of geometry g (strided: adjacent
-// lines adjacent in memory) in place on `data`, in chunks through `work`
-// (null: allocated from the stream-ordered allocator for this call).
+// lines adjacent in memory) in place on `data`.
 int launch_chirp_pass(const FftGeom& g, uint32_t units, int strided, void* data, a2afft_dtype dtype, int N, int dir,
-                      hipStream_t s, void* work, size_t work_bytes);
+                      hipStream_t s);
 
 // Stockham FFT with the radix schedule in the kernel arguments (KF_MIX,
 // fft_mixed.hip) for 2^a 3^b 5^c lengths the fixed schedules do not cover.

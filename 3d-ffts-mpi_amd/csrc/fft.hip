@@ -27,11 +27,15 @@ static KSel cz_sel(int N) { return KSel{1, N, KF_CZ}; }
 
 // Kernel choice for a length no fixed-schedule kernel covers: the
 // runtime-schedule Stockham when its shape fits the pass, else the chirp-z
-// pass where it beats the direct DFT, else the direct DFT.
+// pass (9 points and up), else the direct DFT.  Per GiB of complex64 lines the
+// direct DFT costs O(N): 7.7 ms at N = 11 (one workgroup per line leaves most
+// of its threads idle), 3.3 ms at 97, 13 ms at 509, 124 ms at 4093; the
+// chirp-z pass 0.9-2.1 ms at every length from 11 to 4093.  complex128 agrees,
+// except contiguous lines of 37-53 points, where the DFT is up to 1.3x faster.
 static KSel fallback_sel(const AxisPass& p, a2afft_dtype dt, int N) {
     MixedPlan mp;
     if (mixed_plan(p, N, dt == A2AFFT_C128, &mp)) return KSel{mp.T, mp.E, KF_MIX};
-    if (chirp_pass_ok(N) && chirp_z_preferred(N)) return cz_sel(N);
+    if (chirp_pass_ok(N)) return cz_sel(N);
     return dft_sel(N);
 }
 
@@ -115,8 +119,7 @@ const void* find_kernel(int N, bool dbl, int strided, const KSel& s, int dir) {
     if (is_mix(s)) return s.flags == KF_MIX ? mixed_kernel_for(N, s.T, s.E, dbl, strided, dir) : nullptr;
     if (is_dft(s)) return (s.T == 1 && s.E == N && s.flags == KF_DFT && fft_length_supported(N)) ? dft_kernel_for(dbl, dir)
                                                                                                     : nullptr;
-    if (is_cz(s)) return (s.T == 1 && s.E == N && s.flags == KF_CZ && chirp_pass_ok(N)) ? chirp_pass_kernel(dbl)
-                                                                                        : nullptr;
+    if (is_cz(s)) return (s.T == 1 && s.E == N && s.flags == KF_CZ) ? chirp_pass_kernel(N, dbl, strided) : nullptr;
     init_registry();
     for (int i = 0; i < g_nkernels; ++i) {
         const KEntry& k = g_kernels[i];
@@ -282,7 +285,7 @@ int launch_fft_pass(const AxisPass& p, void* data, a2afft_dtype dt, int N, int d
                                            size_t(mixed_lds_bytes(mp, dt == A2AFFT_C128)), s)),
                        "mixed-radix launch");
     } else if (is_cz(ks)) {
-        st = launch_chirp_pass(g, grid, p.strided, data, dt, N, dir, s, p.work, p.work_bytes);
+        st = launch_chirp_pass(g, grid, p.strided, data, dt, N, dir, s);
     } else if (is_dft(ks)) {
         const size_t line = size_t(N) * size_t(esize_of(dt));
         const size_t lds = 2 * line <= 65536 ? 2 * line : line;
@@ -401,14 +404,11 @@ const void* pass_twiddles(const AxisPass& p, int N, int dir, a2afft_dtype dt, co
 using namespace a2a;
 
 namespace a2a {
-int fft_lines(void* data, int N, int64_t lines, a2afft_dtype dtype, int dir, hipStream_t s, void* work,
-              size_t work_bytes) {
+int fft_lines(void* data, int N, int64_t lines, a2afft_dtype dtype, int dir, hipStream_t s) {
     if (lines >= (int64_t(1) << 32)) return set_error(A2AFFT_E_SIZE, "1d fft: too many lines");
-    if (N > kDftMaxN) return launch_long_fft(data, N, lines, dtype, dir, s, work, work_bytes);
+    if (N > kDftMaxN) return launch_long_fft(data, N, lines, dtype, dir, s);
     const PassMaps pm = batched_pass_maps(N, uint32_t(lines));
-    AxisPass p{pm.fft, pm.a1, pm.a2, 0, 0, 1.0};
-    p.work = work;   // lengths only the direct DFT covers run as chirp-z where it is faster
-    p.work_bytes = work_bytes;
+    const AxisPass p{pm.fft, pm.a1, pm.a2, 0, 0, 1.0};
     return launch_fft_pass(p, data, dtype, N, dir, s, nullptr);
 }
 }  // namespace a2a

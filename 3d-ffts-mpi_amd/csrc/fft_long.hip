@@ -315,28 +315,7 @@ int launch_chirp_z(void* data, int N, int64_t lines, a2afft_dtype dt, int dir, h
     return st < 0 ? st : fs;
 }
 
-// Where chirp-z beats the direct DFT.  Per GiB of complex64 lines the direct
-// DFT costs O(N): 7.7 ms at N = 11 (one workgroup per line leaves most of its
-// threads idle), 3.2 ms at 97, 13 ms at 509, 122 ms at 4093.  The one-kernel
-// convolution (M <= 4096) costs 0.9-2.1 ms at every length from 11 to 2039;
-// above that the tiled form costs about 2 ms times M / N (8.1 ms at 4093).
-// FP64 lines agree, except contiguous lines of 37-53 points, where the DFT
-// is up to 1.3x faster.  Lengths below 9 (7) stay on the DFT.
-bool chirp_z_preferred(int N) {
-    if (N < 9 || N > kBluesteinMaxN) return false;
-    const int64_t M = cz_length(N);
-    return M <= kDftMaxN || int64_t(N) * N >= 108 * M;
-}
-
-// True when n has no prime factor other than 2, 3 and 5 (a Stockham length).
-static bool smooth235(int n) {
-    for (int f : {2, 3, 5})
-        while (n % f == 0) n /= f;
-    return n == 1;
-}
-
-int launch_long_fft(void* data, int N, int64_t lines, a2afft_dtype dt, int dir, hipStream_t s, void* ext_work,
-                    size_t ext_bytes) {
+int launch_long_fft(void* data, int N, int64_t lines, a2afft_dtype dt, int dir, hipStream_t s) {
     int N1 = 0, N2 = 0;
     if (!long_fft_split(N, &N1, &N2)) {
         if (N <= kBluesteinMaxN) return launch_chirp_z(data, N, lines, dt, dir, s);
@@ -344,12 +323,9 @@ int launch_long_fft(void* data, int N, int64_t lines, a2afft_dtype dt, int dir, 
                          "1d fft: length %d is neither a product of two lengths <= %d nor at most %d", N, kDftMaxN,
                          kBluesteinMaxN);
     }
-    // A factor with a prime above 5 runs its column pass as a one-kernel
-    // chirp-z pass up to 2048 points.  Longer such factors (8186 = 4093 * 2)
-    // would need the tiled chirp-z form inside the four-step, with a workspace
-    // of its own: transform the whole line by chirp-z instead.
-    for (const int f : {N1, N2})
-        if (!smooth235(f) && cz_length(f) > kDftMaxN) return launch_chirp_z(data, N, lines, dt, dir, s);
+    // A factor with a prime above 5 runs its column pass as a chirp-z pass
+    // (8194 = 241 * 34: both factors).  A direct DFT factor (7) would cost
+    // O(7) per element, which the four-step absorbs.
     const size_t esize = dt == A2AFFT_C128 ? 16 : 8;
     const LongTw* tw = long_twiddles(N, dir);
     if (!tw) return set_error(A2AFFT_E_ALLOC, "1d fft: twiddles for length %d", N);
@@ -373,15 +349,9 @@ int launch_long_fft(void* data, int N, int64_t lines, a2afft_dtype dt, int dir, 
         }
         return A2AFFT_OK;
     }
-    const int64_t line_bytes = int64_t(N) * int64_t(esize);
-    int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(lines, kLongChunkBytes / line_bytes));
-    void* work = ext_work;
-    if (ext_work) {
-        chunk = std::min<int64_t>(lines, int64_t(ext_bytes) / line_bytes);
-        if (chunk < 1) return set_error(A2AFFT_E_INVALID_ARG, "1d fft: workspace below one line");
-    } else {
-        A2A_TRY(hip_check(int(hipMallocAsync(&work, size_t(chunk) * N * esize, s)), "1d fft: workspace"));
-    }
+    const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(lines, kLongChunkBytes / (int64_t(N) * esize)));
+    void* work = nullptr;
+    A2A_TRY(hip_check(int(hipMallocAsync(&work, size_t(chunk) * N * esize, s)), "1d fft: workspace"));
     const uint32_t tn2s = uint32_t((N2 + 31) / 32), tk1s = uint32_t((N1 + 31) / 32);
     int st = A2AFFT_OK;
     for (int64_t l0 = 0; l0 < lines && st >= 0; l0 += chunk) {
@@ -416,20 +386,16 @@ int launch_long_fft(void* data, int N, int64_t lines, a2afft_dtype dt, int dir, 
         // 4. back in place
         st = launch_copy(D, work, int64_t(cl) * N, dt, s);
     }
-    if (ext_work) return st;
     const int fs = hip_check(int(hipFreeAsync(work, s)), "1d fft: workspace free");
     return st < 0 ? st : fs;
 }
 
 
 // ---------------------------------------------------------------------------
-// Chirp-z passes (KF_CZ) over any axis pass geometry: launch_chirp_z's form
-// for plan passes (and for batched lines up to kDftMaxN), O(N log N) where
-// the direct DFT costs O(N) per element.  Per chunk of lines: gather each
-// line times the chirp into a zero-padded row of M, forward M-point FFT,
-// product with the filter spectrum / M, inverse M-point FFT, then times the
-// chirp (and the pass's scale) back into place.  Every line takes the same
-// arithmetic in every layout and chunking, so fused and staged plans agree
+// Chirp-z passes (KF_CZ) over any axis pass geometry, N <= kDftMaxN:
+// launch_chirp_z's convolution for plan passes and batched lines, in one
+// kernel, O(N log N) where the direct DFT costs O(N) per element.  Every line
+// takes the same arithmetic in every layout, so fused and staged plans agree
 // bit for bit.
 // ---------------------------------------------------------------------------
 namespace {
@@ -453,88 +419,13 @@ __device__ inline HIP_vector_type<R, 2> cz_mul(HIP_vector_type<R, 2> v, double2 
     return o;
 }
 
-// Workspace rows [0, nl) x [0, M) of the chunk from line l0: one tile of
-// kCzTL lines x kCzTM elements per workgroup (cz_slot orders); tiles past
-// element N only write the zero padding.
-template <typename R>
-__global__ __launch_bounds__(256) void cz_gather_kernel(const HIP_vector_type<R, 2>* __restrict__ x,
-                                                        HIP_vector_type<R, 2>* __restrict__ work, FftGeom g,
-                                                        uint32_t l0, uint32_t nl, int N, int M, uint32_t etiles,
-                                                        const double2* __restrict__ chirp, int strided) {
-    using V = HIP_vector_type<R, 2>;
-    __shared__ V tile[kCzTL][kCzTM + 1];
-    const uint32_t lt = blockIdx.x / etiles, et = blockIdx.x - lt * etiles;
-    const uint32_t lb = lt * uint32_t(kCzTL);
-    const int m0 = int(et) * kCzTM;
-    V zero;
-    zero.x = R(0);
-    zero.y = R(0);
-    if (m0 >= N) {   // padding only: the whole workgroup returns here, before any barrier
-        for (int i = int(threadIdx.x); i < kCzTL * kCzTM; i += 256) {
-            int tl, te;
-            cz_slot(false, i, &tl, &te);
-            const uint32_t line = lb + uint32_t(tl);
-            if (line < nl) work[size_t(line) * size_t(M) + size_t(m0 + te)] = zero;
-        }
-        return;
-    }
-    for (int i = int(threadIdx.x); i < kCzTL * kCzTM; i += 256) {
-        int tl, te;
-        cz_slot(strided != 0, i, &tl, &te);
-        const uint32_t line = lb + uint32_t(tl);
-        const int n = m0 + te;
-        if (line < nl && n < N)
-            tile[tl][te] = cz_mul<R>(x[cz_line_base(g, l0 + line) + g.fft.off(uint32_t(n))], chirp[n], 0, 1.0);
-    }
-    __syncthreads();
-    for (int i = int(threadIdx.x); i < kCzTL * kCzTM; i += 256) {
-        int tl, te;
-        cz_slot(false, i, &tl, &te);
-        const uint32_t line = lb + uint32_t(tl);
-        const int m = m0 + te;
-        if (line < nl) work[size_t(line) * size_t(M) + size_t(m)] = m < N ? tile[tl][te] : zero;
-    }
-}
-
-// Elements 0..N-1 of the workspace rows back into the lines, times the chirp
-// and the pass's scale.
-template <typename R>
-__global__ __launch_bounds__(256) void cz_scatter_kernel(const HIP_vector_type<R, 2>* __restrict__ work,
-                                                         HIP_vector_type<R, 2>* __restrict__ x, FftGeom g,
-                                                         uint32_t l0, uint32_t nl, int N, int M, uint32_t ktiles,
-                                                         const double2* __restrict__ chirp, int strided) {
-    using V = HIP_vector_type<R, 2>;
-    __shared__ V tile[kCzTL][kCzTM + 1];
-    const uint32_t lt = blockIdx.x / ktiles, kt = blockIdx.x - lt * ktiles;
-    const uint32_t lb = lt * uint32_t(kCzTL);
-    const int k0 = int(kt) * kCzTM;
-    for (int i = int(threadIdx.x); i < kCzTL * kCzTM; i += 256) {
-        int tl, te;
-        cz_slot(false, i, &tl, &te);
-        const uint32_t line = lb + uint32_t(tl);
-        const int k = k0 + te;
-        if (line < nl && k < N) tile[tl][te] = work[size_t(line) * size_t(M) + size_t(k)];
-    }
-    __syncthreads();
-    for (int i = int(threadIdx.x); i < kCzTL * kCzTM; i += 256) {
-        int tl, te;
-        cz_slot(strided != 0, i, &tl, &te);
-        const uint32_t line = lb + uint32_t(tl);
-        const int k = k0 + te;
-        if (line < nl && k < N)
-            x[cz_line_base(g, l0 + line) + g.fft.off(uint32_t(k))] =
-                cz_mul<R>(tile[tl][te], chirp[k], g.scale_mode, g.scale);
-    }
-}
-
-// The whole convolution of T = 8192 / M lines per workgroup, for M <= 4096:
+// The whole convolution of T = 8192 / M lines per workgroup (M <= 8192):
 // each line times the chirp into registers (zeros past N), the forward
-// M-point Stockham FFT, the product with the filter spectrum / M, the inverse
-// FFT, then times the chirp and the pass's scale back into place.  The steps
-// and their rounding are those of the tiled path (cz_mul, the Stockham
-// passes, blue_mul_kernel's product), in registers and LDS instead of a
-// workspace: one read and one write of each line.  In place: a workgroup
-// reads all its lines before it writes any, and owns them alone.
+// M-point Stockham FFT, the product with the filter spectrum / M (FP64,
+// rounded once, as blue_mul_kernel), the inverse FFT, then times the chirp and
+// the pass's scale back into place.  One read and one write of each line, no
+// workspace.  In place: a workgroup reads all its lines before it writes any,
+// and owns them alone.
 template <typename real, int M, bool STRIDED>
 __global__ __launch_bounds__(kCzFusedThreads) void cz_fused_kernel(
     typename CT<real>::V* __restrict__ data, FftGeom g, uint32_t units, int N,
@@ -589,149 +480,50 @@ const void* cz_fused_for(int M) {
     case 1024: return (const void*)&cz_fused_kernel<real, 1024, S>;
     case 2048: return (const void*)&cz_fused_kernel<real, 2048, S>;
     case 4096: return (const void*)&cz_fused_kernel<real, 4096, S>;
+    case 8192: return (const void*)&cz_fused_kernel<real, 8192, S>;
     default: return nullptr;
     }
-}
-
-const void* cz_fused_fn(int M, bool dbl, int strided) {
-    if (dbl) return strided ? cz_fused_for<double, true>(M) : cz_fused_for<double, false>(M);
-    return strided ? cz_fused_for<float, true>(M) : cz_fused_for<float, false>(M);
-}
-
-// The M-point FFT is a rectangular four-step (M = 8192 = 128 * 64), which
-// needs a workspace of its own.
-bool long_needs_work(int M) {
-    int a = 0, b = 0;
-    return M > kDftMaxN && long_fft_split(M, &a, &b) && a != b;
-}
-
-// Lines per chunk: 256 MiB of rows, a multiple of the tile height.
-int64_t chirp_chunk_lines(int N, size_t esize) {
-    const int64_t row = int64_t(cz_length(N)) * int64_t(esize);
-    return std::max<int64_t>(kCzTL, (kLongChunkBytes / row) / kCzTL * kCzTL);
-}
-
-// The tables of contiguous M-point lines (power of two, <= 2 kDftMaxN), both
-// directions: the Stockham table, or the four-step's twiddles and its two
-// column passes' tables.
-bool lines_tables(int M, a2afft_dtype dt) {
-    if (M <= kDftMaxN) {
-        const PassMaps pm = batched_pass_maps(M, 1);
-        const AxisPass p{pm.fft, pm.a1, pm.a2, 0, 0, 1.0};
-        const KSel k = fft_default_sel(p, dt, M);
-        return pass_twiddles(p, M, -1, dt, k) && pass_twiddles(p, M, 1, dt, k);
-    }
-    int N1 = 0, N2 = 0;
-    if (!long_fft_split(M, &N1, &N2) || !long_twiddles(M, -1) || !long_twiddles(M, 1)) return false;
-    const uint32_t u1 = uint32_t(N1), u2 = uint32_t(N2);
-    const AxisPass p1{make_axis(u1, u1, 0, N2), make_axis(1, 1, 0, M), make_axis(u2, u2, 0, 1), 1, 0, 1.0};
-    const AxisPass p3{make_axis(u2, u2, 0, N1), make_axis(1, 1, 0, M), make_axis(u1, u1, 0, 1), 1, 0, 1.0};
-    const KSel k1 = fft_default_sel(p1, dt, N1), k3 = fft_default_sel(p3, dt, N2);
-    for (int dir : {-1, 1})
-        if (!pass_twiddles(p1, N1, dir, dt, k1) || !pass_twiddles(p3, N2, dir, dt, k3)) return false;
-    return true;
 }
 
 }  // namespace
 
 bool chirp_pass_ok(int N) { return N >= 9 && N <= kDftMaxN; }
 
-size_t chirp_pass_work_bytes(int N, a2afft_dtype dt, uint64_t lines) {
-    if (!chirp_pass_ok(N)) return 0;
-    const size_t esize = size_t(esize_of(dt));
+const void* chirp_pass_kernel(int N, bool dbl, int strided) {
+    if (!chirp_pass_ok(N)) return nullptr;
     const int M = cz_length(N);
-    if (M <= kDftMaxN) return 0;   // one kernel, no workspace
-    const uint64_t chunk = std::min<uint64_t>(lines, uint64_t(chirp_chunk_lines(N, esize)));
-    return size_t(chunk) * size_t(M) * esize * (long_needs_work(M) ? 2 : 1);
+    if (dbl) return strided ? cz_fused_for<double, true>(M) : cz_fused_for<double, false>(M);
+    return strided ? cz_fused_for<float, true>(M) : cz_fused_for<float, false>(M);
 }
 
 const void* chirp_pass_tables(int N, int dir, a2afft_dtype dt) {
     if (!chirp_pass_ok(N)) return nullptr;
     const BluePlan* f = blue_plan(N, -1, nullptr);
     const BluePlan* b = blue_plan(N, 1, nullptr);
-    if (!f || !b || !lines_tables(cz_length(N), dt)) return nullptr;
+    const int M = cz_length(N);
+    if (!f || !b || !twiddle_table(M, -1, dt) || !twiddle_table(M, 1, dt)) return nullptr;
     return dir < 0 ? f->chirp : b->chirp;
 }
 
-const void* chirp_pass_kernel(bool dbl) {
-    return dbl ? (const void*)&cz_gather_kernel<double> : (const void*)&cz_gather_kernel<float>;
-}
-
 int launch_chirp_pass(const FftGeom& g, uint32_t units, int strided, void* data, a2afft_dtype dt, int N, int dir,
-                      hipStream_t s, void* ext_work, size_t ext_bytes) {
-    if (!chirp_pass_ok(N)) return set_error(A2AFFT_E_UNSUPPORTED, "chirp-z pass: length %d", N);
+                      hipStream_t s) {
+    const void* fn = chirp_pass_kernel(N, dt == A2AFFT_C128, strided);
+    if (!fn) return set_error(A2AFFT_E_UNSUPPORTED, "chirp-z pass: length %d", N);
     if (units == 0) return A2AFFT_OK;
     const BluePlan* bp = blue_plan(N, dir, s);
-    if (!bp) return set_error(A2AFFT_E_ALLOC, "chirp-z pass: tables for length %d", N);
-    const int M = bp->M;
-    const bool dbl = dt == A2AFFT_C128;
-    if (M <= kDftMaxN) {   // the whole convolution in one kernel
-        const void* fn = cz_fused_fn(M, dbl, strided);
-        const void* twf = twiddle_table(M, -1, dt);
-        const void* twi = twiddle_table(M, 1, dt);
-        if (!fn || !twf || !twi) return set_error(A2AFFT_E_ALLOC, "chirp-z pass: tables for length %d", N);
-        const uint32_t T = uint32_t(cz_fused_lines(M)), grid = (units + T - 1) / T;
-        const double2* ch = bp->chirp;
-        const double2* bh = bp->bhat;
-        double inv_m = 1.0 / double(M);
-        int n = N;
-        uint32_t nu = units;
-        void* args[] = {&data, (void*)&g, &nu, &n, (void*)&twf, (void*)&twi, (void*)&ch, (void*)&bh, &inv_m};
-        A2A_TRY(hip_check(int(hipLaunchKernel(fn, dim3(grid), dim3(kCzFusedThreads), args, 0, s)),
-                          "chirp-z pass launch"));
-        return hip_check(int(hipGetLastError()), "chirp-z pass launch");
-    }
-    const size_t esize = size_t(esize_of(dt));
-    const size_t rows = size_t(M) * esize;                          // bytes of one workspace row
-    const size_t per_line = rows * (long_needs_work(M) ? 2 : 1);   // with the four-step's share
-    int64_t chunk = std::min<int64_t>(int64_t(units), chirp_chunk_lines(N, esize));
-    // the caller's workspace when it holds a line (a plan sizes it for its
-    // own passes; a tuning candidate of another length may not fit it)
-    const bool own = !ext_work || ext_bytes < per_line;
-    void* work = own ? nullptr : ext_work;
-    if (own)
-        A2A_TRY(hip_check(int(hipMallocAsync(&work, size_t(chunk) * per_line, s)), "chirp-z pass: workspace"));
-    else
-        chunk = std::min<int64_t>(int64_t(units), int64_t(ext_bytes / per_line));
-    void* work2 = per_line > rows ? static_cast<char*>(work) + size_t(chunk) * rows : nullptr;
-    const size_t work2_bytes = per_line > rows ? size_t(chunk) * rows : 0;
-    const uint32_t etiles = uint32_t(M / kCzTM), ktiles = uint32_t((N + kCzTM - 1) / kCzTM);
-    int st = A2AFFT_OK;
-    for (int64_t l0 = 0; l0 < int64_t(units) && st >= 0; l0 += chunk) {
-        const uint32_t cl = uint32_t(std::min<int64_t>(chunk, int64_t(units) - l0));
-        const uint32_t ltiles = (cl + uint32_t(kCzTL) - 1) / uint32_t(kCzTL);
-        const uint64_t tm = uint64_t(cl) * uint64_t(M);
-        const dim3 gg(ltiles * etiles), gs(ltiles * ktiles), gm(uint32_t((tm + 255) / 256));
-        const uint32_t ul0 = uint32_t(l0);
-        if (dbl)
-            hipLaunchKernelGGL(cz_gather_kernel<double>, gg, dim3(256), 0, s, static_cast<const double2*>(data),
-                               static_cast<double2*>(work), g, ul0, cl, N, M, etiles, bp->chirp, strided);
-        else
-            hipLaunchKernelGGL(cz_gather_kernel<float>, gg, dim3(256), 0, s, static_cast<const float2*>(data),
-                               static_cast<float2*>(work), g, ul0, cl, N, M, etiles, bp->chirp, strided);
-        st = hip_check(int(hipGetLastError()), "chirp-z pass: gather");
-        if (st >= 0) st = fft_lines(work, M, cl, dt, -1, s, work2, work2_bytes);
-        if (st < 0) break;
-        if (dbl)
-            hipLaunchKernelGGL(blue_mul_kernel<double>, gm, dim3(256), 0, s, static_cast<double2*>(work), M, tm,
-                               bp->bhat, 1.0 / double(M));
-        else
-            hipLaunchKernelGGL(blue_mul_kernel<float>, gm, dim3(256), 0, s, static_cast<float2*>(work), M, tm,
-                               bp->bhat, 1.0 / double(M));
-        st = hip_check(int(hipGetLastError()), "chirp-z pass: product");
-        if (st >= 0) st = fft_lines(work, M, cl, dt, +1, s, work2, work2_bytes);
-        if (st < 0) break;
-        if (dbl)
-            hipLaunchKernelGGL(cz_scatter_kernel<double>, gs, dim3(256), 0, s, static_cast<const double2*>(work),
-                               static_cast<double2*>(data), g, ul0, cl, N, M, ktiles, bp->chirp, strided);
-        else
-            hipLaunchKernelGGL(cz_scatter_kernel<float>, gs, dim3(256), 0, s, static_cast<const float2*>(work),
-                               static_cast<float2*>(data), g, ul0, cl, N, M, ktiles, bp->chirp, strided);
-        st = hip_check(int(hipGetLastError()), "chirp-z pass: scatter");
-    }
-    if (!own) return st;
-    const int fs = hip_check(int(hipFreeAsync(work, s)), "chirp-z pass: workspace free");
-    return st < 0 ? st : fs;
+    const int M = cz_length(N);
+    const void* twf = twiddle_table(M, -1, dt);
+    const void* twi = twiddle_table(M, 1, dt);
+    if (!bp || !twf || !twi) return set_error(A2AFFT_E_ALLOC, "chirp-z pass: tables for length %d", N);
+    const uint32_t T = uint32_t(cz_fused_lines(M)), grid = (units + T - 1) / T;
+    const double2* ch = bp->chirp;
+    const double2* bh = bp->bhat;
+    double inv_m = 1.0 / double(M);
+    int n = N;
+    uint32_t nu = units;
+    void* args[] = {&data, (void*)&g, &nu, &n, (void*)&twf, (void*)&twi, (void*)&ch, (void*)&bh, &inv_m};
+    A2A_TRY(hip_check(int(hipLaunchKernel(fn, dim3(grid), dim3(kCzFusedThreads), args, 0, s)), "chirp-z pass launch"));
+    return hip_check(int(hipGetLastError()), "chirp-z pass launch");
 }
 
 }  // namespace a2a

@@ -31,7 +31,7 @@ A2A_HD constexpr bool mixed_len_ok(int N) {
 }
 A2A_HD constexpr int npass_of(int N) {
     if (!is_pow2(N)) return (pow2_part(N) > 1 ? 1 : 0) + count_factor(N, 3) + count_factor(N, 5);
-    return N <= 16 ? 1 : (N <= 256 ? 2 : 3);
+    return N <= 16 ? 1 : (N <= 256 ? 2 : (N <= 4096 ? 3 : 4));
 }
 A2A_HD constexpr int radix_of(int N, int p) {
     if (!is_pow2(N)) {
@@ -44,6 +44,7 @@ A2A_HD constexpr int radix_of(int N, int p) {
     }
     // 2:[2] 4:[4] 8:[8] 16:[16] 32:[8,4] 64:[8,8] 128:[16,8] 256:[16,16]
     // 512:[8,8,8] 1024:[16,8,8] 2048:[16,16,8] 4096:[16,16,16]
+    // 8192:[16,16,16,2] (the chirp-z convolution rows of 2049..4096-point lines)
     return N <= 16 ? N
          : N == 32 ? (p == 0 ? 8 : 4)
          : N == 64 ? 8
@@ -52,6 +53,7 @@ A2A_HD constexpr int radix_of(int N, int p) {
          : N == 512 ? 8
          : N == 1024 ? (p == 0 ? 16 : 8)
          : N == 2048 ? (p == 2 ? 8 : 16)
+         : N == 8192 ? (p == 3 ? 2 : 16)
          : 16;
 }
 // Granule of E (elements per thread): every radix must divide it.
@@ -155,37 +157,19 @@ enum : int {
     KF_HALFX = 32, // LDS exchanges move one component at a time: half the LDS, two more barriers
     KF_DFT = 64,   // direct O(N^2) DFT, one line per workgroup: the lengths no Stockham kernel covers
     KF_MIX = 128,  // Stockham FFT with the radix schedule in the arguments: long 2^a 3^b 5^c lengths
-    KF_CZ = 2048,  // Bluestein chirp-z through a workspace of power-of-two rows: large prime factors
+    KF_CZ = 2048,  // Bluestein chirp-z convolution over power-of-two FFTs: prime factors above 5
 };
 constexpr int kDftMaxN = 4096;   // longest direct-DFT line (N complex values in LDS)
 
-// Chirp-z passes (KF_CZ, fft_long.hip).  The lines of a launch window, in
-// chunks of `nl` lines from line l0, are gathered into rows of M elements of
-// a workspace (row u - l0: elements 0..N-1 times the chirp, zeros after) and
-// scattered back after the convolution.  Line u of a window is
-// (u1, u2) = (u / n2, u % n2), its element n at
-// a1.off(u1) + a2.off(a2_start + u2) + fft.off(n).  Workgroup b of a chunk
-// moves the tile (b / etiles, b % etiles) of kCzTL lines by kCzTM elements
-// through LDS.  Slot i of a tile is (line, element) (i % kCzTL, i / kCzTL)
-// on the grid side of a strided pass, where adjacent lines are adjacent in
-// memory, and (i / kCzTM, i % kCzTM) on the grid side of a contiguous pass
-// and on the workspace side.
-constexpr int kCzTL = 64, kCzTM = 32;
+// Chirp-z passes (KF_CZ, fft_long.hip).  Line u of a launch window is
+// (u1, u2) = (u / n2, u % n2) (n2 = window count), its element n at
+// a1.off(u1) + a2.off(a2_start + u2) + fft.off(n).
 A2A_HD int64_t cz_line_base(const FftGeom& g, uint32_t u) {
     const uint32_t u1 = g.dn2.div(u);
     return g.a1.off(u1) + g.a2.off(g.a2_start + (u - u1 * g.n2));
 }
-A2A_HD void cz_slot(bool lines_fast, int i, int* tl, int* te) {
-    if (lines_fast) {
-        *tl = i % kCzTL;
-        *te = i / kCzTL;
-    } else {
-        *tl = i / kCzTM;
-        *te = i % kCzTM;
-    }
-}
-// Convolutions of M <= kDftMaxN points run in one kernel (cz_fused_kernel):
-// a workgroup of 512 threads holds kCzFusedE elements of each of
+// The convolution of M <= 2 kDftMaxN points runs in one kernel
+// (cz_fused_kernel): a workgroup of 512 threads holds kCzFusedE elements of each of
 // T = 8192 / M lines in registers, thread tid taking element t + e * (M / E)
 // of line `line` of its tile as below (the Stockham kernels' lane orders).
 constexpr int kCzFusedE = 16, kCzFusedThreads = 512;

@@ -5,9 +5,6 @@
 // touches.
 #include <stdint.h>
 
-#include <algorithm>
-#include <vector>
-
 #include "fft_shape.h"
 #include "remap_index.h"
 
@@ -254,105 +251,7 @@ int64_t hc_long_pass_enumerate(int N1, int N2, int lines, int dbl, int T, int E,
     return c;
 }
 
-// Every element a chirp-z pass moves (fft_long.hip: cz_gather_kernel, side
-// 0, and cz_scatter_kernel, side 1), workgroup by workgroup and through the
-// LDS tile slot it passes: pass `pass` (0 z, 1 x, 2 y) of an n3 grid over
-// `dims` virtual ranks, window [a2_start, a2_start + a2_count) of the pass's
-// a2 axis (count 0: whole), `chunk` lines per chunk.  Per element: its grid
-// address, its index n along the line, the window line u and the workspace
-// index it meets there, (u - l0) * M + n.  wcount (nchunks * chunk * M
-// entries, chunk c at c * chunk * M) counts the gather's workspace writes,
-// zero padding included.  Returns the count, or a negative status.
-int64_t hc_cz_enumerate(const int* n3, const int* dims, int pass, int side, uint32_t a2_start, uint32_t a2_count,
-                        int64_t chunk, int64_t* addr, int32_t* n_out, int64_t* line_out, int64_t* work_out,
-                        int32_t* wcount, int64_t cap) {
-    int lgs[3];
-    for (int i = 0; i < 3; ++i) lgs[i] = n3[i] / dims[i];
-    const int64_t nlocal = int64_t(lgs[0]) * lgs[1] * lgs[2];
-    PassMaps all[3];
-    fused_pass_maps(n3, dims, lgs, nlocal, all);
-    const PassMaps& pm = all[pass];
-    const int N = int(pm.fft.n), M = cz_length(N);
-    const bool strided = pm.strided != 0;
-    FftGeom g;
-    uint32_t units;
-    const int st = resolve_geom(pm.fft, pm.a1, pm.a2, 0, 1, &g, &units, a2_start, a2_count);
-    if (st < 0) return st;
-    if (M % kCzTM) return -101;
-    int64_t c = 0;
-    std::vector<int64_t> slot_addr(size_t(kCzTL) * kCzTM), slot_work(size_t(kCzTL) * kCzTM);
-    std::vector<int32_t> slot_n(size_t(kCzTL) * kCzTM);
-    std::vector<int64_t> slot_u(size_t(kCzTL) * kCzTM);
-    for (int64_t l0 = 0; l0 < int64_t(units); l0 += chunk) {
-        const uint32_t nl = uint32_t(std::min<int64_t>(chunk, int64_t(units) - l0));
-        const uint32_t ltiles = (nl + kCzTL - 1) / kCzTL;
-        const uint32_t etiles = side == 0 ? uint32_t(M / kCzTM) : uint32_t((N + kCzTM - 1) / kCzTM);
-        int32_t* wc = wcount + (l0 / chunk) * chunk * M;
-        for (uint32_t b = 0; b < ltiles * etiles; ++b) {
-            const uint32_t lt = b / etiles, et = b - lt * etiles;
-            const uint32_t lb = lt * uint32_t(kCzTL);
-            const int m0 = int(et) * kCzTM;
-            std::fill(slot_addr.begin(), slot_addr.end(), -1);
-            std::fill(slot_work.begin(), slot_work.end(), -1);
-            if (side == 0 && m0 >= N) {   // the padding-only tile
-                for (int i = 0; i < kCzTL * kCzTM; ++i) {
-                    int tl, te;
-                    cz_slot(false, i, &tl, &te);
-                    if (lb + uint32_t(tl) < nl) ++wc[int64_t(lb + uint32_t(tl)) * M + m0 + te];
-                }
-                continue;
-            }
-            // phase 1 fills slots, phase 2 drains them (the kernel's two loops)
-            for (int i = 0; i < kCzTL * kCzTM; ++i) {
-                int tl, te;
-                cz_slot(side == 0 ? strided : false, i, &tl, &te);
-                const uint32_t line = lb + uint32_t(tl);
-                const int n = m0 + te;
-                if (line >= nl || n >= N) continue;
-                const size_t k = size_t(tl) * kCzTM + size_t(te);
-                if (side == 0) {
-                    slot_addr[k] = cz_line_base(g, uint32_t(l0) + line) + g.fft.off(uint32_t(n));
-                    slot_n[k] = n;
-                    slot_u[k] = l0 + line;
-                } else {
-                    slot_work[k] = int64_t(line) * M + n;
-                }
-            }
-            for (int i = 0; i < kCzTL * kCzTM; ++i) {
-                int tl, te;
-                cz_slot(side == 0 ? false : strided, i, &tl, &te);
-                const uint32_t line = lb + uint32_t(tl);
-                const int n = m0 + te;
-                if (line >= nl) continue;
-                const size_t k = size_t(tl) * kCzTM + size_t(te);
-                if (side == 0) {
-                    ++wc[int64_t(line) * M + n];
-                    if (n >= N) continue;
-                    if (slot_addr[k] < 0) return -102;   // read a slot phase 1 never filled
-                    if (c >= cap) return -100;
-                    addr[c] = slot_addr[k];
-                    n_out[c] = slot_n[k];
-                    line_out[c] = slot_u[k];
-                    work_out[c] = int64_t(line) * M + n;
-                    ++c;
-                } else {
-                    if (n >= N) continue;
-                    if (slot_work[k] < 0) return -102;
-                    if (c >= cap) return -100;
-                    addr[c] = cz_line_base(g, uint32_t(l0) + line) + g.fft.off(uint32_t(n));
-                    n_out[c] = n;
-                    line_out[c] = l0 + line;
-                    work_out[c] = slot_work[k];
-                    ++c;
-                }
-            }
-        }
-    }
-    return c;
-}
-
-// Every element the single-kernel chirp-z pass (cz_fused_kernel, M <= 4096)
-// loads and stores: pass `pass` of an n3 grid over `dims` virtual ranks,
+// Every element the chirp-z pass (cz_fused_kernel) loads and stores: pass `pass` of an n3 grid over `dims` virtual ranks,
 // window [a2_start, a2_start + a2_count) of its a2 axis.  Per element: its
 // grid address, its index n along the line, and the window line u.  Returns
 // the count, or a negative status.
@@ -365,7 +264,7 @@ int64_t hc_cz_fused_enumerate(const int* n3, const int* dims, int pass, uint32_t
     fused_pass_maps(n3, dims, lgs, nlocal, all);
     const PassMaps& pm = all[pass];
     const int N = int(pm.fft.n), M = cz_length(N);
-    if (M > kDftMaxN || M < kCzTM) return -101;
+    if (N < 9 || N > kDftMaxN) return -101;
     FftGeom g;
     uint32_t units;
     const int st = resolve_geom(pm.fft, pm.a1, pm.a2, 0, 1, &g, &units, a2_start, a2_count);
