@@ -780,11 +780,21 @@ extern "C" int a2afft_plan_set_kernel(a2afft_plan p, int pass, int T, int E, int
     return A2AFFT_OK;
 }
 
+// The kernel family of a selection: variants within one family compute every
+// line with the same arithmetic (the fixed-schedule Stockham shapes, the
+// runtime-schedule tiles, the chirp-z pass, the direct DFT; the interleaved-K
+// or the split-K-halves MFMA kernels).
+static int sel_family(const a2afft_plan_s* p, const KSel& s) {
+    if (is_split(p->prec)) return s.E == 0 ? 1 : 2;
+    return s.flags & (KF_MIX | KF_CZ | KF_DFT);
+}
+
 // Times every built variant that fits each fused pass (forward direction,
-// `iters` launches each, in place on `work`, whose contents are destroyed),
-// keeps the fastest per pass, then the fastest chunk width, and keeps the
-// result only if whole forwards beat the plan's choices before the call.
-// Every variant computes identical bits.
+// `iters` launches each, in place on `work`, whose contents are destroyed)
+// within the family of the pass's selection, keeps the fastest per pass, then
+// the fastest chunk width, and keeps the result only if whole forwards beat
+// the plan's choices before the call.  Every variant computes identical bits,
+// so tuning never changes a result (set_kernel crosses families).
 extern "C" int a2afft_plan_autotune(a2afft_plan p, void* work, int iters, void* stream) {
     if (!p || !work || iters <= 0) return set_error(A2AFFT_E_INVALID_ARG, "autotune: bad arguments");
     if (p->exec != A2AFFT_EXEC_FUSED) return set_error(A2AFFT_E_INVALID_ARG, "autotune: fused plans only");
@@ -816,6 +826,7 @@ extern "C" int a2afft_plan_autotune(a2afft_plan p, void* work, int iters, void* 
         const int n = pass_candidates(p, i, cand, 256);
         float best = 1e30f;
         for (int c = 0; c < n && st >= 0; ++c) {
+            if (sel_family(p, cand[c]) != sel_family(p, start_sel[i])) continue;
             st = launch_pass(p, p->pass[i], work, -1, s, &cand[c]);   // warm
             if (st < 0) break;
             st = hip_check(int(hipEventRecord(e0, s)), "autotune record");

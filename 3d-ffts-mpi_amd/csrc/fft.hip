@@ -32,6 +32,8 @@ static KSel cz_sel(int N) { return KSel{1, N, KF_CZ}; }
 // of its threads idle), 3.3 ms at 97, 13 ms at 509, 124 ms at 4093; the
 // chirp-z pass 0.9-2.1 ms at every length from 11 to 4093.  complex128 agrees,
 // except contiguous lines of 37-53 points, where the DFT is up to 1.3x faster.
+// The choice depends on (N, precision) only, never on the layout, so that
+// fused and staged plans compute every line alike.
 static KSel fallback_sel(const AxisPass& p, a2afft_dtype dt, int N) {
     MixedPlan mp;
     if (mixed_plan(p, N, dt == A2AFFT_C128, &mp)) return KSel{mp.T, mp.E, KF_MIX};
@@ -141,12 +143,12 @@ int fft_candidates(const AxisPass& p, a2afft_dtype dt, int N, KSel* out, int max
         out[n++] = KSel{k.T, k.E, k.flags};
     }
     if (n == 0 && fft_length_supported(N)) {
-        const KSel f = fallback_sel(p, dt, N);
-        if (is_mix(f) && n < max) {
+        MixedPlan mp;
+        if (mixed_plan(p, N, dbl, &mp) && n < max) {
+            const KSel f{mp.T, mp.E, KF_MIX};
             out[n++] = f;
             // every other tile the runtime-schedule kernel accepts on this pass
-            MixedPlan mp;
-            for (int tt : {1, 2, 4, 8, 16, 32})
+            for (int tt : {1, 2, 4, 8, 16, 32, 64})
                 if (tt != f.T && n < max && mixed_plan(p, N, dbl, &mp, tt)) out[n++] = KSel{tt, mp.E, KF_MIX};
         }
         if (chirp_pass_ok(N) && n < max) out[n++] = cz_sel(N);

@@ -250,8 +250,11 @@ bool mixed_plan(const AxisPass& ps, int N, bool dbl, MixedPlan* mp, int force_T)
     } else if (!ps.strided) {
         q.T = std::min(kMixThreads / q.TPL, lds_lines);
     } else {
+        // As many columns as fill the 256 threads: lines of few threads (TPL
+        // 4 for 120 points) would otherwise leave most of a 16-column
+        // workgroup idle.
         q.T = 1;
-        for (int tt : {16, 8, 4, 2})
+        for (int tt : {64, 32, 16, 8, 4, 2})
             if (tt * q.TPL <= kMixThreads && tt <= lds_lines && ps.a2.s_elem == 1 && ps.a2.l % uint32_t(tt) == 0 &&
                 ps.a2.n % uint32_t(tt) == 0) {
                 q.T = tt;

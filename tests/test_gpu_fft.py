@@ -56,6 +56,16 @@ def test_batched_1d_fft_vs_oracle(a2afft_mod, cuda, N, dtype):
 MIXED_LONG = [72, 96, 100, 120, 192, 243, 384, 625, 768, 1000, 1536, 2187, 3072, 3125, 3840, 4000]
 
 
+KF_DFT, KF_CZ = 64, 2048
+
+
+def _chosen_flags(plan, name):
+    for ln in plan.describe().splitlines():
+        if ln.startswith(f"chosen {name} "):
+            return int(ln.split("flags=")[1].split()[0])
+    raise AssertionError(plan.describe())
+
+
 def _smooth(n):
     for f in (2, 3, 5):
         while n % f == 0:
@@ -310,6 +320,28 @@ def test_grid_plan_non_cubic_vs_numpy(a2afft_mod, cuda, shape, prec):
     assert _relerr(d.cpu().numpy(), x.astype(np.complex128)) < 2 * tol
 
 
+def test_chirp_on_smooth_lengths(a2afft_mod, cuda):
+    """The chirp-z pass also takes 2^a 3^b 5^c lengths when forced (a
+    set_kernel candidate beside the runtime-schedule tiles): 120-point x
+    passes, complex128 and complex64, against numpy's fftn."""
+    import torch
+    from a2afft import inputs
+    shape = (120, 4, 64)
+    x = inputs.uniform(19, int(np.prod(shape))).reshape(shape)
+    ref = np.fft.fftn(x)
+    for prec, dt in (("fp64", "c128"), ("fp32", "c64")):
+        tol = TOL_1D[dt] * 3 * np.log2(120)
+        for forced in (False, True):
+            plan = a2afft_mod.GridDfft(shape, prec)
+            if forced:
+                plan.set_kernel(1, 1, 120, KF_CZ)
+            assert _chosen_flags(plan, "x") == (KF_CZ if forced else 128)
+            d = torch.from_numpy(x.astype(np.complex128 if prec == "fp64" else np.complex64)).cuda()
+            plan.forward(d)
+            torch.cuda.synchronize()
+            assert _relerr(d.cpu().numpy(), ref) < tol, (prec, forced)
+
+
 # (grid shape, precision, a tile the plan must choose): strided passes of long
 # 2^a 3^b 5^c lines whose tiles need more than 256 threads (wide runtime-schedule
 # kernels, up to 1024 threads and 128 KiB of LDS)
@@ -346,7 +378,9 @@ def test_wide_mixed_tiles_vs_numpy(a2afft_mod, cuda, case):
 @pytest.mark.parametrize("case", [((768, 32, 16), "fp32", 1, 12, (1, 2, 4, 8, 16)),
                                   ((768, 4, 16), "fp64", 1, 12, (1, 2, 4, 8)),
                                   ((16, 16, 768), "fp32", 0, 12, (1, 2, 4)),
-                                  ((16, 960, 16), "fp32", 2, 30, (1, 2, 4, 8))],
+                                  ((16, 960, 16), "fp32", 2, 30, (1, 2, 4, 8)),
+                                  ((120, 4, 64), "fp32", 1, 30, (1, 2, 4, 8, 16, 32, 64)),
+                                  ((4, 72, 64), "fp32", 2, 12, (1, 2, 4, 8, 16, 32))],
                          ids=lambda c: f"{c[0]}_{c[1]}_pass{c[2]}")
 def test_mixed_tiles_bit_identical(a2afft_mod, cuda, case):
     """Every tile the runtime-schedule kernel accepts (narrow and wide; the
@@ -554,16 +588,6 @@ def test_long_lines_on_two_streams(a2afft_mod, cuda):
         torch.cuda.synchronize()
         assert np.array_equal(_bits(a.cpu().numpy()), _bits(ref_a.cpu().numpy())), rep
         assert np.array_equal(_bits(b.cpu().numpy()), _bits(ref_b.cpu().numpy())), rep
-
-
-KF_DFT, KF_CZ = 64, 2048
-
-
-def _chosen_flags(plan, name):
-    for ln in plan.describe().splitlines():
-        if ln.startswith(f"chosen {name} "):
-            return int(ln.split("flags=")[1].split()[0])
-    raise AssertionError(plan.describe())
 
 
 @pytest.mark.parametrize("kernel", ["default", "dft"])
