@@ -83,6 +83,24 @@ def cpu_baseline(Ng, P, dims, threads, seed, samples=3):
     return float(np.median(times)), times, O.get_threads()
 
 
+def library_cpu_fftn(Ng, threads, samples=3):
+    """scipy's pocketfft fftn of an Ng^3 complex128 grid on `threads` workers:
+    the median of `samples` forwards (ms), or None without scipy."""
+    try:
+        import numpy as np
+        import scipy.fft as sfft
+    except Exception:
+        return None
+    a = np.random.default_rng(0).standard_normal((Ng, Ng, Ng)).astype(np.complex128)
+    times = []
+    for _ in range(samples):
+        t0 = time.perf_counter()
+        y = sfft.fftn(a, workers=threads)
+        times.append((time.perf_counter() - t0) * 1e3)
+        del y
+    return float(np.median(times))
+
+
 def _traffic(key):
     """Measured HBM bytes per launch (profiles/traffic.json), or None."""
     try:
@@ -595,6 +613,11 @@ def main():
                          f"{list(dims)}, reference stage order (oracle C restatement, OpenMP)",
                "samples_ms": [round(t, 1) for t in samples], "affinity_cores": aff, "cgroup_quota_cores": quota,
                "cpu_model": model}
+        lib_ms = library_cpu_fftn(Ng, threads)
+        if lib_ms is not None:
+            cpu["library_fftn"] = {"value": round(lib_ms, 1), "unit": "ms", "cores": threads,
+                                   "what": f"scipy.fft.fftn (pocketfft) of one {Ng}^3 complex128 grid, "
+                                           "median of 3"}
 
     out = {
         "metric": "ms/512^3 c2c 3D FFT + % HBM roofline" if args.config == "c3" else f"ms/{Ng}^3 c2c 3D FFT",
