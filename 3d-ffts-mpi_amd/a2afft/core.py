@@ -153,8 +153,10 @@ class _Plan:
 
     def set_kernel(self, pass_index, T, E, flags=0):
         """Force a kernel variant: flags 1 XCD order, 2 LDS pad, 4 non-temporal, 8 z-tiles outermost,
-        32 half-width LDS exchanges.  Split plans: T=16, E=0 selects the interleaved-K split-BF16
-        kernel (flags 1, 4, 8 as above), E=N the split-K-halves kernels."""
+        32 half-width LDS exchanges; 128 the runtime-schedule Stockham kernel (T = tile lines, E its
+        elements per thread), (1, N, 2048) the chirp-z pass, (1, N, 64) the direct DFT.  Split plans:
+        T=16, E=0 selects the interleaved-K split-BF16 kernel (flags 1, 4, 8 as above), E=N the
+        split-K-halves kernels.  Raises A2AFFTError for a variant not built or not fitting the pass."""
         return L.check(L.lib().a2afft_plan_set_kernel(self._h, pass_index, T, E, flags), "set_kernel")
 
     def set_chunking(self, z_columns):
