@@ -336,18 +336,19 @@ static bool is_split(a2afft_precision prec) { return prec == A2AFFT_SPLIT_BF16 |
 //   512^3 complex64:  1.163 -> 1.134 ms with 128-column windows,
 //   512^3 complex128: 2.40 -> 2.23 ms with 64-column windows.
 // complex64 with full-exchange tiles (1.163 -> 1.189 ms) and smaller grids
-// measured slower chunked: whole passes otherwise.  Split-BF16 plans window
-// when both strided passes run the interleaved-K kernel; the split-K-halves
-// kernels (split-FP16) measured slower windowed (1.41-1.48 against 1.30-1.33
-// ms) and run whole passes.
+// measured slower chunked: whole passes otherwise.  Split-precision plans
+// window when both strided passes run the interleaved-K kernel; the
+// split-K-halves kernels measured slower windowed (1.41-1.48 against
+// 1.30-1.33 ms) and run whole passes.
 static uint32_t default_chunk(const a2afft_plan_s* p) {
-    if (p->prec == A2AFFT_SPLIT_FP16) return 0;
     const bool c64 = p->prec != A2AFFT_FP64;
     for (int i = 1; i < 3; ++i)
         if (p->pass[i].fft.n != 512u || !p->pass[i].strided) return 0;
-    if (p->prec == A2AFFT_SPLIT_BF16) {
+    if (is_split(p->prec)) {
         // the interleaved-K kernel on both strided passes (split-BF16 512^3
-        // complex64: 1.18-1.19 ms whole, 1.104-1.116 ms in 128-column windows)
+        // complex64: 1.18-1.19 ms whole, 1.104-1.116 ms in 128-column
+        // windows); the split-K-halves kernels run whole passes (windows
+        // measured slower for them)
         if (p->sel[1].E != 0 || p->sel[2].E != 0) return 0;
     } else if (!(p->sel[1].flags & KF_HALFX) || (c64 && !(p->sel[2].flags & KF_HALFX))) {
         return 0;

@@ -503,9 +503,9 @@ __global__ __launch_bounds__(64 * NW, NW == 8 ? 4 : 2) void mfma_fft_direct(floa
 }
 
 // ---------------------------------------------------------------------------
-// Split-BF16 passes of 256 = 16 * 16 and 512 = 16 * 16 * 2 point lines with
-// the complex values interleaved along K, one LDS exchange of operands that
-// are already split, and buffer addressing.
+// Split-BF16 and split-FP16 passes of 256 = 16 * 16 and 512 = 16 * 16 * 2
+// point lines with the complex values interleaved along K, one LDS exchange
+// of operands that are already split, and buffer addressing.
 //
 // K order.  The 32 K slots of a radix-16 product hold the 16 inputs as
 // (re, im) pairs: lane group g (= lane / 16) holds inputs r = 4g .. 4g+3, so
@@ -546,6 +546,18 @@ __global__ __launch_bounds__(64 * NW, NW == 8 ? 4 : 2) void mfma_fft_direct(floa
 // checks the pass's extent fits), the wave-uniform part in the scalar
 // offset; loads and stores outside the extent are dropped by the buffer
 // range check.
+//
+// FP16 (F16 = true).  FP16's range needs the operands scaled.  Each line gets
+// one power-of-two scale s that brings its largest component magnitude into
+// [2^14, 2^15): a strided column's maximum over its 4 lane groups (xor
+// shuffles) and the 4 waves (64 floats of LDS, read after the barrier that
+// publishes the twiddles), a contiguous line's over its wave.  Stage 0 takes
+// x s; the stage-1 operands are its outputs times the twiddle / 16 (the LDS
+// twiddle table is stored divided by 16, exactly), so they stay below 2^15
+// whatever the stage-0 growth; the stores multiply by 16 / s.  Every product
+// is exact in its power of two, so the hi + lo split keeps about 22 bits of
+// each value relative to its own line, as the per-butterfly scale of the
+// split-K-halves kernels does.
 // ---------------------------------------------------------------------------
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
@@ -561,23 +573,47 @@ __device__ __forceinline__ uint32_t pk_bf16(float a, float b) {
     const f2 v = {a, b};
     return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, b2));
 }
-// One complex value as packed bf16 hi = round(v) and lo = round(v - hi).
-__device__ __forceinline__ void split_c(float2 v, uint32_t& hi, uint32_t& lo) {
-    hi = pk_bf16(v.x, v.y);
-    lo = pk_bf16(v.x - __uint_as_float(hi << 16), v.y - __uint_as_float(hi & 0xffff0000u));
+// Packed (a, b) rounded to fp16 (a in the low half).
+__device__ __forceinline__ uint32_t pk_f16(float a, float b) {
+    typedef float f2 __attribute__((ext_vector_type(2)));
+    typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+    const f2 v = {a, b};
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, h2));
 }
-__device__ __forceinline__ void split4(const float2 (&x)[4], u32x4& h, u32x4& l) {
+// The two halves of a packed fp16 pair as floats.
+__device__ __forceinline__ float2 unpk_f16(uint32_t p) {
+    typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+    const h2 h = __builtin_bit_cast(h2, p);
+    return make_float2(float(h[0]), float(h[1]));
+}
+// One complex value as a packed 16-bit pair hi = round(v) and lo = round(v - hi)
+// (bf16, or fp16 with F16).
+template <bool F16 = false> __device__ __forceinline__ void split_c(float2 v, uint32_t& hi, uint32_t& lo) {
+    if constexpr (F16) {
+        hi = pk_f16(v.x, v.y);
+        const float2 h = unpk_f16(hi);
+        lo = pk_f16(v.x - h.x, v.y - h.y);
+    } else {
+        hi = pk_bf16(v.x, v.y);
+        lo = pk_bf16(v.x - __uint_as_float(hi << 16), v.y - __uint_as_float(hi & 0xffff0000u));
+    }
+}
+template <bool F16 = false> __device__ __forceinline__ void split4(const float2 (&x)[4], u32x4& h, u32x4& l) {
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         uint32_t a, b;
-        split_c(x[k], a, b);
+        split_c<F16>(x[k], a, b);
         h[k] = a;
         l[k] = b;
     }
 }
-__device__ __forceinline__ bf16x8 as_bf(u32x4 v) { return __builtin_bit_cast(bf16x8, v); }
-__device__ __forceinline__ f32x4 mma_bf(u32x4 a, u32x4 b, f32x4 c) {
-    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(a), as_bf(b), c, 0, 0, 0);
+template <bool F16 = false> __device__ __forceinline__ f32x4 mma_h(u32x4 a, u32x4 b, f32x4 c) {
+    if constexpr (F16)
+        return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8, a), __builtin_bit_cast(f16x8, b), c, 0,
+                                                      0, 0);
+    else
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c,
+                                                       0, 0, 0);
 }
 
 // DFT-16 operand rows (real and imaginary output rows, hi and lo).
@@ -586,26 +622,40 @@ struct F16Op {
 };
 // Normal form: columns = butterflies (data = B operand); lane (c, g) gets
 // outputs 4g .. 4g+3 of column c.
+template <bool F16 = false>
 __device__ __forceinline__ void dft16_cols(const F16Op& F, u32x4 bh, u32x4 bl, f32x4& yr, f32x4& yi) {
     const f32x4 z = {0.f, 0.f, 0.f, 0.f};
-    yr = mma_bf(F.rl, bh, z);
-    yi = mma_bf(F.il, bh, z);
-    yr = mma_bf(F.rh, bl, yr);
-    yi = mma_bf(F.ih, bl, yi);
-    yr = mma_bf(F.rh, bh, yr);
-    yi = mma_bf(F.ih, bh, yi);
+    yr = mma_h<F16>(F.rl, bh, z);
+    yi = mma_h<F16>(F.il, bh, z);
+    yr = mma_h<F16>(F.rh, bl, yr);
+    yi = mma_h<F16>(F.ih, bl, yi);
+    yr = mma_h<F16>(F.rh, bh, yr);
+    yi = mma_h<F16>(F.ih, bh, yi);
 }
 // Transposed form: rows = butterflies (data = A operand); lane (c, g) gets
 // output c of rows (butterflies) 4g .. 4g+3.
+template <bool F16 = false>
 __device__ __forceinline__ void dft16_rows(const F16Op& F, u32x4 ah, u32x4 al, f32x4& yr, f32x4& yi) {
     const f32x4 z = {0.f, 0.f, 0.f, 0.f};
-    yr = mma_bf(al, F.rh, z);
-    yi = mma_bf(al, F.ih, z);
-    yr = mma_bf(ah, F.rl, yr);
-    yi = mma_bf(ah, F.il, yi);
-    yr = mma_bf(ah, F.rh, yr);
-    yi = mma_bf(ah, F.ih, yi);
+    yr = mma_h<F16>(al, F.rh, z);
+    yi = mma_h<F16>(al, F.ih, z);
+    yr = mma_h<F16>(ah, F.rl, yr);
+    yi = mma_h<F16>(ah, F.il, yi);
+    yr = mma_h<F16>(ah, F.rh, yr);
+    yi = mma_h<F16>(ah, F.ih, yi);
 }
+
+// Power-of-two scale for the fp16 operands of a line whose largest component
+// magnitude is mx: brings mx into [2^14, 2^15), inside fp16's range with the
+// headroom a radix-16 stage needs once the next stage's operands are divided
+// by 16.  1 for an all-zero line.
+__device__ __forceinline__ float f16_scale(float mx) {
+    int e = 0;
+    (void)frexpf(mx, &e);
+    const int sh = 15 - e;
+    return mx > 0.f ? ldexpf(1.f, sh < -126 ? -126 : (sh > 126 ? 126 : sh)) : 1.f;
+}
+__device__ __forceinline__ float cmax(float m, float2 v) { return fmaxf(m, fmaxf(fabsf(v.x), fabsf(v.y))); }
 
 template <bool NTMP>
 __device__ __forceinline__ float2 bld(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
@@ -658,7 +708,7 @@ __device__ __forceinline__ void xpose4(float2 (&B)[4][4]) {
         }
 }
 
-template <int N, bool STRIDED, int FLAGS>
+template <int N, bool STRIDED, int FLAGS, bool F16 = false>
 __global__ __launch_bounds__(256) void mf2_kernel(float2* __restrict__ data, FftGeom g, const float2* __restrict__ tw,
                                                   uint32_t extent) {
     static_assert(N == 256 || N == 512, "mf2 lengths");
@@ -674,6 +724,8 @@ __global__ __launch_bounds__(256) void mf2_kernel(float2* __restrict__ data, Fft
     // consecutive lanes, consecutive banks
     __shared__ float2 tw1[256];
     __shared__ float2 tw2[R2 > 1 ? 256 : 1];    // W_512^k, k < 256
+    // fp16: each strided column's largest magnitude, per wave
+    __shared__ float colmax[F16 && STRIDED ? 64 : 1];
 
     const int tid = int(threadIdx.x);
     const int lane = tid & 63, cn = lane & 15, gq = lane >> 4;
@@ -681,7 +733,14 @@ __global__ __launch_bounds__(256) void mf2_kernel(float2* __restrict__ data, Fft
     const uint32_t block = (FLAGS & KF_XCD) ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
     const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(data, 0, int(extent), 0x00020000);
 
-    tw1[tid] = tw[16 + tid];
+    // fp16: the stage-1 operands are the stage-0 outputs divided by 16 (exact),
+    // folded into the twiddles, so they stay inside fp16's range
+    if constexpr (F16) {
+        const float2 t = tw[16 + tid];
+        tw1[tid] = make_float2(t.x * 0.0625f, t.y * 0.0625f);
+    } else {
+        tw1[tid] = tw[16 + tid];
+    }
     if constexpr (R2 > 1) tw2[tid] = tw[16 + 256 + 2 * tid + 1];
 
     F16Op F;
@@ -693,8 +752,8 @@ __global__ __launch_bounds__(256) void mf2_kernel(float2* __restrict__ data, Fft
             fr[e] = make_float2(wv.x, -wv.y);
             fi[e] = make_float2(wv.y, wv.x);
         }
-        split4(fr, F.rh, F.rl);
-        split4(fi, F.ih, F.il);
+        split4<F16>(fr, F.rh, F.rl);
+        split4<F16>(fi, F.ih, F.il);
     }
     // byte offsets of the K rows n = B0 (4 gq + e), and of the outputs 16 q'
     uint32_t offr[4], offq[4];
@@ -707,7 +766,12 @@ __global__ __launch_bounds__(256) void mf2_kernel(float2* __restrict__ data, Fft
     // the inverse's 1/(nx*ny*nz) as one multiplication (exact for the
     // power-of-two lengths these kernels take); 1 when the pass does not scale
     const float fs = g.scale_mode == 0 ? 1.f : g.scale_mode == 2 ? float(1.0 / g.scale) : float(g.scale);
-    auto sc = [&](float2 v) {
+    // u: fp16 only, 16 / (the line's operand scale), undoing the scaling
+    auto sc = [&](float2 v, float u = 1.f) {
+        if constexpr (F16) {
+            v.x *= u;
+            v.y *= u;
+        }
         v.x *= fs;
         v.y *= fs;
         return v;
@@ -739,7 +803,36 @@ __global__ __launch_bounds__(256) void mf2_kernel(float2* __restrict__ data, Fft
                     for (int e = 0; e < 4; ++e) v[s][m][e] = bld<NTMP>(rs, lb + offr[e], oj);
                 }
         }
-        lds_barrier();   // twiddles in LDS
+        // fp16: one power-of-two scale per column (line), from its largest
+        // magnitude over the 4 lane groups and the 4 waves
+        float us = 1.f;
+        if constexpr (F16) {
+            float mx = 0.f;
+#pragma unroll
+            for (int s = 0; s < R2; ++s)
+#pragma unroll
+                for (int m = 0; m < 4; ++m)
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) mx = cmax(mx, v[s][m][e]);
+            mx = fmaxf(mx, __shfl_xor(mx, 16));
+            mx = fmaxf(mx, __shfl_xor(mx, 32));
+            if (gq == 0) colmax[w * 16 + cn] = mx;
+        }
+        lds_barrier();   // twiddles (and column maxima) in LDS
+        if constexpr (F16) {
+            const float mx = fmaxf(fmaxf(colmax[cn], colmax[16 + cn]), fmaxf(colmax[32 + cn], colmax[48 + cn]));
+            const float s0 = f16_scale(mx);
+            us = 16.f / s0;
+#pragma unroll
+            for (int s = 0; s < R2; ++s)
+#pragma unroll
+                for (int m = 0; m < 4; ++m)
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        v[s][m][e].x *= s0;
+                        v[s][m][e].y *= s0;
+                    }
+        }
         // row-adjacent stores: lane group g' stores butterfly 4w + g'
         const uint32_t vq = ROWADJ ? lb + 8u * uint32_t(g.fft.off(uint32_t(4 * w + gq))) : 0u;
         float2 y0[4][4];
@@ -750,8 +843,8 @@ __global__ __launch_bounds__(256) void mf2_kernel(float2* __restrict__ data, Fft
 #pragma unroll
             for (int m = 0; m < 4; ++m) {
                 u32x4 bh, bl;
-                split4(v[s][m], bh, bl);
-                dft16_cols(F, bh, bl, yr[m], yi[m]);
+                split4<F16>(v[s][m], bh, bl);
+                dft16_cols<F16>(F, bh, bl, yr[m], yi[m]);
             }
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
@@ -760,7 +853,7 @@ __global__ __launch_bounds__(256) void mf2_kernel(float2* __restrict__ data, Fft
 #pragma unroll
                 for (int m = 0; m < 4; ++m) c[m] = c_mul(make_float2(yr[m][i], yi[m][i]), tw1[16 * q + 4 * w + m]);
                 u32x4 h, l;
-                split4(c, h, l);
+                split4<F16>(c, h, l);
                 xh[(4 * q + w) * 16 + cn] = h;
                 xl[(4 * q + w) * 16 + cn] = l;
             }
@@ -770,10 +863,10 @@ __global__ __launch_bounds__(256) void mf2_kernel(float2* __restrict__ data, Fft
             for (int a = 0; a < 4; ++a) {
                 const int q1 = 4 * w + a;
                 f32x4 zr, zi;
-                dft16_cols(F, xh[(4 * q1 + gq) * 16 + cn], xl[(4 * q1 + gq) * 16 + cn], zr, zi);
+                dft16_cols<F16>(F, xh[(4 * q1 + gq) * 16 + cn], xl[(4 * q1 + gq) * 16 + cn], zr, zi);
                 if constexpr (R2 == 1) {
 #pragma unroll
-                    for (int i = 0; i < 4; ++i) o[0][a][i] = sc(make_float2(zr[i], zi[i]));
+                    for (int i = 0; i < 4; ++i) o[0][a][i] = sc(make_float2(zr[i], zi[i]), us);
                 } else if (s == 0) {
 #pragma unroll
                     for (int i = 0; i < 4; ++i) y0[a][i] = make_float2(zr[i], zi[i]);
@@ -781,8 +874,8 @@ __global__ __launch_bounds__(256) void mf2_kernel(float2* __restrict__ data, Fft
 #pragma unroll
                     for (int i = 0; i < 4; ++i) {
                         const float2 t = c_mul(make_float2(zr[i], zi[i]), tw2[q1 + 16 * (4 * gq + i)]);
-                        o[0][a][i] = sc(c_add(y0[a][i], t));
-                        o[1][a][i] = sc(c_sub(y0[a][i], t));
+                        o[0][a][i] = sc(c_add(y0[a][i], t), us);
+                        o[1][a][i] = sc(c_sub(y0[a][i], t), us);
                     }
                 }
             }
@@ -841,6 +934,33 @@ __global__ __launch_bounds__(256) void mf2_kernel(float2* __restrict__ data, Fft
                     for (int e = 0; e < 4; ++e)
                         v[ph][la][jh][e] = lval[ph][la] ? bld<NTMP>(rs, vo[e], so) : make_float2(0.f, 0.f);
                 }
+        // fp16: one power-of-two scale per line, from its largest magnitude
+        // over the wave (a wave holds whole lines)
+        float ul[R2][LW];
+#pragma unroll
+        for (int ph = 0; ph < R2; ++ph)
+#pragma unroll
+            for (int la = 0; la < LW; ++la) {
+                ul[ph][la] = 1.f;
+                if constexpr (F16) {
+                    float mx = 0.f;
+#pragma unroll
+                    for (int jh = 0; jh < R2; ++jh)
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) mx = cmax(mx, v[ph][la][jh][e]);
+#pragma unroll
+                    for (int x = 1; x < 64; x <<= 1) mx = fmaxf(mx, __shfl_xor(mx, x));
+                    const float s0 = f16_scale(mx);
+                    ul[ph][la] = 16.f / s0;
+#pragma unroll
+                    for (int jh = 0; jh < R2; ++jh)
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) {
+                            v[ph][la][jh][e].x *= s0;
+                            v[ph][la][jh][e].y *= s0;
+                        }
+                }
+            }
         lds_barrier();   // twiddles in LDS
         u32x4* const wh = xh + w * 256;   // this wave's exchange region
         u32x4* const wl = xl + w * 256;
@@ -852,9 +972,9 @@ __global__ __launch_bounds__(256) void mf2_kernel(float2* __restrict__ data, Fft
 #pragma unroll
                 for (int jh = 0; jh < R2; ++jh) {
                     u32x4 ah, al;
-                    split4(v[ph][la][jh], ah, al);
+                    split4<F16>(v[ph][la][jh], ah, al);
                     f32x4 yr, yi;
-                    dft16_rows(F, ah, al, yr, yi);
+                    dft16_rows<F16>(F, ah, al, yr, yi);
                     // lane holds output q = cn of butterflies j = 16 jh + 4 gq + i
                     float2 c[4];
 #pragma unroll
@@ -864,15 +984,15 @@ __global__ __launch_bounds__(256) void mf2_kernel(float2* __restrict__ data, Fft
                     }
                     if constexpr (R2 == 1) {
                         u32x4 h, l;
-                        split4(c, h, l);
+                        split4<F16>(c, h, l);
                         wh[(la * 4 + gq) * 16 + cn] = h;
                         wl[(la * 4 + gq) * 16 + cn] = l;
                     } else {
 #pragma unroll
                         for (int s = 0; s < 2; ++s) {   // outputs i = s, s + 2: K slots 8 jh + 2 gq + {0, 1}
                             uint32_t h0, l0, h1, l1;
-                            split_c(c[s], h0, l0);
-                            split_c(c[s + 2], h1, l1);
+                            split_c<F16>(c[s], h0, l0);
+                            split_c<F16>(c[s + 2], h1, l1);
                             const int slot = ((la * 2 + s) * 4 + 2 * jh + (gq >> 1)) * 16 + cn;
                             u32x2* const ph2 = reinterpret_cast<u32x2*>(wh + slot) + (gq & 1);
                             u32x2* const pl2 = reinterpret_cast<u32x2*>(wl + slot) + (gq & 1);
@@ -894,19 +1014,20 @@ __global__ __launch_bounds__(256) void mf2_kernel(float2* __restrict__ data, Fft
 #pragma unroll
                 for (int s = 0; s < R2; ++s) {
                     const int slot = ((la * R2 + s) * 4 + gq) * 16 + cn;
-                    dft16_cols(F, wh[slot], wl[slot], zr[s], zi[s]);
+                    dft16_cols<F16>(F, wh[slot], wl[slot], zr[s], zi[s]);
                 }
                 // lane holds outputs q' = 4 gq + i of butterflies j1 = cn + 16 s
                 if constexpr (R2 == 1) {
 #pragma unroll
-                    for (int i = 0; i < 4; ++i) bst<NTMP>(rs, ocn + offq[i], so, sc(make_float2(zr[0][i], zi[0][i])));
+                    for (int i = 0; i < 4; ++i)
+                        bst<NTMP>(rs, ocn + offq[i], so, sc(make_float2(zr[0][i], zi[0][i]), ul[ph][la]));
                 } else {
 #pragma unroll
                     for (int i = 0; i < 4; ++i) {
                         const float2 y0 = make_float2(zr[0][i], zi[0][i]);
                         const float2 t = c_mul(make_float2(zr[1][i], zi[1][i]), tw2[cn + 16 * (4 * gq + i)]);
-                        bst<NTMP>(rs, ocn + offq[i], so, sc(c_add(y0, t)));
-                        bst<NTMP>(rs, ocn + offq[i], so + off256, sc(c_sub(y0, t)));
+                        bst<NTMP>(rs, ocn + offq[i], so, sc(c_add(y0, t), ul[ph][la]));
+                        bst<NTMP>(rs, ocn + offq[i], so + off256, sc(c_sub(y0, t), ul[ph][la]));
                     }
                 }
             }
@@ -1100,13 +1221,27 @@ constexpr int kMfOcc3 = 1 << 9;
 constexpr int kMfOcc2 = 1 << 10;
 
 struct Mf2Entry {
-    int N, strided, flags;
+    int N, strided, flags, f16;
     const void* fn;
 };
 
-template <int N, bool S, int F>
+template <int N, bool S, int F, bool F16 = false>
 void mf2_add(std::vector<Mf2Entry>& v) {
-    v.push_back(Mf2Entry{N, S, F, (const void*)&mf2_kernel<N, S, F>});
+    v.push_back(Mf2Entry{N, S, F, F16 ? 1 : 0, (const void*)&mf2_kernel<N, S, F, F16>});
+}
+
+// Split-FP16 variants: the split-BF16 defaults and their near neighbours.
+template <int N>
+void mf2_add_len_f16(std::vector<Mf2Entry>& v) {
+    mf2_add<N, false, KF_XCD | KF_NT, true>(v);
+    mf2_add<N, false, KF_NT, true>(v);
+    mf2_add<N, true, KF_XCD | KF_NT, true>(v);
+    mf2_add<N, true, KF_NT, true>(v);
+    mf2_add<N, true, KF_XCD, true>(v);
+    mf2_add<N, true, KF_A2MAJ | KF_NT, true>(v);
+    mf2_add<N, true, kMfRowAdj | KF_A2MAJ, true>(v);
+    mf2_add<N, true, kMfRowAdj | KF_A2MAJ | KF_NT, true>(v);
+    mf2_add<N, true, kMfRowAdj | KF_XCD | KF_NT, true>(v);
 }
 
 template <int N>
@@ -1147,6 +1282,8 @@ const std::vector<Mf2Entry>& mf2_registry() {
         std::vector<Mf2Entry> v;
         mf2_add_len<256>(v);
         mf2_add_len<512>(v);
+        mf2_add_len_f16<256>(v);
+        mf2_add_len_f16<512>(v);
         return v;
     }();
     return reg;
@@ -1164,26 +1301,27 @@ const std::vector<Mf2LongEntry>& mf2_long_registry() {
     return reg;
 }
 
-// flags of the built variants of length N (any direction)
-void mf2_flags(int N, int strided, std::vector<int>* out) {
+// flags of the built variants of length N (any direction); f16: split-FP16
+void mf2_flags(int N, int strided, int f16, std::vector<int>* out) {
     if (N >= 1024) {
-        if (strided) return;
+        if (strided || f16) return;
         for (const Mf2LongEntry& e : mf2_long_registry())
             if (e.N == N && e.dir == -1) out->push_back(e.flags);
         return;
     }
     for (const Mf2Entry& e : mf2_registry())
-        if (e.N == N && e.strided == (strided != 0)) out->push_back(e.flags);
+        if (e.N == N && e.strided == (strided != 0) && e.f16 == f16) out->push_back(e.flags);
 }
 
-const void* mf2_find(int N, int strided, int flags, int dir) {
+const void* mf2_find(int N, int strided, int flags, int dir, int f16) {
     if (N >= 1024) {
+        if (f16) return nullptr;
         for (const Mf2LongEntry& e : mf2_long_registry())
             if (e.N == N && !strided && e.flags == flags && e.dir == (dir < 0 ? -1 : 1)) return e.fn;
         return nullptr;
     }
     for (const Mf2Entry& e : mf2_registry())
-        if (e.N == N && e.strided == (strided != 0) && e.flags == flags) return e.fn;
+        if (e.N == N && e.strided == (strided != 0) && e.flags == flags && e.f16 == f16) return e.fn;
     return nullptr;
 }
 
@@ -1252,11 +1390,12 @@ static uint64_t pass_extent(const AxisPass& p, int N) {
     return 8ull * uint64_t(p.a1.off(p.a1.n - 1) + p.a2.off(p.a2.n - 1) + p.fft.off(uint32_t(N - 1))) + 8;
 }
 
-// The interleaved-K kernels run split-BF16 passes of 256 and 512 points (16-
-// line tiles) and contiguous lines of 1024 and 2048 points (8 and 4 lines),
-// whose byte offsets fit 32 bits.
+// The interleaved-K kernels run split-BF16 and split-FP16 passes of 256 and
+// 512 points (16-line tiles) and split-BF16 contiguous lines of 1024 and 2048
+// points (8 and 4 lines), whose byte offsets fit 32 bits.
 static bool mf2_fits(const AxisPass& p, int N, a2afft_precision prec) {
-    if (prec != A2AFFT_SPLIT_BF16 || (N != 256 && N != 512 && N != 1024 && N != 2048)) return false;
+    if (prec != A2AFFT_SPLIT_BF16 && prec != A2AFFT_SPLIT_FP16) return false;
+    if (N != 256 && N != 512 && !(prec == A2AFFT_SPLIT_BF16 && (N == 1024 || N == 2048))) return false;
     if (N >= 1024 && p.strided) return false;
     if (pass_extent(p, N) >= (1ull << 32)) return false;
     FftGeom g;
@@ -1278,7 +1417,7 @@ int mfma_candidates(const AxisPass& p, int N, a2afft_precision prec, KSel* out, 
     int n = 0;
     if (mf2_fits(p, N, prec)) {
         std::vector<int> fl;
-        mf2_flags(N, p.strided, &fl);
+        mf2_flags(N, p.strided, prec == A2AFFT_SPLIT_FP16 ? 1 : 0, &fl);
         for (int f : fl)
             for (int occ : {0, kMfOcc3, kMfOcc2})
                 if (n < max && (N <= 512 || occ == 0)) out[n++] = KSel{mf2_tile_lines(N), 0, f | occ};
@@ -1287,8 +1426,9 @@ int mfma_candidates(const AxisPass& p, int N, a2afft_precision prec, KSel* out, 
     return n;
 }
 
-// Defaults of the interleaved-K kernel, measured on 512^3 complex64 over dims
-// [1,4,4] (ms per pass, whole passes unless noted):
+// Defaults of the interleaved-K kernel (split-BF16; split-FP16 takes the
+// same), measured on 512^3 complex64 over dims [1,4,4] (ms per pass, whole
+// passes unless noted):
 //   contiguous (z): XCD order + non-temporal 0.376-0.380 (hardware order 0.408);
 //   far-strided (x, rows 128 KiB apart): row-adjacent lanes + z tiles
 //     outermost 0.448-0.452 (XCD + non-temporal without row-adjacent lanes
@@ -1315,7 +1455,7 @@ int launch_mfma_pass(const AxisPass& p, void* data, int N, int dir, a2afft_preci
     float2* dp = (float2*)data;
     if (ks.E == 0) {   // interleaved-K kernel
         if (!mf2_fits(p, N, prec)) return set_error(A2AFFT_E_UNSUPPORTED, "split precision: no interleaved-K kernel");
-        const void* fn = mf2_find(N, p.strided, ks.flags & ~(kMfOcc3 | kMfOcc2), dir);
+        const void* fn = mf2_find(N, p.strided, ks.flags & ~(kMfOcc3 | kMfOcc2), dir, prec == A2AFFT_SPLIT_FP16 ? 1 : 0);
         if (!fn) return set_error(A2AFFT_E_UNSUPPORTED, "split precision: no variant flags=%d", ks.flags);
         // occupancy variants: extra dynamic LDS leaves room for 3 or 2 workgroups per CU
         const size_t dyn = (ks.flags & kMfOcc2) ? size_t(28 << 10) : (ks.flags & kMfOcc3) ? size_t(16 << 10) : 0;

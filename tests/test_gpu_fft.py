@@ -447,13 +447,12 @@ def test_halfx_variants_bit_identical(a2afft_mod, cuda, case):
 @pytest.mark.parametrize("case", [((512, 16, (1, 4, 4), "c64", "fp32"), 128), ((512, 16, (1, 4, 4), "c128", "fp64"), 64),
                                   ((256, 8, (1, 2, 4), "c128", "fp64"), 0), ((256, 8, (1, 2, 4), "c64", "fp32"), 0),
                                   ((512, 16, (1, 4, 4), "c64", "split_bf16"), 128),
-                                  ((512, 16, (1, 4, 4), "c64", "split_fp16"), 0)],
+                                  ((512, 16, (1, 4, 4), "c64", "split_fp16"), 128)],
                          ids=lambda c: str(c))
 def test_default_chunk_width(a2afft_mod, cuda, case):
     """New plans alternate the x and y passes over windows of at most 256 MiB
     only for grids above 512 MiB on the 512-point half-width-exchange tiles
-    or the interleaved-K split-BF16 kernel; every other plan runs whole
-    passes."""
+    or the interleaved-K split kernels; every other plan runs whole passes."""
     (Ng, P, dims, dtype, prec), width = case
     dist = a2afft_mod.Distribution(Ng, P, dims=dims, dtype=dtype)
     plan = a2afft_mod.Dfft(dist, precision=prec)

@@ -159,3 +159,73 @@ def test_split_bf16_rank_segments_256(a2afft_mod, cuda):
     del got, ref
     plan.inverse(buf)
     assert _rel(buf.cpu().numpy(), x.astype(np.complex128)) <= 2 * TOL["split_bf16"]
+
+
+# interleaved-K split-FP16 variants (T=16, E=0)
+_MF2_F16_FLAGS = {0: [5, 4], 1: [5, 4, 1, 12, 264, 268, 261], 2: [5, 4, 1, 12, 261]}
+
+
+def test_split_fp16_variants_same_bits(a2afft_mod, cuda):
+    """The interleaved-K kernel in FP16 (per-line power-of-two operand scale):
+    every tile order and cache policy computes the same bits on every pass,
+    within the FP16 bound of FP64; the split-K-halves kernel (E = N, one scale
+    per butterfly) stays within the same bound."""
+    import torch
+    shape = (256, 512, 256)
+    x, d0 = _rand(torch, shape, 23)
+    ref = np.fft.fftn(x.astype(np.complex128))
+    base = a2afft_mod.GridDfft(shape, "split_fp16")
+    assert "interleaved-K" in base.describe()
+    want = d0.clone()
+    base.forward(want)
+    assert _rel(want.cpu().numpy().reshape(shape), ref) <= TOL["split_fp16"]
+    for p in range(3):
+        for fl in _MF2_F16_FLAGS[p]:
+            plan = a2afft_mod.GridDfft(shape, "split_fp16")
+            plan.set_kernel(p, 16, 0, fl)
+            d = d0.clone()
+            plan.forward(d)
+            assert torch.equal(torch.view_as_real(d), torch.view_as_real(want)), (p, fl)
+        n = shape[[2, 0, 1][p]]
+        plan = a2afft_mod.GridDfft(shape, "split_fp16")
+        plan.set_kernel(p, 16, n, 0)
+        d = d0.clone()
+        plan.forward(d)
+        err = _rel(d.cpu().numpy().reshape(shape), ref)
+        assert err <= TOL["split_fp16"], (p, err)
+
+
+def test_split_fp16_line_scales(a2afft_mod, cuda):
+    """Lines of very different magnitude in one 16-line tile (each line scaled
+    by 2^k, k from -60 to 60): every line's transform keeps its own relative
+    accuracy, on a contiguous and both strided passes, forward and inverse."""
+    import torch
+    shape = (16, 16, 512)   # 512-point z lines; x and y columns of 16
+    g = np.random.default_rng(5)
+    x = (g.uniform(-1, 1, shape) + 1j * g.uniform(-1, 1, shape))
+    k = g.integers(-60, 61, size=shape[:2])
+    x = (x * (2.0 ** k)[:, :, None]).astype(np.complex64)
+    d = torch.from_numpy(x.reshape(-1).copy()).cuda()
+    plan = a2afft_mod.GridDfft(shape, "split_fp16")
+    a2afft_mod.execute_pass(plan, d, 0, -1)   # z pass: the contiguous kernel
+    got = d.cpu().numpy().reshape(shape)
+    ref = np.fft.fft(x.astype(np.complex128), axis=2)
+    err = np.linalg.norm(got - ref, axis=2) / np.linalg.norm(ref, axis=2)
+    assert err.max() <= TOL["split_fp16"], err.max()
+    # strided 512-point columns: the x pass of a 512 x 16 x 64 grid, each (y, z) line scaled by its own 2^k
+    shape2 = (512, 16, 64)
+    x2 = g.uniform(-1, 1, shape2) + 1j * g.uniform(-1, 1, shape2)
+    k2 = g.integers(-60, 61, size=shape2[1:])
+    x2 = (x2 * (2.0 ** k2)[None, :, :]).astype(np.complex64)
+    d2 = torch.from_numpy(x2.reshape(-1).copy()).cuda()
+    plan2 = a2afft_mod.GridDfft(shape2, "split_fp16")
+    a2afft_mod.execute_pass(plan2, d2, 1, -1)
+    got2 = d2.cpu().numpy().reshape(shape2)
+    ref2 = np.fft.fft(x2.astype(np.complex128), axis=0)
+    err2 = np.linalg.norm(got2 - ref2, axis=0) / np.linalg.norm(ref2, axis=0)
+    assert err2.max() <= TOL["split_fp16"], err2.max()
+    a2afft_mod.execute_pass(plan2, d2, 1, 1)
+    back = d2.cpu().numpy().reshape(shape2)
+    want = 512 * x2.astype(np.complex128)
+    err3 = np.linalg.norm(back - want, axis=0) / np.linalg.norm(want, axis=0)
+    assert err3.max() <= 2 * TOL["split_fp16"], err3.max()
