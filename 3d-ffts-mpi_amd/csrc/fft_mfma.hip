@@ -1048,7 +1048,7 @@ __global__ __launch_bounds__(256) void mf2_kernel(float2* __restrict__ data, Fft
 // q + 16 (j % R2); stage-1 output q' of butterflies j2 % 16 + 16 s (s < R2)
 // are the R2 inputs of radix-R2 butterfly j2 = j1 % 16 + 16 q', twiddled by
 // W_N^(s j2), written to j2 + 256 q2.
-template <int N, int DIR, int FLAGS>
+template <int N, int DIR, int FLAGS, bool F16 = false>
 __global__ __launch_bounds__(256) void mf2_long_kernel(float2* __restrict__ data, FftGeom g,
                                                        const float2* __restrict__ tw, uint32_t extent) {
     static_assert(N == 1024 || N == 2048, "long lengths");
@@ -1066,7 +1066,12 @@ __global__ __launch_bounds__(256) void mf2_long_kernel(float2* __restrict__ data
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
     const uint32_t block = (FLAGS & KF_XCD) ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
     const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(data, 0, int(extent), 0x00020000);
-    tw1[tid] = tw[16 + tid];
+    if constexpr (F16) {   // fp16: stage-1 operands divided by 16 through the twiddles
+        const float2 t = tw[16 + tid];
+        tw1[tid] = make_float2(t.x * 0.0625f, t.y * 0.0625f);
+    } else {
+        tw1[tid] = tw[16 + tid];
+    }
 
     F16Op F;
     {
@@ -1077,8 +1082,8 @@ __global__ __launch_bounds__(256) void mf2_long_kernel(float2* __restrict__ data
             fr[e] = make_float2(wv.x, -wv.y);
             fi[e] = make_float2(wv.y, wv.x);
         }
-        split4(fr, F.rh, F.rl);
-        split4(fi, F.ih, F.il);
+        split4<F16>(fr, F.rh, F.rl);
+        split4<F16>(fi, F.ih, F.il);
     }
     const uint32_t ocn = 8u * uint32_t(g.fft.off(uint32_t(cn)));
     uint32_t vo[4], offq[4];
@@ -1107,6 +1112,31 @@ __global__ __launch_bounds__(256) void mf2_long_kernel(float2* __restrict__ data
 #pragma unroll
             for (int e = 0; e < 4; ++e) v[la][jh][e] = lval[la] ? bld<NTMP>(rs, vo[e], so) : make_float2(0.f, 0.f);
         }
+    // fp16: one power-of-two scale per line (a wave holds whole lines); the
+    // stores multiply by ul = 16 / scale
+    float ul[LW];
+#pragma unroll
+    for (int la = 0; la < LW; ++la) {
+        ul[la] = 1.f;
+        if constexpr (F16) {
+            float mx = 0.f;
+#pragma unroll
+            for (int jh = 0; jh < R2; ++jh)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) mx = cmax(mx, v[la][jh][e]);
+#pragma unroll
+            for (int x = 1; x < 64; x <<= 1) mx = fmaxf(mx, __shfl_xor(mx, x));
+            const float s0 = f16_scale(mx);
+            ul[la] = 16.f / s0;
+#pragma unroll
+            for (int jh = 0; jh < R2; ++jh)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    v[la][jh][e].x *= s0;
+                    v[la][jh][e].y *= s0;
+                }
+        }
+    }
     lds_barrier();   // twiddles in LDS
     uint32_t* const wh = reinterpret_cast<uint32_t*>(xh + w * 512);
     uint32_t* const wl = reinterpret_cast<uint32_t*>(xl + w * 512);
@@ -1117,15 +1147,15 @@ __global__ __launch_bounds__(256) void mf2_long_kernel(float2* __restrict__ data
 #pragma unroll
         for (int jh = 0; jh < R2; ++jh) {
             u32x4 ah, al;
-            split4(v[la][jh], ah, al);
+            split4<F16>(v[la][jh], ah, al);
             f32x4 yr, yi;
-            dft16_rows(F, ah, al, yr, yi);
+            dft16_rows<F16>(F, ah, al, yr, yi);
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const int j = 16 * jh + 4 * gq + i;
                 const int s = j % R2, r = j / R2;
                 uint32_t h, l;
-                split_c(c_mul(make_float2(yr[i], yi[i]), tw1[16 * r + cn]), h, l);
+                split_c<F16>(c_mul(make_float2(yr[i], yi[i]), tw1[16 * r + cn]), h, l);
                 const int slot = ((la * R2 + s) * 4 + (r >> 2)) * 16 + cn;
                 wh[4 * slot + (r & 3)] = h;
                 wl[4 * slot + (r & 3)] = l;
@@ -1140,7 +1170,7 @@ __global__ __launch_bounds__(256) void mf2_long_kernel(float2* __restrict__ data
         for (int s = 0; s < R2; ++s) {
             const int slot = ((la * R2 + s) * 4 + gq) * 16 + cn;
             f32x4 zr, zi;
-            dft16_cols(F, xh[w * 512 + slot], xl[w * 512 + slot], zr, zi);
+            dft16_cols<F16>(F, xh[w * 512 + slot], xl[w * 512 + slot], zr, zi);
 #pragma unroll
             for (int i = 0; i < 4; ++i) z[s][i] = make_float2(zr[i], zi[i]);
         }
@@ -1155,9 +1185,15 @@ __global__ __launch_bounds__(256) void mf2_long_kernel(float2* __restrict__ data
                 for (int s = 1; s < R2; ++s) x[s] = c_mul(z[s][i], tw[16 + 256 + j2 * R2 + s]);
                 dft<R2, DIR>(x);
 #pragma unroll
-                for (int q2 = 0; q2 < R2; ++q2)
+                for (int q2 = 0; q2 < R2; ++q2) {
+                    float2 o = x[q2];
+                    if constexpr (F16) {
+                        o.x *= ul[la];
+                        o.y *= ul[la];
+                    }
                     bst<NTMP>(rs, ocn + offq[i], lbase[la] + 8u * uint32_t(g.fft.off(uint32_t(256 * q2))),
-                              make_float2(x[q2].x * fs, x[q2].y * fs));
+                              make_float2(o.x * fs, o.y * fs));
+                }
             }
         }
     }
@@ -1267,14 +1303,14 @@ void mf2_add_len(std::vector<Mf2Entry>& v) {
 
 // Long contiguous lines: one entry per direction (the radix-R2 stage's sign).
 struct Mf2LongEntry {
-    int N, dir, flags;
+    int N, dir, flags, f16;
     const void* fn;
 };
 
-template <int N, int F>
+template <int N, int F, bool F16 = false>
 void mf2_long_add(std::vector<Mf2LongEntry>& v) {
-    v.push_back(Mf2LongEntry{N, -1, F, (const void*)&mf2_long_kernel<N, -1, F>});
-    v.push_back(Mf2LongEntry{N, 1, F, (const void*)&mf2_long_kernel<N, 1, F>});
+    v.push_back(Mf2LongEntry{N, -1, F, F16 ? 1 : 0, (const void*)&mf2_long_kernel<N, -1, F, F16>});
+    v.push_back(Mf2LongEntry{N, 1, F, F16 ? 1 : 0, (const void*)&mf2_long_kernel<N, 1, F, F16>});
 }
 
 const std::vector<Mf2Entry>& mf2_registry() {
@@ -1296,6 +1332,10 @@ const std::vector<Mf2LongEntry>& mf2_long_registry() {
         mf2_long_add<1024, KF_NT>(v);
         mf2_long_add<2048, KF_XCD | KF_NT>(v);
         mf2_long_add<2048, KF_NT>(v);
+        mf2_long_add<1024, KF_XCD | KF_NT, true>(v);
+        mf2_long_add<1024, KF_NT, true>(v);
+        mf2_long_add<2048, KF_XCD | KF_NT, true>(v);
+        mf2_long_add<2048, KF_NT, true>(v);
         return v;
     }();
     return reg;
@@ -1304,9 +1344,9 @@ const std::vector<Mf2LongEntry>& mf2_long_registry() {
 // flags of the built variants of length N (any direction); f16: split-FP16
 void mf2_flags(int N, int strided, int f16, std::vector<int>* out) {
     if (N >= 1024) {
-        if (strided || f16) return;
+        if (strided) return;
         for (const Mf2LongEntry& e : mf2_long_registry())
-            if (e.N == N && e.dir == -1) out->push_back(e.flags);
+            if (e.N == N && e.dir == -1 && e.f16 == f16) out->push_back(e.flags);
         return;
     }
     for (const Mf2Entry& e : mf2_registry())
@@ -1315,9 +1355,8 @@ void mf2_flags(int N, int strided, int f16, std::vector<int>* out) {
 
 const void* mf2_find(int N, int strided, int flags, int dir, int f16) {
     if (N >= 1024) {
-        if (f16) return nullptr;
         for (const Mf2LongEntry& e : mf2_long_registry())
-            if (e.N == N && !strided && e.flags == flags && e.dir == (dir < 0 ? -1 : 1)) return e.fn;
+            if (e.N == N && !strided && e.flags == flags && e.dir == (dir < 0 ? -1 : 1) && e.f16 == f16) return e.fn;
         return nullptr;
     }
     for (const Mf2Entry& e : mf2_registry())
@@ -1391,11 +1430,11 @@ static uint64_t pass_extent(const AxisPass& p, int N) {
 }
 
 // The interleaved-K kernels run split-BF16 and split-FP16 passes of 256 and
-// 512 points (16-line tiles) and split-BF16 contiguous lines of 1024 and 2048
-// points (8 and 4 lines), whose byte offsets fit 32 bits.
+// 512 points (16-line tiles) and contiguous lines of 1024 and 2048 points (8
+// and 4 lines), whose byte offsets fit 32 bits.
 static bool mf2_fits(const AxisPass& p, int N, a2afft_precision prec) {
     if (prec != A2AFFT_SPLIT_BF16 && prec != A2AFFT_SPLIT_FP16) return false;
-    if (N != 256 && N != 512 && !(prec == A2AFFT_SPLIT_BF16 && (N == 1024 || N == 2048))) return false;
+    if (N != 256 && N != 512 && N != 1024 && N != 2048) return false;
     if (N >= 1024 && p.strided) return false;
     if (pass_extent(p, N) >= (1ull << 32)) return false;
     FftGeom g;
@@ -1408,7 +1447,7 @@ int mfma_pass_check(const AxisPass& p, int N, a2afft_precision prec) {
     if (mf2_fits(p, N, prec) || legacy_fits(p, N)) return A2AFFT_OK;
     return set_error(A2AFFT_E_UNSUPPORTED,
                      "split precision: no kernel for a pass of length %d (16..512 in tiles of the axis, or "
-                     "split-BF16 contiguous 1024 and 2048)", N);
+                     "contiguous 1024 and 2048)", N);
 }
 
 // Kernel choices of a split pass: T = tile lines; E = 0 for the interleaved-K

@@ -186,6 +186,10 @@ def run_c3b(args, torch, a2afft):
     ws = pencils.clone()
     a2afft.execute_pass(plan_s, ws, 0, -1)
     rel_s, max_s = a2afft.error_metrics(ws, ref_p)
+    plan_h = a2afft.GridDfft((256, 256, 2048), "split_fp16")
+    ws.copy_(pencils)
+    a2afft.execute_pass(plan_h, ws, 0, -1)
+    rel_h, max_h = a2afft.error_metrics(ws, ref_p)
     del ref_p, ws
     plan = a2afft.GridDfft(shape, "fp32")
     ref_b = block.to(torch.complex128)
@@ -210,6 +214,7 @@ def run_c3b(args, torch, a2afft):
     t_staged = _events_ms(torch, staged, rot, args.steps, args.warmup)
     t_pencil = _events_ms(torch, lambda buf: a2afft.forward_1d_fft(buf, 2048), rot, args.steps, args.warmup)
     t_split = _events_ms(torch, lambda buf: a2afft.execute_pass(plan_s, buf, 0, -1), rot, args.steps, args.warmup)
+    t_split16 = _events_ms(torch, lambda buf: a2afft.execute_pass(plan_h, buf, 0, -1), rot, args.steps, args.warmup)
     finite = rot.finite()
     del rot
     rot = Rotation(torch, block, nrot)
@@ -232,9 +237,12 @@ def run_c3b(args, torch, a2afft):
         "config": {"workload": "one rank's 2048x256x256 complex64 share of 2048^3 over dims [1,8,8], "
                                "GRF P(k)~k^-3 seed 2 + 1e-3 noise, x stage", "name": "c3b"},
         "ms": {"staged_pack_fft_unpack": round(t_staged, 4), "fused_pencil_fft": round(t_pencil, 4),
-               "fused_block_x_pass": round(t_block, 4), "split_bf16_pencil_fft": round(t_split, 4)},
+               "fused_block_x_pass": round(t_block, 4), "split_bf16_pencil_fft": round(t_split, 4),
+               "split_fp16_pencil_fft": round(t_split16, 4)},
         "split_bf16_pencils": {"ms": round(t_split, 4), "rel_l2_vs_fp64": rel_s, "max_abs_vs_fp64": max_s,
                                "hbm_gbs": round(bytes_fft / (t_split * 1e-3) / 1e9, 1)},
+        "split_fp16_pencils": {"ms": round(t_split16, 4), "rel_l2_vs_fp64": rel_h, "max_abs_vs_fp64": max_h,
+                               "hbm_gbs": round(bytes_fft / (t_split16 * 1e-3) / 1e9, 1)},
         "gflops": round(5.0 * nloc * math.log2(2048) / (best * 1e-3) / 1e9, 1),
         "pattern_copy_gbs": {"pencil_fft": round(bytes_fft / (pc_pencil * 1e-3) / 1e9, 1),
                              "block_x_pass": round(bytes_fft / (pc_block * 1e-3) / 1e9, 1)},
