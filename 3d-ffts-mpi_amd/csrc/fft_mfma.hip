@@ -655,7 +655,13 @@ __device__ __forceinline__ float f16_scale(float mx) {
     const int sh = 15 - e;
     return mx > 0.f ? ldexpf(1.f, sh < -126 ? -126 : (sh > 126 ? 126 : sh)) : 1.f;
 }
-__device__ __forceinline__ float cmax(float m, float2 v) { return fmaxf(m, fmaxf(fabsf(v.x), fabsf(v.y))); }
+// Largest component magnitude, on the bit patterns: for magnitudes (sign
+// cleared) unsigned order is float order, and integer max needs no NaN
+// canonicalisation of its inputs (one v_max3_u32 per complex value).
+__device__ __forceinline__ uint32_t cmax(uint32_t m, float2 v) {
+    const uint32_t a = __float_as_uint(v.x) & 0x7fffffffu, b = __float_as_uint(v.y) & 0x7fffffffu;
+    return max(m, max(a, b));
+}
 
 template <bool NTMP>
 __device__ __forceinline__ float2 bld(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
@@ -725,7 +731,7 @@ __global__ __launch_bounds__(256) void mf2_kernel(float2* __restrict__ data, Fft
     __shared__ float2 tw1[256];
     __shared__ float2 tw2[R2 > 1 ? 256 : 1];    // W_512^k, k < 256
     // fp16: each strided column's largest magnitude, per wave
-    __shared__ float colmax[F16 && STRIDED ? 64 : 1];
+    __shared__ uint32_t colmax[F16 && STRIDED ? 64 : 1];
 
     const int tid = int(threadIdx.x);
     const int lane = tid & 63, cn = lane & 15, gq = lane >> 4;
@@ -766,14 +772,13 @@ __global__ __launch_bounds__(256) void mf2_kernel(float2* __restrict__ data, Fft
     // the inverse's 1/(nx*ny*nz) as one multiplication (exact for the
     // power-of-two lengths these kernels take); 1 when the pass does not scale
     const float fs = g.scale_mode == 0 ? 1.f : g.scale_mode == 2 ? float(1.0 / g.scale) : float(g.scale);
-    // u: fp16 only, 16 / (the line's operand scale), undoing the scaling
+    // u: fp16 only, fs * 16 / (the line's operand scale), undoing the scaling
+    // with the pass's scale in one multiplication (both factors are powers of
+    // two or fs is applied to a value the power of two scales exactly)
     auto sc = [&](float2 v, float u = 1.f) {
-        if constexpr (F16) {
-            v.x *= u;
-            v.y *= u;
-        }
-        v.x *= fs;
-        v.y *= fs;
+        const float f = F16 ? u : fs;
+        v.x *= f;
+        v.y *= f;
         return v;
     };
 
@@ -807,22 +812,22 @@ __global__ __launch_bounds__(256) void mf2_kernel(float2* __restrict__ data, Fft
         // magnitude over the 4 lane groups and the 4 waves
         float us = 1.f;
         if constexpr (F16) {
-            float mx = 0.f;
+            uint32_t mx = 0;
 #pragma unroll
             for (int s = 0; s < R2; ++s)
 #pragma unroll
                 for (int m = 0; m < 4; ++m)
 #pragma unroll
                     for (int e = 0; e < 4; ++e) mx = cmax(mx, v[s][m][e]);
-            mx = fmaxf(mx, __shfl_xor(mx, 16));
-            mx = fmaxf(mx, __shfl_xor(mx, 32));
+            mx = max(mx, uint32_t(__shfl_xor(int(mx), 16)));
+            mx = max(mx, uint32_t(__shfl_xor(int(mx), 32)));
             if (gq == 0) colmax[w * 16 + cn] = mx;
         }
         lds_barrier();   // twiddles (and column maxima) in LDS
         if constexpr (F16) {
-            const float mx = fmaxf(fmaxf(colmax[cn], colmax[16 + cn]), fmaxf(colmax[32 + cn], colmax[48 + cn]));
-            const float s0 = f16_scale(mx);
-            us = 16.f / s0;
+            const uint32_t mx = max(max(colmax[cn], colmax[16 + cn]), max(colmax[32 + cn], colmax[48 + cn]));
+            const float s0 = f16_scale(__uint_as_float(mx));
+            us = (16.f / s0) * fs;
 #pragma unroll
             for (int s = 0; s < R2; ++s)
 #pragma unroll
@@ -943,15 +948,15 @@ __global__ __launch_bounds__(256) void mf2_kernel(float2* __restrict__ data, Fft
             for (int la = 0; la < LW; ++la) {
                 ul[ph][la] = 1.f;
                 if constexpr (F16) {
-                    float mx = 0.f;
+                    uint32_t mx = 0;
 #pragma unroll
                     for (int jh = 0; jh < R2; ++jh)
 #pragma unroll
                         for (int e = 0; e < 4; ++e) mx = cmax(mx, v[ph][la][jh][e]);
 #pragma unroll
-                    for (int x = 1; x < 64; x <<= 1) mx = fmaxf(mx, __shfl_xor(mx, x));
-                    const float s0 = f16_scale(mx);
-                    ul[ph][la] = 16.f / s0;
+                    for (int x = 1; x < 64; x <<= 1) mx = max(mx, uint32_t(__shfl_xor(int(mx), x)));
+                    const float s0 = f16_scale(__uint_as_float(mx));
+                    ul[ph][la] = (16.f / s0) * fs;
 #pragma unroll
                     for (int jh = 0; jh < R2; ++jh)
 #pragma unroll
@@ -1113,21 +1118,21 @@ __global__ __launch_bounds__(256) void mf2_long_kernel(float2* __restrict__ data
             for (int e = 0; e < 4; ++e) v[la][jh][e] = lval[la] ? bld<NTMP>(rs, vo[e], so) : make_float2(0.f, 0.f);
         }
     // fp16: one power-of-two scale per line (a wave holds whole lines); the
-    // stores multiply by ul = 16 / scale
+    // stores multiply by ul = fs * 16 / scale
     float ul[LW];
 #pragma unroll
     for (int la = 0; la < LW; ++la) {
         ul[la] = 1.f;
         if constexpr (F16) {
-            float mx = 0.f;
+            uint32_t mx = 0;
 #pragma unroll
             for (int jh = 0; jh < R2; ++jh)
 #pragma unroll
                 for (int e = 0; e < 4; ++e) mx = cmax(mx, v[la][jh][e]);
 #pragma unroll
-            for (int x = 1; x < 64; x <<= 1) mx = fmaxf(mx, __shfl_xor(mx, x));
-            const float s0 = f16_scale(mx);
-            ul[la] = 16.f / s0;
+            for (int x = 1; x < 64; x <<= 1) mx = max(mx, uint32_t(__shfl_xor(int(mx), x)));
+            const float s0 = f16_scale(__uint_as_float(mx));
+            ul[la] = (16.f / s0) * fs;
 #pragma unroll
             for (int jh = 0; jh < R2; ++jh)
 #pragma unroll
@@ -1186,13 +1191,9 @@ __global__ __launch_bounds__(256) void mf2_long_kernel(float2* __restrict__ data
                 dft<R2, DIR>(x);
 #pragma unroll
                 for (int q2 = 0; q2 < R2; ++q2) {
-                    float2 o = x[q2];
-                    if constexpr (F16) {
-                        o.x *= ul[la];
-                        o.y *= ul[la];
-                    }
+                    const float f = F16 ? ul[la] : fs;   // fp16: fs * 16 / (the line's scale)
                     bst<NTMP>(rs, ocn + offq[i], lbase[la] + 8u * uint32_t(g.fft.off(uint32_t(256 * q2))),
-                              make_float2(o.x * fs, o.y * fs));
+                              make_float2(x[q2].x * f, x[q2].y * f));
                 }
             }
         }
