@@ -227,3 +227,35 @@ def test_split_fp16_line_scales(a2afft_mod, cuda):
     want = 512 * x2.astype(np.complex128)
     err3 = np.linalg.norm(back - want, axis=0) / np.linalg.norm(want, axis=0)
     assert err3.max() <= 2 * TOL["split_fp16"], err3.max()
+
+
+@pytest.mark.parametrize("prec", ["split_bf16", "split_fp16"])
+def test_split_autotune_and_graph_keep_bits(a2afft_mod, cuda, prec):
+    """Autotune of a split plan stays within the interleaved-K family (every
+    tile order, cache policy and occupancy computes the same bits), and graph
+    replays of the tuned plan equal its eager forward, with eager work on the
+    device between replays."""
+    import torch
+    shape = (128, 512, 256)
+    x, d0 = _rand(torch, shape, 29)
+    plan = a2afft_mod.GridDfft(shape, prec)
+    want = d0.clone()
+    plan.forward(want)
+    work = d0.clone()
+    plan.autotune(work, iters=1)
+    assert "interleaved-K" in plan.describe().splitlines()[0]   # the 256-point z pass
+    got = d0.clone()
+    plan.forward(got)
+    assert torch.equal(torch.view_as_real(got), torch.view_as_real(want))
+    other = torch.empty_like(d0)
+    for rep in range(3):
+        got.copy_(d0)
+        torch.cuda.synchronize()
+        plan.graph_launch(got, -1)
+        torch.cuda.synchronize()
+        assert torch.equal(torch.view_as_real(got), torch.view_as_real(want)), rep
+        other.copy_(d0)
+        plan.forward(other)
+        torch.cuda.synchronize()
+    ref = np.fft.fftn(x.astype(np.complex128))
+    assert _rel(want.cpu().numpy().reshape(shape), ref) <= TOL[prec]
