@@ -31,6 +31,11 @@ int register_kernels_f32(KEntry* out) {
     add_extra_flags<float, 512, 32, 32, true>(out, &n);
     add_shape<float, 512, 8, 4, true>(out, &n);      // 256 threads, 16 KiB LDS, 32 B rows
     add_shape<float, 512, 8, 4, false>(out, &n);     // 256 threads, 16 KiB LDS
+    add_shape<float, 512, 8, 2, false>(out, &n);     // 128 threads, 8 KiB LDS
+    add_extra_flags<float, 512, 8, 2, false>(out, &n);
+    add_shape<float, 512, 8, 1, false>(out, &n);     // 64 threads, one line
+    add_variant<float, 512, 8, 1, false, KF_XCD | KF_NT>(out, &n);
+    add_variant<float, 512, 8, 1, false, KF_XCD | KF_PAD | KF_NT>(out, &n);
     add_shape<float, 512, 16, 16, false>(out, &n);   // 512 threads, 64 KiB LDS
     add_extra_flags<float, 512, 16, 16, false>(out, &n);
     add_shape<float, 512, 32, 16, false>(out, &n);   // 256 threads, 64 KiB LDS
