@@ -175,7 +175,10 @@ int fft_candidates(const AxisPass& p, a2afft_dtype dt, int N, KSel* out, int max
 // segments (the z pass of 512^3 over dims [1,4,4]) run fastest two lines to
 // a 128-thread workgroup: 0.366 ms against 0.380 ms for 16-line tiles
 // (pattern copies 6.05 against 5.80 TB/s); whole lines (512^3 in one block)
-// do not gain (0.406 against 0.393 ms).
+// do not gain (0.406 against 0.393 ms).  Complex128 256-point lines in 4
+// segments (256^3 over dims [1,2,4]) likewise prefer 4-line tiles: 0.097
+// against 0.102 ms; complex128 512-point lines keep 8-line tiles (two-line
+// tiles 0.81 against 0.78 ms).
 struct Pref {
     int N, dbl, strided, far, seg;
     KSel s;
@@ -190,6 +193,7 @@ static const Pref kPrefs[] = {
     {512, 1, 1, -1, -1, {8, 16, KF_XCD | KF_NT}},
     {256, 0, 1, 1, -1, {32, 32, KF_XCD | KF_PAD}},
     {256, 0, 1, -1, -1, {16, 32, KF_XCD}},
+    {256, 1, 0, -1, 1, {4, 16, KF_XCD | KF_PAD | KF_HALFX}},
     {256, 1, 0, -1, -1, {16, 16, KF_XCD | KF_PAD | KF_HALFX}},
     {256, 1, 1, 1, -1, {8, 16, KF_XCD | KF_A2MAJ | KF_HALFX}},
     {256, 1, 1, -1, -1, {8, 16, KF_XCD | KF_PAD | KF_HALFX}},

@@ -31,6 +31,13 @@ int register_kernels_f64(KEntry* out) {
     add_shape<double, 512, 8, 4, true>(out, &n);     // 256 threads, 32 KiB LDS
     add_shape<double, 512, 8, 4, false>(out, &n);
     add_shape<double, 512, 16, 4, false>(out, &n);
+    // few-line contiguous tiles (lines that cross rank blocks)
+    add_shape<double, 512, 8, 2, false>(out, &n);    // 128 threads
+    add_extra_flags<double, 512, 8, 2, false>(out, &n);
+    add_halfx<double, 512, 8, 2, false>(out, &n);
+    add_shape<double, 256, 16, 4, false>(out, &n);   // 64 threads
+    add_extra_flags<double, 256, 16, 4, false>(out, &n);
+    add_halfx<double, 256, 16, 4, false>(out, &n);
     // half-width exchanges: the 152-VGPR FP64 kernels fit 3 waves a SIMD, a
     // full complex128 exchange leaves room for 2
     add_halfx<double, 256, 16, 8, false>(out, &n);
