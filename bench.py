@@ -58,6 +58,49 @@ def host_cores():
     return (min(aff, quota) if quota else aff), aff, quota, model
 
 
+def library_gpu_fftn(a2afft, torch, Ng, precision, rounds=3, iters=5):
+    """The same grid size through the GPU's vendor FFT library (rocFFT, via
+    torch.fft.fftn / ifftn, out of place) and through this library's plain-grid
+    plan (natural C order, no virtual ranks, in place), on one natural-order
+    Ng^3 array: ms per transform, timed as forward + inverse pairs (so values
+    stay finite) and halved, median of `rounds` rounds of `iters` pairs."""
+    dt = torch.complex64 if precision == "fp32" else torch.complex128
+    x = torch.empty((Ng, Ng, Ng), dtype=dt, device="cuda")
+    a2afft.fill_random(x, 7, "uniform")
+    y = torch.empty_like(x)
+    plan = a2afft.GridDfft((Ng, Ng, Ng), precision)
+
+    def pair_ms(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        res = []
+        for _ in range(rounds):
+            e0.record()
+            for _ in range(iters):
+                fn()
+            e1.record()
+            torch.cuda.synchronize()
+            res.append(e0.elapsed_time(e1) / iters / 2)
+        return sorted(res)[len(res) // 2]
+
+    def roc():
+        torch.fft.fftn(x, out=y)
+        torch.fft.ifftn(y, out=x)
+
+    def ours():
+        plan.forward(x)
+        plan.inverse(x)
+
+    roc_ms = pair_ms(roc)
+    ours_ms = pair_ms(ours)
+    plan.destroy()
+    del x, y
+    return {"what": f"one natural-order {Ng}^3 {'complex64' if precision == 'fp32' else 'complex128'} grid, "
+                    "ms per transform (forward + inverse pairs, halved, median of 3)",
+            "rocfft_fftn_ms": round(roc_ms, 4), "a2afft_grid_plan_ms": round(ours_ms, 4)}
+
+
 def cpu_baseline(Ng, P, dims, threads, seed, samples=3):
     """The oracle (C restatement of the reference stage order, FP64) on full
     transforms of the same grid shape, OpenMP over `threads` cores: the median
@@ -672,6 +715,8 @@ def main():
                      "traffic": traffic},
         "cpu_baseline": cpu,
     }
+    if args.config == "c3" and args.precision in ("fp32", "fp64"):
+        out["library_gpu"] = library_gpu_fftn(a2afft, torch, Ng, args.precision)
     if args.config == "c3" and not args.no_variants and args.ng in (None, 512):
         log("[bench] precision variants on the seed-3 field")
         out["precision_variants_seed3"] = precision_variants(a2afft, torch, Ng, P, dims, 3, args.steps, args.warmup)
