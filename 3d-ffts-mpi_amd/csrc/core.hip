@@ -783,11 +783,11 @@ extern "C" int a2afft_plan_set_kernel(a2afft_plan p, int pass, int T, int E, int
 
 // The kernel family of a selection: variants within one family compute every
 // line with the same arithmetic (the fixed-schedule Stockham shapes, the
-// runtime-schedule tiles, the chirp-z pass, the direct DFT; the interleaved-K
+// runtime-schedule tiles, the per-pass-mapped tiles, the chirp-z pass, the direct DFT; the interleaved-K
 // or the split-K-halves MFMA kernels).
 static int sel_family(const a2afft_plan_s* p, const KSel& s) {
     if (is_split(p->prec)) return s.E == 0 ? 1 : 2;
-    return s.flags & (KF_MIX | KF_CZ | KF_DFT);
+    return s.flags & (KF_MIX | KF_MIXL | KF_CZ | KF_DFT);
 }
 
 // Times every built variant that fits each fused pass (forward direction,

@@ -87,6 +87,32 @@ struct MixedPlan {
 };
 // force_T > 0: that tile (lines per workgroup) if it fits the pass, else false.
 bool mixed_plan(const AxisPass& pass, int N, bool dbl, MixedPlan* mp, int force_T = 0);
+// Complex values a thread holds in the runtime schedule of a length (0: none).
+int mixed_plan_regs(int N, bool dbl);
+
+// Stockham FFT with butterflies mapped to threads pass by pass (KF_MIXL,
+// fft_mixl.hip) for 2^a 3^b 5^c lengths the fixed schedules do not cover:
+// every pass spreads its T*N/R butterflies over the workgroup, so radices
+// 16, 8, 4, 3 and 5 mix freely (480 = 8*4*3*5) instead of being limited to
+// divisors of one per-thread element count.
+struct MixlPlan {
+    int N, npass, T, NT, EM, twsize;   // EM: register budget, complex values per thread
+    int radix[kMixMaxPasses], ns[kMixMaxPasses], toff[kMixMaxPasses], pad[kMixMaxPasses];
+    FastDiv dns[kMixMaxPasses];   // j / ns[p]
+    FastDiv dB[kMixMaxPasses];    // butterfly index / (N / radix[p]) (contiguous tiles)
+    FastDiv dT;                   // butterfly index / T (strided tiles)
+    // Address steps of contiguous tiles, fixed per pass: the LDS slots of a
+    // butterfly's inputs (outputs) lie rds[p] (wrs[p]) apart; in global
+    // memory, gin between the first pass's inputs and gout between the last
+    // pass's outputs, or 0 where the axis's block extent breaks the step (then
+    // each address is computed).
+    int rds[kMixMaxPasses], wrs[kMixMaxPasses];
+    int64_t gin, gout;
+};
+bool mixl_plan(const AxisPass& pass, int N, bool dbl, MixlPlan* lp, int force_T = 0);
+const void* mixl_kernel_for(int EM, bool dbl, int strided, int flags, int dir);
+int mixl_lds_bytes(const MixlPlan& lp, bool dbl, int strided);
+const void* mixl_twiddles(const MixlPlan& lp, int dir, a2afft_dtype dtype);
 const void* mixed_kernel_for(int N, int T, int E, bool dbl, int strided, int dir);
 bool mixed_is_wide(int N, int T, int E);   // more than 256 threads: strided tiles only
 int mixed_lds_bytes(const MixedPlan& mp, bool dbl);

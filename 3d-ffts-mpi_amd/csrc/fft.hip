@@ -21,13 +21,20 @@ bool fft_length_supported(int N) { return N >= 2 && N <= kDftMaxN; }
 
 static bool is_dft(const KSel& s) { return (s.flags & KF_DFT) != 0; }
 static bool is_mix(const KSel& s) { return (s.flags & KF_MIX) != 0; }
+static bool is_mixl(const KSel& s) { return (s.flags & KF_MIXL) != 0; }
 static bool is_cz(const KSel& s) { return (s.flags & KF_CZ) != 0; }
 static KSel dft_sel(int N) { return KSel{1, N, KF_DFT}; }
 static KSel cz_sel(int N) { return KSel{1, N, KF_CZ}; }
 
-// Kernel choice for a length no fixed-schedule kernel covers: the
-// runtime-schedule Stockham when its shape fits the pass, else the chirp-z
-// pass (9 points and up), else the direct DFT.  Per GiB of complex64 lines the
+// Kernel choice for a length no fixed-schedule kernel covers: for 2^a 3^b 5^c
+// lengths, the runtime-schedule Stockham when it holds at most 24 values a
+// thread, else the per-pass-mapped Stockham, else the runtime schedule with
+// more; then the chirp-z pass (9 points and up), else the direct DFT.  The
+// runtime schedule needs E divisible by every radix: 2^a 3 and 2^a 5 lengths
+// (384, 640, 768) run at E = 12 or 20 and 1.0-1.3x faster than the
+// per-pass-mapped kernel, lengths with both 3 and 5 (120, 360, 480, 960) at
+// E = 30 and 1.1-4.4x slower (49 lengths, 72 to 4000, both precisions, both
+// layouts).  Per GiB of complex64 lines the
 // direct DFT costs O(N): 7.7 ms at N = 11 (one workgroup per line leaves most
 // of its threads idle), 3.3 ms at 97, 13 ms at 509, 124 ms at 4093; the
 // chirp-z pass 0.9-2.1 ms at every length from 11 to 4093.  complex128 agrees,
@@ -35,8 +42,13 @@ static KSel cz_sel(int N) { return KSel{1, N, KF_CZ}; }
 // The choice depends on (N, precision) only, never on the layout, so that
 // fused and staged plans compute every line alike.
 static KSel fallback_sel(const AxisPass& p, a2afft_dtype dt, int N) {
+    const bool dbl = dt == A2AFFT_C128;
     MixedPlan mp;
-    if (mixed_plan(p, N, dt == A2AFFT_C128, &mp)) return KSel{mp.T, mp.E, KF_MIX};
+    const int e = mixed_plan_regs(N, dbl);
+    if (e > 0 && e <= 24 && mixed_plan(p, N, dbl, &mp)) return KSel{mp.T, mp.E, KF_MIX};
+    MixlPlan lp;
+    if (mixl_plan(p, N, dbl, &lp)) return KSel{lp.T, lp.EM, KF_MIXL | KF_XCD};
+    if (mixed_plan(p, N, dbl, &mp)) return KSel{mp.T, mp.E, KF_MIX};
     if (chirp_pass_ok(N)) return cz_sel(N);
     return dft_sel(N);
 }
@@ -119,6 +131,8 @@ void init_registry() {
 
 const void* find_kernel(int N, bool dbl, int strided, const KSel& s, int dir) {
     if (is_mix(s)) return s.flags == KF_MIX ? mixed_kernel_for(N, s.T, s.E, dbl, strided, dir) : nullptr;
+    if (is_mixl(s)) return (s.flags & ~(KF_XCD | KF_NT)) == KF_MIXL ? mixl_kernel_for(s.E, dbl, strided, s.flags, dir)
+                                                                     : nullptr;
     if (is_dft(s)) return (s.T == 1 && s.E == N && s.flags == KF_DFT && fft_length_supported(N)) ? dft_kernel_for(dbl, dir)
                                                                                                     : nullptr;
     if (is_cz(s)) return (s.T == 1 && s.E == N && s.flags == KF_CZ) ? chirp_pass_kernel(N, dbl, strided) : nullptr;
@@ -143,6 +157,17 @@ int fft_candidates(const AxisPass& p, a2afft_dtype dt, int N, KSel* out, int max
         out[n++] = KSel{k.T, k.E, k.flags};
     }
     if (n == 0 && fft_length_supported(N)) {
+        MixlPlan lp;
+        if (mixl_plan(p, N, dbl, &lp)) {
+            // the per-pass-mapped kernel: every tile it accepts on this pass, each
+            // in the built flag variants
+            for (int tt : {1, 2, 4, 8, 12, 16, 24, 32, 48, 64}) {
+                if (!mixl_plan(p, N, dbl, &lp, tt)) continue;
+                for (int fl : {int(KF_XCD), KF_XCD | KF_NT, 0})
+                    if (n < max && mixl_kernel_for(lp.EM, dbl, p.strided, fl, -1))
+                        out[n++] = KSel{tt, lp.EM, KF_MIXL | fl};
+            }
+        }
         MixedPlan mp;
         if (mixed_plan(p, N, dbl, &mp) && n < max) {
             const KSel f{mp.T, mp.E, KF_MIX};
@@ -236,7 +261,19 @@ static int resolve_dft(const AxisPass& p, int N, FftGeom* g, uint32_t* grid) {
 }
 
 static int resolve(const AxisPass& p, a2afft_dtype dt, int N, const KSel& s, FftGeom* g, uint32_t* grid,
-                   MixedPlan* mp_out = nullptr) {
+                   MixedPlan* mp_out = nullptr, MixlPlan* lp_out = nullptr) {
+    if (is_mixl(s)) {
+        MixlPlan lp;
+        if (!mixl_plan(p, N, dt == A2AFFT_C128, &lp, s.T) || lp.T != s.T || lp.EM != s.E)
+            return set_error(A2AFFT_E_UNSUPPORTED, "fft pass: length %d (T=%d, E=%d) not supported", N, s.T, s.E);
+        if (p.fft.n != uint32_t(N)) return set_error(A2AFFT_E_INVALID_ARG, "fft pass: bad geometry (N=%d)", N);
+        const int st = resolve_geom(p.fft, p.a1, p.a2, p.strided, lp.T, g, grid, p.a2_start, p.a2_count);
+        if (st < 0) return set_error(st, "fft pass: bad geometry (N=%d)", N);
+        g->scale_mode = p.scale_mode;
+        g->scale = p.scale;
+        if (lp_out) *lp_out = lp;
+        return A2AFFT_OK;
+    }
     if (is_mix(s)) {
         MixedPlan mp;
         if (!mixed_plan(p, N, dt == A2AFFT_C128, &mp, s.T) || mp.T != s.T || mp.E != s.E)
@@ -273,7 +310,7 @@ int fft_pass_shape(const AxisPass& p, a2afft_dtype dt, int N, const KSel* sel, F
     out->E = s.E;
     // the launched thread count: the runtime-schedule kernel rounds its
     // workgroup to its own width (mp.NT)
-    out->NT = (is_dft(s) || is_cz(s)) ? 256 : is_mix(s) ? mp.NT : s.T * (N / s.E);
+    out->NT = (is_dft(s) || is_cz(s) || is_mixl(s)) ? 256 : is_mix(s) ? mp.NT : s.T * (N / s.E);
     out->grid = int(grid);
     out->flags = s.flags;
     return A2AFFT_OK;
@@ -285,7 +322,8 @@ int launch_fft_pass(const AxisPass& p, void* data, a2afft_dtype dt, int N, int d
     FftGeom g;
     uint32_t grid;
     MixedPlan mp{};
-    A2A_TRY(resolve(p, dt, N, ks, &g, &grid, &mp));
+    MixlPlan lp{};
+    A2A_TRY(resolve(p, dt, N, ks, &g, &grid, &mp, &lp));
     if (grid == 0) return A2AFFT_OK;
     const void* fn = find_kernel(N, dt == A2AFFT_C128, p.strided, ks, dir);
     if (!fn)
@@ -294,7 +332,12 @@ int launch_fft_pass(const AxisPass& p, void* data, a2afft_dtype dt, int N, int d
     const void* tw = pass_twiddles(p, N, dir, dt, ks);
     if (!tw) return set_error(A2AFFT_E_ALLOC, "fft: twiddle table for N=%d", N);
     int st;
-    if (is_mix(ks)) {
+    if (is_mixl(ks)) {
+        void* args[] = {&data, &g, (void*)&tw, &lp};
+        st = hip_check(int(hipLaunchKernel(fn, dim3(grid), dim3(256), args,
+                                           size_t(mixl_lds_bytes(lp, dt == A2AFFT_C128, p.strided)), s)),
+                       "mixed-radix launch");
+    } else if (is_mix(ks)) {
         void* args[] = {&data, &g, (void*)&tw, &mp};
         st = hip_check(int(hipLaunchKernel(fn, dim3(grid), dim3(mp.NT), args,
                                            size_t(mixed_lds_bytes(mp, dt == A2AFFT_C128)), s)),
@@ -409,6 +452,11 @@ const void* pass_twiddles(const AxisPass& p, int N, int dir, a2afft_dtype dt, co
         MixedPlan mp;
         if (!mixed_plan(p, N, dt == A2AFFT_C128, &mp)) return nullptr;
         return mixed_twiddles(mp, dir, dt);
+    }
+    if (is_mixl(s)) {
+        MixlPlan lp;
+        if (!mixl_plan(p, N, dt == A2AFFT_C128, &lp)) return nullptr;
+        return mixl_twiddles(lp, dir, dt);
     }
     if (is_cz(s)) return chirp_pass_tables(N, dir, dt);
     return cached_table(N, dir, dt, is_dft(s) ? 2 : 1);

@@ -189,7 +189,8 @@ const void* mixed_fn_e(int E, bool strided, int dir) {
 // layout of a line computes the same bits; the pass decides the tile (T) and
 // whether the shape fits at all (additive offsets: the axis block extent and
 // TPL divide one another).
-bool mixed_plan(const AxisPass& ps, int N, bool dbl, MixedPlan* mp, int force_T) {
+// The radix schedule and register count of a length (layout-independent).
+static bool mixed_schedule(int N, bool dbl, MixedPlan* out) {
     if (N < 2 || N > kDftMaxN || is_pow2(N)) return false;
     int m = N, a = 0, b3 = 0, c5 = 0;
     while (m % 2 == 0) { m /= 2; ++a; }
@@ -197,7 +198,6 @@ bool mixed_plan(const AxisPass& ps, int N, bool dbl, MixedPlan* mp, int force_T)
     while (m % 5 == 0) { m /= 5; ++c5; }
     if (m != 1) return false;
     const int emax = dbl ? 30 : 60;
-    const int esize = dbl ? 16 : 8;
     MixedPlan q{};
     bool found = false;
     for (int twos = 0; twos < 2; ++twos) {   // radix-4 passes first, then radix-2 passes
@@ -229,7 +229,19 @@ bool mixed_plan(const AxisPass& ps, int N, bool dbl, MixedPlan* mp, int force_T)
         if (!found || (c.E < q.E && q.E * (dbl ? 4 : 2) > 48)) q = c;
         found = true;
     }
-    if (!found) return false;
+    if (found) *out = q;
+    return found;
+}
+
+int mixed_plan_regs(int N, bool dbl) {
+    MixedPlan q{};
+    return mixed_schedule(N, dbl, &q) ? q.E : 0;
+}
+
+bool mixed_plan(const AxisPass& ps, int N, bool dbl, MixedPlan* mp, int force_T) {
+    const int esize = dbl ? 16 : 8;
+    MixedPlan q{};
+    if (!mixed_schedule(N, dbl, &q)) return false;
     q.TPL = N / q.E;
     if (!(ps.fft.l % uint32_t(q.TPL) == 0 || uint32_t(q.TPL) % ps.fft.l == 0)) return false;
     const int lds_lines = 65536 / (N * esize);

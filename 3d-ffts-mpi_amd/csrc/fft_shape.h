@@ -158,6 +158,7 @@ enum : int {
     KF_DFT = 64,   // direct O(N^2) DFT, one line per workgroup: the lengths no Stockham kernel covers
     KF_MIX = 128,  // Stockham FFT with the radix schedule in the arguments: long 2^a 3^b 5^c lengths
     KF_CZ = 2048,  // Bluestein chirp-z convolution over power-of-two FFTs: prime factors above 5
+    KF_MIXL = 4096,  // Stockham FFT with per-pass butterfly mapping: long 2^a 3^b 5^c lengths
 };
 constexpr int kDftMaxN = 4096;   // longest direct-DFT line (N complex values in LDS)
 

@@ -64,7 +64,7 @@ static const void* pattern_kernel_for(int E, int threads) {
 
 int launch_pattern_copy(const AxisPass& p, void* data, a2afft_dtype dt, int N, const KSel& ks, hipStream_t s) {
     if (!data) return set_error(A2AFFT_E_INVALID_ARG, "pattern copy: null data");
-    if (ks.flags & (KF_DFT | KF_MIX | KF_COH | KF_CZ))
+    if (ks.flags & (KF_DFT | KF_MIX | KF_MIXL | KF_COH | KF_CZ))
         return set_error(A2AFFT_E_UNSUPPORTED, "pattern copy: fixed-schedule Stockham passes only");
     FftGeom g;
     uint32_t grid;

@@ -293,7 +293,11 @@ int a2afft_execute_pass(a2afft_plan plan, void* d_data, int pass, int direction,
  * Every variant of a pass computes identical bits; they differ in tile width
  * T, elements per thread E and flags: 1 XCD-aware workgroup order, 2 padded
  * LDS, 4 non-temporal loads/stores, 8 z-tiles-outermost unit order, 32 LDS
- * exchanges one component at a time (half the LDS per workgroup).
+ * exchanges one component at a time (half the LDS per workgroup).  Kernel
+ * families, which compute different bits from one another: the fixed-schedule
+ * Stockham shapes (no family flag), 128 runtime radix schedule, 4096
+ * butterflies mapped per pass (E = data registers a thread), 2048 chirp-z,
+ * 64 direct DFT; autotune stays within the family of each pass's selection.
  * autotune times each built variant that fits each fused pass, `iters`
  * launches in place on `work` (a [P][nlocal] buffer whose contents are
  * destroyed), keeps the fastest per pass and then the fastest chunk width,

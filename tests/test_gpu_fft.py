@@ -56,7 +56,7 @@ def test_batched_1d_fft_vs_oracle(a2afft_mod, cuda, N, dtype):
 MIXED_LONG = [72, 96, 100, 120, 192, 243, 384, 625, 768, 1000, 1536, 2187, 3072, 3125, 3840, 4000]
 
 
-KF_DFT, KF_CZ = 64, 2048
+KF_DFT, KF_MIX, KF_CZ, KF_MIXL = 64, 128, 2048, 4096
 
 
 def _chosen_flags(plan, name):
@@ -178,7 +178,10 @@ CONFIGS_3D = [
     (45, 9, None, "c128"),        # no fixed complex128 kernel for 45: runtime schedule
     (96, 8, (1, 2, 4), "c128"),   # 2^5 * 3 over blocks of 96 x 48 x 24
     (96, 8, (1, 2, 4), "c64"),
-    (120, 8, (2, 2, 2), "c128"),  # 2^3 * 3 * 5: radix-2 schedule in complex128
+    (120, 8, (2, 2, 2), "c128"),  # 2^3 * 3 * 5: the per-pass-mapped schedule (radices 8, 3, 5)
+    (120, 8, (1, 2, 4), "c64"),
+    (120, 8, (1, 1, 8), "c64"),    # z blocks of 15: the first pass's inputs step across rank blocks
+    (120, 15, (1, 1, 15), "c128"),  # z blocks of 8: the last pass's outputs step across rank blocks
     (12, 6, None, "c128"),
     (12, 6, None, "c64"),
     (24, 12, None, "c128"),
@@ -335,7 +338,7 @@ def test_chirp_on_smooth_lengths(a2afft_mod, cuda):
             plan = a2afft_mod.GridDfft(shape, prec)
             if forced:
                 plan.set_kernel(1, 1, 120, KF_CZ)
-            assert _chosen_flags(plan, "x") == (KF_CZ if forced else 128)
+            assert _chosen_flags(plan, "x") == (KF_CZ if forced else KF_MIXL | 1)
             d = torch.from_numpy(x.astype(np.complex128 if prec == "fp64" else np.complex64)).cuda()
             plan.forward(d)
             torch.cuda.synchronize()
@@ -385,18 +388,19 @@ def test_wide_mixed_tiles_vs_numpy(a2afft_mod, cuda, case):
 def test_mixed_tiles_bit_identical(a2afft_mod, cuda, case):
     """Every tile the runtime-schedule kernel accepts (narrow and wide; the
     autotune candidates) computes each line with the same arithmetic: forced on
-    its pass, each reproduces the default plan bit for bit."""
+    its pass, each reproduces the first tile's transform bit for bit."""
     import torch
     shape, prec, pass_, E, tiles = case
     dt = torch.complex128 if prec == "fp64" else torch.complex64
     x = torch.empty(shape, dtype=dt, device="cuda")
     a2afft_mod.fill_random(x, 17, "uniform")
     base = a2afft_mod.GridDfft(shape, prec)
+    base.set_kernel(pass_, tiles[0], E, KF_MIX)
     ref = x.clone()
     base.forward(ref)
     torch.cuda.synchronize()
     ref_bits = _bits(ref.cpu().numpy())
-    for T in tiles:
+    for T in tiles[1:]:
         plan = a2afft_mod.GridDfft(shape, prec)
         plan.set_kernel(pass_, T, E, 128)
         assert f"T={T} E={E} " in plan.describe().splitlines()[pass_], plan.describe()
@@ -404,6 +408,55 @@ def test_mixed_tiles_bit_identical(a2afft_mod, cuda, case):
         plan.forward(y)
         torch.cuda.synchronize()
         assert np.array_equal(_bits(y.cpu().numpy()), ref_bits), (T, E)
+
+
+# (grid shape, precision, pass): lengths with both 3 and 5 as factors, which
+# run the per-pass-mapped Stockham kernel by default, contiguous and strided
+# (3000-point complex64 columns two to a tile, complex128 one)
+MIXL_CASES = [((16, 16, 480), "fp32", 0), ((480, 8, 64), "fp32", 1), ((8, 360, 64), "fp64", 2),
+              ((4, 4, 3000), "fp32", 0), ((3000, 2, 16), "fp64", 1), ((3000, 2, 16), "fp32", 1),
+              ((16, 16, 120), "fp64", 0),
+              ((120, 4, 64), "fp32", 1)]
+
+
+@pytest.mark.parametrize("case", MIXL_CASES, ids=lambda c: f"{c[0]}_{c[1]}_pass{c[2]}")
+def test_mixl_tiles_vs_numpy_and_bit_identical(a2afft_mod, cuda, case):
+    """The per-pass-mapped kernel: the plan picks it, forward and inverse match
+    numpy's fftn in FP64 to the precision's bound, and every tile and flag
+    variant it accepts on the pass (the autotune candidates) reproduces the
+    default plan bit for bit."""
+    import torch
+    from a2afft import inputs
+    shape, prec, pass_ = case
+    n = int(np.prod(shape))
+    x = inputs.uniform(23, n).reshape(shape).astype(np.complex128 if prec == "fp64" else np.complex64)
+    base = a2afft_mod.GridDfft(shape, prec)
+    assert _chosen_flags(base, "zxy"[pass_]) & KF_MIXL, base.describe()
+    d = torch.from_numpy(x).cuda()
+    base.forward(d)
+    torch.cuda.synchronize()
+    ref_bits = _bits(d.cpu().numpy())
+    lg = np.log2(max(shape))
+    tol = 1e-15 * lg if prec == "fp64" else 1.5e-7 * lg
+    assert _relerr(d.cpu().numpy(), np.fft.fftn(x.astype(np.complex128))) < tol
+    base.inverse(d)
+    torch.cuda.synchronize()
+    assert _relerr(d.cpu().numpy(), x.astype(np.complex128)) < 2 * tol
+    ran = 0
+    for T in (1, 2, 3, 4, 8, 16):
+        for E in (16, 24, 32, 48):
+            for fl in (1, 5, 0):
+                plan = a2afft_mod.GridDfft(shape, prec)
+                try:
+                    plan.set_kernel(pass_, T, E, KF_MIXL | fl)
+                except a2afft_mod.A2AFFTError:
+                    continue
+                y = torch.from_numpy(x).cuda()
+                plan.forward(y)
+                torch.cuda.synchronize()
+                assert np.array_equal(_bits(y.cpu().numpy()), ref_bits), (T, E, fl)
+                ran += 1
+    assert ran >= 3, ran
 
 
 # (grid shape, precision, [(pass, T, E)]): the shapes built with half-width
