@@ -481,6 +481,10 @@ extern "C" int a2afft_set_tuning_cache(const char* path) {
     return A2AFFT_OK;
 }
 
+// The key's version changes whenever a default kernel family or schedule
+// changes (v3: 2^a 3^b 5^c lengths whose runtime schedule needs more than 24
+// values a thread moved to the per-pass-mapped kernel), so that entries tuned
+// within an old family are ignored and re-tuned rather than applied.
 static std::string plan_key(const a2afft_plan_s* p) {
     int dev = 0;
     hipDeviceProp_t prop;
@@ -491,7 +495,7 @@ static std::string plan_key(const a2afft_plan_s* p) {
         if (colon != std::string::npos) arch.resize(colon);
     }
     char buf[256];
-    snprintf(buf, sizeof(buf), "a2afft-tune-v2 arch=%s grid=%dx%dx%d P=%d dims=%dx%dx%d dtype=%d prec=%d exec=%d",
+    snprintf(buf, sizeof(buf), "a2afft-tune-v3 arch=%s grid=%dx%dx%d P=%d dims=%dx%dx%d dtype=%d prec=%d exec=%d",
              arch.c_str(), p->n[0], p->n[1], p->n[2], p->d.P, p->d.dims[0], p->d.dims[1], p->d.dims[2],
              int(p->d.dtype), int(p->prec), int(p->exec));
     return buf;
