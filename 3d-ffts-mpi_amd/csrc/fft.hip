@@ -222,6 +222,12 @@ static const Pref kPrefs[] = {
     {256, 1, 0, -1, -1, {16, 16, KF_XCD | KF_PAD | KF_HALFX}},
     {256, 1, 1, 1, -1, {8, 16, KF_XCD | KF_A2MAJ | KF_HALFX}},
     {256, 1, 1, -1, -1, {8, 16, KF_XCD | KF_PAD | KF_HALFX}},
+    // strided 1024-point columns on 128 B rows (1024^3: complex64 x pass
+    // 6.68 -> 4.81 ms, y 3.45 -> 3.34; complex128 x 12.9 -> 9.3, y 6.85 -> 6.17)
+    {1024, 0, 1, 1, -1, {16, 32, KF_XCD | KF_NT | KF_HALFX}},
+    {1024, 0, 1, 0, -1, {16, 32, KF_XCD | KF_PAD}},
+    {1024, 1, 1, 1, -1, {8, 16, KF_XCD | KF_NT | KF_HALFX}},
+    {1024, 1, 1, 0, -1, {8, 16, KF_XCD}},
     {2048, 0, 0, -1, -1, {2, 16, KF_XCD | KF_PAD | KF_NT}},
     {2048, 0, 1, 1, -1, {16, 64, KF_XCD | KF_A2MAJ | KF_HALFX}},
     {2048, 0, 1, -1, -1, {16, 64, KF_XCD | KF_HALFX}},

@@ -71,6 +71,15 @@ int register_kernels_f32(KEntry* out) {
     add_variant<float, 512, 32, 16, true, KF_A2MAJ | KF_PAD | KF_NT | KF_HALFX>(out, &n);
     add_halfx<float, 512, 16, 16, false>(out, &n);
     add_halfx<float, 2048, 16, 2, false>(out, &n);
+    // strided 1024-point columns in 16-column tiles (128 B rows; the default
+    // shape reads 64 B rows)
+    add_shape<float, 1024, 32, 16, true>(out, &n);   // 512 threads, 128 KiB LDS
+    add_halfx<float, 1024, 32, 16, true>(out, &n);   // 512 threads, 64 KiB LDS
+    add_variant<float, 1024, 32, 16, true, KF_HALFX>(out, &n);
+    add_variant<float, 1024, 32, 16, true, KF_NT | KF_HALFX>(out, &n);
+    add_variant<float, 1024, 32, 16, true, KF_XCD | KF_NT | KF_HALFX>(out, &n);
+    add_halfx<float, 1024, 64, 16, true>(out, &n);   // 256 threads
+    add_halfx<float, 1024, 64, 32, true>(out, &n);   // 512 threads, 256 B rows, 128 KiB LDS
     return n;
 }
 

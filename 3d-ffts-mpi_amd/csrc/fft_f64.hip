@@ -46,6 +46,12 @@ int register_kernels_f64(KEntry* out) {
     add_halfx<double, 256, 16, 4, true>(out, &n);
     add_halfx<double, 512, 16, 8, false>(out, &n);
     add_halfx<double, 512, 16, 8, true>(out, &n);
+    // strided 1024-point columns in 8-column tiles (128 B rows; the default
+    // shape reads 64 B rows)
+    add_shape<double, 1024, 16, 8, true>(out, &n);   // 512 threads, 128 KiB LDS
+    add_halfx<double, 1024, 16, 8, true>(out, &n);   // 512 threads, 64 KiB LDS
+    add_variant<double, 1024, 16, 8, true, KF_XCD | KF_NT | KF_HALFX>(out, &n);
+    add_halfx<double, 1024, 32, 8, true>(out, &n);   // 256 threads
     return n;
 }
 
