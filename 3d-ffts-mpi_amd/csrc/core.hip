@@ -619,8 +619,11 @@ extern "C" int a2afft_grid_plan_create(a2afft_plan* out, int nx, int ny, int nz,
     *out = nullptr;
     const a2afft_dtype dt = prec == A2AFFT_FP64 ? A2AFFT_C128 : A2AFFT_C64;
     A2A_TRY(check_precision(prec, dt));
+    // Addresses are 64-bit throughout the fused passes (each pass's line or
+    // tile count stays below 2^32, checked per launch), so a grid may fill
+    // the device: 2048^3 complex64 is 64 GiB.
     const int64_t total = int64_t(nx) * ny * nz;
-    if (total >= (int64_t(1) << 32)) return set_error(A2AFFT_E_SIZE, "grid_plan_create: grid >= 2^32 points");
+    if (total >= (int64_t(1) << 40)) return set_error(A2AFFT_E_SIZE, "grid_plan_create: grid >= 2^40 points");
     a2afft_plan_s* p = new (std::nothrow) a2afft_plan_s;
     if (!p) return set_error(A2AFFT_E_ALLOC, "grid_plan_create");
     p->d.Ng = 0;

@@ -229,6 +229,10 @@ static const Pref kPrefs[] = {
     {1024, 1, 1, 1, -1, {8, 16, KF_XCD | KF_NT | KF_HALFX}},
     {1024, 1, 1, 0, -1, {8, 16, KF_XCD}},
     {2048, 0, 0, -1, -1, {2, 16, KF_XCD | KF_PAD | KF_NT}},
+    // complex128 strided 2048-point columns on 128 B rows (2048^2 x 1024: x pass
+    // 95.5 -> 37.8 ms, y 39.7 -> 26.4 ms against 32 B rows)
+    {2048, 1, 1, 1, -1, {8, 32, KF_XCD | KF_PAD | KF_NT | KF_HALFX}},
+    {2048, 1, 1, 0, -1, {8, 32, KF_XCD | KF_HALFX}},
     {2048, 0, 1, 1, -1, {16, 64, KF_XCD | KF_A2MAJ | KF_HALFX}},
     {2048, 0, 1, -1, -1, {16, 64, KF_XCD | KF_HALFX}},
     {4096, 0, 0, -1, -1, {1, 16, KF_PAD}},

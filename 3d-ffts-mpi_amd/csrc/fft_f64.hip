@@ -52,6 +52,10 @@ int register_kernels_f64(KEntry* out) {
     add_halfx<double, 1024, 16, 8, true>(out, &n);   // 512 threads, 64 KiB LDS
     add_variant<double, 1024, 16, 8, true, KF_XCD | KF_NT | KF_HALFX>(out, &n);
     add_halfx<double, 1024, 32, 8, true>(out, &n);   // 256 threads
+    // strided 2048-point columns on 64-128 B rows (the default shape reads 32 B)
+    add_halfx<double, 2048, 32, 8, true>(out, &n);   // 512 threads, 128 KiB LDS
+    add_halfx<double, 2048, 16, 4, true>(out, &n);   // 512 threads, 64 KiB LDS
+    add_halfx<double, 2048, 32, 4, true>(out, &n);   // 256 threads, 64 KiB LDS
     return n;
 }
 

@@ -190,7 +190,9 @@ int a2afft_plan_create(a2afft_plan* out, a2afft_dist dist, a2afft_precision prec
  * grid per distribution, a2adistribution.h:9).  Each extent a transform
  * length of forward_1d_fft; complex128 for FP64, complex64 otherwise.  make_plans is not
  * needed; forward / inverse / autotune / execute_* apply as for other fused
- * plans, and the inverse scales by 1/(nx*ny*nz). */
+ * plans, and the inverse scales by 1/(nx*ny*nz).  Up to 2^40 points (addresses
+ * are 64-bit; 2048^3 complex64 takes 64 GiB); A2AFFT_E_SIZE beyond, or when
+ * one pass would need 2^32 lines or tiles. */
 int a2afft_grid_plan_create(a2afft_plan* out, int nx, int ny, int nz, a2afft_precision precision);
 /* a2adfft::make_plans(d_scratch) (a2adfft.cpp:18-24).  scratch is a
  * [P][nlocal] device buffer; the staged strategy needs it, the fused one

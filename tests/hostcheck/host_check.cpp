@@ -140,6 +140,66 @@ int64_t hc_grid_enumerate(int nx, int ny, int nz, int pass, int T, int E, int64_
     return c;
 }
 
+// Grids of 2^32 points and more: for the hardware workgroups [b_begin,
+// b_end) of pass `pass` of a plain nx x ny x nz C-order grid with shape
+// (T, E) and flags (KF_XCD, KF_A2MAJ honoured), every address the kernel
+// computes (line_base + elem_off) against the element's C-order index
+// computed directly in 64 bits from its (x, y, z).  grid_out: the launch's
+// workgroup count.  Returns the number of mismatches or out-of-range
+// addresses (0 = all agree), or a negative status.
+int64_t hc_grid_check(int nx, int ny, int nz, int pass, int T, int E, int flags, uint32_t b_begin, uint32_t b_end,
+                      int64_t* grid_out, int64_t* checked_out) {
+    const int n3[3] = {nx, ny, nz};
+    const int dims[3] = {1, 1, 1};
+    PassMaps all[3];
+    const int64_t total = int64_t(nx) * ny * nz;
+    fused_pass_maps(n3, dims, n3, total, all);
+    const PassMaps pm = all[pass];
+    const int N = int(pm.fft.n);
+    FftGeom g;
+    uint32_t grid;
+    const int st = resolve_geom(pm.fft, pm.a1, pm.a2, pm.strided, T, &g, &grid, 0, 0);
+    if (st < 0) return st;
+    *grid_out = grid;
+    const int TPL = N / E, NT = T * TPL;
+    const bool strided = pm.strided != 0, a2maj = (flags & KF_A2MAJ) != 0;
+    int64_t bad = 0, checked = 0;
+    for (uint32_t hb = b_begin; hb < b_end && hb < grid; ++hb) {
+        const uint32_t b = (flags & KF_XCD) ? xcd_block(hb, grid) : hb;
+        for (int tid = 0; tid < NT; ++tid) {
+            bool valid;
+            const int64_t base = line_base(g, strided, T, TPL, b, tid, &valid, a2maj);
+            if (!valid) continue;
+            const int t = strided ? tid / T : tid % TPL;
+            const int line = strided ? tid % T : tid / TPL;
+            // the unit's (a1, a2) coordinates, straight from the unit order
+            uint64_t u1, u2;
+            if (strided) {
+                const uint64_t n2 = uint64_t(pm.a2.n) / uint64_t(T), n1 = pm.a1.n;
+                u1 = a2maj ? b % n1 : b / n2;
+                u2 = (a2maj ? b / n1 : b % n2) * uint64_t(T) + uint64_t(line);
+            } else {
+                const uint64_t ln = uint64_t(b) * uint64_t(T) + uint64_t(line);
+                u1 = ln / pm.a2.n;
+                u2 = ln % pm.a2.n;
+            }
+            for (int e = 0; e < E; ++e) {
+                const uint64_t n = uint64_t(t) + uint64_t(e) * uint64_t(TPL);
+                uint64_t x, y, z;
+                if (pass == 0) { x = u1; y = u2; z = n; }
+                else if (pass == 1) { x = n; y = u1; z = u2; }
+                else { x = u1; y = n; z = u2; }
+                const int64_t want = int64_t((x * uint64_t(ny) + y) * uint64_t(nz) + z);
+                const int64_t got = base + elem_off(g, e * TPL);   // line_base includes off(t)
+                if (got != want || got < 0 || got >= total) ++bad;
+                ++checked;
+            }
+        }
+    }
+    *checked_out = checked;
+    return bad;
+}
+
 void hc_xcd_blocks(uint32_t G, uint32_t* out) {
     for (uint32_t b = 0; b < G; ++b) out[b] = xcd_block(b, G);
 }

@@ -303,7 +303,8 @@ def test_chunked_and_tuned_plans_bit_identical(a2afft_mod, cuda, cfg):
 @pytest.mark.parametrize("shape,prec", [((32, 64, 16), "fp64"), ((128, 256, 64), "fp32"), ((64, 32, 128), "fp64"),
                                         ((16, 512, 8), "fp32"), ((2, 4, 2048), "fp64"),
                                         ((1024, 8, 1024), "fp32"), ((1024, 8, 512), "fp64"),
-                                        ((8, 1024, 64), "fp32"), ((8, 1024, 64), "fp64")],
+                                        ((8, 1024, 64), "fp32"), ((8, 1024, 64), "fp64"),
+                                        ((2048, 64, 64), "fp64"), ((8, 2048, 64), "fp64")],
                          ids=lambda v: str(v))
 def test_grid_plan_non_cubic_vs_numpy(a2afft_mod, cuda, shape, prec):
     """Single-grid plans (no decomposition) of non-cubic shapes, the shapes of
@@ -470,7 +471,8 @@ HALFX_CASES = [((256, 256, 256), "fp64", [(0, 8, 16), (0, 16, 16), (1, 8, 16), (
                ((2048, 4, 32), "fp32", [(1, 8, 32), (1, 8, 64), (1, 16, 64)]),
                ((4, 2048, 32), "fp32", [(2, 16, 64)]),
                ((1024, 4, 64), "fp32", [(1, 16, 32), (1, 16, 64), (1, 32, 64)]),
-               ((1024, 4, 64), "fp64", [(1, 8, 16), (1, 8, 32)])]
+               ((1024, 4, 64), "fp64", [(1, 8, 16), (1, 8, 32)]),
+               ((2048, 4, 32), "fp64", [(1, 8, 32), (1, 4, 16), (1, 4, 32)])]
 HALFX_FLAGS = (32 | 1, 32 | 1 | 2, 32 | 1 | 2 | 4, 32 | 8, 32 | 8 | 1)
 
 

@@ -254,6 +254,47 @@ def _c5_shapes():
     return [cand[min(5, int(v * 6))] for v in u]
 
 
+@pytest.mark.parametrize("shape,prec,tol", [((2048, 2048, 2048), "fp32", 1e-5), ((2048, 2048, 1024), "fp64", 1e-12)],
+                         ids=["2048cube_c64", "2048x2048x1024_c128"])
+def test_whole_grid_of_2e32_points_and_more(a2afft_mod, cuda, shape, prec, tol):
+    """A whole grid of 2^32 points or more on one device (64 GiB each; the
+    passes address it in 64 bits).  A unit impulse at r0 transforms to
+    exp(-2 pi i k.r0 / n) at every k, so every output is checked against that
+    closed form (slab by slab, in FP64); the inverse returns the impulse: 1 at
+    r0 and below `tol` everywhere else."""
+    import torch
+    nx, ny, nz = shape
+    cdt = torch.complex64 if prec == "fp32" else torch.complex128
+    need = nx * ny * nz * (8 if prec == "fp32" else 16)
+    free, _ = torch.cuda.mem_get_info()
+    if free < need + (32 << 30):
+        pytest.skip(f"needs {need >> 30} GiB plus 32 GiB of work space on the device")
+    r0 = (1234, 77, 1999 % nz)
+    x = torch.zeros(shape, dtype=cdt, device="cuda")
+    x[r0] = 1
+    plan = a2afft_mod.GridDfft(shape, prec)
+    plan.forward(x)
+    torch.cuda.synchronize()
+    ph = []
+    for n, c in zip(shape, r0):
+        k = torch.arange(n, device="cuda", dtype=torch.int64)
+        ph.append(torch.exp(((k * c) % n).to(torch.float64) * (-2j * math.pi / n)))   # exact phase reduction
+    yz = ph[1][:, None] * ph[2][None, :]
+    worst = 0.0
+    for s in range(0, nx, 64):
+        want = ph[0][s:s + 64, None, None] * yz[None]
+        worst = max(worst, float((x[s:s + 64].to(torch.complex128) - want).abs().max()))
+        del want
+    assert worst < tol, worst
+    plan.inverse(x)
+    torch.cuda.synchronize()
+    peak = complex(x[r0].item())
+    assert abs(peak - 1) < tol, peak
+    x[r0] = 0
+    rest = max(float(x[s:s + 64].abs().max()) for s in range(0, nx, 64))
+    assert rest < tol, rest
+
+
 def test_c5_batch_of_64_grids(a2afft_mod, cuda):
     """The 64-grid batch (shapes drawn with seed 4): every grid equals its
     single-grid plan bit for bit, for the default and a grouped, skewed order;
