@@ -194,13 +194,18 @@ int fft_candidates(const AxisPass& p, a2afft_dtype dt, int N, KSel* out, int max
 // first entry that is built and fits the geometry wins; otherwise the generic
 // default with XCD order.
 // `far`: 1 = the transform axis steps >= 64 KiB between elements (the x
-// axis of a block), 0 = nearer, -1 = either.  `seg`: 1 = a line runs across
+// axis of a block), 2 = >= 1 MiB (the x axis of a whole 512^3 grid; also
+// matched by 1), 0 = nearer, -1 = either.  `seg`: 1 = a line runs across
 // rank blocks (the block extent along the axis is shorter than the line), 0 =
 // a line sits in one block, -1 = either.  Contiguous 512-point lines in 4
 // segments (the z pass of 512^3 over dims [1,4,4]) run fastest two lines to
 // a 128-thread workgroup: 0.366 ms against 0.380 ms for 16-line tiles
 // (pattern copies 6.05 against 5.80 TB/s); whole lines (512^3 in one block)
-// do not gain (0.406 against 0.393 ms).  Complex128 256-point lines in 4
+// do not gain (0.406 against 0.393 ms); whole lines run fastest one to a
+// 64-thread workgroup (0.373 against 0.382 ms on a 512^3 grid).  On a whole
+// 512^3 grid the x pass (2 MiB between elements) runs 32 columns wide (256 B
+// rows, full exchanges, hardware order): 1.195 ms a forward against 1.328 ms
+// on the block layout's choice.  Complex128 256-point lines in 4
 // segments (256^3 over dims [1,2,4]) likewise prefer 4-line tiles: 0.097
 // against 0.102 ms; complex128 512-point lines keep 8-line tiles (two-line
 // tiles 0.81 against 0.78 ms).
@@ -210,7 +215,9 @@ struct Pref {
 };
 static const Pref kPrefs[] = {
     {512, 0, 0, -1, 1, {2, 8, KF_XCD | KF_NT}},
+    {512, 0, 0, -1, 0, {1, 8, 0}},
     {512, 0, 0, -1, -1, {16, 16, KF_XCD | KF_PAD | KF_NT}},
+    {512, 0, 1, 2, -1, {32, 32, KF_PAD}},
     {512, 0, 1, 1, -1, {16, 32, KF_A2MAJ | KF_HALFX}},
     {512, 0, 1, -1, -1, {16, 32, KF_XCD | KF_PAD | KF_NT | KF_HALFX}},
     {512, 1, 0, -1, -1, {8, 16, KF_XCD | KF_PAD | KF_HALFX}},
@@ -241,11 +248,12 @@ static const Pref kPrefs[] = {
 KSel fft_default_sel(const AxisPass& p, a2afft_dtype dt, int N) {
     if (!fft_len_ok(N)) return fallback_sel(p, dt, N);
     const bool dbl = dt == A2AFFT_C128;
-    const int far = p.fft.s_elem * (dbl ? 16 : 8) >= 65536 ? 1 : 0;
+    const int64_t step = p.fft.s_elem * (dbl ? 16 : 8);
+    const int far = step >= (1 << 20) ? 2 : step >= 65536 ? 1 : 0;
     const int seg = p.fft.l < p.fft.n ? 1 : 0;
     for (const Pref& pr : kPrefs) {
         if (pr.N != N || pr.dbl != int(dbl) || pr.strided != p.strided) continue;
-        if (pr.far >= 0 && pr.far != far) continue;
+        if (pr.far >= 0 && (pr.far == 1 ? far < 1 : pr.far != far)) continue;
         if (pr.seg >= 0 && pr.seg != seg) continue;
         if (shape_fits(p.fft, p.a2, p.strided, N, pr.s.T, pr.s.E) && find_kernel(N, dbl, p.strided, pr.s, -1))
             return pr.s;
