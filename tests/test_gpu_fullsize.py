@@ -254,6 +254,37 @@ def _c5_shapes():
     return [cand[min(5, int(v * 6))] for v in u]
 
 
+def test_natural_512cube_grid(a2afft_mod, cuda):
+    """The plain-grid plan on a natural-order 512^3 complex64 grid (its own
+    tiles: 32-column x pass, single-line z pass) against rocFFT's FP64 fftn
+    on the same GPU (rel-L2 <= 1e-6), and bit for bit against the same plan
+    on the rank-block layout's tiles (16-column x pass, 16-line z pass)."""
+    import torch
+    sh = (512, 512, 512)
+    x = torch.empty(sh, dtype=torch.complex64, device="cuda")
+    a2afft_mod.fill_random(x, 29, "uniform")
+    plan = a2afft_mod.GridDfft(sh, "fp32")
+    desc = plan.describe()
+    assert "chosen x T=32 E=32" in desc and "chosen z T=1 E=8" in desc, desc
+    y = x.clone()
+    plan.forward(y)
+    alt = a2afft_mod.GridDfft(sh, "fp32")
+    alt.set_kernel(0, 16, 16, 7)
+    alt.set_kernel(1, 16, 32, 40)
+    z = x.clone()
+    alt.forward(z)
+    torch.cuda.synchronize()
+    assert _same_bits(y, z)
+    del z
+    ref = torch.fft.fftn(x.to(torch.complex128))
+    err = float(torch.linalg.vector_norm(y.to(torch.complex128) - ref) / torch.linalg.vector_norm(ref))
+    assert err < 1e-6, err
+    plan.inverse(y)
+    torch.cuda.synchronize()
+    back = float(torch.linalg.vector_norm((y - x).to(torch.complex128)) / torch.linalg.vector_norm(x.to(torch.complex128)))
+    assert back < 1e-6, back
+
+
 @pytest.mark.parametrize("shape,prec,tol", [((2048, 2048, 2048), "fp32", 1e-5), ((2048, 2048, 1024), "fp64", 1e-12)],
                          ids=["2048cube_c64", "2048x2048x1024_c128"])
 def test_whole_grid_of_2e32_points_and_more(a2afft_mod, cuda, shape, prec, tol):
