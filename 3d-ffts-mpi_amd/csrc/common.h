@@ -17,6 +17,10 @@
 
 namespace a2a {
 
+// Workgroups of a grid-stride element kernel (256 threads each): a dispatch
+// holds fewer than 2^32 work-items, so kernels over more elements loop.
+constexpr uint64_t kStrideBlocks = uint64_t(1) << 20;
+
 // Division of a uint32 by a runtime-invariant divisor with one mul_hi, an
 // add and a shift (round-up multiplier method, Granlund & Montgomery 1994).
 // Exact for every 32-bit dividend.

@@ -9,6 +9,8 @@
 // maps move runs of l_axis contiguous elements (idx % l_axis is preserved), so
 // one thread moves a 16-byte unit and both the load and the store coalesce.
 // The transposes stage 32x32 tiles through LDS.
+#include <algorithm>
+
 #include "common.h"
 #include "remap.h"
 #include "remap_index.h"
@@ -133,8 +135,8 @@ typedef uint32_t copy_v4 __attribute__((ext_vector_type(4)));
 typedef uint32_t copy_v2 __attribute__((ext_vector_type(2)));
 template <typename V>
 __global__ __launch_bounds__(256) void copy_kernel(V* __restrict__ out, const V* __restrict__ in, uint64_t n) {
-    const uint64_t i = uint64_t(blockIdx.x) * 256u + threadIdx.x;
-    if (i < n) __builtin_nontemporal_store(__builtin_nontemporal_load(in + i), out + i);
+    for (uint64_t i = uint64_t(blockIdx.x) * 256u + threadIdx.x; i < n; i += uint64_t(gridDim.x) * 256u)
+        __builtin_nontemporal_store(__builtin_nontemporal_load(in + i), out + i);
 }
 
 // out = data / scale, the division of scale_fft (a2affts.cu:83-88) per component,
@@ -143,8 +145,7 @@ template <typename R, int W>
 __global__ __launch_bounds__(256) void scale_kernel(R* __restrict__ out, const R* __restrict__ in, uint64_t n,
                                                     R scale) {
     typedef R vec __attribute__((ext_vector_type(W)));
-    const uint64_t i = uint64_t(blockIdx.x) * 256u + threadIdx.x;
-    if (i < n) {
+    for (uint64_t i = uint64_t(blockIdx.x) * 256u + threadIdx.x; i < n; i += uint64_t(gridDim.x) * 256u) {
         vec v = __builtin_nontemporal_load(reinterpret_cast<const vec*>(in) + i);
 #pragma unroll
         for (int c = 0; c < W; ++c) v[c] = v[c] / scale;
@@ -291,8 +292,7 @@ int launch_copy(void* out, const void* in, int64_t n, a2afft_dtype dtype, hipStr
     const uint64_t bytes = uint64_t(n) * (dtype == A2AFFT_C128 ? 16u : 8u);
     const bool wide = bytes % 16 == 0 && (reinterpret_cast<uintptr_t>(out) | reinterpret_cast<uintptr_t>(in)) % 16 == 0;
     const uint64_t units = wide ? bytes / 16 : bytes / 8;
-    const uint64_t blocks = (units + 255) / 256;
-    if (blocks >= (uint64_t(1) << 31)) return set_error(A2AFFT_E_SIZE, "copy: too large");
+    const uint64_t blocks = std::min<uint64_t>((units + 255) / 256, kStrideBlocks);
     if (wide)
         hipLaunchKernelGGL(copy_kernel<copy_v4>, dim3(blocks), dim3(256), 0, s, (copy_v4*)out, (const copy_v4*)in,
                            units);
@@ -311,8 +311,7 @@ int launch_scale(void* out, const void* in, int64_t n, double scale, a2afft_dtyp
     const bool aligned = (reinterpret_cast<uintptr_t>(out) | reinterpret_cast<uintptr_t>(in)) % 16 == 0;
     const bool wide = aligned && (c128 || n % 2 == 0);
     const uint64_t units = c128 ? (wide ? uint64_t(n) : uint64_t(n) * 2) : wide ? uint64_t(n) / 2 : uint64_t(n);
-    const uint64_t blocks = (units + 255) / 256;
-    if (blocks >= (uint64_t(1) << 31)) return set_error(A2AFFT_E_SIZE, "scale: too large");
+    const uint64_t blocks = std::min<uint64_t>((units + 255) / 256, kStrideBlocks);
     if (c128 && wide)
         hipLaunchKernelGGL((scale_kernel<double, 2>), dim3(blocks), dim3(256), 0, s, (double*)out, (const double*)in,
                            units, scale);

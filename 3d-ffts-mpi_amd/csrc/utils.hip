@@ -3,6 +3,8 @@
 // the host generator in a2afft/inputs.py.  Uniform values are exact
 // integer-derived doubles, so device and host agree bit for bit; normal
 // values use the device's log/sin/cos and agree to rounding.
+#include <algorithm>
+
 #include "common.h"
 
 namespace a2a {
@@ -23,9 +25,7 @@ __device__ __forceinline__ uint64_t bits53(uint32_t hi, uint32_t lo) {
 }
 
 template <typename V, int KIND>
-__global__ __launch_bounds__(256) void fill_kernel(V* __restrict__ out, uint64_t n, uint64_t seed, uint64_t g0) {
-    const uint64_t i = uint64_t(blockIdx.x) * 256 + threadIdx.x;
-    if (i >= n) return;
+__device__ __forceinline__ void fill_one(V* __restrict__ out, uint64_t i, uint64_t seed, uint64_t g0) {
     const uint64_t g = g0 + i;
     uint32_t c[4] = {uint32_t(g), uint32_t(g >> 32), uint32_t(KIND), 0u};
     philox(c, uint32_t(seed), uint32_t(seed >> 32));
@@ -48,6 +48,14 @@ __global__ __launch_bounds__(256) void fill_kernel(V* __restrict__ out, uint64_t
     out[i] = v;
 }
 
+// Grid-stride: a dispatch holds fewer than 2^32 work-items, so grids of more
+// elements loop (each element's value depends on its index only).
+template <typename V, int KIND>
+__global__ __launch_bounds__(256) void fill_kernel(V* __restrict__ out, uint64_t n, uint64_t seed, uint64_t g0) {
+    for (uint64_t i = uint64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += uint64_t(gridDim.x) * 256)
+        fill_one<V, KIND>(out, i, seed, g0);
+}
+
 }  // namespace a2a
 
 using namespace a2a;
@@ -57,8 +65,7 @@ extern "C" int a2afft_fill_random(void* out, a2afft_dtype dtype, int64_t n, uint
     if (!out || n < 0 || g0 < 0 || (kind != 0 && kind != 1) || (dtype != A2AFFT_C128 && dtype != A2AFFT_C64))
         return set_error(A2AFFT_E_INVALID_ARG, "fill_random: bad arguments");
     if (n == 0) return A2AFFT_OK;
-    const uint64_t blocks = (uint64_t(n) + 255) / 256;
-    if (blocks >= (uint64_t(1) << 31)) return set_error(A2AFFT_E_SIZE, "fill_random: too large");
+    const uint64_t blocks = std::min<uint64_t>((uint64_t(n) + 255) / 256, kStrideBlocks);
     hipStream_t s = (hipStream_t)stream;
     if (dtype == A2AFFT_C128) {
         if (kind == 0) hipLaunchKernelGGL((fill_kernel<double2, 0>), dim3(blocks), dim3(256), 0, s, (double2*)out, uint64_t(n), seed, uint64_t(g0));

@@ -301,6 +301,98 @@ def run_c3b(args, torch, a2afft):
     print(json.dumps(out), flush=True)
 
 
+def run_grid(args, torch, a2afft):
+    """One natural-order nx x ny x nz grid (no virtual ranks) through the
+    plain-grid plan, in place: uniform Philox input (seed 7).  Reports ms per
+    forward (rotating copies of the input, refilled outside the timed region),
+    per-pass ms, rel-L2 and max-abs against rocFFT's FP64 fftn of the same
+    input (grids up to 2^28 points), and rocFFT's own forward time on the
+    grid's precision (out of place, where two copies fit)."""
+    shape = tuple(int(v) for v in args.shape.lower().split("x"))
+    if len(shape) != 3:
+        raise SystemExit("--shape NXxNYxNZ")
+    prec = args.precision or "fp32"
+    if prec not in ("fp32", "fp64"):
+        raise SystemExit("--config grid runs fp32 or fp64")
+    cdt = torch.complex64 if prec == "fp32" else torch.complex128
+    esize = 8 if prec == "fp32" else 16
+    n = shape[0] * shape[1] * shape[2]
+    log(f"[bench] grid {shape} {prec}: {n * esize / 2**30:.2f} GiB")
+    x = torch.empty(shape, dtype=cdt, device="cuda")
+    a2afft.fill_random(x, 7, "uniform")
+    plan = a2afft.GridDfft(shape, prec)
+    if args.autotune:
+        work = torch.empty_like(x)
+        work.copy_(x)
+        plan.autotune(work, iters=3)
+        del work
+    log(plan.describe())
+    rel = mx = None
+    if n <= (1 << 28):
+        y = x.clone()
+        plan.forward(y)
+        ref = torch.fft.fftn(x.to(torch.complex128))
+        d = y.to(torch.complex128) - ref
+        rel = float(torch.linalg.vector_norm(d) / torch.linalg.vector_norm(ref))
+        mx = float(d.abs().max())
+        del y, ref, d
+    nrot = rotation_count(x, args.steps, 16 << 30)
+    rot = Rotation(torch, x, nrot)
+    # At most 3 timed forwards land on one copy between refills: values grow
+    # by at most N per forward, so 3 stay finite in float32 even at 2^33
+    # points (a 64 GiB grid gets one copy, so 3 timed forwards).
+    steps = min(args.steps, 3 * nrot)
+    warm = min(args.warmup, 3 * nrot)
+    t_fwd = _events_ms(torch, lambda buf: plan.forward(buf), rot, steps, warm)
+    t_pass = [_events_ms(torch, lambda buf, p=p: a2afft.execute_pass(plan, buf, p, -1), rot, steps, 1)
+              for p in range(3)]
+    finite = rot.finite()
+    del rot
+    t_roc = None
+    free, _ = torch.cuda.mem_get_info()
+    if free > 3 * n * esize:
+        out_buf = torch.empty_like(x)
+        torch.fft.fftn(x, out=out_buf)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(args.steps):
+            torch.fft.fftn(x, out=out_buf)
+        e1.record()
+        torch.cuda.synchronize()
+        t_roc = e0.elapsed_time(e1) / args.steps
+        del out_buf
+    bytes_pass = 2 * n * esize
+    dom = max(range(3), key=lambda p: t_pass[p])
+    ach = bytes_pass / (t_pass[dom] * 1e-3) / 1e9
+    chosen = [ln for ln in plan.describe().splitlines() if ln.startswith("chosen")]
+    out = {
+        "metric": f"ms per {shape[0]}x{shape[1]}x{shape[2]} c2c 3D FFT (natural order)",
+        "value": round(t_fwd, 4), "unit": "ms", "n_gpus": 1, "steps": steps, "warmup": warm,
+        "ms_per_step": round(t_fwd, 4), "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
+        "dtype": "f32" if prec == "fp32" else "f64", "data": "synthetic",
+        "config": {"workload": f"one natural-order {shape[0]}x{shape[1]}x{shape[2]} "
+                               f"{'complex64' if prec == 'fp32' else 'complex128'} grid, uniform Philox seed 7, "
+                               "plain-grid plan, forward", "name": "grid", "shape": list(shape), "precision": prec,
+                   "autotune": bool(args.autotune)},
+        "kernels": chosen,
+        "pass_ms": {"z": round(t_pass[0], 4), "x": round(t_pass[1], 4), "y": round(t_pass[2], 4)},
+        "pass_gbs": {k: round(bytes_pass / (t * 1e-3) / 1e9, 1) for k, t in zip("zxy", t_pass)},
+        "hbm_gbs_whole": round(3 * bytes_pass / (t_fwd * 1e-3) / 1e9, 1),
+        "roofline_frac_whole": round(3 * bytes_pass / (t_fwd * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "gflops": round(5.0 * n * math.log2(n) / (t_fwd * 1e-3) / 1e9, 1),
+        "rel_l2_vs_fp64": rel, "max_abs_vs_fp64": mx, "finite": finite,
+        "library_gpu": {"what": "rocFFT through torch.fft.fftn, out of place, same precision, forward",
+                        "rocfft_fftn_ms": None if t_roc is None else round(t_roc, 4)},
+        "roofline": {"bound": "hbm", "kernel": f"{'zxy'[dom]} pass", "bytes_per_launch": bytes_pass,
+                     "launch_ms": round(t_pass[dom], 4), "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None},
+        "cpu_baseline": None,
+    }
+    plan.destroy()
+    print(json.dumps(out), flush=True)
+
+
 C5_SHAPES = [(32, 32, 32), (64, 64, 64), (128, 128, 128), (256, 256, 256), (128, 256, 64), (256, 64, 512)]
 
 
@@ -478,16 +570,18 @@ def main():
     # 30 untimed forwards: after the FP64 accuracy check, 5 left the timed
     # loop 1.5 % slower than the steady state
     ap.add_argument("--warmup", type=int, default=30)
-    ap.add_argument("--config", default="c3", choices=["c3", "c2", "c3b", "c4", "c5"],
+    ap.add_argument("--config", default="c3", choices=["c3", "c2", "c3b", "c4", "c5", "grid"],
                     help="c3: 512^3 complex64 over dims [1,4,4], GRF seed 2 (the headline); "
                          "c2: 256^3 complex128 over dims [1,2,4], N(0,1) seed 1; "
                          "c3b: one rank's 2048x256x256 complex64 share of 2048^3 over dims [1,8,8], x stage; "
                          "c4: 512^3 GRF seed 3, FP32 vs split-BF16 vs split-FP16; "
-                         "c5: 64 uneven complex64 grids (seed 4) in one persistent launch")
+                         "c5: 64 uneven complex64 grids (seed 4) in one persistent launch; "
+                         "grid: one natural-order grid of --shape (any supported extents, up to 2^40 points)")
     ap.add_argument("--no-variants", action="store_true",
                     help="c3: skip the FP32 / split-BF16 / split-FP16 comparison on the seed-3 field")
     ap.add_argument("--precision", default=None, choices=["fp32", "fp64", "split_bf16", "split_fp16"])
     ap.add_argument("--ng", type=int, default=None)
+    ap.add_argument("--shape", default="480x480x480", help="--config grid: NXxNYxNZ")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--autotune", action="store_true", help="time every kernel variant per pass and keep the fastest")
     ap.add_argument("--chunk", type=int, default=-1, help="z columns per x/y chunk (-1: plan default / autotuned)")
@@ -507,6 +601,8 @@ def main():
         return run_c3b(args, torch, a2afft)
     if args.config == "c5":
         return run_c5(args, torch, a2afft)
+    if args.config == "grid":
+        return run_grid(args, torch, a2afft)
     if args.config == "c4":
         return run_c4(args, torch, a2afft)
     if args.config == "c2":
