@@ -141,19 +141,23 @@ int64_t hc_grid_enumerate(int nx, int ny, int nz, int pass, int T, int E, int64_
 }
 
 // Grids of 2^32 points and more: for the hardware workgroups [b_begin,
-// b_end) of pass `pass` of a plain nx x ny x nz C-order grid with shape
-// (T, E) and flags (KF_XCD, KF_A2MAJ honoured), every address the kernel
-// computes (line_base + elem_off) against the element's C-order index
-// computed directly in 64 bits from its (x, y, z).  grid_out: the launch's
-// workgroup count.  Returns the number of mismatches or out-of-range
-// addresses (0 = all agree), or a negative status.
-int64_t hc_grid_check(int nx, int ny, int nz, int pass, int T, int E, int flags, uint32_t b_begin, uint32_t b_end,
-                      int64_t* grid_out, int64_t* checked_out) {
+// b_end) of pass `pass` of an nx x ny x nz grid in the virtual-rank block
+// layout over `dims` ([1,1,1]: one C-order grid) with shape (T, E) and flags
+// (KF_XCD, KF_A2MAJ honoured), every address the kernel computes (line_base
+// + elem_off) against the element's index computed directly in 64 bits from
+// its global (x, y, z): rank (x / lx, y / ly, z / lz) row-major times the
+// block size, plus the C-order offset inside the block.  grid_out: the
+// launch's workgroup count.  Returns the number of mismatches or
+// out-of-range addresses (0 = all agree), or a negative status.
+int64_t hc_grid_check(int nx, int ny, int nz, const int* dims, int pass, int T, int E, int flags, uint32_t b_begin,
+                      uint32_t b_end, int64_t* grid_out, int64_t* checked_out) {
     const int n3[3] = {nx, ny, nz};
-    const int dims[3] = {1, 1, 1};
+    const int lgs[3] = {nx / dims[0], ny / dims[1], nz / dims[2]};
+    const uint64_t lx = uint64_t(lgs[0]), ly = uint64_t(lgs[1]), lz = uint64_t(lgs[2]);
     PassMaps all[3];
+    const int64_t nlocal = int64_t(lx * ly * lz);
     const int64_t total = int64_t(nx) * ny * nz;
-    fused_pass_maps(n3, dims, n3, total, all);
+    fused_pass_maps(n3, dims, lgs, nlocal, all);
     const PassMaps pm = all[pass];
     const int N = int(pm.fft.n);
     FftGeom g;
@@ -189,7 +193,9 @@ int64_t hc_grid_check(int nx, int ny, int nz, int pass, int T, int E, int flags,
                 if (pass == 0) { x = u1; y = u2; z = n; }
                 else if (pass == 1) { x = n; y = u1; z = u2; }
                 else { x = u1; y = n; z = u2; }
-                const int64_t want = int64_t((x * uint64_t(ny) + y) * uint64_t(nz) + z);
+                const uint64_t rank = ((x / lx) * uint64_t(dims[1]) + y / ly) * uint64_t(dims[2]) + z / lz;
+                const int64_t want =
+                    int64_t(rank * uint64_t(nlocal) + ((x % lx) * ly + (y % ly)) * lz + (z % lz));
                 const int64_t got = base + elem_off(g, e * TPL);   // line_base includes off(t)
                 if (got != want || got < 0 || got >= total) ++bad;
                 ++checked;

@@ -326,6 +326,49 @@ def test_whole_grid_of_2e32_points_and_more(a2afft_mod, cuda, shape, prec, tol):
     assert rest < tol, rest
 
 
+def test_2048cube_over_64_virtual_ranks(a2afft_mod, cuda):
+    """The whole grid of the rank-share configuration on one device: 2048^3
+    complex64 over 64 virtual ranks, dims [1,8,8] (64 GiB, the reference's
+    64-process layout).  An impulse at global r0, placed in its rank's block,
+    transforms to exp(-2 pi i k.r0 / N) at every k: each rank's block is
+    checked against that closed form in FP64; the inverse returns the
+    impulse."""
+    import torch
+    Ng, P, dims, tol = 2048, 64, (1, 8, 8), 1e-5
+    free, _ = torch.cuda.mem_get_info()
+    if free < Ng ** 3 * 8 + (32 << 30):
+        pytest.skip("needs 64 GiB plus 32 GiB of work space on the device")
+    lx, ly, lz = Ng // dims[0], Ng // dims[1], Ng // dims[2]
+    dist = a2afft_mod.Distribution(Ng, P, dims=dims, dtype="c64")
+    x = torch.zeros((P, lx, ly, lz), dtype=torch.complex64, device="cuda")
+    r0 = (1234, 777, 1999)
+    rank0 = ((r0[0] // lx) * dims[1] + r0[1] // ly) * dims[2] + r0[2] // lz   # rank2coords, row-major
+    at0 = (rank0, r0[0] % lx, r0[1] % ly, r0[2] % lz)
+    x[at0] = 1
+    plan = a2afft_mod.Dfft(dist)
+    plan.make_plans(None)
+    plan.forward(x)
+    torch.cuda.synchronize()
+    k = torch.arange(Ng, device="cuda", dtype=torch.int64)
+    ph = [torch.exp(((k * c) % Ng).to(torch.float64) * (-2j * math.pi / Ng)) for c in r0]
+    worst = 0.0
+    for r in range(P):
+        c0, c1, c2 = r // (dims[1] * dims[2]), (r // dims[2]) % dims[1], r % dims[2]
+        yz = ph[1][c1 * ly:(c1 + 1) * ly, None] * ph[2][None, c2 * lz:(c2 + 1) * lz]
+        for s in range(0, lx, 512):
+            want = ph[0][c0 * lx + s:c0 * lx + s + 512, None, None] * yz[None]
+            worst = max(worst, float((x[r, s:s + 512].to(torch.complex128) - want).abs().max()))
+            del want
+    assert worst < tol, worst
+    plan.inverse(x)
+    torch.cuda.synchronize()
+    peak = complex(x[at0].item())
+    assert abs(peak - 1) < tol, peak
+    x[at0] = 0
+    rest = max(float(x[r].abs().max()) for r in range(P))
+    assert rest < tol, rest
+
+
 def test_c5_batch_of_64_grids(a2afft_mod, cuda):
     """The 64-grid batch (shapes drawn with seed 4): every grid equals its
     single-grid plan bit for bit, for the default and a grouped, skewed order;

@@ -48,7 +48,7 @@ def hc(tmp_path_factory):
     L.hc_cz_fused_enumerate.argtypes = [P, P, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, P, P, P,
                                         ctypes.c_int64]
     L.hc_cz_fused_enumerate.restype = ctypes.c_int64
-    L.hc_grid_check.argtypes = [ctypes.c_int] * 7 + [ctypes.c_uint32, ctypes.c_uint32, P, P]
+    L.hc_grid_check.argtypes = [ctypes.c_int] * 3 + [P] + [ctypes.c_int] * 4 + [ctypes.c_uint32, ctypes.c_uint32, P, P]
     L.hc_grid_check.restype = ctypes.c_int64
     return L
 
@@ -389,31 +389,38 @@ def test_chirp_pass_addresses(hc, cfg):
             assert (ks == ks[:, :1]).all(), pass_
 
 
-# (grid, pass, T, E, flags): the default launches of whole grids of 2^32
-# points and more (2048^3 complex64, 2048^2 x 1024 complex128), and a small
-# grid where every workgroup is checked.
-BIG_GRID_CASES = [((2048, 2048, 2048), 0, 2, 16, 7), ((2048, 2048, 2048), 1, 16, 64, 41),
-                  ((2048, 2048, 2048), 2, 16, 64, 33), ((2048, 2048, 1024), 0, 4, 16, 1),
-                  ((2048, 2048, 1024), 1, 8, 32, 39), ((2048, 2048, 1024), 2, 8, 32, 33),
-                  ((1024, 1024, 1024), 1, 16, 32, 37), ((64, 64, 64), 1, 16, 16, 9)]
+# (grid, dims, pass, T, E, flags): the default launches of whole grids of
+# 2^32 points and more (2048^3 complex64 as one grid and over 64 virtual
+# ranks, 2048^2 x 1024 complex128), and small grids where every workgroup is
+# checked.
+BIG_GRID_CASES = [((2048, 2048, 2048), (1, 1, 1), 0, 2, 16, 7), ((2048, 2048, 2048), (1, 1, 1), 1, 16, 64, 41),
+                  ((2048, 2048, 2048), (1, 1, 1), 2, 16, 64, 33), ((2048, 2048, 1024), (1, 1, 1), 0, 4, 16, 1),
+                  ((2048, 2048, 1024), (1, 1, 1), 1, 8, 32, 39), ((2048, 2048, 1024), (1, 1, 1), 2, 8, 32, 33),
+                  ((1024, 1024, 1024), (1, 1, 1), 1, 16, 32, 37), ((64, 64, 64), (1, 1, 1), 1, 16, 16, 9),
+                  ((2048, 2048, 2048), (1, 8, 8), 0, 2, 16, 7), ((2048, 2048, 2048), (1, 8, 8), 1, 16, 64, 41),
+                  ((2048, 2048, 2048), (1, 8, 8), 2, 16, 64, 33), ((2048, 2048, 2048), (4, 4, 4), 0, 2, 16, 7),
+                  ((2048, 2048, 2048), (4, 4, 4), 1, 16, 64, 41), ((2048, 2048, 2048), (4, 4, 4), 2, 16, 64, 33),
+                  ((64, 64, 64), (2, 2, 2), 0, 4, 16, 1), ((64, 64, 64), (1, 4, 4), 2, 16, 16, 8)]
 
 
-@pytest.mark.parametrize("case", BIG_GRID_CASES, ids=lambda c: f"{'x'.join(map(str, c[0]))}_pass{c[1]}")
+@pytest.mark.parametrize("case", BIG_GRID_CASES,
+                         ids=lambda c: f"{'x'.join(map(str, c[0]))}_dims{''.join(map(str, c[1]))}_pass{c[2]}")
 def test_grid_addresses_64bit(hc, case):
     """Every address a pass computes (line_base + elem_off, XCD renumbering and
-    unit order included) equals the element's C-order index computed directly
-    in 64 bits from its (x, y, z): for the first and last 512 workgroups and 36
-    runs of 64 spread over the launch (every workgroup at 64^3).  A 32-bit
-    overflow anywhere in the geometry shows up as a mismatch or an address
-    beyond the grid."""
-    shape, pass_, T, E, flags = case
+    unit order included) equals the element's index computed directly in 64
+    bits from its global (x, y, z) in the block layout over dims: for the
+    first and last 512 workgroups and 36 runs of 64 spread over the launch
+    (every workgroup on the 64^3 grids).  A 32-bit overflow anywhere in the
+    geometry shows up as a mismatch or an address beyond the grid."""
+    shape, dims, pass_, T, E, flags = case
+    d = (ctypes.c_int * 3)(*dims)
     grid, cnt = ctypes.c_int64(), ctypes.c_int64()
-    assert hc.hc_grid_check(*shape, pass_, T, E, flags, 0, 1, ctypes.byref(grid), ctypes.byref(cnt)) == 0
+    assert hc.hc_grid_check(*shape, d, pass_, T, E, flags, 0, 1, ctypes.byref(grid), ctypes.byref(cnt)) == 0
     G = grid.value
     runs = [(0, G)] if G <= 4096 else [(0, 512), (G - 512, G)] + [(G * k // 37, G * k // 37 + 64) for k in range(1, 37)]
     total = 0
     for a, b in runs:
-        bad = hc.hc_grid_check(*shape, pass_, T, E, flags, a, b, ctypes.byref(grid), ctypes.byref(cnt))
+        bad = hc.hc_grid_check(*shape, d, pass_, T, E, flags, a, b, ctypes.byref(grid), ctypes.byref(cnt))
         assert bad == 0, (a, b, bad)
         total += cnt.value
     assert total > 0
