@@ -240,7 +240,7 @@ int launch_transpose(int which, const void* src, void* dst, const int lgs[3], in
     if (aligned) {
         const uint32_t T = c128 ? 32 : 64;
         const uint32_t ta = (A + T - 1) / T, tc = (C + T - 1) / T;
-        if (uint64_t(ta) * tc >= (uint64_t(1) << 31)) return set_error(A2AFFT_E_SIZE, "transpose: grid too large");
+        if (uint64_t(ta) * tc >= (uint64_t(1) << 24)) return set_error(A2AFFT_E_SIZE, "transpose: grid too large");
         dim3 grid(ta * tc, B * M, nbuffers);
         if (c128)
             hipLaunchKernelGGL((transpose16_kernel<double2, 32, 32>), grid, dim3(256), 0, s, (const double2*)src,
@@ -251,7 +251,8 @@ int launch_transpose(int which, const void* src, void* dst, const int lgs[3], in
         return hip_check(int(hipGetLastError()), "transpose kernel launch");
     }
     const uint32_t ta = (A + 31) / 32, tc = (C + 31) / 32;
-    if (uint64_t(ta) * tc >= (uint64_t(1) << 31)) return set_error(A2AFFT_E_SIZE, "transpose: grid too large");
+    // a dispatch holds fewer than 2^32 work-items along x (256 a workgroup)
+    if (uint64_t(ta) * tc >= (uint64_t(1) << 24)) return set_error(A2AFFT_E_SIZE, "transpose: grid too large");
     dim3 grid(ta * tc, B * M, nbuffers);
     if (dtype == A2AFFT_C128)
         hipLaunchKernelGGL(transpose_kernel<double2>, grid, dim3(256), 0, s, (const double2*)src, (double2*)dst, A,

@@ -297,6 +297,7 @@ def test_whole_grid_of_2e32_points_and_more(a2afft_mod, cuda, shape, prec, tol):
     nx, ny, nz = shape
     cdt = torch.complex64 if prec == "fp32" else torch.complex128
     need = nx * ny * nz * (8 if prec == "fp32" else 16)
+    torch.cuda.empty_cache()   # blocks cached by earlier tests count as used
     free, _ = torch.cuda.mem_get_info()
     if free < need + (32 << 30):
         pytest.skip(f"needs {need >> 30} GiB plus 32 GiB of work space on the device")
@@ -335,6 +336,7 @@ def test_2048cube_over_64_virtual_ranks(a2afft_mod, cuda):
     impulse."""
     import torch
     Ng, P, dims, tol = 2048, 64, (1, 8, 8), 1e-5
+    torch.cuda.empty_cache()   # blocks cached by earlier tests count as used
     free, _ = torch.cuda.mem_get_info()
     if free < Ng ** 3 * 8 + (32 << 30):
         pytest.skip("needs 64 GiB plus 32 GiB of work space on the device")
@@ -367,6 +369,37 @@ def test_2048cube_over_64_virtual_ranks(a2afft_mod, cuda):
     x[at0] = 0
     rest = max(float(x[r].abs().max()) for r in range(P))
     assert rest < tol, rest
+
+
+def test_2048cube_over_64_virtual_ranks_staged_equals_fused(a2afft_mod, cuda):
+    """The reference's stage order (18 launches: transposes, device-local
+    all-to-alls, pack / unpack, batched 2048-point FFTs, copy) on the whole
+    2048^3 complex64 grid over 64 virtual ranks, dims [1,8,8]: the same bits
+    as the fused plan (input, scratch and the fused result take 192 GiB)."""
+    import torch
+    Ng, P, dims = 2048, 64, (1, 8, 8)
+    n = Ng ** 3
+    torch.cuda.empty_cache()   # blocks cached by earlier tests count as used
+    free, _ = torch.cuda.mem_get_info()
+    if free < 3 * n * 8 + (16 << 30):
+        pytest.skip("needs 192 GiB plus 16 GiB of work space on the device")
+    dist = a2afft_mod.Distribution(Ng, P, dims=dims, dtype="c64")
+    x = torch.empty(n, dtype=torch.complex64, device="cuda")
+    a2afft_mod.fill_random(x, 31, "uniform")
+    y = x.clone()
+    fused = a2afft_mod.Dfft(dist)
+    fused.make_plans(None)
+    fused.forward(y)
+    torch.cuda.synchronize()
+    staged = a2afft_mod.Dfft(dist, exec="staged")
+    scratch = torch.empty_like(x)
+    staged.make_plans(scratch)
+    staged.forward(x)
+    torch.cuda.synchronize()
+    assert _same_bits(x, y)
+    del staged, scratch, x
+    step = 1 << 30
+    assert all(bool(torch.isfinite(torch.view_as_real(y[s:s + step])).all()) for s in range(0, n, step))
 
 
 def test_c5_batch_of_64_grids(a2afft_mod, cuda):
