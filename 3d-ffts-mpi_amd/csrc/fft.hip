@@ -240,6 +240,7 @@ static const Pref kPrefs[] = {
     // 95.5 -> 37.8 ms, y 39.7 -> 26.4 ms against 32 B rows)
     {2048, 1, 1, 1, -1, {8, 32, KF_XCD | KF_PAD | KF_NT | KF_HALFX}},
     {2048, 1, 1, 0, -1, {8, 32, KF_XCD | KF_HALFX}},
+    {2048, 0, 1, 2, -1, {16, 64, KF_XCD | KF_PAD | KF_NT | KF_HALFX}},   // 2048^3 grid x pass: 46.7 -> 44.8 ms
     {2048, 0, 1, 1, -1, {16, 64, KF_XCD | KF_A2MAJ | KF_HALFX}},
     {2048, 0, 1, -1, -1, {16, 64, KF_XCD | KF_HALFX}},
     {4096, 0, 0, -1, -1, {1, 16, KF_PAD}},

@@ -393,7 +393,8 @@ def test_chirp_pass_addresses(hc, cfg):
 # 2^32 points and more (2048^3 complex64 as one grid and over 64 virtual
 # ranks, 2048^2 x 1024 complex128), and small grids where every workgroup is
 # checked.
-BIG_GRID_CASES = [((2048, 2048, 2048), (1, 1, 1), 0, 2, 16, 7), ((2048, 2048, 2048), (1, 1, 1), 1, 16, 64, 41),
+BIG_GRID_CASES = [((2048, 2048, 2048), (1, 1, 1), 0, 2, 16, 7), ((2048, 2048, 2048), (1, 1, 1), 1, 16, 64, 39),
+                  ((2048, 2048, 2048), (1, 1, 1), 1, 16, 64, 41),
                   ((2048, 2048, 2048), (1, 1, 1), 2, 16, 64, 33), ((2048, 2048, 1024), (1, 1, 1), 0, 4, 16, 1),
                   ((2048, 2048, 1024), (1, 1, 1), 1, 8, 32, 39), ((2048, 2048, 1024), (1, 1, 1), 2, 8, 32, 33),
                   ((1024, 1024, 1024), (1, 1, 1), 1, 16, 32, 37), ((64, 64, 64), (1, 1, 1), 1, 16, 16, 9),
