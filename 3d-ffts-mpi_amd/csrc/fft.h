@@ -108,6 +108,9 @@ struct MixlPlan {
     // each address is computed).
     int rds[kMixMaxPasses], wrs[kMixMaxPasses];
     int64_t gin, gout;
+    // Where the twiddles sit in LDS (in values, after the data tile), pass p's
+    // as [r][k] at toff[p] + r ns[p] + k; 0 = read from the global table.
+    int twl;
 };
 bool mixl_plan(const AxisPass& pass, int N, bool dbl, MixlPlan* lp, int force_T = 0);
 const void* mixl_kernel_for(int EM, bool dbl, int strided, int flags, int dir);

@@ -110,9 +110,15 @@ __device__ __forceinline__ void mixl_pass(V (&v)[EM], V* __restrict__ lds, V* __
     for (int k = 0; k < KM; ++k) {
         if (!first) {
             const int kk = int(lp.dns[p].mod(uint32_t(jj[k])));
-            const V* const w = tw + lp.toff[p] + kk * R;
+            if (STRIDED && sizeof(real) == 4 && lp.twl) {
+                const V* const w = lds + lp.twl + lp.toff[p] + kk;
 #pragma unroll
-            for (int r = 1; r < R; ++r) v[k * R + r] = c_mul(v[k * R + r], w[r]);
+                for (int r = 1; r < R; ++r) v[k * R + r] = c_mul(v[k * R + r], w[r * Ns]);
+            } else {
+                const V* const w = tw + lp.toff[p] + kk * R;
+#pragma unroll
+                for (int r = 1; r < R; ++r) v[k * R + r] = c_mul(v[k * R + r], w[r]);
+            }
         }
         dft<R, DIR>(&v[k * R]);
     }
@@ -160,6 +166,19 @@ __global__ __launch_bounds__(kMixlThreads) void mixl_kernel(typename CT<real>::V
     extern __shared__ __align__(16) unsigned char mixl_smem[];
     V* const lds = reinterpret_cast<V*>(mixl_smem);
     const uint32_t block = XCD ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+    if (STRIDED && sizeof(real) == 4 && lp.twl) {
+        // the twiddles of passes 1.. into LDS, transposed to [r][k] so that
+        // neighbouring butterflies read neighbouring slots; the first pass
+        // needs none, and its closing barrier orders these writes before the
+        // second pass reads them
+        for (int p = 1; p < lp.npass; ++p) {
+            const int R = lp.radix[p], Ns = lp.ns[p];
+            for (int d = int(threadIdx.x); d < R * Ns; d += kMixlThreads) {
+                const int r = int(lp.dns[p].div(uint32_t(d))), kk = d - r * Ns;
+                lds[lp.twl + lp.toff[p] + d] = tw[lp.toff[p] + kk * R + r];
+            }
+        }
+    }
     V v[EM];
     for (int p = 0; p < lp.npass; ++p) {
         // the thread index as the compiler must re-read it every pass: left
@@ -328,6 +347,16 @@ bool mixl_plan(const AxisPass& ps, int N, bool dbl, MixlPlan* out, int force_T) 
         ns *= q.radix[i];
     }
     q.twsize = off;
+    // Strided complex64 tiles stage the twiddles in LDS when they take at
+    // most 16 KiB and fit beside the tile (long lines read the global table:
+    // its size grows like N): 480^3 x and y passes 0.528 -> 0.468 ms, half
+    // the vector-memory reads.  Contiguous tiles (the LDS path cost them a
+    // wave per SIMD: z pass 0.487 -> 0.501 ms) and complex128 (121-125 ->
+    // 173-185 VGPRs, 240^3 0.257 -> 0.351 ms) read the global table.
+    if (!dbl && ps.strided) {
+        const int tile = mixl_lds_bytes(q, dbl, ps.strided) / esize;
+        q.twl = (np > 1 && off * esize <= 16384 && (tile + off) * esize <= kMixlMaxLds) ? tile : 0;
+    }
     // LDS steps.  Padded rows put slot d at d + (d >> sh).  A pass reads
     // d = j + r B, with B = N / R a multiple of the previous pass's 2^sh (the
     // power-of-two radices come first, largest first, so B keeps every factor
@@ -369,6 +398,7 @@ const void* mixl_kernel_for(int EM, bool dbl, int strided, int flags, int dir) {
 
 int mixl_lds_bytes(const MixlPlan& lp, bool dbl, int strided) {
     const int esize = dbl ? 16 : 8;
+    if (lp.twl) return (lp.twl + lp.twsize) * esize;   // the tile, then the twiddles
     if (strided) return lp.N * (lp.T + 1) * esize;
     int row = lp.N;
     for (int i = 0; i < lp.npass; ++i)
