@@ -77,6 +77,18 @@ const void* chirp_pass_kernel(int N, bool dbl, int strided);
 int launch_chirp_pass(const FftGeom& g, uint32_t units, int strided, void* data, a2afft_dtype dtype, int N, int dir,
                       hipStream_t s);
 
+// Whether `extra` bytes of twiddles (at most 16 KiB) may sit beside a
+// workgroup's `tile` bytes of LDS: where they leave as many workgroups per CU
+// (160 KiB of LDS), or at least two.  Measured: 960-point strided tiles gain
+// though they drop from 4 to 3 workgroups a CU (0.741 -> 0.647 ms per
+// 960x960x128 x pass), 640-point wide tiles lose from 2 to 1 (0.467 -> 0.620).
+inline bool lds_room_for_twiddles(int tile, int extra) {
+    constexpr int kCuLds = 163840;
+    if (extra > 16384 || tile <= 0) return false;
+    const int before = kCuLds / tile, after = kCuLds / (tile + extra);
+    return after == before || after >= 2;
+}
+
 // Stockham FFT with the radix schedule in the kernel arguments (KF_MIX,
 // fft_mixed.hip) for 2^a 3^b 5^c lengths the fixed schedules do not cover.
 constexpr int kMixMaxPasses = 16;
@@ -84,6 +96,7 @@ struct MixedPlan {
     int N, E, npass, T, TPL, twsize, NT;   // NT: threads per workgroup
     int radix[kMixMaxPasses], ns[kMixMaxPasses], toff[kMixMaxPasses];
     FastDiv dns[kMixMaxPasses];   // division by ns[p]
+    int twl;   // LDS offset of the twiddles (pass p's as [r][k] at toff[p] + r ns[p] + k), 0 = global table
 };
 // force_T > 0: that tile (lines per workgroup) if it fits the pass, else false.
 bool mixed_plan(const AxisPass& pass, int N, bool dbl, MixedPlan* mp, int force_T = 0);

@@ -348,14 +348,17 @@ bool mixl_plan(const AxisPass& ps, int N, bool dbl, MixlPlan* out, int force_T) 
     }
     q.twsize = off;
     // Strided complex64 tiles stage the twiddles in LDS when they take at
-    // most 16 KiB and fit beside the tile (long lines read the global table:
+    // most 16 KiB and there is room beside the tile (long lines read the
+    // global table:
     // its size grows like N): 480^3 x and y passes 0.528 -> 0.468 ms, half
     // the vector-memory reads.  Contiguous tiles (the LDS path cost them a
     // wave per SIMD: z pass 0.487 -> 0.501 ms) and complex128 (121-125 ->
     // 173-185 VGPRs, 240^3 0.257 -> 0.351 ms) read the global table.
     if (!dbl && ps.strided) {
         const int tile = mixl_lds_bytes(q, dbl, ps.strided) / esize;
-        q.twl = (np > 1 && off * esize <= 16384 && (tile + off) * esize <= kMixlMaxLds) ? tile : 0;
+        q.twl = (np > 1 && lds_room_for_twiddles(tile * esize, off * esize) && (tile + off) * esize <= kMixlMaxLds)
+                    ? tile
+                    : 0;
     }
     // LDS steps.  Padded rows put slot d at d + (d >> sh).  A pass reads
     // d = j + r B, with B = N / R a multiple of the previous pass's 2^sh (the
