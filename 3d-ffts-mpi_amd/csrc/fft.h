@@ -108,18 +108,19 @@ int mixed_plan_regs(int N, bool dbl);
 // every pass spreads its T*N/R butterflies over the workgroup, so radices
 // 16, 8, 4, 3 and 5 mix freely (480 = 8*4*3*5) instead of being limited to
 // divisors of one per-thread element count.
+constexpr int kMixlMaxPasses = 8;   // every 2^a 3^b 5^c length up to 4096 takes at most 7
 struct MixlPlan {
     int N, npass, T, NT, EM, twsize;   // EM: register budget, complex values per thread
-    int radix[kMixMaxPasses], ns[kMixMaxPasses], toff[kMixMaxPasses], pad[kMixMaxPasses];
-    FastDiv dns[kMixMaxPasses];   // j / ns[p]
-    FastDiv dB[kMixMaxPasses];    // butterfly index / (N / radix[p]) (contiguous tiles)
+    int radix[kMixlMaxPasses], ns[kMixlMaxPasses], toff[kMixlMaxPasses], pad[kMixlMaxPasses];
+    FastDiv dns[kMixlMaxPasses];   // j / ns[p]
+    FastDiv dB[kMixlMaxPasses];    // butterfly index / (N / radix[p]) (contiguous tiles)
     FastDiv dT;                   // butterfly index / T (strided tiles)
     // Address steps of contiguous tiles, fixed per pass: the LDS slots of a
     // butterfly's inputs (outputs) lie rds[p] (wrs[p]) apart; in global
     // memory, gin between the first pass's inputs and gout between the last
     // pass's outputs, or 0 where the axis's block extent breaks the step (then
     // each address is computed).
-    int rds[kMixMaxPasses], wrs[kMixMaxPasses];
+    int rds[kMixlMaxPasses], wrs[kMixlMaxPasses];
     int64_t gin, gout;
     // Where the twiddles sit in LDS (in values, after the data tile), pass p's
     // as [r][k] at toff[p] + r ns[p] + k; 0 = read from the global table.

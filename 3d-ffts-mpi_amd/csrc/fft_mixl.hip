@@ -297,7 +297,7 @@ bool mixl_plan(const AxisPass& ps, int N, bool dbl, MixlPlan* out, int force_T) 
         if (kPow2[a][i]) q.radix[np++] = kPow2[a][i];
     for (int i = 0; i < b3; ++i) q.radix[np++] = 3;
     for (int i = 0; i < c5; ++i) q.radix[np++] = 5;
-    if (np > kMixMaxPasses || np < 1) return false;
+    if (np > kMixlMaxPasses || np < 1) return false;
     q.npass = np;
     q.NT = kMixlThreads;
     // LDS row of a contiguous line: N plus the largest padding of a pass
