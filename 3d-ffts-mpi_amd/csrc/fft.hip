@@ -28,13 +28,17 @@ static KSel cz_sel(int N) { return KSel{1, N, KF_CZ}; }
 
 // Kernel choice for a length no fixed-schedule kernel covers: for 2^a 3^b 5^c
 // lengths, the runtime-schedule Stockham when it holds at most 24 values a
-// thread, else the per-pass-mapped Stockham, else the runtime schedule with
-// more; then the chirp-z pass (9 points and up), else the direct DFT.  The
-// runtime schedule needs E divisible by every radix: 2^a 3 and 2^a 5 lengths
-// (384, 640, 768) run at E = 12 or 20 and 1.0-1.3x faster than the
-// per-pass-mapped kernel, lengths with both 3 and 5 (120, 360, 480, 960) at
-// E = 30 and 1.1-4.4x slower (49 lengths, 72 to 4000, both precisions, both
-// layouts).  Per GiB of complex64 lines the
+// thread and the length is not a multiple of 20, else the per-pass-mapped
+// Stockham, else the runtime schedule with more; then the chirp-z pass (9
+// points and up), else the direct DFT.  The runtime schedule needs E
+// divisible by every radix: lengths with both 3 and 5 (120, 360, 480, 960)
+// run at E = 30, 1.1-4.3x slower per 3D transform (one contiguous and two
+// strided passes) than the per-pass-mapped kernel, which also pairs 2s with
+// 5s into radix-10 passes: multiples of 20 (100, 400, 800, 1000, 1600) run
+// 1.06x (complex64) and 1.14x (complex128) faster on it, geometric means over
+// 12 lengths.  2^a 3^b lengths and odd 3^b 5^c ones (243, 375, 729, 2187)
+// stay on the runtime schedule, up to 1.7x faster there (49 lengths, 72 to
+// 4000, both precisions, both layouts).  Per GiB of complex64 lines the
 // direct DFT costs O(N): 7.7 ms at N = 11 (one workgroup per line leaves most
 // of its threads idle), 3.3 ms at 97, 13 ms at 509, 124 ms at 4093; the
 // chirp-z pass 0.9-2.1 ms at every length from 11 to 4093.  complex128 agrees,
@@ -45,7 +49,7 @@ static KSel fallback_sel(const AxisPass& p, a2afft_dtype dt, int N) {
     const bool dbl = dt == A2AFFT_C128;
     MixedPlan mp;
     const int e = mixed_plan_regs(N, dbl);
-    if (e > 0 && e <= 24 && mixed_plan(p, N, dbl, &mp)) return KSel{mp.T, mp.E, KF_MIX};
+    if (e > 0 && e <= 24 && N % 20 != 0 && mixed_plan(p, N, dbl, &mp)) return KSel{mp.T, mp.E, KF_MIX};
     MixlPlan lp;
     if (mixl_plan(p, N, dbl, &lp)) return KSel{lp.T, lp.EM, KF_MIXL | KF_XCD};
     if (mixed_plan(p, N, dbl, &mp)) return KSel{mp.T, mp.E, KF_MIX};

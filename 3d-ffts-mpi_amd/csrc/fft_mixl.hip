@@ -13,7 +13,7 @@
 // last pass straight to global memory, j + q*N/R).  The tile lives in LDS
 // between passes, in place: a pass reads all its inputs, waits for the
 // workgroup, then writes.  Because no register slot is tied to an element,
-// any radix sequence runs: 480 = 8*4*3*5 in 4 passes, where a kernel that
+// any radix sequence runs: 480 = 16*10*3 in 3 passes, where a kernel that
 // keeps element n = t + e*TPL in register e needs every radix to divide E and
 // takes 480 as 2*2*2*2*2*3*5.  The arithmetic of a butterfly does not depend
 // on the tile or the thread that runs it, so every tile and every layout of a
@@ -190,6 +190,8 @@ __global__ __launch_bounds__(kMixlThreads) void mixl_kernel(typename CT<real>::V
         case 3: mixl_pass<3, EM, STRIDED, DIR, NTMP>(v, lds, data, g, tw, lp, p, t, block); break;
         case 4: mixl_pass<4, EM, STRIDED, DIR, NTMP>(v, lds, data, g, tw, lp, p, t, block); break;
         case 5: mixl_pass<5, EM, STRIDED, DIR, NTMP>(v, lds, data, g, tw, lp, p, t, block); break;
+        case 6: mixl_pass<6, EM, STRIDED, DIR, NTMP>(v, lds, data, g, tw, lp, p, t, block); break;
+        case 10: mixl_pass<10, EM, STRIDED, DIR, NTMP>(v, lds, data, g, tw, lp, p, t, block); break;
         case 8: mixl_pass<8, EM, STRIDED, DIR, NTMP>(v, lds, data, g, tw, lp, p, t, block); break;
         default: mixl_pass<16, EM, STRIDED, DIR, NTMP>(v, lds, data, g, tw, lp, p, t, block); break;
         }
@@ -264,17 +266,22 @@ int em_for(int N, int T, const int* radix, int npass, bool dbl) {
 }  // namespace
 
 // Schedule: the power-of-two factor as 16s, 8s and 4s (8 x 4 for 32, 8 x 8 x
-// 8 for 512), then the 3s, then the 5s.  It depends on N only, so every pass
-// and layout of a length computes the same bits.  The default tile, from a
+// 8 for 512), then 6s and 10s where they save passes, then the 3s, then the
+// 5s.  It depends on N only, so every pass and layout of a length computes
+// the same bits.  The default tile, from a
 // sweep of every tile at 49 lengths from 72 to 4000 on 128-256 MiB grids:
 // contiguous, the most lines that hold at most 4096 complex64 or 2560
 // complex128 values (occupancy: the complex128 kernels take 109 VGPRs) and
 // need at most 16 data registers a thread (one radix-16 butterfly, two of
 // radix 8, ...); strided, the most columns of 16, 8, 4, 2 within 4096
-// complex64 values at 24 registers or 3072 complex128 values at 16, and two
-// columns rather than one wherever they fit LDS at 32 (complex64) or 24
-// (complex128) registers: 16- and 32-byte row segments read up to 1.6x
-// faster than 8- and 16-byte ones.  One line otherwise.
+// complex64 values at 24 registers or 3072 complex128 values at 16, or, for
+// complex64 rows at least 256 KiB apart, of 8, 4, 2 within 8192 values at 32
+// registers (the x passes of 640x640x256, 960x960x128 and 1600x1600x32:
+// 0.543 -> 0.456, 0.648 -> 0.529 and 0.689 -> 0.517 ms on 64- and 32-byte
+// rows; nearer rows lost, 720^3 y pass 1.86 -> 2.57 ms), and two columns
+// rather than one wherever they fit LDS at 32 (complex64) or 24 (complex128)
+// registers: 16- and 32-byte row segments read up to 1.6x faster than 8- and
+// 16-byte ones.  One line otherwise.
 bool mixl_plan(const AxisPass& ps, int N, bool dbl, MixlPlan* out, int force_T) {
     if (N < 2 || N > kDftMaxN || is_pow2(N)) return false;
     int m = N, a = 0, b3 = 0, c5 = 0;
@@ -293,10 +300,28 @@ bool mixl_plan(const AxisPass& ps, int N, bool dbl, MixlPlan* out, int force_T) 
     static const int kPow2[13][3] = {{0, 0, 0}, {2, 0, 0}, {4, 0, 0}, {8, 0, 0}, {16, 0, 0}, {8, 4, 0}, {8, 8, 0},
                                      {16, 8, 0}, {16, 16, 0}, {8, 8, 8}, {16, 8, 8}, {16, 16, 8}, {16, 16, 16}};
     if (a > 12) return false;
+    // 6 = 2*3 and 10 = 2*5 as single prime-factor passes where pairing 2s
+    // with the odd factors takes the power-of-two part to fewer passes (480 =
+    // 16*10*3, 360 = 6*6*10, 960 = 16*6*10, 1000 = 10*10*10): the fewest
+    // passes, then the fewest composite radices.
+    auto npow2 = [&](int e) { return (kPow2[e][0] != 0) + (kPow2[e][1] != 0) + (kPow2[e][2] != 0); };
+    int n6 = 0, n10 = 0, best = npow2(a) + b3 + c5;
+    for (int s6 = 0; s6 <= std::min(a, b3); ++s6)
+        for (int s10 = 0; s10 <= std::min(a - s6, c5); ++s10) {
+            const int passes = npow2(a - s6 - s10) + b3 + c5;
+            if (passes < best || (passes == best && s6 + s10 < n6 + n10)) {
+                best = passes;
+                n6 = s6;
+                n10 = s10;
+            }
+        }
+    const int a2 = a - n6 - n10;
     for (int i = 0; i < 3; ++i)
-        if (kPow2[a][i]) q.radix[np++] = kPow2[a][i];
-    for (int i = 0; i < b3; ++i) q.radix[np++] = 3;
-    for (int i = 0; i < c5; ++i) q.radix[np++] = 5;
+        if (kPow2[a2][i]) q.radix[np++] = kPow2[a2][i];
+    for (int i = 0; i < n6; ++i) q.radix[np++] = 6;
+    for (int i = 0; i < n10; ++i) q.radix[np++] = 10;
+    for (int i = 0; i < b3 - n6; ++i) q.radix[np++] = 3;
+    for (int i = 0; i < c5 - n10; ++i) q.radix[np++] = 5;
     if (np > kMixlMaxPasses || np < 1) return false;
     q.npass = np;
     q.NT = kMixlThreads;
@@ -328,8 +353,9 @@ bool mixl_plan(const AxisPass& ps, int N, bool dbl, MixlPlan* out, int force_T) 
             if (t * N <= cap && fits(t, 16)) T = t;
     } else {
         const int cap = dbl ? 3072 : 4096, emlim = dbl ? 16 : 24;
+        const bool far = ps.fft.s_elem * esize >= (256 << 10);   // rows at least 256 KiB apart
         for (int t = 16; t >= 1 && T == 0; t /= 2)
-            if (t * N <= cap && fits(t, emlim)) T = t;
+            if ((t * N <= cap && fits(t, emlim)) || (!dbl && far && t <= 8 && t * N <= 8192 && fits(t, 32))) T = t;
         if (T == 1 && fits(2, dbl ? 24 : 32)) T = 2;
     }
     if (T == 0 && fits(1, 1 << 30)) T = 1;

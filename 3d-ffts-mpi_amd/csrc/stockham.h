@@ -197,9 +197,33 @@ template <int DIR, typename V> __device__ __forceinline__ void dft5(V* x) {
     x[2] = c_add(m2, r2);
     x[3] = c_sub(m2, r2);
 }
+// 6- and 10-point DFTs as prime-factor (Good-Thomas) products of 2 and 3 or
+// 5 points, no twiddles: input n = (3 n1 + 2 n2) mod 6, resp. (5 n1 + 2 n2)
+// mod 10; output k = (3 k1 + 4 k2) mod 6, resp. (5 k1 + 6 k2) mod 10, where
+// n k reduces to the sub-DFTs' own exponents.
+template <int DIR, typename V> __device__ __forceinline__ void dft6(V* x) {
+    V a0[3] = {x[0], x[2], x[4]}, a1[3] = {x[3], x[5], x[1]};
+    dft3<DIR>(a0);
+    dft3<DIR>(a1);
+    x[0] = c_add(a0[0], a1[0]); x[3] = c_sub(a0[0], a1[0]);
+    x[4] = c_add(a0[1], a1[1]); x[1] = c_sub(a0[1], a1[1]);
+    x[2] = c_add(a0[2], a1[2]); x[5] = c_sub(a0[2], a1[2]);
+}
+template <int DIR, typename V> __device__ __forceinline__ void dft10(V* x) {
+    V a0[5] = {x[0], x[2], x[4], x[6], x[8]}, a1[5] = {x[5], x[7], x[9], x[1], x[3]};
+    dft5<DIR>(a0);
+    dft5<DIR>(a1);
+    x[0] = c_add(a0[0], a1[0]); x[5] = c_sub(a0[0], a1[0]);
+    x[6] = c_add(a0[1], a1[1]); x[1] = c_sub(a0[1], a1[1]);
+    x[2] = c_add(a0[2], a1[2]); x[7] = c_sub(a0[2], a1[2]);
+    x[8] = c_add(a0[3], a1[3]); x[3] = c_sub(a0[3], a1[3]);
+    x[4] = c_add(a0[4], a1[4]); x[9] = c_sub(a0[4], a1[4]);
+}
 template <int R, int DIR, typename V> __device__ __forceinline__ void dft(V* x) {
     if constexpr (R == 3) dft3<DIR>(x);
     else if constexpr (R == 5) dft5<DIR>(x);
+    else if constexpr (R == 6) dft6<DIR>(x);
+    else if constexpr (R == 10) dft10<DIR>(x);
     else if constexpr (R == 2) dft2<DIR>(x[0], x[1]);
     else if constexpr (R == 4) dft4<DIR>(x[0], x[1], x[2], x[3]);
     else if constexpr (R == 8) dft8<DIR>(x);

@@ -174,7 +174,7 @@ CONFIGS_3D = [
     (14, 2, None, "c128"),        # the direct DFT (a factor 7) on every axis, blocks of 7 x 14 x 14
     (14, 2, None, "c64"),
     (21, 3, None, "c128"),        # 3 * 7 over dims [3,1,1]
-    (100, 4, (2, 2, 1), "c64"),   # 2^2 * 5^2, above the one-thread-per-line limit: runtime schedule
+    (100, 4, (2, 2, 1), "c64"),   # 2^2 * 5^2, above the one-thread-per-line limit: radix-10 passes (4 * 5 * 5)
     (45, 9, None, "c128"),        # no fixed complex128 kernel for 45: runtime schedule
     (96, 8, (1, 2, 4), "c128"),   # 2^5 * 3 over blocks of 96 x 48 x 24
     (96, 8, (1, 2, 4), "c64"),
@@ -360,14 +360,18 @@ WIDE_MIX_CASES = [((384, 384, 16), "fp32", "x T=16 E=12 NT=512"), ((768, 32, 16)
 @pytest.mark.parametrize("case", WIDE_MIX_CASES, ids=lambda c: f"{c[0]}_{c[1]}")
 def test_wide_mixed_tiles_vs_numpy(a2afft_mod, cuda, case):
     """Wide strided tiles of the runtime-schedule Stockham kernel: the plan
-    picks them, and forward and inverse match numpy's fftn in FP64 to the
-    precision's bound."""
+    picks them (set by hand for the E=20 tiles of multiples of 20, which
+    default to the per-pass-mapped kernel), and forward and inverse match
+    numpy's fftn in FP64 to the precision's bound."""
     import torch
     from a2afft import inputs
     shape, prec, chosen = case
     n = int(np.prod(shape))
     x = inputs.uniform(13, n).reshape(shape).astype(np.complex128 if prec == "fp64" else np.complex64)
     plan = a2afft_mod.GridDfft(shape, prec)
+    if "E=20" in chosen:   # multiples of 20 default to the per-pass-mapped kernel: force the tile
+        T, E = (int(f.split("=")[1]) for f in chosen.split()[1:3])
+        plan.set_kernel("zxy".index(chosen[0]), T, E, KF_MIX)
     assert chosen in plan.describe(), plan.describe()
     d = torch.from_numpy(x).cuda()
     plan.forward(d)
