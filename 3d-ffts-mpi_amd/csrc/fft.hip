@@ -38,18 +38,47 @@ static KSel cz_sel(int N) { return KSel{1, N, KF_CZ}; }
 // 1.06x (complex64) and 1.14x (complex128) faster on it, geometric means over
 // 12 lengths.  2^a 3^b lengths and odd 3^b 5^c ones (243, 375, 729, 2187)
 // stay on the runtime schedule, up to 1.7x faster there (49 lengths, 72 to
-// 4000, both precisions, both layouts).  Per GiB of complex64 lines the
+// 4000, both precisions, both layouts).  Where the same sweep's cost of a 3D
+// transform favours the other family by more than 7 %, kFamilySwap overrides
+// the rule.  Per GiB of complex64 lines the
 // direct DFT costs O(N): 7.7 ms at N = 11 (one workgroup per line leaves most
 // of its threads idle), 3.3 ms at 97, 13 ms at 509, 124 ms at 4093; the
 // chirp-z pass 0.9-2.1 ms at every length from 11 to 4093.  complex128 agrees,
 // except contiguous lines of 37-53 points, where the DFT is up to 1.3x faster.
 // The choice depends on (N, precision) only, never on the layout, so that
 // fused and staged plans compute every line alike.
+//
+// kFamilySwap: (N, precision) pairs whose family is the other one: the sweep
+// (one contiguous pass plus two strided passes of 128-256 MiB, best tile of
+// each family) favoured it by more than 7 %, and so did the strided passes of
+// a grid (ms per x + y pass, this family against the other's best tile).
+// Long 2^a 3 lines run their strided passes faster per-pass-mapped, which
+// outweighs their contiguous passes; 4000 (complex64) and 1280, 2000
+// (complex128) run theirs faster on the runtime schedule.  768 (complex64)
+// and 288 (complex128) passed the sweep but not the grids (768x768x256 x pass
+// 0.84 per-pass-mapped against 0.75 ms), so they keep the rule.
+struct FamilySwap { int N; bool dbl; };
+static constexpr FamilySwap kFamilySwap[] = {
+    {1536, false},   // 1536x1536x64: 1.94 against 2.17 ms
+    {3072, false},   // 3072x3072x16: 3.02 against 3.20 ms
+    {1536, true},    // 1536x1536x32: 1.85 against 2.12 ms
+    {4000, false},   // 4000x4000x8: 2.49 against 3.27 ms
+    {1280, true},    // 1280x1280x64: 2.42 against 2.57 ms
+    {2000, true},    // 2000x2000x16: 1.73 against 2.25 ms
+};
+
+static bool family_swapped(int N, bool dbl) {
+    for (const FamilySwap& f : kFamilySwap)
+        if (f.N == N && f.dbl == dbl) return true;
+    return false;
+}
+
 static KSel fallback_sel(const AxisPass& p, a2afft_dtype dt, int N) {
     const bool dbl = dt == A2AFFT_C128;
     MixedPlan mp;
     const int e = mixed_plan_regs(N, dbl);
-    if (e > 0 && e <= 24 && N % 20 != 0 && mixed_plan(p, N, dbl, &mp)) return KSel{mp.T, mp.E, KF_MIX};
+    const bool runtime = e > 0 && e <= 24 && (N % 20 != 0) != family_swapped(N, dbl);
+    if (runtime && mixed_plan(p, N, dbl, &mp)) return KSel{mp.T, mp.E, KF_MIX};
     MixlPlan lp;
     if (mixl_plan(p, N, dbl, &lp)) return KSel{lp.T, lp.EM, KF_MIXL | KF_XCD};
     if (mixed_plan(p, N, dbl, &mp)) return KSel{mp.T, mp.E, KF_MIX};

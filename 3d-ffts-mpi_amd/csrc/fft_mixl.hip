@@ -281,7 +281,21 @@ int em_for(int N, int T, const int* radix, int npass, bool dbl) {
 // rows; nearer rows lost, 720^3 y pass 1.86 -> 2.57 ms), and two columns
 // rather than one wherever they fit LDS at 32 (complex64) or 24 (complex128)
 // registers: 16- and 32-byte row segments read up to 1.6x faster than 8- and
-// 16-byte ones.  One line otherwise.
+// 16-byte ones.  One line otherwise.  kMixlStridedTile overrides the strided
+// rule for lengths where a sweep of every tile on 0.6-13 GB grids (N^3 and
+// N x N x 16..64, where the pass is bound by HBM rather than served from the
+// 256 MB Infinity Cache) found another tile faster on both near and far rows.
+struct MixlTile { int N; bool dbl; int T; };
+static constexpr MixlTile kMixlStridedTile[] = {
+    {600, false, 8},    // 600^3 y pass 1.06 -> 0.80 ms
+    {640, false, 8},    // 640^3 y pass 1.25 -> 1.09 ms
+    {720, false, 4},    // 720^3 x pass 2.49 -> 2.32 ms, 720x720x64 x 0.24 -> 0.15 ms
+    {960, false, 8},    // 960^3 y pass 4.40 -> 3.91 ms
+    {1536, false, 4},   // 1536x1536x32 y pass 0.47 -> 0.44 ms
+    {1600, false, 4},   // 1600x1600x32 y pass 0.54 -> 0.49 ms
+    {960, true, 4},     // 960^3 complex128 x pass 12.2 -> 8.3 ms, y 9.2 -> 7.3 ms
+};
+
 bool mixl_plan(const AxisPass& ps, int N, bool dbl, MixlPlan* out, int force_T) {
     if (N < 2 || N > kDftMaxN || is_pow2(N)) return false;
     int m = N, a = 0, b3 = 0, c5 = 0;
@@ -354,6 +368,8 @@ bool mixl_plan(const AxisPass& ps, int N, bool dbl, MixlPlan* out, int force_T) 
     } else {
         const int cap = dbl ? 3072 : 4096, emlim = dbl ? 16 : 24;
         const bool far = ps.fft.s_elem * esize >= (256 << 10);   // rows at least 256 KiB apart
+        for (const MixlTile& mt : kMixlStridedTile)
+            if (mt.N == N && mt.dbl == dbl && fits(mt.T, dbl ? 24 : 32)) T = mt.T;
         for (int t = 16; t >= 1 && T == 0; t /= 2)
             if ((t * N <= cap && fits(t, emlim)) || (!dbl && far && t <= 8 && t * N <= 8192 && fits(t, 32))) T = t;
         if (T == 1 && fits(2, dbl ? 24 : 32)) T = 2;

@@ -360,16 +360,16 @@ WIDE_MIX_CASES = [((384, 384, 16), "fp32", "x T=16 E=12 NT=512"), ((768, 32, 16)
 @pytest.mark.parametrize("case", WIDE_MIX_CASES, ids=lambda c: f"{c[0]}_{c[1]}")
 def test_wide_mixed_tiles_vs_numpy(a2afft_mod, cuda, case):
     """Wide strided tiles of the runtime-schedule Stockham kernel: the plan
-    picks them (set by hand for the E=20 tiles of multiples of 20, which
-    default to the per-pass-mapped kernel), and forward and inverse match
-    numpy's fftn in FP64 to the precision's bound."""
+    picks them (set by hand for lengths that default to the per-pass-mapped
+    kernel: multiples of 20, and 1536 in complex64), and forward and
+    inverse match numpy's fftn in FP64 to the precision's bound."""
     import torch
     from a2afft import inputs
     shape, prec, chosen = case
     n = int(np.prod(shape))
     x = inputs.uniform(13, n).reshape(shape).astype(np.complex128 if prec == "fp64" else np.complex64)
     plan = a2afft_mod.GridDfft(shape, prec)
-    if "E=20" in chosen:   # multiples of 20 default to the per-pass-mapped kernel: force the tile
+    if not _chosen_flags(plan, chosen[0]) & KF_MIX:   # per-pass-mapped by default: force the tile
         T, E = (int(f.split("=")[1]) for f in chosen.split()[1:3])
         plan.set_kernel("zxy".index(chosen[0]), T, E, KF_MIX)
     assert chosen in plan.describe(), plan.describe()
@@ -423,7 +423,8 @@ def test_mixed_tiles_bit_identical(a2afft_mod, cuda, case):
 MIXL_CASES = [((16, 16, 480), "fp32", 0), ((480, 8, 64), "fp32", 1), ((8, 360, 64), "fp64", 2),
               ((4, 4, 3000), "fp32", 0), ((3000, 2, 16), "fp64", 1), ((3000, 2, 16), "fp32", 1),
               ((16, 16, 120), "fp64", 0),
-              ((120, 4, 64), "fp32", 1)]
+              ((120, 4, 64), "fp32", 1),
+              ((1536, 2, 16), "fp32", 1), ((16, 16, 3072), "fp32", 0), ((8, 1536, 16), "fp64", 2)]
 
 
 @pytest.mark.parametrize("case", MIXL_CASES, ids=lambda c: f"{c[0]}_{c[1]}_pass{c[2]}")
@@ -464,6 +465,35 @@ def test_mixl_tiles_vs_numpy_and_bit_identical(a2afft_mod, cuda, case):
                 assert np.array_equal(_bits(y.cpu().numpy()), ref_bits), (T, E, fl)
                 ran += 1
     assert ran >= 3, ran
+
+
+# (grid shape, precision, pass): multiples of 20 that stay on the runtime
+# schedule (4000 complex64, 1280 and 2000 complex128)
+RUNTIME_20_CASES = [((4000, 2, 16), "fp32", 1), ((2000, 2, 16), "fp64", 1), ((16, 8, 1280), "fp64", 0),
+                    ((4, 4000, 16), "fp32", 2)]
+
+
+@pytest.mark.parametrize("case", RUNTIME_20_CASES, ids=lambda c: f"{c[0]}_{c[1]}_pass{c[2]}")
+def test_runtime_schedule_multiples_of_20(a2afft_mod, cuda, case):
+    """Multiples of 20 whose strided passes run faster on the runtime schedule:
+    the plan picks that family, and forward and inverse match numpy's fftn in
+    FP64 to the precision's bound."""
+    import torch
+    from a2afft import inputs
+    shape, prec, pass_ = case
+    n = int(np.prod(shape))
+    x = inputs.uniform(29, n).reshape(shape).astype(np.complex128 if prec == "fp64" else np.complex64)
+    plan = a2afft_mod.GridDfft(shape, prec)
+    assert _chosen_flags(plan, "zxy"[pass_]) & KF_MIX, plan.describe()
+    d = torch.from_numpy(x).cuda()
+    plan.forward(d)
+    torch.cuda.synchronize()
+    lg = np.log2(max(shape))
+    tol = 1e-15 * lg if prec == "fp64" else 1.5e-7 * lg
+    assert _relerr(d.cpu().numpy(), np.fft.fftn(x.astype(np.complex128))) < tol
+    plan.inverse(d)
+    torch.cuda.synchronize()
+    assert _relerr(d.cpu().numpy(), x.astype(np.complex128)) < 2 * tol
 
 
 # (grid shape, precision, [(pass, T, E)]): the shapes built with half-width
