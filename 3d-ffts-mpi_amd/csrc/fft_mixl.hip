@@ -292,15 +292,20 @@ int em_for(int N, int T, const int* radix, int npass, bool dbl, bool strided) {
 // 16-byte ones.  One line otherwise.  kMixlStridedTile overrides the strided
 // rule for lengths where a sweep of every tile on 0.6-13 GB grids (N^3 and
 // N x N x 16..64, where the pass is bound by HBM rather than served from the
-// 256 MB Infinity Cache) found another tile faster on both near and far rows.
+// 256 MB Infinity Cache) found another tile faster on both near and far rows
+// (radix-15 lengths measured on their radix-15 schedules).
 struct MixlTile { int N; bool dbl; int T; };
 static constexpr MixlTile kMixlStridedTile[] = {
     {600, false, 8},    // 600^3 y pass 1.06 -> 0.80 ms
     {640, false, 8},    // 640^3 y pass 1.25 -> 1.09 ms
     {720, false, 4},    // 720^3 x pass 2.49 -> 2.32 ms, 720x720x64 x 0.24 -> 0.15 ms
     {960, false, 8},    // 960^3 y pass 4.40 -> 3.91 ms
+    {1440, false, 4},   // 1440x1440x32 y pass 0.436 -> 0.421 ms
     {1536, false, 4},   // 1536x1536x32 y pass 0.47 -> 0.44 ms
     {1600, false, 4},   // 1600x1600x32 y pass 0.54 -> 0.49 ms
+    {1800, false, 4},   // 1800x1800x32 y pass 0.87 -> 0.55 ms
+    {1920, false, 4},   // 1920x1920x32 y pass 0.97 -> 0.67 ms
+    {900, true, 4},     // 900^3 complex128 x pass 9.76 -> 6.38 ms, y 8.35 -> 5.50 ms
     {960, true, 4},     // 960^3 complex128 x pass 12.2 -> 8.3 ms, y 9.2 -> 7.3 ms
 };
 
