@@ -305,8 +305,7 @@ static constexpr MixlTile kMixlStridedTile[] = {
     {1600, false, 4},   // 1600x1600x32 y pass 0.54 -> 0.49 ms
     {1800, false, 4},   // 1800x1800x32 y pass 0.87 -> 0.55 ms
     {1920, false, 4},   // 1920x1920x32 y pass 0.97 -> 0.67 ms
-    {900, true, 4},     // 900^3 complex128 x pass 9.76 -> 6.38 ms, y 8.35 -> 5.50 ms
-    {960, true, 4},     // 960^3 complex128 x pass 12.2 -> 8.3 ms, y 9.2 -> 7.3 ms
+    {1080, false, 2},   // 1080x1080x64 x pass 0.58 -> 0.53 ms
 };
 
 bool mixl_plan(const AxisPass& ps, int N, bool dbl, MixlPlan* out, int force_T) {
@@ -386,8 +385,13 @@ bool mixl_plan(const AxisPass& ps, int N, bool dbl, MixlPlan* out, int force_T) 
         const bool far = ps.fft.s_elem * esize >= (256 << 10);   // rows at least 256 KiB apart
         for (const MixlTile& mt : kMixlStridedTile)
             if (mt.N == N && mt.dbl == dbl && fits(mt.T, dbl ? 24 : 32)) T = mt.T;
+        // complex128: four columns up to 4000 values at 24 registers (750-1000
+        // points; 800^3 x pass 6.72 -> 5.05 ms, 1000^3 13.0 -> 9.0 ms, 960^3
+        // 12.2 -> 8.3 ms; 1080 points lose, 0.43 -> 0.60 ms)
         for (int t = 16; t >= 1 && T == 0; t /= 2)
-            if ((t * N <= cap && fits(t, emlim)) || (!dbl && far && t <= 8 && t * N <= 8192 && fits(t, 32))) T = t;
+            if ((t * N <= cap && fits(t, emlim)) || (!dbl && far && t <= 8 && t * N <= 8192 && fits(t, 32)) ||
+                (dbl && t == 4 && t * N <= 4000 && fits(t, 24)))
+                T = t;
         if (T == 1 && fits(2, dbl ? 24 : 32)) T = 2;
     }
     if (T == 0 && fits(1, 1 << 30)) T = 1;
